@@ -1,0 +1,870 @@
+// Time-step engine: device buffers, plans and the stage sequence of one step.
+//
+// The stage order and time levels follow the reference step loop
+// (main_cuda.cu:28427-31623):
+//   elastic solve (28760-29166) -> phi update (29752-29825) ->
+//   Y update (30283-30806) -> zero mode (30833-30911, 4269-4405) -> dY/dt (31618-31623)
+// but every transform is a pruned pass sequence and every point-wise stage is
+// fused into a z pass (see pf_ops.hpp), so no real-space intermediate other
+// than tr(sigma) is written.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/pfcore.h"
+#include "pf_engine_api.hpp"
+#include "pf_error.hpp"
+#include "pf_ops.hpp"
+
+#define PF_HIP(call)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return pf::fail(PF_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+namespace pf {
+
+// Supported transform lengths.  Each must factor into 4/2/3/5 and at most
+// further primes handled by the direct prime stage.
+#define PF_SIZES(X) \
+  X(4) X(6) X(8) X(12) X(16) X(20) X(24) X(32) X(40) X(48) X(64) X(96) X(128) X(192) X(246) X(256) X(384) X(512)
+
+inline bool size_supported(int n) {
+  switch (n) {
+#define PF_CASE_OK(v) case v:
+    PF_SIZES(PF_CASE_OK)
+    return true;
+#undef PF_CASE_OK
+    default:
+      return false;
+  }
+}
+
+struct AxisPlan {
+  int N = 0, A = 0, a = 0;
+  bool wrap = true;
+  std::vector<double> k;   // k of each compact index, reference expression order
+};
+
+// Retained modes of one axis: |k| < (2/3) pi / d with k = 2 pi ii / (d N)
+// evaluated exactly as dealias_23_kernel does (cuda_kernels.cu:118-131,
+// cuda_common.h:69-83).
+inline AxisPlan make_axis(int n, double d, bool wrap) {
+  AxisPlan ap;
+  ap.N = n;
+  ap.wrap = wrap;
+  const double kmax = M_PI * (2.0 / 3.0) / d;
+  if (wrap) {
+    int a = 0;
+    while (a <= n / 2) {
+      const double k = 2.0 * M_PI * a / (d * n);
+      if (!(std::fabs(k) < kmax)) break;
+      ++a;
+    }
+    ap.a = a;
+    ap.A = 2 * a - 1;
+    for (int c = 0; c < ap.A; ++c) {
+      int i = c < a ? c : c + n - ap.A;
+      int ii = (i <= n / 2) ? i : i - n;
+      ap.k.push_back(2.0 * M_PI * ii / (d * n));
+    }
+  } else {
+    int a = 0;
+    while (a <= n / 2) {
+      const double k = 2.0 * M_PI * a / (d * n);
+      if (!(std::fabs(k) < kmax)) break;
+      ++a;
+    }
+    ap.a = a;
+    ap.A = a;
+    for (int c = 0; c < a; ++c) ap.k.push_back(2.0 * M_PI * c / (d * n));
+  }
+  return ap;
+}
+
+template <typename T>
+std::vector<cx<T>> twiddles(int n) {
+  std::vector<cx<T>> w(n);
+  for (int t = 0; t < n; ++t) {
+    long double ang = 2.0L * 3.141592653589793238462643383279502884L * (long double)t / (long double)n;
+    w[t] = cx<T>{(T)cosl(ang), (T)(-sinl(ang))};
+  }
+  return w;
+}
+
+// ------------------------------------------------------------ misc kernels
+struct RedKinds {
+  int k[4];
+};
+
+namespace {
+
+// Sums (or min/max) of block partials in a fixed order; one workgroup.
+__global__ void __launch_bounds__(NT) finalize_kernel(const double* partials, int nblocks, int nred,
+                                                      RedKinds kinds, double* out, double scale) {
+  double v[4];
+  for (int r = 0; r < 4; ++r) v[r] = red_identity(kinds.k[r]);
+  for (int b = threadIdx.x; b < nblocks; b += NT)
+    for (int r = 0; r < nred; ++r) v[r] = red_combine(kinds.k[r], v[r], partials[(size_t)b * nred + r]);
+  __shared__ double outv[4];
+  block_reduce_store<4>(v, kinds.k, outv);
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int r = 0; r < nred; ++r) out[r] = kinds.k[r] == RED_SUM ? outv[r] * scale : outv[r];
+}
+
+// sum (1-h) x + v_B h and sum h(clamp01 phi)  (cuda_kernels.cu:3792-3828, 5580-5607)
+__global__ void __launch_bounds__(NT) mass_vf_kernel(const double* phi, const double* xB, double vB,
+                                                     long n, double* partials) {
+  double acc[2] = {0.0, 0.0};
+  for (long g = blockIdx.x * (long)NT + threadIdx.x; g < n; g += (long)gridDim.x * NT) {
+    const double p = phi[g];
+    const double h = h_of_phi(p);
+    acc[0] += (1.0 - h) * xB[g] + vB * h;
+    acc[1] += h_of_phi(clamp01(p));
+  }
+  int kinds[2] = {RED_SUM, RED_SUM};
+  block_reduce_store<2>(acc, kinds, partials + (size_t)blockIdx.x * 2);
+}
+
+// 0.5 kappa |grad phi|^2 by periodic central differences
+// (compute_grad_energy_density_kernel, cuda_kernels.cu:773-809)
+__global__ void __launch_bounds__(NT) grad_energy_kernel(const double* phi, int Nx, int Ny, int Nz,
+                                                         double dx, double dy, double dz,
+                                                         double kappa, double* partials) {
+  double acc[1] = {0.0};
+  const long n = (long)Nx * Ny * Nz;
+  for (long g = blockIdx.x * (long)NT + threadIdx.x; g < n; g += (long)gridDim.x * NT) {
+    const int k = (int)(g % Nz);
+    const long rem = g / Nz;
+    const int j = (int)(rem % Ny);
+    const int i = (int)(rem / Ny);
+    const int ip = (i + 1) % Nx, im = (i - 1 + Nx) % Nx;
+    const int jp = (j + 1) % Ny, jm = (j - 1 + Ny) % Ny;
+    const int kp = (k + 1) % Nz, km = (k - 1 + Nz) % Nz;
+    const double gx = (phi[((long)ip * Ny + j) * Nz + k] - phi[((long)im * Ny + j) * Nz + k]) / (2.0 * dx);
+    const double gy = (phi[((long)i * Ny + jp) * Nz + k] - phi[((long)i * Ny + jm) * Nz + k]) / (2.0 * dy);
+    const double gz = (phi[((long)i * Ny + j) * Nz + kp] - phi[((long)i * Ny + j) * Nz + km]) / (2.0 * dz);
+    acc[0] += 0.5 * kappa * (gx * gx + gy * gy + gz * gz);
+  }
+  int kinds[1] = {RED_SUM};
+  block_reduce_store<1>(acc, kinds, partials + blockIdx.x);
+}
+
+// Zero-mode evaluation (compute_pf_Y_zero_mode_terms_kernel, cuda_kernels.cu:3409-3420):
+// mass(lambda) = sum (1-h) sigma(Y*+lambda) + h v_B, its derivative, and the
+// count of shifted values outside [Y_lo, Y_hi] (validate_pf_Y_zero_mode_bounds_kernel 3422-3432).
+__global__ void __launch_bounds__(NT) zm_eval_kernel(const double* Ystar, const double* phi,
+                                                     double lam, double vB, double ylo, double yhi,
+                                                     long n, double* partials) {
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (long g = blockIdx.x * (long)NT + threadIdx.x; g < n; g += (long)gridDim.x * NT) {
+    const double h = h_of_phi(phi[g]);
+    const double alpha = 1.0 - h;
+    const double y = Ystar[g] + lam;
+    const double x = sigmoid_unclipped(y);
+    acc[0] += alpha * x + h * vB;
+    acc[1] += alpha * x * (1.0 - x);
+    if (!isfinite(y) || y < ylo || y > yhi) acc[2] += 1.0;
+  }
+  int kinds[3] = {RED_SUM, RED_SUM, RED_SUM};
+  block_reduce_store<3>(acc, kinds, partials + (size_t)blockIdx.x * 3);
+}
+
+// Y = Y* + lambda, x_B = sigma(Y) unclipped (apply_pf_Y_zero_mode_shift_kernel
+// 3434-3443), dY/dt = (Y - Y^n)/dt (update_dY_dt_prev_kernel 3545-3556).
+__global__ void __launch_bounds__(NT) zm_apply_kernel(const double* Ystar, double lam, double* Y,
+                                                      double* xB, double* dYdt, double dt, long n) {
+  for (long g = blockIdx.x * (long)NT + threadIdx.x; g < n; g += (long)gridDim.x * NT) {
+    const double y = Ystar[g] + lam;
+    const double yn = Y[g];
+    Y[g] = y;
+    xB[g] = sigmoid_unclipped(y);
+    dYdt[g] = (y - yn) / dt;
+  }
+}
+
+}  // namespace
+
+// identity operators used by the transform test entry points
+template <typename T>
+struct OpLoadReal {
+  struct Args {
+    const double* f;
+  };
+  static constexpr int NRED = 0;
+  static constexpr int kinds[1] = {0};
+  __device__ static void apply(const Args& a, size_t g, const T*, T* out, double*) { out[0] = T(a.f[g]); }
+};
+template <typename T>
+struct OpStoreReal {
+  struct Args {
+    double* f;
+  };
+  static constexpr int NRED = 0;
+  static constexpr int kinds[1] = {0};
+  __device__ static void apply(const Args& a, size_t g, const T* vin, T*, double*) { a.f[g] = double(vin[0]); }
+};
+
+// ------------------------------------------------------------ dispatch
+template <typename T, int N, int NCI, int NCO>
+constexpr int zpass_rows_for() {
+  constexpr int NCM = NCI > NCO ? NCI : NCO;
+  return zpass_rows<T, N, (NCM > 0 ? NCM : 1)>();
+}
+
+template <typename T, int NCI, int NCO>
+int zpass_grid(int N, int rows) {
+  switch (N) {
+#define PF_CASE_ZG(v) \
+  case v:             \
+    return (rows + zpass_rows_for<T, v, NCI, NCO>() - 1) / zpass_rows_for<T, v, NCI, NCO>();
+    PF_SIZES(PF_CASE_ZG)
+#undef PF_CASE_ZG
+    default:
+      return -1;
+  }
+}
+
+template <typename T, int NCI, int NCO, class Op>
+void launch_zpass(int N, hipStream_t s, const ZPassArgs<T>& za, const typename Op::Args& oa) {
+  const int grid = zpass_grid<T, NCI, NCO>(N, za.rows);
+  switch (N) {
+#define PF_CASE_Z(v)                                                                              \
+  case v:                                                                                         \
+    hipLaunchKernelGGL((zpass_kernel<T, v, NCI, NCO, Op>), dim3(grid), dim3(NT), 0, s, za, oa); \
+    break;
+    PF_SIZES(PF_CASE_Z)
+#undef PF_CASE_Z
+    default:
+      break;
+  }
+}
+
+template <typename T, bool INV>
+void launch_cpass(int N, hipStream_t s, const CPassArgs<T>& ca, int planes, int comps) {
+  switch (N) {
+#define PF_CASE_C(v)                                                                            \
+  case v: {                                                                                     \
+    constexpr int TC = cpass_cols<T, v>();                                                      \
+    dim3 grid((ca.cols + TC - 1) / TC, planes, comps);                                          \
+    hipLaunchKernelGGL((cpass_kernel<T, v, INV>), grid, dim3(NT), 0, s, ca);                    \
+    break;                                                                                      \
+  }
+    PF_SIZES(PF_CASE_C)
+#undef PF_CASE_C
+    default:
+      break;
+  }
+}
+
+inline int grid_for(long n) {
+  long g = (n + NT - 1) / NT;
+  if (g > 2048) g = 2048;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+// ------------------------------------------------------------ engine
+template <typename T>
+struct Engine final : EngineBase {
+  using C = cx<T>;
+  pf_params prm{};
+  PhysParams P{};
+  hipStream_t stream = nullptr;
+  int Nx = 0, Ny = 0, Nz = 0;
+  AxisPlan ax, ay, az;
+  long R = 0, nSz = 0, nSy = 0, nSc = 0;
+  // device memory
+  double *phi[2] = {nullptr, nullptr}, *Y = nullptr, *Ystar = nullptr, *xB = nullptr, *dYdt = nullptr;
+  T* trs = nullptr;
+  C *Sz = nullptr, *Sy = nullptr, *Sc = nullptr, *u = nullptr, *un = nullptr;
+  C *twx = nullptr, *twy = nullptr, *twz = nullptr;
+  double *kx = nullptr, *ky = nullptr, *kz = nullptr;
+  double* partials = nullptr;
+  double* dscal = nullptr;   // device scalars
+  double* hscal = nullptr;   // pinned host mirror
+  size_t npartials = 0;
+  std::vector<void*> allocations;
+  int cur = 0;
+  bool warm_valid = false;
+  double target_mass = 0.0;
+  // statistics
+  long long steps_done = 0, elastic_total = 0;
+  int last_iters = 0, zm_iters = 0;
+  double last_res = NAN, zm_lambda = 0.0, mean_DY = 0.0;
+  double e_el = 0.0, e_dw = 0.0, e_chem = 0.0, e_grad = 0.0;
+
+  ~Engine() override {
+    for (void* q : allocations) (void)hipFree(q);
+    if (hscal) (void)hipHostFree(hscal);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  template <typename U>
+  int dalloc(U** ptr, size_t count) {
+    void* q = nullptr;
+    PF_HIP(hipMalloc(&q, count * sizeof(U) + 256));
+    allocations.push_back(q);
+    *ptr = static_cast<U*>(q);
+    return PF_OK;
+  }
+
+  KDims kd() const {
+    return KDims{ax.A, ay.A, az.A, kx, ky, kz, (Nz % 2) == 0};
+  }
+
+  int init(const pf_params& p) override {
+    prm = p;
+    Nx = p.Nx;
+    Ny = p.Ny;
+    Nz = p.Nz;
+    if (!size_supported(Nx) || !size_supported(Ny) || !size_supported(Nz))
+      return fail(PF_E_ARG, "unsupported grid size " + std::to_string(Nx) + "x" + std::to_string(Ny) +
+                                "x" + std::to_string(Nz));
+    P = make_phys(p);
+    ax = make_axis(Nx, p.dx, true);
+    ay = make_axis(Ny, p.dy, true);
+    az = make_axis(Nz, p.dz, false);
+    if (az.A < 1 || ax.A < 1 || ay.A < 1 || 2 * az.A > Nz + 1)
+      return fail(PF_E_ARG, "degenerate retained-mode set");
+    R = (long)Nx * Ny * Nz;
+    nSz = (long)Nx * Ny * az.A;
+    nSy = (long)Nx * ay.A * az.A;
+    nSc = (long)ax.A * ay.A * az.A;
+    PF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    int rc;
+    if ((rc = dalloc(&phi[0], R)) || (rc = dalloc(&phi[1], R)) || (rc = dalloc(&Y, R)) ||
+        (rc = dalloc(&Ystar, R)) || (rc = dalloc(&xB, R)) || (rc = dalloc(&dYdt, R)) ||
+        (rc = dalloc(&trs, R)) || (rc = dalloc(&Sz, 6 * nSz)) || (rc = dalloc(&Sy, 6 * nSy)) ||
+        (rc = dalloc(&Sc, 6 * nSc)) || (rc = dalloc(&u, 3 * nSc)) || (rc = dalloc(&un, 3 * nSc)))
+      return rc;
+    auto wx = twiddles<T>(Nx), wy = twiddles<T>(Ny), wz = twiddles<T>(Nz);
+    if ((rc = dalloc(&twx, Nx)) || (rc = dalloc(&twy, Ny)) || (rc = dalloc(&twz, Nz)) ||
+        (rc = dalloc(&kx, ax.A)) || (rc = dalloc(&ky, ay.A)) || (rc = dalloc(&kz, az.A)))
+      return rc;
+    PF_HIP(hipMemcpy(twx, wx.data(), Nx * sizeof(C), hipMemcpyHostToDevice));
+    PF_HIP(hipMemcpy(twy, wy.data(), Ny * sizeof(C), hipMemcpyHostToDevice));
+    PF_HIP(hipMemcpy(twz, wz.data(), Nz * sizeof(C), hipMemcpyHostToDevice));
+    PF_HIP(hipMemcpy(kx, ax.k.data(), ax.A * sizeof(double), hipMemcpyHostToDevice));
+    PF_HIP(hipMemcpy(ky, ay.k.data(), ay.A * sizeof(double), hipMemcpyHostToDevice));
+    PF_HIP(hipMemcpy(kz, az.k.data(), az.A * sizeof(double), hipMemcpyHostToDevice));
+    // partial-sum scratch: largest grid times 4 reductions
+    long maxg = 2048;
+    const int rows = Nx * Ny;
+    maxg = std::max<long>(maxg, zpass_grid<T, 1, 1>(Nz, rows));
+    maxg = std::max<long>(maxg, zpass_grid<T, 6, 1>(Nz, rows));
+    maxg = std::max<long>(maxg, zpass_grid<T, 1, 0>(Nz, rows));
+    maxg = std::max<long>(maxg, zpass_grid<T, 0, 1>(Nz, rows));
+    npartials = (size_t)maxg * 4;
+    if ((rc = dalloc(&partials, npartials)) || (rc = dalloc(&dscal, 64))) return rc;
+    PF_HIP(hipHostMalloc((void**)&hscal, 64 * sizeof(double), hipHostMallocDefault));
+    return PF_OK;
+  }
+
+  // ---------------------------------------------------------------- helpers
+  C* sz(int c) { return Sz + (size_t)c * nSz; }
+  C* sy(int c) { return Sy + (size_t)c * nSy; }
+  C* sc(int c) { return Sc + (size_t)c * nSc; }
+  C* uu(C* base, int c) { return base + (size_t)c * nSc; }
+
+  ZPassArgs<T> zargs(int c_in0, int nci, int c_out0, int nco) {
+    ZPassArgs<T> za{};
+    za.rows = Nx * Ny;
+    za.Az = az.A;
+    for (int c = 0; c < nci; ++c) za.in[c] = sz(c_in0 + c);
+    for (int c = 0; c < nco; ++c) za.out[c] = sz(c_out0 + c);
+    za.tw = twz;
+    za.partials = partials;
+    return za;
+  }
+
+  // y pass over comps [c0, c0+nc): Sz <-> Sy
+  void ypass(bool inv, int c0, int nc) {
+    CPassArgs<T> ca{};
+    ca.cols = az.A;
+    ca.A = ay.A;
+    ca.a = ay.a;
+    ca.tw = twy;
+    if (!inv) {
+      ca.in = sz(c0);
+      ca.out = sy(c0);
+      ca.in_pstride = (long)Ny * az.A;
+      ca.out_pstride = (long)ay.A * az.A;
+      ca.in_cstride = nSz;
+      ca.out_cstride = nSy;
+      launch_cpass<T, false>(Ny, stream, ca, Nx, nc);
+    } else {
+      ca.in = sy(c0);
+      ca.out = sz(c0);
+      ca.in_pstride = (long)ay.A * az.A;
+      ca.out_pstride = (long)Ny * az.A;
+      ca.in_cstride = nSy;
+      ca.out_cstride = nSz;
+      launch_cpass<T, true>(Ny, stream, ca, Nx, nc);
+    }
+  }
+  // x pass: Sy <-> compact (dst/src compact base with component stride nSc)
+  void xpass(bool inv, int c0, int nc, C* compact) {
+    CPassArgs<T> ca{};
+    ca.cols = ay.A * az.A;
+    ca.A = ax.A;
+    ca.a = ax.a;
+    ca.tw = twx;
+    ca.in_pstride = ca.out_pstride = 0;
+    if (!inv) {
+      ca.in = sy(c0);
+      ca.out = compact;
+      ca.in_cstride = nSy;
+      ca.out_cstride = nSc;
+      launch_cpass<T, false>(Nx, stream, ca, 1, nc);
+    } else {
+      ca.in = compact;
+      ca.out = sy(c0);
+      ca.in_cstride = nSc;
+      ca.out_cstride = nSy;
+      launch_cpass<T, true>(Nx, stream, ca, 1, nc);
+    }
+  }
+
+  void finalize(int nblocks, int nred, RedKinds k, double* out, double scale) {
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(NT), 0, stream, partials, nblocks, nred, k, out,
+                       scale);
+  }
+
+  int fetch(int count) {
+    PF_HIP(hipMemcpyAsync(hscal, dscal, count * sizeof(double), hipMemcpyDeviceToHost, stream));
+    PF_HIP(hipStreamSynchronize(stream));
+    return PF_OK;
+  }
+
+  // ---------------------------------------------------------------- fields
+  int set_fields(const double* phi0, const double* xB0, const double* dYdt0, double xB_cap) override {
+    std::vector<double> ph(R), xb(R), yy(R), dd(R, 0.0);
+    for (long i = 0; i < R; ++i) {
+      double p = phi0[i], x = xB0[i];
+      if (!std::isfinite(p) || !std::isfinite(x) || (dYdt0 && !std::isfinite(dYdt0[i])))
+        return fail(PF_E_NONFINITE, "raw init contains NaN/Inf at idx=" + std::to_string(i));
+      p = clamp01(p);
+      x = clamp_eps(x, P.xB_eps);
+      if (std::isfinite(xB_cap) && x > xB_cap) x = xB_cap;
+      ph[i] = p;
+      xb[i] = x;
+      yy[i] = logit_from_fraction(x, P.xB_eps, P.Y_clip);
+      if (dYdt0) dd[i] = dYdt0[i];
+    }
+    cur = 0;
+    PF_HIP(hipMemcpy(phi[0], ph.data(), R * sizeof(double), hipMemcpyHostToDevice));
+    PF_HIP(hipMemcpy(xB, xb.data(), R * sizeof(double), hipMemcpyHostToDevice));
+    PF_HIP(hipMemcpy(Y, yy.data(), R * sizeof(double), hipMemcpyHostToDevice));
+    PF_HIP(hipMemcpy(dYdt, dd.data(), R * sizeof(double), hipMemcpyHostToDevice));
+    warm_valid = false;
+    steps_done = 0;
+    elastic_total = 0;
+    double m, vf;
+    int rc = mass_vf(&m, &vf);
+    if (rc) return rc;
+    target_mass = m;
+    return PF_OK;
+  }
+
+  int mass_vf(double* m, double* vf) {
+    const int g = grid_for(R);
+    hipLaunchKernelGGL(mass_vf_kernel, dim3(g), dim3(NT), 0, stream, phi[cur], xB, P.v_B, R, partials);
+    finalize(g, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal, 1.0);
+    int rc = fetch(2);
+    if (rc) return rc;
+    *m = hscal[0];
+    *vf = hscal[1] / (double)R;
+    return PF_OK;
+  }
+
+  // ---------------------------------------------------------------- elastic
+  int elastic_solve() {
+    const bool accel = prm.elastic_warm_start_enabled || prm.elastic_residual_control_enabled;
+    const bool warm = accel && warm_valid;
+    const KDims d = kd();
+    const int gk = grid_for(nSc);
+    if (!warm) {
+      typename OpElasticCold<T>::Args oa{P, phi[cur], xB};
+      launch_zpass<T, 0, 6, OpElasticCold<T>>(Nz, stream, zargs(0, 0, 0, 6), oa);
+      ypass(false, 0, 6);
+      xpass(false, 0, 6, Sc);
+      GreenArgs<T> ga{};
+      ga.d = d;
+      ga.n = nSc;
+      for (int i = 0; i < 6; ++i) {
+        ga.h[i] = sc(i);
+        ga.eps[i] = sc(i);
+      }
+      for (int i = 0; i < 3; ++i) ga.u_new[i] = uu(u, i);
+      ga.partials = nullptr;
+      for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
+      ga.have_old = 0;
+      hipLaunchKernelGGL(kop_green<T>, dim3(gk), dim3(NT), 0, stream, ga);
+    } else {
+      StrainArgs<T> sa{};
+      sa.d = d;
+      sa.n = nSc;
+      for (int i = 0; i < 3; ++i) sa.u[i] = uu(u, i);
+      for (int i = 0; i < 6; ++i) sa.eps[i] = sc(i);
+      hipLaunchKernelGGL(kop_strain<T>, dim3(gk), dim3(NT), 0, stream, sa);
+    }
+    xpass(true, 0, 6, Sc);
+    ypass(true, 0, 6);
+    int it = accel ? 0 : 1;
+    bool converged = !accel;
+    double res = NAN;
+    const T invN = T(1.0 / (double)R);
+    while (true) {
+      typename OpHij<T>::Args oa{P, phi[cur], xB, invN, 0};
+      launch_zpass<T, 6, 6, OpHij<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
+      ypass(false, 0, 6);
+      xpass(false, 0, 6, Sc);
+      GreenArgs<T> ga{};
+      ga.d = d;
+      ga.n = nSc;
+      for (int i = 0; i < 6; ++i) {
+        ga.h[i] = sc(i);
+        ga.eps[i] = sc(i);
+      }
+      for (int i = 0; i < 3; ++i) {
+        ga.u_old[i] = uu(u, i);
+        ga.u_new[i] = uu(un, i);
+      }
+      for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
+      ga.have_old = 1;
+      ga.partials = accel ? partials : nullptr;
+      hipLaunchKernelGGL(kop_green<T>, dim3(gk), dim3(NT), 0, stream, ga);
+      if (accel) finalize(gk, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal, 1.0);
+      if (accel)
+        PF_HIP(hipMemcpyAsync(hscal, dscal, 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
+      // strain of the new iterate back to Sz: needed by the next h_ij or the final stress
+      xpass(true, 0, 6, Sc);
+      ypass(true, 0, 6);
+      std::swap(u, un);
+      ++it;
+      if (accel) {
+        PF_HIP(hipStreamSynchronize(stream));
+        const double ds = hscal[0], rs = hscal[1];
+        if (!std::isfinite(ds) || !std::isfinite(rs) || ds < 0.0 || rs < 0.0) {
+          last_iters = it;
+          return fail(PF_E_NONFINITE, "elastic residual is non-finite at step " +
+                                          std::to_string(steps_done + 1));
+        }
+        res = std::sqrt(ds / std::fmax(rs, prm.elastic_residual_absolute_floor));
+        if (it >= prm.elastic_iter_min && res <= prm.elastic_residual_tolerance) {
+          converged = true;
+          break;
+        }
+        if (it >= prm.elastic_iter_max) break;
+      } else {
+        if (it >= prm.elastic_iter_max) break;
+      }
+    }
+    last_iters = it;
+    last_res = res;
+    elastic_total += it;
+    if (accel) {
+      warm_valid = true;
+      if (!converged && prm.elastic_fail_on_nonconvergence) {
+        char buf[256];
+        std::snprintf(buf, sizeof buf,
+                      "accelerated elastic solver did not converge at step %lld: iterations=%d "
+                      "residual=%.17e tolerance=%.17e",
+                      steps_done + 1, it, res, prm.elastic_residual_tolerance);
+        return fail(PF_E_NONCONVERGED, buf);
+      }
+    }
+    return PF_OK;
+  }
+
+  // ---------------------------------------------------------------- step
+  int step() override {
+    const T invN = T(1.0 / (double)R);
+    const int rows = Nx * Ny;
+    const bool zm = prm.zero_mode != 0;
+    double* phi_old = phi[cur];
+    double* phi_new = phi[1 - cur];
+    int rc;
+    // energies at the start of the step: gradient term of phi^n
+    {
+      const int g = grid_for(R);
+      hipLaunchKernelGGL(grad_energy_kernel, dim3(g), dim3(NT), 0, stream, phi_old, Nx, Ny, Nz, prm.dx,
+                         prm.dy, prm.dz, prm.kappa_phi, partials);
+      finalize(g, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 8, 1.0 / (double)R);
+    }
+    if (P.elastic) {
+      if ((rc = elastic_solve())) return rc;
+      typename OpPhiRhs<T, true>::Args oa{P, phi_old, xB, trs, invN};
+      launch_zpass<T, 6, 1, OpPhiRhs<T, true>>(Nz, stream, zargs(0, 6, 0, 1), oa);
+      const int g = zpass_grid<T, 6, 1>(Nz, rows);
+      finalize(g, 3, RedKinds{{RED_SUM, RED_SUM, RED_SUM, 0}}, dscal + 9, 1.0 / (double)R);
+    } else {
+      typename OpPhiRhs<T, false>::Args oa{P, phi_old, xB, trs, invN};
+      launch_zpass<T, 0, 1, OpPhiRhs<T, false>>(Nz, stream, zargs(0, 0, 0, 1), oa);
+      const int g = zpass_grid<T, 0, 1>(Nz, rows);
+      finalize(g, 3, RedKinds{{RED_SUM, RED_SUM, RED_SUM, 0}}, dscal + 9, 1.0 / (double)R);
+    }
+    const KDims d = kd();
+    const int gk = grid_for(nSc);
+    // phi semi-implicit update
+    ypass(false, 0, 1);
+    xpass(false, 0, 1, Sc);
+    hipLaunchKernelGGL(kop_phi<T>, dim3(gk), dim3(NT), 0, stream, d, nSc, sc(0), prm.L_phi * prm.dt * prm.kappa_phi);
+    xpass(true, 0, 1, Sc);
+    ypass(true, 0, 1);
+    {
+      const int phiY_old = (zm || prm.enable_Y_rhs_previous_time_level) ? 1 : 0;
+      typename OpPhiFinish<T>::Args oa{P, phi_new, phi_old, Y, xB, trs, invN, phiY_old};
+      launch_zpass<T, 1, 1, OpPhiFinish<T>>(Nz, stream, zargs(0, 1, 1, 1), oa);
+      const int g = zpass_grid<T, 1, 1>(Nz, rows);
+      finalize(g, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 4, 1.0 / (double)R);
+    }
+    // Y: spectral gradient of mu, fluxes, update
+    ypass(false, 1, 1);
+    xpass(false, 1, 1, sc(1));
+    hipLaunchKernelGGL(kop_grad<T>, dim3(gk), dim3(NT), 0, stream, d, nSc, sc(1), sc(2), sc(3), sc(4));
+    xpass(true, 2, 3, sc(2));
+    ypass(true, 2, 3);
+    {
+      const double* phiY = (zm || prm.enable_Y_rhs_previous_time_level) ? phi_old : phi_new;
+      typename OpFlux<T>::Args oa{P,    phiY, xB,  phi_old, phi_new, Y, dYdt, invN, zm ? 1 : 0,
+                                  prm.enable_Y_rhs_previous_time_level, prm.disable_Y_rhs_gamma_term,
+                                  prm.Y_rhs_term_h_scale};
+      launch_zpass<T, 3, 5, OpFlux<T>>(Nz, stream, zargs(2, 3, 0, 5), oa);
+    }
+    ypass(false, 0, 5);
+    xpass(false, 0, 5, Sc);
+    hipLaunchKernelGGL(kop_yupdate<T>, dim3(gk), dim3(NT), 0, stream, d, nSc, sc(0), sc(1), sc(2), sc(3),
+                       sc(4), sc(5), (const double*)(dscal + 4), prm.dt);
+    xpass(true, 5, 1, sc(5));
+    ypass(true, 5, 1);
+    if (zm) {
+      typename OpYStar<T>::Args oa{Ystar, invN};
+      launch_zpass<T, 1, 0, OpYStar<T>>(Nz, stream, zargs(5, 1, 0, 0), oa);
+      const int g = zpass_grid<T, 1, 0>(Nz, rows);
+      finalize(g, 2, RedKinds{{RED_MIN, RED_MAX, 0, 0}}, dscal, 1.0);
+      if ((rc = fetch(13))) return rc;
+      mean_DY = hscal[4];
+      e_grad = hscal[8];
+      e_el = hscal[9];
+      e_dw = hscal[10];
+      e_chem = hscal[11];
+      if ((rc = zero_mode(hscal[0], hscal[1], phi_new))) {
+        // restore the time level n state as the reference does before exiting
+        return rc;
+      }
+    } else {
+      typename OpYFinal<T>::Args oa{P, Y, xB, dYdt, invN, P.Y_clip};
+      launch_zpass<T, 1, 0, OpYFinal<T>>(Nz, stream, zargs(5, 1, 0, 0), oa);
+      if ((rc = fetch(13))) return rc;
+      mean_DY = hscal[4];
+      e_grad = hscal[8];
+      e_el = hscal[9];
+      e_dw = hscal[10];
+      e_chem = hscal[11];
+    }
+    cur = 1 - cur;
+    ++steps_done;
+    return PF_OK;
+  }
+
+  // run_pf_conserved_y_zero_mode_host (main_cuda.cu:4269-4405)
+  int zero_mode(double ymin, double ymax, const double* phi_np1) {
+    const double Ycap = P.Y_clip;
+    double lo = -P.Y_clip - ymin;
+    double hi = Ycap - ymax;
+    const double roundoff = 128.0 * 2.220446049250313e-16 * std::fmax(std::fmax(std::fabs(lo), std::fabs(hi)), 1.0);
+    if (!std::isfinite(lo) || !std::isfinite(hi) || lo > hi + roundoff)
+      return fail(PF_E_ZERO_MODE, "PF zero-mode rejected: empty bracket");
+    const double target = target_mass;
+    const double tol = prm.zero_mode_tolerance_relative * std::fmax(std::fabs(target), 1.0);
+    const int g = grid_for(R);
+    double mass = 0, deriv = 0, invalid = 0;
+    auto eval = [&](double lam) -> int {
+      hipLaunchKernelGGL(zm_eval_kernel, dim3(g), dim3(NT), 0, stream, Ystar, phi_np1, lam, P.v_B, -P.Y_clip,
+                         Ycap, R, partials);
+      finalize(g, 3, RedKinds{{RED_SUM, RED_SUM, RED_SUM, 0}}, dscal + 16, 1.0);
+      PF_HIP(hipMemcpyAsync(hscal + 16, dscal + 16, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+      PF_HIP(hipStreamSynchronize(stream));
+      mass = hscal[16];
+      deriv = hscal[17];
+      invalid = hscal[18];
+      if (!std::isfinite(mass) || !std::isfinite(deriv)) return fail(PF_E_ZERO_MODE, "zero-mode evaluation non-finite");
+      return PF_OK;
+    };
+    int rc;
+    if ((rc = eval(lo))) return rc;
+    const double F_lo = mass - target;
+    if ((rc = eval(hi))) return rc;
+    const double F_hi = mass - target;
+    if (F_lo > tol || F_hi < -tol) return fail(PF_E_ZERO_MODE, "PF zero-mode rejected: target outside bracket");
+    double lam = std::fmin(std::fmax(0.0, lo), hi);
+    if ((rc = eval(lam))) return rc;
+    double r = mass - target;
+    int iters = 0;
+    for (int iteration = 0; iteration < prm.zero_mode_max_iterations && std::fabs(r) > tol; ++iteration) {
+      if (r < 0.0)
+        lo = lam;
+      else
+        hi = lam;
+      double cand = NAN;
+      if (deriv > 0.0 && std::isfinite(deriv)) cand = lam - r / deriv;
+      if (!(std::isfinite(cand) && cand > lo && cand < hi)) cand = 0.5 * (lo + hi);
+      lam = cand;
+      if ((rc = eval(lam))) return rc;
+      r = mass - target;
+      iters = iteration + 1;
+    }
+    zm_iters = iters;
+    if (std::fabs(r) > tol || invalid > 0.0) {
+      char buf[256];
+      std::snprintf(buf, sizeof buf, "PF zero-mode rejected: residual=%.17e lambda=%.17e invalid=%.0f", r, lam,
+                    invalid);
+      return fail(PF_E_ZERO_MODE, buf);
+    }
+    zm_lambda = lam;
+    hipLaunchKernelGGL(zm_apply_kernel, dim3(g), dim3(NT), 0, stream, Ystar, lam, Y, xB, dYdt, prm.dt, R);
+    return PF_OK;
+  }
+
+  // ---------------------------------------------------------------- queries
+  int get_field(int which, double* out) override {
+    PF_HIP(hipStreamSynchronize(stream));
+    const double* src = nullptr;
+    switch (which) {
+      case PF_PHI: src = phi[cur]; break;
+      case PF_XB: src = xB; break;
+      case PF_Y: src = Y; break;
+      case PF_DYDT: src = dYdt; break;
+      case PF_XBTOT: {
+        std::vector<double> ph(R), xb(R);
+        PF_HIP(hipMemcpy(ph.data(), phi[cur], R * sizeof(double), hipMemcpyDeviceToHost));
+        PF_HIP(hipMemcpy(xb.data(), xB, R * sizeof(double), hipMemcpyDeviceToHost));
+        for (long i = 0; i < R; ++i) {
+          const double h = h_of_phi(ph[i]);
+          out[i] = (1.0 - h) * xb[i] + P.v_B * h;
+        }
+        return PF_OK;
+      }
+      case PF_TRSIGMA: {
+        std::vector<T> t(R);
+        PF_HIP(hipMemcpy(t.data(), trs, R * sizeof(T), hipMemcpyDeviceToHost));
+        for (long i = 0; i < R; ++i) out[i] = double(t[i]);
+        return PF_OK;
+      }
+      default:
+        return fail(PF_E_ARG, "unknown field id");
+    }
+    PF_HIP(hipMemcpy(out, src, R * sizeof(double), hipMemcpyDeviceToHost));
+    return PF_OK;
+  }
+
+  int scalars(pf_scalars* s) override {
+    std::memset(s, 0, sizeof(*s));
+    double m, vf;
+    int rc = mass_vf(&m, &vf);
+    if (rc) return rc;
+    s->mass_xBtot = m;
+    s->mean_xBtot = m / (double)R;
+    s->volume_fraction = vf;
+    s->f_grad = e_grad;
+    s->f_dw = e_dw;
+    s->f_elastic = e_el;
+    s->f_chem = e_chem;
+    s->f_total = e_grad + e_dw + e_el + e_chem;
+    s->mean_DY = mean_DY;
+    s->last_elastic_residual = last_res;
+    s->zero_mode_lambda = zm_lambda;
+    s->last_elastic_iterations = last_iters;
+    s->zero_mode_iterations = zm_iters;
+    s->steps_done = steps_done;
+    s->elastic_iterations_total = elastic_total;
+    return PF_OK;
+  }
+
+  long long elastic_state_size() override { return nSc; }
+  int get_elastic_state(double* out) override {
+    PF_HIP(hipStreamSynchronize(stream));
+    std::vector<C> t(3 * nSc);
+    PF_HIP(hipMemcpy(t.data(), u, 3 * nSc * sizeof(C), hipMemcpyDeviceToHost));
+    for (long i = 0; i < 3 * nSc; ++i) {
+      out[2 * i] = double(t[i].re);
+      out[2 * i + 1] = double(t[i].im);
+    }
+    return PF_OK;
+  }
+  int set_elastic_state(const double* in) override {
+    std::vector<C> t(3 * nSc);
+    for (long i = 0; i < 3 * nSc; ++i) t[i] = C{T(in[2 * i]), T(in[2 * i + 1])};
+    PF_HIP(hipMemcpy(u, t.data(), 3 * nSc * sizeof(C), hipMemcpyHostToDevice));
+    warm_valid = true;
+    return PF_OK;
+  }
+
+  // ---------------------------------------------------------------- transform test
+  int fft_forward(const double* in, double* out_half) override {
+    double* f = phi[0];
+    PF_HIP(hipMemcpy(f, in, R * sizeof(double), hipMemcpyHostToDevice));
+    typename OpLoadReal<T>::Args oa{f};
+    launch_zpass<T, 0, 1, OpLoadReal<T>>(Nz, stream, zargs(0, 0, 0, 1), oa);
+    ypass(false, 0, 1);
+    xpass(false, 0, 1, Sc);
+    PF_HIP(hipStreamSynchronize(stream));
+    std::vector<C> c(nSc);
+    PF_HIP(hipMemcpy(c.data(), Sc, nSc * sizeof(C), hipMemcpyDeviceToHost));
+    const int nzc = Nz / 2 + 1;
+    std::memset(out_half, 0, sizeof(double) * 2 * (size_t)Nx * Ny * nzc);
+    for (int cx_ = 0; cx_ < ax.A; ++cx_) {
+      const int ix = cx_ < ax.a ? cx_ : cx_ + Nx - ax.A;
+      for (int cy = 0; cy < ay.A; ++cy) {
+        const int iy = cy < ay.a ? cy : cy + Ny - ay.A;
+        for (int cz = 0; cz < az.A; ++cz) {
+          const C v = c[((size_t)cx_ * ay.A + cy) * az.A + cz];
+          const size_t o = ((size_t)ix * Ny + iy) * nzc + cz;
+          out_half[2 * o] = double(v.re);
+          out_half[2 * o + 1] = double(v.im);
+        }
+      }
+    }
+    return PF_OK;
+  }
+  int fft_inverse(const double* in_half, double* out) override {
+    const int nzc = Nz / 2 + 1;
+    std::vector<C> c(nSc);
+    for (int cx_ = 0; cx_ < ax.A; ++cx_) {
+      const int ix = cx_ < ax.a ? cx_ : cx_ + Nx - ax.A;
+      for (int cy = 0; cy < ay.A; ++cy) {
+        const int iy = cy < ay.a ? cy : cy + Ny - ay.A;
+        for (int cz = 0; cz < az.A; ++cz) {
+          const size_t o = ((size_t)ix * Ny + iy) * nzc + cz;
+          c[((size_t)cx_ * ay.A + cy) * az.A + cz] = C{T(in_half[2 * o]), T(in_half[2 * o + 1])};
+        }
+      }
+    }
+    PF_HIP(hipMemcpy(Sc, c.data(), nSc * sizeof(C), hipMemcpyHostToDevice));
+    xpass(true, 0, 1, Sc);
+    ypass(true, 0, 1);
+    double* f = phi[0];
+    typename OpStoreReal<T>::Args oa{f};
+    launch_zpass<T, 1, 0, OpStoreReal<T>>(Nz, stream, zargs(0, 1, 0, 0), oa);
+    PF_HIP(hipStreamSynchronize(stream));
+    PF_HIP(hipMemcpy(out, f, R * sizeof(double), hipMemcpyDeviceToHost));
+    return PF_OK;
+  }
+
+  int sync() override {
+    PF_HIP(hipStreamSynchronize(stream));
+    return PF_OK;
+  }
+};
+
+}  // namespace pf

@@ -1,0 +1,30 @@
+// Engine interface shared by the C ABI and the two precision instantiations.
+#pragma once
+#include <memory>
+
+#include "../../include/pfcore.h"
+#include "pf_physics.hpp"
+
+namespace pf {
+
+PhysParams make_phys(const pf_params& p);
+
+struct EngineBase {
+  virtual ~EngineBase() = default;
+  virtual int init(const pf_params& p) = 0;
+  virtual int set_fields(const double* phi, const double* xB, const double* dYdt, double xB_cap) = 0;
+  virtual int step() = 0;
+  virtual int get_field(int which, double* out) = 0;
+  virtual int scalars(pf_scalars* out) = 0;
+  virtual long long elastic_state_size() = 0;
+  virtual int get_elastic_state(double* out) = 0;
+  virtual int set_elastic_state(const double* in) = 0;
+  virtual int fft_forward(const double* in, double* out_half) = 0;
+  virtual int fft_inverse(const double* in_half, double* out) = 0;
+  virtual int sync() = 0;
+};
+
+std::unique_ptr<EngineBase> make_engine_f64();
+std::unique_ptr<EngineBase> make_engine_f32();
+
+}  // namespace pf

@@ -1,0 +1,245 @@
+// LDS-staged mixed-radix FFT core for gfx950 (wave64).
+//
+// Replaces the cuFFT D2Z/Z2D (FP64) and R2C/C2R (FP32) plans the reference
+// creates at main_cuda.cu:28158-28214.  A workgroup holds BATCH transforms of
+// compile-time length N in LDS and runs a Stockham auto-sort sequence of
+// radix-4/2/3/5 stages (in place, butterflies staged through registers);
+// any other prime factor p (e.g. the 41 of 246 = 2*3*41) runs as a direct
+// p-point DFT stage out of place through a second LDS buffer.
+//
+// Element (b, j) of the batch lives at  COLMAJ ? j*BATCH + b : b*N + j.
+// Twiddles W[t] = exp(-2 pi i t / N) come from a per-length table staged in
+// LDS; INV uses conj(W).  Transforms are unnormalised, as cuFFT's are.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace pf {
+
+template <typename T>
+struct cx {
+  T re, im;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+  return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> cadd(cx<T> a, cx<T> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename T>
+__device__ __forceinline__ cx<T> csub(cx<T> a, cx<T> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename T>
+__device__ __forceinline__ cx<T> cconj(cx<T> a) { return {a.re, -a.im}; }
+
+// ----------------------------------------------------------- radix selection
+constexpr int smallest_prime_factor(int n) {
+  for (int p = 2; p * p <= n; ++p)
+    if (n % p == 0) return p;
+  return n;
+}
+constexpr int pick_radix(int n) {
+  return (n % 4 == 0) ? 4 : smallest_prime_factor(n);
+}
+constexpr bool plan_has_big_prime(int n) {
+  while (n > 1) {
+    int r = pick_radix(n);
+    if (r > 5) return true;
+    n /= r;
+  }
+  return false;
+}
+
+// ----------------------------------------------------------- small DFTs
+template <typename T, int R, bool INV>
+struct Dft;
+
+template <typename T, bool INV>
+struct Dft<T, 2, INV> {
+  static __device__ __forceinline__ void run(cx<T>* v) {
+    cx<T> a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+  }
+};
+
+template <typename T, bool INV>
+struct Dft<T, 4, INV> {
+  static __device__ __forceinline__ void run(cx<T>* v) {
+    cx<T> a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+    cx<T> a2 = cadd(v[1], v[3]), a3 = csub(v[1], v[3]);
+    // multiply a3 by -i (forward) or +i (inverse)
+    cx<T> a3r = INV ? cx<T>{-a3.im, a3.re} : cx<T>{a3.im, -a3.re};
+    v[0] = cadd(a0, a2);
+    v[2] = csub(a0, a2);
+    v[1] = cadd(a1, a3r);
+    v[3] = csub(a1, a3r);
+  }
+};
+
+template <typename T, bool INV>
+struct Dft<T, 3, INV> {
+  static __device__ __forceinline__ void run(cx<T>* v) {
+    const T c = T(-0.5);
+    const T s = INV ? T(0.86602540378443864676372317075294) : T(-0.86602540378443864676372317075294);
+    cx<T> t = cadd(v[1], v[2]);
+    cx<T> d = csub(v[1], v[2]);
+    cx<T> m = {v[0].re + c * t.re, v[0].im + c * t.im};
+    cx<T> r = {-s * d.im, s * d.re};   // i*s*d
+    v[0] = cadd(v[0], t);
+    v[1] = cadd(m, r);
+    v[2] = csub(m, r);
+  }
+};
+
+template <typename T, bool INV>
+struct Dft<T, 5, INV> {
+  static __device__ __forceinline__ void run(cx<T>* v) {
+    const T c1 = T(0.30901699437494742410229341718282);    // cos(2pi/5)
+    const T c2 = T(-0.80901699437494742410229341718282);   // cos(4pi/5)
+    const T s1a = T(0.95105651629515357211643933337938);   // sin(2pi/5)
+    const T s2a = T(0.58778525229247312916870595463907);   // sin(4pi/5)
+    const T s1 = INV ? s1a : -s1a;
+    const T s2 = INV ? s2a : -s2a;
+    cx<T> t1 = cadd(v[1], v[4]), d1 = csub(v[1], v[4]);
+    cx<T> t2 = cadd(v[2], v[3]), d2 = csub(v[2], v[3]);
+    cx<T> m1 = {v[0].re + c1 * t1.re + c2 * t2.re, v[0].im + c1 * t1.im + c2 * t2.im};
+    cx<T> m2 = {v[0].re + c2 * t1.re + c1 * t2.re, v[0].im + c2 * t1.im + c1 * t2.im};
+    // n1 = i*(s1 d1 + s2 d2), n2 = i*(s2 d1 - s1 d2)
+    T a1r = s1 * d1.re + s2 * d2.re, a1i = s1 * d1.im + s2 * d2.im;
+    T a2r = s2 * d1.re - s1 * d2.re, a2i = s2 * d1.im - s1 * d2.im;
+    cx<T> n1 = {-a1i, a1r};
+    cx<T> n2 = {-a2i, a2r};
+    v[0] = cadd(v[0], cadd(t1, t2));
+    v[1] = cadd(m1, n1);
+    v[4] = csub(m1, n1);
+    v[2] = cadd(m2, n2);
+    v[3] = csub(m2, n2);
+  }
+};
+
+// ----------------------------------------------------------- one stage
+template <int N, int BATCH, bool COLMAJ>
+__device__ __forceinline__ int lds_index(int b, int j) {
+  return COLMAJ ? j * BATCH + b : b * N + j;
+}
+
+// In-place Stockham stage for radix R <= 5 (butterflies staged in registers).
+template <typename T, int N, int R, int NS, int BATCH, bool COLMAJ, bool INV, int NT>
+__device__ __forceinline__ void stage_inplace(cx<T>* buf, const cx<T>* tw) {
+  constexpr int NB = N / R;
+  constexpr int TOT = NB * BATCH;
+  constexpr int PER = (TOT + NT - 1) / NT;
+  constexpr int TSTEP = N / (NS * R);
+  cx<T> v[PER][R];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int t = threadIdx.x + q * NT;
+    if (TOT % NT == 0 || t < TOT) {
+      int b, j;
+      if (COLMAJ) {
+        b = t % BATCH;
+        j = t / BATCH;
+      } else {
+        j = t % NB;
+        b = t / NB;
+      }
+      const int jm = j % NS;
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        cx<T> a = buf[lds_index<N, BATCH, COLMAJ>(b, j + s * NB)];
+        if (NS > 1 && s > 0) {
+          cx<T> w = tw[jm * s * TSTEP];
+          if (INV) w.im = -w.im;
+          a = cmul(a, w);
+        }
+        v[q][s] = a;
+      }
+      Dft<T, R, INV>::run(v[q]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int t = threadIdx.x + q * NT;
+    if (TOT % NT == 0 || t < TOT) {
+      int b, j;
+      if (COLMAJ) {
+        b = t % BATCH;
+        j = t / BATCH;
+      } else {
+        j = t % NB;
+        b = t / NB;
+      }
+      const int jm = j % NS;
+      const int d = (j / NS) * NS * R + jm;
+#pragma unroll
+      for (int s = 0; s < R; ++s) buf[lds_index<N, BATCH, COLMAJ>(b, d + s * NS)] = v[q][s];
+    }
+  }
+  __syncthreads();
+}
+
+// Out-of-place direct DFT stage for a prime radix R > 5.
+template <typename T, int N, int R, int NS, int BATCH, bool COLMAJ, bool INV, int NT>
+__device__ __forceinline__ void stage_prime(const cx<T>* src, cx<T>* dst, const cx<T>* tw) {
+  constexpr int NB = N / R;
+  constexpr int TSTEP = N / (NS * R);
+  constexpr int WSTEP = N / R;   // W_R^m = W_N^(m*N/R)
+  constexpr int TOT = NB * BATCH * R;   // one output element per work item
+  for (int t = threadIdx.x; t < TOT; t += NT) {
+    const int qo = t % R;
+    const int bt = t / R;
+    int b, j;
+    if (COLMAJ) {
+      b = bt % BATCH;
+      j = bt / BATCH;
+    } else {
+      j = bt % NB;
+      b = bt / NB;
+    }
+    const int jm = j % NS;
+    cx<T> acc = {T(0), T(0)};
+    int m = 0;   // (qo * s) mod R
+    for (int s = 0; s < R; ++s) {
+      cx<T> a = src[lds_index<N, BATCH, COLMAJ>(b, j + s * NB)];
+      if (NS > 1 && s > 0) {
+        cx<T> w = tw[jm * s * TSTEP];
+        if (INV) w.im = -w.im;
+        a = cmul(a, w);
+      }
+      cx<T> w2 = tw[m * WSTEP];
+      if (INV) w2.im = -w2.im;
+      acc = cadd(acc, cmul(a, w2));
+      m += qo;
+      if (m >= R) m -= R;
+    }
+    const int d = (j / NS) * NS * R + jm;
+    dst[lds_index<N, BATCH, COLMAJ>(b, d + qo * NS)] = acc;
+  }
+  __syncthreads();
+}
+
+// Runs all stages.  `buf` holds the data on entry; the result is returned
+// (either `buf` or `alt`, the second buffer needed only by big-prime plans).
+template <typename T, int N, int NS, int BATCH, bool COLMAJ, bool INV, int NT>
+__device__ __forceinline__ cx<T>* fft_stages(cx<T>* buf, cx<T>* alt, const cx<T>* tw) {
+  if constexpr (NS >= N) {
+    return buf;
+  } else {
+    constexpr int R = pick_radix(N / NS);
+    if constexpr (R <= 5) {
+      stage_inplace<T, N, R, NS, BATCH, COLMAJ, INV, NT>(buf, tw);
+      return fft_stages<T, N, NS * R, BATCH, COLMAJ, INV, NT>(buf, alt, tw);
+    } else {
+      stage_prime<T, N, R, NS, BATCH, COLMAJ, INV, NT>(buf, alt, tw);
+      return fft_stages<T, N, NS * R, BATCH, COLMAJ, INV, NT>(alt, buf, tw);
+    }
+  }
+}
+
+template <typename T, int N, int BATCH, bool COLMAJ, bool INV, int NT>
+__device__ __forceinline__ cx<T>* fft_lds(cx<T>* buf, cx<T>* alt, const cx<T>* tw) {
+  return fft_stages<T, N, 1, BATCH, COLMAJ, INV, NT>(buf, alt, tw);
+}
+
+}  // namespace pf

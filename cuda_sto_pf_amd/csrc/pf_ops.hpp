@@ -1,0 +1,530 @@
+// Point-wise stage operators fused into the z pass, and k-space operators on
+// the compact spectrum.
+//
+// z-pass operators (real space, one voxel per call):
+//   OpElasticCold  C:eps0(phi,xB)                       -> 6 spectra
+//   OpHij          eps = inv(6)/N -> h_ij               -> 6 spectra
+//   OpPhiRhs       eps(+E0) -> sigma, dF/dphi, tr sigma -> g = phi - L dt rhs
+//   OpPhiFinish    phi^{n+1} = clamp(inv/N); x_B, mu_x  -> mu spectrum, sum D x(1-x)
+//   OpFlux         grad mu -> J = M_eff grad mu; Y^n; s -> 5 spectra
+//   OpYStar        Y* = inv/N (zero mode) with min/max
+//   OpYFinal       Y = clamp(inv/N), x_B, dY/dt (no zero mode)
+// k-space operators (compact spectrum, one retained mode per thread):
+//   Green update with residual, strain from displacement, phi and Y
+//   semi-implicit updates, spectral gradient.
+#pragma once
+#include "pf_passes.hpp"
+#include "pf_physics.hpp"
+
+namespace pf {
+
+// ----------------------------------------------------------------- elastic
+// C : eps0 (the homogeneous-solution source of cuda_kernels.cu:6734-6991).
+template <typename T>
+struct OpElasticCold {
+  struct Args {
+    PhysParams P;
+    const double* phi;
+    const double* xB;
+  };
+  static constexpr int NRED = 0;
+  static constexpr int kinds[1] = {0};
+  __device__ static void apply(const Args& a, size_t g, const T*, T* out, double*) {
+    double h = h_of_phi(a.phi[g]);
+    double e0[6], s[6];
+    eigenstrain(a.P, h, clamp01(a.xB[g]), e0);
+    voigt_apply<double>(a.P.C, e0, s);
+#pragma unroll
+    for (int c = 0; c < 6; ++c) out[c] = T(s[c]);
+  }
+};
+
+// h_ij (cuda_kernels.cu:7002-7153).  FP64: all double.  FP32: eps0 and h in
+// double rounded to float, the 21-coefficient products in float, as the
+// reference does.  `increment` drops the affine part (increments of u).
+template <typename T>
+struct OpHij {
+  struct Args {
+    PhysParams P;
+    const double* phi;
+    const double* xB;
+    T invN;
+    int increment;
+  };
+  static constexpr int NRED = 0;
+  static constexpr int kinds[1] = {0};
+  __device__ static void apply(const Args& a, size_t g, const T* vin, T* out, double*) {
+    T eps[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) eps[c] = vin[c] * a.invN;
+    const double h = h_of_phi(a.phi[g]);
+    if (a.increment) {
+      hij_increment_point<T>(a.P, T(h), eps, out);
+    } else {
+      double e0[6];
+      eigenstrain(a.P, h, clamp01(a.xB[g]), e0);
+      T e0t[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) e0t[c] = T(e0[c]);
+      hij_point<T>(a.P, T(h), e0t, eps, out);
+    }
+  }
+};
+
+// ----------------------------------------------------------------- phi
+// dF/dphi explicit part (compute_phi_rhs_kernel, cuda_kernels.cu:390-552,
+// two-phase branch) and the combined semi-implicit numerator
+// g = phi - L dt rhs (phi^ and rhs^ enter the update linearly,
+// phi_semi_implicit_update_kernel cuda_kernels.cu:821-843).
+__device__ __forceinline__ double phi_rhs_chem(const PhysParams& P, double phi, double xB,
+                                               double& hp_out, double& h_out) {
+  const double hp = h_prime_of_phi(phi);
+  const double gp = g_prime_of_phi(phi);
+  const double h = h_of_phi(phi);
+  double f = P.W * gp;
+  const double muA = mu_A(P, xB);
+  const double muB = mu_B(P, xB);
+  const double delta_mu = P.mu0 - P.v_A * muA - P.v_B * muB - (P.elastic ? P.shift : 0.0);
+  const double c_bulk = c_xB_phi(P, xB, h);
+  const double mu_total = mu_tot_mix(muA, muB, xB, P.mu0, h);
+  const double Vm_alpha = P.Vm_a0 + P.dVm * xB;
+  const double volume_term = P.Vm_c - Vm_alpha + P.dVm * (xB - P.v_B);
+  f += c_bulk * hp * (delta_mu - c_bulk * mu_total * volume_term);
+  hp_out = hp;
+  h_out = h;
+  return f;
+}
+
+// Energy densities accumulated on the way (mean values after finalisation):
+//   red[0] 0.5 sigma:eps_el   (compute_gel_density_kernel, cuda_kernels.cu:1789-1850)
+//   red[1] W g(phi)           (compute_dw_energy_density_kernel, 811-815)
+//   red[2] (1-h) mu_mix + h mu0 with clamp01(phi), x clamped to [1e-12, 1-1e-12]
+//          (compute_gbulk_excess_hat_kernel 730-759, without the baseline shift)
+template <typename T, bool ELASTIC>
+struct OpPhiRhs {
+  struct Args {
+    PhysParams P;
+    const double* phi;
+    const double* xB;
+    T* trs;        // out: tr(sigma) per voxel (ELASTIC)
+    T invN;
+  };
+  static constexpr int NRED = 3;
+  static constexpr int kinds[3] = {RED_SUM, RED_SUM, RED_SUM};
+  __device__ static void apply(const Args& a, size_t g, const T* vin, T* out, double* red) {
+    const PhysParams& P = a.P;
+    const double phi = a.phi[g];
+    const double xraw = a.xB[g];
+    const double xB = clamp01(xraw);
+    double hp, h;
+    double f = phi_rhs_chem(P, phi, xB, hp, h);
+    red[1] += P.W * g_of_phi(phi);
+    {
+      const double hc = h_of_phi(clamp01(phi));
+      const double xc = clamp_fraction_eps(xraw);
+      const double mA = mu_A(P, xc), mB = mu_B(P, xc);
+      red[2] += (1.0 - hc) * ((1.0 - xc) * mA + xc * mB) + hc * P.mu0;
+    }
+    if constexpr (ELASTIC) {
+      double eps[6], e0[6], el[6], sig[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) eps[c] = double(vin[c] * a.invN) + P.E0[c];
+      eigenstrain(P, h, xB, e0);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) el[c] = eps[c] - e0[c];
+      // sigma = (C + h dC):(eps - eps0)  (cuda_kernels.cu:7381-7480)
+      voigt_apply_h<double>(P, h, el, sig);
+      const double eps_c = xB * P.eps_iso;
+      const double diag_iso = -eps_c + P.eps_iso * (xB - P.v_B);
+      const double d0 = hp * (P.eps00[0] + diag_iso);
+      const double d1 = hp * (P.eps00[1] + diag_iso);
+      const double d2 = hp * (P.eps00[2] + diag_iso);
+      const double dgel_eigen = -(sig[0] * d0 + sig[1] * d1 + sig[2] * d2 +
+                                  2.0 * sig[5] * (hp * P.eps00[5]) +
+                                  2.0 * sig[4] * (hp * P.eps00[4]) +
+                                  2.0 * sig[3] * (hp * P.eps00[3]));
+      // Q = e~^T dC e~ (cuda_kernels.cu:167-185)
+      double q[6];
+      voigt_apply<double>(P.dC, el, q);
+      const double Q = el[0] * q[0] + el[1] * q[1] + el[2] * q[2] +
+                       2.0 * el[3] * q[3] + 2.0 * el[4] * q[4] + 2.0 * el[5] * q[5];
+      f += dgel_eigen + 0.5 * hp * Q;
+      a.trs[g] = T(sig[0] + sig[1] + sig[2]);
+      red[0] += 0.5 * (sig[0] * el[0] + sig[1] * el[1] + sig[2] * el[2] +
+                       2.0 * (sig[5] * el[5] + sig[4] * el[4] + sig[3] * el[3]));
+    }
+    out[0] = T(phi - (P.L_phi * P.dt) * f);
+  }
+};
+
+// phi^{n+1} = clamp(inv/N, -1e-6, 1+1e-6) (cuda_kernels.cu:1071-1084), then the
+// mu_x evaluation (cuda_kernels.cu:1266-1324) and the D_mix x(1-x) sum of
+// reduce_sum_DY_kernel (cuda_kernels.cu:3870-3921).
+template <typename T>
+struct OpPhiFinish {
+  struct Args {
+    PhysParams P;
+    double* phi_new;
+    const double* phi_old;
+    const double* Y;
+    double* xB;
+    const T* trs;
+    T invN;
+    int phiY_is_old;   // zero mode / previous-time-level: mu uses phi^n
+  };
+  static constexpr int NRED = 1;
+  static constexpr int kinds[1] = {RED_SUM};
+  __device__ static void apply(const Args& a, size_t g, const T* vin, T* out, double* red) {
+    const PhysParams& P = a.P;
+    double v = double(vin[0]) * double(a.invN);
+    if (v < -1e-6) v = -1e-6;
+    if (v > 1.0 + 1e-6) v = 1.0 + 1e-6;
+    a.phi_new[g] = v;
+    const double phiY = a.phiY_is_old ? a.phi_old[g] : v;
+    const double Y = a.Y[g];
+    const double xB = sigmoid_from_logit(Y, P.Y_clip, P.xB_eps);
+    a.xB[g] = xB;
+    const double h = h_of_phi(phiY);
+    const double muA = mu_A(P, xB);
+    const double muB = mu_B(P, xB);
+    const double c_bulk = c_xB_phi(P, xB, h);
+    const double mu_total = mu_tot_mix(muA, muB, xB, P.mu0, h);
+    double mu = c_bulk * (muB - muA - c_bulk * mu_total * P.dVm);
+    if (P.elastic) mu += -P.eps_iso * double(a.trs[g]);
+    out[0] = T(mu);
+    double x = sigmoid_unclipped(Y);
+    if (x < 1e-12) x = 1e-12;
+    if (x > 1.0 - 1e-12) x = 1.0 - 1e-12;
+    red[0] += D_mix(P, h) * (x * (1.0 - x));
+  }
+};
+
+// J_a = M_eff grad_a mu (compute_flux_single_component_kernel,
+// cuda_kernels.cu:1985-2012) and the explicit source of the Y right-hand side
+// s = term_h + term_gamma (compute_Y_rhs_sm_tangent_n_kernel 2268-2359 with the
+// zero mode, compute_Y_rhs_kernel 2185-2266 without), plus Y^n itself.
+template <typename T>
+struct OpFlux {
+  struct Args {
+    PhysParams P;
+    const double* phiY;
+    const double* xBp;
+    const double* phi_n;
+    const double* phi_np1;
+    const double* Y;
+    const double* dYdt;
+    T invN;
+    int zero_mode;
+    int prev_level;      // enable_Y_rhs_previous_time_level
+    int no_gamma;        // disable_Y_rhs_gamma_term
+    double term_h_scale;
+  };
+  static constexpr int NRED = 0;
+  static constexpr int kinds[1] = {0};
+  __device__ static void apply(const Args& a, size_t g, const T* vin, T* out, double*) {
+    const PhysParams& P = a.P;
+    const double hY = h_of_phi(a.phiY[g]);
+    const double Dm = D_mix(P, hY);
+    const double G = gamma_thermo(P, a.xBp[g], hY);
+    const double Meff = stabilized_meff(Dm, G);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c] = T(Meff * (double(vin[c]) * double(a.invN)));
+    const double Y = a.Y[g];
+    double x = sigmoid_unclipped(Y);
+    if (x < 1e-12) x = 1e-12;
+    if (x > 1.0 - 1e-12) x = 1.0 - 1e-12;
+    const double lg = x * (1.0 - x);
+    const double pn = a.phi_n[g];
+    const double pn1 = a.phi_np1[g];
+    double s;
+    if (a.zero_mode) {
+      const double h_n = h_of_phi(pn);
+      const double term_h = h_prime_of_phi(pn) * ((pn1 - pn) / P.dt) * (P.v_B - x);
+      const double gamma_local = (1.0 - h_n) * lg - 1.0;
+      s = term_h + gamma_local * a.dYdt[g];
+    } else {
+      const double h_new = h_of_phi(pn1);
+      const double h_old = h_of_phi(pn);
+      const double term_h = a.term_h_scale * ((h_new - h_old) / P.dt) * (P.v_B - x);
+      const double hx = a.prev_level ? h_old : h_new;
+      const double gamma_local = (1.0 - hx) * lg - 1.0;
+      const double term_gamma = a.no_gamma ? 0.0 : gamma_local * a.dYdt[g];
+      s = term_h + term_gamma;
+    }
+    out[3] = T(Y);
+    out[4] = T(s);
+  }
+};
+
+// Y* = inv/N before the zero-mode shift (copy_scaled_pf_Y_kernel
+// cuda_kernels.cu:3401-3407), with its min/max for the bracket.
+template <typename T>
+struct OpYStar {
+  struct Args {
+    double* Ystar;
+    T invN;
+  };
+  static constexpr int NRED = 2;
+  static constexpr int kinds[2] = {RED_MIN, RED_MAX};
+  __device__ static void apply(const Args& a, size_t g, const T* vin, T*, double* red) {
+    const double y = double(vin[0]) * double(a.invN);
+    a.Ystar[g] = y;
+    red[0] = fmin(red[0], y);
+    red[1] = fmax(red[1], y);
+  }
+};
+
+// Without the zero mode: Y = clamp(inv/N), x_B = sigma_clip(Y)
+// (Y_normalize_and_clamp_kernel cuda_kernels.cu:2554-2573) and dY/dt
+// (update_dY_dt_prev_kernel cuda_kernels.cu:3545-3556).
+template <typename T>
+struct OpYFinal {
+  struct Args {
+    PhysParams P;
+    double* Y;
+    double* xB;
+    double* dYdt;
+    T invN;
+    double Y_upper_cap;
+  };
+  static constexpr int NRED = 0;
+  static constexpr int kinds[1] = {0};
+  __device__ static void apply(const Args& a, size_t g, const T* vin, T*, double*) {
+    double y = double(vin[0]) * double(a.invN);
+    if (y > a.Y_upper_cap) y = a.Y_upper_cap;
+    if (y < -a.P.Y_clip) y = -a.P.Y_clip;
+    const double yn = a.Y[g];
+    a.Y[g] = y;
+    a.xB[g] = sigmoid_from_logit(y, a.P.Y_clip, a.P.xB_eps);
+    a.dYdt[g] = (y - yn) / a.P.dt;
+  }
+};
+
+// ----------------------------------------------------------------- k space
+struct KDims {
+  int Ax, Ay, Az;
+  const double* kx;   // [Ax] wrapped k of the compact x index
+  const double* ky;   // [Ay]
+  const double* kz;   // [Az]
+  int Nz_even;
+};
+
+__device__ __forceinline__ void kvec(const KDims& d, long c, double& kx, double& ky, double& kz,
+                                     int& iz) {
+  iz = (int)(c % d.Az);
+  const long r = c / d.Az;
+  const int iy = (int)(r % d.Ay);
+  const int ix = (int)(r / d.Ay);
+  kx = d.kx[ix];
+  ky = d.ky[iy];
+  kz = d.kz[iz];
+}
+
+// Acoustic tensor K_ij = C_ipjq k_p k_q and its inverse by cofactors
+// (cuda_kernels.cu:6484-6633, 6818-6878).  Returns false when |det| < 1e-30.
+__device__ __forceinline__ bool green_tensor(const double* C, double kx, double ky, double kz,
+                                             double* G) {
+  const double kx2 = kx * kx, ky2 = ky * ky, kz2 = kz * kz;
+#define CC(i, j) C[vidx(i - 1, j - 1)]
+  const double K11 = kx2 * CC(1, 1) + 2.0 * kx * (kz * CC(1, 5) + ky * CC(1, 6)) + kz2 * CC(5, 5) +
+                     2.0 * ky * kz * CC(5, 6) + ky2 * CC(6, 6);
+  const double K22 = ky2 * CC(2, 2) + 2.0 * ky * (kz * CC(2, 4) + kx * CC(2, 6)) + kz2 * CC(4, 4) +
+                     2.0 * kx * kz * CC(4, 6) + kx2 * CC(6, 6);
+  const double K33 = kz2 * CC(3, 3) + 2.0 * kz * (ky * CC(3, 4) + kx * CC(3, 5)) + ky2 * CC(4, 4) +
+                     2.0 * kx * ky * CC(4, 5) + kx2 * CC(5, 5);
+  const double K13 = kx2 * CC(1, 5) + kz2 * CC(3, 5) + ky * kz * (CC(3, 6) + CC(4, 5)) +
+                     ky2 * CC(4, 6) + kx * kz * (CC(1, 3) + CC(5, 5)) + kx * ky * (CC(1, 4) + CC(5, 6));
+  const double K23 = ky2 * CC(2, 4) + kz2 * CC(3, 4) + ky * kz * (CC(2, 3) + CC(4, 4)) +
+                     kx * kz * (CC(3, 6) + CC(4, 5)) + kx * ky * (CC(2, 5) + CC(4, 6)) + kx2 * CC(5, 6);
+  const double K12 = kx2 * CC(1, 6) + ky2 * CC(2, 6) + kz2 * CC(4, 5) + ky * kz * (CC(2, 5) + CC(4, 6)) +
+                     kx * kz * (CC(1, 4) + CC(5, 6)) + kx * ky * (CC(1, 2) + CC(6, 6));
+#undef CC
+  const double det = K13 * (-K13 * K22 + K23 * K12) - K23 * (K23 * K11 - K13 * K12) +
+                     K33 * (K22 * K11 - K12 * K12);
+  if (fabs(det) < 1e-30) return false;
+  G[0] = (-K23 * K23 + K33 * K22) / det;   // 11
+  G[1] = (K23 * K13 - K33 * K12) / det;    // 12
+  G[2] = (-K13 * K22 + K23 * K12) / det;   // 13
+  G[3] = (-K13 * K13 + K33 * K11) / det;   // 22
+  G[4] = (-K23 * K11 + K13 * K12) / det;   // 23
+  G[5] = (K22 * K11 - K12 * K12) / det;    // 33
+  return true;
+}
+
+// eps_ij(k) = i (k_i u_j + k_j u_i)/2 (cuda_kernels.cu:6673-6726)
+template <typename T>
+__device__ __forceinline__ void strain_of(double kx, double ky, double kz, const cx<double>* u,
+                                          cx<T>* const* e, long c) {
+  auto ik = [](double k, cx<double> v) { return cx<double>{-k * v.im, k * v.re}; };
+  cx<double> exx = ik(kx, u[0]);
+  cx<double> eyy = ik(ky, u[1]);
+  cx<double> ezz = ik(kz, u[2]);
+  cx<double> a = ik(ky, u[2]), b = ik(kz, u[1]);
+  cx<double> eyz = {0.5 * (a.re + b.re), 0.5 * (a.im + b.im)};
+  a = ik(kz, u[0]);
+  b = ik(kx, u[2]);
+  cx<double> exz = {0.5 * (a.re + b.re), 0.5 * (a.im + b.im)};
+  a = ik(ky, u[0]);
+  b = ik(kx, u[1]);
+  cx<double> exy = {0.5 * (a.re + b.re), 0.5 * (a.im + b.im)};
+  e[0][c] = cx<T>{T(exx.re), T(exx.im)};
+  e[1][c] = cx<T>{T(eyy.re), T(eyy.im)};
+  e[2][c] = cx<T>{T(ezz.re), T(ezz.im)};
+  e[3][c] = cx<T>{T(eyz.re), T(eyz.im)};
+  e[4][c] = cx<T>{T(exz.re), T(exz.im)};
+  e[5][c] = cx<T>{T(exy.re), T(exy.im)};
+}
+
+template <typename T>
+struct GreenArgs {
+  KDims d;
+  long n;
+  const cx<T>* h[6];     // h_ij spectra
+  const cx<T>* u_old[3]; // previous iterate (residual)
+  cx<T>* u_new[3];       // out
+  cx<T>* eps[6];         // out: strain of u_new
+  double* partials;      // [grid][2] (|du|^2, |u|^2) hermitian-weighted, or null
+  double C[21];
+  int have_old;
+};
+
+// u(k) = -i Omega(k) (k . h(k)); k = 0 and singular det give 0.
+template <typename T>
+__global__ void __launch_bounds__(NT) kop_green(GreenArgs<T> ga) {
+  double acc[2] = {0.0, 0.0};
+  for (long c = blockIdx.x * (long)NT + threadIdx.x; c < ga.n; c += (long)gridDim.x * NT) {
+    double kx, ky, kz;
+    int iz;
+    kvec(ga.d, c, kx, ky, kz, iz);
+    cx<double> hh[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) hh[i] = cx<double>{double(ga.h[i][c].re), double(ga.h[i][c].im)};
+    double G[6];
+    cx<double> u[3] = {{0, 0}, {0, 0}, {0, 0}};
+    if (c != 0 && green_tensor(ga.C, kx, ky, kz, G)) {
+      // t = k . h   (rows of the symmetric h)
+      cx<double> t1 = {kx * hh[0].re + ky * hh[5].re + kz * hh[4].re, kx * hh[0].im + ky * hh[5].im + kz * hh[4].im};
+      cx<double> t2 = {kx * hh[5].re + ky * hh[1].re + kz * hh[3].re, kx * hh[5].im + ky * hh[1].im + kz * hh[3].im};
+      cx<double> t3 = {kx * hh[4].re + ky * hh[3].re + kz * hh[2].re, kx * hh[4].im + ky * hh[3].im + kz * hh[2].im};
+      const double g[3][3] = {{G[0], G[1], G[2]}, {G[1], G[3], G[4]}, {G[2], G[4], G[5]}};
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        cx<double> s = {g[i][0] * t1.re + g[i][1] * t2.re + g[i][2] * t3.re,
+                        g[i][0] * t1.im + g[i][1] * t2.im + g[i][2] * t3.im};
+        u[i] = cx<double>{s.im, -s.re};   // -i * s
+      }
+    }
+    cx<double> uq[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      cx<T> q = {T(u[i].re), T(u[i].im)};
+      ga.u_new[i][c] = q;
+      uq[i] = cx<double>{double(q.re), double(q.im)};
+    }
+    if (ga.partials) {
+      // Hermitian weight of compute_elastic_displacement_residual_kernel
+      // (cuda_kernels.cu:6436-6467): 1 on the kz = 0 plane, 2 elsewhere (the
+      // Nyquist plane is never retained).
+      const double w = (iz == 0) ? 1.0 : 2.0;
+      double dsum = 0.0, rsum = 0.0;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        double dr = uq[i].re, di = uq[i].im;
+        if (ga.have_old) {
+          dr -= double(ga.u_old[i][c].re);
+          di -= double(ga.u_old[i][c].im);
+        }
+        dsum += dr * dr + di * di;
+        rsum += uq[i].re * uq[i].re + uq[i].im * uq[i].im;
+      }
+      acc[0] += w * dsum;
+      acc[1] += w * rsum;
+    }
+    strain_of<T>(kx, ky, kz, uq, ga.eps, c);
+  }
+  if (ga.partials) {
+    int kinds[2] = {RED_SUM, RED_SUM};
+    block_reduce_store<2>(acc, kinds, ga.partials + (size_t)blockIdx.x * 2);
+  }
+}
+
+template <typename T>
+struct StrainArgs {
+  KDims d;
+  long n;
+  const cx<T>* u[3];
+  cx<T>* eps[6];
+};
+template <typename T>
+__global__ void __launch_bounds__(NT) kop_strain(StrainArgs<T> sa) {
+  for (long c = blockIdx.x * (long)NT + threadIdx.x; c < sa.n; c += (long)gridDim.x * NT) {
+    double kx, ky, kz;
+    int iz;
+    kvec(sa.d, c, kx, ky, kz, iz);
+    cx<double> u[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) u[i] = cx<double>{double(sa.u[i][c].re), double(sa.u[i][c].im)};
+    strain_of<T>(kx, ky, kz, u, sa.eps, c);
+  }
+}
+
+// phi^ <- g^ / (1 + L dt kappa k^2)
+template <typename T>
+__global__ void __launch_bounds__(NT) kop_phi(KDims d, long n, cx<T>* g, double Ldt_kappa) {
+  for (long c = blockIdx.x * (long)NT + threadIdx.x; c < n; c += (long)gridDim.x * NT) {
+    double kx, ky, kz;
+    int iz;
+    kvec(d, c, kx, ky, kz, iz);
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    const double den = 1.0 + Ldt_kappa * k2;
+    cx<T> v = g[c];
+    g[c] = cx<T>{T(double(v.re) / den), T(double(v.im) / den)};
+  }
+}
+
+// grad_a mu = i k_a mu^ (compute_gradient_single_component_k_kernel 1887-1911)
+template <typename T>
+__global__ void __launch_bounds__(NT) kop_grad(KDims d, long n, const cx<T>* mu, cx<T>* gx,
+                                               cx<T>* gy, cx<T>* gz) {
+  for (long c = blockIdx.x * (long)NT + threadIdx.x; c < n; c += (long)gridDim.x * NT) {
+    double kx, ky, kz;
+    int iz;
+    kvec(d, c, kx, ky, kz, iz);
+    const double mr = double(mu[c].re), mi = double(mu[c].im);
+    gx[c] = cx<T>{T(-kx * mi), T(kx * mr)};
+    gy[c] = cx<T>{T(-ky * mi), T(ky * mr)};
+    gz[c] = cx<T>{T(-kz * mi), T(kz * mr)};
+  }
+}
+
+// Y^ <- (Y^n + dt f^)/(1 + dt Dbar k^2),  f^ = i k.J^ + Dbar k^2 Y^n - s^
+// (Y_semi_implicit_update_kernel 2528-2550 with divJ_accumulate 2112-2137
+// and the spectral Laplacian 3528-3538 folded in)
+template <typename T>
+__global__ void __launch_bounds__(NT) kop_yupdate(KDims d, long n, const cx<T>* jx,
+                                                  const cx<T>* jy, const cx<T>* jz,
+                                                  const cx<T>* yn, const cx<T>* s, cx<T>* out,
+                                                  const double* dbar, double dt) {
+  const double D = *dbar;
+  for (long c = blockIdx.x * (long)NT + threadIdx.x; c < n; c += (long)gridDim.x * NT) {
+    double kx, ky, kz;
+    int iz;
+    kvec(d, c, kx, ky, kz, iz);
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    // div J^ = i (kx Jx + ky Jy + kz Jz), accumulated x, y, z in order
+    double dr = 0.0, di = 0.0;
+    dr += -kx * double(jx[c].im);
+    di += kx * double(jx[c].re);
+    dr += -ky * double(jy[c].im);
+    di += ky * double(jy[c].re);
+    dr += -kz * double(jz[c].im);
+    di += kz * double(jz[c].re);
+    const double ynr = double(yn[c].re), yni = double(yn[c].im);
+    const double fr = dr + D * k2 * ynr - double(s[c].re);
+    const double fi = di + D * k2 * yni - double(s[c].im);
+    const double den = 1.0 + dt * D * k2;
+    out[c] = cx<T>{T((ynr + dt * fr) / den), T((yni + dt * fi) / den)};
+  }
+}
+
+}  // namespace pf

@@ -1,0 +1,263 @@
+// FFT pass kernels of the pruned 3-D transform.
+//
+// Spectral data never exists in the reference's full half-spectrum form: the
+// reference zeroes every mode with |k_i| >= (2/3) pi / d_i after every
+// transform (dealias_23_kernel, cuda_kernels.cu:103-161, applied at every
+// cufftExec* site of main_cuda.cu:28846-30806), so only the retained modes are
+// ever computed or stored.  Per axis the retained wrapped indices are
+// |ii| < a (x, y) and kz < az (z); they are computed on the host with the
+// reference's own floating-point expression.
+//
+// Layouts (complex, last index fastest):
+//   Sz  [Nx][Ny][Az]   after the z pass (real rows -> Az retained kz)
+//   Sy  [Nx][Ay][Az]   after the y pass
+//   Sc  [Ax][Ay][Az]   compact spectrum after the x pass
+// Compact index c of a wrapped axis maps to the full index c (c < a) or
+// c + N - A (c >= a), A = 2a - 1.
+#pragma once
+#include "pf_fft.hpp"
+
+namespace pf {
+
+constexpr int NT = 256;                 // threads per workgroup (4 wave64)
+constexpr int LDS_BUDGET = 48 * 1024;   // bytes of FFT data per workgroup
+
+template <typename T, int N, int NC>
+constexpr int zpass_rows() {
+  constexpr int per_pair = NC * N * (int)sizeof(cx<T>) * (plan_has_big_prime(N) ? 2 : 1);
+  constexpr int pairs = LDS_BUDGET / per_pair;
+  constexpr int p = pairs < 1 ? 1 : (pairs > 32 ? 32 : pairs);
+  return 2 * p;
+}
+
+template <typename T, int N>
+constexpr int cpass_cols() {
+  constexpr int per_col = N * (int)sizeof(cx<T>) * (plan_has_big_prime(N) ? 2 : 1);
+  constexpr int c = LDS_BUDGET / per_col;
+  return c >= 64 ? 64 : c >= 32 ? 32 : c >= 16 ? 16 : c >= 8 ? 8 : c >= 4 ? 4 : c >= 2 ? 2 : 1;
+}
+
+// --------------------------------------------------------------- reductions
+enum RedKind { RED_SUM = 0, RED_MIN = 1, RED_MAX = 2 };
+
+__device__ __forceinline__ double red_identity(int kind) {
+  return kind == RED_SUM ? 0.0 : (kind == RED_MIN ? 1.0e300 : -1.0e300);
+}
+__device__ __forceinline__ double red_combine(int kind, double a, double b) {
+  return kind == RED_SUM ? a + b : (kind == RED_MIN ? fmin(a, b) : fmax(a, b));
+}
+
+// Deterministic block reduction: fixed xor-butterfly inside each wave64,
+// then wave partials combined in wave order by thread 0.
+template <int NRED>
+__device__ __forceinline__ void block_reduce_store(double* vals, const int* kinds, double* out) {
+  __shared__ double wsum[NRED > 0 ? NRED : 1][NT / 64];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int r = 0; r < NRED; ++r) {
+    double v = vals[r];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = red_combine(kinds[r], v, __shfl_xor(v, off, 64));
+    if (lane == 0) wsum[r][wave] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int r = 0; r < NRED; ++r) {
+      double v = wsum[r][0];
+      for (int w = 1; w < NT / 64; ++w) v = red_combine(kinds[r], v, wsum[r][w]);
+      out[r] = v;
+    }
+  }
+}
+
+// --------------------------------------------------------------- z pass
+template <typename T>
+struct ZPassArgs {
+  int rows;          // Nx*Ny
+  int Az;            // retained kz count
+  const cx<T>* in[6];
+  cx<T>* out[6];
+  const cx<T>* tw;   // twiddles of length Nz
+  double* partials;  // [gridDim.x][Op::NRED]
+};
+
+// One workgroup: RT consecutive rows, all components.
+//  1. NCI>0: Hermitian-extend the Az retained kz of row pairs, inverse complex
+//     FFT (two real rows per complex row), giving N * (real rows).
+//  2. Op::apply at every (row, z): consumes NCI reals, produces NCO reals and
+//     may read/write real fields and accumulate reductions.
+//  3. NCO>0: forward complex FFT of row pairs, split into two real-row
+//     spectra, store the Az retained kz.
+template <typename T, int N, int NCI, int NCO, class Op>
+__global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op::Args oa) {
+  constexpr int NCM = NCI > NCO ? NCI : NCO;
+  constexpr int NCB = NCM > 0 ? NCM : 1;
+  constexpr int RT = zpass_rows<T, N, NCB>();
+  constexpr int NP = RT / 2;
+  constexpr bool BIG = plan_has_big_prime(N);
+  __shared__ cx<T> buf[NCB * NP * N];
+  __shared__ cx<T> alt[BIG ? NCB * NP * N : 1];
+  __shared__ cx<T> tw[N];
+
+  const int row0 = blockIdx.x * RT;
+  const int Az = za.Az;
+  for (int i = threadIdx.x; i < N; i += NT) tw[i] = za.tw[i];
+
+  if constexpr (NCI > 0) {
+    // zero the unretained middle, then scatter the retained modes
+    constexpr int TOTZ = NCI * NP * N;
+    for (int e = threadIdx.x; e < TOTZ; e += NT) {
+      const int k = e % N;
+      if (k >= Az && N - k >= Az) buf[e] = cx<T>{T(0), T(0)};
+    }
+    const int TOTL = NCI * NP * Az;
+    for (int e = threadIdx.x; e < TOTL; e += NT) {
+      const int k = e % Az;
+      const int cp = e / Az;   // c*NP + p
+      const int p = cp % NP;
+      const int c = cp / NP;
+      const int ra = row0 + 2 * p, rb = ra + 1;
+      cx<T> A = (ra < za.rows) ? za.in[c][(size_t)ra * Az + k] : cx<T>{T(0), T(0)};
+      cx<T> B = (rb < za.rows) ? za.in[c][(size_t)rb * Az + k] : cx<T>{T(0), T(0)};
+      if (k == 0) {
+        A.im = T(0);
+        B.im = T(0);
+      }
+      cx<T>* rowp = buf + (size_t)cp * N;
+      rowp[k] = cx<T>{A.re - B.im, A.im + B.re};
+      if (k > 0) rowp[N - k] = cx<T>{A.re + B.im, B.re - A.im};
+    }
+    __syncthreads();
+    cx<T>* res = fft_lds<T, N, NCI * NP, false, true, NT>(buf, alt, tw);
+    if (BIG && res != buf) {
+      for (int e = threadIdx.x; e < NCI * NP * N; e += NT) buf[e] = res[e];
+      __syncthreads();
+    }
+  }
+
+  // point-wise operator
+  double red[Op::NRED > 0 ? Op::NRED : 1];
+#pragma unroll
+  for (int r = 0; r < Op::NRED; ++r) red[r] = red_identity(Op::kinds[r]);
+  for (int e = threadIdx.x; e < RT * N; e += NT) {
+    const int z = e % N;
+    const int r = e / N;
+    const int row = row0 + r;
+    const int p = r >> 1;
+    const bool hi = r & 1;
+    T vin[NCI > 0 ? NCI : 1];
+    T vout[NCO > 0 ? NCO : 1];
+#pragma unroll
+    for (int c = 0; c < NCI; ++c)
+      vin[c] = reinterpret_cast<const T*>(&buf[(c * NP + p) * N + z])[hi ? 1 : 0];
+    if (row < za.rows) {
+      Op::apply(oa, (size_t)row * N + z, vin, vout, red);
+    } else {
+#pragma unroll
+      for (int c = 0; c < NCO; ++c) vout[c] = T(0);
+    }
+    if constexpr (NCO > 0) {
+      // both rows of a pair are handled by different threads: write halves
+#pragma unroll
+      for (int c = 0; c < NCO; ++c) {
+        T* slot = reinterpret_cast<T*>(&buf[(c * NP + p) * N + z]);
+        slot[hi ? 1 : 0] = vout[c];
+      }
+    }
+  }
+  if constexpr (Op::NRED > 0) {
+    int kinds[Op::NRED > 0 ? Op::NRED : 1];
+#pragma unroll
+    for (int r = 0; r < Op::NRED; ++r) kinds[r] = Op::kinds[r];
+    block_reduce_store<Op::NRED>(red, kinds, za.partials + (size_t)blockIdx.x * Op::NRED);
+  }
+
+  if constexpr (NCO > 0) {
+    __syncthreads();
+    cx<T>* res = fft_lds<T, N, NCO * NP, false, false, NT>(buf, alt, tw);
+    const int TOTS = NCO * NP * Az;
+    for (int e = threadIdx.x; e < TOTS; e += NT) {
+      const int k = e % Az;
+      const int cp = e / Az;
+      const int p = cp % NP;
+      const int c = cp / NP;
+      const int ra = row0 + 2 * p, rb = ra + 1;
+      const cx<T>* rowp = res + (size_t)cp * N;
+      cx<T> Zk = rowp[k];
+      cx<T> Zm = rowp[k == 0 ? 0 : N - k];
+      cx<T> A = {T(0.5) * (Zk.re + Zm.re), T(0.5) * (Zk.im - Zm.im)};
+      cx<T> B = {T(0.5) * (Zk.im + Zm.im), T(0.5) * (Zm.re - Zk.re)};
+      if (ra < za.rows) za.out[c][(size_t)ra * Az + k] = A;
+      if (rb < za.rows) za.out[c][(size_t)rb * Az + k] = B;
+    }
+  }
+}
+
+// --------------------------------------------------------------- column pass
+// Strided complex FFT along y (planes = Nx) or x (planes = 1) with pruning.
+// Forward: N full inputs -> A retained outputs.  Inverse: A -> N.
+template <typename T>
+struct CPassArgs {
+  const cx<T>* in;
+  cx<T>* out;
+  int cols;                 // contiguous columns per j (Az for y, Ay*Az for x)
+  long in_pstride, out_pstride;   // plane strides (elements)
+  long in_cstride, out_cstride;   // component strides (elements)
+  int A, a;                 // compact count and retained half-width
+  const cx<T>* tw;
+};
+
+template <typename T, int N, bool INV>
+__global__ void __launch_bounds__(NT) cpass_kernel(CPassArgs<T> ca) {
+  constexpr int TC = cpass_cols<T, N>();
+  constexpr bool BIG = plan_has_big_prime(N);
+  __shared__ cx<T> buf[N * TC];
+  __shared__ cx<T> alt[BIG ? N * TC : 1];
+  __shared__ cx<T> tw[N];
+  const int col0 = blockIdx.x * TC;
+  const int plane = blockIdx.y;
+  const int comp = blockIdx.z;
+  const int cols = ca.cols;
+  const int A = ca.A, a = ca.a;
+  const cx<T>* in = ca.in + comp * ca.in_cstride + plane * ca.in_pstride;
+  cx<T>* out = ca.out + comp * ca.out_cstride + plane * ca.out_pstride;
+  for (int i = threadIdx.x; i < N; i += NT) tw[i] = ca.tw[i];
+  if (!INV) {
+    for (int e = threadIdx.x; e < N * TC; e += NT) {
+      const int b = e % TC, j = e / TC;
+      const int col = col0 + b;
+      buf[e] = (col < cols) ? in[(size_t)j * cols + col] : cx<T>{T(0), T(0)};
+    }
+  } else {
+    for (int e = threadIdx.x; e < N * TC; e += NT) {
+      const int j = e / TC;
+      if (j >= a && j <= N - a) buf[e] = cx<T>{T(0), T(0)};
+    }
+    for (int e = threadIdx.x; e < A * TC; e += NT) {
+      const int b = e % TC, jc = e / TC;
+      const int col = col0 + b;
+      const int jf = jc < a ? jc : jc + N - A;
+      buf[jf * TC + b] = (col < cols) ? in[(size_t)jc * cols + col] : cx<T>{T(0), T(0)};
+    }
+  }
+  __syncthreads();
+  cx<T>* res = fft_lds<T, N, TC, true, INV, NT>(buf, alt, tw);
+  if (!INV) {
+    for (int e = threadIdx.x; e < A * TC; e += NT) {
+      const int b = e % TC, jc = e / TC;
+      const int col = col0 + b;
+      const int jf = jc < a ? jc : jc + N - A;
+      if (col < cols) out[(size_t)jc * cols + col] = res[jf * TC + b];
+    }
+  } else {
+    for (int e = threadIdx.x; e < N * TC; e += NT) {
+      const int b = e % TC, j = e / TC;
+      const int col = col0 + b;
+      if (col < cols) out[(size_t)j * cols + col] = res[e];
+    }
+  }
+}
+
+}  // namespace pf

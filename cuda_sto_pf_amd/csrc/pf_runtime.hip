@@ -1,0 +1,254 @@
+// C ABI of libpfcore: handles, parameters and the physical constants hoisted
+// from the per-voxel thermodynamics.
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "../../include/pfcore.h"
+#include "pf_engine_api.hpp"
+#include "pf_error.hpp"
+
+namespace pf {
+
+std::string& last_error() {
+  static thread_local std::string msg;
+  return msg;
+}
+
+namespace {
+const double kR = 8.31446261815324;   // thermo_utils.h:260
+
+// thermo_utils.h:316-351 (SGTE lattice stabilities, Gierlotka 2010)
+double GHSER_Pb(double T) {
+  if (T < 600.61)
+    return -7650.085 + 101.700244 * T - 24.5242231 * T * std::log(T) - 0.00365895 * T * T -
+           2.4395e-7 * std::pow(T, 3);
+  return -10531.095 + 154.243182 * T - 32.4913959 * T * std::log(T) + 0.00154613 * T * T +
+         8.05448e25 * std::pow(T, -9);
+}
+double GHSER_Ag(double T) {
+  if (T < 1234.93)
+    return -7209.512 + 118.202013 * T - 23.8463314 * T * std::log(T) - 0.001790585 * T * T -
+           3.98587e-7 * std::pow(T, 3) - 12011.0 / T;
+  return -15095.252 + 190.266404 * T - 33.472 * T * std::log(T) + 1.411773e29 * std::pow(T, -9);
+}
+double GHSER_Te(double T) {
+  if (T < 722.66)
+    return -10544.679 + 183.372894 * T - 35.6687 * T * std::log(T) + 0.01583435 * T * T -
+           5.240417e-6 * std::pow(T, 3) + 155015.0 / T;
+  return 9160.595 - 129.265373 * T + 13.004 * T * std::log(T) - 0.0362361 * T * T +
+         5.006367e-6 * std::pow(T, 3) - 1.28681e30 * std::pow(T, -9);
+}
+double G_PbTe(double T) { return (-76063.2138 + 9.67716633 * T) + GHSER_Pb(T) + GHSER_Te(T); }
+double G_Ag2Te(double T) {
+  double base = -10128.93 - 12.645115 * T;
+  double g = base + (2.0 / 3.0) * GHSER_Ag(T) + (1.0 / 3.0) * GHSER_Te(T);
+  return 3.0 * g;
+}
+// thermo_utils.h:453-481
+double solve_x_eq(double T) {
+  double L = 41212.9 - 18.05 * T, RT = kR * T;
+  double x = std::exp(-L / RT);
+  if (x < 1e-9) x = 1e-9;
+  if (x > 0.99) x = 0.5;
+  for (int i = 0; i < 20; ++i) {
+    double f = RT * std::log(x) + L * (1.0 - x) * (1.0 - x);
+    double df = RT / x - 2.0 * L * (1.0 - x);
+    double delta = f / df;
+    x -= delta;
+    if (x < 1e-10) x = 1e-10;
+    if (x > 0.999) x = 0.999;
+    if (std::fabs(delta) < 1e-8) break;
+  }
+  return x;
+}
+}  // namespace
+
+PhysParams make_phys(const pf_params& p) {
+  PhysParams P;
+  std::memset(&P, 0, sizeof(P));
+  double T = p.temperature_C + 273.15;
+  if (T < 1.0) T = 1.0;
+  P.T = T;
+  P.RT = kR * T;
+  P.Lparam = 41212.9 - 18.05 * T;
+  P.G_A0 = G_PbTe(T);
+  P.G_B0 = G_Ag2Te(T);
+  double sc = p.mu_reference_scale;
+  if (std::fabs(sc) < 1e-30) sc = (sc >= 0.0) ? 1e-30 : -1e-30;
+  P.mu_scale = sc;
+  P.convex = p.thermo_convex_extrapolation_enabled ? 1 : 0;
+  {
+    // constants of the convex continuation at x = 0.09 (thermo_utils.h:400-430)
+    const double xc = 0.09;
+    P.muA_c = P.G_A0 + P.RT * std::log(1.0 - xc) + P.Lparam * xc * xc;
+    P.dmuA_c = -kR * T / (1.0 - xc) + 2.0 * P.Lparam * xc;
+    P.muB_c = P.G_B0 + P.RT * std::log(xc) + P.Lparam * (1.0 - xc) * (1.0 - xc);
+    P.dmuB_c = kR * T / xc - 2.0 * P.Lparam * (1.0 - xc);
+  }
+  // mu0_compound (pfparams_refresh_thermo, main_cuda.cu:15604-15614)
+  {
+    const double xeq = solve_x_eq(T);
+    const double muA = mu_A(P, xeq), muB = mu_B(P, xeq);
+    double vs = p.v_A + p.v_B;
+    if (std::fabs(vs) < 1e-30) vs = (vs >= 0.0) ? 1e-30 : -1e-30;
+    P.mu0 = (p.v_A * muA + p.v_B * muB) / vs;
+  }
+  P.v_A = p.v_A;
+  P.v_B = p.v_B;
+  P.Vm_c = p.Vm_compound;
+  P.Vm_a0 = p.Vm_alpha_0;
+  P.dVm = p.dVm_alpha_dxB;
+  P.W = p.W;
+  P.kappa = p.kappa_phi;
+  P.L_phi = p.L_phi;
+  P.D_a = p.D_alpha;
+  P.D_c = p.D_compound;
+  P.eps_iso = p.eps_iso_over_vB;
+  P.shift = p.elastic_shift_dimless;
+  P.Y_clip = p.Y_clip;
+  P.xB_eps = p.xB_eps;
+  P.dt = p.dt;
+  P.elastic = p.elastic_enabled ? 1 : 0;
+  for (int i = 0; i < 21; ++i) {
+    P.C[i] = p.S[i];
+    P.dC[i] = p.S_p[i];
+  }
+  for (int i = 0; i < 6; ++i) {
+    P.eps00[i] = p.eps00[i];
+    P.E0[i] = p.E0[i];
+  }
+  return P;
+}
+
+}  // namespace pf
+
+struct pf_handle {
+  std::unique_ptr<pf::EngineBase> eng;
+  int precision = PF_FP64;
+};
+
+extern "C" {
+
+int pf_version(void) { return PFCORE_ABI_VERSION; }
+
+int pf_last_error(char* buf, size_t n) {
+  if (!buf || n == 0) return PF_E_ARG;
+  std::strncpy(buf, pf::last_error().c_str(), n - 1);
+  buf[n - 1] = '\0';
+  return PF_OK;
+}
+
+static int validate(const pf_params* p) {
+  if (p->Nx < 2 || p->Ny < 2 || p->Nz < 2) return pf::fail(PF_E_ARG, "grid dimensions must be >= 2");
+  if (!(p->dt > 0.0) || !(p->dx > 0.0) || !(p->dy > 0.0) || !(p->dz > 0.0))
+    return pf::fail(PF_E_ARG, "dt and spacings must be positive");
+  if (!(p->W >= 0.0) || !(p->kappa_phi >= 0.0) || !(p->L_phi >= 0.0) || !(p->D_alpha >= 0.0) ||
+      !(p->D_compound >= 0.0))
+    return pf::fail(PF_E_PARAMS, "physical parameters W, kappa_phi, L_phi, D_alpha, D_compound must be set");
+  const bool accel = p->elastic_warm_start_enabled || p->elastic_residual_control_enabled;
+  if (p->elastic_enabled && accel && !(p->elastic_warm_start_enabled && p->elastic_residual_control_enabled))
+    return pf::fail(PF_E_PARAMS,
+                    "accelerated elastic solver requires elastic_warm_start_enabled=1 and "
+                    "elastic_residual_control_enabled=1");
+  if (p->elastic_iter_max < 1) return pf::fail(PF_E_PARAMS, "elastic_iter_max must be >= 1");
+  return PF_OK;
+}
+
+int pf_create(const pf_params* p, int precision, pf_handle** out) {
+  if (!p || !out) return pf::fail(PF_E_ARG, "pf_create: null");
+  int rc = validate(p);
+  if (rc) return rc;
+  auto h = std::make_unique<pf_handle>();
+  h->precision = precision;
+  if (precision == PF_FP64)
+    h->eng = pf::make_engine_f64();
+  else if (precision == PF_MIXED)
+    h->eng = pf::make_engine_f32();
+  else
+    return pf::fail(PF_E_ARG, "unknown precision");
+  rc = h->eng->init(*p);
+  if (rc) return rc;
+  *out = h.release();
+  return PF_OK;
+}
+
+int pf_set_fields(pf_handle* h, const double* phi, const double* xB, const double* dYdt, double xB_cap) {
+  if (!h || !phi || !xB) return pf::fail(PF_E_ARG, "pf_set_fields: null");
+  return h->eng->set_fields(phi, xB, dYdt, xB_cap);
+}
+
+int pf_step(pf_handle* h, int nsteps) {
+  if (!h || nsteps < 0) return pf::fail(PF_E_ARG, "pf_step: bad argument");
+  for (int s = 0; s < nsteps; ++s) {
+    int rc = h->eng->step();
+    if (rc) return rc;
+  }
+  return PF_OK;
+}
+
+int pf_get_field(pf_handle* h, int which, double* out) {
+  if (!h || !out) return pf::fail(PF_E_ARG, "pf_get_field: null");
+  return h->eng->get_field(which, out);
+}
+
+int pf_scalars_get(pf_handle* h, pf_scalars* out) {
+  if (!h || !out) return pf::fail(PF_E_ARG, "pf_scalars_get: null");
+  return h->eng->scalars(out);
+}
+
+long long pf_elastic_state_size(pf_handle* h) { return h ? h->eng->elastic_state_size() : -1; }
+
+int pf_get_elastic_state(pf_handle* h, double* out) {
+  if (!h || !out) return pf::fail(PF_E_ARG, "pf_get_elastic_state: null");
+  return h->eng->get_elastic_state(out);
+}
+
+int pf_set_elastic_state(pf_handle* h, const double* in) {
+  if (!h || !in) return pf::fail(PF_E_ARG, "pf_set_elastic_state: null");
+  return h->eng->set_elastic_state(in);
+}
+
+int pf_sync(pf_handle* h) {
+  if (!h) return pf::fail(PF_E_ARG, "pf_sync: null");
+  return h->eng->sync();
+}
+
+int pf_destroy(pf_handle* h) {
+  delete h;
+  return PF_OK;
+}
+
+static int fft_handle(int Nx, int Ny, int Nz, int precision, pf_handle** h) {
+  pf_params p;
+  pf_params_default(&p);
+  p.Nx = Nx;
+  p.Ny = Ny;
+  p.Nz = Nz;
+  p.W = p.kappa_phi = p.L_phi = p.D_alpha = p.D_compound = 1.0;
+  p.mu_reference_scale = 1.0;
+  return pf_create(&p, precision, h);
+}
+
+int pf_fft_forward(int Nx, int Ny, int Nz, int precision, const double* in, double* out) {
+  if (!in || !out) return pf::fail(PF_E_ARG, "pf_fft_forward: null");
+  pf_handle* h = nullptr;
+  int rc = fft_handle(Nx, Ny, Nz, precision, &h);
+  if (rc) return rc;
+  rc = h->eng->fft_forward(in, out);
+  pf_destroy(h);
+  return rc;
+}
+
+int pf_fft_inverse(int Nx, int Ny, int Nz, int precision, const double* in, double* out) {
+  if (!in || !out) return pf::fail(PF_E_ARG, "pf_fft_inverse: null");
+  pf_handle* h = nullptr;
+  int rc = fft_handle(Nx, Ny, Nz, precision, &h);
+  if (rc) return rc;
+  rc = h->eng->fft_inverse(in, out);
+  pf_destroy(h);
+  return rc;
+}
+
+}  // extern "C"

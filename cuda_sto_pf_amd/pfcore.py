@@ -1,0 +1,235 @@
+"""ctypes binding of libpfcore.so (the C ABI declared in include/pfcore.h).
+
+The reference drivers start ``./main_cuda`` as a subprocess
+(tools/analysis/run_pure_dynamics_mass_drift_benchmark.py:63-70); this module is
+the in-process replacement they can call instead.  The product path is the
+HIP library only: if ``libpfcore.so`` is missing, importing the binding raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("PFCORE_LIB", os.path.join(HERE, "libpfcore.so"))
+
+PF_FP64 = 0
+PF_MIXED = 1
+PF_PHI, PF_XB, PF_Y, PF_DYDT, PF_XBTOT, PF_TRSIGMA = range(6)
+
+STATUS = {0: "PF_OK", -1: "PF_E_ARG", -2: "PF_E_PARAMS", -3: "PF_E_HIP", -4: "PF_E_NONCONVERGED",
+          -5: "PF_E_ZERO_MODE", -6: "PF_E_NONFINITE"}
+
+VOIGT21 = ["11", "12", "13", "14", "15", "16", "22", "23", "24", "25", "26", "33", "34", "35",
+           "36", "44", "45", "46", "55", "56", "66"]
+
+
+class PfParams(C.Structure):
+    _fields_ = [
+        ("Nx", C.c_int), ("Ny", C.c_int), ("Nz", C.c_int),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("dt", C.c_double),
+        ("nsteps", C.c_int), ("out_every", C.c_int), ("csv_out_every", C.c_int),
+        ("elastic_enabled", C.c_int),
+        ("pf_params_schema_version", C.c_int),
+        ("temperature_C", C.c_double), ("mu_reference_scale", C.c_double), ("t_real_unit", C.c_double),
+        ("v_A", C.c_double), ("v_B", C.c_double), ("Vm_compound", C.c_double),
+        ("Vm_alpha_0", C.c_double), ("dVm_alpha_dxB", C.c_double),
+        ("W", C.c_double), ("kappa_phi", C.c_double), ("L_phi", C.c_double),
+        ("D_alpha", C.c_double), ("D_compound", C.c_double),
+        ("eps_iso_over_vB", C.c_double), ("elastic_shift_dimless", C.c_double),
+        ("Y_clip", C.c_double), ("xB_eps", C.c_double),
+        ("thermo_convex_extrapolation_enabled", C.c_int),
+        ("S", C.c_double * 21), ("S_p", C.c_double * 21),
+        ("eps00", C.c_double * 6), ("E0", C.c_double * 6),
+        ("elastic_warm_start_enabled", C.c_int), ("elastic_residual_control_enabled", C.c_int),
+        ("elastic_iter_min", C.c_int), ("elastic_iter_max", C.c_int),
+        ("elastic_residual_tolerance", C.c_double), ("elastic_residual_absolute_floor", C.c_double),
+        ("elastic_fail_on_nonconvergence", C.c_int),
+        ("zero_mode", C.c_int), ("zero_mode_tolerance_relative", C.c_double),
+        ("zero_mode_max_iterations", C.c_int),
+        ("enable_Y_rhs_previous_time_level", C.c_int), ("disable_Y_rhs_gamma_term", C.c_int),
+        ("Y_rhs_term_h_scale", C.c_double),
+        ("gamma_Jm2", C.c_double), ("lambda_sm_m", C.c_double), ("ic_phi_iface_w", C.c_double),
+        ("ic_vf_init_phi", C.c_double), ("ic_vf_target_phi", C.c_double),
+        ("Vm_alpha_0_phys_m3mol", C.c_double),
+        ("mode", C.c_int),
+    ]
+
+
+class PfScalars(C.Structure):
+    _fields_ = [
+        ("mass_xBtot", C.c_double), ("mean_xBtot", C.c_double), ("volume_fraction", C.c_double),
+        ("f_grad", C.c_double), ("f_dw", C.c_double), ("f_elastic", C.c_double),
+        ("f_chem", C.c_double), ("f_total", C.c_double), ("mean_DY", C.c_double),
+        ("last_elastic_residual", C.c_double), ("zero_mode_lambda", C.c_double),
+        ("last_elastic_iterations", C.c_int), ("zero_mode_iterations", C.c_int),
+        ("steps_done", C.c_longlong), ("elastic_iterations_total", C.c_longlong),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+# every entry point declared in include/pfcore.h
+EXPORTS = {
+    "pf_version": (C.c_int, []),
+    "pf_last_error": (C.c_int, [C.c_char_p, C.c_size_t]),
+    "pf_params_default": (C.c_int, [C.POINTER(PfParams)]),
+    "pf_params_set": (C.c_int, [C.POINTER(PfParams), C.c_char_p, C.c_char_p]),
+    "pf_params_load": (C.c_int, [C.c_char_p, C.POINTER(PfParams)]),
+    "pf_create": (C.c_int, [C.POINTER(PfParams), C.c_int, C.POINTER(C.c_void_p)]),
+    "pf_set_fields": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]),
+    "pf_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "pf_get_field": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "pf_scalars_get": (C.c_int, [C.c_void_p, C.POINTER(PfScalars)]),
+    "pf_elastic_state_size": (C.c_longlong, [C.c_void_p]),
+    "pf_get_elastic_state": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pf_set_elastic_state": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pf_sync": (C.c_int, [C.c_void_p]),
+    "pf_destroy": (C.c_int, [C.c_void_p]),
+    "pf_fft_forward": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "pf_fft_inverse": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+}
+
+
+class PfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def load_library(path=LIB_PATH):
+    if not os.path.exists(path):
+        raise FileNotFoundError(
+            f"{path} is not built; run `python -m cuda_sto_pf_amd.build` (there is no CPU fallback)")
+    lib = C.CDLL(path)
+    for name, (res, args) in EXPORTS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        _LIB = load_library()
+    return _LIB
+
+
+def _check(rc):
+    if rc < 0:
+        buf = C.create_string_buffer(1024)
+        lib().pf_last_error(buf, 1024)
+        raise PfError(rc, buf.value.decode(errors="replace"))
+    return rc
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def params_from_dict(d):
+    """Fill a PfParams from a dict (keys of the reference .params file plus
+    C / dC 6x6 Voigt matrices and eps00 / E0 6-tuples)."""
+    p = PfParams()
+    _check(lib().pf_params_default(C.byref(p)))
+    for k, v in d.items():
+        if k in ("C", "dC"):
+            m = np.asarray(v, dtype=np.float64)
+            arr = p.S if k == "C" else p.S_p
+            for idx, suf in enumerate(VOIGT21):
+                arr[idx] = float(m[int(suf[0]) - 1, int(suf[1]) - 1])
+        elif k in ("eps00", "E0"):
+            arr = getattr(p, k)
+            for i in range(6):
+                arr[i] = float(v[i])
+        elif k == "zero_mode_tolerance_relative":
+            p.zero_mode_tolerance_relative = float(v)
+        elif hasattr(p, k):
+            cur = getattr(p, k)
+            setattr(p, k, type(cur)(v))
+    return p
+
+
+def load_params_file(path):
+    p = PfParams()
+    _check(lib().pf_params_default(C.byref(p)))
+    _check(lib().pf_params_load(path.encode(), C.byref(p)))
+    return p
+
+
+class Simulation:
+    """One device-resident simulation (a pf_handle)."""
+
+    def __init__(self, params, precision=PF_FP64):
+        if isinstance(params, dict):
+            params = params_from_dict(params)
+        self.params = params
+        self.shape = (params.Nx, params.Ny, params.Nz)
+        h = C.c_void_p()
+        _check(lib().pf_create(C.byref(params), int(precision), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().pf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_fields(self, phi, xB, dYdt=None, xB_cap=float("nan")):
+        phi = np.ascontiguousarray(phi, dtype=np.float64).reshape(self.shape)
+        xB = np.ascontiguousarray(xB, dtype=np.float64).reshape(self.shape)
+        d = None if dYdt is None else np.ascontiguousarray(dYdt, dtype=np.float64).reshape(self.shape)
+        _check(lib().pf_set_fields(self._h, _ptr(phi), _ptr(xB), None if d is None else _ptr(d),
+                                   float(xB_cap)))
+
+    def step(self, n=1):
+        _check(lib().pf_step(self._h, int(n)))
+
+    def field(self, which):
+        out = np.empty(self.shape, dtype=np.float64)
+        _check(lib().pf_get_field(self._h, int(which), _ptr(out)))
+        return out
+
+    def scalars(self):
+        s = PfScalars()
+        _check(lib().pf_scalars_get(self._h, C.byref(s)))
+        return s.as_dict()
+
+    def elastic_state(self):
+        n = lib().pf_elastic_state_size(self._h)
+        out = np.empty(3 * n * 2, dtype=np.float64)
+        _check(lib().pf_get_elastic_state(self._h, _ptr(out)))
+        return out.view(np.complex128).reshape(3, n)
+
+    def set_elastic_state(self, u):
+        u = np.ascontiguousarray(u, dtype=np.complex128)
+        _check(lib().pf_set_elastic_state(self._h, _ptr(u)))
+
+    def sync(self):
+        _check(lib().pf_sync(self._h))
+
+
+def fft_forward(a, precision=PF_FP64):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    nx, ny, nz = a.shape
+    out = np.empty((nx, ny, nz // 2 + 1), dtype=np.complex128)
+    _check(lib().pf_fft_forward(nx, ny, nz, int(precision), _ptr(a), _ptr(out)))
+    return out
+
+
+def fft_inverse(ak, shape, precision=PF_FP64):
+    nx, ny, nz = shape
+    ak = np.ascontiguousarray(ak, dtype=np.complex128).reshape(nx, ny, nz // 2 + 1)
+    out = np.empty(shape, dtype=np.float64)
+    _check(lib().pf_fft_inverse(nx, ny, nz, int(precision), _ptr(ak), _ptr(out)))
+    return out

@@ -1,0 +1,162 @@
+/*
+ * pfcore.h - C ABI of the MI355X phase-field time-step library (libpfcore.so).
+ *
+ * The reference has no library boundary: its drivers build argv and run the
+ * `main_cuda` process (tools/analysis/run_pure_dynamics_mass_drift_benchmark.py:63-70,
+ * scripts/run_pf_only_baseline_workstation.py:41-48), and inside that process
+ * the step is ~150 `launch_*` C++ wrappers declared in cuda_kernels.h:167-1219,
+ * called from the step loop main_cuda.cu:28427-33604.  This header is the seam
+ * those wrappers and the loop collapse into.  Each entry point names the
+ * reference code it replaces.
+ *
+ * Conventions: every function returns 0 on success and a negative PF_E_* code
+ * on failure (the message is kept per thread, see pf_last_error).  Host
+ * arrays are owned by the caller and copied; device memory is owned by the
+ * handle.  Real fields are row-major [Nx][Ny][Nz] doubles (z fastest), the
+ * reference's layout (cuda_kernels.cu:4157-4160).  One HIP stream per handle;
+ * a handle is not thread-safe, distinct handles may be used from distinct
+ * threads.
+ */
+#ifndef PFCORE_H
+#define PFCORE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PFCORE_ABI_VERSION 1
+
+enum pf_status {
+  PF_OK = 0,
+  PF_E_ARG = -1,            /* invalid argument / unsupported size */
+  PF_E_PARAMS = -2,         /* .params file error or missing required key */
+  PF_E_HIP = -3,            /* HIP runtime failure */
+  PF_E_NONCONVERGED = -4,   /* elastic fixed point did not reach the tolerance */
+  PF_E_ZERO_MODE = -5,      /* zero-mode shift rejected */
+  PF_E_NONFINITE = -6       /* non-finite residual or field */
+};
+
+enum pf_precision {
+  PF_FP64 = 0,   /* all transforms and updates in FP64 */
+  PF_MIXED = 1   /* FP32 transforms/updates; FP64 reductions, zero mode, residual */
+};
+
+enum pf_field {
+  PF_PHI = 0, PF_XB = 1, PF_Y = 2, PF_DYDT = 3, PF_XBTOT = 4, PF_TRSIGMA = 5
+};
+
+/* Code-unit parameters the time step reads (subset of PFParams, pf_params.h:6-702).
+ * C and dC are the 21 upper-triangle Voigt entries 11,12,...,16,22,...,66
+ * (keys S_ij and S_p_ij); strains are ordered xx, yy, zz, yz, xz, xy. */
+typedef struct pf_params {
+  int Nx, Ny, Nz;
+  double dx, dy, dz, dt;
+  int nsteps, out_every, csv_out_every, elastic_enabled;
+  int pf_params_schema_version;
+  double temperature_C, mu_reference_scale, t_real_unit;
+  double v_A, v_B, Vm_compound, Vm_alpha_0, dVm_alpha_dxB;
+  double W, kappa_phi, L_phi, D_alpha, D_compound;
+  double eps_iso_over_vB, elastic_shift_dimless;
+  double Y_clip, xB_eps;
+  int thermo_convex_extrapolation_enabled;
+  double S[21], S_p[21];
+  double eps00[6], E0[6];
+  int elastic_warm_start_enabled, elastic_residual_control_enabled;
+  int elastic_iter_min, elastic_iter_max;
+  double elastic_residual_tolerance, elastic_residual_absolute_floor;
+  int elastic_fail_on_nonconvergence;
+  int zero_mode;                       /* --pf-zero-mode PF_CONSERVED_Y_ZERO_MODE_V1 */
+  double zero_mode_tolerance_relative; /* default 1e-12 (main_cuda.cu:22487) */
+  int zero_mode_max_iterations;        /* default 24 (main_cuda.cu:22488) */
+  int enable_Y_rhs_previous_time_level, disable_Y_rhs_gamma_term;
+  double Y_rhs_term_h_scale;
+  /* keys the loader records for the Results tree and unit reporting */
+  double gamma_Jm2, lambda_sm_m, ic_phi_iface_w, ic_vf_init_phi, ic_vf_target_phi;
+  double Vm_alpha_0_phys_m3mol;
+  int mode;                            /* 0 dynamics, 1 minimize */
+} pf_params;
+
+typedef struct pf_scalars {
+  double mass_xBtot;          /* sum (1-h) x_B + v_B h      (reduce_sum_xBtot_kernel) */
+  double mean_xBtot;
+  double volume_fraction;     /* <h(clamp01 phi)>           (reduce_sum_h_kernel) */
+  double f_grad, f_dw, f_elastic, f_chem, f_total;   /* mean energy densities */
+  double mean_DY;             /* last D-bar of the Y update */
+  double last_elastic_residual;
+  double zero_mode_lambda;
+  int last_elastic_iterations;
+  int zero_mode_iterations;
+  long long steps_done;
+  long long elastic_iterations_total;
+} pf_scalars;
+
+typedef struct pf_handle pf_handle;
+
+/* Library ABI version (PFCORE_ABI_VERSION). */
+int pf_version(void);
+
+/* Copies the last error message of the calling thread. */
+int pf_last_error(char* buf, size_t n);
+
+/* Defaults of params_default (main_cuda.cu:15781-16433) for the hot-path keys. */
+int pf_params_default(pf_params* p);
+
+/* Apply one key=value (apply_pfparams_override_key, main_cuda.cu:21226-22305).
+ * Returns 1 when the key is known, 0 when unknown (ignored), <0 on a bad value. */
+int pf_params_set(pf_params* p, const char* key, const char* value);
+
+/* Load a schema-v1 .params file (load_pfparams_override_file,
+ * main_cuda.cu:22306-22360) including the required-key check
+ * (validate_required_pfparams_presence, main_cuda.cu:1089-1144). */
+int pf_params_load(const char* path, pf_params* p);
+
+/* Allocate device state and plans (main_cuda.cu:27707-28214).
+ * Sizes must factor into radices 2,3,5 and one further prime (e.g. 246). */
+int pf_create(const pf_params* p, int precision, pf_handle** out);
+
+/* Initial fields, raw_fields semantics (load_raw_init_fields,
+ * main_cuda.cu:2905-2965): phi clamped to [0,1], x_B clamped to
+ * [xB_eps, 1-xB_eps] and capped at xB_cap (pass NAN for none),
+ * Y = logit(x_B); dY/dt defaults to 0.  Resets the elastic warm state and,
+ * with the zero mode, freezes the target mass (main_cuda.cu:28080-28100). */
+int pf_set_fields(pf_handle* h, const double* phi, const double* xB, const double* dYdt_or_null,
+                  double xB_cap);
+
+/* Advance nsteps time steps (the loop body main_cuda.cu:28427-31623). */
+int pf_step(pf_handle* h, int nsteps);
+
+/* Copy a real field to the host (N doubles). */
+int pf_get_field(pf_handle* h, int which, double* out);
+
+/* Mass, volume fraction, energy densities and solver statistics. */
+int pf_scalars_get(pf_handle* h, pf_scalars* out);
+
+/* Elastic warm state u(k): 3 components of the compact retained spectrum,
+ * interleaved (re,im) doubles.  pf_elastic_state_size gives the count of
+ * complex values per component. */
+long long pf_elastic_state_size(pf_handle* h);
+int pf_get_elastic_state(pf_handle* h, double* out);
+int pf_set_elastic_state(pf_handle* h, const double* in);
+
+/* Synchronise the handle's stream. */
+int pf_sync(pf_handle* h);
+
+int pf_destroy(pf_handle* h);
+
+/* ---------------------------------------------------------------------------
+ * FFT test entry points (the 3-D transforms that replace the cuFFT plans of
+ * main_cuda.cu:28158-28214): forward R2C of a real [Nx][Ny][Nz] field to the
+ * full half spectrum [Nx][Ny][Nz/2+1] with the 2/3 dealias applied (modes
+ * outside the retained set are 0), and the matching unnormalised C2R.
+ * Complex values are interleaved doubles. */
+int pf_fft_forward(int Nx, int Ny, int Nz, int precision, const double* in, double* out_halfspec);
+int pf_fft_inverse(int Nx, int Ny, int Nz, int precision, const double* in_halfspec, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PFCORE_H */

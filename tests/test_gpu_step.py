@@ -1,0 +1,96 @@
+"""Whole time step, HIP (libpfcore) against the FP64 CPU oracle on identical
+seeded inputs.  FP64 mode must agree to 1e-10 in phi, Y and x_B; the mixed
+mode to 1e-5 in x_B (the accuracy gate named for the mixed path)."""
+import math
+
+import numpy as np
+import pytest
+
+from cuda_sto_pf_amd.configs import base_params, composition, config, spheres_phi
+from oracle.step_ref import Oracle, Params
+
+FP64_TOL = 1e-10
+MIXED_XB_TOL = 1e-5
+
+
+def _oracle(pd, phi, xB, cap):
+    return Oracle(Params(**pd), phi, xB, xB_cap=cap)
+
+
+def _gpu(pf, pd, phi, xB, cap, prec):
+    sim = pf.Simulation(pd, prec)
+    sim.set_fields(phi, xB, xB_cap=cap)
+    return sim
+
+
+def _rect_case(shape=(16, 12, 8), hetero=True, zero_mode=True, elastic=True, seed=11):
+    pd = base_params(shape, hetero=hetero, zero_mode=zero_mode, elastic=elastic)
+    rng = np.random.default_rng(seed)
+    phi = spheres_phi(shape, [(shape[0] / 2, shape[1] / 2, shape[2] / 2)], [3.0], width=1.5)
+    xB = composition(phi, rng)
+    return pd, phi, xB, 0.5
+
+
+def _cases():
+    out = []
+    pd, phi, xB, meta = config(1, 16)
+    out.append(("cfg1_16", pd, phi, xB, meta["xB_max_safe"]))
+    pd, phi, xB, meta = config(2, 16)
+    out.append(("cfg2_16", pd, phi, xB, meta["xB_max_safe"]))
+    out.append(("rect_hetero_zm",) + _rect_case())
+    out.append(("rect_hetero_nozm",) + _rect_case(zero_mode=False))
+    out.append(("rect_noelastic",) + _rect_case(elastic=False))
+    return out
+
+
+CASES = _cases()
+
+
+def _compare(o, sim, pf):
+    phi = sim.field(pf.PF_PHI)
+    Y = sim.field(pf.PF_Y)
+    xB = sim.field(pf.PF_XB)
+    return (np.abs(phi - o.phi).max(), np.abs(Y - o.Y).max(), np.abs(xB - o.xB).max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,pd,phi,xB,cap", CASES, ids=[c[0] for c in CASES])
+def test_fp64_step_parity(pf, name, pd, phi, xB, cap):
+    nsteps = 5
+    o = _oracle(pd, phi, xB, cap)
+    sim = _gpu(pf, pd, phi, xB, cap, pf.PF_FP64)
+    for s in range(nsteps):
+        o.step()
+        sim.step()
+        sc = sim.scalars()
+        dphi, dY, dx = _compare(o, sim, pf)
+        print(f"{name} step {s + 1}: dphi={dphi:.2e} dY={dY:.2e} dxB={dx:.2e} "
+              f"iters gpu={sc['last_elastic_iterations']} cpu={o.last_elastic_iterations} "
+              f"res gpu={sc['last_elastic_residual']:.3e} cpu={o.last_elastic_residual:.3e}", flush=True)
+        assert dphi < FP64_TOL and dY < FP64_TOL and dx < FP64_TOL
+        if pd["elastic_enabled"]:
+            assert sc["last_elastic_iterations"] == o.last_elastic_iterations
+    sc = sim.scalars()
+    assert math.isclose(sc["mass_xBtot"], o.mass(), rel_tol=1e-12)
+    if pd["zero_mode"]:
+        assert abs(sc["mass_xBtot"] - o.target_mass) <= 1e-12 * max(abs(o.target_mass), 1.0)
+    en = o.last_energies
+    for key, gkey in (("grad", "f_grad"), ("dw", "f_dw"), ("elastic", "f_elastic"), ("chem", "f_chem")):
+        assert math.isclose(sc[gkey], en[key], rel_tol=1e-10, abs_tol=1e-14), (key, sc[gkey], en[key])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,pd,phi,xB,cap", CASES[:3], ids=[c[0] for c in CASES[:3]])
+def test_mixed_step_accuracy(pf, name, pd, phi, xB, cap):
+    nsteps = 5
+    o = _oracle(pd, phi, xB, cap)
+    sim = _gpu(pf, pd, phi, xB, cap, pf.PF_MIXED)
+    for s in range(nsteps):
+        o.step()
+        sim.step()
+        dphi, dY, dx = _compare(o, sim, pf)
+        print(f"mixed {name} step {s + 1}: dphi={dphi:.2e} dY={dY:.2e} dxB={dx:.2e}", flush=True)
+    assert dx < MIXED_XB_TOL
+    sc = sim.scalars()
+    if pd["zero_mode"]:
+        assert abs(sc["mass_xBtot"] - o.target_mass) <= 1e-12 * max(abs(o.target_mass), 1.0)
