@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Throughput of the phase-field time step on one MI355X.
+
+Workload: 256^3, 40 seeded particles (radii U(5,12), tanh width 4), heterogeneous
+Ag2Te/PbTe elasticity solved to a relative residual of 1e-8, zero mode on,
+dt = 0.01 (the 256^3 heterogeneous workload the metric is quoted on).
+A step is one call of the time step on the resident fields (elastic fixed point,
+phi update, composition update with the mass-conserving zero mode).
+
+Reported: timesteps/s of the mixed-precision path (FP32 transforms, FP64
+state/reductions/zero mode/residual) as ``value``; the plain FP64 path on the
+same inputs; max |x_B mixed - x_B FP64| after the same steps; elastic updates
+per step; the roofline of the dominant kernel (HIP events inside the library);
+and the CPU FP64 oracle timed on a bounded sample as ``cpu_baseline``.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "timesteps/s at 256^3 mixed+learned vs FP64 plain at max|delta c|<=1e-5; elastic iters"
+HBM_PEAK_GBS = 8000.0
+
+
+def run_path(pf, pd, phi, xB, cap, prec, warmup, steps):
+    sim = pf.Simulation(pd, prec)
+    sim.set_fields(phi, xB, xB_cap=cap)
+    sim.mass0 = sim.scalars()["mass_xBtot"]
+    sim.step(warmup)
+    sim.sync()
+    it0 = sim.scalars()["elastic_iterations_total"]
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.sync()
+    t1 = time.perf_counter()
+    sc = sim.scalars()
+    return sim, (t1 - t0), (sc["elastic_iterations_total"] - it0) / steps, sc
+
+
+def kernel_table(pf, sim, iters_per_step, mixed):
+    """Per-kernel mean duration, bytes and share of the step (launch counts
+    from the stage sequence with I elastic updates per step)."""
+    counts = {
+        "zpass_hij": iters_per_step,
+        "ypass_fwd6": iters_per_step,
+        "ypass_inv6": iters_per_step + 1 + (1 if mixed else 0),
+        "xpass_fwd6": iters_per_step,
+        "xpass_inv6": iters_per_step + 1 + (1 if mixed else 0),
+        "green": iters_per_step,
+        "zpass_phirhs": 1,
+        "zpass_flux": 1,
+    }
+    table = {}
+    for name, kid in pf.KERNELS.items():
+        us, b = sim.time_kernel(kid, reps=10)
+        table[name] = {"us": us, "bytes": b, "GBps": b / us * 1e-3, "frac_hbm": b / us * 1e-3 / HBM_PEAK_GBS,
+                       "per_step": counts[name], "us_per_step": us * counts[name]}
+    return table
+
+
+def cpu_baseline(budget_s=30.0):
+    """The FP64 numpy oracle (one thread, pocketfft) on a bounded sample of the
+    same workload family: the 40-particle geometry generated at 80^3; one cold
+    step (untimed) then one timed warm step; extrapolated to 256^3 by R log R."""
+    from cuda_sto_pf_amd.configs import config
+    from oracle.step_ref import Oracle, Params
+    n = 80
+    pd, phi, xB, meta = config(5, n=n)
+    o = Oracle(Params(**pd), phi, xB, xB_cap=meta["xB_max_safe"])
+    t0 = time.perf_counter()
+    o.step()
+    cold = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    o.step()
+    warm = time.perf_counter() - t0
+    r_s, r_t = n ** 3, 256 ** 3
+    scale = (r_t * math.log(r_t)) / (r_s * math.log(r_s))
+    return {"value": 1.0 / (warm * scale), "unit": "timesteps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle/step_ref.py FP64 numpy, 1 warm step at {n}^3 of the same geometry "
+                      f"({warm:.2f} s, {o.last_elastic_iterations} elastic updates; cold step {cold:.2f} s "
+                      f"untimed), extrapolated to 256^3 by R log R (x{scale:.1f})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
+              "the multi-GPU decomposition is a design note, not a measured path", file=sys.stderr)
+        sys.exit(2)
+
+    from cuda_sto_pf_amd import pfcore as pf
+    from cuda_sto_pf_amd.configs import config
+
+    pd, phi, xB, meta = config(5)
+    cap = meta["xB_max_safe"]
+    print(f"[bench] workload {meta['workload']} {phi.shape} warmup={args.warmup} steps={args.steps}",
+          file=sys.stderr, flush=True)
+
+    sim_m, t_m, it_m, sc_m = run_path(pf, pd, phi, xB, cap, pf.PF_MIXED, args.warmup, args.steps)
+    print(f"[bench] mixed: {args.steps / t_m:.2f} steps/s, {it_m:.2f} elastic updates/step",
+          file=sys.stderr, flush=True)
+    xb_m = sim_m.field(pf.PF_XB)
+    result_fp64 = None
+    max_dc = None
+    if not args.no_fp64:
+        sim_d, t_d, it_d, sc_d = run_path(pf, pd, phi, xB, cap, pf.PF_FP64, args.warmup, args.steps)
+        xb_d = sim_d.field(pf.PF_XB)
+        max_dc = float(np.abs(xb_m - xb_d).max())
+        result_fp64 = {"steps_per_s": args.steps / t_d, "ms_per_step": 1e3 * t_d / args.steps,
+                       "elastic_updates_per_step": it_d,
+                       "mass_drift_rel": abs(sc_d["mass_xBtot"] / sim_d.mass0 - 1.0)}
+        print(f"[bench] fp64: {args.steps / t_d:.2f} steps/s, {it_d:.2f} updates/step, "
+              f"max|dc| mixed vs fp64 = {max_dc:.3e}", file=sys.stderr, flush=True)
+        table_d = kernel_table(pf, sim_d, it_d, mixed=False)
+        sim_d.close()
+
+    table = kernel_table(pf, sim_m, it_m, mixed=True)
+    dom_name = max(table, key=lambda k: table[k]["us_per_step"])
+    dom = table[dom_name]
+    for k, v in table.items():
+        print(f"[bench] kernel {k:14s} {v['us']:8.1f} us  {v['GBps']:7.0f} GB/s  x{v['per_step']:.1f}/step",
+              file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline()
+        print(f"[bench] cpu baseline {cpu['value']:.4g} steps/s ({cpu['sample']})", file=sys.stderr, flush=True)
+
+    value = args.steps / t_m
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "timesteps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_m / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 transforms / f64 state, reductions, zero mode, residual",
+        "data": "synthetic",
+        "config": {"workload": "256^3, 40 particles R~U(5,12), hetero elastic tol 1e-8, zero mode, dt 0.01",
+                   "grid": list(phi.shape), "learned_warm_start": False, "path": "mixed"},
+        "elastic_updates_per_step": it_m,
+        "mass_drift_rel": abs(sc_m["mass_xBtot"] / sim_m.mass0 - 1.0),
+        "max_abs_dc_vs_fp64": max_dc,
+        "fp64_plain": result_fp64,
+        "mixed_over_fp64": (value / result_fp64["steps_per_s"]) if result_fp64 else None,
+        "roofline": {"kernel": dom_name, "bound": "hbm", "achieved": dom["GBps"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": dom["bytes"], "us_per_launch": dom["us"]},
+        "kernels_mixed": table,
+        "kernels_fp64": table_d if not args.no_fp64 else None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

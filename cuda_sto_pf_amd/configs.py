@@ -62,38 +62,44 @@ def base_params(n, dt=0.01, hetero=False, tol=1e-8, zero_mode=True, elastic=True
     )
 
 
-def _grid(shape):
+def spheres_phi(shape, centres, radii, width=4.0, aspects=None, rotations=None, tail=10.0):
+    """phi = max_p 1/2 (1 + tanh((R_p - r_p)/w)) with periodic distances.
+
+    Each particle is evaluated inside a periodic box of half-width
+    R/aspect + tail*w around its centre (the profile there is below
+    e^(-2 tail) ~ 2e-9); outside the box its contribution is taken as 0.
+    """
     nx, ny, nz = shape
-    return np.meshgrid(np.arange(nx, dtype=np.float64), np.arange(ny, dtype=np.float64),
-                       np.arange(nz, dtype=np.float64), indexing="ij")
-
-
-def _periodic_dist2(g, c, n):
-    d = np.abs(g - c)
-    d = np.minimum(d, n - d)
-    return d * d
-
-
-def spheres_phi(shape, centres, radii, width=4.0, aspects=None, rotations=None):
-    """phi = max_p 1/2 (1 + tanh((R_p - r_p)/w)), periodic distances."""
-    X, Y, Z = _grid(shape)
     phi = np.zeros(shape)
     for p, (c, R) in enumerate(zip(centres, radii)):
+        a = 1.0 if aspects is None else aspects[p]
+        half = int(math.ceil(R / min(a, 1.0) + tail * width))
+        idx = []
+        for ax, n in enumerate(shape):
+            if 2 * half + 1 >= n:
+                idx.append(np.arange(n))
+            else:
+                c0 = int(math.floor(c[ax]))
+                idx.append(np.arange(c0 - half, c0 + half + 2) % n)
+        X = idx[0].astype(np.float64)[:, None, None]
+        Yv = idx[1].astype(np.float64)[None, :, None]
+        Z = idx[2].astype(np.float64)[None, None, :]
         dx = X - c[0]
-        dy = Y - c[1]
+        dy = Yv - c[1]
         dz = Z - c[2]
-        dx = dx - shape[0] * np.round(dx / shape[0])
-        dy = dy - shape[1] * np.round(dy / shape[1])
-        dz = dz - shape[2] * np.round(dz / shape[2])
+        dx = dx - nx * np.round(dx / nx)
+        dy = dy - ny * np.round(dy / ny)
+        dz = dz - nz * np.round(dz / nz)
         if rotations is not None:
             Rm = rotations[p]
             vx = Rm[0, 0] * dx + Rm[0, 1] * dy + Rm[0, 2] * dz
             vy = Rm[1, 0] * dx + Rm[1, 1] * dy + Rm[1, 2] * dz
             vz = Rm[2, 0] * dx + Rm[2, 1] * dy + Rm[2, 2] * dz
             dx, dy, dz = vx, vy, vz
-        a = 1.0 if aspects is None else aspects[p]
         r = np.sqrt(dx * dx + dy * dy + (dz / a) ** 2)
-        phi = np.maximum(phi, 0.5 * (1.0 + np.tanh((R - r) / width)))
+        val = 0.5 * (1.0 + np.tanh((R - r) / width))
+        sub = np.ix_(idx[0], idx[1], idx[2])
+        phi[sub] = np.maximum(phi[sub], val)
     return phi
 
 

@@ -1,0 +1,381 @@
+// main_pf: drop-in for the reference `main_cuda` binary's dynamics path.
+//
+// Keeps the reference command line (main_cuda.cu:22457-22771):
+//   main_pf Nx Ny Nz dt nsteps out_every csv_out_every elastic_enabled
+//           --pf-param-file <file> [--mode=dynamics] [--init-mode raw_fields
+//           --init-phi-raw f --init-xB-raw f --init-meta f [--init-dY-dt-prev-raw f]]
+//           [--pf-zero-mode PF_CONSERVED_Y_ZERO_MODE_V1|OFF] [--init-case-tag tag]
+//           [--precision fp64|mixed]
+// and its Results/ layout (main_cuda.cu:25270-25660): run directory
+// <ch|chel>_T<T>_cuda_<Nx>x<Ny>x<Nz>_dt<dt>_steps<n>_xB<xB>/<case_tag>/ holding
+// pf_input.params, vf_precip_vs_time_<tag>.csv, elastic_solver_trace.csv,
+// pf_zero_mode_trace.csv, performance_summary.{csv,json} and phi/xB VTKs
+// (io_vtk_cuda.h:35-131; CUDA_STO_SUPPRESS_VTK_OUTPUT=1 suppresses them).
+// Errors print "[fatal] ..." and exit 2, as the reference does.
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/pfcore.h"
+
+namespace {
+
+int fatal(const std::string& msg) {
+  std::fprintf(stderr, "[fatal] %s\n", msg.c_str());
+  return 2;
+}
+
+std::string lib_error() {
+  char buf[1024];
+  pf_last_error(buf, sizeof buf);
+  return buf;
+}
+
+bool starts_with(const char* s, const char* p) { return std::strncmp(s, p, std::strlen(p)) == 0; }
+
+bool copy_text_file(const std::string& src, const std::string& dst) {
+  std::ifstream in(src, std::ios::binary);
+  if (!in) return false;
+  std::ofstream out(dst, std::ios::binary);
+  out << in.rdbuf();
+  return bool(out);
+}
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// Minimal reader for the flat init_meta.json of a raw_fields bundle
+// (main_cuda.cu:2733-2763): numbers and strings at top level.
+struct Meta {
+  int Nx = -1, Ny = -1, Nz = -1;
+  std::string dtype = "float64", order = "C";
+  double xB_max_safe = NAN;
+};
+
+bool read_meta(const std::string& path, Meta* m) {
+  std::ifstream in(path);
+  if (!in) return false;
+  std::stringstream ss;
+  ss << in.rdbuf();
+  std::string t = ss.str();
+  auto find_val = [&](const std::string& key, std::string* out) {
+    size_t k = t.find("\"" + key + "\"");
+    if (k == std::string::npos) return false;
+    size_t c = t.find(':', k);
+    if (c == std::string::npos) return false;
+    size_t b = t.find_first_not_of(" \t\r\n", c + 1);
+    if (b == std::string::npos) return false;
+    if (t[b] == '"') {
+      size_t e = t.find('"', b + 1);
+      *out = t.substr(b + 1, e - b - 1);
+    } else {
+      size_t e = t.find_first_of(",}\n", b);
+      *out = t.substr(b, e - b);
+    }
+    return true;
+  };
+  std::string v;
+  if (find_val("Nx", &v)) m->Nx = std::atoi(v.c_str());
+  if (find_val("Ny", &v)) m->Ny = std::atoi(v.c_str());
+  if (find_val("Nz", &v)) m->Nz = std::atoi(v.c_str());
+  if (find_val("dtype", &v)) m->dtype = v;
+  if (find_val("order", &v)) m->order = v;
+  if (find_val("xB_max_safe", &v)) m->xB_max_safe = std::atof(v.c_str());
+  return m->Nx > 0 && m->Ny > 0 && m->Nz > 0;
+}
+
+bool read_raw(const std::string& path, const std::string& dtype, size_t n, std::vector<double>* out) {
+  std::ifstream in(path, std::ios::binary);
+  if (!in) return false;
+  out->resize(n);
+  if (dtype == "float32" || dtype == "f4" || dtype == "<f4") {
+    std::vector<float> f(n);
+    in.read(reinterpret_cast<char*>(f.data()), n * sizeof(float));
+    if ((size_t)in.gcount() != n * sizeof(float)) return false;
+    for (size_t i = 0; i < n; ++i) (*out)[i] = f[i];
+  } else {
+    in.read(reinterpret_cast<char*>(out->data()), n * sizeof(double));
+    if ((size_t)in.gcount() != n * sizeof(double)) return false;
+  }
+  return true;
+}
+
+// legacy ASCII VTK, %.5f, z-y-x order (io_vtk_cuda.h:35-131)
+bool write_vtk(const std::string& path, const std::vector<double>& f, int Nx, int Ny, int Nz,
+               const char* name, int step) {
+  const char* sup = std::getenv("CUDA_STO_SUPPRESS_VTK_OUTPUT");
+  if (sup && std::strcmp(sup, "1") == 0) {
+    std::printf("VTK_OUTPUT_SUPPRESSED field=%s step=%d path=%s\n", name, step, path.c_str());
+    return true;
+  }
+  FILE* fo = std::fopen(path.c_str(), "w");
+  if (!fo) return false;
+  const int ny_out = (Ny == 2) ? 1 : Ny;
+  std::fprintf(fo, "# vtk DataFile Version 3.0\nPhase Field Simulation\nASCII\nDATASET STRUCTURED_POINTS\n");
+  std::fprintf(fo, "DIMENSIONS %d %d %d\nASPECT_RATIO 1 1 1\nORIGIN 0 0 0\nPOINT_DATA %d\n", Nx, ny_out, Nz,
+               Nx * ny_out * Nz);
+  std::fprintf(fo, "SCALARS %s double 1\nLOOKUP_TABLE default\n", name);
+  for (int k = 0; k < Nz; ++k)
+    for (int j = 0; j < ny_out; ++j)
+      for (int i = 0; i < Nx; ++i) std::fprintf(fo, "%.5f ", f[((size_t)i * Ny + j) * Nz + k]);
+  std::fprintf(fo, "\n");
+  std::fclose(fo);
+  return true;
+}
+
+void usage() {
+  std::printf(
+      "Usage (positional, legacy):\n"
+      "  ./main_pf Nx Ny Nz dt nsteps out_every csv_out_every elastic_enabled(0/1)\n"
+      "    dt from --pf-param-file takes precedence over the positional dt.\n"
+      "Flags:\n"
+      "  --pf-param-file <path>   required: complete physical PF inputs (key=value)\n"
+      "  --mode=dynamics          (minimize modes are not part of this build)\n"
+      "  --init-mode raw_fields --init-phi-raw <f> --init-xB-raw <f> --init-meta <f>\n"
+      "  --init-dY-dt-prev-raw <f>\n"
+      "  --pf-zero-mode PF_CONSERVED_Y_ZERO_MODE_V1|OFF\n"
+      "  --init-case-tag <tag>\n"
+      "  --precision fp64|mixed   (default fp64)\n");
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  const char* param_file = nullptr;
+  std::string mode = "dynamics", init_mode, phi_raw, xB_raw, meta_path, dydt_raw, case_tag;
+  std::string zero_mode_sel = "OFF";
+  int precision = PF_FP64;
+  for (int i = 1; i < argc; ++i) {
+    std::string a = argv[i];
+    auto next = [&](std::string* dst) {
+      if (i + 1 >= argc) return false;
+      *dst = argv[++i];
+      return true;
+    };
+    if (a == "--help" || a == "-h") {
+      usage();
+      return 0;
+    } else if (a == "--pf-param-file") {
+      if (i + 1 >= argc) return fatal("--pf-param-file needs a path");
+      param_file = argv[++i];
+    } else if (starts_with(a.c_str(), "--mode=")) {
+      mode = a.substr(7);
+    } else if (a == "--init-mode") {
+      if (!next(&init_mode)) return fatal("--init-mode needs a value");
+    } else if (a == "--init-phi-raw") {
+      if (!next(&phi_raw)) return fatal("--init-phi-raw needs a path");
+    } else if (a == "--init-xB-raw") {
+      if (!next(&xB_raw)) return fatal("--init-xB-raw needs a path");
+    } else if (a == "--init-meta") {
+      if (!next(&meta_path)) return fatal("--init-meta needs a path");
+    } else if (a == "--init-dY-dt-prev-raw") {
+      if (!next(&dydt_raw)) return fatal("--init-dY-dt-prev-raw needs a path");
+    } else if (a == "--pf-zero-mode") {
+      if (!next(&zero_mode_sel)) return fatal("--pf-zero-mode needs a value");
+    } else if (a == "--init-case-tag") {
+      if (!next(&case_tag)) return fatal("--init-case-tag needs a value");
+    } else if (a == "--precision") {
+      std::string v;
+      if (!next(&v)) return fatal("--precision needs a value");
+      if (v == "fp64")
+        precision = PF_FP64;
+      else if (v == "mixed")
+        precision = PF_MIXED;
+      else
+        return fatal("--precision must be fp64 or mixed");
+    } else if (starts_with(a.c_str(), "--")) {
+      std::fprintf(stderr, "[warn] flag not handled by this build: %s\n", a.c_str());
+    }
+  }
+  if (!param_file)
+    return fatal("physical parameters have no built-in defaults: pass --pf-param-file <path>");
+  if (mode != "dynamics") return fatal("--mode=" + mode + " is not part of this build (dynamics only)");
+
+  pf_params P;
+  pf_params_default(&P);
+  // positional arguments (main_cuda.cu:22527-22534); the param file is loaded
+  // afterwards, so its dt wins (main_cuda.cu:22535-22542)
+  auto pos = [&](int k) -> const char* { return (argc > k && argv[k][0] != '-') ? argv[k] : nullptr; };
+  if (pos(1)) P.Nx = std::atoi(pos(1));
+  if (pos(2)) P.Ny = std::atoi(pos(2));
+  if (pos(3)) P.Nz = std::atoi(pos(3));
+  if (pos(4)) P.dt = std::atof(pos(4));
+  if (pos(5)) P.nsteps = std::atoi(pos(5));
+  if (pos(6)) P.out_every = std::atoi(pos(6));
+  if (pos(7)) P.csv_out_every = std::atoi(pos(7));
+  if (pos(8)) P.elastic_enabled = std::atoi(pos(8));
+  if (pf_params_load(param_file, &P) != PF_OK) return fatal(lib_error());
+  if (P.pf_params_schema_version != 1)
+    return fatal("pf_params_schema_version mismatch: file=" + std::to_string(P.pf_params_schema_version) +
+                 " expected=1");
+  if (zero_mode_sel == "PF_CONSERVED_Y_ZERO_MODE_V1")
+    P.zero_mode = 1;
+  else if (zero_mode_sel == "OFF")
+    P.zero_mode = 0;
+  else
+    return fatal("unknown --pf-zero-mode selector: " + zero_mode_sel);
+  if (P.out_every > P.nsteps) P.out_every = P.nsteps;
+  if (P.csv_out_every > P.nsteps) P.csv_out_every = P.nsteps;
+  if (P.out_every < 1) P.out_every = 1;
+  if (P.csv_out_every < 1) P.csv_out_every = 1;
+
+  const size_t N = (size_t)P.Nx * P.Ny * P.Nz;
+  std::vector<double> phi, xB, dydt;
+  double xB_cap = NAN;
+  double xB_out = 0.0;
+  if (init_mode == "raw_fields") {
+    Meta m;
+    if (meta_path.empty() || !read_meta(meta_path, &m)) return fatal("cannot read init meta: " + meta_path);
+    if (m.Nx != P.Nx || m.Ny != P.Ny || m.Nz != P.Nz) return fatal("init meta grid does not match the run grid");
+    if (!read_raw(phi_raw, m.dtype, N, &phi)) return fatal("cannot read phi raw field: " + phi_raw);
+    if (!read_raw(xB_raw, m.dtype, N, &xB)) return fatal("cannot read xB raw field: " + xB_raw);
+    if (!dydt_raw.empty() && !read_raw(dydt_raw, m.dtype, N, &dydt))
+      return fatal("cannot read dY_dt_prev raw field: " + dydt_raw);
+    xB_cap = std::isfinite(m.xB_max_safe) ? m.xB_max_safe : 0.07;   // minimize_xB_max_safe default
+    if (case_tag.empty()) case_tag = "raw_fields_relax";
+  } else {
+    return fatal("this build initialises from --init-mode raw_fields only");
+  }
+  for (size_t i = 0; i < N; ++i) xB_out += xB[i];
+  xB_out /= (double)N;
+
+  pf_handle* h = nullptr;
+  if (pf_create(&P, precision, &h) != PF_OK) return fatal(lib_error());
+  if (pf_set_fields(h, phi.data(), xB.data(), dydt.empty() ? nullptr : dydt.data(), xB_cap) != PF_OK)
+    return fatal(lib_error());
+
+  // Results tree (main_cuda.cu:25270-25462)
+  const char* root_env = std::getenv("CUDA_STO_RESULTS_ROOT");
+  const std::string root = (root_env && root_env[0]) ? root_env : "Results";
+  char run_dir[512];
+  std::snprintf(run_dir, sizeof run_dir, "%s_T%.0f_cuda_%dx%dx%d_dt%.3g_steps%d_xB%.3f",
+                P.elastic_enabled ? "chel" : "ch", P.temperature_C, P.Nx, P.Ny, P.Nz, P.dt, P.nsteps, xB_out);
+  const std::string out_dir = root + "/" + run_dir;
+  const std::string case_dir = out_dir + "/" + case_tag;
+  mkdir(root.c_str(), 0755);
+  mkdir(out_dir.c_str(), 0755);
+  mkdir(case_dir.c_str(), 0755);
+  copy_text_file(param_file, out_dir + "/pf_input.params");
+  copy_text_file(param_file, case_dir + "/pf_input.params");
+
+  FILE* csv = std::fopen((case_dir + "/vf_precip_vs_time_" + case_tag + ".csv").c_str(), "w");
+  if (csv) std::fprintf(csv, "step,t_code,t_real_s,vf_precip,R_avg\n");
+  const bool accel = P.elastic_enabled && (P.elastic_warm_start_enabled || P.elastic_residual_control_enabled);
+  FILE* etrace = accel ? std::fopen((case_dir + "/elastic_solver_trace.csv").c_str(), "w") : nullptr;
+  if (etrace)
+    std::fprintf(etrace,
+                 "step,warm_start_used,iterations,relative_residual,converged,iter_max,tolerance,"
+                 "elastic_solver_wall_s,source_field_step\n");
+  FILE* ztrace = P.zero_mode ? std::fopen((case_dir + "/pf_zero_mode_trace.csv").c_str(), "w") : nullptr;
+  if (ztrace)
+    std::fprintf(ztrace, "step,target_mass_code,lambda,iterations,mass_code\n");
+
+  pf_scalars s0;
+  pf_scalars_get(h, &s0);
+  const double target = s0.mass_xBtot;
+  auto write_row = [&](int step, const pf_scalars& sc) {
+    double R_avg = 0.0;
+    if (sc.n_if > 0.0) {
+      const double ratio = sc.n_in / (sc.n_if / (P.ic_phi_iface_w * 2.0));
+      R_avg = (P.Ny == 2 ? 2.0 : 3.0) * P.dx * ratio;
+    }
+    if (csv) {
+      std::fprintf(csv, "%d,%.8f,%.8e,%.8e,%.8e\n", step, step * P.dt, step * P.dt * P.t_real_unit,
+                   sc.volume_fraction, R_avg);
+      std::fflush(csv);
+    }
+    std::printf("[3D] step=%d <x_Btot>=%.10f vf_precip=%.6e R_avg=%.4f\n", step, sc.mean_xBtot,
+                sc.volume_fraction, R_avg);
+  };
+  write_row(0, s0);
+  auto dump_vtk = [&](int step, bool final_) {
+    std::vector<double> f(N);
+    const std::string sfx = final_ ? "final" : std::to_string(step);
+    pf_get_field(h, PF_PHI, f.data());
+    write_vtk(case_dir + "/phi_" + sfx + ".vtk", f, P.Nx, P.Ny, P.Nz, "phi", step);
+    pf_get_field(h, PF_XB, f.data());
+    write_vtk(case_dir + "/xB_" + sfx + ".vtk", f, P.Nx, P.Ny, P.Nz, "xB", step);
+    pf_get_field(h, PF_XBTOT, f.data());
+    write_vtk(case_dir + "/xBtot_" + sfx + ".vtk", f, P.Nx, P.Ny, P.Nz, "xB_tot", step);
+  };
+
+  std::vector<double> wall;
+  wall.reserve(P.nsteps);
+  for (int step = 1; step <= P.nsteps; ++step) {
+    const double t0 = now_s();
+    const int rc = pf_step(h, 1);
+    pf_sync(h);
+    const double w = now_s() - t0;
+    if (rc != PF_OK) {
+      const std::string msg = lib_error();
+      if (etrace) std::fclose(etrace);
+      return fatal(msg + " (step " + std::to_string(step) + ")");
+    }
+    wall.push_back(w);
+    pf_scalars sc;
+    pf_scalars_get(h, &sc);
+    if (etrace)
+      std::fprintf(etrace, "%d,%d,%d,%.17e,%d,%d,%.17e,%.9f,%d\n", step, step > 1 ? 1 : 0,
+                   sc.last_elastic_iterations, sc.last_elastic_residual,
+                   sc.last_elastic_residual <= P.elastic_residual_tolerance ? 1 : 0, P.elastic_iter_max,
+                   P.elastic_residual_tolerance, w, step - 1);
+    if (ztrace)
+      std::fprintf(ztrace, "%d,%.17e,%.17e,%d,%.17e\n", step, target, sc.zero_mode_lambda, sc.zero_mode_iterations,
+                   sc.mass_xBtot);
+    if (step % P.csv_out_every == 0 || step == P.nsteps) write_row(step, sc);
+    if (step % P.out_every == 0) dump_vtk(step, false);
+  }
+  dump_vtk(P.nsteps, true);
+  if (csv) std::fclose(csv);
+  if (etrace) std::fclose(etrace);
+  if (ztrace) std::fclose(ztrace);
+
+  // performance summary (main_cuda.cu:34245-34299)
+  if (!wall.empty()) {
+    std::vector<double> srt = wall;
+    std::sort(srt.begin(), srt.end());
+    double total = 0.0;
+    for (double w : wall) total += w;
+    const size_t n = wall.size();
+    const double median = (n % 2 == 0) ? 0.5 * (srt[n / 2 - 1] + srt[n / 2]) : srt[n / 2];
+    const size_t warm = n > 10 ? 10 : (n > 5 ? 5 : 0);
+    double ws = 0.0;
+    for (size_t i = warm; i < n; ++i) ws += wall[i];
+    const double warm_avg = (n > warm) ? ws / (double)(n - warm) : total / (double)n;
+    FILE* pc = std::fopen((case_dir + "/performance_summary.csv").c_str(), "w");
+    if (pc) {
+      std::fprintf(pc,
+                   "case_name,picard_enabled,picard_iters,picard_omega,nsteps,dt,total_walltime_s,"
+                   "avg_walltime_per_step_s,median_walltime_per_step_s,min_walltime_per_step_s,max_walltime_per_step_s,"
+                   "warmup_excluded_avg_walltime_per_step_s,estimated_steps_per_hour,relative_slowdown_vs_baseline\n");
+      std::fprintf(pc, "%s,%.17e,%.17e,%.17e,%.17e,%.17e,%.17e,%.17e,%.17e,%.17e,%.17e,%.17e,%.17e,%.17e\n",
+                   case_tag.c_str(), 0.0, 1.0, 1.0, (double)n, P.dt, total, total / n, median, srt.front(),
+                   srt.back(), warm_avg, 3600.0 / warm_avg, 1.0);
+      std::fclose(pc);
+    }
+    FILE* pj = std::fopen((case_dir + "/performance_summary.json").c_str(), "w");
+    if (pj) {
+      std::fprintf(pj,
+                   "{\n  \"case_name\": \"%s\",\n  \"nsteps\": %zu,\n  \"dt\": %.17e,\n  \"total_walltime_s\": %.17e,\n"
+                   "  \"avg_walltime_per_step_s\": %.17e,\n  \"median_walltime_per_step_s\": %.17e,\n"
+                   "  \"min_walltime_per_step_s\": %.17e,\n  \"max_walltime_per_step_s\": %.17e,\n"
+                   "  \"warmup_excluded_avg_walltime_per_step_s\": %.17e,\n  \"estimated_steps_per_hour\": %.17e,\n"
+                   "  \"precision\": \"%s\"\n}\n",
+                   case_tag.c_str(), n, P.dt, total, total / n, median, srt.front(), srt.back(), warm_avg,
+                   3600.0 / warm_avg, precision == PF_FP64 ? "fp64" : "mixed");
+      std::fclose(pj);
+    }
+  }
+  pf_destroy(h);
+  return 0;
+}

@@ -120,18 +120,21 @@ __global__ void __launch_bounds__(NT) finalize_kernel(const double* partials, in
     for (int r = 0; r < nred; ++r) out[r] = kinds.k[r] == RED_SUM ? outv[r] * scale : outv[r];
 }
 
-// sum (1-h) x + v_B h and sum h(clamp01 phi)  (cuda_kernels.cu:3792-3828, 5580-5607)
+// sum (1-h) x + v_B h, sum h(clamp01 phi), count(phi > 0.5), count(0.12 < phi < 0.88)
+// (cuda_kernels.cu:3792-3828, 5580-5607, 3746-3790)
 __global__ void __launch_bounds__(NT) mass_vf_kernel(const double* phi, const double* xB, double vB,
                                                      long n, double* partials) {
-  double acc[2] = {0.0, 0.0};
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   for (long g = blockIdx.x * (long)NT + threadIdx.x; g < n; g += (long)gridDim.x * NT) {
     const double p = phi[g];
     const double h = h_of_phi(p);
     acc[0] += (1.0 - h) * xB[g] + vB * h;
     acc[1] += h_of_phi(clamp01(p));
+    acc[2] += (p > 0.5) ? 1.0 : 0.0;
+    acc[3] += (p > 0.12 && p < 0.88) ? 1.0 : 0.0;
   }
-  int kinds[2] = {RED_SUM, RED_SUM};
-  block_reduce_store<2>(acc, kinds, partials + (size_t)blockIdx.x * 2);
+  int kinds[4] = {RED_SUM, RED_SUM, RED_SUM, RED_SUM};
+  block_reduce_store<4>(acc, kinds, partials + (size_t)blockIdx.x * 4);
 }
 
 // 0.5 kappa |grad phi|^2 by periodic central differences
@@ -286,6 +289,8 @@ struct Engine final : EngineBase {
   double *phi[2] = {nullptr, nullptr}, *Y = nullptr, *Ystar = nullptr, *xB = nullptr, *dYdt = nullptr;
   T* trs = nullptr;
   C *Sz = nullptr, *Sy = nullptr, *Sc = nullptr, *u = nullptr, *un = nullptr;
+  cx<double>* Ud = nullptr;   // FP64 accumulated displacement (mixed precision)
+  static constexpr bool kMixed = sizeof(T) == 4;
   C *twx = nullptr, *twy = nullptr, *twz = nullptr;
   double *kx = nullptr, *ky = nullptr, *kz = nullptr;
   double* partials = nullptr;
@@ -346,6 +351,7 @@ struct Engine final : EngineBase {
         (rc = dalloc(&trs, R)) || (rc = dalloc(&Sz, 6 * nSz)) || (rc = dalloc(&Sy, 6 * nSy)) ||
         (rc = dalloc(&Sc, 6 * nSc)) || (rc = dalloc(&u, 3 * nSc)) || (rc = dalloc(&un, 3 * nSc)))
       return rc;
+    if (kMixed && (rc = dalloc(&Ud, 3 * nSc))) return rc;
     auto wx = twiddles<T>(Nx), wy = twiddles<T>(Ny), wz = twiddles<T>(Nz);
     if ((rc = dalloc(&twx, Nx)) || (rc = dalloc(&twy, Ny)) || (rc = dalloc(&twz, Nz)) ||
         (rc = dalloc(&kx, ax.A)) || (rc = dalloc(&ky, ay.A)) || (rc = dalloc(&kz, az.A)))
@@ -475,19 +481,124 @@ struct Engine final : EngineBase {
     return PF_OK;
   }
 
-  int mass_vf(double* m, double* vf) {
+  int mass_vf(double* m, double* vf, double* nin = nullptr, double* nif = nullptr) {
     const int g = grid_for(R);
     hipLaunchKernelGGL(mass_vf_kernel, dim3(g), dim3(NT), 0, stream, phi[cur], xB, P.v_B, R, partials);
-    finalize(g, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal, 1.0);
-    int rc = fetch(2);
-    if (rc) return rc;
-    *m = hscal[0];
-    *vf = hscal[1] / (double)R;
+    finalize(g, 4, RedKinds{{RED_SUM, RED_SUM, RED_SUM, RED_SUM}}, dscal + 24, 1.0);
+    PF_HIP(hipMemcpyAsync(hscal + 24, dscal + 24, 4 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    PF_HIP(hipStreamSynchronize(stream));
+    *m = hscal[24];
+    *vf = hscal[25] / (double)R;
+    if (nin) *nin = hscal[26];
+    if (nif) *nif = hscal[27];
     return PF_OK;
   }
 
   // ---------------------------------------------------------------- elastic
-  int elastic_solve() {
+  int elastic_solve() { return kMixed ? elastic_solve_mixed() : elastic_solve_plain(); }
+
+  // Mixed precision (main_cuda.cu:28760-29095 semantics, F6 of the plan):
+  // iterate on FP32 increments d_{n+1} = A d_n with the FP64 sum U += d and
+  // the FP64 residual |d|/|U|.  The first update of a step evaluates the full
+  // affine map, later ones only its linear part -h dC:eps(d).
+  int elastic_solve_mixed() {
+    const bool accel = prm.elastic_warm_start_enabled || prm.elastic_residual_control_enabled;
+    const bool warm = accel && warm_valid;
+    const KDims d = kd();
+    const int gk = grid_for(nSc);
+    auto green = [&](int mode, bool resid) {
+      GreenAccArgs<T> ga{};
+      ga.d = d;
+      ga.n = nSc;
+      for (int i = 0; i < 6; ++i) {
+        ga.h[i] = sc(i);
+        ga.eps[i] = sc(i);
+      }
+      for (int i = 0; i < 3; ++i) ga.U[i] = Ud + (size_t)i * nSc;
+      ga.partials = resid ? partials : nullptr;
+      for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
+      ga.mode = mode;
+      hipLaunchKernelGGL(kop_green_acc<T>, dim3(gk), dim3(NT), 0, stream, ga);
+    };
+    auto strain_of_U = [&]() {
+      StrainArgs<T, double> sa{};
+      sa.d = d;
+      sa.n = nSc;
+      for (int i = 0; i < 3; ++i) sa.u[i] = Ud + (size_t)i * nSc;
+      for (int i = 0; i < 6; ++i) sa.eps[i] = sc(i);
+      hipLaunchKernelGGL((kop_strain<T, double>), dim3(gk), dim3(NT), 0, stream, sa);
+    };
+    if (!warm) {
+      typename OpElasticCold<T>::Args oa{P, phi[cur], xB};
+      launch_zpass<T, 0, 6, OpElasticCold<T>>(Nz, stream, zargs(0, 0, 0, 6), oa);
+      ypass(false, 0, 6);
+      xpass(false, 0, 6, Sc);
+      green(0, false);
+    } else {
+      strain_of_U();
+    }
+    xpass(true, 0, 6, Sc);
+    ypass(true, 0, 6);
+    int it = accel ? 0 : 1;
+    bool converged = !accel;
+    double res = NAN;
+    const T invN = T(1.0 / (double)R);
+    while (true) {
+      const int mode = !accel ? 3 : (it == 0 ? 1 : 2);
+      typename OpHij<T>::Args oa{P, phi[cur], xB, invN, mode == 2 ? 1 : 0};
+      launch_zpass<T, 6, 6, OpHij<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
+      ypass(false, 0, 6);
+      xpass(false, 0, 6, Sc);
+      green(mode, accel);
+      if (accel) {
+        finalize(gk, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal, 1.0);
+        PF_HIP(hipMemcpyAsync(hscal, dscal, 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
+      }
+      xpass(true, 0, 6, Sc);
+      ypass(true, 0, 6);
+      ++it;
+      if (accel) {
+        PF_HIP(hipStreamSynchronize(stream));
+        const double ds = hscal[0], rs = hscal[1];
+        if (!std::isfinite(ds) || !std::isfinite(rs) || ds < 0.0 || rs < 0.0) {
+          last_iters = it;
+          return fail(PF_E_NONFINITE, "elastic residual is non-finite at step " +
+                                          std::to_string(steps_done + 1));
+        }
+        res = std::sqrt(ds / std::fmax(rs, prm.elastic_residual_absolute_floor));
+        if (it >= prm.elastic_iter_min && res <= prm.elastic_residual_tolerance) {
+          converged = true;
+          break;
+        }
+        if (it >= prm.elastic_iter_max) break;
+      } else {
+        if (it >= prm.elastic_iter_max) break;
+      }
+    }
+    if (accel) {
+      // the loop leaves strain(increment) behind; the stress needs strain(U)
+      strain_of_U();
+      xpass(true, 0, 6, Sc);
+      ypass(true, 0, 6);
+    }
+    last_iters = it;
+    last_res = res;
+    elastic_total += it;
+    if (accel) {
+      warm_valid = true;
+      if (!converged && prm.elastic_fail_on_nonconvergence) {
+        char buf[256];
+        std::snprintf(buf, sizeof buf,
+                      "accelerated elastic solver did not converge at step %lld: iterations=%d "
+                      "residual=%.17e tolerance=%.17e",
+                      steps_done + 1, it, res, prm.elastic_residual_tolerance);
+        return fail(PF_E_NONCONVERGED, buf);
+      }
+    }
+    return PF_OK;
+  }
+
+  int elastic_solve_plain() {
     const bool accel = prm.elastic_warm_start_enabled || prm.elastic_residual_control_enabled;
     const bool warm = accel && warm_valid;
     const KDims d = kd();
@@ -770,9 +881,11 @@ struct Engine final : EngineBase {
 
   int scalars(pf_scalars* s) override {
     std::memset(s, 0, sizeof(*s));
-    double m, vf;
-    int rc = mass_vf(&m, &vf);
+    double m, vf, nin, nif;
+    int rc = mass_vf(&m, &vf, &nin, &nif);
     if (rc) return rc;
+    s->n_in = nin;
+    s->n_if = nif;
     s->mass_xBtot = m;
     s->mean_xBtot = m / (double)R;
     s->volume_fraction = vf;
@@ -794,6 +907,10 @@ struct Engine final : EngineBase {
   long long elastic_state_size() override { return nSc; }
   int get_elastic_state(double* out) override {
     PF_HIP(hipStreamSynchronize(stream));
+    if (kMixed) {
+      PF_HIP(hipMemcpy(out, Ud, 3 * nSc * sizeof(cx<double>), hipMemcpyDeviceToHost));
+      return PF_OK;
+    }
     std::vector<C> t(3 * nSc);
     PF_HIP(hipMemcpy(t.data(), u, 3 * nSc * sizeof(C), hipMemcpyDeviceToHost));
     for (long i = 0; i < 3 * nSc; ++i) {
@@ -803,6 +920,11 @@ struct Engine final : EngineBase {
     return PF_OK;
   }
   int set_elastic_state(const double* in) override {
+    if (kMixed) {
+      PF_HIP(hipMemcpy(Ud, in, 3 * nSc * sizeof(cx<double>), hipMemcpyHostToDevice));
+      warm_valid = true;
+      return PF_OK;
+    }
     std::vector<C> t(3 * nSc);
     for (long i = 0; i < 3 * nSc; ++i) t[i] = C{T(in[2 * i]), T(in[2 * i + 1])};
     PF_HIP(hipMemcpy(u, t.data(), 3 * nSc * sizeof(C), hipMemcpyHostToDevice));
@@ -863,6 +985,76 @@ struct Engine final : EngineBase {
 
   int sync() override {
     PF_HIP(hipStreamSynchronize(stream));
+    return PF_OK;
+  }
+
+  // One launch of kernel `id` with the current buffers; returns its bytes.
+  double launch_one(int id) {
+    const double cs = (double)sizeof(C);
+    const T invN = T(1.0 / (double)R);
+    const KDims d = kd();
+    const int gk = grid_for(nSc);
+    switch (id) {
+      case PF_K_ZPASS_HIJ: {
+        typename OpHij<T>::Args oa{P, phi[cur], xB, invN, 0};
+        launch_zpass<T, 6, 6, OpHij<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
+        return 12.0 * nSz * cs + 2.0 * R * 8.0;
+      }
+      case PF_K_YPASS_FWD6: ypass(false, 0, 6); return 6.0 * (nSz + nSy) * cs;
+      case PF_K_YPASS_INV6: ypass(true, 0, 6); return 6.0 * (nSz + nSy) * cs;
+      case PF_K_XPASS_FWD6: xpass(false, 0, 6, Sc); return 6.0 * (nSy + nSc) * cs;
+      case PF_K_XPASS_INV6: xpass(true, 0, 6, Sc); return 6.0 * (nSy + nSc) * cs;
+      case PF_K_GREEN: {
+        GreenArgs<T> ga{};
+        ga.d = d;
+        ga.n = nSc;
+        for (int i = 0; i < 6; ++i) {
+          ga.h[i] = sc(i);
+          ga.eps[i] = sc(i);
+        }
+        for (int i = 0; i < 3; ++i) {
+          ga.u_old[i] = uu(u, i);
+          ga.u_new[i] = uu(un, i);
+        }
+        for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
+        ga.have_old = 1;
+        ga.partials = partials;
+        hipLaunchKernelGGL(kop_green<T>, dim3(gk), dim3(NT), 0, stream, ga);
+        return (12.0 + 6.0) * nSc * cs;
+      }
+      case PF_K_ZPASS_PHIRHS: {
+        typename OpPhiRhs<T, true>::Args oa{P, phi[cur], xB, trs, invN};
+        launch_zpass<T, 6, 1, OpPhiRhs<T, true>>(Nz, stream, zargs(0, 6, 0, 1), oa);
+        return 7.0 * nSz * cs + 2.0 * R * 8.0 + R * (double)sizeof(T);
+      }
+      case PF_K_ZPASS_FLUX: {
+        const double* ph = phi[cur];
+        typename OpFlux<T>::Args oa{P, ph, xB, ph, phi[1 - cur], Y, dYdt, invN, 1, 0, 0, 1.0};
+        launch_zpass<T, 3, 5, OpFlux<T>>(Nz, stream, zargs(0, 3, 0, 5), oa);
+        return 8.0 * nSz * cs + 6.0 * R * 8.0;
+      }
+      default:
+        return 0.0;
+    }
+  }
+
+  int time_kernel(int id, int reps, double* us, double* bytes) override {
+    hipEvent_t e0, e1;
+    PF_HIP(hipEventCreate(&e0));
+    PF_HIP(hipEventCreate(&e1));
+    launch_one(id);   // warm-up launch
+    PF_HIP(hipEventRecord(e0, stream));
+    double b = 0.0;
+    for (int r = 0; r < reps; ++r) b = launch_one(id);
+    PF_HIP(hipEventRecord(e1, stream));
+    PF_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    PF_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *us = 1000.0 * (double)ms / reps;
+    *bytes = b;
+    warm_valid = false;
     return PF_OK;
   }
 };

@@ -22,6 +22,7 @@ struct EngineBase {
   virtual int fft_forward(const double* in, double* out_half) = 0;
   virtual int fft_inverse(const double* in_half, double* out) = 0;
   virtual int sync() = 0;
+  virtual int time_kernel(int id, int reps, double* us, double* bytes) = 0;
 };
 
 std::unique_ptr<EngineBase> make_engine_f64();

@@ -448,15 +448,92 @@ __global__ void __launch_bounds__(NT) kop_green(GreenArgs<T> ga) {
   }
 }
 
+// Mixed-precision Green update with an FP64 accumulated displacement U:
+//   mode 0 (cold):      U = G(h),             eps = strain(U)
+//   mode 1 (full map):  d = G(h) - U, U += d, eps = strain(d)
+//   mode 2 (increment): d = G(h),     U += d, eps = strain(d)   (h from the linear part)
+//   mode 3 (legacy):    d = G(h) - U, U += d, eps = strain(U)
+// The residual sums w|d|^2 and w|U|^2 are FP64, so |d|/|U| has no FP32 floor.
 template <typename T>
+struct GreenAccArgs {
+  KDims d;
+  long n;
+  const cx<T>* h[6];
+  cx<double>* U[3];
+  cx<T>* eps[6];
+  double* partials;
+  double C[21];
+  int mode;
+};
+
+template <typename T>
+__global__ void __launch_bounds__(NT) kop_green_acc(GreenAccArgs<T> ga) {
+  double acc[2] = {0.0, 0.0};
+  for (long c = blockIdx.x * (long)NT + threadIdx.x; c < ga.n; c += (long)gridDim.x * NT) {
+    double kx, ky, kz;
+    int iz;
+    kvec(ga.d, c, kx, ky, kz, iz);
+    cx<double> hh[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) hh[i] = cx<double>{double(ga.h[i][c].re), double(ga.h[i][c].im)};
+    double G[6];
+    cx<double> g[3] = {{0, 0}, {0, 0}, {0, 0}};
+    if (c != 0 && green_tensor(ga.C, kx, ky, kz, G)) {
+      cx<double> t1 = {kx * hh[0].re + ky * hh[5].re + kz * hh[4].re, kx * hh[0].im + ky * hh[5].im + kz * hh[4].im};
+      cx<double> t2 = {kx * hh[5].re + ky * hh[1].re + kz * hh[3].re, kx * hh[5].im + ky * hh[1].im + kz * hh[3].im};
+      cx<double> t3 = {kx * hh[4].re + ky * hh[3].re + kz * hh[2].re, kx * hh[4].im + ky * hh[3].im + kz * hh[2].im};
+      const double gm[3][3] = {{G[0], G[1], G[2]}, {G[1], G[3], G[4]}, {G[2], G[4], G[5]}};
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        cx<double> s = {gm[i][0] * t1.re + gm[i][1] * t2.re + gm[i][2] * t3.re,
+                        gm[i][0] * t1.im + gm[i][1] * t2.im + gm[i][2] * t3.im};
+        g[i] = cx<double>{s.im, -s.re};
+      }
+    }
+    cx<double> dlt[3], un[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      cx<double> uo = ga.mode == 0 ? cx<double>{0.0, 0.0} : ga.U[i][c];
+      if (ga.mode == 0) {
+        un[i] = g[i];
+        dlt[i] = g[i];
+      } else if (ga.mode == 2) {
+        dlt[i] = g[i];
+        un[i] = cx<double>{uo.re + g[i].re, uo.im + g[i].im};
+      } else {
+        dlt[i] = cx<double>{g[i].re - uo.re, g[i].im - uo.im};
+        un[i] = g[i];
+      }
+      ga.U[i][c] = un[i];
+    }
+    if (ga.partials) {
+      const double w = (iz == 0) ? 1.0 : 2.0;
+      double dsum = 0.0, rsum = 0.0;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        dsum += dlt[i].re * dlt[i].re + dlt[i].im * dlt[i].im;
+        rsum += un[i].re * un[i].re + un[i].im * un[i].im;
+      }
+      acc[0] += w * dsum;
+      acc[1] += w * rsum;
+    }
+    strain_of<T>(kx, ky, kz, (ga.mode == 0 || ga.mode == 3) ? un : dlt, ga.eps, c);
+  }
+  if (ga.partials) {
+    int kinds[2] = {RED_SUM, RED_SUM};
+    block_reduce_store<2>(acc, kinds, ga.partials + (size_t)blockIdx.x * 2);
+  }
+}
+
+template <typename T, typename UT = T>
 struct StrainArgs {
   KDims d;
   long n;
-  const cx<T>* u[3];
+  const cx<UT>* u[3];
   cx<T>* eps[6];
 };
-template <typename T>
-__global__ void __launch_bounds__(NT) kop_strain(StrainArgs<T> sa) {
+template <typename T, typename UT = T>
+__global__ void __launch_bounds__(NT) kop_strain(StrainArgs<T, UT> sa) {
   for (long c = blockIdx.x * (long)NT + threadIdx.x; c < sa.n; c += (long)gridDim.x * NT) {
     double kx, ky, kz;
     int iz;

@@ -210,6 +210,12 @@ int pf_set_elastic_state(pf_handle* h, const double* in) {
   return h->eng->set_elastic_state(in);
 }
 
+int pf_time_kernel(pf_handle* h, int kernel_id, int reps, double* us, double* bytes) {
+  if (!h || !us || !bytes || reps < 1 || kernel_id < 0 || kernel_id >= PF_K_COUNT)
+    return pf::fail(PF_E_ARG, "pf_time_kernel: bad argument");
+  return h->eng->time_kernel(kernel_id, reps, us, bytes);
+}
+
 int pf_sync(pf_handle* h) {
   if (!h) return pf::fail(PF_E_ARG, "pf_sync: null");
   return h->eng->sync();

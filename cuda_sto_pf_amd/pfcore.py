@@ -16,6 +16,8 @@ LIB_PATH = os.environ.get("PFCORE_LIB", os.path.join(HERE, "libpfcore.so"))
 PF_FP64 = 0
 PF_MIXED = 1
 PF_PHI, PF_XB, PF_Y, PF_DYDT, PF_XBTOT, PF_TRSIGMA = range(6)
+KERNELS = {"zpass_hij": 0, "ypass_fwd6": 1, "ypass_inv6": 2, "xpass_fwd6": 3, "xpass_inv6": 4,
+           "green": 5, "zpass_phirhs": 6, "zpass_flux": 7}
 
 STATUS = {0: "PF_OK", -1: "PF_E_ARG", -2: "PF_E_PARAMS", -3: "PF_E_HIP", -4: "PF_E_NONCONVERGED",
           -5: "PF_E_ZERO_MODE", -6: "PF_E_NONFINITE"}
@@ -64,6 +66,7 @@ class PfScalars(C.Structure):
         ("last_elastic_residual", C.c_double), ("zero_mode_lambda", C.c_double),
         ("last_elastic_iterations", C.c_int), ("zero_mode_iterations", C.c_int),
         ("steps_done", C.c_longlong), ("elastic_iterations_total", C.c_longlong),
+        ("n_in", C.c_double), ("n_if", C.c_double),
     ]
 
     def as_dict(self):
@@ -86,6 +89,8 @@ EXPORTS = {
     "pf_get_elastic_state": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pf_set_elastic_state": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pf_sync": (C.c_int, [C.c_void_p]),
+    "pf_time_kernel": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double)]),
     "pf_destroy": (C.c_int, [C.c_void_p]),
     "pf_fft_forward": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "pf_fft_inverse": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
@@ -217,6 +222,13 @@ class Simulation:
 
     def sync(self):
         _check(lib().pf_sync(self._h))
+
+    def time_kernel(self, kernel_id, reps=20):
+        """Mean microseconds and bytes of one launch of a step kernel (HIP events)."""
+        us = C.c_double()
+        b = C.c_double()
+        _check(lib().pf_time_kernel(self._h, int(kernel_id), int(reps), C.byref(us), C.byref(b)))
+        return us.value, b.value
 
 
 def fft_forward(a, precision=PF_FP64):

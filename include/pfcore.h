@@ -91,6 +91,8 @@ typedef struct pf_scalars {
   int zero_mode_iterations;
   long long steps_done;
   long long elastic_iterations_total;
+  double n_in;                /* count of phi > 0.5          (reduce_sum_N_in_N_if_kernel) */
+  double n_if;                /* count of 0.12 < phi < 0.88 */
 } pf_scalars;
 
 typedef struct pf_handle pf_handle;
@@ -140,6 +142,26 @@ int pf_scalars_get(pf_handle* h, pf_scalars* out);
 long long pf_elastic_state_size(pf_handle* h);
 int pf_get_elastic_state(pf_handle* h, double* out);
 int pf_set_elastic_state(pf_handle* h, const double* in);
+
+/* Kernel ids for pf_time_kernel. */
+enum pf_kernel_id {
+  PF_K_ZPASS_HIJ = 0,      /* z pass: inverse 6 strains -> h_ij -> forward 6 */
+  PF_K_YPASS_FWD6 = 1,     /* y pass, forward, 6 components */
+  PF_K_YPASS_INV6 = 2,     /* y pass, inverse, 6 components */
+  PF_K_XPASS_FWD6 = 3,     /* x pass, forward, 6 components */
+  PF_K_XPASS_INV6 = 4,     /* x pass, inverse, 6 components */
+  PF_K_GREEN = 5,          /* k-space Green update + residual + strain */
+  PF_K_ZPASS_PHIRHS = 6,   /* z pass: inverse 6 strains -> stress, dF/dphi -> forward 1 */
+  PF_K_ZPASS_FLUX = 7,     /* z pass: inverse 3 gradients -> fluxes -> forward 5 */
+  PF_K_COUNT = 8
+};
+
+/* Time `reps` back-to-back launches of one kernel of the step on the
+ * handle's stream with HIP events, using the handle's current buffers, and
+ * report the mean duration (microseconds) and the bytes a launch must move
+ * (each input read once, each output written once).  Measurement only:
+ * the simulation state is not valid afterwards (call pf_set_fields). */
+int pf_time_kernel(pf_handle* h, int kernel_id, int reps, double* us_per_launch, double* bytes_per_launch);
 
 /* Synchronise the handle's stream. */
 int pf_sync(pf_handle* h);

@@ -166,7 +166,7 @@ def fwd(a, ks):
 
 def inv(ak, shape):
     """Z2D/C2R followed by the explicit x(1/N)."""
-    return np.fft.irfftn(ak, s=shape)
+    return np.fft.irfftn(ak, s=shape, axes=(0, 1, 2))
 
 
 # ------------------------------------------------------------------- elasticity
