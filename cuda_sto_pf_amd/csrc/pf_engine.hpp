@@ -120,6 +120,22 @@ __global__ void __launch_bounds__(NT) finalize_kernel(const double* partials, in
     for (int r = 0; r < nred; ++r) out[r] = kinds.k[r] == RED_SUM ? outv[r] * scale : outv[r];
 }
 
+// Up to 16 sums of block partials in a fixed order; one workgroup.
+__global__ void __launch_bounds__(NT) finalize_sums_kernel(const double* partials, int nblocks, int nred,
+                                                           double* out) {
+  double v[16];
+  for (int r = 0; r < 16; ++r) v[r] = 0.0;
+  for (int b = threadIdx.x; b < nblocks; b += NT)
+    for (int r = 0; r < nred; ++r) v[r] += partials[(size_t)b * nred + r];
+  __shared__ double outv[16];
+  int kinds[16];
+  for (int r = 0; r < 16; ++r) kinds[r] = RED_SUM;
+  block_reduce_store<16>(v, kinds, outv);
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int r = 0; r < nred; ++r) out[r] = outv[r];
+}
+
 // sum (1-h) x + v_B h, sum h(clamp01 phi), count(phi > 0.5), count(0.12 < phi < 0.88)
 // (cuda_kernels.cu:3792-3828, 5580-5607, 3746-3790)
 __global__ void __launch_bounds__(NT) mass_vf_kernel(const double* phi, const double* xB, double vB,
@@ -289,7 +305,14 @@ struct Engine final : EngineBase {
   double *phi[2] = {nullptr, nullptr}, *Y = nullptr, *Ystar = nullptr, *xB = nullptr, *dYdt = nullptr;
   T* trs = nullptr;
   C *Sz = nullptr, *Sy = nullptr, *Sc = nullptr, *u = nullptr, *un = nullptr;
-  cx<double>* Ud = nullptr;   // FP64 accumulated displacement (mixed precision)
+  cx<double>* Ud = nullptr;   // FP64 accumulated displacement (accelerated solver)
+  cx<double>* Gd = nullptr;   // FP64 fixed-point residual g = F(U) - U
+  cx<double>* Hist = nullptr; // last three converged U (warm-start extrapolation)
+  C* dinc = nullptr;          // current increment
+  C* dU = nullptr;            // Anderson history of increments [AA_MAX][3]
+  C* dG = nullptr;            // Anderson history of residual differences [AA_MAX][3]
+  int hist_count = 0, hist_head = 0;
+  bool extended = false;      // accelerated solver with U/g state in use
   static constexpr bool kMixed = sizeof(T) == 4;
   C *twx = nullptr, *twy = nullptr, *twz = nullptr;
   double *kx = nullptr, *ky = nullptr, *kz = nullptr;
@@ -351,7 +374,16 @@ struct Engine final : EngineBase {
         (rc = dalloc(&trs, R)) || (rc = dalloc(&Sz, 6 * nSz)) || (rc = dalloc(&Sy, 6 * nSy)) ||
         (rc = dalloc(&Sc, 6 * nSc)) || (rc = dalloc(&u, 3 * nSc)) || (rc = dalloc(&un, 3 * nSc)))
       return rc;
-    if (kMixed && (rc = dalloc(&Ud, 3 * nSc))) return rc;
+    {
+      const bool accel = p.elastic_enabled && (p.elastic_warm_start_enabled || p.elastic_residual_control_enabled);
+      extended = accel && (kMixed || p.elastic_warm_start_order > 1 || p.elastic_anderson_depth > 0);
+      if ((kMixed || extended) && (rc = dalloc(&Ud, 3 * nSc))) return rc;
+      if (extended) {
+        if ((rc = dalloc(&Gd, 3 * nSc)) || (rc = dalloc(&Hist, 9 * nSc)) || (rc = dalloc(&dinc, 3 * nSc)) ||
+            (rc = dalloc(&dU, (size_t)AA_MAX * 3 * nSc)) || (rc = dalloc(&dG, (size_t)AA_MAX * 3 * nSc)))
+          return rc;
+      }
+    }
     auto wx = twiddles<T>(Nx), wy = twiddles<T>(Ny), wz = twiddles<T>(Nz);
     if ((rc = dalloc(&twx, Nx)) || (rc = dalloc(&twy, Ny)) || (rc = dalloc(&twz, Nz)) ||
         (rc = dalloc(&kx, ax.A)) || (rc = dalloc(&ky, ay.A)) || (rc = dalloc(&kz, az.A)))
@@ -369,7 +401,7 @@ struct Engine final : EngineBase {
     maxg = std::max<long>(maxg, zpass_grid<T, 6, 1>(Nz, rows));
     maxg = std::max<long>(maxg, zpass_grid<T, 1, 0>(Nz, rows));
     maxg = std::max<long>(maxg, zpass_grid<T, 0, 1>(Nz, rows));
-    npartials = (size_t)maxg * 4;
+    npartials = (size_t)maxg * 16;   // up to 16 reductions per block (Anderson dots use AA_NRED)
     if ((rc = dalloc(&partials, npartials)) || (rc = dalloc(&dscal, 64))) return rc;
     PF_HIP(hipHostMalloc((void**)&hscal, 64 * sizeof(double), hipHostMallocDefault));
     return PF_OK;
@@ -472,6 +504,8 @@ struct Engine final : EngineBase {
     PF_HIP(hipMemcpy(Y, yy.data(), R * sizeof(double), hipMemcpyHostToDevice));
     PF_HIP(hipMemcpy(dYdt, dd.data(), R * sizeof(double), hipMemcpyHostToDevice));
     warm_valid = false;
+    hist_count = 0;
+    hist_head = 0;
     steps_done = 0;
     elastic_total = 0;
     double m, vf;
@@ -495,7 +529,178 @@ struct Engine final : EngineBase {
   }
 
   // ---------------------------------------------------------------- elastic
-  int elastic_solve() { return kMixed ? elastic_solve_mixed() : elastic_solve_plain(); }
+  int elastic_solve() {
+    if (extended) return elastic_solve_accel();
+    return kMixed ? elastic_solve_mixed() : elastic_solve_plain();
+  }
+
+  cx<double>* hist_slot(int i) { return Hist + (size_t)i * 3 * nSc; }
+  C* du_slot(int j, int i) { return dU + ((size_t)j * 3 + i) * nSc; }
+  C* dg_slot(int j, int i) { return dG + ((size_t)j * 3 + i) * nSc; }
+
+  // Accelerated fixed point (same map F, residual |F(U)-U|/|F(U)| and test as
+  // main_cuda.cu:28887-29046): warm start by polynomial extrapolation of the
+  // last converged displacements, FP64 accumulation of U and g, increments in
+  // T through the linear part of the map, and Anderson mixing of depth m.
+  int elastic_solve_accel() {
+    const int m = prm.elastic_anderson_depth;
+    const KDims d = kd();
+    const int gk = grid_for(nSc);
+    const T invN = T(1.0 / (double)R);
+    if (warm_valid && hist_count > 0) {
+      const int order = std::min(prm.elastic_warm_start_order, hist_count);
+      const double coef[3][3] = {{1.0, 0.0, 0.0}, {2.0, -1.0, 0.0}, {3.0, -3.0, 1.0}};
+      auto h_at = [&](int back) -> const cx<double>* {   // back = 0: latest
+        return back < order ? hist_slot((hist_head - 1 - back + 3) % 3) : nullptr;
+      };
+      hipLaunchKernelGGL(kop_extrapolate<0>, dim3(grid_for(3 * nSc)), dim3(NT), 0, stream, 3 * nSc, Ud, h_at(0),
+                         h_at(1), h_at(2), coef[order - 1][0], coef[order - 1][1], coef[order - 1][2]);
+      StrainArgs<T, double> sa{};
+      sa.d = d;
+      sa.n = nSc;
+      for (int i = 0; i < 3; ++i) sa.u[i] = Ud + (size_t)i * nSc;
+      for (int i = 0; i < 6; ++i) sa.eps[i] = sc(i);
+      hipLaunchKernelGGL((kop_strain<T, double>), dim3(gk), dim3(NT), 0, stream, sa);
+    } else {
+      typename OpElasticCold<T>::Args oa{P, phi[cur], xB};
+      launch_zpass<T, 0, 6, OpElasticCold<T>>(Nz, stream, zargs(0, 0, 0, 6), oa);
+      ypass(false, 0, 6);
+      xpass(false, 0, 6, Sc);
+      GreenAccArgs<T> ga{};
+      ga.d = d;
+      ga.n = nSc;
+      for (int i = 0; i < 6; ++i) {
+        ga.h[i] = sc(i);
+        ga.eps[i] = sc(i);
+      }
+      for (int i = 0; i < 3; ++i) ga.U[i] = Ud + (size_t)i * nSc;
+      ga.partials = nullptr;
+      for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
+      ga.mode = 0;
+      hipLaunchKernelGGL(kop_green_acc<T>, dim3(gk), dim3(NT), 0, stream, ga);
+    }
+    double H[AA_MAX][AA_MAX] = {};
+    double bvec[AA_MAX] = {};
+    int nh = 0, next = 0, it = 0;
+    bool converged = false;
+    double res = NAN;
+    while (true) {
+      const int mode = (it == 0) ? 0 : 1;
+      if (mode == 1) {
+        // Anderson coefficients: min | g - sum gamma_j dg_j |
+        double gam[AA_MAX] = {};
+        if (nh > 0) {
+          double A[AA_MAX][AA_MAX + 1];
+          double tr = 0.0;
+          for (int i = 0; i < nh; ++i) tr += H[i][i];
+          for (int i = 0; i < nh; ++i) {
+            for (int j = 0; j < nh; ++j) A[i][j] = H[i][j] + (i == j ? 1e-12 * tr / nh : 0.0);
+            A[i][nh] = bvec[i];
+          }
+          for (int col = 0; col < nh; ++col) {   // Gaussian elimination, partial pivoting
+            int piv = col;
+            for (int r = col + 1; r < nh; ++r)
+              if (std::fabs(A[r][col]) > std::fabs(A[piv][col])) piv = r;
+            if (piv != col)
+              for (int q = 0; q <= nh; ++q) std::swap(A[col][q], A[piv][q]);
+            if (A[col][col] == 0.0) continue;
+            for (int r = col + 1; r < nh; ++r) {
+              const double f = A[r][col] / A[col][col];
+              for (int q = col; q <= nh; ++q) A[r][q] -= f * A[col][q];
+            }
+          }
+          for (int i = nh - 1; i >= 0; --i) {
+            double v = A[i][nh];
+            for (int j = i + 1; j < nh; ++j) v -= A[i][j] * gam[j];
+            gam[i] = (A[i][i] != 0.0) ? v / A[i][i] : 0.0;
+          }
+        }
+        AAUpdateArgs<T> ua{};
+        ua.d = d;
+        ua.n = nSc;
+        for (int i = 0; i < 3; ++i) {
+          ua.g[i] = Gd + (size_t)i * nSc;
+          ua.U[i] = Ud + (size_t)i * nSc;
+          ua.dinc[i] = dinc + (size_t)i * nSc;
+          for (int j = 0; j < AA_MAX; ++j) {
+            ua.du[j][i] = du_slot(j, i);
+            ua.dg[j][i] = dg_slot(j, i);
+          }
+        }
+        for (int i = 0; i < 6; ++i) ua.eps[i] = sc(i);
+        for (int j = 0; j < AA_MAX; ++j) ua.gamma[j] = gam[j];
+        ua.nhist = nh;
+        ua.slot = (m > 0) ? next : -1;
+        hipLaunchKernelGGL(kop_aa_update<T>, dim3(gk), dim3(NT), 0, stream, ua);
+      }
+      xpass(true, 0, 6, Sc);
+      ypass(true, 0, 6);
+      typename OpHij<T>::Args oa{P, phi[cur], xB, invN, mode == 1 ? 1 : 0};
+      launch_zpass<T, 6, 6, OpHij<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
+      ypass(false, 0, 6);
+      xpass(false, 0, 6, Sc);
+      const int nh_after = (mode == 1 && m > 0) ? std::min(nh + 1, m) : nh;
+      GreenAAArgs<T> ga{};
+      ga.d = d;
+      ga.n = nSc;
+      for (int i = 0; i < 6; ++i) ga.h[i] = sc(i);
+      for (int i = 0; i < 3; ++i) {
+        ga.U[i] = Ud + (size_t)i * nSc;
+        ga.g[i] = Gd + (size_t)i * nSc;
+        ga.dinc[i] = dinc + (size_t)i * nSc;
+        for (int j = 0; j < AA_MAX; ++j) ga.dg[j][i] = dg_slot(j, i);
+      }
+      ga.nhist = nh_after;
+      ga.slot = (mode == 1 && m > 0) ? next : -1;
+      ga.mode = mode;
+      ga.partials = partials;
+      for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
+      hipLaunchKernelGGL(kop_green_aa<T>, dim3(gk), dim3(NT), 0, stream, ga);
+      hipLaunchKernelGGL(finalize_sums_kernel, dim3(1), dim3(NT), 0, stream, partials, gk, AA_NRED, dscal + 32);
+      PF_HIP(hipMemcpyAsync(hscal + 32, dscal + 32, AA_NRED * sizeof(double), hipMemcpyDeviceToHost, stream));
+      PF_HIP(hipStreamSynchronize(stream));
+      ++it;
+      const double gg = hscal[32], ff = hscal[33];
+      if (!std::isfinite(gg) || !std::isfinite(ff) || gg < 0.0 || ff < 0.0) {
+        last_iters = it;
+        return fail(PF_E_NONFINITE, "elastic residual is non-finite at step " + std::to_string(steps_done + 1));
+      }
+      res = std::sqrt(gg / std::fmax(ff, prm.elastic_residual_absolute_floor));
+      if (mode == 1 && m > 0) {
+        for (int j = 0; j < nh_after; ++j) {
+          H[next][j] = H[j][next] = hscal[34 + j];
+        }
+        nh = nh_after;
+        next = (next + 1) % m;
+      }
+      for (int j = 0; j < nh; ++j) bvec[j] = hscal[34 + AA_MAX + j];
+      if (it >= prm.elastic_iter_min && res <= prm.elastic_residual_tolerance) {
+        converged = true;
+        break;
+      }
+      if (it >= prm.elastic_iter_max) break;
+    }
+    hipLaunchKernelGGL(kop_aa_final<T>, dim3(gk), dim3(NT), 0, stream, d, nSc, Ud, Ud + nSc, Ud + 2 * nSc, Gd,
+                       Gd + nSc, Gd + 2 * nSc, sc(0), sc(1), sc(2), sc(3), sc(4), sc(5));
+    xpass(true, 0, 6, Sc);
+    ypass(true, 0, 6);
+    PF_HIP(hipMemcpyAsync(hist_slot(hist_head), Ud, 3 * nSc * sizeof(cx<double>), hipMemcpyDeviceToDevice, stream));
+    hist_head = (hist_head + 1) % 3;
+    hist_count = std::min(hist_count + 1, 3);
+    last_iters = it;
+    last_res = res;
+    elastic_total += it;
+    warm_valid = true;
+    if (!converged && prm.elastic_fail_on_nonconvergence) {
+      char buf[256];
+      std::snprintf(buf, sizeof buf,
+                    "accelerated elastic solver did not converge at step %lld: iterations=%d residual=%.17e "
+                    "tolerance=%.17e",
+                    steps_done + 1, it, res, prm.elastic_residual_tolerance);
+      return fail(PF_E_NONCONVERGED, buf);
+    }
+    return PF_OK;
+  }
 
   // Mixed precision (main_cuda.cu:28760-29095 semantics, F6 of the plan):
   // iterate on FP32 increments d_{n+1} = A d_n with the FP64 sum U += d and
@@ -907,7 +1112,7 @@ struct Engine final : EngineBase {
   long long elastic_state_size() override { return nSc; }
   int get_elastic_state(double* out) override {
     PF_HIP(hipStreamSynchronize(stream));
-    if (kMixed) {
+    if (kMixed || extended) {
       PF_HIP(hipMemcpy(out, Ud, 3 * nSc * sizeof(cx<double>), hipMemcpyDeviceToHost));
       return PF_OK;
     }
@@ -920,8 +1125,13 @@ struct Engine final : EngineBase {
     return PF_OK;
   }
   int set_elastic_state(const double* in) override {
-    if (kMixed) {
+    if (kMixed || extended) {
       PF_HIP(hipMemcpy(Ud, in, 3 * nSc * sizeof(cx<double>), hipMemcpyHostToDevice));
+      if (extended) {
+        PF_HIP(hipMemcpy(hist_slot(0), Ud, 3 * nSc * sizeof(cx<double>), hipMemcpyDeviceToDevice));
+        hist_head = 1;
+        hist_count = 1;
+      }
       warm_valid = true;
       return PF_OK;
     }
@@ -1005,6 +1215,26 @@ struct Engine final : EngineBase {
       case PF_K_XPASS_FWD6: xpass(false, 0, 6, Sc); return 6.0 * (nSy + nSc) * cs;
       case PF_K_XPASS_INV6: xpass(true, 0, 6, Sc); return 6.0 * (nSy + nSc) * cs;
       case PF_K_GREEN: {
+        if (extended) {
+          const int m = prm.elastic_anderson_depth;
+          GreenAAArgs<T> ga{};
+          ga.d = d;
+          ga.n = nSc;
+          for (int i = 0; i < 6; ++i) ga.h[i] = sc(i);
+          for (int i = 0; i < 3; ++i) {
+            ga.U[i] = Ud + (size_t)i * nSc;
+            ga.g[i] = Gd + (size_t)i * nSc;
+            ga.dinc[i] = dinc + (size_t)i * nSc;
+            for (int j = 0; j < AA_MAX; ++j) ga.dg[j][i] = dg_slot(j, i);
+          }
+          ga.nhist = m;
+          ga.slot = m > 0 ? 0 : -1;
+          ga.mode = 1;
+          ga.partials = partials;
+          for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
+          hipLaunchKernelGGL(kop_green_aa<T>, dim3(gk), dim3(NT), 0, stream, ga);
+          return (6.0 + 3.0 + 3.0 * m + (m > 0 ? 3.0 : 0.0)) * nSc * cs + 9.0 * nSc * 16.0;
+        }
         GreenArgs<T> ga{};
         ga.d = d;
         ga.n = nSc;

@@ -525,6 +525,220 @@ __global__ void __launch_bounds__(NT) kop_green_acc(GreenAccArgs<T> ga) {
   }
 }
 
+// ---------------------------------------------------------------- accelerated solver
+// Anderson-mixed fixed point on the FP64 displacement U with residual g = F(U) - U.
+// kop_green_aa evaluates one Green application:
+//   mode 0 (full map at U):          g  = G(h) - U
+//   mode 1 (linear map on increment d): g' = g + G(h) - d, dg = g' - g  (stored in slot `slot`)
+// and reduces (Hermitian-weighted, FP64):
+//   [0] |g'|^2, [1] |U + g'|^2, [2+i] <dg, v_i>, [2+AA_MAX+i] <g', v_i>
+// with v_i the dg history (slot `slot` replaced by the new dg).
+constexpr int AA_MAX = 4;
+constexpr int AA_NRED = 2 + 2 * AA_MAX;
+
+template <typename T>
+struct GreenAAArgs {
+  KDims d;
+  long n;
+  const cx<T>* h[6];
+  const cx<double>* U[3];
+  cx<double>* g[3];
+  const cx<T>* dinc[3];
+  cx<T>* dg[AA_MAX][3];
+  int nhist;        // valid dg slots after this update (including `slot`)
+  int slot;         // slot receiving the new dg (mode 1)
+  int mode;
+  double* partials;
+  double C[21];
+};
+
+__device__ __forceinline__ double wdot(const cx<double>* a, const cx<double>* b) {
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < 3; ++i) s += a[i].re * b[i].re + a[i].im * b[i].im;
+  return s;
+}
+
+template <typename T>
+__device__ __forceinline__ void green_apply_point(const double* C, double kx, double ky, double kz,
+                                                  const cx<double>* hh, cx<double>* u) {
+  double G[6];
+  u[0] = u[1] = u[2] = cx<double>{0.0, 0.0};
+  if (!green_tensor(C, kx, ky, kz, G)) return;
+  cx<double> t1 = {kx * hh[0].re + ky * hh[5].re + kz * hh[4].re, kx * hh[0].im + ky * hh[5].im + kz * hh[4].im};
+  cx<double> t2 = {kx * hh[5].re + ky * hh[1].re + kz * hh[3].re, kx * hh[5].im + ky * hh[1].im + kz * hh[3].im};
+  cx<double> t3 = {kx * hh[4].re + ky * hh[3].re + kz * hh[2].re, kx * hh[4].im + ky * hh[3].im + kz * hh[2].im};
+  const double gm[3][3] = {{G[0], G[1], G[2]}, {G[1], G[3], G[4]}, {G[2], G[4], G[5]}};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    cx<double> s = {gm[i][0] * t1.re + gm[i][1] * t2.re + gm[i][2] * t3.re,
+                    gm[i][0] * t1.im + gm[i][1] * t2.im + gm[i][2] * t3.im};
+    u[i] = cx<double>{s.im, -s.re};
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(NT) kop_green_aa(GreenAAArgs<T> a) {
+  double acc[AA_NRED];
+#pragma unroll
+  for (int r = 0; r < AA_NRED; ++r) acc[r] = 0.0;
+  for (long c = blockIdx.x * (long)NT + threadIdx.x; c < a.n; c += (long)gridDim.x * NT) {
+    double kx, ky, kz;
+    int iz;
+    kvec(a.d, c, kx, ky, kz, iz);
+    cx<double> hh[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) hh[i] = cx<double>{double(a.h[i][c].re), double(a.h[i][c].im)};
+    cx<double> Gh[3];
+    if (c != 0) green_apply_point<T>(a.C, kx, ky, kz, hh, Gh);
+    else Gh[0] = Gh[1] = Gh[2] = cx<double>{0.0, 0.0};
+    cx<double> U[3], gn[3], dgn[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      U[i] = a.U[i][c];
+      if (a.mode == 0) {
+        gn[i] = cx<double>{Gh[i].re - U[i].re, Gh[i].im - U[i].im};
+        dgn[i] = cx<double>{0.0, 0.0};
+      } else {
+        const cx<double> go = a.g[i][c];
+        const cx<double> di = {double(a.dinc[i][c].re), double(a.dinc[i][c].im)};
+        gn[i] = cx<double>{go.re + Gh[i].re - di.re, go.im + Gh[i].im - di.im};
+        dgn[i] = cx<double>{gn[i].re - go.re, gn[i].im - go.im};
+      }
+      a.g[i][c] = gn[i];
+    }
+    const double w = (iz == 0) ? 1.0 : 2.0;
+    cx<double> Fu[3] = {{U[0].re + gn[0].re, U[0].im + gn[0].im},
+                        {U[1].re + gn[1].re, U[1].im + gn[1].im},
+                        {U[2].re + gn[2].re, U[2].im + gn[2].im}};
+    acc[0] += w * wdot(gn, gn);
+    acc[1] += w * wdot(Fu, Fu);
+    if (a.mode == 1) {
+      cx<T> q[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) q[i] = cx<T>{T(dgn[i].re), T(dgn[i].im)};
+#pragma unroll
+      for (int j = 0; j < AA_MAX; ++j)
+        if (j == a.slot)
+#pragma unroll
+          for (int i = 0; i < 3; ++i) a.dg[j][i][c] = q[i];
+#pragma unroll
+      for (int j = 0; j < AA_MAX; ++j) {
+        if (j < a.nhist) {
+          cx<double> v[3];
+#pragma unroll
+          for (int i = 0; i < 3; ++i) {
+            cx<T> t = (j == a.slot) ? q[i] : a.dg[j][i][c];
+            v[i] = cx<double>{double(t.re), double(t.im)};
+          }
+          cx<double> dq[3] = {{double(q[0].re), double(q[0].im)},
+                              {double(q[1].re), double(q[1].im)},
+                              {double(q[2].re), double(q[2].im)}};
+          acc[2 + j] += w * wdot(dq, v);
+          acc[2 + AA_MAX + j] += w * wdot(gn, v);
+        }
+      }
+    }
+  }
+  int kinds[AA_NRED];
+#pragma unroll
+  for (int r = 0; r < AA_NRED; ++r) kinds[r] = RED_SUM;
+  block_reduce_store<AA_NRED>(acc, kinds, a.partials + (size_t)blockIdx.x * AA_NRED);
+}
+
+// d = g - sum_j gamma_j (du_j + dg_j);  U += d (the T-rounded d);  du[slot] = d;
+// eps = strain(d) for the next linear-map application.
+template <typename T>
+struct AAUpdateArgs {
+  KDims d;
+  long n;
+  const cx<double>* g[3];
+  cx<double>* U[3];
+  cx<T>* du[AA_MAX][3];
+  const cx<T>* dg[AA_MAX][3];
+  cx<T>* dinc[3];
+  cx<T>* eps[6];
+  double gamma[AA_MAX];
+  int nhist;
+  int slot;   // du slot receiving d (or -1 when no history is kept)
+};
+
+template <typename T>
+__global__ void __launch_bounds__(NT) kop_aa_update(AAUpdateArgs<T> a) {
+  for (long c = blockIdx.x * (long)NT + threadIdx.x; c < a.n; c += (long)gridDim.x * NT) {
+    double kx, ky, kz;
+    int iz;
+    kvec(a.d, c, kx, ky, kz, iz);
+    cx<double> dd[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      cx<double> v = a.g[i][c];
+#pragma unroll
+      for (int j = 0; j < AA_MAX; ++j) {
+        if (j < a.nhist) {
+          const cx<T> u1 = a.du[j][i][c];
+          const cx<T> g1 = a.dg[j][i][c];
+          v.re -= a.gamma[j] * (double(u1.re) + double(g1.re));
+          v.im -= a.gamma[j] * (double(u1.im) + double(g1.im));
+        }
+      }
+      const cx<T> q = {T(v.re), T(v.im)};
+      dd[i] = cx<double>{double(q.re), double(q.im)};
+      a.dinc[i][c] = q;
+#pragma unroll
+      for (int j = 0; j < AA_MAX; ++j)
+        if (j == a.slot) a.du[j][i][c] = q;
+      cx<double> uo = a.U[i][c];
+      a.U[i][c] = cx<double>{uo.re + dd[i].re, uo.im + dd[i].im};
+    }
+    strain_of<T>(kx, ky, kz, dd, a.eps, c);
+  }
+}
+
+// U <- U + g (the accepted result F(U)), eps = strain(U)
+template <typename T>
+__global__ void __launch_bounds__(NT) kop_aa_final(KDims d, long n, cx<double>* U0, cx<double>* U1,
+                                                   cx<double>* U2, const cx<double>* g0,
+                                                   const cx<double>* g1, const cx<double>* g2,
+                                                   cx<T>* e0, cx<T>* e1, cx<T>* e2, cx<T>* e3, cx<T>* e4,
+                                                   cx<T>* e5) {
+  cx<double>* U[3] = {U0, U1, U2};
+  const cx<double>* g[3] = {g0, g1, g2};
+  cx<T>* eps[6] = {e0, e1, e2, e3, e4, e5};
+  for (long c = blockIdx.x * (long)NT + threadIdx.x; c < n; c += (long)gridDim.x * NT) {
+    double kx, ky, kz;
+    int iz;
+    kvec(d, c, kx, ky, kz, iz);
+    cx<double> u[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const cx<double> a = U[i][c], b = g[i][c];
+      u[i] = cx<double>{a.re + b.re, a.im + b.im};
+      U[i][c] = u[i];
+    }
+    strain_of<T>(kx, ky, kz, u, eps, c);
+  }
+}
+
+// U = c1 H1 + c2 H2 + c3 H3 (polynomial extrapolation of the last converged states)
+template <int UNUSED = 0>
+__global__ void __launch_bounds__(NT) kop_extrapolate(long n3, cx<double>* U, const cx<double>* H1,
+                                                      const cx<double>* H2, const cx<double>* H3,
+                                                      double c1, double c2, double c3) {
+  for (long c = blockIdx.x * (long)NT + threadIdx.x; c < n3; c += (long)gridDim.x * NT) {
+    cx<double> v = {c1 * H1[c].re, c1 * H1[c].im};
+    if (H2) {
+      v.re += c2 * H2[c].re;
+      v.im += c2 * H2[c].im;
+    }
+    if (H3) {
+      v.re += c3 * H3[c].re;
+      v.im += c3 * H3[c].im;
+    }
+    U[c] = v;
+  }
+}
+
 template <typename T, typename UT = T>
 struct StrainArgs {
   KDims d;

@@ -131,6 +131,8 @@ extern "C" int pf_params_default(pf_params* p) {
   p->disable_Y_rhs_gamma_term = 0;
   p->Y_rhs_term_h_scale = 1.0;
   p->mode = 0;
+  p->elastic_warm_start_order = 1;
+  p->elastic_anderson_depth = 0;
   return PF_OK;
 }
 
@@ -195,6 +197,8 @@ extern "C" int pf_params_set(pf_params* p, const char* key, const char* value) {
                {"pf_zero_mode_max_iterations", &p->zero_mode_max_iterations},
                {"enable_Y_rhs_previous_time_level", &p->enable_Y_rhs_previous_time_level},
                {"disable_Y_rhs_gamma_term", &p->disable_Y_rhs_gamma_term},
+               {"elastic_warm_start_order", &p->elastic_warm_start_order},
+               {"elastic_anderson_depth", &p->elastic_anderson_depth},
                {nullptr, nullptr}};
   for (int i = 0; ikeys[i].name; ++i) {
     if (k == ikeys[i].name) {

@@ -153,6 +153,10 @@ static int validate(const pf_params* p) {
                     "accelerated elastic solver requires elastic_warm_start_enabled=1 and "
                     "elastic_residual_control_enabled=1");
   if (p->elastic_iter_max < 1) return pf::fail(PF_E_PARAMS, "elastic_iter_max must be >= 1");
+  if (p->elastic_warm_start_order < 1 || p->elastic_warm_start_order > 3)
+    return pf::fail(PF_E_PARAMS, "elastic_warm_start_order must be 1, 2 or 3");
+  if (p->elastic_anderson_depth < 0 || p->elastic_anderson_depth > 4)
+    return pf::fail(PF_E_PARAMS, "elastic_anderson_depth must be in [0, 4]");
   return PF_OK;
 }
 

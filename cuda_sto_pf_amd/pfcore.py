@@ -55,6 +55,7 @@ class PfParams(C.Structure):
         ("ic_vf_init_phi", C.c_double), ("ic_vf_target_phi", C.c_double),
         ("Vm_alpha_0_phys_m3mol", C.c_double),
         ("mode", C.c_int),
+        ("elastic_warm_start_order", C.c_int), ("elastic_anderson_depth", C.c_int),
     ]
 
 

@@ -77,6 +77,13 @@ typedef struct pf_params {
   double gamma_Jm2, lambda_sm_m, ic_phi_iface_w, ic_vf_init_phi, ic_vf_target_phi;
   double Vm_alpha_0_phys_m3mol;
   int mode;                            /* 0 dynamics, 1 minimize */
+  /* Extensions of the accelerated elastic solver (no reference counterpart;
+   * the defaults 1 and 0 reproduce main_cuda.cu:28760-29095): warm start from
+   * a constant/linear/quadratic extrapolation of the last 1/2/3 converged
+   * displacements, and Anderson mixing depth (0 = plain fixed point, <= 4).
+   * The map, the residual and the tolerance test are unchanged. */
+  int elastic_warm_start_order;
+  int elastic_anderson_depth;
 } pf_params;
 
 typedef struct pf_scalars {

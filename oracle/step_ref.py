@@ -99,6 +99,11 @@ class Params:
     enable_Y_rhs_previous_time_level: int = 0
     disable_Y_rhs_gamma_term: int = 0
     Y_rhs_term_h_scale: float = 1.0
+    # build extensions of the accelerated elastic solver (0/1 reproduce the reference):
+    # warm start from the last 1, 2 or 3 converged steps (constant, linear or
+    # quadratic extrapolation in time) and Anderson mixing depth.
+    elastic_warm_start_order: int = 1
+    elastic_anderson_depth: int = 0
 
     @property
     def T(self):
@@ -366,10 +371,88 @@ class Oracle:
         return out
 
     # ------------------------------------------------------------- elastic
+    def _wdot(self, a, b):
+        """Hermitian-weighted inner product of 3-component half spectra (the
+        norm of compute_elastic_displacement_residual_kernel, 6436-6467)."""
+        w = self.ks.herm_w
+        return math.fsum(sum(w * (a[i] * np.conj(b[i])).real for i in range(3)).ravel())
+
+    def elastic_solve_extended(self):
+        """Accelerated fixed point with the build's extensions: the same map
+        F and the same residual |F(u)-u|/|F(u)| as main_cuda.cu:28887-29046,
+        started from a polynomial extrapolation of the last converged steps
+        and updated by Anderson mixing of depth m (m = 0: plain iteration).
+        The result on convergence is F(u_k), as in the reference."""
+        p, ks, shape = self.p, self.ks, self.shape
+        m = int(p.elastic_anderson_depth)
+        hist = getattr(self, "u_hist", [])
+        if not self.warm_valid or not hist:
+            e0k = [fwd(c, ks) for c in eigenstrain(p, self.phi, self.xB)]
+            U = green_apply(p.C, ks, voigt_apply(p.C, e0k))
+        else:
+            order = min(int(p.elastic_warm_start_order), len(hist))
+            coef = {1: (1.0,), 2: (2.0, -1.0), 3: (3.0, -3.0, 1.0)}[order]
+            U = [sum(coef[j] * hist[-1 - j][i] for j in range(order)) for i in range(3)]
+
+        def apply_map(v, increment):
+            eps = [inv(c, shape) for c in strain_from_disp(ks, v)]
+            hh = hij_field(p, eps, self.phi, self.xB, increment=increment)
+            return green_apply(p.C, ks, [fwd(c, ks) for c in hh])
+
+        F = apply_map(U, False)
+        g = [F[i] - U[i] for i in range(3)]
+        du, dg = [], []
+        it = 1
+        converged = False
+        while True:
+            Fu = [U[i] + g[i] for i in range(3)]
+            res = math.sqrt(self._wdot(g, g) / max(self._wdot(Fu, Fu), p.elastic_residual_absolute_floor))
+            if not math.isfinite(res):
+                raise ElasticNonConvergence("elastic residual is non-finite")
+            if it >= p.elastic_iter_min and res <= p.elastic_residual_tolerance:
+                converged = True
+                break
+            if it >= p.elastic_iter_max:
+                break
+            if dg:
+                H = np.array([[self._wdot(a, b) for b in dg] for a in dg])
+                bv = np.array([self._wdot(g, a) for a in dg])
+                H = H + (1e-12 * np.trace(H) / len(dg)) * np.eye(len(dg))
+                gam = np.linalg.solve(H, bv)
+            else:
+                gam = np.zeros(0)
+            d = [g[i] - sum(gam[j] * (du[j][i] + dg[j][i]) for j in range(len(gam))) for i in range(3)]
+            U = [U[i] + d[i] for i in range(3)]
+            Ad = apply_map(d, True)
+            gn = [g[i] + Ad[i] - d[i] for i in range(3)]
+            if m > 0:
+                du.append(d)
+                dg.append([gn[i] - g[i] for i in range(3)])
+                if len(du) > m:
+                    du.pop(0)
+                    dg.pop(0)
+            g = gn
+            it += 1
+        U = [U[i] + g[i] for i in range(3)]
+        self.last_elastic_iterations = it
+        self.last_elastic_residual = res
+        self.u_k = U
+        self.warm_valid = True
+        self.u_hist = (hist + [[c.copy() for c in U]])[-3:]
+        if not converged and p.elastic_fail_on_nonconvergence:
+            raise ElasticNonConvergence(
+                f"elastic solver did not converge: iterations={it} residual={res:.17e}")
+        ek = strain_from_disp(ks, U)
+        eps = [inv(c, shape) + p.E0[i] for i, c in enumerate(ek)]
+        self.eps = eps
+        self.sigma = stress_field(p, eps, self.phi, self.xB)
+
     def elastic_solve(self, increment_mode=False):
         """main_cuda.cu:28760-29166."""
         p, ks, shape = self.p, self.ks, self.shape
         accel = bool(p.elastic_warm_start_enabled or p.elastic_residual_control_enabled)
+        if accel and (p.elastic_anderson_depth > 0 or p.elastic_warm_start_order > 1):
+            return self.elastic_solve_extended()
         warm = accel and self.warm_valid
         if not warm:
             e0 = eigenstrain(p, self.phi, self.xB)

@@ -94,3 +94,54 @@ def test_mixed_step_accuracy(pf, name, pd, phi, xB, cap):
     sc = sim.scalars()
     if pd["zero_mode"]:
         assert abs(sc["mass_xBtot"] - o.target_mass) <= 1e-12 * max(abs(o.target_mass), 1.0)
+
+
+def _ext(pd, order=3, depth=3):
+    q = dict(pd)
+    q.update(elastic_warm_start_order=order, elastic_anderson_depth=depth)
+    return q
+
+
+EXT_CASES = [("cfg2_16_ext",) + (lambda c: (_ext(c[0]), c[1], c[2], c[3]["xB_max_safe"]))(config(2, 16)),
+             ("rect_ext_m2",) + (lambda c: (_ext(c[0], 2, 2), c[1], c[2], c[3]))(_rect_case())]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,pd,phi,xB,cap", EXT_CASES, ids=[c[0] for c in EXT_CASES])
+def test_fp64_extended_solver_parity(pf, name, pd, phi, xB, cap):
+    """Extrapolated warm start + Anderson mixing: GPU FP64 against the oracle
+    running the same scheme, 1e-10 and identical iteration counts."""
+    o = _oracle(pd, phi, xB, cap)
+    sim = _gpu(pf, pd, phi, xB, cap, pf.PF_FP64)
+    for s in range(6):
+        o.step()
+        sim.step()
+        sc = sim.scalars()
+        dphi, dY, dx = _compare(o, sim, pf)
+        print(f"{name} step {s + 1}: dphi={dphi:.2e} dY={dY:.2e} dxB={dx:.2e} "
+              f"iters gpu={sc['last_elastic_iterations']} cpu={o.last_elastic_iterations}", flush=True)
+        assert sc["last_elastic_iterations"] == o.last_elastic_iterations
+        assert dphi < FP64_TOL and dY < FP64_TOL and dx < FP64_TOL
+
+
+@pytest.mark.gpu
+def test_mixed_extended_against_fp64_reference(pf):
+    """Mixed precision with the accelerated solver against the plain FP64
+    oracle (the reference iteration): x_B within the 1e-5 gate after 8 steps,
+    mass conserved to rounding, fewer elastic updates."""
+    pd, phi, xB, meta = config(2, 16)
+    cap = meta["xB_max_safe"]
+    o = _oracle(pd, phi, xB, cap)
+    sim = _gpu(pf, _ext(pd), phi, xB, cap, pf.PF_MIXED)
+    it_o = it_g = 0
+    for s in range(8):
+        o.step()
+        sim.step()
+        it_o += o.last_elastic_iterations
+        it_g += sim.scalars()["last_elastic_iterations"]
+    dphi, dY, dx = _compare(o, sim, pf)
+    print(f"mixed+ext vs fp64 plain: dxB={dx:.2e} dphi={dphi:.2e} updates {it_g} vs {it_o}", flush=True)
+    assert dx < MIXED_XB_TOL
+    assert it_g < it_o
+    sc = sim.scalars()
+    assert abs(sc["mass_xBtot"] - o.target_mass) <= 1e-12 * max(abs(o.target_mass), 1.0)
