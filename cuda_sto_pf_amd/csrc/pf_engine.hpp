@@ -12,6 +12,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -21,6 +22,7 @@
 #include "pf_engine_api.hpp"
 #include "pf_error.hpp"
 #include "pf_ops.hpp"
+#include "pf_regfft.hpp"
 
 #define PF_HIP(call)                                                                  \
   do {                                                                                \
@@ -239,11 +241,23 @@ constexpr int zpass_rows_for() {
   return zpass_rows<T, N, (NCM > 0 ? NCM : 1)>();
 }
 
+// z-pass implementation: register-resident transform for power-of-two
+// lengths up to 256 (default), or the LDS-staged kernel (PF_ZPASS=lds).
+inline bool zpass_use_reg() {
+  static const bool use = [] {
+    const char* e = std::getenv("PF_ZPASS");
+    return !(e && std::strcmp(e, "lds") == 0);
+  }();
+  return use;
+}
+
 template <typename T, int NCI, int NCO>
 int zpass_grid(int N, int rows) {
   switch (N) {
-#define PF_CASE_ZG(v) \
-  case v:             \
+#define PF_CASE_ZG(v)                                                                  \
+  case v:                                                                              \
+    if constexpr (RegPlan<v>::ok)                                                      \
+      if (zpass_use_reg()) return (rows + 64 / RegPlan<v>::L - 1) / (64 / RegPlan<v>::L); \
     return (rows + zpass_rows_for<T, v, NCI, NCO>() - 1) / zpass_rows_for<T, v, NCI, NCO>();
     PF_SIZES(PF_CASE_ZG)
 #undef PF_CASE_ZG
@@ -256,9 +270,15 @@ template <typename T, int NCI, int NCO, class Op>
 void launch_zpass(int N, hipStream_t s, const ZPassArgs<T>& za, const typename Op::Args& oa) {
   const int grid = zpass_grid<T, NCI, NCO>(N, za.rows);
   switch (N) {
-#define PF_CASE_Z(v)                                                                              \
-  case v:                                                                                         \
-    hipLaunchKernelGGL((zpass_kernel<T, v, NCI, NCO, Op>), dim3(grid), dim3(NT), 0, s, za, oa); \
+#define PF_CASE_Z(v)                                                                                \
+  case v:                                                                                           \
+    if constexpr (RegPlan<v>::ok) {                                                                 \
+      if (zpass_use_reg()) {                                                                        \
+        hipLaunchKernelGGL((zpass_reg_kernel<T, v, NCI, NCO, Op>), dim3(grid), dim3(64), 0, s, za, oa); \
+        break;                                                                                      \
+      }                                                                                             \
+    }                                                                                               \
+    hipLaunchKernelGGL((zpass_kernel<T, v, NCI, NCO, Op>), dim3(grid), dim3(NT), 0, s, za, oa);   \
     break;
     PF_SIZES(PF_CASE_Z)
 #undef PF_CASE_Z

@@ -1,0 +1,82 @@
+"""The main_pf binary: the reference main_cuda command line and Results/ layout
+(main_cuda.cu:22457-22771, 25270-25660).  CPU tests cover the fail-closed
+paths (no device call); the GPU test runs a short raw_fields case."""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from cuda_sto_pf_amd import params as pfparams
+from cuda_sto_pf_amd.configs import config
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CLI = os.path.join(ROOT, "cuda_sto_pf_amd", "bin", "main_pf")
+
+
+def _bundle(tmp_path, n=16, steps=4, zero_mode=True):
+    pd, phi, xB, meta = config(2, n)
+    prm = pfparams.write_params_file(str(tmp_path / "run.params"), pd)
+    phi.astype(np.float64).tofile(tmp_path / "phi_init.raw")
+    xB.astype(np.float64).tofile(tmp_path / "xB_init.raw")
+    (tmp_path / "init_meta.json").write_text(json.dumps(
+        {"Nx": n, "Ny": n, "Nz": n, "dx_nm": 2.0, "interface_width_nm": 4.0, "dtype": "float64",
+         "order": "C", "xB_max_safe": meta["xB_max_safe"]}))
+    args = [CLI, str(n), str(n), str(n), "0.01", str(steps), "2", "1", "1", "--pf-param-file", prm,
+            "--init-mode", "raw_fields", "--init-phi-raw", str(tmp_path / "phi_init.raw"),
+            "--init-xB-raw", str(tmp_path / "xB_init.raw"), "--init-meta", str(tmp_path / "init_meta.json")]
+    if zero_mode:
+        args += ["--pf-zero-mode", "PF_CONSERVED_Y_ZERO_MODE_V1"]
+    return args, pd, phi, xB, meta
+
+
+def _run(args, tmp_path, extra_env=None):
+    env = dict(os.environ, CUDA_STO_RESULTS_ROOT=str(tmp_path / "Results"))
+    env.update(extra_env or {})
+    return subprocess.run(args, capture_output=True, text=True, env=env, timeout=300)
+
+
+def test_missing_param_file_is_fatal(tmp_path):
+    r = _run([CLI, "16", "16", "16", "0.01", "2", "1", "1", "1"], tmp_path)
+    assert r.returncode == 2 and "[fatal]" in r.stderr
+
+
+def test_incomplete_param_file_is_fatal(tmp_path):
+    args, *_ = _bundle(tmp_path)
+    prm = args[args.index("--pf-param-file") + 1]
+    lines = [ln for ln in open(prm) if not ln.startswith("S_p_11")]
+    open(prm, "w").writelines(lines)
+    r = _run(args, tmp_path)
+    assert r.returncode == 2 and "S_p_" in r.stderr
+
+
+def test_minimize_mode_is_refused(tmp_path):
+    args, *_ = _bundle(tmp_path)
+    r = _run(args + ["--mode=minimize"], tmp_path)
+    assert r.returncode == 2 and "[fatal]" in r.stderr
+
+
+@pytest.mark.gpu
+def test_cli_raw_fields_run_matches_library(pf, tmp_path):
+    args, pd, phi, xB, meta = _bundle(tmp_path, steps=4)
+    r = _run(args, tmp_path, {"CUDA_STO_SUPPRESS_VTK_OUTPUT": "0"})
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0, r.stderr
+    runs = os.listdir(tmp_path / "Results")
+    assert len(runs) == 1 and runs[0].startswith("chel_T380_cuda_16x16x16_dt0.01_steps4_xB")
+    case = tmp_path / "Results" / runs[0] / "raw_fields_relax"
+    for f in ("pf_input.params", "vf_precip_vs_time_raw_fields_relax.csv", "elastic_solver_trace.csv",
+              "pf_zero_mode_trace.csv", "performance_summary.csv", "performance_summary.json",
+              "phi_2.vtk", "xB_4.vtk", "phi_final.vtk"):
+        assert (case / f).exists(), f
+    csv = (case / "vf_precip_vs_time_raw_fields_relax.csv").read_text().splitlines()
+    assert csv[0] == "step,t_code,t_real_s,vf_precip,R_avg" and len(csv) == 1 + 5
+    # the VTK holds the same phi as the library run to %.5f
+    sim = pf.Simulation(pd, pf.PF_FP64)
+    sim.set_fields(phi, xB, xB_cap=meta["xB_max_safe"])
+    sim.step(4)
+    ref = sim.field(pf.PF_PHI)
+    txt = (case / "phi_final.vtk").read_text().split("LOOKUP_TABLE default\n")[1].split()
+    vals = np.array([float(v) for v in txt]).reshape(16, 16, 16)   # z, y, x order
+    assert np.abs(vals.transpose(2, 1, 0) - ref).max() <= 6e-6
