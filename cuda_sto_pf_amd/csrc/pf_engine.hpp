@@ -222,7 +222,9 @@ struct OpLoadReal {
   };
   static constexpr int NRED = 0;
   static constexpr int kinds[1] = {0};
-  __device__ static void apply(const Args& a, size_t g, const T*, T* out, double*) { out[0] = T(a.f[g]); }
+  static constexpr int NPRE = 1;
+  __device__ static void load(const Args& a, size_t g, double* pre) { pre[0] = a.f[g]; }
+  __device__ static void apply(const Args&, size_t, const double* pre, const T*, T* out, double*) { out[0] = T(pre[0]); }
 };
 template <typename T>
 struct OpStoreReal {
@@ -231,7 +233,9 @@ struct OpStoreReal {
   };
   static constexpr int NRED = 0;
   static constexpr int kinds[1] = {0};
-  __device__ static void apply(const Args& a, size_t g, const T* vin, T*, double*) { a.f[g] = double(vin[0]); }
+  static constexpr int NPRE = 0;
+  __device__ static void load(const Args&, size_t, double*) {}
+  __device__ static void apply(const Args& a, size_t g, const double*, const T* vin, T*, double*) { a.f[g] = double(vin[0]); }
 };
 
 // ------------------------------------------------------------ dispatch
@@ -241,12 +245,13 @@ constexpr int zpass_rows_for() {
   return zpass_rows<T, N, (NCM > 0 ? NCM : 1)>();
 }
 
-// z-pass implementation: register-resident transform for power-of-two
-// lengths up to 256 (default), or the LDS-staged kernel (PF_ZPASS=lds).
+// z-pass implementation: the LDS-staged kernel (default), or the
+// register-resident transform for power-of-two lengths up to 256 (PF_ZPASS=reg;
+// measured slower at 256^3: its 256-VGPR footprint allows one wave per SIMD).
 inline bool zpass_use_reg() {
   static const bool use = [] {
     const char* e = std::getenv("PF_ZPASS");
-    return !(e && std::strcmp(e, "lds") == 0);
+    return e && std::strcmp(e, "reg") == 0;
   }();
   return use;
 }

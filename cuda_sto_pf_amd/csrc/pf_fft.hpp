@@ -37,17 +37,39 @@ constexpr int smallest_prime_factor(int n) {
     if (n % p == 0) return p;
   return n;
 }
+// Largest power-of-two radix up to 16 first (fewer LDS round trips), then
+// 3, 5 and any other prime.
 constexpr int pick_radix(int n) {
-  return (n % 4 == 0) ? 4 : smallest_prime_factor(n);
+  return (n % 16 == 0) ? 16 : (n % 8 == 0) ? 8 : (n % 4 == 0) ? 4 : smallest_prime_factor(n);
 }
+constexpr bool is_pow2_radix(int r) { return r == 2 || r == 4 || r == 8 || r == 16; }
 constexpr bool plan_has_big_prime(int n) {
   while (n > 1) {
     int r = pick_radix(n);
-    if (r > 5) return true;
+    if (r > 5 && !is_pow2_radix(r)) return true;
     n /= r;
   }
   return false;
 }
+
+// cos(2 pi k / 32), k = 0..8, as literals (constant-folded after unrolling)
+__host__ __device__ constexpr double cos32_quarter(int k) {
+  return k == 0 ? 1.0
+       : k == 1 ? 0.980785280403230449126182236134
+       : k == 2 ? 0.923879532511286756128183189397
+       : k == 3 ? 0.831469612302545237078788377618
+       : k == 4 ? 0.707106781186547524400844362105
+       : k == 5 ? 0.555570233019602224742830813949
+       : k == 6 ? 0.382683432365089771728459984030
+       : k == 7 ? 0.195090322016128267848284868477
+       : 0.0;
+}
+__host__ __device__ constexpr double cos32(int m) {
+  m = ((m % 32) + 32) % 32;
+  return m <= 8 ? cos32_quarter(m)
+                : (m <= 16 ? -cos32_quarter(16 - m) : (m <= 24 ? -cos32_quarter(m - 16) : cos32_quarter(32 - m)));
+}
+__host__ __device__ constexpr double sin32(int m) { return cos32(m - 8); }
 
 // ----------------------------------------------------------- small DFTs
 template <typename T, int R, bool INV>
@@ -117,13 +139,76 @@ struct Dft<T, 5, INV> {
   }
 };
 
+// W_E^m = exp(-2 pi i m / E) for E | 32 (conjugated for the inverse)
+template <typename T, int E, bool INV>
+__device__ __forceinline__ cx<T> wconst(int m) {
+  const int q = (m * (32 / E)) % 32;
+  return cx<T>{T(cos32(q)), INV ? T(sin32(q)) : T(-sin32(q))};
+}
+
+// In-register E-point DFT (E | 32), natural order in and out: Stockham stages
+// of radix 4/2 with compile-time twiddles.
+template <typename T, int E, int NS, bool INV>
+__device__ __forceinline__ void dft_reg_stages(cx<T>* a) {
+  if constexpr (NS < E) {
+    constexpr int R = ((E / NS) % 4 == 0) ? 4 : 2;
+    constexpr int NB = E / R;
+    cx<T> b[E];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+      cx<T> v[R];
+      const int jm = j % NS;
+#pragma unroll
+      for (int s = 0; s < R; ++s) {
+        v[s] = a[j + s * NB];
+        if (NS > 1 && s > 0 && jm != 0) v[s] = cmul(v[s], wconst<T, E, INV>(jm * s * (E / (NS * R))));
+      }
+      Dft<T, R, INV>::run(v);
+      const int d = (j / NS) * NS * R + jm;
+#pragma unroll
+      for (int s = 0; s < R; ++s) b[d + s * NS] = v[s];
+    }
+#pragma unroll
+    for (int i = 0; i < E; ++i) a[i] = b[i];
+    dft_reg_stages<T, E, NS * R, INV>(a);
+  }
+}
+
+template <typename T, int E, bool INV>
+__device__ __forceinline__ void dft_reg(cx<T>* a) {
+  if constexpr (E > 1) dft_reg_stages<T, E, 1, INV>(a);
+}
+
+template <typename T, bool INV>
+struct Dft<T, 8, INV> {
+  static __device__ __forceinline__ void run(cx<T>* v) { dft_reg<T, 8, INV>(v); }
+};
+template <typename T, bool INV>
+struct Dft<T, 16, INV> {
+  static __device__ __forceinline__ void run(cx<T>* v) { dft_reg<T, 16, INV>(v); }
+};
+
+// ----------------------------------------------------------- LDS layout
+// Padded to keep the strided Stockham writes free of bank conflicts: one
+// extra element after every 16 along the transform axis, and (column-major)
+// an odd row pitch.
+template <int N, int BATCH, bool COLMAJ>
+struct Lds {
+  static constexpr int NPAD = N + N / 16;
+  static constexpr int PITCH = BATCH + 1;
+  static constexpr int size = COLMAJ ? NPAD * PITCH : BATCH * NPAD;
+  static __device__ __forceinline__ int idx(int b, int j) {
+    return COLMAJ ? (j + (j >> 4)) * PITCH + b : b * NPAD + j + (j >> 4);
+  }
+};
+
 // ----------------------------------------------------------- one stage
 template <int N, int BATCH, bool COLMAJ>
 __device__ __forceinline__ int lds_index(int b, int j) {
-  return COLMAJ ? j * BATCH + b : b * N + j;
+  return Lds<N, BATCH, COLMAJ>::idx(b, j);
 }
 
-// In-place Stockham stage for radix R <= 5 (butterflies staged in registers).
+// In-place Stockham stage for radix 2/3/4/5/8/16 (butterflies staged in registers).
 template <typename T, int N, int R, int NS, int BATCH, bool COLMAJ, bool INV, int NT>
 __device__ __forceinline__ void stage_inplace(cx<T>* buf, const cx<T>* tw) {
   constexpr int NB = N / R;
@@ -145,14 +230,17 @@ __device__ __forceinline__ void stage_inplace(cx<T>* buf, const cx<T>* tw) {
       }
       const int jm = j % NS;
 #pragma unroll
-      for (int s = 0; s < R; ++s) {
-        cx<T> a = buf[lds_index<N, BATCH, COLMAJ>(b, j + s * NB)];
-        if (NS > 1 && s > 0) {
-          cx<T> w = tw[jm * s * TSTEP];
-          if (INV) w.im = -w.im;
-          a = cmul(a, w);
+      for (int s = 0; s < R; ++s) v[q][s] = buf[lds_index<N, BATCH, COLMAJ>(b, j + s * NB)];
+      if (NS > 1) {
+        // twiddles W^(jm s TSTEP): one table read, powers by multiplication
+        cx<T> w1 = tw[jm * TSTEP];
+        if (INV) w1.im = -w1.im;
+        cx<T> w = w1;
+#pragma unroll
+        for (int s = 1; s < R; ++s) {
+          v[q][s] = cmul(v[q][s], w);
+          if (s + 1 < R) w = cmul(w, w1);
         }
-        v[q][s] = a;
       }
       Dft<T, R, INV>::run(v[q]);
     }
@@ -227,7 +315,7 @@ __device__ __forceinline__ cx<T>* fft_stages(cx<T>* buf, cx<T>* alt, const cx<T>
     return buf;
   } else {
     constexpr int R = pick_radix(N / NS);
-    if constexpr (R <= 5) {
+    if constexpr (R <= 5 || is_pow2_radix(R)) {
       stage_inplace<T, N, R, NS, BATCH, COLMAJ, INV, NT>(buf, tw);
       return fft_stages<T, N, NS * R, BATCH, COLMAJ, INV, NT>(buf, alt, tw);
     } else {

@@ -29,10 +29,15 @@ struct OpElasticCold {
   };
   static constexpr int NRED = 0;
   static constexpr int kinds[1] = {0};
-  __device__ static void apply(const Args& a, size_t g, const T*, T* out, double*) {
-    double h = h_of_phi(a.phi[g]);
+  static constexpr int NPRE = 2;
+  __device__ static void load(const Args& a, size_t g, double* pre) {
+    pre[0] = a.phi[g];
+    pre[1] = a.xB[g];
+  }
+  __device__ static void apply(const Args& a, size_t g, const double* pre, const T*, T* out, double*) {
+    double h = h_of_phi(pre[0]);
     double e0[6], s[6];
-    eigenstrain(a.P, h, clamp01(a.xB[g]), e0);
+    eigenstrain(a.P, h, clamp01(pre[1]), e0);
     voigt_apply<double>(a.P.C, e0, s);
 #pragma unroll
     for (int c = 0; c < 6; ++c) out[c] = T(s[c]);
@@ -53,16 +58,21 @@ struct OpHij {
   };
   static constexpr int NRED = 0;
   static constexpr int kinds[1] = {0};
-  __device__ static void apply(const Args& a, size_t g, const T* vin, T* out, double*) {
+  static constexpr int NPRE = 2;
+  __device__ static void load(const Args& a, size_t g, double* pre) {
+    pre[0] = a.phi[g];
+    pre[1] = a.increment ? 0.0 : a.xB[g];
+  }
+  __device__ static void apply(const Args& a, size_t g, const double* pre, const T* vin, T* out, double*) {
     T eps[6];
 #pragma unroll
     for (int c = 0; c < 6; ++c) eps[c] = vin[c] * a.invN;
-    const double h = h_of_phi(a.phi[g]);
+    const double h = h_of_phi(pre[0]);
     if (a.increment) {
       hij_increment_point<T>(a.P, T(h), eps, out);
     } else {
       double e0[6];
-      eigenstrain(a.P, h, clamp01(a.xB[g]), e0);
+      eigenstrain(a.P, h, clamp01(pre[1]), e0);
       T e0t[6];
 #pragma unroll
       for (int c = 0; c < 6; ++c) e0t[c] = T(e0[c]);
@@ -111,10 +121,15 @@ struct OpPhiRhs {
   };
   static constexpr int NRED = 3;
   static constexpr int kinds[3] = {RED_SUM, RED_SUM, RED_SUM};
-  __device__ static void apply(const Args& a, size_t g, const T* vin, T* out, double* red) {
+  static constexpr int NPRE = 2;
+  __device__ static void load(const Args& a, size_t g, double* pre) {
+    pre[0] = a.phi[g];
+    pre[1] = a.xB[g];
+  }
+  __device__ static void apply(const Args& a, size_t g, const double* pre, const T* vin, T* out, double* red) {
     const PhysParams& P = a.P;
-    const double phi = a.phi[g];
-    const double xraw = a.xB[g];
+    const double phi = pre[0];
+    const double xraw = pre[1];
     const double xB = clamp01(xraw);
     double hp, h;
     double f = phi_rhs_chem(P, phi, xB, hp, h);
@@ -174,14 +189,20 @@ struct OpPhiFinish {
   };
   static constexpr int NRED = 1;
   static constexpr int kinds[1] = {RED_SUM};
-  __device__ static void apply(const Args& a, size_t g, const T* vin, T* out, double* red) {
+  static constexpr int NPRE = 3;
+  __device__ static void load(const Args& a, size_t g, double* pre) {
+    pre[0] = a.phiY_is_old ? a.phi_old[g] : 0.0;
+    pre[1] = a.Y[g];
+    pre[2] = a.P.elastic ? double(a.trs[g]) : 0.0;
+  }
+  __device__ static void apply(const Args& a, size_t g, const double* pre, const T* vin, T* out, double* red) {
     const PhysParams& P = a.P;
     double v = double(vin[0]) * double(a.invN);
     if (v < -1e-6) v = -1e-6;
     if (v > 1.0 + 1e-6) v = 1.0 + 1e-6;
     a.phi_new[g] = v;
-    const double phiY = a.phiY_is_old ? a.phi_old[g] : v;
-    const double Y = a.Y[g];
+    const double phiY = a.phiY_is_old ? pre[0] : v;
+    const double Y = pre[1];
     const double xB = sigmoid_from_logit(Y, P.Y_clip, P.xB_eps);
     a.xB[g] = xB;
     const double h = h_of_phi(phiY);
@@ -190,7 +211,7 @@ struct OpPhiFinish {
     const double c_bulk = c_xB_phi(P, xB, h);
     const double mu_total = mu_tot_mix(muA, muB, xB, P.mu0, h);
     double mu = c_bulk * (muB - muA - c_bulk * mu_total * P.dVm);
-    if (P.elastic) mu += -P.eps_iso * double(a.trs[g]);
+    if (P.elastic) mu += -P.eps_iso * pre[2];
     out[0] = T(mu);
     double x = sigmoid_unclipped(Y);
     if (x < 1e-12) x = 1e-12;
@@ -221,34 +242,43 @@ struct OpFlux {
   };
   static constexpr int NRED = 0;
   static constexpr int kinds[1] = {0};
-  __device__ static void apply(const Args& a, size_t g, const T* vin, T* out, double*) {
+  static constexpr int NPRE = 6;
+  __device__ static void load(const Args& a, size_t g, double* pre) {
+    pre[0] = a.phiY[g];
+    pre[1] = a.xBp[g];
+    pre[2] = a.phi_n[g];
+    pre[3] = a.phi_np1[g];
+    pre[4] = a.Y[g];
+    pre[5] = a.dYdt[g];
+  }
+  __device__ static void apply(const Args& a, size_t g, const double* pre, const T* vin, T* out, double*) {
     const PhysParams& P = a.P;
-    const double hY = h_of_phi(a.phiY[g]);
+    const double hY = h_of_phi(pre[0]);
     const double Dm = D_mix(P, hY);
-    const double G = gamma_thermo(P, a.xBp[g], hY);
+    const double G = gamma_thermo(P, pre[1], hY);
     const double Meff = stabilized_meff(Dm, G);
 #pragma unroll
     for (int c = 0; c < 3; ++c) out[c] = T(Meff * (double(vin[c]) * double(a.invN)));
-    const double Y = a.Y[g];
+    const double Y = pre[4];
     double x = sigmoid_unclipped(Y);
     if (x < 1e-12) x = 1e-12;
     if (x > 1.0 - 1e-12) x = 1.0 - 1e-12;
     const double lg = x * (1.0 - x);
-    const double pn = a.phi_n[g];
-    const double pn1 = a.phi_np1[g];
+    const double pn = pre[2];
+    const double pn1 = pre[3];
     double s;
     if (a.zero_mode) {
       const double h_n = h_of_phi(pn);
       const double term_h = h_prime_of_phi(pn) * ((pn1 - pn) / P.dt) * (P.v_B - x);
       const double gamma_local = (1.0 - h_n) * lg - 1.0;
-      s = term_h + gamma_local * a.dYdt[g];
+      s = term_h + gamma_local * pre[5];
     } else {
       const double h_new = h_of_phi(pn1);
       const double h_old = h_of_phi(pn);
       const double term_h = a.term_h_scale * ((h_new - h_old) / P.dt) * (P.v_B - x);
       const double hx = a.prev_level ? h_old : h_new;
       const double gamma_local = (1.0 - hx) * lg - 1.0;
-      const double term_gamma = a.no_gamma ? 0.0 : gamma_local * a.dYdt[g];
+      const double term_gamma = a.no_gamma ? 0.0 : gamma_local * pre[5];
       s = term_h + term_gamma;
     }
     out[3] = T(Y);
@@ -266,7 +296,9 @@ struct OpYStar {
   };
   static constexpr int NRED = 2;
   static constexpr int kinds[2] = {RED_MIN, RED_MAX};
-  __device__ static void apply(const Args& a, size_t g, const T* vin, T*, double* red) {
+  static constexpr int NPRE = 0;
+  __device__ static void load(const Args&, size_t, double*) {}
+  __device__ static void apply(const Args& a, size_t g, const double*, const T* vin, T*, double* red) {
     const double y = double(vin[0]) * double(a.invN);
     a.Ystar[g] = y;
     red[0] = fmin(red[0], y);
@@ -289,11 +321,13 @@ struct OpYFinal {
   };
   static constexpr int NRED = 0;
   static constexpr int kinds[1] = {0};
-  __device__ static void apply(const Args& a, size_t g, const T* vin, T*, double*) {
+  static constexpr int NPRE = 1;
+  __device__ static void load(const Args& a, size_t g, double* pre) { pre[0] = a.Y[g]; }
+  __device__ static void apply(const Args& a, size_t g, const double* pre, const T* vin, T*, double*) {
     double y = double(vin[0]) * double(a.invN);
     if (y > a.Y_upper_cap) y = a.Y_upper_cap;
     if (y < -a.P.Y_clip) y = -a.P.Y_clip;
-    const double yn = a.Y[g];
+    const double yn = pre[0];
     a.Y[g] = y;
     a.xB[g] = sigmoid_from_logit(y, a.P.Y_clip, a.P.xB_eps);
     a.dYdt[g] = (y - yn) / a.P.dt;

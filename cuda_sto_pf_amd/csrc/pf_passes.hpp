@@ -22,18 +22,20 @@ namespace pf {
 constexpr int NT = 256;                 // threads per workgroup (4 wave64)
 constexpr int LDS_BUDGET = 48 * 1024;   // bytes of FFT data per workgroup
 
+constexpr int ZPASS_LDS_BUDGET = 32 * 1024;   // smaller tiles: more resident workgroups
+
 template <typename T, int N, int NC>
 constexpr int zpass_rows() {
-  constexpr int per_pair = NC * N * (int)sizeof(cx<T>) * (plan_has_big_prime(N) ? 2 : 1);
-  constexpr int pairs = LDS_BUDGET / per_pair;
+  constexpr int per_pair = NC * (N + N / 16) * (int)sizeof(cx<T>) * (plan_has_big_prime(N) ? 2 : 1);
+  constexpr int pairs = ZPASS_LDS_BUDGET / per_pair;
   constexpr int p = pairs < 1 ? 1 : (pairs > 32 ? 32 : pairs);
   return 2 * p;
 }
 
 template <typename T, int N>
 constexpr int cpass_cols() {
-  constexpr int per_col = N * (int)sizeof(cx<T>) * (plan_has_big_prime(N) ? 2 : 1);
-  constexpr int c = LDS_BUDGET / per_col;
+  constexpr int per_col = (N + N / 16) * (int)sizeof(cx<T>) * (plan_has_big_prime(N) ? 2 : 1);
+  constexpr int c = (LDS_BUDGET / per_col) - 1;
   return c >= 64 ? 64 : c >= 32 ? 32 : c >= 16 ? 16 : c >= 8 ? 8 : c >= 4 ? 4 : c >= 2 ? 2 : 1;
 }
 
@@ -97,42 +99,55 @@ __global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op:
   constexpr int RT = zpass_rows<T, N, NCB>();
   constexpr int NP = RT / 2;
   constexpr bool BIG = plan_has_big_prime(N);
-  __shared__ cx<T> buf[NCB * NP * N];
-  __shared__ cx<T> alt[BIG ? NCB * NP * N : 1];
+  using LY = Lds<N, NCB * NP, false>;   // row-major: layout independent of the batch
+  __shared__ cx<T> buf[LY::size];
+  __shared__ cx<T> alt[BIG ? LY::size : 1];
   __shared__ cx<T> tw[N];
 
   const int row0 = blockIdx.x * RT;
   const int Az = za.Az;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int i = threadIdx.x; i < N; i += NT) tw[i] = za.tw[i];
+  // real-space operands of the operator, issued before the inverse transform
+  // so their latency overlaps it
+  constexpr int PTS = (RT * N + NT - 1) / NT;
+  constexpr int NPRE = Op::NPRE > 0 ? Op::NPRE : 1;
+  double pre[PTS][NPRE];
+#pragma unroll
+  for (int i = 0; i < PTS; ++i) {
+    const int e = threadIdx.x + i * NT;
+    const int row = row0 + e / N;
+    if (e < RT * N && row < za.rows) Op::load(oa, (size_t)row * N + (e % N), pre[i]);
+  }
 
   if constexpr (NCI > 0) {
-    // zero the unretained middle, then scatter the retained modes
+    // zero the unretained middle, then scatter the retained modes (one wave per row)
     constexpr int TOTZ = NCI * NP * N;
     for (int e = threadIdx.x; e < TOTZ; e += NT) {
       const int k = e % N;
-      if (k >= Az && N - k >= Az) buf[e] = cx<T>{T(0), T(0)};
+      if (k >= Az && N - k >= Az) buf[LY::idx(e / N, k)] = cx<T>{T(0), T(0)};
     }
-    const int TOTL = NCI * NP * Az;
-    for (int e = threadIdx.x; e < TOTL; e += NT) {
-      const int k = e % Az;
-      const int cp = e / Az;   // c*NP + p
+    for (int cp = wave; cp < NCI * NP; cp += NT / 64) {
       const int p = cp % NP;
       const int c = cp / NP;
       const int ra = row0 + 2 * p, rb = ra + 1;
-      cx<T> A = (ra < za.rows) ? za.in[c][(size_t)ra * Az + k] : cx<T>{T(0), T(0)};
-      cx<T> B = (rb < za.rows) ? za.in[c][(size_t)rb * Az + k] : cx<T>{T(0), T(0)};
-      if (k == 0) {
-        A.im = T(0);
-        B.im = T(0);
+      const cx<T>* ia = za.in[c] + (size_t)ra * Az;
+      const cx<T>* ib = za.in[c] + (size_t)rb * Az;
+      for (int k = lane; k < Az; k += 64) {
+        cx<T> A = (ra < za.rows) ? ia[k] : cx<T>{T(0), T(0)};
+        cx<T> B = (rb < za.rows) ? ib[k] : cx<T>{T(0), T(0)};
+        if (k == 0) {
+          A.im = T(0);
+          B.im = T(0);
+        }
+        buf[LY::idx(cp, k)] = cx<T>{A.re - B.im, A.im + B.re};
+        if (k > 0) buf[LY::idx(cp, N - k)] = cx<T>{A.re + B.im, B.re - A.im};
       }
-      cx<T>* rowp = buf + (size_t)cp * N;
-      rowp[k] = cx<T>{A.re - B.im, A.im + B.re};
-      if (k > 0) rowp[N - k] = cx<T>{A.re + B.im, B.re - A.im};
     }
     __syncthreads();
     cx<T>* res = fft_lds<T, N, NCI * NP, false, true, NT>(buf, alt, tw);
     if (BIG && res != buf) {
-      for (int e = threadIdx.x; e < NCI * NP * N; e += NT) buf[e] = res[e];
+      for (int e = threadIdx.x; e < LY::size; e += NT) buf[e] = res[e];
       __syncthreads();
     }
   }
@@ -141,30 +156,29 @@ __global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op:
   double red[Op::NRED > 0 ? Op::NRED : 1];
 #pragma unroll
   for (int r = 0; r < Op::NRED; ++r) red[r] = red_identity(Op::kinds[r]);
-  for (int e = threadIdx.x; e < RT * N; e += NT) {
+#pragma unroll
+  for (int i = 0; i < PTS; ++i) {
+    const int e = threadIdx.x + i * NT;
+    if (e >= RT * N) break;
     const int z = e % N;
     const int r = e / N;
     const int row = row0 + r;
     const int p = r >> 1;
-    const bool hi = r & 1;
+    const int hi = r & 1;
     T vin[NCI > 0 ? NCI : 1];
     T vout[NCO > 0 ? NCO : 1];
 #pragma unroll
-    for (int c = 0; c < NCI; ++c)
-      vin[c] = reinterpret_cast<const T*>(&buf[(c * NP + p) * N + z])[hi ? 1 : 0];
+    for (int c = 0; c < NCI; ++c) vin[c] = reinterpret_cast<const T*>(&buf[LY::idx(c * NP + p, z)])[hi];
     if (row < za.rows) {
-      Op::apply(oa, (size_t)row * N + z, vin, vout, red);
+      Op::apply(oa, (size_t)row * N + z, pre[i], vin, vout, red);
     } else {
 #pragma unroll
       for (int c = 0; c < NCO; ++c) vout[c] = T(0);
     }
     if constexpr (NCO > 0) {
-      // both rows of a pair are handled by different threads: write halves
+      // the two rows of a pair are handled by different threads: write halves
 #pragma unroll
-      for (int c = 0; c < NCO; ++c) {
-        T* slot = reinterpret_cast<T*>(&buf[(c * NP + p) * N + z]);
-        slot[hi ? 1 : 0] = vout[c];
-      }
+      for (int c = 0; c < NCO; ++c) reinterpret_cast<T*>(&buf[LY::idx(c * NP + p, z)])[hi] = vout[c];
     }
   }
   if constexpr (Op::NRED > 0) {
@@ -177,20 +191,18 @@ __global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op:
   if constexpr (NCO > 0) {
     __syncthreads();
     cx<T>* res = fft_lds<T, N, NCO * NP, false, false, NT>(buf, alt, tw);
-    const int TOTS = NCO * NP * Az;
-    for (int e = threadIdx.x; e < TOTS; e += NT) {
-      const int k = e % Az;
-      const int cp = e / Az;
+    for (int cp = wave; cp < NCO * NP; cp += NT / 64) {
       const int p = cp % NP;
       const int c = cp / NP;
       const int ra = row0 + 2 * p, rb = ra + 1;
-      const cx<T>* rowp = res + (size_t)cp * N;
-      cx<T> Zk = rowp[k];
-      cx<T> Zm = rowp[k == 0 ? 0 : N - k];
-      cx<T> A = {T(0.5) * (Zk.re + Zm.re), T(0.5) * (Zk.im - Zm.im)};
-      cx<T> B = {T(0.5) * (Zk.im + Zm.im), T(0.5) * (Zm.re - Zk.re)};
-      if (ra < za.rows) za.out[c][(size_t)ra * Az + k] = A;
-      if (rb < za.rows) za.out[c][(size_t)rb * Az + k] = B;
+      cx<T>* oa_ = za.out[c] + (size_t)ra * Az;
+      cx<T>* ob_ = za.out[c] + (size_t)rb * Az;
+      for (int k = lane; k < Az; k += 64) {
+        const cx<T> Zk = res[LY::idx(cp, k)];
+        const cx<T> Zm = res[LY::idx(cp, k == 0 ? 0 : N - k)];
+        if (ra < za.rows) oa_[k] = cx<T>{T(0.5) * (Zk.re + Zm.re), T(0.5) * (Zk.im - Zm.im)};
+        if (rb < za.rows) ob_[k] = cx<T>{T(0.5) * (Zk.im + Zm.im), T(0.5) * (Zm.re - Zk.re)};
+      }
     }
   }
 }
@@ -213,8 +225,9 @@ template <typename T, int N, bool INV>
 __global__ void __launch_bounds__(NT) cpass_kernel(CPassArgs<T> ca) {
   constexpr int TC = cpass_cols<T, N>();
   constexpr bool BIG = plan_has_big_prime(N);
-  __shared__ cx<T> buf[N * TC];
-  __shared__ cx<T> alt[BIG ? N * TC : 1];
+  using LY = Lds<N, TC, true>;
+  __shared__ cx<T> buf[LY::size];
+  __shared__ cx<T> alt[BIG ? LY::size : 1];
   __shared__ cx<T> tw[N];
   const int col0 = blockIdx.x * TC;
   const int plane = blockIdx.y;
@@ -228,18 +241,18 @@ __global__ void __launch_bounds__(NT) cpass_kernel(CPassArgs<T> ca) {
     for (int e = threadIdx.x; e < N * TC; e += NT) {
       const int b = e % TC, j = e / TC;
       const int col = col0 + b;
-      buf[e] = (col < cols) ? in[(size_t)j * cols + col] : cx<T>{T(0), T(0)};
+      buf[LY::idx(b, j)] = (col < cols) ? in[(size_t)j * cols + col] : cx<T>{T(0), T(0)};
     }
   } else {
     for (int e = threadIdx.x; e < N * TC; e += NT) {
-      const int j = e / TC;
-      if (j >= a && j <= N - a) buf[e] = cx<T>{T(0), T(0)};
+      const int b = e % TC, j = e / TC;
+      if (j >= a && j <= N - a) buf[LY::idx(b, j)] = cx<T>{T(0), T(0)};
     }
     for (int e = threadIdx.x; e < A * TC; e += NT) {
       const int b = e % TC, jc = e / TC;
       const int col = col0 + b;
       const int jf = jc < a ? jc : jc + N - A;
-      buf[jf * TC + b] = (col < cols) ? in[(size_t)jc * cols + col] : cx<T>{T(0), T(0)};
+      buf[LY::idx(b, jf)] = (col < cols) ? in[(size_t)jc * cols + col] : cx<T>{T(0), T(0)};
     }
   }
   __syncthreads();
@@ -249,13 +262,13 @@ __global__ void __launch_bounds__(NT) cpass_kernel(CPassArgs<T> ca) {
       const int b = e % TC, jc = e / TC;
       const int col = col0 + b;
       const int jf = jc < a ? jc : jc + N - A;
-      if (col < cols) out[(size_t)jc * cols + col] = res[jf * TC + b];
+      if (col < cols) out[(size_t)jc * cols + col] = res[LY::idx(b, jf)];
     }
   } else {
     for (int e = threadIdx.x; e < N * TC; e += NT) {
       const int b = e % TC, j = e / TC;
       const int col = col0 + b;
-      if (col < cols) out[(size_t)j * cols + col] = res[e];
+      if (col < cols) out[(size_t)j * cols + col] = res[LY::idx(b, j)];
     }
   }
 }

@@ -16,65 +16,6 @@
 
 namespace pf {
 
-// cos(2 pi k / 32), k = 0..8, as literals (constant-folded after unrolling)
-__host__ __device__ constexpr double cos32_quarter(int k) {
-  return k == 0 ? 1.0
-       : k == 1 ? 0.980785280403230449126182236134
-       : k == 2 ? 0.923879532511286756128183189397
-       : k == 3 ? 0.831469612302545237078788377618
-       : k == 4 ? 0.707106781186547524400844362105
-       : k == 5 ? 0.555570233019602224742830813949
-       : k == 6 ? 0.382683432365089771728459984030
-       : k == 7 ? 0.195090322016128267848284868477
-       : 0.0;
-}
-
-__host__ __device__ constexpr double cos32(int m) {
-  m = ((m % 32) + 32) % 32;
-  return m <= 8 ? cos32_quarter(m)
-                : (m <= 16 ? -cos32_quarter(16 - m) : (m <= 24 ? -cos32_quarter(m - 16) : cos32_quarter(32 - m)));
-}
-__host__ __device__ constexpr double sin32(int m) { return cos32(m - 8); }
-
-// W_E^m = exp(-2 pi i m / E) for E | 32 (conjugated for the inverse)
-template <typename T, int E, bool INV>
-__device__ __forceinline__ cx<T> wconst(int m) {
-  const int q = (m * (32 / E)) % 32;
-  return cx<T>{T(cos32(q)), INV ? T(sin32(q)) : T(-sin32(q))};
-}
-
-// In-register E-point DFT, natural order in and out (Stockham, radix 4/2).
-template <typename T, int E, int NS, bool INV>
-__device__ __forceinline__ void dft_reg_stages(cx<T>* a) {
-  if constexpr (NS < E) {
-    constexpr int R = ((E / NS) % 4 == 0) ? 4 : 2;
-    constexpr int NB = E / R;
-    cx<T> b[E];
-#pragma unroll
-    for (int j = 0; j < NB; ++j) {
-      cx<T> v[R];
-      const int jm = j % NS;
-#pragma unroll
-      for (int s = 0; s < R; ++s) {
-        v[s] = a[j + s * NB];
-        if (NS > 1 && s > 0 && jm != 0) v[s] = cmul(v[s], wconst<T, E, INV>(jm * s * (E / (NS * R))));
-      }
-      Dft<T, R, INV>::run(v);
-      const int d = (j / NS) * NS * R + jm;
-#pragma unroll
-      for (int s = 0; s < R; ++s) b[d + s * NS] = v[s];
-    }
-#pragma unroll
-    for (int i = 0; i < E; ++i) a[i] = b[i];
-    dft_reg_stages<T, E, NS * R, INV>(a);
-  }
-}
-
-template <typename T, int E, bool INV>
-__device__ __forceinline__ void dft_reg(cx<T>* a) {
-  if constexpr (E > 1) dft_reg_stages<T, E, 1, INV>(a);
-}
-
 // Length-N transform over the L-lane group; v[e] holds position l + L e on
 // entry and on exit.  `scr` is this wave's LDS scratch of 64 * (E + 1) values.
 template <typename T, int N, int L, bool INV>
@@ -174,7 +115,9 @@ __global__ void __launch_bounds__(64) zpass_reg_kernel(ZPassArgs<T> za, typename
 #pragma unroll
     for (int c = 0; c < NCI; ++c) xin[c] = (c & 1) ? v[c >> 1][e].im : v[c >> 1][e].re;
     if (live) {
-      Op::apply(oa, (size_t)row * N + (l + L * e), xin, yout, red);
+      double pre[Op::NPRE > 0 ? Op::NPRE : 1];
+      Op::load(oa, (size_t)row * N + (l + L * e), pre);
+      Op::apply(oa, (size_t)row * N + (l + L * e), pre, xin, yout, red);
     } else {
 #pragma unroll
       for (int c = 0; c < NCO; ++c) yout[c] = T(0);
