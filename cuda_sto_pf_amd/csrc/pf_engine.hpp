@@ -22,6 +22,7 @@
 #include "pf_engine_api.hpp"
 #include "pf_error.hpp"
 #include "pf_ops.hpp"
+#include "pf_minimize.hpp"
 #include "pf_regfft.hpp"
 
 #define PF_HIP(call)                                                                  \
@@ -354,6 +355,18 @@ struct Engine final : EngineBase {
   int last_iters = 0, zm_iters = 0;
   double last_res = NAN, zm_lambda = 0.0, mean_DY = 0.0;
   double e_el = 0.0, e_dw = 0.0, e_chem = 0.0, e_grad = 0.0;
+  // phi-only minimize state
+  bool minimizing = false;
+  pf_minimize_opts mopt{};
+  double* m_gexp = nullptr;   // g_exp of the current phi
+  double* m_gfull = nullptr;  // g_exp - kappa Lap phi
+  double* m_s00 = nullptr;    // sigma : eps00 of the last elastic solve
+  double* m_xzero = nullptr;  // composition absent: x_B = 0 for the eigenstrain
+  T* m_eps = nullptr;         // total strain of the last elastic solve, 6 x R
+  double m_lambda = 0.0, m_V0 = 0.0;
+  int m_iter = 0, m_conv = 0;
+  bool m_converged = false;
+  const double* xB_el() const { return minimizing ? m_xzero : xB; }
 
   ~Engine() override {
     for (void* q : allocations) (void)hipFree(q);
@@ -529,6 +542,7 @@ struct Engine final : EngineBase {
     PF_HIP(hipMemcpy(Y, yy.data(), R * sizeof(double), hipMemcpyHostToDevice));
     PF_HIP(hipMemcpy(dYdt, dd.data(), R * sizeof(double), hipMemcpyHostToDevice));
     warm_valid = false;
+    minimizing = false;
     hist_count = 0;
     hist_head = 0;
     steps_done = 0;
@@ -587,7 +601,7 @@ struct Engine final : EngineBase {
       for (int i = 0; i < 6; ++i) sa.eps[i] = sc(i);
       hipLaunchKernelGGL((kop_strain<T, double>), dim3(gk), dim3(NT), 0, stream, sa);
     } else {
-      typename OpElasticCold<T>::Args oa{P, phi[cur], xB};
+      typename OpElasticCold<T>::Args oa{P, phi[cur], xB_el()};
       launch_zpass<T, 0, 6, OpElasticCold<T>>(Nz, stream, zargs(0, 0, 0, 6), oa);
       ypass(false, 0, 6);
       xpass(false, 0, 6, Sc);
@@ -660,7 +674,7 @@ struct Engine final : EngineBase {
       }
       xpass(true, 0, 6, Sc);
       ypass(true, 0, 6);
-      typename OpHij<T>::Args oa{P, phi[cur], xB, invN, mode == 1 ? 1 : 0};
+      typename OpHij<T>::Args oa{P, phi[cur], xB_el(), invN, mode == 1 ? 1 : 0};
       launch_zpass<T, 6, 6, OpHij<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
       ypass(false, 0, 6);
       xpass(false, 0, 6, Sc);
@@ -759,7 +773,7 @@ struct Engine final : EngineBase {
       hipLaunchKernelGGL((kop_strain<T, double>), dim3(gk), dim3(NT), 0, stream, sa);
     };
     if (!warm) {
-      typename OpElasticCold<T>::Args oa{P, phi[cur], xB};
+      typename OpElasticCold<T>::Args oa{P, phi[cur], xB_el()};
       launch_zpass<T, 0, 6, OpElasticCold<T>>(Nz, stream, zargs(0, 0, 0, 6), oa);
       ypass(false, 0, 6);
       xpass(false, 0, 6, Sc);
@@ -775,7 +789,7 @@ struct Engine final : EngineBase {
     const T invN = T(1.0 / (double)R);
     while (true) {
       const int mode = !accel ? 3 : (it == 0 ? 1 : 2);
-      typename OpHij<T>::Args oa{P, phi[cur], xB, invN, mode == 2 ? 1 : 0};
+      typename OpHij<T>::Args oa{P, phi[cur], xB_el(), invN, mode == 2 ? 1 : 0};
       launch_zpass<T, 6, 6, OpHij<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
       ypass(false, 0, 6);
       xpass(false, 0, 6, Sc);
@@ -834,7 +848,7 @@ struct Engine final : EngineBase {
     const KDims d = kd();
     const int gk = grid_for(nSc);
     if (!warm) {
-      typename OpElasticCold<T>::Args oa{P, phi[cur], xB};
+      typename OpElasticCold<T>::Args oa{P, phi[cur], xB_el()};
       launch_zpass<T, 0, 6, OpElasticCold<T>>(Nz, stream, zargs(0, 0, 0, 6), oa);
       ypass(false, 0, 6);
       xpass(false, 0, 6, Sc);
@@ -865,7 +879,7 @@ struct Engine final : EngineBase {
     double res = NAN;
     const T invN = T(1.0 / (double)R);
     while (true) {
-      typename OpHij<T>::Args oa{P, phi[cur], xB, invN, 0};
+      typename OpHij<T>::Args oa{P, phi[cur], xB_el(), invN, 0};
       launch_zpass<T, 6, 6, OpHij<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
       ypass(false, 0, 6);
       xpass(false, 0, 6, Sc);
@@ -929,6 +943,7 @@ struct Engine final : EngineBase {
 
   // ---------------------------------------------------------------- step
   int step() override {
+    minimizing = false;
     const T invN = T(1.0 / (double)R);
     const int rows = Nx * Ny;
     const bool zm = prm.zero_mode != 0;
@@ -1074,6 +1089,195 @@ struct Engine final : EngineBase {
     }
     zm_lambda = lam;
     hipLaunchKernelGGL(zm_apply_kernel, dim3(g), dim3(NT), 0, stream, Ystar, lam, Y, xB, dYdt, prm.dt, R);
+    return PF_OK;
+  }
+
+  // ---------------------------------------------------------------- minimize
+  int minimize_begin(const pf_minimize_opts& o) override {
+    int rc;
+    if (!m_gexp) {
+      if ((rc = dalloc(&m_gexp, R)) || (rc = dalloc(&m_gfull, R)) || (rc = dalloc(&m_s00, R)) ||
+          (rc = dalloc(&m_xzero, R)) || (rc = dalloc(&m_eps, 6 * (size_t)R)))
+        return rc;
+      PF_HIP(hipMemsetAsync(m_xzero, 0, R * sizeof(double), stream));
+    }
+    mopt = o;
+    minimizing = true;
+    m_lambda = 0.0;
+    m_V0 = o.V0;
+    m_iter = 0;
+    m_conv = 0;
+    m_converged = false;
+    warm_valid = false;   // the eigenstrain differs from the dynamics one
+    hist_count = 0;
+    hist_head = 0;
+    return PF_OK;
+  }
+
+  // phi (real, FP64) -> forward z via Op, then y and x passes into sc(0)
+  void min_forward_to_sc0() {
+    ypass(false, 0, 1);
+    xpass(false, 0, 1, Sc);
+  }
+  // Lap^ of sc(0) -> inverse x, y passes into Sz(1)
+  void min_lap_to_sz1() {
+    const KDims d = kd();
+    hipLaunchKernelGGL(kop_lap<T>, dim3(grid_for(nSc)), dim3(NT), 0, stream, d, nSc, (const C*)sc(0), sc(1));
+    xpass(true, 1, 1, sc(1));
+    ypass(true, 1, 1);
+  }
+  // g_full of phi from Sz(1); sums of h' g_full and h'^2 into dscal[o], dscal[o+1]
+  void min_gfull(const double* ph, int o) {
+    const T invN = T(1.0 / (double)R);
+    typename OpMinGfull<T>::Args oa{ph, m_gexp, m_gfull, prm.kappa_phi, invN};
+    launch_zpass<T, 1, 0, OpMinGfull<T>>(Nz, stream, zargs(1, 1, 0, 0), oa);
+    finalize(zpass_grid<T, 1, 0>(Nz, Nx * Ny), 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal + o, 1.0);
+  }
+  // semi-implicit phi update from the field phi - L dt (g_exp + lambda h') of ph
+  void min_update(const double* ph, double lambda) {
+    const double Ldt = prm.L_phi * mopt.dt;
+    typename OpMinUpd<T>::Args oa{ph, m_gexp, lambda, Ldt};
+    launch_zpass<T, 0, 1, OpMinUpd<T>>(Nz, stream, zargs(0, 0, 0, 1), oa);
+    min_forward_to_sc0();
+    hipLaunchKernelGGL(kop_phi<T>, dim3(grid_for(nSc)), dim3(NT), 0, stream, kd(), nSc, sc(0),
+                       Ldt * prm.kappa_phi);
+    xpass(true, 0, 1, Sc);
+    ypass(true, 0, 1);
+  }
+  static double min_multiplier(double s_hg, double s_h2) { return std::fabs(s_h2) > 1e-30 ? -s_hg / s_h2 : 0.0; }
+
+  int minimize_iterate(pf_minimize_stats* st) override {
+    if (!minimizing) return fail(PF_E_ARG, "pf_minimize_begin has not been called on this handle");
+    const T invN = T(1.0 / (double)R);
+    const int rows = Nx * Ny;
+    const bool el = P.elastic != 0;
+    double* phi_old = phi[cur];
+    double* phi_new = phi[1 - cur];
+    const int gR = grid_for(R);
+    int rc;
+    ++m_iter;
+    last_iters = 0;
+    // energies and <h> of phi^n (main_cuda.cu:29264-29346)
+    hipLaunchKernelGGL(grad_energy_kernel, dim3(gR), dim3(NT), 0, stream, phi_old, Nx, Ny, Nz, prm.dx, prm.dy,
+                       prm.dz, prm.kappa_phi, partials);
+    finalize(gR, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 8, 1.0 / (double)R);
+    hipLaunchKernelGGL(min_volume_kernel, dim3(gR), dim3(NT), 0, stream, phi_old, R, partials);
+    finalize(gR, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal + 48, 1.0);
+    if (el) {
+      if ((rc = elastic_solve())) return rc;
+      typename OpMinRhs<T, true>::Args oa{P, phi_old, m_gexp, m_eps, m_s00, (size_t)R, invN};
+      launch_zpass<T, 6, 1, OpMinRhs<T, true>>(Nz, stream, zargs(0, 6, 0, 1), oa);
+      finalize(zpass_grid<T, 6, 1>(Nz, rows), 3, RedKinds{{RED_SUM, RED_SUM, RED_SUM, 0}}, dscal + 9, 1.0);
+    } else {
+      typename OpMinRhs<T, false>::Args oa{P, phi_old, m_gexp, m_eps, m_s00, (size_t)R, invN};
+      launch_zpass<T, 0, 1, OpMinRhs<T, false>>(Nz, stream, zargs(0, 0, 0, 1), oa);
+      finalize(zpass_grid<T, 0, 1>(Nz, rows), 3, RedKinds{{RED_SUM, RED_SUM, RED_SUM, 0}}, dscal + 9, 1.0);
+    }
+    // g_full(phi^n) and the under-relaxed multiplier (main_cuda.cu:29436-29500)
+    min_forward_to_sc0();
+    min_lap_to_sz1();
+    min_gfull(phi_old, 50);
+    if ((rc = fetch(58))) return rc;
+    const double grad_mean = hscal[8];
+    const double gel_sum = hscal[9], dw_sum = hscal[10], core_sum = hscal[11];
+    const double mean_h0 = hscal[48] / (double)R;
+    if (m_iter == 1 && m_V0 <= 0.0) m_V0 = mean_h0;
+    const double lam_raw = min_multiplier(hscal[50], hscal[51]);
+    const double eta = std::fmin(std::fmax(mopt.eta_lambda_vol, 0.0), 1.0);
+    m_lambda = (1.0 - eta) * m_lambda + eta * lam_raw;
+    // phi^{n+1}
+    min_update(phi_old, m_lambda);
+    if (el) {
+      typename OpMinFinish<T, true, true>::Args oa{P, phi_old, phi_new, m_gexp, m_eps, m_s00, (size_t)R, invN};
+      launch_zpass<T, 1, 1, OpMinFinish<T, true, true>>(Nz, stream, zargs(0, 1, 0, 1), oa);
+    } else {
+      typename OpMinFinish<T, false, true>::Args oa{P, phi_old, phi_new, m_gexp, m_eps, m_s00, (size_t)R, invN};
+      launch_zpass<T, 1, 1, OpMinFinish<T, false, true>>(Nz, stream, zargs(0, 1, 0, 1), oa);
+    }
+    finalize(zpass_grid<T, 1, 1>(Nz, rows), 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 52, 1.0);
+    // KKT residual at phi^{n+1} with the exact multiplier (main_cuda.cu:29596-29728)
+    min_forward_to_sc0();
+    min_lap_to_sz1();
+    min_gfull(phi_new, 53);
+    if ((rc = fetch(58))) return rc;
+    const double rms_dphi = std::sqrt(hscal[52] / (double)R) / mopt.dt;
+    const double lam_kkt = min_multiplier(hscal[53], hscal[54]);
+    hipLaunchKernelGGL(min_kkt_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, m_gfull, lam_kkt, R, partials);
+    finalize(gR, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 55, 1.0);
+    // post-projection sub-steps, eta = 1, stress frozen (main_cuda.cu:32104-32197)
+    double lam_pp = lam_kkt;
+    for (int k = 0; k < mopt.post_projection_iters; ++k) {
+      if (k > 0) {
+        if (el) {
+          typename OpMinEval<T, true>::Args oa{P, phi_new, m_gexp, m_eps, m_s00, (size_t)R};
+          launch_zpass<T, 0, 1, OpMinEval<T, true>>(Nz, stream, zargs(0, 0, 0, 1), oa);
+        } else {
+          typename OpMinEval<T, false>::Args oa{P, phi_new, m_gexp, m_eps, m_s00, (size_t)R};
+          launch_zpass<T, 0, 1, OpMinEval<T, false>>(Nz, stream, zargs(0, 0, 0, 1), oa);
+        }
+        min_forward_to_sc0();
+        min_lap_to_sz1();
+        min_gfull(phi_new, 56);
+        if ((rc = fetch(58))) return rc;
+        lam_pp = min_multiplier(hscal[56], hscal[57]);
+      }
+      m_lambda = lam_pp;
+      min_update(phi_new, lam_pp);
+      typename OpMinFinish<T, false, false>::Args oa{P, phi_new, phi_new, m_gexp, m_eps, m_s00, (size_t)R, invN};
+      launch_zpass<T, 1, 0, OpMinFinish<T, false, false>>(Nz, stream, zargs(0, 1, 0, 0), oa);
+    }
+    // first-order volume projection (main_cuda.cu:32274-32290)
+    hipLaunchKernelGGL(min_volume_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, R, partials);
+    finalize(gR, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal + 48, 1.0);
+    if ((rc = fetch(58))) return rc;
+    const double rms_res = std::sqrt(hscal[55] / (double)R);
+    double mean_h = hscal[48] / (double)R;
+    if (m_V0 > 1e-30 && std::isfinite(mean_h)) {
+      const double vol_err = m_V0 - mean_h;
+      if (std::fabs(vol_err) > 1e-14) {
+        const double mean_hp2 = hscal[49] / (double)R;
+        const double lc = mean_hp2 > 1e-30 ? vol_err / mean_hp2 : 0.0;
+        hipLaunchKernelGGL(min_project_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, lc, R, partials);
+        finalize(gR, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 48, 1.0);
+        if ((rc = fetch(58))) return rc;
+        mean_h = hscal[48] / (double)R;
+      }
+    }
+    cur = 1 - cur;
+    // convergence bookkeeping (main_cuda.cu:32400-32560); condB needs the
+    // excess free energy, which phi-only minimize leaves undefined
+    const double vol_err_rel = m_V0 > 1e-30 ? std::fabs(mean_h - m_V0) / (m_V0 + 1e-30) : 0.0;
+    const bool condA = std::isfinite(rms_dphi) && rms_dphi < mopt.rms_dphi_threshold && std::isfinite(rms_res) &&
+                       rms_res < mopt.rms_res_threshold && std::isfinite(vol_err_rel) &&
+                       vol_err_rel < mopt.vol_err_rel_threshold;
+    const bool t_ok = (double)m_iter * mopt.dt + 1e-14 >= mopt.min_pseudo_time;
+    m_conv = (condA && t_ok) ? m_conv + 1 : 0;
+    if (m_conv >= mopt.convergence_steps) m_converged = true;
+    e_grad = grad_mean;
+    e_dw = dw_sum / (double)R;
+    e_el = gel_sum / (double)R;
+    e_chem = 0.0;
+    ++steps_done;
+    if (st) {
+      st->iteration = m_iter;
+      st->converged = m_converged ? 1 : 0;
+      st->convergence_count = m_conv;
+      st->elastic_iterations = el ? last_iters : 0;
+      st->mean_h = mean_h;
+      st->V0 = m_V0;
+      st->lambda_vol = m_lambda;
+      st->F_surf_hat = grad_mean + dw_sum / (double)R;
+      st->F_el_hat = gel_sum / (double)R;
+      st->interface_sum = grad_mean * (double)R + dw_sum;
+      st->el_core_sum = core_sum;
+      st->rms_res = rms_res;
+      st->rms_dphi = rms_dphi;
+      st->vol_err_rel = vol_err_rel;
+      st->lambda_raw = lam_raw;
+      st->lambda_kkt = lam_kkt;
+    }
+    if (!std::isfinite(mean_h) || !std::isfinite(m_lambda))
+      return fail(PF_E_NONFINITE, "minimize produced a non-finite phi");
     return PF_OK;
   }
 

@@ -23,6 +23,8 @@ struct EngineBase {
   virtual int fft_inverse(const double* in_half, double* out) = 0;
   virtual int sync() = 0;
   virtual int time_kernel(int id, int reps, double* us, double* bytes) = 0;
+  virtual int minimize_begin(const pf_minimize_opts& o) = 0;
+  virtual int minimize_iterate(pf_minimize_stats* st) = 0;
 };
 
 std::unique_ptr<EngineBase> make_engine_f64();

@@ -1,0 +1,272 @@
+// Phi-only minimize mode (main_cuda --mode=minimize without
+// --minimize-full-model): point-wise operators fused into the z passes and
+// the element-wise reductions of the volume constraint.
+//
+// The iteration (main_cuda.cu:29264-29750, 32104-32290, 32400-32560):
+//   g_exp  = W g'(phi) + dgel/dphi with stress and total strain held at the
+//            values of the last elastic solve (compute_phi_rhs_minimize_kernel,
+//            cuda_kernels.cu:6298-6368; composition absent, eps0 = h eps00),
+//   g_full = g_exp - kappa Lap phi,
+//   lambda = (1 - eta) lambda + eta (-sum h' g_full / sum h'^2),
+//   phi^  <- (phi^ - L dt (g_exp + lambda h')^) / (1 + L dt kappa k^2),
+// then the KKT residual at the new phi, the post-projection sub-steps with
+// eta = 1, and the first-order volume projection phi += lc h'(phi).
+//
+// The elastic part of g_exp depends on phi only through h and h', so after
+// the solve the stress is kept as one scalar per voxel, s00 = sigma : eps00
+// (engineering shear), together with the total strain (6 components):
+//   dgel/dphi = -h' s00 + 0.5 h' (eps - h eps00) : dC : (eps - h eps00).
+#pragma once
+#include "pf_ops.hpp"
+
+namespace pf {
+
+// elastic part of the phi-derivative of the energy density at frozen strain
+PF_HD double min_dgel(const PhysParams& P, double phi, const double* eps, double s00) {
+  const double hp = h_prime_of_phi(phi);
+  const double h = h_of_phi(phi);
+  double el[6], q[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) el[c] = eps[c] - h * P.eps00[c];
+  voigt_apply<double>(P.dC, el, q);
+  const double Q = el[0] * q[0] + el[1] * q[1] + el[2] * q[2] + 2.0 * el[3] * q[3] + 2.0 * el[4] * q[4] +
+                   2.0 * el[5] * q[5];
+  return -hp * s00 + 0.5 * hp * Q;
+}
+
+// After the elastic solve: inverse z of the 6 strains -> stress at phi^n,
+// keep (eps, s00), g_exp(phi^n), energies; forward z of phi^n.
+//   red[0] sum 0.5 sigma:eps_el        (compute_gel_density_kernel, cuda_kernels.cu:1789-1850)
+//   red[1] sum W g(phi)                (compute_dw_energy_density_kernel, 811-815)
+//   red[2] sum 0.5 sigma:eps_el h(clamp01 phi)  (scale_field_by_h_kernel, 761-771)
+template <typename T, bool ELASTIC>
+struct OpMinRhs {
+  struct Args {
+    PhysParams P;
+    const double* phi;
+    double* gexp;
+    T* eps;            // 6 components, stride R
+    double* s00;
+    size_t R;
+    T invN;
+  };
+  static constexpr int NRED = 3;
+  static constexpr int kinds[3] = {RED_SUM, RED_SUM, RED_SUM};
+  static constexpr int NPRE = 1;
+  __device__ static void load(const Args& a, size_t g, double* pre) { pre[0] = a.phi[g]; }
+  __device__ static void apply(const Args& a, size_t g, const double* pre, const T* vin, T* out, double* red) {
+    const PhysParams& P = a.P;
+    const double phi = pre[0];
+    double f = P.W * g_prime_of_phi(phi);
+    red[1] += P.W * g_of_phi(phi);
+    if constexpr (ELASTIC) {
+      double eps[6], el[6], sig[6];
+      const double h = h_of_phi(phi);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        eps[c] = double(vin[c] * a.invN) + P.E0[c];
+        el[c] = eps[c] - h * P.eps00[c];
+      }
+      voigt_apply_h<double>(P, h, el, sig);
+      const double s00 = sig[0] * P.eps00[0] + sig[1] * P.eps00[1] + sig[2] * P.eps00[2] +
+                         2.0 * (sig[3] * P.eps00[3] + sig[4] * P.eps00[4] + sig[5] * P.eps00[5]);
+      const double gel = 0.5 * (sig[0] * el[0] + sig[1] * el[1] + sig[2] * el[2] +
+                                2.0 * (sig[5] * el[5] + sig[4] * el[4] + sig[3] * el[3]));
+      red[0] += gel;
+      red[2] += gel * h_of_phi(clamp01(phi));
+#pragma unroll
+      for (int c = 0; c < 6; ++c) a.eps[c * a.R + g] = T(eps[c]);
+      a.s00[g] = s00;
+      f += min_dgel(P, phi, eps, s00);
+    }
+    a.gexp[g] = f;
+    out[0] = T(phi);
+  }
+};
+
+// inverse z of kappa-free Lap phi -> g_full = g_exp - kappa Lap phi;
+//   red[0] sum h'(phi) g_full, red[1] sum h'(phi)^2
+// (compute_hprime_times_rhs_kernel / compute_hprime_sq_values_kernel,
+//  cuda_kernels.cu:661-682)
+template <typename T>
+struct OpMinGfull {
+  struct Args {
+    const double* phi;
+    const double* gexp;
+    double* gfull;
+    double kappa;
+    T invN;
+  };
+  static constexpr int NRED = 2;
+  static constexpr int kinds[2] = {RED_SUM, RED_SUM};
+  static constexpr int NPRE = 2;
+  __device__ static void load(const Args& a, size_t g, double* pre) {
+    pre[0] = a.phi[g];
+    pre[1] = a.gexp[g];
+  }
+  __device__ static void apply(const Args& a, size_t g, const double* pre, const T* vin, T*, double* red) {
+    const double lap = double(vin[0]) * double(a.invN);
+    const double gf = pre[1] - a.kappa * lap;
+    const double hp = h_prime_of_phi(pre[0]);
+    a.gfull[g] = gf;
+    red[0] += hp * gf;
+    red[1] += hp * hp;
+  }
+};
+
+// phi - L dt (g_exp + lambda h'(phi)) -> forward z (the update is linear, so
+// the transform of this field is phi^ - L dt rhs^)
+template <typename T>
+struct OpMinUpd {
+  struct Args {
+    const double* phi;
+    const double* gexp;
+    double lambda;
+    double Ldt;
+  };
+  static constexpr int NRED = 0;
+  static constexpr int kinds[1] = {0};
+  static constexpr int NPRE = 2;
+  __device__ static void load(const Args& a, size_t g, double* pre) {
+    pre[0] = a.phi[g];
+    pre[1] = a.gexp[g];
+  }
+  __device__ static void apply(const Args& a, size_t, const double* pre, const T*, T* out, double*) {
+    const double phi = pre[0];
+    out[0] = T(phi - a.Ldt * (pre[1] + a.lambda * h_prime_of_phi(phi)));
+  }
+};
+
+// inverse z of the updated phi^: phi = clamp(v/N, -1e-6, 1+1e-6)
+// (phi_normalize_and_clamp_kernel, cuda_kernels.cu:1071-1084);
+//   red[0] sum (phi - phi_ref)^2  (diff_sq_kernel, main_cuda.cu:2236-2240)
+// With NEXT: g_exp at the new phi (frozen strain) and forward z of phi.
+template <typename T, bool ELASTIC, bool NEXT>
+struct OpMinFinish {
+  struct Args {
+    PhysParams P;
+    const double* phi_ref;
+    double* phi_out;
+    double* gexp;
+    const T* eps;
+    const double* s00;
+    size_t R;
+    T invN;
+  };
+  static constexpr int NRED = 1;
+  static constexpr int kinds[1] = {RED_SUM};
+  static constexpr int NPRE = 1;
+  __device__ static void load(const Args& a, size_t g, double* pre) { pre[0] = a.phi_ref[g]; }
+  __device__ static void apply(const Args& a, size_t g, const double* pre, const T* vin, T* out, double* red) {
+    double v = double(vin[0]) * double(a.invN);
+    if (v < -1e-6) v = -1e-6;
+    if (v > 1.0 + 1e-6) v = 1.0 + 1e-6;
+    a.phi_out[g] = v;
+    const double d = v - pre[0];
+    red[0] += d * d;
+    if constexpr (NEXT) {
+      double f = a.P.W * g_prime_of_phi(v);
+      if constexpr (ELASTIC) {
+        double eps[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) eps[c] = double(a.eps[c * a.R + g]);
+        f += min_dgel(a.P, v, eps, a.s00[g]);
+      }
+      a.gexp[g] = f;
+      out[0] = T(v);
+    }
+  }
+};
+
+// g_exp at the current phi (frozen strain) and forward z of phi: the start
+// of a further post-projection sub-step
+template <typename T, bool ELASTIC>
+struct OpMinEval {
+  struct Args {
+    PhysParams P;
+    const double* phi;
+    double* gexp;
+    const T* eps;
+    const double* s00;
+    size_t R;
+  };
+  static constexpr int NRED = 0;
+  static constexpr int kinds[1] = {0};
+  static constexpr int NPRE = 1;
+  __device__ static void load(const Args& a, size_t g, double* pre) { pre[0] = a.phi[g]; }
+  __device__ static void apply(const Args& a, size_t g, const double* pre, const T*, T* out, double*) {
+    const double v = pre[0];
+    double f = a.P.W * g_prime_of_phi(v);
+    if constexpr (ELASTIC) {
+      double eps[6];
+#pragma unroll
+      for (int c = 0; c < 6; ++c) eps[c] = double(a.eps[c * a.R + g]);
+      f += min_dgel(a.P, v, eps, a.s00[g]);
+    }
+    a.gexp[g] = f;
+    out[0] = T(v);
+  }
+};
+
+// ---------------------------------------------------------------- element-wise
+namespace {
+// res = g_full + lambda h'(phi), zeroed where it only points out of [0,1]
+// (compute_projected_phi_kkt_residual_sq_kernel, cuda_kernels.cu:684-706);
+//   red[0] sum res^2
+__global__ void __launch_bounds__(NT) min_kkt_kernel(const double* phi, const double* gfull, double lambda,
+                                                     long n, double* partials) {
+  double s = 0.0;
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
+    const double p = phi[i];
+    double r = gfull[i] + lambda * h_prime_of_phi(p);
+    if ((p <= 1e-12 && r > 0.0) || (p >= 1.0 - 1e-12 && r < 0.0)) r = 0.0;
+    s += r * r;
+  }
+  const int kinds[1] = {RED_SUM};
+  block_reduce_store<1>(&s, kinds, partials + blockIdx.x);
+}
+
+// red[0] sum h(clamp01 phi) (reduce_sum_h_kernel, cuda_kernels.cu:5580-5607),
+// red[1] sum h'(phi)^2
+__global__ void __launch_bounds__(NT) min_volume_kernel(const double* phi, long n, double* partials) {
+  double v[2] = {0.0, 0.0};
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
+    const double p = phi[i];
+    const double hp = h_prime_of_phi(p);
+    v[0] += h_of_phi(clamp01(p));
+    v[1] += hp * hp;
+  }
+  const int kinds[2] = {RED_SUM, RED_SUM};
+  block_reduce_store<2>(v, kinds, partials + 2 * (size_t)blockIdx.x);
+}
+
+// phi <- clamp01(phi + lc h'(phi)) (apply_volume_projection_kernel,
+// cuda_kernels.cu:639-647), then red[0] sum h(clamp01 phi)
+__global__ void __launch_bounds__(NT) min_project_kernel(double* phi, double lc, long n, double* partials) {
+  double s = 0.0;
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
+    const double p = phi[i];
+    const double q = clamp01(p + lc * h_prime_of_phi(p));
+    phi[i] = q;
+    s += h_of_phi(q);
+  }
+  const int kinds[1] = {RED_SUM};
+  block_reduce_store<1>(&s, kinds, partials + blockIdx.x);
+}
+
+}  // namespace
+
+// Lap^ = -k^2 phi^ on the compact spectrum (compute_laplacian_k_kernel,
+// cuda_kernels.cu:3528-3538)
+template <typename T>
+__global__ void __launch_bounds__(NT) kop_lap(KDims d, long n, const cx<T>* in, cx<T>* out) {
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
+    double kx, ky, kz;
+    int iz;
+    kvec(d, i, kx, ky, kz, iz);
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    out[i] = cx<T>{T(-k2 * double(in[i].re)), T(-k2 * double(in[i].im))};
+  }
+}
+
+}  // namespace pf

@@ -110,14 +110,19 @@ __global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op:
   for (int i = threadIdx.x; i < N; i += NT) tw[i] = za.tw[i];
   // real-space operands of the operator, issued before the inverse transform
   // so their latency overlaps it
+  // (only while the operands of all of a thread's points fit a modest
+  // register budget; otherwise each point loads its operands just before use)
   constexpr int PTS = (RT * N + NT - 1) / NT;
   constexpr int NPRE = Op::NPRE > 0 ? Op::NPRE : 1;
-  double pre[PTS][NPRE];
+  constexpr bool PREFETCH = PTS * NPRE <= 48;
+  double pre[PREFETCH ? PTS : 1][NPRE];
+  if constexpr (PREFETCH) {
 #pragma unroll
-  for (int i = 0; i < PTS; ++i) {
-    const int e = threadIdx.x + i * NT;
-    const int row = row0 + e / N;
-    if (e < RT * N && row < za.rows) Op::load(oa, (size_t)row * N + (e % N), pre[i]);
+    for (int i = 0; i < PTS; ++i) {
+      const int e = threadIdx.x + i * NT;
+      const int row = row0 + e / N;
+      if (e < RT * N && row < za.rows) Op::load(oa, (size_t)row * N + (e % N), pre[i]);
+    }
   }
 
   if constexpr (NCI > 0) {
@@ -156,10 +161,7 @@ __global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op:
   double red[Op::NRED > 0 ? Op::NRED : 1];
 #pragma unroll
   for (int r = 0; r < Op::NRED; ++r) red[r] = red_identity(Op::kinds[r]);
-#pragma unroll
-  for (int i = 0; i < PTS; ++i) {
-    const int e = threadIdx.x + i * NT;
-    if (e >= RT * N) break;
+  auto point = [&](int e, const double* pp) {
     const int z = e % N;
     const int r = e / N;
     const int row = row0 + r;
@@ -170,7 +172,12 @@ __global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op:
 #pragma unroll
     for (int c = 0; c < NCI; ++c) vin[c] = reinterpret_cast<const T*>(&buf[LY::idx(c * NP + p, z)])[hi];
     if (row < za.rows) {
-      Op::apply(oa, (size_t)row * N + z, pre[i], vin, vout, red);
+      double lp[NPRE];
+      if (pp == nullptr) {
+        Op::load(oa, (size_t)row * N + z, lp);
+        pp = lp;
+      }
+      Op::apply(oa, (size_t)row * N + z, pp, vin, vout, red);
     } else {
 #pragma unroll
       for (int c = 0; c < NCO; ++c) vout[c] = T(0);
@@ -180,6 +187,15 @@ __global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op:
 #pragma unroll
       for (int c = 0; c < NCO; ++c) reinterpret_cast<T*>(&buf[LY::idx(c * NP + p, z)])[hi] = vout[c];
     }
+  };
+  if constexpr (PREFETCH) {
+#pragma unroll
+    for (int i = 0; i < PTS; ++i) {
+      const int e = threadIdx.x + i * NT;
+      if (e < RT * N) point(e, pre[i]);
+    }
+  } else {
+    for (int e = threadIdx.x; e < RT * N; e += NT) point(e, nullptr);
   }
   if constexpr (Op::NRED > 0) {
     int kinds[Op::NRED > 0 ? Op::NRED : 1];

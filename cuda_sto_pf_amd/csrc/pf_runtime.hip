@@ -220,6 +220,46 @@ int pf_time_kernel(pf_handle* h, int kernel_id, int reps, double* us, double* by
   return h->eng->time_kernel(kernel_id, reps, us, bytes);
 }
 
+int pf_minimize_opts_default(pf_minimize_opts* o) {
+  if (!o) return pf::fail(PF_E_ARG, "pf_minimize_opts_default: null");
+  o->dt = 0.01;
+  o->max_iter = 2000;
+  o->V0 = 0.0;
+  o->eta_lambda_vol = 0.2;
+  o->post_projection_iters = 1;
+  o->rms_dphi_threshold = 1.0e-6;
+  o->rms_res_threshold = 1.0e-4;
+  o->vol_err_rel_threshold = 1.0e-4;
+  o->convergence_steps = 10;
+  o->min_pseudo_time = 0.0;
+  return PF_OK;
+}
+
+int pf_minimize_begin(pf_handle* h, const pf_minimize_opts* o) {
+  if (!h || !o) return pf::fail(PF_E_ARG, "pf_minimize_begin: null");
+  if (!(o->dt > 0.0) || !std::isfinite(o->dt)) return pf::fail(PF_E_ARG, "minimize dt must be positive");
+  if (o->max_iter < 0 || o->post_projection_iters < 0 || o->convergence_steps < 1)
+    return pf::fail(PF_E_ARG, "minimize: bad iteration counts");
+  return h->eng->minimize_begin(*o);
+}
+
+int pf_minimize_iterate(pf_handle* h, pf_minimize_stats* st) {
+  if (!h) return pf::fail(PF_E_ARG, "pf_minimize_iterate: null");
+  return h->eng->minimize_iterate(st);
+}
+
+int pf_minimize(pf_handle* h, const pf_minimize_opts* o, pf_minimize_stats* st) {
+  int rc = pf_minimize_begin(h, o);
+  if (rc) return rc;
+  pf_minimize_stats last{};
+  for (int it = 0; it < o->max_iter; ++it) {
+    if ((rc = h->eng->minimize_iterate(&last))) return rc;
+    if (last.converged) break;
+  }
+  if (st) *st = last;
+  return PF_OK;
+}
+
 int pf_sync(pf_handle* h) {
   if (!h) return pf::fail(PF_E_ARG, "pf_sync: null");
   return h->eng->sync();

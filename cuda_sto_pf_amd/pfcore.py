@@ -75,6 +75,30 @@ class PfScalars(C.Structure):
 
 
 # every entry point declared in include/pfcore.h
+class PfMinimizeOpts(C.Structure):
+    _fields_ = [
+        ("dt", C.c_double), ("max_iter", C.c_int), ("V0", C.c_double), ("eta_lambda_vol", C.c_double),
+        ("post_projection_iters", C.c_int), ("rms_dphi_threshold", C.c_double),
+        ("rms_res_threshold", C.c_double), ("vol_err_rel_threshold", C.c_double),
+        ("convergence_steps", C.c_int), ("min_pseudo_time", C.c_double),
+    ]
+
+
+class PfMinimizeStats(C.Structure):
+    _fields_ = [
+        ("iteration", C.c_int), ("converged", C.c_int), ("convergence_count", C.c_int),
+        ("elastic_iterations", C.c_int),
+        ("mean_h", C.c_double), ("V0", C.c_double), ("lambda_vol", C.c_double),
+        ("F_surf_hat", C.c_double), ("F_el_hat", C.c_double),
+        ("interface_sum", C.c_double), ("el_core_sum", C.c_double),
+        ("rms_res", C.c_double), ("rms_dphi", C.c_double), ("vol_err_rel", C.c_double),
+        ("lambda_raw", C.c_double), ("lambda_kkt", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 EXPORTS = {
     "pf_version": (C.c_int, []),
     "pf_last_error": (C.c_int, [C.c_char_p, C.c_size_t]),
@@ -92,6 +116,10 @@ EXPORTS = {
     "pf_sync": (C.c_int, [C.c_void_p]),
     "pf_time_kernel": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double),
                                  C.POINTER(C.c_double)]),
+    "pf_minimize_opts_default": (C.c_int, [C.POINTER(PfMinimizeOpts)]),
+    "pf_minimize_begin": (C.c_int, [C.c_void_p, C.POINTER(PfMinimizeOpts)]),
+    "pf_minimize_iterate": (C.c_int, [C.c_void_p, C.POINTER(PfMinimizeStats)]),
+    "pf_minimize": (C.c_int, [C.c_void_p, C.POINTER(PfMinimizeOpts), C.POINTER(PfMinimizeStats)]),
     "pf_destroy": (C.c_int, [C.c_void_p]),
     "pf_fft_forward": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "pf_fft_inverse": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
@@ -224,12 +252,37 @@ class Simulation:
     def sync(self):
         _check(lib().pf_sync(self._h))
 
+    def minimize_begin(self, opts):
+        self._opts = opts
+        _check(lib().pf_minimize_begin(self._h, C.byref(opts)))
+
+    def minimize_iterate(self):
+        st = PfMinimizeStats()
+        _check(lib().pf_minimize_iterate(self._h, C.byref(st)))
+        return st.as_dict()
+
+    def minimize(self, opts):
+        st = PfMinimizeStats()
+        _check(lib().pf_minimize(self._h, C.byref(opts), C.byref(st)))
+        return st.as_dict()
+
     def time_kernel(self, kernel_id, reps=20):
         """Mean microseconds and bytes of one launch of a step kernel (HIP events)."""
         us = C.c_double()
         b = C.c_double()
         _check(lib().pf_time_kernel(self._h, int(kernel_id), int(reps), C.byref(us), C.byref(b)))
         return us.value, b.value
+
+
+def minimize_opts(**kw):
+    """pf_minimize_opts with the reference defaults, overridden by keywords."""
+    o = PfMinimizeOpts()
+    _check(lib().pf_minimize_opts_default(C.byref(o)))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise KeyError(k)
+        setattr(o, k, v)
+    return o
 
 
 def fft_forward(a, precision=PF_FP64):

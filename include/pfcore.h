@@ -170,6 +170,51 @@ enum pf_kernel_id {
  * the simulation state is not valid afterwards (call pf_set_fields). */
 int pf_time_kernel(pf_handle* h, int kernel_id, int reps, double* us_per_launch, double* bytes_per_launch);
 
+/* Phi-only minimize mode (--mode=minimize without --minimize-full-model).
+ * Defaults of params_default (main_cuda.cu:16319-16347); CLI flags
+ * main_cuda.cu:23762-23851.  Composition is absent: the eigenstrain is
+ * h(phi) eps00 (eigenstrain_phi_only_point, cuda_kernels.cu:363-383). */
+typedef struct pf_minimize_opts {
+  double dt;                     /* --minimize-dt: pseudo-time step of the phi update */
+  int max_iter;                  /* --minimize-max-iter (2000) */
+  double V0;                     /* --V0: target <h>; <= 0 locks <h> of the first iteration */
+  double eta_lambda_vol;         /* --eta-lambda-vol: multiplier under-relaxation (0.2) */
+  int post_projection_iters;     /* --minimize-post-projection-iters (1) */
+  double rms_dphi_threshold;     /* --minimize-rms-dphi-threshold (1e-6) */
+  double rms_res_threshold;      /* --minimize-rms-res-threshold (1e-4) */
+  double vol_err_rel_threshold;  /* --minimize-vol-err-rel-threshold (1e-4) */
+  int convergence_steps;         /* --minimize-convergence-steps (10) */
+  double min_pseudo_time;        /* minimize_min_pseudo_time (0) */
+} pf_minimize_opts;
+
+/* One row of energy_minimize_*.csv (main_cuda.cu:32455-32489) for phi-only
+ * minimize, plus the convergence state (main_cuda.cu:32494-32560). */
+typedef struct pf_minimize_stats {
+  int iteration;
+  int converged;                 /* condA held for convergence_steps iterations */
+  int convergence_count;
+  int elastic_iterations;
+  double mean_h, V0, lambda_vol;
+  double F_surf_hat, F_el_hat;   /* <grad + dw>, <0.5 sigma:eps_el> of phi at the iteration start */
+  double interface_sum, el_core_sum;
+  double rms_res, rms_dphi, vol_err_rel;
+  double lambda_raw, lambda_kkt;
+} pf_minimize_stats;
+
+int pf_minimize_opts_default(pf_minimize_opts* o);
+
+/* Start phi-only minimize from the handle's current phi (the minimize branch
+ * of the step loop, main_cuda.cu:28341-28344, 29264-32600): resets the
+ * multiplier, the volume target and the elastic warm state. */
+int pf_minimize_begin(pf_handle* h, const pf_minimize_opts* o);
+
+/* One minimize iteration; st may be NULL. */
+int pf_minimize_iterate(pf_handle* h, pf_minimize_stats* st);
+
+/* pf_minimize_begin, then iterations until convergence or max_iter; st (may
+ * be NULL) receives the last iteration. */
+int pf_minimize(pf_handle* h, const pf_minimize_opts* o, pf_minimize_stats* st);
+
 /* Synchronise the handle's stream. */
 int pf_sync(pf_handle* h);
 

@@ -1,0 +1,207 @@
+"""Oracle (test infrastructure only): CPU FP64 restatement of ``main_cuda``'s
+phi-only minimize mode (``--mode=minimize`` without ``--minimize-full-model``).
+
+It is the checker for the HIP minimize path, never the thing measured or
+shipped: only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU
+leg may import it.
+
+One iteration follows ``main_cuda.cu:28427-32600`` with ``P.mode == 1``:
+
+1. elastic fixed point with composition absent (``xB_r == NULL``), i.e. the
+   eigenstrain h(phi) eps00 (``eigenstrain_phi_only_point``,
+   ``cuda_kernels.cu:363-383``) -- the dynamics solve with x_B = 0;
+2. energies of phi^n (``main_cuda.cu:29264-29346``): F_surf = <grad + dw>,
+   F_el = <0.5 sigma:eps_el>, sum(gel * h(clamp01 phi)); V0 is locked to
+   <h> at iteration 1 when it is not positive;
+3. phi with an under-relaxed volume multiplier (``main_cuda.cu:29357-29750``):
+   g_exp = W g'(phi) + dgel/dphi (``compute_phi_rhs_minimize_kernel``,
+   ``cuda_kernels.cu:6298-6368``), g_full = g_exp - kappa Lap phi,
+   lambda_raw = -sum(h' g_full) / sum(h'^2), lambda = (1-eta) lambda + eta
+   lambda_raw, phi^ <- (phi^ - L dt (g_exp + lambda h')^) / (1 + L dt kappa k^2),
+   clamp to [-1e-6, 1+1e-6];
+4. KKT residual at phi^{n+1} with the stress frozen and the exact multiplier,
+   projected at the bounds (``cuda_kernels.cu:684-706``); rms_dphi;
+5. post-projection sub-steps with eta = 1 (``main_cuda.cu:32104-32197``);
+6. first-order volume projection phi <- clamp01(phi + lc h'(phi)),
+   lc = (V0 - <h>) / <h'^2> (``main_cuda.cu:32274-32290``,
+   ``cuda_kernels.cu:639-647``);
+7. convergence: condA (rms_dphi, rms_res, vol_err_rel below their thresholds)
+   held for ``minimize_convergence_steps`` iterations; condB needs the excess
+   free energy, which phi-only minimize leaves NaN, so it never fires
+   (``main_cuda.cu:32400-32560``).
+
+Pinning (tests/test_oracle_minimize.py): the phi RHS against a numerical
+derivative of the elastic energy density at frozen total strain; the
+multiplier against the orthogonality sum(h' (g_full + lambda_raw h')) = 0;
+the volume projection against its first-order closed form; a uniform field
+as an exact stationary point; monotone decrease of the free energy for a
+small pseudo-time step without elasticity.
+"""
+import math
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import thermo as th
+from .step_ref import Oracle, Params, fwd, inv, voigt_apply
+
+
+@dataclass
+class MinimizeOptions:
+    """``params_default`` minimize entries (``main_cuda.cu:16319-16347``)."""
+    dt: float = 0.05
+    max_iter: int = 2000
+    V0: float = 0.0
+    eta_lambda_vol: float = 0.2
+    post_projection_iters: int = 1
+    rms_dphi_threshold: float = 1.0e-6
+    rms_res_threshold: float = 1.0e-4
+    vol_err_rel_threshold: float = 1.0e-4
+    convergence_steps: int = 10
+    min_pseudo_time: float = 0.0
+
+
+class MinimizeOracle:
+    def __init__(self, p: Params, phi0, opts: MinimizeOptions):
+        self.p = p
+        self.opts = opts
+        shape = (p.Nx, p.Ny, p.Nz)
+        self.o = Oracle(p, phi0, np.zeros(shape))
+        self.o.xB = np.zeros(shape)   # composition absent: eps0 = h eps00
+        self.shape = shape
+        self.lambda_vol = 0.0
+        self.V0_target = opts.V0
+        self.iteration = 0
+        self.conv_count = 0
+        self.converged = False
+        self.stats = {}
+
+    @property
+    def phi(self):
+        return self.o.phi
+
+    # ``compute_phi_rhs_minimize_kernel`` (cuda_kernels.cu:6298-6368) with the
+    # stress and total strain of the last elastic solve held fixed
+    def g_explicit(self, phi):
+        p, o = self.p, self.o
+        f = p.W * th.g_prime_of_phi(phi)
+        if p.elastic_enabled:
+            hp = th.h_prime_of_phi(phi)
+            s = o.sigma
+            e00 = p.eps00
+            dgel_eigen = -(s[0] * hp * e00[0] + s[1] * hp * e00[1] + s[2] * hp * e00[2]
+                           + 2.0 * (s[5] * hp * e00[5] + s[4] * hp * e00[4] + s[3] * hp * e00[3]))
+            h = th.h_of_phi(phi)
+            el = [o.eps[i] - h * e00[i] for i in range(6)]
+            q = voigt_apply(p.dC, el)
+            ee = [el[0], el[1], el[2], 2.0 * el[3], 2.0 * el[4], 2.0 * el[5]]
+            Q = sum(ee[i] * q[i] for i in range(6))
+            f = f + dgel_eigen + 0.5 * hp * Q
+        return f
+
+    def laplacian(self, phik):
+        """``compute_laplacian_k_kernel`` (cuda_kernels.cu:3528-3538), dealiased."""
+        return inv(-self.o.ks.k2 * phik * self.o.ks.mask, self.shape)
+
+    @staticmethod
+    def multiplier(phi, g):
+        hp = th.h_prime_of_phi(phi)
+        s_hg = math.fsum((hp * g).ravel())
+        s_h2 = math.fsum((hp * hp).ravel())
+        return (-s_hg / s_h2) if abs(s_h2) > 1e-30 else 0.0
+
+    def semi_implicit(self, phik, rhs):
+        p, ks = self.p, self.o.ks
+        Ldt = p.L_phi * self.opts.dt
+        out = (phik - Ldt * fwd(rhs, ks)) / (1.0 + Ldt * p.kappa_phi * ks.k2) * ks.mask
+        return np.clip(inv(out, self.shape), -1e-6, 1.0 + 1e-6)
+
+    def mean_h(self, phi):
+        return math.fsum(th.h_of_phi(th.clamp01(phi)).ravel()) / phi.size
+
+    def energies(self):
+        p, o = self.p, self.o
+        e = o.energies()
+        out = {"F_surf": e["grad"] + e["dw"], "F_el": e["elastic"],
+               "interface_sum": (e["grad"] + e["dw"]) * o.phi.size, "el_core_sum": 0.0}
+        if p.elastic_enabled:
+            h = th.h_of_phi(o.phi)
+            el = [o.eps[i] - h * p.eps00[i] for i in range(6)]
+            s = o.sigma
+            gel = 0.5 * (s[0] * el[0] + s[1] * el[1] + s[2] * el[2]
+                         + 2.0 * (s[5] * el[5] + s[4] * el[4] + s[3] * el[3]))
+            out["el_core_sum"] = math.fsum((gel * th.h_of_phi(th.clamp01(o.phi))).ravel())
+        return out
+
+    def iterate(self):
+        p, opts, o = self.p, self.opts, self.o
+        self.iteration += 1
+        n = o.phi.size
+        phi_n = o.phi.copy()
+        if p.elastic_enabled:
+            o.elastic_solve()
+        mean_h0 = self.mean_h(phi_n)
+        if self.iteration == 1 and self.V0_target <= 0.0:
+            self.V0_target = mean_h0
+        en = self.energies()
+        # phi with the under-relaxed multiplier
+        gexp = self.g_explicit(phi_n)
+        phik = fwd(phi_n, o.ks)
+        gfull = gexp - p.kappa_phi * self.laplacian(phik)
+        lam_raw = self.multiplier(phi_n, gfull)
+        eta = min(max(opts.eta_lambda_vol, 0.0), 1.0)
+        self.lambda_vol = (1.0 - eta) * self.lambda_vol + eta * lam_raw
+        phi = self.semi_implicit(phik, gexp + self.lambda_vol * th.h_prime_of_phi(phi_n))
+        # KKT residual at phi^{n+1}, exact multiplier, projected at the bounds
+        gexp1 = self.g_explicit(phi)
+        phik1 = fwd(phi, o.ks)
+        gfull1 = gexp1 - p.kappa_phi * self.laplacian(phik1)
+        lam_kkt = self.multiplier(phi, gfull1)
+        res = gfull1 + lam_kkt * th.h_prime_of_phi(phi)
+        res = np.where(((phi <= 1e-12) & (res > 0.0)) | ((phi >= 1.0 - 1e-12) & (res < 0.0)), 0.0, res)
+        rms_res = math.sqrt(math.fsum((res * res).ravel()) / n)
+        rms_dphi = math.sqrt(math.fsum(((phi - phi_n) ** 2).ravel()) / n) / opts.dt
+        # post-projection sub-steps (eta = 1), stress frozen
+        for k in range(max(opts.post_projection_iters, 0)):
+            if k == 0:
+                g_k, phik_k, gf_k = gexp1, phik1, gfull1
+            else:
+                g_k = self.g_explicit(phi)
+                phik_k = fwd(phi, o.ks)
+                gf_k = g_k - p.kappa_phi * self.laplacian(phik_k)
+            self.lambda_vol = self.multiplier(phi, gf_k)
+            phi = self.semi_implicit(phik_k, g_k + self.lambda_vol * th.h_prime_of_phi(phi))
+        mean_h = self.mean_h(phi)
+        # first-order volume projection
+        if self.V0_target > 1e-30 and math.isfinite(mean_h):
+            vol_err = self.V0_target - mean_h
+            if abs(vol_err) > 1e-14:
+                hp = th.h_prime_of_phi(phi)
+                mean_hp2 = math.fsum((hp * hp).ravel()) / n
+                lc = vol_err / mean_hp2 if mean_hp2 > 1e-30 else 0.0
+                phi = th.clamp01(phi + lc * hp)
+                mean_h = self.mean_h(phi)
+        o.phi = phi
+        if self.V0_target > 1e-30:
+            vol_err_rel = abs(mean_h - self.V0_target) / (self.V0_target + 1e-30)
+        else:
+            vol_err_rel = 0.0
+        condA = (math.isfinite(rms_dphi) and rms_dphi < opts.rms_dphi_threshold
+                 and math.isfinite(rms_res) and rms_res < opts.rms_res_threshold
+                 and math.isfinite(vol_err_rel) and vol_err_rel < opts.vol_err_rel_threshold)
+        t_ok = self.iteration * opts.dt + 1e-14 >= opts.min_pseudo_time
+        self.conv_count = self.conv_count + 1 if (condA and t_ok) else 0
+        if self.conv_count >= opts.convergence_steps:
+            self.converged = True
+        self.stats = dict(mean_h=mean_h, V0=self.V0_target, lambda_vol=self.lambda_vol,
+                          F_surf=en["F_surf"], F_el=en["F_el"], interface_sum=en["interface_sum"],
+                          el_core_sum=en["el_core_sum"], rms_res=rms_res, rms_dphi=rms_dphi,
+                          vol_err_rel=vol_err_rel, lambda_raw=lam_raw, lambda_kkt=lam_kkt,
+                          elastic_iterations=o.last_elastic_iterations)
+        return self.stats
+
+    def run(self, max_iter=None):
+        m = self.opts.max_iter if max_iter is None else max_iter
+        while self.iteration < m and not self.converged:
+            self.iterate()
+        return self
