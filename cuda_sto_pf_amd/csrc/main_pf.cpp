@@ -2,7 +2,7 @@
 //
 // Keeps the reference command line (main_cuda.cu:22457-22771):
 //   main_pf Nx Ny Nz dt nsteps out_every csv_out_every elastic_enabled
-//           --pf-param-file <file> [--mode=dynamics] [--init-mode raw_fields
+//           --pf-param-file <file> [--mode=dynamics|minimize] [--init-mode raw_fields
 //           --init-phi-raw f --init-xB-raw f --init-meta f [--init-dY-dt-prev-raw f]]
 //           [--pf-zero-mode PF_CONSERVED_Y_ZERO_MODE_V1|OFF] [--init-case-tag tag]
 //           [--precision fp64|mixed]
@@ -11,6 +11,9 @@
 // pf_input.params, vf_precip_vs_time_<tag>.csv, elastic_solver_trace.csv,
 // pf_zero_mode_trace.csv, performance_summary.{csv,json} and phi/xB VTKs
 // (io_vtk_cuda.h:35-131; CUDA_STO_SUPPRESS_VTK_OUTPUT=1 suppresses them).
+// --mode=minimize runs phi-only minimize (main_cuda.cu:28341-32600) and writes
+// energy_minimize_T<T>_r<r>_xB<xB>_<tag>.csv and phi_<iter>_<tag>.vtk into
+// <prefix>_T<T>_cuda_<Nx>x<Ny>x<Nz>_dt<dt>_steps<n>_r<R>nm_xB<xB>/<tag>/.
 // Errors print "[fatal] ..." and exit 2, as the reference does.
 #include <sys/stat.h>
 
@@ -140,12 +143,81 @@ void usage() {
       "    dt from --pf-param-file takes precedence over the positional dt.\n"
       "Flags:\n"
       "  --pf-param-file <path>   required: complete physical PF inputs (key=value)\n"
-      "  --mode=dynamics          (minimize modes are not part of this build)\n"
+      "  --mode=dynamics|minimize (phi-only minimize; the full-model and continue modes are not built)\n"
+      "  --minimize-max-iter <n> --minimize-dt <dt> --V0 <v> --eta-lambda-vol <v>\n"
+      "  --minimize-post-projection-iters <n> --minimize-rms-dphi-threshold <v>\n"
+      "  --minimize-rms-res-threshold <v> --minimize-vol-err-rel-threshold <v>\n"
+      "  --minimize-convergence-steps <n> --minimize-min-pseudo-time <t> --radius <r>\n"
       "  --init-mode raw_fields --init-phi-raw <f> --init-xB-raw <f> --init-meta <f>\n"
       "  --init-dY-dt-prev-raw <f>\n"
       "  --pf-zero-mode PF_CONSERVED_Y_ZERO_MODE_V1|OFF\n"
       "  --init-case-tag <tag>\n"
       "  --precision fp64|mixed   (default fp64)\n");
+}
+
+// Phi-only minimize loop with the reference's energy log (main_cuda.cu:27440-27466,
+// 32455-32489), VTK outputs (build_case_vtk_path, main_cuda.cu:126-170) and
+// convergence report (main_cuda.cu:32546-32600).
+int run_minimize(pf_handle* h, const pf_params& P, const pf_minimize_opts& mo, const std::string& case_dir,
+                 const std::string& tag, double seed_radius, double xB_out) {
+  const size_t N = (size_t)P.Nx * P.Ny * P.Nz;
+  char name[512];
+  std::snprintf(name, sizeof name, "/energy_minimize_T%.0f_r%.2f_xB%.3f_%s.csv", P.temperature_C, seed_radius,
+                xB_out, tag.c_str());
+  FILE* efp = std::fopen((case_dir + name).c_str(), "w");
+  if (efp)
+    std::fprintf(efp,
+                 "iter,dt,mean_h,V0,lambda,F_surf_hat,F_el_hat,F_chem_excess_hat,F_total_excess_hat,"
+                 "F_chem_CNT_hat,F_total_CNT_hat,total_interface_sum,total_el_core_sum,"
+                 "rms_res,rms_dphi,rms_dY,rel_dF,vol_err_rel,mass_target_code,mass_current_code,mass_err_rel,"
+                 "mass_lambda,post_proj_iters_used\n");
+  auto dump = [&](const std::string& sfx, int step) {
+    std::vector<double> f(N);
+    pf_get_field(h, PF_PHI, f.data());
+    write_vtk(case_dir + "/phi_" + sfx + "_" + tag + ".vtk", f, P.Nx, P.Ny, P.Nz, "phi", step);
+  };
+  dump("init", 0);
+  if (pf_minimize_begin(h, &mo) != PF_OK) return fatal(lib_error());
+  pf_minimize_stats st{};
+  const double t0 = now_s();
+  for (int it = 1; it <= mo.max_iter; ++it) {
+    if (pf_minimize_iterate(h, &st) != PF_OK) {
+      if (efp) std::fclose(efp);
+      return fatal(lib_error() + " (minimize iteration " + std::to_string(it) + ")");
+    }
+    if (efp) {
+      const double nan = NAN;
+      std::fprintf(efp,
+                   "%d,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,"
+                   "%.17e,%.17e,%.17e,%.17e,%d\n",
+                   it, mo.dt, st.mean_h, st.V0, st.lambda_vol, st.F_surf_hat, st.F_el_hat, nan, nan, nan, nan,
+                   st.interface_sum, st.el_core_sum, st.rms_res, st.rms_dphi, nan, nan, st.vol_err_rel, nan, nan,
+                   nan, nan, mo.post_projection_iters);
+      std::fflush(efp);
+    }
+    if (P.out_every > 0 && it % P.out_every == 0) dump(std::to_string(it), it);
+    if (st.converged) {
+      std::printf("[minimize] converged iter=%d trigger=condA rms_res=%.3e rms_dphi=%.3e vol_err_rel=%.3e\n", it,
+                  st.rms_res, st.rms_dphi, st.vol_err_rel);
+      break;
+    }
+  }
+  if (!st.converged)
+    std::printf("[minimize] reached max_iter=%d without convergence (rms_res=%.3e rms_dphi=%.3e)\n", mo.max_iter,
+                st.rms_res, st.rms_dphi);
+  dump("final", st.iteration);
+  if (efp) std::fclose(efp);
+  FILE* pj = std::fopen((case_dir + "/minimize_summary.json").c_str(), "w");
+  if (pj) {
+    std::fprintf(pj,
+                 "{\n  \"case_name\": \"%s\",\n  \"converged\": %d,\n  \"iterations\": %d,\n"
+                 "  \"final_rms_res\": %.17e,\n  \"final_rms_dphi\": %.17e,\n  \"final_vol_err_rel\": %.17e,\n"
+                 "  \"mean_h\": %.17e,\n  \"V0\": %.17e,\n  \"walltime_s\": %.9f\n}\n",
+                 tag.c_str(), st.converged, st.iteration, st.rms_res, st.rms_dphi, st.vol_err_rel, st.mean_h, st.V0,
+                 now_s() - t0);
+    std::fclose(pj);
+  }
+  return 0;
 }
 
 }  // namespace
@@ -155,6 +227,10 @@ int main(int argc, char** argv) {
   std::string mode = "dynamics", init_mode, phi_raw, xB_raw, meta_path, dydt_raw, case_tag;
   std::string zero_mode_sel = "OFF";
   int precision = PF_FP64;
+  pf_minimize_opts mo;
+  pf_minimize_opts_default(&mo);
+  bool minimize_dt_set = false, max_iter_set = false;
+  double seed_radius = 10.0;   // ic_phi_seed_radius default (main_cuda.cu:16221)
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
     auto next = [&](std::string* dst) {
@@ -193,13 +269,52 @@ int main(int argc, char** argv) {
         precision = PF_MIXED;
       else
         return fatal("--precision must be fp64 or mixed");
+    } else if (a == "--minimize-max-iter" || a == "--minimize-dt" || a == "--V0" || a == "--eta-lambda-vol" ||
+               a == "--eta_lambda_vol" || a == "--minimize-post-projection-iters" ||
+               a == "--minimize-rms-dphi-threshold" || a == "--minimize-rms-res-threshold" ||
+               a == "--minimize-vol-err-rel-threshold" || a == "--minimize-convergence-steps" ||
+               a == "--minimize-min-pseudo-time" || a == "--radius") {
+      // main_cuda.cu:23762-23862
+      std::string v;
+      if (!next(&v)) return fatal(a + " needs a value");
+      const double x = std::atof(v.c_str());
+      if (a == "--minimize-max-iter") {
+        mo.max_iter = std::atoi(v.c_str());
+        max_iter_set = true;
+      } else if (a == "--minimize-dt") {
+        mo.dt = x;
+        minimize_dt_set = true;
+      } else if (a == "--V0") {
+        mo.V0 = x;
+      } else if (a == "--eta-lambda-vol" || a == "--eta_lambda_vol") {
+        mo.eta_lambda_vol = x;
+      } else if (a == "--minimize-post-projection-iters") {
+        mo.post_projection_iters = std::max(0, std::atoi(v.c_str()));
+      } else if (a == "--minimize-rms-dphi-threshold") {
+        mo.rms_dphi_threshold = x;
+      } else if (a == "--minimize-rms-res-threshold") {
+        mo.rms_res_threshold = x;
+      } else if (a == "--minimize-vol-err-rel-threshold") {
+        mo.vol_err_rel_threshold = x;
+      } else if (a == "--minimize-convergence-steps") {
+        mo.convergence_steps = std::atoi(v.c_str());
+      } else if (a == "--minimize-min-pseudo-time") {
+        mo.min_pseudo_time = x;
+      } else {
+        seed_radius = x;
+      }
+    } else if (a == "--minimize-full-model" || a == "--minimize-full" || a == "--minimize-mass-constraint" ||
+               a == "--minimize-freeze-phi" || a == "--minimize-component-volume-constraints") {
+      return fatal(a + " is not part of this build (phi-only minimize)");
     } else if (starts_with(a.c_str(), "--")) {
       std::fprintf(stderr, "[warn] flag not handled by this build: %s\n", a.c_str());
     }
   }
   if (!param_file)
     return fatal("physical parameters have no built-in defaults: pass --pf-param-file <path>");
-  if (mode != "dynamics") return fatal("--mode=" + mode + " is not part of this build (dynamics only)");
+  if (mode != "dynamics" && mode != "minimize")
+    return fatal("--mode=" + mode + " is not part of this build (dynamics and phi-only minimize)");
+  const bool minimize = mode == "minimize";
 
   pf_params P;
   pf_params_default(&P);
@@ -218,6 +333,15 @@ int main(int argc, char** argv) {
   if (P.pf_params_schema_version != 1)
     return fatal("pf_params_schema_version mismatch: file=" + std::to_string(P.pf_params_schema_version) +
                  " expected=1");
+  if (minimize) {
+    // P.nsteps = minimize_max_iter, P.dt = minimize_dt, the latter defaulting
+    // to the physical-input dt (main_cuda.cu:22540-22554, 24432-24435)
+    if (!minimize_dt_set) mo.dt = P.dt;
+    if (!max_iter_set) mo.max_iter = 2000;
+    P.nsteps = mo.max_iter;
+    P.dt = mo.dt;
+    P.mode = 1;
+  }
   if (zero_mode_sel == "PF_CONSERVED_Y_ZERO_MODE_V1")
     P.zero_mode = 1;
   else if (zero_mode_sel == "OFF")
@@ -238,7 +362,11 @@ int main(int argc, char** argv) {
     if (meta_path.empty() || !read_meta(meta_path, &m)) return fatal("cannot read init meta: " + meta_path);
     if (m.Nx != P.Nx || m.Ny != P.Ny || m.Nz != P.Nz) return fatal("init meta grid does not match the run grid");
     if (!read_raw(phi_raw, m.dtype, N, &phi)) return fatal("cannot read phi raw field: " + phi_raw);
-    if (!read_raw(xB_raw, m.dtype, N, &xB)) return fatal("cannot read xB raw field: " + xB_raw);
+    if (minimize && xB_raw.empty()) {
+      xB.assign(N, 0.0);   // phi-only minimize allocates no composition (main_cuda.cu:2609)
+    } else if (!read_raw(xB_raw, m.dtype, N, &xB)) {
+      return fatal("cannot read xB raw field: " + xB_raw);
+    }
     if (!dydt_raw.empty() && !read_raw(dydt_raw, m.dtype, N, &dydt))
       return fatal("cannot read dY_dt_prev raw field: " + dydt_raw);
     xB_cap = std::isfinite(m.xB_max_safe) ? m.xB_max_safe : 0.07;   // minimize_xB_max_safe default
@@ -258,15 +386,29 @@ int main(int argc, char** argv) {
   const char* root_env = std::getenv("CUDA_STO_RESULTS_ROOT");
   const std::string root = (root_env && root_env[0]) ? root_env : "Results";
   char run_dir[512];
-  std::snprintf(run_dir, sizeof run_dir, "%s_T%.0f_cuda_%dx%dx%d_dt%.3g_steps%d_xB%.3f",
-                P.elastic_enabled ? "chel" : "ch", P.temperature_C, P.Nx, P.Ny, P.Nz, P.dt, P.nsteps, xB_out);
+  if (minimize) {
+    // the seed radius in nm from the physical dx (main_cuda.cu:25283-25396)
+    const double dx_m = P.ic_phi_iface_w > 1e-30 ? P.lambda_sm_m / (2.0 * P.ic_phi_iface_w) : P.dx * 1e-9;
+    const double unit_to_m = std::fabs(P.dx) > 1e-30 ? dx_m / P.dx : 1e-9;
+    std::snprintf(run_dir, sizeof run_dir, "%s_T%.0f_cuda_%dx%dx%d_dt%.3g_steps%d_r%.3fnm_xB%.3f",
+                  P.elastic_enabled ? "chel" : "ch", P.temperature_C, P.Nx, P.Ny, P.Nz, P.dt, P.nsteps,
+                  seed_radius * unit_to_m * 1e9, xB_out);
+  } else {
+    std::snprintf(run_dir, sizeof run_dir, "%s_T%.0f_cuda_%dx%dx%d_dt%.3g_steps%d_xB%.3f",
+                  P.elastic_enabled ? "chel" : "ch", P.temperature_C, P.Nx, P.Ny, P.Nz, P.dt, P.nsteps, xB_out);
+  }
   const std::string out_dir = root + "/" + run_dir;
   const std::string case_dir = out_dir + "/" + case_tag;
   mkdir(root.c_str(), 0755);
   mkdir(out_dir.c_str(), 0755);
   mkdir(case_dir.c_str(), 0755);
-  copy_text_file(param_file, out_dir + "/pf_input.params");
+  if (!minimize) copy_text_file(param_file, out_dir + "/pf_input.params");
   copy_text_file(param_file, case_dir + "/pf_input.params");
+  if (minimize) {
+    const int rc = run_minimize(h, P, mo, case_dir, case_tag, seed_radius, xB_out);
+    pf_destroy(h);
+    return rc;
+  }
 
   FILE* csv = std::fopen((case_dir + "/vf_precip_vs_time_" + case_tag + ".csv").c_str(), "w");
   if (csv) std::fprintf(csv, "step,t_code,t_real_s,vf_precip,R_avg\n");

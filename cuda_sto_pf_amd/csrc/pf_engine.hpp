@@ -23,6 +23,7 @@
 #include "pf_error.hpp"
 #include "pf_ops.hpp"
 #include "pf_minimize.hpp"
+#include "pf_eta.hpp"
 #include "pf_regfft.hpp"
 
 #define PF_HIP(call)                                                                  \
@@ -1278,6 +1279,47 @@ struct Engine final : EngineBase {
     }
     if (!std::isfinite(mean_h) || !std::isfinite(m_lambda))
       return fail(PF_E_NONFINITE, "minimize produced a non-finite phi");
+    return PF_OK;
+  }
+
+  // ---------------------------------------------------------------- GP eta
+  double* eta_buf = nullptr;   // eta, phi, x_B(alpha), tr sigma, result: 5 x R
+  int eta_update(const pf_eta_params& q, const double* eta, const double* ph, const double* xa, const double* trs,
+                 double* out) override {
+    int rc;
+    if (!eta_buf && (rc = dalloc(&eta_buf, 5 * (size_t)R))) return rc;
+    double* b[5];
+    for (int i = 0; i < 5; ++i) b[i] = eta_buf + (size_t)i * R;
+    const size_t bytes = R * sizeof(double);
+    PF_HIP(hipMemcpyAsync(b[0], eta, bytes, hipMemcpyHostToDevice, stream));
+    PF_HIP(hipMemcpyAsync(b[1], ph, bytes, hipMemcpyHostToDevice, stream));
+    PF_HIP(hipMemcpyAsync(b[2], xa, bytes, hipMemcpyHostToDevice, stream));
+    if (trs) PF_HIP(hipMemcpyAsync(b[3], trs, bytes, hipMemcpyHostToDevice, stream));
+    EtaConst c{};
+    c.xB_gp = clamp_eps(q.gp_xB_fixed, 1.0e-12);
+    c.delta_g0 = q.gp_delta_g0;
+    c.W_eta = q.gp_W_eta;
+    c.eps_gp = q.gp_eps_iso;
+    c.el_scale = q.gp_elastic_derivative_scale;
+    c.nu_A = q.gp_reaction_nu_A;
+    c.nu_B = q.gp_reaction_nu_B;
+    c.mu_ref_raw = q.gp_mu_reference_raw;
+    c.Ldt = q.gp_L_eta * prm.dt;
+    c.raw_drive = q.gp_raw_reaction_drive_only;
+    c.raw_units = q.gp_raw_reaction_drive_use_raw_units_debug;
+    c.elastic = q.gp_elastic;
+    typename OpEtaRhs<T>::Args oa{P, c, b[0], b[1], b[2], trs ? b[3] : nullptr};
+    launch_zpass<T, 0, 1, OpEtaRhs<T>>(Nz, stream, zargs(0, 0, 0, 1), oa);
+    ypass(false, 0, 1);
+    xpass(false, 0, 1, Sc);
+    hipLaunchKernelGGL(kop_phi<T>, dim3(grid_for(nSc)), dim3(NT), 0, stream, kd(), nSc, sc(0),
+                       c.Ldt * q.gp_kappa_eta);
+    xpass(true, 0, 1, Sc);
+    ypass(true, 0, 1);
+    typename OpEtaFinish<T>::Args fa{b[4], T(1.0 / (double)R)};
+    launch_zpass<T, 1, 0, OpEtaFinish<T>>(Nz, stream, zargs(0, 1, 0, 0), fa);
+    PF_HIP(hipMemcpyAsync(out, b[4], bytes, hipMemcpyDeviceToHost, stream));
+    PF_HIP(hipStreamSynchronize(stream));
     return PF_OK;
   }
 

@@ -25,6 +25,8 @@ struct EngineBase {
   virtual int time_kernel(int id, int reps, double* us, double* bytes) = 0;
   virtual int minimize_begin(const pf_minimize_opts& o) = 0;
   virtual int minimize_iterate(pf_minimize_stats* st) = 0;
+  virtual int eta_update(const pf_eta_params& q, const double* eta, const double* phi, const double* xa,
+                         const double* trs, double* out) = 0;
 };
 
 std::unique_ptr<EngineBase> make_engine_f64();

@@ -260,6 +260,22 @@ int pf_minimize(pf_handle* h, const pf_minimize_opts* o, pf_minimize_stats* st) 
   return PF_OK;
 }
 
+int pf_eta_params_default(pf_eta_params* q) {
+  if (!q) return pf::fail(PF_E_ARG, "pf_eta_params_default: null");
+  std::memset(q, 0, sizeof(*q));
+  q->gp_xB_fixed = 0.3529411764705882;
+  q->gp_elastic_derivative_scale = 1.0;
+  q->gp_reaction_nu_A = 0.65;
+  q->gp_reaction_nu_B = 0.35;
+  return PF_OK;
+}
+
+int pf_eta_update(pf_handle* h, const pf_eta_params* q, const double* eta, const double* phi,
+                  const double* xB_alpha, const double* tr_sigma_or_null, double* eta_out) {
+  if (!h || !q || !eta || !phi || !xB_alpha || !eta_out) return pf::fail(PF_E_ARG, "pf_eta_update: null");
+  return h->eng->eta_update(*q, eta, phi, xB_alpha, tr_sigma_or_null, eta_out);
+}
+
 int pf_sync(pf_handle* h) {
   if (!h) return pf::fail(PF_E_ARG, "pf_sync: null");
   return h->eng->sync();

@@ -99,6 +99,16 @@ class PfMinimizeStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class PfEtaParams(C.Structure):
+    _fields_ = [
+        ("gp_xB_fixed", C.c_double), ("gp_delta_g0", C.c_double), ("gp_W_eta", C.c_double),
+        ("gp_kappa_eta", C.c_double), ("gp_L_eta", C.c_double), ("gp_eps_iso", C.c_double),
+        ("gp_elastic_derivative_scale", C.c_double), ("gp_elastic", C.c_int),
+        ("gp_raw_reaction_drive_only", C.c_int), ("gp_raw_reaction_drive_use_raw_units_debug", C.c_int),
+        ("gp_reaction_nu_A", C.c_double), ("gp_reaction_nu_B", C.c_double), ("gp_mu_reference_raw", C.c_double),
+    ]
+
+
 EXPORTS = {
     "pf_version": (C.c_int, []),
     "pf_last_error": (C.c_int, [C.c_char_p, C.c_size_t]),
@@ -120,6 +130,9 @@ EXPORTS = {
     "pf_minimize_begin": (C.c_int, [C.c_void_p, C.POINTER(PfMinimizeOpts)]),
     "pf_minimize_iterate": (C.c_int, [C.c_void_p, C.POINTER(PfMinimizeStats)]),
     "pf_minimize": (C.c_int, [C.c_void_p, C.POINTER(PfMinimizeOpts), C.POINTER(PfMinimizeStats)]),
+    "pf_eta_params_default": (C.c_int, [C.POINTER(PfEtaParams)]),
+    "pf_eta_update": (C.c_int, [C.c_void_p, C.POINTER(PfEtaParams), C.c_void_p, C.c_void_p, C.c_void_p,
+                                C.c_void_p, C.c_void_p]),
     "pf_destroy": (C.c_int, [C.c_void_p]),
     "pf_fft_forward": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "pf_fft_inverse": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
@@ -265,6 +278,20 @@ class Simulation:
         st = PfMinimizeStats()
         _check(lib().pf_minimize(self._h, C.byref(opts), C.byref(st)))
         return st.as_dict()
+
+    def eta_update(self, q, eta, phi, xB_alpha, tr_sigma=None):
+        """One GP-zone eta update on this grid (pf_eta_update); q is a dict of
+        pf_eta_params fields over the defaults."""
+        qq = PfEtaParams()
+        _check(lib().pf_eta_params_default(C.byref(qq)))
+        for k, v in q.items():
+            setattr(qq, k, v)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(self.shape) for a in (eta, phi, xB_alpha)]
+        t = None if tr_sigma is None else np.ascontiguousarray(tr_sigma, dtype=np.float64).reshape(self.shape)
+        out = np.empty(self.shape, dtype=np.float64)
+        _check(lib().pf_eta_update(self._h, C.byref(qq), _ptr(arrs[0]), _ptr(arrs[1]), _ptr(arrs[2]),
+                                   None if t is None else _ptr(t), _ptr(out)))
+        return out
 
     def time_kernel(self, kernel_id, reps=20):
         """Mean microseconds and bytes of one launch of a step kernel (HIP events)."""

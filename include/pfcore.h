@@ -215,6 +215,27 @@ int pf_minimize_iterate(pf_handle* h, pf_minimize_stats* st);
  * be NULL) receives the last iteration. */
 int pf_minimize(pf_handle* h, const pf_minimize_opts* o, pf_minimize_stats* st);
 
+/* GP-zone order parameter (not exercised by the two-phase workloads; kernel
+ * level).  Keys of params_default (main_cuda.cu:15793-15828, 16164-16167). */
+typedef struct pf_eta_params {
+  double gp_xB_fixed, gp_delta_g0, gp_W_eta, gp_kappa_eta, gp_L_eta;
+  double gp_eps_iso, gp_elastic_derivative_scale;
+  int gp_elastic;                          /* gp elastic solver on and eta coupling active */
+  int gp_raw_reaction_drive_only, gp_raw_reaction_drive_use_raw_units_debug;
+  double gp_reaction_nu_A, gp_reaction_nu_B;
+  double gp_mu_reference_raw;              /* compute_gp_mu_reference_mech_mix_host result */
+} pf_eta_params;
+
+int pf_eta_params_default(pf_eta_params* q);
+
+/* One eta update on the handle's grid and dt (main_cuda.cu:29948-30006):
+ * compute_eta_rhs_kernel (cuda_kernels.cu:845-946), the dealiased
+ * semi-implicit update eta_semi_implicit_update_kernel (1045-1069) and
+ * eta_normalize_and_clamp_kernel (1086-1098).  Host arrays of N doubles;
+ * tr_sigma may be NULL (zero stress).  The simulation state is not touched. */
+int pf_eta_update(pf_handle* h, const pf_eta_params* q, const double* eta, const double* phi,
+                  const double* xB_alpha, const double* tr_sigma_or_null, double* eta_out);
+
 /* Synchronise the handle's stream. */
 int pf_sync(pf_handle* h);
 

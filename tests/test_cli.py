@@ -80,3 +80,41 @@ def test_cli_raw_fields_run_matches_library(pf, tmp_path):
     txt = (case / "phi_final.vtk").read_text().split("LOOKUP_TABLE default\n")[1].split()
     vals = np.array([float(v) for v in txt]).reshape(16, 16, 16)   # z, y, x order
     assert np.abs(vals.transpose(2, 1, 0) - ref).max() <= 6e-6
+
+
+def test_minimize_full_model_is_refused(tmp_path):
+    args, *_ = _bundle(tmp_path)
+    r = _run(args + ["--mode=minimize", "--minimize-full-model"], tmp_path)
+    assert r.returncode == 2 and "[fatal]" in r.stderr
+
+
+@pytest.mark.gpu
+def test_cli_minimize_matches_library(pf, tmp_path):
+    """--mode=minimize: the energy log of main_cuda (energy_minimize_*.csv,
+    main_cuda.cu:27440-27466) and phi VTKs with the case tag; the logged
+    multiplier and volume equal the library's own minimize run."""
+    args, pd, phi, xB, meta = _bundle(tmp_path, steps=4, zero_mode=False)
+    V0 = 0.05
+    args = args + ["--mode=minimize", "--minimize-max-iter", "5", "--minimize-dt", "0.05", "--V0", str(V0),
+                   "--init-case-tag", "mincase"]
+    r = _run(args, tmp_path, {"CUDA_STO_SUPPRESS_VTK_OUTPUT": "0"})
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0, r.stderr
+    runs = os.listdir(tmp_path / "Results")
+    assert len(runs) == 1 and "_steps5_r" in runs[0] and "_dt0.05_" in runs[0]
+    case = tmp_path / "Results" / runs[0] / "mincase"
+    logs = [f for f in os.listdir(case) if f.startswith("energy_minimize_T380_")]
+    assert len(logs) == 1 and logs[0].endswith("_mincase.csv")
+    rows = (case / logs[0]).read_text().splitlines()
+    assert rows[0].startswith("iter,dt,mean_h,V0,lambda,F_surf_hat") and len(rows) == 1 + 5
+    for f in ("phi_init_mincase.vtk", "phi_final_mincase.vtk", "phi_2_mincase.vtk", "minimize_summary.json"):
+        assert (case / f).exists(), f
+    sim = pf.Simulation(pd, pf.PF_FP64)
+    sim.set_fields(phi, xB, xB_cap=meta["xB_max_safe"])
+    sim.minimize_begin(pf.minimize_opts(dt=0.05, V0=V0, max_iter=5))
+    for i in range(5):
+        st = sim.minimize_iterate()
+        cols = rows[1 + i].split(",")
+        assert int(cols[0]) == i + 1
+        assert abs(float(cols[2]) - st["mean_h"]) <= 1e-8 * abs(st["mean_h"])
+        assert abs(float(cols[4]) - st["lambda_vol"]) <= 1e-8 * abs(st["lambda_vol"])
