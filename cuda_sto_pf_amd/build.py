@@ -17,7 +17,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpfcore.so")
 CLI = os.path.join(HERE, "bin", "main_pf")
 
-HIP_SOURCES = ["pf_engine_f64.hip", "pf_engine_f32.hip", "pf_runtime.hip"]
+HIP_SOURCES = ["pf_engine_f64.hip", "pf_engine_f32.hip", "pf_runtime.hip", "pf_nn.hip"]
 CXX_SOURCES = ["pf_params.cpp"]
 CLI_SOURCES = ["main_pf.cpp"]
 

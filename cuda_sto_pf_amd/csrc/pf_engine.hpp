@@ -240,6 +240,95 @@ struct OpStoreReal {
   __device__ static void apply(const Args& a, size_t g, const double*, const T* vin, T*, double*) { a.f[g] = double(vin[0]); }
 };
 
+// ------------------------------------------------------------ learned warm start
+// Network input at each voxel: phi s_phi and the eigenstress
+// (C + h dC):eps0 s_tau, BF16 channels-last [S][16].
+namespace {
+__global__ void __launch_bounds__(NT) nn_features_kernel(PhysParams P, const double* phi, const double* xB, long n,
+                                                         double s_phi, double s_tau, uint16_t* X) {
+  for (long e = (long)blockIdx.x * NT + threadIdx.x; e < n; e += (long)gridDim.x * NT) {
+    const double ph = phi[e];
+    const double h = h_of_phi(ph);
+    double e0[6], tau[6];
+    eigenstrain(P, h, clamp01(xB[e]), e0);
+    voigt_apply_h<double>(P, h, e0, tau);
+    uint16_t o[16];
+    o[0] = nn::f2bf((float)(ph * s_phi));
+#pragma unroll
+    for (int c = 0; c < 6; ++c) o[1 + c] = nn::f2bf((float)(tau[c] * s_tau));
+#pragma unroll
+    for (int c = 7; c < 16; ++c) o[c] = 0;
+    uint4* dst = reinterpret_cast<uint4*>(X + (size_t)e * 16);
+    dst[0] = make_uint4(nn::pk(o[0], o[1]), nn::pk(o[2], o[3]), nn::pk(o[4], o[5]), nn::pk(o[6], o[7]));
+    dst[1] = make_uint4(nn::pk(o[8], o[9]), nn::pk(o[10], o[11]), nn::pk(o[12], o[13]), nn::pk(o[14], o[15]));
+  }
+}
+}  // namespace
+
+// three FP32 planar real fields [3][R] (times scale) -> forward z of 3 components
+template <typename T>
+struct OpLoadF3 {
+  struct Args {
+    const float* y3;
+    size_t R;
+    double scale;
+  };
+  static constexpr int NRED = 0;
+  static constexpr int kinds[1] = {0};
+  static constexpr int NPRE = 3;
+  __device__ static void load(const Args& a, size_t g, double* pre) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) pre[c] = (double)a.y3[c * a.R + g];
+  }
+  __device__ static void apply(const Args& a, size_t, const double* pre, const T*, T* out, double*) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) out[c] = T(pre[c] * a.scale);
+  }
+};
+// inverse z of 3 components -> FP32 planar real fields [3][R]
+template <typename T>
+struct OpStoreF3 {
+  struct Args {
+    float* y3;
+    size_t R;
+    T invN;
+  };
+  static constexpr int NRED = 0;
+  static constexpr int kinds[1] = {0};
+  static constexpr int NPRE = 0;
+  __device__ static void load(const Args&, size_t, double*) {}
+  __device__ static void apply(const Args& a, size_t g, const double*, const T* vin, T*, double*) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) a.y3[c * a.R + g] = (float)(double(vin[c]) * double(a.invN));
+  }
+};
+
+// u_i += d_i on the compact spectrum, k = 0 excluded (the Green operator's
+// zero mode stays 0)
+template <typename T, typename S>
+__global__ void __launch_bounds__(NT) kop_add3(long n, const cx<T>* d0, const cx<T>* d1, const cx<T>* d2, S* u0,
+                                               S* u1, S* u2) {
+  for (long c = (long)blockIdx.x * NT + threadIdx.x; c < n; c += (long)gridDim.x * NT) {
+    if (c == 0) continue;
+    u0[c].re += d0[c].re;
+    u0[c].im += d0[c].im;
+    u1[c].re += d1[c].re;
+    u1[c].im += d1[c].im;
+    u2[c].re += d2[c].re;
+    u2[c].im += d2[c].im;
+  }
+}
+// d_i = u_i - v_i
+template <typename T, typename S>
+__global__ void __launch_bounds__(NT) kop_diff3(long n, const S* u0, const S* u1, const S* u2, const cx<T>* v0,
+                                                const cx<T>* v1, const cx<T>* v2, cx<T>* d0, cx<T>* d1, cx<T>* d2) {
+  for (long c = (long)blockIdx.x * NT + threadIdx.x; c < n; c += (long)gridDim.x * NT) {
+    d0[c] = cx<T>{T(double(u0[c].re) - double(v0[c].re)), T(double(u0[c].im) - double(v0[c].im))};
+    d1[c] = cx<T>{T(double(u1[c].re) - double(v1[c].re)), T(double(u1[c].im) - double(v1[c].im))};
+    d2[c] = cx<T>{T(double(u2[c].re) - double(v2[c].re)), T(double(u2[c].im) - double(v2[c].im))};
+  }
+}
+
 // ------------------------------------------------------------ dispatch
 template <typename T, int N, int NCI, int NCO>
 constexpr int zpass_rows_for() {
@@ -368,6 +457,14 @@ struct Engine final : EngineBase {
   int m_iter = 0, m_conv = 0;
   bool m_converged = false;
   const double* xB_el() const { return minimizing ? m_xzero : xB; }
+  // cold-start hooks: learned correction and caller callback
+  nn::Net* nn_net = nullptr;
+  uint16_t *nn_x0 = nullptr, *nn_a = nullptr, *nn_b = nullptr;
+  float* nn_y3 = nullptr;
+  pf_warm_start_fn ws_fn = nullptr;
+  void* ws_user = nullptr;
+  pf_handle* ws_handle = nullptr;
+  bool in_cold_hook = false;
 
   ~Engine() override {
     for (void* q : allocations) (void)hipFree(q);
@@ -618,6 +715,16 @@ struct Engine final : EngineBase {
       for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
       ga.mode = 0;
       hipLaunchKernelGGL(kop_green_acc<T>, dim3(gk), dim3(NT), 0, stream, ga);
+      if (cold_hooks()) {
+        int rc = cold_hook();
+        if (rc) return rc;
+        StrainArgs<T, double> sa{};
+        sa.d = d;
+        sa.n = nSc;
+        for (int i = 0; i < 3; ++i) sa.u[i] = Ud + (size_t)i * nSc;
+        for (int i = 0; i < 6; ++i) sa.eps[i] = sc(i);
+        hipLaunchKernelGGL((kop_strain<T, double>), dim3(gk), dim3(NT), 0, stream, sa);
+      }
     }
     double H[AA_MAX][AA_MAX] = {};
     double bvec[AA_MAX] = {};
@@ -779,6 +886,11 @@ struct Engine final : EngineBase {
       ypass(false, 0, 6);
       xpass(false, 0, 6, Sc);
       green(0, false);
+      if (cold_hooks()) {
+        int rc = cold_hook();
+        if (rc) return rc;
+        strain_of_U();
+      }
     } else {
       strain_of_U();
     }
@@ -865,6 +977,16 @@ struct Engine final : EngineBase {
       for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
       ga.have_old = 0;
       hipLaunchKernelGGL(kop_green<T>, dim3(gk), dim3(NT), 0, stream, ga);
+      if (cold_hooks()) {
+        int rc = cold_hook();
+        if (rc) return rc;
+        StrainArgs<T> sa{};
+        sa.d = d;
+        sa.n = nSc;
+        for (int i = 0; i < 3; ++i) sa.u[i] = uu(u, i);
+        for (int i = 0; i < 6; ++i) sa.eps[i] = sc(i);
+        hipLaunchKernelGGL(kop_strain<T>, dim3(gk), dim3(NT), 0, stream, sa);
+      }
     } else {
       StrainArgs<T> sa{};
       sa.d = d;
@@ -1091,6 +1213,138 @@ struct Engine final : EngineBase {
     zm_lambda = lam;
     hipLaunchKernelGGL(zm_apply_kernel, dim3(g), dim3(NT), 0, stream, Ystar, lam, Y, xB, dYdt, prm.dt, R);
     return PF_OK;
+  }
+
+  // ---------------------------------------------------------------- cold-start hooks
+  bool state_fp64() const { return kMixed || extended; }   // warm state in Ud (FP64) or u (T)
+  bool cold_hooks() const { return nn_net != nullptr || ws_fn != nullptr; }
+
+  int nn_buffers() {
+    int rc;
+    if (!nn_x0 && ((rc = dalloc(&nn_x0, 16 * (size_t)R)) || (rc = dalloc(&nn_a, 16 * (size_t)R)) ||
+                   (rc = dalloc(&nn_b, 16 * (size_t)R)) || (rc = dalloc(&nn_y3, 3 * (size_t)R))))
+      return rc;
+    return PF_OK;
+  }
+
+  // network prediction of the correction, transformed to sc(0..2)
+  int nn_predict_to_sc() {
+    int rc;
+    if ((rc = nn_buffers())) return rc;
+    double sp, st, so;
+    nn::scales(nn_net, &sp, &st, &so);
+    hipLaunchKernelGGL(nn_features_kernel, dim3(grid_for(R)), dim3(NT), 0, stream, P, (const double*)phi[cur],
+                       xB_el(), R, sp, st, nn_x0);
+    if ((rc = nn::infer(nn_net, stream, Nx, Ny, Nz, nn_x0, nn_a, nn_b, nn_y3))) return rc;
+    typename OpLoadF3<T>::Args oa{nn_y3, (size_t)R, so};
+    launch_zpass<T, 0, 3, OpLoadF3<T>>(Nz, stream, zargs(0, 0, 0, 3), oa);
+    ypass(false, 0, 3);
+    xpass(false, 0, 3, Sc);
+    return PF_OK;
+  }
+
+  int cold_hook() {
+    const int gk = grid_for(nSc);
+    if (nn_net) {
+      int rc = nn_predict_to_sc();
+      if (rc) return rc;
+      if (state_fp64())
+        hipLaunchKernelGGL((kop_add3<T, cx<double>>), dim3(gk), dim3(NT), 0, stream, nSc, (const C*)sc(0),
+                           (const C*)sc(1), (const C*)sc(2), Ud, Ud + nSc, Ud + 2 * nSc);
+      else
+        hipLaunchKernelGGL((kop_add3<T, C>), dim3(gk), dim3(NT), 0, stream, nSc, (const C*)sc(0), (const C*)sc(1),
+                           (const C*)sc(2), uu(u, 0), uu(u, 1), uu(u, 2));
+    }
+    if (ws_fn) {
+      PF_HIP(hipStreamSynchronize(stream));
+      in_cold_hook = true;
+      const int r = ws_fn(ws_handle, ws_user);
+      in_cold_hook = false;
+      if (r) return fail(PF_E_ARG, "warm-start callback returned " + std::to_string(r));
+    }
+    return PF_OK;
+  }
+
+  int set_warm_start(const double* in) override {
+    if (!in) {
+      if (in_cold_hook) return PF_OK;
+      warm_valid = false;
+      hist_count = 0;
+      hist_head = 0;
+      return PF_OK;
+    }
+    if (!in_cold_hook) return set_elastic_state(in);
+    // inside a cold start: replace the start of this solve only
+    if (state_fp64()) {
+      PF_HIP(hipMemcpyAsync(Ud, in, 3 * nSc * sizeof(cx<double>), hipMemcpyHostToDevice, stream));
+    } else {
+      std::vector<C> t(3 * nSc);
+      for (long i = 0; i < 3 * nSc; ++i) t[i] = C{T(in[2 * i]), T(in[2 * i + 1])};
+      PF_HIP(hipMemcpyAsync(u, t.data(), 3 * nSc * sizeof(C), hipMemcpyHostToDevice, stream));
+      PF_HIP(hipStreamSynchronize(stream));
+    }
+    return PF_OK;
+  }
+
+  int set_warm_start_callback(pf_warm_start_fn fn, void* user, pf_handle* h) override {
+    ws_fn = fn;
+    ws_user = user;
+    ws_handle = h;
+    return PF_OK;
+  }
+
+  int attach_nn(nn::Net* net) override {
+    nn_net = net;
+    return net ? nn_buffers() : PF_OK;
+  }
+
+  // Converge the solve for the current phi, then store the network input
+  // and the real-space correction (converged - homogeneous cold start).
+  int nn_collect(nn::Net* net) override {
+    uint16_t* feat;
+    float* tgt;
+    int rc;
+    if ((rc = nn::dataset_slot(net, Nx, Ny, Nz, &feat, &tgt))) return rc;
+    if (!P.elastic) return fail(PF_E_ARG, "pf_nn_collect: elasticity is disabled");
+    if ((rc = elastic_solve())) return rc;
+    double sp, st, so;
+    nn::scales(net, &sp, &st, &so);
+    hipLaunchKernelGGL(nn_features_kernel, dim3(grid_for(R)), dim3(NT), 0, stream, P, (const double*)phi[cur],
+                       xB_el(), R, sp, st, feat);
+    // homogeneous cold start into un
+    typename OpElasticCold<T>::Args oa{P, phi[cur], xB_el()};
+    launch_zpass<T, 0, 6, OpElasticCold<T>>(Nz, stream, zargs(0, 0, 0, 6), oa);
+    ypass(false, 0, 6);
+    xpass(false, 0, 6, Sc);
+    const KDims d = kd();
+    const int gk = grid_for(nSc);
+    GreenArgs<T> ga{};
+    ga.d = d;
+    ga.n = nSc;
+    for (int i = 0; i < 6; ++i) {
+      ga.h[i] = sc(i);
+      ga.eps[i] = sc(i);
+    }
+    for (int i = 0; i < 3; ++i) ga.u_new[i] = uu(un, i);
+    ga.partials = nullptr;
+    for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
+    ga.have_old = 0;
+    hipLaunchKernelGGL(kop_green<T>, dim3(gk), dim3(NT), 0, stream, ga);
+    if (state_fp64())
+      hipLaunchKernelGGL((kop_diff3<T, cx<double>>), dim3(gk), dim3(NT), 0, stream, nSc, (const cx<double>*)Ud,
+                         (const cx<double>*)(Ud + nSc), (const cx<double>*)(Ud + 2 * nSc), (const C*)uu(un, 0),
+                         (const C*)uu(un, 1), (const C*)uu(un, 2), sc(0), sc(1), sc(2));
+    else
+      hipLaunchKernelGGL((kop_diff3<T, C>), dim3(gk), dim3(NT), 0, stream, nSc, (const C*)uu(u, 0),
+                         (const C*)uu(u, 1), (const C*)uu(u, 2), (const C*)uu(un, 0), (const C*)uu(un, 1),
+                         (const C*)uu(un, 2), sc(0), sc(1), sc(2));
+    xpass(true, 0, 3, Sc);
+    ypass(true, 0, 3);
+    typename OpStoreF3<T>::Args sa{tgt, (size_t)R, T(1.0 / (double)R)};
+    launch_zpass<T, 3, 0, OpStoreF3<T>>(Nz, stream, zargs(0, 3, 0, 0), sa);
+    PF_HIP(hipStreamSynchronize(stream));
+    rc = nn::dataset_commit(net);
+    return rc < 0 ? rc : PF_OK;
   }
 
   // ---------------------------------------------------------------- minimize

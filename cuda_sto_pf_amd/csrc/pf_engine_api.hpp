@@ -3,6 +3,7 @@
 #include <memory>
 
 #include "../../include/pfcore.h"
+#include "pf_nn_api.hpp"
 #include "pf_physics.hpp"
 
 namespace pf {
@@ -25,6 +26,10 @@ struct EngineBase {
   virtual int time_kernel(int id, int reps, double* us, double* bytes) = 0;
   virtual int minimize_begin(const pf_minimize_opts& o) = 0;
   virtual int minimize_iterate(pf_minimize_stats* st) = 0;
+  virtual int set_warm_start(const double* u_or_null) = 0;
+  virtual int set_warm_start_callback(pf_warm_start_fn fn, void* user, pf_handle* h) = 0;
+  virtual int attach_nn(nn::Net* net) = 0;
+  virtual int nn_collect(nn::Net* net) = 0;
   virtual int eta_update(const pf_eta_params& q, const double* eta, const double* phi, const double* xa,
                          const double* trs, double* out) = 0;
 };

@@ -1,0 +1,257 @@
+// Learned elastic warm start: a 5-layer 3x3x3 convolutional network, 16
+// channels, BF16 operands on the CDNA matrix cores with FP32 accumulation.
+//
+// There is no counterpart in the reference: its warm start is the previous
+// step's displacement (main_cuda.cu:28760-28805) and a cold start is the
+// homogeneous Khachaturyan solution (compute_displacement_from_eigenstrain_k_kernel,
+// cuda_kernels.cu:6734-6991).  The network predicts the correction from that
+// homogeneous cold start to the converged heterogeneous displacement, so a
+// cold start can begin closer to the fixed point; the fixed-point iteration,
+// its residual and tolerance are unchanged, so converged results are too.
+//
+// Tensors are channels-last: a field of S = Nx*Ny*Nz voxels with 16 channels
+// is [S][16] BF16 (32 bytes per voxel).  Convolutions are periodic.
+// Weights of a layer are W[tap][cin][cout] (tap = 9(dx+1) + 3(dy+1) + (dz+1))
+// plus a bias[16]; the input layer uses cin < 7 (phi and six eigenstress
+// components), the output layer cout < 3 (displacement correction).
+//
+// MFMA mapping (v_mfma_f32_16x16x16_bf16, wave64): M = 16 consecutive z
+// voxels, N = 16 output channels, K = 16 input channels of one tap; 27 MFMAs
+// per 16-voxel tile.  Lane l holds A[l%16][4(l/16)..+3], B[4(l/16)..+3][l%16]
+// and D[4(l/16)..+3][l%16].  The B operands (weights) of all 27 taps live in
+// registers for the whole kernel; the compute copies are stored pre-arranged
+// in that lane order (8 contiguous bytes per lane and tap).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pf_nn_api.hpp"
+
+namespace pf {
+namespace nn {
+
+constexpr int CH = 16;
+constexpr int TAPS = 27;
+constexpr int LAYERS = 5;
+constexpr int IN_CH = 7;
+constexpr int OUT_CH = 3;
+constexpr int W_PER_LAYER = TAPS * CH * CH;         // 6912
+constexpr int P_PER_LAYER = W_PER_LAYER + CH;       // + bias
+constexpr int P_TOTAL = LAYERS * P_PER_LAYER;       // 34640
+constexpr float SLOPE = 0.1f;                        // leaky ReLU
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+struct Geom {
+  int B, Nx, Ny, Nz;
+  long S;   // Nx*Ny*Nz
+};
+
+__device__ __forceinline__ int wrap(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+enum Epilogue { EPI_ACT = 0, EPI_OUT3 = 1, EPI_GRAD = 2 };
+
+// Y = epilogue(conv(X, W) + bias):
+//   EPI_ACT  leaky ReLU, BF16 [S][16]
+//   EPI_OUT3 linear, FP32 planar [B][3][S] (channels 0..2)
+//   EPI_GRAD multiplied by the leaky-ReLU derivative of aux (the forward
+//            activation of the layer below), BF16 [S][16]; no bias
+// With the transposed, tap-flipped weights EPI_GRAD is the input gradient.
+template <int EPI>
+__global__ void __launch_bounds__(256) conv3_kernel(Geom g, const uint16_t* __restrict__ X,
+                                                    const uint16_t* __restrict__ Wb, const float* __restrict__ bias,
+                                                    uint16_t* __restrict__ Y, float* __restrict__ Y3,
+                                                    const uint16_t* __restrict__ aux) {
+  const int lane = threadIdx.x & 63;
+  const int row = lane & 15, kq = lane >> 4;
+  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long nwaves = ((long)gridDim.x * blockDim.x) >> 6;
+  const int zt_n = (g.Nz + 15) >> 4;
+  const long ntiles = (long)g.B * g.Nx * g.Ny * zt_n;
+  v4s w[TAPS];
+#pragma unroll
+  for (int t = 0; t < TAPS; ++t) w[t] = *reinterpret_cast<const v4s*>(Wb + ((size_t)t * 64 + lane) * 4);
+  const float bn = (EPI != EPI_GRAD && bias) ? bias[row] : 0.0f;
+  for (long tile = wave; tile < ntiles; tile += nwaves) {
+    const int zt = (int)(tile % zt_n);
+    long r = tile / zt_n;
+    const int y = (int)(r % g.Ny);
+    r /= g.Ny;
+    const int x = (int)(r % g.Nx);
+    const int b = (int)(r / g.Nx);
+    const int z0 = zt << 4;
+    const uint16_t* Xb = X + (size_t)b * g.S * CH;
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < TAPS; ++t) {
+      const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+      const int xx = wrap(x + dx, g.Nx), yy = wrap(y + dy, g.Ny);
+      int zz = z0 + row + dz;
+      zz = ((zz % g.Nz) + g.Nz) % g.Nz;
+      const v4s a = *reinterpret_cast<const v4s*>(Xb + (((size_t)xx * g.Ny + yy) * g.Nz + zz) * CH + kq * 4);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, w[t], acc, 0, 0, 0);
+    }
+    const size_t vbase = ((size_t)x * g.Ny + y) * g.Nz;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int z = z0 + kq * 4 + q;
+      if (z >= g.Nz) continue;
+      const size_t v = vbase + z;
+      float val = acc[q] + bn;
+      if constexpr (EPI == EPI_ACT) {
+        val = val > 0.f ? val : SLOPE * val;
+        Y[((size_t)b * g.S + v) * CH + row] = f2bf(val);
+      } else if constexpr (EPI == EPI_OUT3) {
+        if (row < OUT_CH) Y3[((size_t)b * OUT_CH + row) * g.S + v] = val;
+      } else {
+        const float a = bf2f(aux[((size_t)b * g.S + v) * CH + row]);
+        Y[((size_t)b * g.S + v) * CH + row] = f2bf(a > 0.f ? val : SLOPE * val);
+      }
+    }
+  }
+}
+
+// Per-wave partial weight and bias gradients of one layer:
+//   dW[t][cin][cout] = sum_v X[v + off(t)][cin] dY[v][cout],  db[cout] = sum_v dY[v][cout]
+// MFMA with M = cin, N = cout, K = 16 voxels of a z tile.  partials is
+// [nwaves][P_PER_LAYER], reduced in wave order by wgrad_reduce_kernel.
+__global__ void __launch_bounds__(256) conv3_wgrad_kernel(Geom g, const uint16_t* __restrict__ X,
+                                                          const uint16_t* __restrict__ dY, float* __restrict__ partials) {
+  const int lane = threadIdx.x & 63;
+  const int row = lane & 15, kq = lane >> 4;
+  const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const long nwaves = ((long)gridDim.x * blockDim.x) >> 6;
+  const int zt_n = (g.Nz + 15) >> 4;
+  const long ntiles = (long)g.B * g.Nx * g.Ny * zt_n;
+  v4f acc[TAPS];
+#pragma unroll
+  for (int t = 0; t < TAPS; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+  float bsum = 0.f;
+  for (long tile = wave; tile < ntiles; tile += nwaves) {
+    const int zt = (int)(tile % zt_n);
+    long r = tile / zt_n;
+    const int y = (int)(r % g.Ny);
+    r /= g.Ny;
+    const int x = (int)(r % g.Nx);
+    const int b = (int)(r / g.Nx);
+    const int z0 = zt << 4;
+    const uint16_t* Xb = X + (size_t)b * g.S * CH;
+    const uint16_t* dYb = dY + (size_t)b * g.S * CH;
+    const size_t vbase = ((size_t)x * g.Ny + y) * g.Nz;
+    v4s bop;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int z = z0 + kq * 4 + j;
+      const uint16_t v = (z < g.Nz) ? dYb[(vbase + z) * CH + row] : (uint16_t)0;
+      bop[j] = (short)v;
+      bsum += bf2f(v);
+    }
+#pragma unroll
+    for (int t = 0; t < TAPS; ++t) {
+      const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+      const int xx = wrap(x + dx, g.Nx), yy = wrap(y + dy, g.Ny);
+      const size_t rb = ((size_t)xx * g.Ny + yy) * g.Nz;
+      v4s a;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        int zz = z0 + kq * 4 + j + dz;
+        zz = ((zz % g.Nz) + g.Nz) % g.Nz;
+        a[j] = (short)Xb[(rb + zz) * CH + row];
+      }
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, bop, acc[t], 0, 0, 0);
+    }
+  }
+  float* out = partials + (size_t)wave * P_PER_LAYER;
+#pragma unroll
+  for (int t = 0; t < TAPS; ++t)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) out[t * CH * CH + (kq * 4 + q) * CH + row] = acc[t][q];
+  bsum += __shfl_xor(bsum, 16, 64);
+  bsum += __shfl_xor(bsum, 32, 64);
+  if (kq == 0) out[W_PER_LAYER + row] = bsum;
+}
+
+// grad[p] = sum over waves in order (deterministic)
+__global__ void __launch_bounds__(256) wgrad_reduce_kernel(const float* __restrict__ partials, long nwaves,
+                                                           float* __restrict__ grad) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P_PER_LAYER) return;
+  float s = 0.f;
+  for (long w = 0; w < nwaves; ++w) s += partials[(size_t)w * P_PER_LAYER + p];
+  grad[p] = s;
+}
+
+// Mean-square loss against targets scaled by inv_t; writes the output-layer
+// gradient 2 (y - t) / count as BF16 [S][16] (channels >= 3 zero) and block
+// partial sums of squares.
+__global__ void __launch_bounds__(256) loss_kernel(const float* __restrict__ Y3, const float* __restrict__ T3,
+                                                   Geom g, float inv_t, float inv_count, uint16_t* __restrict__ dY,
+                                                   float* __restrict__ partials) {
+  float s = 0.f;
+  const long n = (long)g.B * g.S;
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    const long b = e / g.S, v = e % g.S;
+    uint16_t o[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) o[c] = 0;
+#pragma unroll
+    for (int c = 0; c < OUT_CH; ++c) {
+      const float d = Y3[(b * OUT_CH + c) * g.S + v] - T3[(b * OUT_CH + c) * g.S + v] * inv_t;
+      s += d * d;
+      o[c] = f2bf(2.f * d * inv_count);
+    }
+    uint4* dst = reinterpret_cast<uint4*>(dY + (size_t)e * CH);
+    dst[0] = make_uint4(pk(o[0], o[1]), pk(o[2], o[3]), pk(o[4], o[5]), pk(o[6], o[7]));
+    dst[1] = make_uint4(pk(o[8], o[9]), pk(o[10], o[11]), pk(o[12], o[13]), pk(o[14], o[15]));
+  }
+  __shared__ float ws[4];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
+}
+
+// Compute copies of the FP32 master weights: the forward B-operand order
+// Wb[t][lane][j] = W[t][4(lane/16)+j][lane%16] and, for the input gradient,
+// the transposed and tap-flipped Wt[t][lane][j] = W[26-t][lane%16][4(lane/16)+j].
+__global__ void __launch_bounds__(256) pack_weights_kernel(const float* __restrict__ master, uint16_t* __restrict__ Wb,
+                                                           uint16_t* __restrict__ Wt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= LAYERS * W_PER_LAYER) return;
+  const int L = i / W_PER_LAYER, e = i % W_PER_LAYER;
+  const int t = e / (CH * CH), k = (e / CH) % CH, n = e % CH;
+  const uint16_t v = f2bf(master[(size_t)L * P_PER_LAYER + e]);
+  Wb[(size_t)L * W_PER_LAYER + ((size_t)t * 64 + (n + 16 * (k >> 2))) * 4 + (k & 3)] = v;
+  Wt[(size_t)L * W_PER_LAYER + ((size_t)(26 - t) * 64 + (k + 16 * (n >> 2))) * 4 + (n & 3)] = v;
+}
+
+// Adam (Kingma & Ba 2015) on all parameters; t is the 1-based step count.
+__global__ void __launch_bounds__(256) adam_kernel(float* __restrict__ w, float* __restrict__ m, float* __restrict__ v,
+                                                   const float* __restrict__ grad, int n, float lr, float b1, float b2,
+                                                   float eps, float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float g = grad[i];
+  const float mi = b1 * m[i] + (1.f - b1) * g;
+  const float vi = b2 * v[i] + (1.f - b2) * g * g;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// FP32 host-order input [B][S][7] -> BF16 [B][S][16]
+__global__ void __launch_bounds__(256) pack_input_kernel(const float* __restrict__ in7, long n, uint16_t* __restrict__ X) {
+  for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long)gridDim.x * blockDim.x) {
+    uint16_t o[CH];
+#pragma unroll
+    for (int c = 0; c < CH; ++c) o[c] = c < IN_CH ? f2bf(in7[e * IN_CH + c]) : (uint16_t)0;
+    uint4* dst = reinterpret_cast<uint4*>(X + (size_t)e * CH);
+    dst[0] = make_uint4(pk(o[0], o[1]), pk(o[2], o[3]), pk(o[4], o[5]), pk(o[6], o[7]));
+    dst[1] = make_uint4(pk(o[8], o[9]), pk(o[10], o[11]), pk(o[12], o[13]), pk(o[14], o[15]));
+  }
+}
+
+}  // namespace nn
+}  // namespace pf

@@ -276,6 +276,27 @@ int pf_eta_update(pf_handle* h, const pf_eta_params* q, const double* eta, const
   return h->eng->eta_update(*q, eta, phi, xB_alpha, tr_sigma_or_null, eta_out);
 }
 
+int pf_set_warm_start(pf_handle* h, const double* u_k_or_null) {
+  if (!h) return pf::fail(PF_E_ARG, "pf_set_warm_start: null handle");
+  return h->eng->set_warm_start(u_k_or_null);
+}
+
+int pf_set_warm_start_callback(pf_handle* h, pf_warm_start_fn fn, void* user) {
+  if (!h) return pf::fail(PF_E_ARG, "pf_set_warm_start_callback: null handle");
+  return h->eng->set_warm_start_callback(fn, user, h);
+}
+
+int pf_attach_learned_warm_start(pf_handle* h, pf_nn* net) {
+  if (!h) return pf::fail(PF_E_ARG, "pf_attach_learned_warm_start: null handle");
+  return h->eng->attach_nn(pf_nn_impl(net));
+}
+
+int pf_nn_collect(pf_nn* net, pf_handle* h) {
+  if (!h || !net) return pf::fail(PF_E_ARG, "pf_nn_collect: null");
+  int rc = h->eng->nn_collect(pf_nn_impl(net));
+  return rc ? rc : pf_nn_dataset_size(net);
+}
+
 int pf_sync(pf_handle* h) {
   if (!h) return pf::fail(PF_E_ARG, "pf_sync: null");
   return h->eng->sync();

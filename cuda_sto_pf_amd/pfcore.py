@@ -133,6 +133,23 @@ EXPORTS = {
     "pf_eta_params_default": (C.c_int, [C.POINTER(PfEtaParams)]),
     "pf_eta_update": (C.c_int, [C.c_void_p, C.POINTER(PfEtaParams), C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_void_p, C.c_void_p]),
+    "pf_set_warm_start": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pf_set_warm_start_callback": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pf_nn_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "pf_nn_destroy": (C.c_int, [C.c_void_p]),
+    "pf_nn_set_scales": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
+    "pf_nn_get_scales": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)]),
+    "pf_nn_set_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pf_nn_get_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pf_nn_collect": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pf_nn_dataset_size": (C.c_int, [C.c_void_p]),
+    "pf_nn_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_float, C.POINTER(C.c_float)]),
+    "pf_nn_eval_loss": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_float)]),
+    "pf_nn_forward": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "pf_nn_gradients": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                  C.POINTER(C.c_float), C.c_void_p]),
+    "pf_attach_learned_warm_start": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pf_destroy": (C.c_int, [C.c_void_p]),
     "pf_fft_forward": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "pf_fft_inverse": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
@@ -293,12 +310,106 @@ class Simulation:
                                    None if t is None else _ptr(t), _ptr(out)))
         return out
 
+    def set_warm_start(self, u=None):
+        """Warm state for the next elastic solve (3 x pf_elastic_state_size
+        complex values), or None for the homogeneous cold start."""
+        if u is None:
+            _check(lib().pf_set_warm_start(self._h, None))
+        else:
+            u = np.ascontiguousarray(u, dtype=np.complex128)
+            _check(lib().pf_set_warm_start(self._h, _ptr(u)))
+
+    def attach_net(self, net):
+        """Use a learned network at every cold start (None detaches)."""
+        self._net = net
+        _check(lib().pf_attach_learned_warm_start(self._h, None if net is None else net._h))
+
     def time_kernel(self, kernel_id, reps=20):
         """Mean microseconds and bytes of one launch of a step kernel (HIP events)."""
         us = C.c_double()
         b = C.c_double()
         _check(lib().pf_time_kernel(self._h, int(kernel_id), int(reps), C.byref(us), C.byref(b)))
         return us.value, b.value
+
+
+NN_PARAMS = 34640
+
+
+class Net:
+    """The learned warm-start network (a pf_nn): training geometry `shape`,
+    batches of up to `max_batch` grids, a device dataset of `capacity`
+    snapshots."""
+
+    def __init__(self, shape, max_batch=4, capacity=0):
+        self.shape = tuple(shape)
+        h = C.c_void_p()
+        _check(lib().pf_nn_create(*[int(v) for v in self.shape], int(max_batch), int(capacity), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().pf_nn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_scales(self, s_phi, s_tau, s_out):
+        _check(lib().pf_nn_set_scales(self._h, float(s_phi), float(s_tau), float(s_out)))
+
+    def scales(self):
+        a, b, c = C.c_double(), C.c_double(), C.c_double()
+        _check(lib().pf_nn_get_scales(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == NN_PARAMS
+        _check(lib().pf_nn_set_params(self._h, _ptr(p)))
+
+    def params(self):
+        out = np.empty(NN_PARAMS, dtype=np.float32)
+        _check(lib().pf_nn_get_params(self._h, _ptr(out)))
+        return out
+
+    def collect(self, sim):
+        rc = lib().pf_nn_collect(self._h, sim._h)
+        _check(min(rc, 0))
+        return rc
+
+    def size(self):
+        return lib().pf_nn_dataset_size(self._h)
+
+    def train_step(self, idx, lr):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        loss = C.c_float()
+        _check(lib().pf_nn_train_step(self._h, _ptr(idx), int(idx.size), float(lr), C.byref(loss)))
+        return loss.value
+
+    def eval_loss(self, idx):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        loss = C.c_float()
+        _check(lib().pf_nn_eval_loss(self._h, _ptr(idx), int(idx.size), C.byref(loss)))
+        return loss.value
+
+    def forward(self, in7, shape):
+        in7 = np.ascontiguousarray(in7, dtype=np.float32)
+        B = in7.shape[0]
+        out = np.empty((B, 3, int(np.prod(shape))), dtype=np.float32)
+        _check(lib().pf_nn_forward(self._h, B, *[int(v) for v in shape], _ptr(in7), _ptr(out)))
+        return out
+
+    def gradients(self, in7, target, shape):
+        in7 = np.ascontiguousarray(in7, dtype=np.float32)
+        target = np.ascontiguousarray(target, dtype=np.float32)
+        g = np.empty(NN_PARAMS, dtype=np.float32)
+        loss = C.c_float()
+        _check(lib().pf_nn_gradients(self._h, in7.shape[0], *[int(v) for v in shape], _ptr(in7), _ptr(target),
+                                     C.byref(loss), _ptr(g)))
+        return loss.value, g
 
 
 def minimize_opts(**kw):

@@ -236,6 +236,72 @@ int pf_eta_params_default(pf_eta_params* q);
 int pf_eta_update(pf_handle* h, const pf_eta_params* q, const double* eta, const double* phi,
                   const double* xB_alpha, const double* tr_sigma_or_null, double* eta_out);
 
+/* ---------------------------------------------------------------------------
+ * Warm start of the elastic fixed point.  The reference warm-starts from the
+ * previous step's displacement and otherwise from the homogeneous solution
+ * (main_cuda.cu:28760-28805).  These entry points let a caller supply the
+ * start (the warm-start hook of the drop-in boundary); the iteration, its
+ * residual and tolerance are unchanged. */
+
+/* Set the warm state u(k) (3 compact components, interleaved re/im doubles,
+ * pf_elastic_state_size values each) for the next solve; NULL forces the
+ * homogeneous cold start. */
+int pf_set_warm_start(pf_handle* h, const double* u_k_or_null);
+
+/* Called at every cold start after the homogeneous start is formed; the
+ * callback may replace it with pf_set_warm_start (return 0) or leave it
+ * (return 0 without a call); a nonzero return aborts the step. */
+typedef int (*pf_warm_start_fn)(pf_handle* h, void* user);
+int pf_set_warm_start_callback(pf_handle* h, pf_warm_start_fn fn, void* user);
+
+/* ---------------------------------------------------------------------------
+ * Learned warm start (no reference counterpart): a 5-layer 3x3x3 convolution
+ * network with 16 channels, BF16 weights and FP32 accumulation on the matrix
+ * cores.  Input: phi and the 6 eigenstress components (C + h dC):eps0 of the
+ * solve, scaled; output: 3 channels, the real-space correction from the
+ * homogeneous cold start to the converged displacement.  Training runs on the
+ * GPU with the library's own forward and backward kernels and Adam. */
+typedef struct pf_nn pf_nn;
+
+/* Parameters: 5 layers of W[27][16][16] (tap = 9(dx+1)+3(dy+1)+(dz+1),
+ * [cin][cout]) followed by bias[16] each, FP32. */
+#define PF_NN_PARAMS 34640
+
+/* Training geometry: batches of up to max_batch grids of Nx*Ny*Nz voxels and
+ * a device-resident dataset of up to capacity snapshots of that grid. */
+int pf_nn_create(int Nx, int Ny, int Nz, int max_batch, int capacity, pf_nn** out);
+int pf_nn_destroy(pf_nn* net);
+
+/* Feature scales: phi * s_phi, eigenstress * s_tau; network output times
+ * s_out is the displacement correction. */
+int pf_nn_set_scales(pf_nn* net, double s_phi, double s_tau, double s_out);
+int pf_nn_get_scales(pf_nn* net, double* s_phi, double* s_tau, double* s_out);
+int pf_nn_set_params(pf_nn* net, const float* params);   /* PF_NN_PARAMS values; resets Adam */
+int pf_nn_get_params(pf_nn* net, float* params);
+
+/* Converge the handle's elastic solve for its current phi (from its warm
+ * state), then append (features, correction) to the dataset.  The handle's
+ * grid must equal the training grid.  Returns the dataset size. */
+int pf_nn_collect(pf_nn* net, pf_handle* h);
+int pf_nn_dataset_size(pf_nn* net);
+
+/* One Adam step on the snapshots idx[0..batch) (mean-square loss in output
+ * units); loss may be NULL.  pf_nn_eval_loss evaluates without updating. */
+int pf_nn_train_step(pf_nn* net, const int* idx, int batch, float lr, float* loss);
+int pf_nn_eval_loss(pf_nn* net, const int* idx, int batch, float* loss);
+
+/* Check entry points on host data for a grid of any size: in7 is
+ * [batch][Nx*Ny*Nz][7] (already scaled), target [batch][3][Nx*Ny*Nz].
+ * pf_nn_forward writes the output [batch][3][Nx*Ny*Nz]; pf_nn_gradients the
+ * loss and the PF_NN_PARAMS gradients of the mean-square loss. */
+int pf_nn_forward(pf_nn* net, int batch, int Nx, int Ny, int Nz, const float* in7, float* out3);
+int pf_nn_gradients(pf_nn* net, int batch, int Nx, int Ny, int Nz, const float* in7, const float* target,
+                    float* loss, float* grad);
+
+/* Use the network at every cold start of the handle's elastic solve (NULL
+ * detaches).  The network must outlive the attachment. */
+int pf_attach_learned_warm_start(pf_handle* h, pf_nn* net);
+
 /* Synchronise the handle's stream. */
 int pf_sync(pf_handle* h);
 
