@@ -8,10 +8,13 @@ A step is one call of the time step on the resident fields (elastic fixed point,
 phi update, composition update with the mass-conserving zero mode).
 
 Reported: timesteps/s of the mixed-precision path (FP32 transforms, FP64
-state/reductions/zero mode/residual) as ``value``; the plain FP64 path on the
+state/reductions/zero mode/residual, the learned network at the cold start of
+the elastic fixed point) as ``value``; the plain FP64 path on the
 same inputs; max |x_B mixed - x_B FP64| after the same steps; elastic updates
 per step; the roofline of the dominant kernel (HIP events inside the library);
-and the CPU FP64 oracle timed on a bounded sample as ``cpu_baseline``.
+the transforms against the survey's byte model; the cold-start gain of the
+learned warm start; and the CPU FP64 oracle timed on a bounded sample as
+``cpu_baseline``.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
 """
@@ -36,9 +39,11 @@ HBM_PEAK_GBS = 8000.0
 ACCEL = dict(elastic_warm_start_order=3, elastic_anderson_depth=3)
 
 
-def run_path(pf, pd, phi, xB, cap, prec, warmup, steps):
+def run_path(pf, pd, phi, xB, cap, prec, warmup, steps, net=None):
     sim = pf.Simulation(pd, prec)
     sim.set_fields(phi, xB, xB_cap=cap)
+    if net is not None:
+        sim.attach_net(net)
     sim.mass0 = sim.scalars()["mass_xBtot"]
     sim.step(warmup)
     sim.sync()
@@ -70,6 +75,41 @@ def kernel_table(pf, sim, iters_per_step, mixed):
         table[name] = {"us": us, "bytes": b, "GBps": b / us * 1e-3, "frac_hbm": b / us * 1e-3 / HBM_PEAK_GBS,
                        "per_step": counts[name], "us_per_step": us * counts[name]}
     return table
+
+
+def learned_cold_start(pf, learned, net, pd, pd_fast, phi, xB):
+    """First (cold) step of the workload with and without the learned warm
+    start: elastic updates to the 1e-8 residual, step time, and the change in
+    x_B (the tolerance-level difference of two converged solves)."""
+    out = {}
+    for label, p, prec in (("mixed_accelerated", pd_fast, pf.PF_MIXED), ("fp64_reference_iteration", pd, pf.PF_FP64)):
+        i0, t0, x0 = learned.cold_start(p, phi, xB, prec)
+        i1, t1, x1 = learned.cold_start(p, phi, xB, prec, net)
+        out[label] = {"updates_without": i0, "updates_with": i1, "step_ms_without": 1e3 * t0,
+                      "step_ms_with": 1e3 * t1, "max_abs_dxB": float(np.abs(x1 - x0).max())}
+        print(f"[bench] learned cold start {label}: {i0} -> {i1} updates, {1e3 * t0:.1f} -> {1e3 * t1:.1f} ms",
+              file=sys.stderr, flush=True)
+    us = net.time_inference(phi.shape, reps=5)
+    flop = 2.0 * 27 * 16 * 16 * 5 * float(np.prod(phi.shape))
+    out["inference"] = {"us": us, "flop": flop, "achieved_tflops": flop / us * 1e-6, "peak_tflops_bf16": 2500.0,
+                        "frac": flop / us * 1e-6 / 2500.0}
+    return out
+
+
+def fft_survey_model(table, n):
+    """Survey accounting of an FP32 R2C/C2R 3-D transform (each of the three
+    axis passes reads and writes its array once: 4R + 40C bytes, C =
+    N*N*(N/2+1)) against the measured per-update transform time of the
+    fused passes (two 6-component transforms per elastic update)."""
+    R = n ** 3
+    Cc = n * n * (n // 2 + 1)
+    model = 12 * (4 * R + 40 * Cc)
+    us = sum(table[k]["us"] for k in ("zpass_hij", "ypass_fwd6", "ypass_inv6", "xpass_fwd6", "xpass_inv6"))
+    ach = model / us * 1e-3
+    return {"model_bytes_per_update": model, "us_per_update": us, "achieved_GBps": ach, "peak_GBps": HBM_PEAK_GBS,
+            "frac": ach / HBM_PEAK_GBS,
+            "note": "12 FP32 transforms of the survey model per elastic update vs the measured zpass_hij + y/x "
+                    "passes (the z pass includes the h_ij operator)"}
 
 
 def cpu_baseline(budget_s=30.0):
@@ -117,7 +157,9 @@ def main():
           file=sys.stderr, flush=True)
 
     pd_fast = dict(pd, **ACCEL)
-    sim_m, t_m, it_m, sc_m = run_path(pf, pd_fast, phi, xB, cap, pf.PF_MIXED, args.warmup, args.steps)
+    from cuda_sto_pf_amd import learned
+    net = learned.load() if os.path.exists(learned.DEFAULT_WEIGHTS) else None
+    sim_m, t_m, it_m, sc_m = run_path(pf, pd_fast, phi, xB, cap, pf.PF_MIXED, args.warmup, args.steps, net)
     print(f"[bench] mixed: {args.steps / t_m:.2f} steps/s, {it_m:.2f} elastic updates/step",
           file=sys.stderr, flush=True)
     xb_m = sim_m.field(pf.PF_XB)
@@ -143,6 +185,8 @@ def main():
         print(f"[bench] fp64 + accelerated solver: {args.steps / t_a:.2f} steps/s, {it_a:.2f} updates/step",
               file=sys.stderr, flush=True)
         sim_a.close()
+
+    learned_stats = learned_cold_start(pf, learned, net, pd, pd_fast, phi, xB) if net is not None else None
 
     table = kernel_table(pf, sim_m, it_m, mixed=True)
     dom_name = max(table, key=lambda k: table[k]["us_per_step"])
@@ -171,8 +215,9 @@ def main():
         "dtype": "f32 transforms / f64 state, reductions, zero mode, residual",
         "data": "synthetic",
         "config": {"workload": "256^3, 40 particles R~U(5,12), hetero elastic tol 1e-8, zero mode, dt 0.01",
-                   "grid": list(phi.shape), "learned_warm_start": False,
-                   "path": "mixed precision, extrapolated warm start (order 3) + Anderson depth 3"},
+                   "grid": list(phi.shape), "learned_warm_start": net is not None,
+                   "path": "mixed precision, learned cold start + extrapolated warm start (order 3) + "
+                           "Anderson depth 3"},
         "elastic_updates_per_step": it_m,
         "mass_drift_rel": abs(sc_m["mass_xBtot"] / sim_m.mass0 - 1.0),
         "max_abs_dc_vs_fp64": max_dc,
@@ -181,6 +226,8 @@ def main():
         "roofline": {"kernel": dom_name, "bound": "hbm", "achieved": dom["GBps"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": dom["bytes"], "us_per_launch": dom["us"]},
+        "fft_survey_model": fft_survey_model(table, phi.shape[0]),
+        "learned_warm_start": learned_stats,
         "kernels_mixed": table,
         "kernels_fp64": table_d if not args.no_fp64 else None,
         "cpu_baseline": cpu,

@@ -318,6 +318,52 @@ __global__ void __launch_bounds__(NT) kop_add3(long n, const cx<T>* d0, const cx
     u2[c].im += d2[c].im;
   }
 }
+// The network's 3 channels are a force density f whose Green response is the
+// displacement correction: u^ = Omega(k) f^ with Omega = K^-1 the Green
+// tensor of the homogeneous stiffness (the operator of the reference's
+// Green update, cuda_kernels.cu:6484-6633).  TO_FORCE applies K (target
+// side), otherwise Omega (prediction side); k = 0 maps to 0 either way.
+template <typename T>
+struct ForceMapArgs {
+  KDims d;
+  long n;
+  cx<T>* a[3];
+  double C[21];
+  int to_force;
+};
+template <typename T>
+__global__ void __launch_bounds__(NT) kop_force_map(ForceMapArgs<T> fa) {
+  for (long c = (long)blockIdx.x * NT + threadIdx.x; c < fa.n; c += (long)gridDim.x * NT) {
+    double kx, ky, kz;
+    int iz;
+    kvec(fa.d, c, kx, ky, kz, iz);
+    double M[6];
+    bool ok = c != 0;
+    if (ok) {
+      if (fa.to_force)
+        acoustic_tensor(fa.C, kx, ky, kz, M);
+      else
+        ok = green_tensor(fa.C, kx, ky, kz, M);
+    }
+    cx<double> v[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) v[i] = cx<double>{double(fa.a[i][c].re), double(fa.a[i][c].im)};
+    const double m[3][3] = {{M[0], M[1], M[2]}, {M[1], M[3], M[4]}, {M[2], M[4], M[5]}};
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      double re = 0.0, im = 0.0;
+      if (ok) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+          re += m[i][j] * v[j].re;
+          im += m[i][j] * v[j].im;
+        }
+      }
+      fa.a[i][c] = cx<T>{T(re), T(im)};
+    }
+  }
+}
+
 // d_i = u_i - v_i
 template <typename T, typename S>
 __global__ void __launch_bounds__(NT) kop_diff3(long n, const S* u0, const S* u1, const S* u2, const cx<T>* v0,
@@ -1227,7 +1273,7 @@ struct Engine final : EngineBase {
     return PF_OK;
   }
 
-  // network prediction of the correction, transformed to sc(0..2)
+  // network prediction (force density), transformed to sc(0..2)
   int nn_predict_to_sc() {
     int rc;
     if ((rc = nn_buffers())) return rc;
@@ -1243,11 +1289,22 @@ struct Engine final : EngineBase {
     return PF_OK;
   }
 
+  void force_map(bool to_force) {
+    ForceMapArgs<T> fa{};
+    fa.d = kd();
+    fa.n = nSc;
+    for (int i = 0; i < 3; ++i) fa.a[i] = sc(i);
+    for (int i = 0; i < 21; ++i) fa.C[i] = P.C[i];
+    fa.to_force = to_force ? 1 : 0;
+    hipLaunchKernelGGL(kop_force_map<T>, dim3(grid_for(nSc)), dim3(NT), 0, stream, fa);
+  }
+
   int cold_hook() {
     const int gk = grid_for(nSc);
     if (nn_net) {
       int rc = nn_predict_to_sc();
       if (rc) return rc;
+      force_map(false);
       if (state_fp64())
         hipLaunchKernelGGL((kop_add3<T, cx<double>>), dim3(gk), dim3(NT), 0, stream, nSc, (const C*)sc(0),
                            (const C*)sc(1), (const C*)sc(2), Ud, Ud + nSc, Ud + 2 * nSc);
@@ -1299,7 +1356,8 @@ struct Engine final : EngineBase {
   }
 
   // Converge the solve for the current phi, then store the network input
-  // and the real-space correction (converged - homogeneous cold start).
+  // and the target: the force density K(k) (u_converged - u_cold)^ in real
+  // space, u_cold the homogeneous cold start.
   int nn_collect(nn::Net* net) override {
     uint16_t* feat;
     float* tgt;
@@ -1338,6 +1396,7 @@ struct Engine final : EngineBase {
       hipLaunchKernelGGL((kop_diff3<T, C>), dim3(gk), dim3(NT), 0, stream, nSc, (const C*)uu(u, 0),
                          (const C*)uu(u, 1), (const C*)uu(u, 2), (const C*)uu(un, 0), (const C*)uu(un, 1),
                          (const C*)uu(un, 2), sc(0), sc(1), sc(2));
+    force_map(true);
     xpass(true, 0, 3, Sc);
     ypass(true, 0, 3);
     typename OpStoreF3<T>::Args sa{tgt, (size_t)R, T(1.0 / (double)R)};

@@ -100,25 +100,18 @@ struct Net {
   }
 };
 
-static int grid_blocks(long nwaves) { return (int)std::max<long>(1, nwaves / 4); }
-
-static long conv_waves(const Geom& g) {
-  const long ntiles = (long)g.B * g.Nx * g.Ny * ((g.Nz + 15) / 16);
-  return std::min<long>(std::max<long>(ntiles, 4), 65536);
-}
-
 // forward: act[0] -> act[1..4] -> y3
 static int forward(Net* net, Buffers& bf, hipStream_t s) {
   const Geom& g = bf.g;
-  const int blocks = grid_blocks(conv_waves(g) + 3);
+  const int blocks = (int)conv3_lds_blocks(g);
   for (int L = 0; L < LAYERS; ++L) {
     const uint16_t* W = net->Wb + (size_t)L * W_PER_LAYER;
     const float* bias = net->master + (size_t)L * P_PER_LAYER + W_PER_LAYER;
     if (L < LAYERS - 1)
-      hipLaunchKernelGGL(conv3_kernel<EPI_ACT>, dim3(blocks), dim3(256), 0, s, g, bf.act[L], W, bias, bf.act[L + 1],
-                         (float*)nullptr, (const uint16_t*)nullptr);
+      hipLaunchKernelGGL(conv3_lds_kernel<EPI_ACT>, dim3(blocks), dim3(256), 0, s, g, bf.act[L], W, bias,
+                         bf.act[L + 1], (float*)nullptr, (const uint16_t*)nullptr);
     else
-      hipLaunchKernelGGL(conv3_kernel<EPI_OUT3>, dim3(blocks), dim3(256), 0, s, g, bf.act[L], W, bias,
+      hipLaunchKernelGGL(conv3_lds_kernel<EPI_OUT3>, dim3(blocks), dim3(256), 0, s, g, bf.act[L], W, bias,
                          (uint16_t*)nullptr, bf.y3, (const uint16_t*)nullptr);
   }
   NN_HIP(hipGetLastError());
@@ -132,14 +125,14 @@ static int backward(Net* net, Buffers& bf, hipStream_t s, float inv_t, float* gr
   hipLaunchKernelGGL(loss_kernel, dim3(bf.lblocks), dim3(256), 0, s, bf.y3, bf.t3, g, inv_t, inv_count, bf.grad[0],
                      bf.lossp);
   int cur = 0;
-  const int cblocks = grid_blocks(conv_waves(g) + 3);
+  const int cblocks = (int)conv3_lds_blocks(g);
   for (int L = LAYERS - 1; L >= 0; --L) {
     hipLaunchKernelGGL(conv3_wgrad_kernel, dim3(bf.nwaves / 4), dim3(256), 0, s, g, bf.act[L], bf.grad[cur],
                        bf.partials);
     hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((P_PER_LAYER + 255) / 256), dim3(256), 0, s, bf.partials, bf.nwaves,
                        grad_out + (size_t)L * P_PER_LAYER);
     if (L > 0) {
-      hipLaunchKernelGGL(conv3_kernel<EPI_GRAD>, dim3(cblocks), dim3(256), 0, s, g, bf.grad[cur],
+      hipLaunchKernelGGL(conv3_lds_kernel<EPI_GRAD>, dim3(cblocks), dim3(256), 0, s, g, bf.grad[cur],
                          net->Wt + (size_t)L * W_PER_LAYER, (const float*)nullptr, bf.grad[1 - cur], (float*)nullptr,
                          bf.act[L]);
       cur = 1 - cur;
@@ -173,20 +166,20 @@ static int gather(Net* net, const int* idx, int batch) {
 }
 
 int infer(Net* net, hipStream_t s, int Nx, int Ny, int Nz, const uint16_t* X0, uint16_t* a, uint16_t* b, float* Y3) {
-  NN_HIP(hipStreamSynchronize(net->stream));   // weights packed on the network's stream
+  if (s != net->stream) NN_HIP(hipStreamSynchronize(net->stream));   // weights are packed on the network's stream
   const Geom g{1, Nx, Ny, Nz, (long)Nx * Ny * Nz};
-  const int blocks = grid_blocks(conv_waves(g) + 3);
+  const int blocks = (int)conv3_lds_blocks(g);
   const uint16_t* in = X0;
   uint16_t* bufs[2] = {a, b};
   for (int L = 0; L < LAYERS; ++L) {
     const uint16_t* W = net->Wb + (size_t)L * W_PER_LAYER;
     const float* bias = net->master + (size_t)L * P_PER_LAYER + W_PER_LAYER;
     if (L < LAYERS - 1) {
-      hipLaunchKernelGGL(conv3_kernel<EPI_ACT>, dim3(blocks), dim3(256), 0, s, g, in, W, bias, bufs[L & 1],
+      hipLaunchKernelGGL(conv3_lds_kernel<EPI_ACT>, dim3(blocks), dim3(256), 0, s, g, in, W, bias, bufs[L & 1],
                          (float*)nullptr, (const uint16_t*)nullptr);
       in = bufs[L & 1];
     } else {
-      hipLaunchKernelGGL(conv3_kernel<EPI_OUT3>, dim3(blocks), dim3(256), 0, s, g, in, W, bias, (uint16_t*)nullptr,
+      hipLaunchKernelGGL(conv3_lds_kernel<EPI_OUT3>, dim3(blocks), dim3(256), 0, s, g, in, W, bias, (uint16_t*)nullptr,
                          Y3, (const uint16_t*)nullptr);
     }
   }
@@ -371,6 +364,35 @@ int pf_nn_gradients(pf_nn* net, int batch, int Nx, int Ny, int Nz, const float* 
                     float* loss, float* grad) {
   if (!target || !grad) return fail(PF_E_ARG, "pf_nn_gradients: null");
   return host_pass(net, batch, Nx, Ny, Nz, in7, target, nullptr, loss, grad);
+}
+
+int pf_nn_time_inference(pf_nn* net, int Nx, int Ny, int Nz, int reps, double* us) {
+  if (!net || !us || Nx < 1 || Ny < 1 || Nz < 1 || reps < 1) return fail(PF_E_ARG, "pf_nn_time_inference: bad argument");
+  Net& n = net->net;
+  Buffers bf;
+  int rc;
+  const size_t S = (size_t)Nx * Ny * Nz;
+  uint16_t *x0, *a, *b;
+  float* y3;
+  if ((rc = bf.alloc(&x0, S * CH)) || (rc = bf.alloc(&a, S * CH)) || (rc = bf.alloc(&b, S * CH)) ||
+      (rc = bf.alloc(&y3, S * OUT_CH)))
+    return rc;
+  NN_HIP(hipMemsetAsync(x0, 0, S * CH * sizeof(uint16_t), n.stream));
+  if ((rc = infer(&n, n.stream, Nx, Ny, Nz, x0, a, b, y3))) return rc;   // warm-up
+  hipEvent_t e0, e1;
+  NN_HIP(hipEventCreate(&e0));
+  NN_HIP(hipEventCreate(&e1));
+  NN_HIP(hipEventRecord(e0, n.stream));
+  for (int r = 0; r < reps; ++r)
+    if ((rc = infer(&n, n.stream, Nx, Ny, Nz, x0, a, b, y3))) return rc;
+  NN_HIP(hipEventRecord(e1, n.stream));
+  NN_HIP(hipEventSynchronize(e1));
+  float ms = 0.f;
+  NN_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *us = 1000.0 * ms / reps;
+  return PF_OK;
 }
 
 }  // extern "C"

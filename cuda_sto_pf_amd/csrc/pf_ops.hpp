@@ -354,25 +354,33 @@ __device__ __forceinline__ void kvec(const KDims& d, long c, double& kx, double&
   kz = d.kz[iz];
 }
 
-// Acoustic tensor K_ij = C_ipjq k_p k_q and its inverse by cofactors
-// (cuda_kernels.cu:6484-6633, 6818-6878).  Returns false when |det| < 1e-30.
-__device__ __forceinline__ bool green_tensor(const double* C, double kx, double ky, double kz,
-                                             double* G) {
+// Acoustic tensor K_ij = C_ipjq k_p k_q (cuda_kernels.cu:6818-6870), as
+// K = {K11, K12, K13, K22, K23, K33}.
+__device__ __forceinline__ void acoustic_tensor(const double* C, double kx, double ky, double kz, double* K) {
   const double kx2 = kx * kx, ky2 = ky * ky, kz2 = kz * kz;
 #define CC(i, j) C[vidx(i - 1, j - 1)]
-  const double K11 = kx2 * CC(1, 1) + 2.0 * kx * (kz * CC(1, 5) + ky * CC(1, 6)) + kz2 * CC(5, 5) +
-                     2.0 * ky * kz * CC(5, 6) + ky2 * CC(6, 6);
-  const double K22 = ky2 * CC(2, 2) + 2.0 * ky * (kz * CC(2, 4) + kx * CC(2, 6)) + kz2 * CC(4, 4) +
-                     2.0 * kx * kz * CC(4, 6) + kx2 * CC(6, 6);
-  const double K33 = kz2 * CC(3, 3) + 2.0 * kz * (ky * CC(3, 4) + kx * CC(3, 5)) + ky2 * CC(4, 4) +
-                     2.0 * kx * ky * CC(4, 5) + kx2 * CC(5, 5);
-  const double K13 = kx2 * CC(1, 5) + kz2 * CC(3, 5) + ky * kz * (CC(3, 6) + CC(4, 5)) +
-                     ky2 * CC(4, 6) + kx * kz * (CC(1, 3) + CC(5, 5)) + kx * ky * (CC(1, 4) + CC(5, 6));
-  const double K23 = ky2 * CC(2, 4) + kz2 * CC(3, 4) + ky * kz * (CC(2, 3) + CC(4, 4)) +
-                     kx * kz * (CC(3, 6) + CC(4, 5)) + kx * ky * (CC(2, 5) + CC(4, 6)) + kx2 * CC(5, 6);
-  const double K12 = kx2 * CC(1, 6) + ky2 * CC(2, 6) + kz2 * CC(4, 5) + ky * kz * (CC(2, 5) + CC(4, 6)) +
-                     kx * kz * (CC(1, 4) + CC(5, 6)) + kx * ky * (CC(1, 2) + CC(6, 6));
+  K[0] = kx2 * CC(1, 1) + 2.0 * kx * (kz * CC(1, 5) + ky * CC(1, 6)) + kz2 * CC(5, 5) +
+         2.0 * ky * kz * CC(5, 6) + ky2 * CC(6, 6);
+  K[3] = ky2 * CC(2, 2) + 2.0 * ky * (kz * CC(2, 4) + kx * CC(2, 6)) + kz2 * CC(4, 4) +
+         2.0 * kx * kz * CC(4, 6) + kx2 * CC(6, 6);
+  K[5] = kz2 * CC(3, 3) + 2.0 * kz * (ky * CC(3, 4) + kx * CC(3, 5)) + ky2 * CC(4, 4) +
+         2.0 * kx * ky * CC(4, 5) + kx2 * CC(5, 5);
+  K[2] = kx2 * CC(1, 5) + kz2 * CC(3, 5) + ky * kz * (CC(3, 6) + CC(4, 5)) + ky2 * CC(4, 6) +
+         kx * kz * (CC(1, 3) + CC(5, 5)) + kx * ky * (CC(1, 4) + CC(5, 6));
+  K[4] = ky2 * CC(2, 4) + kz2 * CC(3, 4) + ky * kz * (CC(2, 3) + CC(4, 4)) + kx * kz * (CC(3, 6) + CC(4, 5)) +
+         kx * ky * (CC(2, 5) + CC(4, 6)) + kx2 * CC(5, 6);
+  K[1] = kx2 * CC(1, 6) + ky2 * CC(2, 6) + kz2 * CC(4, 5) + ky * kz * (CC(2, 5) + CC(4, 6)) +
+         kx * kz * (CC(1, 4) + CC(5, 6)) + kx * ky * (CC(1, 2) + CC(6, 6));
 #undef CC
+}
+
+// The acoustic tensor and its inverse by cofactors (cuda_kernels.cu:6484-6633,
+// 6818-6878).  Returns false when |det| < 1e-30.
+__device__ __forceinline__ bool green_tensor(const double* C, double kx, double ky, double kz,
+                                             double* G) {
+  double K[6];
+  acoustic_tensor(C, kx, ky, kz, K);
+  const double K11 = K[0], K12 = K[1], K13 = K[2], K22 = K[3], K23 = K[4], K33 = K[5];
   const double det = K13 * (-K13 * K22 + K23 * K12) - K23 * (K23 * K11 - K13 * K12) +
                      K33 * (K22 * K11 - K12 * K12);
   if (fabs(det) < 1e-30) return false;

@@ -1,0 +1,223 @@
+"""Learned elastic warm start: training data from the solver itself, GPU
+training with the library's own kernels (libpfcore pf_nn_*), evaluation of
+the cold-start gain, and the weights file.
+
+The network (5 x 3x3x3 convolutions, 16 channels, BF16 weights, FP32
+accumulation) maps phi and the six eigenstress components (C + h dC):eps0 to
+a 3-channel force density f; its Green response Omega(k) f^(k) is the
+correction from the homogeneous cold start
+(compute_displacement_from_eigenstrain_k_kernel, cuda_kernels.cu:6734-6991)
+to the converged heterogeneous displacement.  It only changes where the
+fixed-point iteration of main_cuda.cu:28760-29095 starts; the map, the
+residual and the tolerance are those of the reference, so converged results
+are unchanged.
+
+Weights file (npz): params float32[34640] (per layer W[27][16][16] then
+bias[16]), s_phi, s_tau, s_out, and a JSON string `meta`.
+
+    python -m cuda_sto_pf_amd.learned train --out weights.npz
+    python -m cuda_sto_pf_amd.learned evaluate --weights weights.npz
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import pfcore as pf
+from .configs import base_params, composition, config, place_nonoverlapping, spheres_phi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_WEIGHTS = os.path.join(HERE, "weights", "warmstart_c16x5.npz")
+
+CH, TAPS, LAYERS, IN_CH, OUT_CH = 16, 27, 5, 7, 3
+P_PER_LAYER = TAPS * CH * CH + CH
+S_PHI, S_TAU = 1.0, 0.1      # phi is O(1); (C + h dC):eps0 is O(10) in code units
+ACCEL = dict(elastic_warm_start_order=3, elastic_anderson_depth=3)
+
+
+def init_params(seed):
+    """He initialisation of the used channels; the output layer starts small."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(LAYERS * P_PER_LAYER, dtype=np.float32)
+    for L in range(LAYERS):
+        cin = IN_CH if L == 0 else CH
+        cout = OUT_CH if L == LAYERS - 1 else CH
+        std = math.sqrt(2.0 / (TAPS * cin)) * (0.1 if L == LAYERS - 1 else 1.0)
+        W = np.zeros((TAPS, CH, CH), dtype=np.float32)
+        W[:, :cin, :cout] = rng.normal(0.0, std, (TAPS, cin, cout))
+        p[L * P_PER_LAYER:L * P_PER_LAYER + TAPS * CH * CH] = W.ravel()
+    return p
+
+
+def training_fields(rng, n):
+    """A seeded configuration shaped like the benchmark workloads 1-2: 1-6
+    non-overlapping spheres with radii U(5, 12), tanh width 4, x_B of the
+    workload generator."""
+    k = int(rng.integers(1, 7))
+    radii = rng.uniform(5.0, 12.0, size=k)
+    centres = place_nonoverlapping(rng, k, (n, n, n), radii, gap=1.0)
+    phi = spheres_phi((n, n, n), centres, radii)
+    return phi, composition(phi, rng)
+
+
+def save(path, net, meta):
+    s_phi, s_tau, s_out = net.scales()
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    np.savez(path, params=net.params(), s_phi=s_phi, s_tau=s_tau, s_out=s_out, meta=json.dumps(meta))
+
+
+def load(path=DEFAULT_WEIGHTS):
+    """A network ready for inference (attach with Simulation.attach_net)."""
+    d = np.load(path, allow_pickle=False)
+    net = pf.Net((16, 16, 16), max_batch=1, capacity=0)
+    net.set_scales(float(d["s_phi"]), float(d["s_tau"]), float(d["s_out"]))
+    net.set_params(d["params"])
+    return net
+
+
+def train(n=64, snapshots=4000, per_run=4, steps_between=5, batch=4, iters=3000, lr=2e-3, seed=0, log=print):
+    pd = dict(base_params(n, hetero=True), **ACCEL)
+    sim = pf.Simulation(pd, pf.PF_MIXED)
+    net = pf.Net((n, n, n), max_batch=batch, capacity=snapshots)
+    net.set_scales(S_PHI, S_TAU, 1.0)
+    rng = np.random.default_rng(seed)
+    t0 = last = time.time()
+    runs = 0
+    while net.size() < snapshots:
+        phi, xB = training_fields(rng, n)
+        sim.set_fields(phi, xB, xB_cap=0.5)
+        runs += 1
+        for j in range(per_run):
+            if j:
+                sim.step(steps_between)
+            net.collect(sim)
+            if net.size() >= snapshots:
+                break
+        if time.time() - last > 20:
+            last = time.time()
+            log(f"[learned] collected {net.size()}/{snapshots} snapshots from {runs} runs, {last - t0:.0f} s")
+    t_collect = time.time() - t0
+    # output scale: RMS of the correction (loss of the zero network with s_out = 1)
+    net.set_params(np.zeros(LAYERS * P_PER_LAYER, dtype=np.float32))
+    ms = np.mean([net.eval_loss(np.arange(i, i + batch)) for i in range(0, min(snapshots, 64 * batch), batch)])
+    s_out = float(math.sqrt(ms))
+    net.set_scales(S_PHI, S_TAU, s_out)
+    net.set_params(init_params(seed))
+    nval = max(batch, snapshots // 20)
+    train_idx = np.arange(snapshots - nval)
+    val_idx = np.arange(snapshots - nval, snapshots)
+
+    def val_loss():
+        return float(np.mean([net.eval_loss(val_idx[i:i + batch]) for i in range(0, nval - batch + 1, batch)]))
+
+    hist = []
+    v0 = val_loss()
+    log(f"[learned] target rms {s_out:.4e}; initial validation loss {v0:.4e}")
+    t1 = last = time.time()
+    for it in range(iters):
+        lr_t = 0.5 * lr * (1.0 + math.cos(math.pi * it / iters)) + 1e-5
+        loss = net.train_step(rng.choice(train_idx, batch, replace=False), lr_t)
+        if not math.isfinite(loss):
+            raise RuntimeError(f"training diverged at step {it}")
+        if (it + 1) % 250 == 0 or time.time() - last > 20:
+            last = time.time()
+            v = val_loss()
+            hist.append((it + 1, loss, v))
+            log(f"[learned] step {it + 1}/{iters} lr {lr_t:.2e} train {loss:.4e} val {v:.4e} ({last - t1:.0f} s)")
+    t_train = time.time() - t1
+    meta = dict(grid=n, snapshots=snapshots, runs=runs, per_run=per_run, steps_between=steps_between, batch=batch,
+                iters=iters, lr=lr, seed=seed, s_out=s_out, initial_val_loss=v0, final_val_loss=val_loss(),
+                collect_s=t_collect, train_s=t_train, history=hist,
+                data="seeded runs of 1-6 spheres, radii U(5,12), heterogeneous elasticity, mixed precision")
+    return net, meta
+
+
+def cold_start(pd, phi, xB, prec, net=None):
+    """Updates and wall time of the first (cold) step, and the x_B it gives."""
+    sim = pf.Simulation(pd, prec)
+    sim.set_fields(phi, xB, xB_cap=0.5)
+    if net is not None:
+        sim.attach_net(net)
+    sim.sync()
+    t = time.perf_counter()
+    sim.step()
+    sim.sync()
+    dt = time.perf_counter() - t
+    it = sim.scalars()["last_elastic_iterations"]
+    x = sim.field(pf.PF_XB)
+    sim.close()
+    return it, dt, x
+
+
+def evaluate(net, cases=None, log=print):
+    """Cold-start updates to the 1e-8 residual with and without the network,
+    for the reference iteration (FP64) and the mixed accelerated path."""
+    out = []
+    if cases is None:
+        rng = np.random.default_rng(12345)
+        cases = []
+        for k in range(3):
+            phi, xB = training_fields(rng, 64)
+            cases.append((f"heldout64_{k}", base_params(64, hetero=True), phi, xB))
+        for idx in (2, 5):
+            pd, phi, xB, _ = config(idx)
+            cases.append((f"workload{idx}_{phi.shape[0]}", pd, phi, xB))
+    for shp in ((64, 64, 64), (128, 128, 128), (256, 256, 256)):
+        us = net.time_inference(shp, reps=5)
+        flop = 2.0 * TAPS * CH * CH * LAYERS * float(np.prod(shp))
+        out.append(dict(case=f"inference_{shp[0]}", us=us, tflops=flop / us * 1e-6))
+        log(f"[learned] inference {shp}: {us:.0f} us ({flop / us * 1e-6:.1f} TFLOP/s on the 16-channel MFMA work)")
+    for name, pd0, phi, xB in cases:
+        for label, pd, prec in (("fp64_reference_iteration", pd0, pf.PF_FP64),
+                                ("mixed_accelerated", dict(pd0, **ACCEL), pf.PF_MIXED)):
+            i0, t0, x0 = cold_start(pd, phi, xB, prec)
+            i1, t1, x1 = cold_start(pd, phi, xB, prec, net)
+            r = dict(case=name, path=label, updates_without=i0, updates_with=i1, step_s_without=t0,
+                     step_s_with=t1, max_abs_dxB=float(np.abs(x1 - x0).max()))
+            out.append(r)
+            log(f"[learned] {name} {label}: cold-start updates {i0} -> {i1}, step {t0 * 1e3:.1f} -> "
+                f"{t1 * 1e3:.1f} ms, max|dxB| {r['max_abs_dxB']:.1e}")
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--out", default=DEFAULT_WEIGHTS)
+    t.add_argument("--grid", type=int, default=64)
+    t.add_argument("--snapshots", type=int, default=4000)
+    t.add_argument("--iters", type=int, default=3000)
+    t.add_argument("--batch", type=int, default=4)
+    t.add_argument("--lr", type=float, default=2e-3)
+    t.add_argument("--seed", type=int, default=0)
+    t.add_argument("--evaluate", action="store_true")
+    t.add_argument("--report", default=None)
+    e = sub.add_parser("evaluate")
+    e.add_argument("--weights", default=DEFAULT_WEIGHTS)
+    e.add_argument("--report", default=None)
+    a = ap.parse_args(argv)
+    log = lambda s: print(s, flush=True)
+    report = {}
+    if a.cmd == "train":
+        net, meta = train(n=a.grid, snapshots=a.snapshots, iters=a.iters, batch=a.batch, lr=a.lr, seed=a.seed,
+                          log=log)
+        save(a.out, net, meta)
+        log(f"[learned] wrote {a.out}")
+        report["training"] = meta
+        if a.evaluate:
+            report["cold_start"] = evaluate(net, log=log)
+    else:
+        report["cold_start"] = evaluate(load(a.weights), log=log)
+    if a.report:
+        with open(a.report, "w") as f:
+            json.dump(report, f, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -149,6 +149,7 @@ EXPORTS = {
     "pf_nn_forward": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "pf_nn_gradients": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
                                   C.POINTER(C.c_float), C.c_void_p]),
+    "pf_nn_time_inference": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "pf_attach_learned_warm_start": (C.c_int, [C.c_void_p, C.c_void_p]),
     "pf_destroy": (C.c_int, [C.c_void_p]),
     "pf_fft_forward": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
@@ -401,6 +402,11 @@ class Net:
         out = np.empty((B, 3, int(np.prod(shape))), dtype=np.float32)
         _check(lib().pf_nn_forward(self._h, B, *[int(v) for v in shape], _ptr(in7), _ptr(out)))
         return out
+
+    def time_inference(self, shape, reps=5):
+        us = C.c_double()
+        _check(lib().pf_nn_time_inference(self._h, *[int(v) for v in shape], int(reps), C.byref(us)))
+        return us.value
 
     def gradients(self, in7, target, shape):
         in7 = np.ascontiguousarray(in7, dtype=np.float32)

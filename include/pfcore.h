@@ -258,8 +258,9 @@ int pf_set_warm_start_callback(pf_handle* h, pf_warm_start_fn fn, void* user);
  * Learned warm start (no reference counterpart): a 5-layer 3x3x3 convolution
  * network with 16 channels, BF16 weights and FP32 accumulation on the matrix
  * cores.  Input: phi and the 6 eigenstress components (C + h dC):eps0 of the
- * solve, scaled; output: 3 channels, the real-space correction from the
- * homogeneous cold start to the converged displacement.  Training runs on the
+ * solve, scaled; output: 3 channels, a force density f whose Green
+ * response Omega(k) f^(k) is the correction from the homogeneous cold start
+ * to the converged displacement.  Training runs on the
  * GPU with the library's own forward and backward kernels and Adam. */
 typedef struct pf_nn pf_nn;
 
@@ -297,6 +298,10 @@ int pf_nn_eval_loss(pf_nn* net, const int* idx, int batch, float* loss);
 int pf_nn_forward(pf_nn* net, int batch, int Nx, int Ny, int Nz, const float* in7, float* out3);
 int pf_nn_gradients(pf_nn* net, int batch, int Nx, int Ny, int Nz, const float* in7, const float* target,
                     float* loss, float* grad);
+
+/* Mean duration (microseconds, HIP events) of one inference over an
+ * Nx*Ny*Nz grid: the five convolutions, on zero features. */
+int pf_nn_time_inference(pf_nn* net, int Nx, int Ny, int Nz, int reps, double* us);
 
 /* Use the network at every cold start of the handle's elastic solve (NULL
  * detaches).  The network must outlive the attachment. */
