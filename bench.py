@@ -112,6 +112,21 @@ def fft_survey_model(table, n):
                     "passes (the z pass includes the h_ij operator)"}
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_256cube_mixed.json")
+
+
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    FETCH_SIZE / WRITE_SIZE passes of this benchmark (calibrated against a
+    streaming kernel of known size), or None when that record is absent."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        rec = json.load(f)
+    k = rec.get("kernels", {}).get(kernel)
+    return None if k is None else k["traffic_bytes"]
+
+
 def cpu_baseline(budget_s=30.0):
     """The FP64 numpy oracle (one thread, pocketfft) on a bounded sample of the
     same workload family: the 40-particle geometry generated at 80^3; one cold
@@ -142,6 +157,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-learned-eval", action="store_true",
+                    help="skip the cold-start comparison with/without the network (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -186,7 +203,8 @@ def main():
               file=sys.stderr, flush=True)
         sim_a.close()
 
-    learned_stats = learned_cold_start(pf, learned, net, pd, pd_fast, phi, xB) if net is not None else None
+    learned_stats = (learned_cold_start(pf, learned, net, pd, pd_fast, phi, xB)
+                     if net is not None and not args.no_learned_eval else None)
 
     table = kernel_table(pf, sim_m, it_m, mixed=True)
     dom_name = max(table, key=lambda k: table[k]["us_per_step"])
@@ -224,7 +242,7 @@ def main():
         "fp64_plain": result_fp64,
         "mixed_over_fp64": (value / result_fp64["steps_per_s"]) if result_fp64 else None,
         "roofline": {"kernel": dom_name, "bound": "hbm", "achieved": dom["GBps"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": pmc_traffic(dom_name),
                      "bytes_per_launch": dom["bytes"], "us_per_launch": dom["us"]},
         "fft_survey_model": fft_survey_model(table, phi.shape[0]),
         "learned_warm_start": learned_stats,

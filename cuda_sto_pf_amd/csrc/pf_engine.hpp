@@ -1790,9 +1790,13 @@ struct Engine final : EngineBase {
     const int gk = grid_for(nSc);
     switch (id) {
       case PF_K_ZPASS_HIJ: {
-        typename OpHij<T>::Args oa{P, phi[cur], xB, invN, 0};
+        // the form the solve repeats: increments through the linear part of
+        // the map (phi only) in the mixed and accelerated solvers, the full
+        // map (phi and x_B) in the reference iteration
+        const int inc = (kMixed || extended) ? 1 : 0;
+        typename OpHij<T>::Args oa{P, phi[cur], xB, invN, inc};
         launch_zpass<T, 6, 6, OpHij<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
-        return 12.0 * nSz * cs + 2.0 * R * 8.0;
+        return 12.0 * nSz * cs + (inc ? 1.0 : 2.0) * R * 8.0;
       }
       case PF_K_YPASS_FWD6: ypass(false, 0, 6); return 6.0 * (nSz + nSy) * cs;
       case PF_K_YPASS_INV6: ypass(true, 0, 6); return 6.0 * (nSz + nSy) * cs;

@@ -152,7 +152,8 @@ int pf_set_elastic_state(pf_handle* h, const double* in);
 
 /* Kernel ids for pf_time_kernel. */
 enum pf_kernel_id {
-  PF_K_ZPASS_HIJ = 0,      /* z pass: inverse 6 strains -> h_ij -> forward 6 */
+  PF_K_ZPASS_HIJ = 0,      /* z pass: inverse 6 strains -> h_ij -> forward 6 (the increment form
+                              where the solver iterates increments) */
   PF_K_YPASS_FWD6 = 1,     /* y pass, forward, 6 components */
   PF_K_YPASS_INV6 = 2,     /* y pass, inverse, 6 components */
   PF_K_XPASS_FWD6 = 3,     /* x pass, forward, 6 components */
