@@ -161,7 +161,7 @@ def main():
                     help="skip the cold-start comparison with/without the network (profiling runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
+        print(f"bench.py: measures one GPU only (--gpus {args.gpus} requested); "
               "the multi-GPU decomposition is a design note, not a measured path", file=sys.stderr)
         sys.exit(2)
 

@@ -182,3 +182,40 @@ def config(idx: int, n=None):
         raise ValueError(f"unknown workload {idx}")
     meta["xB_max_safe"] = 0.5
     return p, phi, xB, meta
+
+
+# ---------------------------------------------------------------- workload 4
+def haar_rotation(rng):
+    """Uniformly distributed rotation (QR of a Gaussian matrix, sign-fixed)."""
+    Q, Rr = np.linalg.qr(rng.standard_normal((3, 3)))
+    Q = Q * np.sign(np.diag(Rr))
+    if np.linalg.det(Q) < 0:
+        Q[:, 0] = -Q[:, 0]
+    return Q
+
+
+def rotated_eigenstrain(Q, principal=(0.046, -0.022, -0.017)):
+    """Q diag(principal) Q^T in the order xx, yy, zz, yz, xz, xy."""
+    E = Q @ np.diag(principal) @ Q.T
+    return (E[0, 0], E[1, 1], E[2, 2], E[1, 2], E[0, 2], E[0, 1])
+
+
+def sweep_instances(seed=4, counts=((128, 8), (96, 16), (64, 24))):
+    """Workload 4: 48 minimize instances of uneven size, each one sphere or
+    oblate spheroid (radius U(10, 30), aspect 1 or U(0.5, 1)) whose principal
+    eigenstrains (0.046, -0.022, -0.017) are rotated by a seeded Haar-random
+    rotation; the spheroid's axes follow the same rotation.  Heterogeneous
+    elasticity; largest grids first."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for n, k in counts:
+        for i in range(k):
+            R = float(rng.uniform(10.0, 30.0))
+            a = float(rng.uniform(0.5, 1.0)) if rng.random() < 0.5 else 1.0
+            Q = haar_rotation(rng)
+            pd = base_params(n, hetero=True, zero_mode=False)
+            pd["eps00"] = rotated_eigenstrain(Q)
+            c = (n / 2.0, n / 2.0, n / 2.0)
+            phi = spheres_phi((n, n, n), [c], [R], aspects=[a], rotations=[Q.T])
+            out.append(dict(name=f"n{n}_{i}", shape=(n, n, n), params=pd, phi=phi, radius=R, aspect=a))
+    return out

@@ -305,15 +305,15 @@ int main(int argc, char** argv) {
       }
     } else if (a == "--minimize-full-model" || a == "--minimize-full" || a == "--minimize-mass-constraint" ||
                a == "--minimize-freeze-phi" || a == "--minimize-component-volume-constraints") {
-      return fatal(a + " is not part of this build (phi-only minimize)");
+      return fatal(a + " is not implemented (phi-only minimize only)");
     } else if (starts_with(a.c_str(), "--")) {
-      std::fprintf(stderr, "[warn] flag not handled by this build: %s\n", a.c_str());
+      std::fprintf(stderr, "[warn] flag not handled by main_pf: %s\n", a.c_str());
     }
   }
   if (!param_file)
     return fatal("physical parameters have no built-in defaults: pass --pf-param-file <path>");
   if (mode != "dynamics" && mode != "minimize")
-    return fatal("--mode=" + mode + " is not part of this build (dynamics and phi-only minimize)");
+    return fatal("--mode=" + mode + " is not implemented (dynamics and phi-only minimize)");
   const bool minimize = mode == "minimize";
 
   pf_params P;
@@ -372,7 +372,7 @@ int main(int argc, char** argv) {
     xB_cap = std::isfinite(m.xB_max_safe) ? m.xB_max_safe : 0.07;   // minimize_xB_max_safe default
     if (case_tag.empty()) case_tag = "raw_fields_relax";
   } else {
-    return fatal("this build initialises from --init-mode raw_fields only");
+    return fatal("main_pf initialises from --init-mode raw_fields only");
   }
   for (size_t i = 0; i < N; ++i) xB_out += xB[i];
   xB_out /= (double)N;

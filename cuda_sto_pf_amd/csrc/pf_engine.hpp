@@ -895,7 +895,7 @@ struct Engine final : EngineBase {
     return PF_OK;
   }
 
-  // Mixed precision (main_cuda.cu:28760-29095 semantics, F6 of the plan):
+  // Mixed precision (main_cuda.cu:28760-29095 semantics):
   // iterate on FP32 increments d_{n+1} = A d_n with the FP64 sum U += d and
   // the FP64 residual |d|/|U|.  The first update of a step evaluates the full
   // affine map, later ones only its linear part -h dC:eps(d).

@@ -151,6 +151,10 @@ EXPORTS = {
                                   C.POINTER(C.c_float), C.c_void_p]),
     "pf_nn_time_inference": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_double)]),
     "pf_attach_learned_warm_start": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "pf_batch_create": (C.c_int, [C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "pf_batch_step": (C.c_int, [C.c_void_p, C.c_int]),
+    "pf_batch_minimize": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pf_batch_destroy": (C.c_int, [C.c_void_p]),
     "pf_destroy": (C.c_int, [C.c_void_p]),
     "pf_fft_forward": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "pf_fft_inverse": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
@@ -416,6 +420,42 @@ class Net:
         _check(lib().pf_nn_gradients(self._h, in7.shape[0], *[int(v) for v in shape], _ptr(in7), _ptr(target),
                                      C.byref(loss), _ptr(g)))
         return loss.value, g
+
+
+class Batch:
+    """Independent simulations driven concurrently by `threads` host workers
+    (pf_batch_*); each keeps its own stream, results equal solo runs."""
+
+    def __init__(self, sims, threads=8):
+        self.sims = list(sims)
+        arr = (C.c_void_p * len(self.sims))(*[s._h.value for s in self.sims])
+        h = C.c_void_p()
+        _check(lib().pf_batch_create(len(self.sims), arr, int(threads), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().pf_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, n=1):
+        _check(lib().pf_batch_step(self._h, int(n)))
+
+    def minimize(self, opts):
+        """opts: one PfMinimizeOpts for all instances or a list of them."""
+        n = len(self.sims)
+        if isinstance(opts, PfMinimizeOpts):
+            opts = [opts] * n
+        arr = (PfMinimizeOpts * n)(*opts)
+        st = (PfMinimizeStats * n)()
+        _check(lib().pf_batch_minimize(self._h, arr, st))
+        return [s.as_dict() for s in st]
 
 
 def minimize_opts(**kw):

@@ -1,0 +1,105 @@
+"""Workload 4: the batched minimize sweep.
+
+48 phi-only minimize instances of uneven size (24 at 64^3, 16 at 96^3, 8 at
+128^3), each one sphere or oblate spheroid with a Haar-rotated eigenstrain
+(configs.sweep_instances), run with --minimize-dt 0.05, --minimize-max-iter
+500, --V0 0.1 (the minimize branch of main_cuda.cu:28341-32600).  They run
+once one at a time and once as a batch (pf_batch_minimize: one stream per
+instance, host workers overlapping the instances on the GPU); the final phi
+of every instance is compared between the two and the throughput reported.
+
+    python -m cuda_sto_pf_amd.sweep --threads 8 --report sweep.json
+"""
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import pfcore as pf
+from .configs import sweep_instances
+
+ACCEL = dict(elastic_warm_start_order=3, elastic_anderson_depth=3)
+
+
+def make_sims(instances, prec, accel=True, net=None):
+    sims = []
+    for inst in instances:
+        pd = dict(inst["params"], **ACCEL) if accel else inst["params"]
+        s = pf.Simulation(pd, prec)
+        s.set_fields(inst["phi"], np.zeros_like(inst["phi"]))
+        if net is not None:
+            s.attach_net(net)
+        sims.append(s)
+    return sims
+
+
+def run(instances, threads, opts, prec, accel=True, net=None):
+    """Wall time, per-instance stats and final phi of one sweep."""
+    sims = make_sims(instances, prec, accel, net)
+    for s in sims:
+        s.sync()
+    t0 = time.perf_counter()
+    if threads <= 1:
+        stats = [s.minimize(opts) for s in sims]
+    else:
+        b = pf.Batch(sims, threads)
+        stats = b.minimize(opts)
+        b.close()
+    for s in sims:
+        s.sync()
+    wall = time.perf_counter() - t0
+    phis = [s.field(pf.PF_PHI) for s in sims]
+    for s in sims:
+        s.close()
+    return wall, stats, phis
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--threads", type=int, default=8)
+    ap.add_argument("--max-iter", type=int, default=500)
+    ap.add_argument("--dt", type=float, default=0.05)
+    ap.add_argument("--V0", type=float, default=0.1)
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed")
+    ap.add_argument("--learned", action="store_true", help="learned cold start of every instance")
+    ap.add_argument("--subset", type=int, default=0, help="only the first n instances")
+    ap.add_argument("--report", default=None)
+    a = ap.parse_args(argv)
+    log = lambda s: print(s, flush=True)
+    inst = sweep_instances()
+    if a.subset:
+        inst = inst[:a.subset]
+    prec = pf.PF_FP64 if a.precision == "fp64" else pf.PF_MIXED
+    net = None
+    if a.learned:
+        from . import learned
+        net = learned.load()
+    opts = pf.minimize_opts(dt=a.dt, V0=a.V0, max_iter=a.max_iter)
+    log(f"[sweep] {len(inst)} instances, {a.precision}, max_iter {a.max_iter}, learned={a.learned}")
+    t_seq, st_seq, phi_seq = run(inst, 1, opts, prec, net=net)
+    it_seq = sum(s["iteration"] for s in st_seq)
+    log(f"[sweep] one at a time: {t_seq:.2f} s, {it_seq} iterations, {it_seq / t_seq:.0f} iterations/s")
+    t_bat, st_bat, phi_bat = run(inst, a.threads, opts, prec, net=net)
+    it_bat = sum(s["iteration"] for s in st_bat)
+    log(f"[sweep] batch ({a.threads} workers): {t_bat:.2f} s, {it_bat} iterations, {it_bat / t_bat:.0f} iterations/s")
+    dmax = max(float(np.abs(x - y).max()) for x, y in zip(phi_seq, phi_bat))
+    same_iters = all(x["iteration"] == y["iteration"] for x, y in zip(st_seq, st_bat))
+    conv = sum(s["converged"] for s in st_bat)
+    log(f"[sweep] max|phi batch - phi alone| = {dmax:.1e}, same iteration counts: {same_iters}, converged {conv}")
+    rep = dict(instances=len(inst), precision=a.precision, learned=a.learned, max_iter=a.max_iter, dt=a.dt, V0=a.V0,
+               threads=a.threads, one_at_a_time_s=t_seq, batch_s=t_bat, speedup=t_seq / t_bat,
+               iterations=it_bat, iterations_per_s_batch=it_bat / t_bat, iterations_per_s_alone=it_seq / t_seq,
+               converged=conv, max_abs_dphi=dmax, same_iteration_counts=same_iters,
+               per_instance=[dict(name=i["name"], radius=i["radius"], aspect=i["aspect"], iterations=s["iteration"],
+                                  converged=s["converged"], mean_h=s["mean_h"], rms_res=s["rms_res"])
+                             for i, s in zip(inst, st_bat)])
+    if a.report:
+        with open(a.report, "w") as f:
+            json.dump(rep, f, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -308,6 +308,20 @@ int pf_nn_time_inference(pf_nn* net, int Nx, int Ny, int Nz, int reps, double* u
  * detaches).  The network must outlive the attachment. */
 int pf_attach_learned_warm_start(pf_handle* h, pf_nn* net);
 
+/* ---------------------------------------------------------------------------
+ * Batched sweep: many independent instances (e.g. the 48 minimize runs of
+ * uneven size of workload 4) driven concurrently.  Each handle keeps its own
+ * stream; `threads` host workers take instances in the given order, so the
+ * kernels of different instances overlap on the GPU.  Results equal those of
+ * running each handle alone.  The handles must outlive the batch. */
+typedef struct pf_batch pf_batch;
+int pf_batch_create(int n, pf_handle* const* handles, int threads, pf_batch** out);
+/* nsteps dynamics steps on every instance */
+int pf_batch_step(pf_batch* b, int nsteps);
+/* pf_minimize on every instance with opts[i]; stats (may be NULL) gets n entries */
+int pf_batch_minimize(pf_batch* b, const pf_minimize_opts* opts, pf_minimize_stats* stats);
+int pf_batch_destroy(pf_batch* b);
+
 /* Synchronise the handle's stream. */
 int pf_sync(pf_handle* h);
 

@@ -1,0 +1,40 @@
+"""Workload 4 in reduced form (GPU): a batch of minimize instances of three
+sizes with rotated eigenstrains, run concurrently by pf_batch_minimize,
+reproduces every instance's solo run exactly (same phi, same iteration
+count); the batched dynamics entry does the same for time steps."""
+import numpy as np
+import pytest
+
+from cuda_sto_pf_amd import sweep
+from cuda_sto_pf_amd.configs import config, sweep_instances
+
+
+@pytest.mark.gpu
+def test_batch_minimize_equals_solo_runs(pf):
+    inst = sweep_instances()
+    pick = [inst[0], inst[8], inst[9], inst[24], inst[25], inst[47]]   # 128^3, 96^3, 64^3
+    opts = pf.minimize_opts(dt=0.05, V0=0.1, max_iter=15)
+    t1, s1, p1 = sweep.run(pick, 1, opts, pf.PF_MIXED)
+    t2, s2, p2 = sweep.run(pick, 4, opts, pf.PF_MIXED)
+    print(f"solo {t1:.2f} s, batch {t2:.2f} s", flush=True)
+    for a, b, x, y in zip(s1, s2, p1, p2):
+        assert a["iteration"] == b["iteration"] == 15
+        assert np.array_equal(x, y)
+        assert abs(b["mean_h"] - 0.1) < 1e-3
+
+
+@pytest.mark.gpu
+def test_batch_step_equals_solo_steps(pf):
+    sims, solo = [], []
+    for n in (16, 20, 24):
+        pd, phi, xB, meta = config(2, n)
+        for lst in (sims, solo):
+            s = pf.Simulation(pd, pf.PF_FP64)
+            s.set_fields(phi, xB, xB_cap=meta["xB_max_safe"])
+            lst.append(s)
+    b = pf.Batch(sims, threads=3)
+    b.step(3)
+    for s in solo:
+        s.step(3)
+    for s, t in zip(sims, solo):
+        assert np.array_equal(s.field(pf.PF_XB), t.field(pf.PF_XB))
