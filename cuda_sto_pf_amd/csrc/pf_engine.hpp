@@ -337,6 +337,7 @@ __global__ void __launch_bounds__(NT) kop_force_map(ForceMapArgs<T> fa) {
     double kx, ky, kz;
     int iz;
     kvec(fa.d, c, kx, ky, kz, iz);
+    if (iz >= fa.d.Azr) continue;   // kz padding: left at zero
     double M[6];
     bool ok = c != 0;
     if (ok) {
@@ -462,7 +463,9 @@ struct Engine final : EngineBase {
   hipStream_t stream = nullptr;
   int Nx = 0, Ny = 0, Nz = 0;
   AxisPlan ax, ay, az;
-  long R = 0, nSz = 0, nSy = 0, nSc = 0;
+  long R = 0, nSz = 0, nSy = 0, nSc = 0;   // spectral array sizes (kz stride AzS)
+  long nSzA = 0, nSyA = 0, nScA = 0;       // retained modes only (Az), for byte counts
+  int AzS = 0;
   // device memory
   double *phi[2] = {nullptr, nullptr}, *Y = nullptr, *Ystar = nullptr, *xB = nullptr, *dYdt = nullptr;
   T* trs = nullptr;
@@ -528,7 +531,7 @@ struct Engine final : EngineBase {
   }
 
   KDims kd() const {
-    return KDims{ax.A, ay.A, az.A, kx, ky, kz, (Nz % 2) == 0};
+    return KDims{ax.A, ay.A, AzS, kx, ky, kz, (Nz % 2) == 0, az.A};
   }
 
   int init(const pf_params& p) override {
@@ -546,9 +549,20 @@ struct Engine final : EngineBase {
     if (az.A < 1 || ax.A < 1 || ay.A < 1 || 2 * az.A > Nz + 1)
       return fail(PF_E_ARG, "degenerate retained-mode set");
     R = (long)Nx * Ny * Nz;
-    nSz = (long)Nx * Ny * az.A;
-    nSy = (long)Nx * ay.A * az.A;
-    nSc = (long)ax.A * ay.A * az.A;
+    {
+      // kz stride: whole 128-byte lines on large grids (aligned column tiles
+      // in the y and x passes), the retained count on small ones
+      const int line = 128 / (int)sizeof(C);
+      AzS = az.A >= 40 ? (az.A + line - 1) / line * line : az.A;
+      if (const char* e = std::getenv("PF_KZ_PAD"))
+        if (e[0] == '0') AzS = az.A;
+    }
+    nSz = (long)Nx * Ny * AzS;
+    nSy = (long)Nx * ay.A * AzS;
+    nSc = (long)ax.A * ay.A * AzS;
+    nSzA = (long)Nx * Ny * az.A;
+    nSyA = (long)Nx * ay.A * az.A;
+    nScA = (long)ax.A * ay.A * az.A;
     PF_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
     int rc;
     if ((rc = dalloc(&phi[0], R)) || (rc = dalloc(&phi[1], R)) || (rc = dalloc(&Y, R)) ||
@@ -568,14 +582,33 @@ struct Engine final : EngineBase {
     }
     auto wx = twiddles<T>(Nx), wy = twiddles<T>(Ny), wz = twiddles<T>(Nz);
     if ((rc = dalloc(&twx, Nx)) || (rc = dalloc(&twy, Ny)) || (rc = dalloc(&twz, Nz)) ||
-        (rc = dalloc(&kx, ax.A)) || (rc = dalloc(&ky, ay.A)) || (rc = dalloc(&kz, az.A)))
+        (rc = dalloc(&kx, ax.A)) || (rc = dalloc(&ky, ay.A)) || (rc = dalloc(&kz, AzS)))
       return rc;
     PF_HIP(hipMemcpy(twx, wx.data(), Nx * sizeof(C), hipMemcpyHostToDevice));
     PF_HIP(hipMemcpy(twy, wy.data(), Ny * sizeof(C), hipMemcpyHostToDevice));
     PF_HIP(hipMemcpy(twz, wz.data(), Nz * sizeof(C), hipMemcpyHostToDevice));
     PF_HIP(hipMemcpy(kx, ax.k.data(), ax.A * sizeof(double), hipMemcpyHostToDevice));
     PF_HIP(hipMemcpy(ky, ay.k.data(), ay.A * sizeof(double), hipMemcpyHostToDevice));
-    PF_HIP(hipMemcpy(kz, az.k.data(), az.A * sizeof(double), hipMemcpyHostToDevice));
+    {
+      std::vector<double> kzp(AzS, 0.0);   // padding entries: k = 0
+      std::copy(az.k.begin(), az.k.begin() + az.A, kzp.begin());
+      PF_HIP(hipMemcpy(kz, kzp.data(), AzS * sizeof(double), hipMemcpyHostToDevice));
+    }
+    // spectral arrays start at zero so the kz padding holds zeros
+    {
+      const std::pair<void*, size_t> zero[] = {
+          {Sz, 6 * nSz * sizeof(C)}, {Sy, 6 * nSy * sizeof(C)}, {Sc, 6 * nSc * sizeof(C)},
+          {u, 3 * nSc * sizeof(C)},  {un, 3 * nSc * sizeof(C)}};
+      for (const auto& z : zero) PF_HIP(hipMemset(z.first, 0, z.second));
+      if (Ud) PF_HIP(hipMemset(Ud, 0, 3 * nSc * sizeof(cx<double>)));
+      if (extended) {
+        PF_HIP(hipMemset(Gd, 0, 3 * nSc * sizeof(cx<double>)));
+        PF_HIP(hipMemset(Hist, 0, 9 * nSc * sizeof(cx<double>)));
+        PF_HIP(hipMemset(dinc, 0, 3 * nSc * sizeof(C)));
+        PF_HIP(hipMemset(dU, 0, (size_t)AA_MAX * 3 * nSc * sizeof(C)));
+        PF_HIP(hipMemset(dG, 0, (size_t)AA_MAX * 3 * nSc * sizeof(C)));
+      }
+    }
     // partial-sum scratch: largest grid times 4 reductions
     long maxg = 2048;
     const int rows = Nx * Ny;
@@ -599,6 +632,7 @@ struct Engine final : EngineBase {
     ZPassArgs<T> za{};
     za.rows = Nx * Ny;
     za.Az = az.A;
+    za.AzS = AzS;
     for (int c = 0; c < nci; ++c) za.in[c] = sz(c_in0 + c);
     for (int c = 0; c < nco; ++c) za.out[c] = sz(c_out0 + c);
     za.tw = twz;
@@ -609,23 +643,23 @@ struct Engine final : EngineBase {
   // y pass over comps [c0, c0+nc): Sz <-> Sy
   void ypass(bool inv, int c0, int nc) {
     CPassArgs<T> ca{};
-    ca.cols = az.A;
+    ca.cols = AzS;
     ca.A = ay.A;
     ca.a = ay.a;
     ca.tw = twy;
     if (!inv) {
       ca.in = sz(c0);
       ca.out = sy(c0);
-      ca.in_pstride = (long)Ny * az.A;
-      ca.out_pstride = (long)ay.A * az.A;
+      ca.in_pstride = (long)Ny * AzS;
+      ca.out_pstride = (long)ay.A * AzS;
       ca.in_cstride = nSz;
       ca.out_cstride = nSy;
       launch_cpass<T, false>(Ny, stream, ca, Nx, nc);
     } else {
       ca.in = sy(c0);
       ca.out = sz(c0);
-      ca.in_pstride = (long)ay.A * az.A;
-      ca.out_pstride = (long)Ny * az.A;
+      ca.in_pstride = (long)ay.A * AzS;
+      ca.out_pstride = (long)Ny * AzS;
       ca.in_cstride = nSy;
       ca.out_cstride = nSz;
       launch_cpass<T, true>(Ny, stream, ca, Nx, nc);
@@ -634,7 +668,7 @@ struct Engine final : EngineBase {
   // x pass: Sy <-> compact (dst/src compact base with component stride nSc)
   void xpass(bool inv, int c0, int nc, C* compact) {
     CPassArgs<T> ca{};
-    ca.cols = ay.A * az.A;
+    ca.cols = ay.A * AzS;
     ca.A = ax.A;
     ca.a = ax.a;
     ca.tw = twx;
@@ -1744,7 +1778,7 @@ struct Engine final : EngineBase {
       for (int cy = 0; cy < ay.A; ++cy) {
         const int iy = cy < ay.a ? cy : cy + Ny - ay.A;
         for (int cz = 0; cz < az.A; ++cz) {
-          const C v = c[((size_t)cx_ * ay.A + cy) * az.A + cz];
+          const C v = c[((size_t)cx_ * ay.A + cy) * AzS + cz];
           const size_t o = ((size_t)ix * Ny + iy) * nzc + cz;
           out_half[2 * o] = double(v.re);
           out_half[2 * o + 1] = double(v.im);
@@ -1762,7 +1796,7 @@ struct Engine final : EngineBase {
         const int iy = cy < ay.a ? cy : cy + Ny - ay.A;
         for (int cz = 0; cz < az.A; ++cz) {
           const size_t o = ((size_t)ix * Ny + iy) * nzc + cz;
-          c[((size_t)cx_ * ay.A + cy) * az.A + cz] = C{T(in_half[2 * o]), T(in_half[2 * o + 1])};
+          c[((size_t)cx_ * ay.A + cy) * AzS + cz] = C{T(in_half[2 * o]), T(in_half[2 * o + 1])};
         }
       }
     }
@@ -1796,12 +1830,12 @@ struct Engine final : EngineBase {
         const int inc = (kMixed || extended) ? 1 : 0;
         typename OpHij<T>::Args oa{P, phi[cur], xB, invN, inc};
         launch_zpass<T, 6, 6, OpHij<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
-        return 12.0 * nSz * cs + (inc ? 1.0 : 2.0) * R * 8.0;
+        return 12.0 * nSzA * cs + (inc ? 1.0 : 2.0) * R * 8.0;
       }
-      case PF_K_YPASS_FWD6: ypass(false, 0, 6); return 6.0 * (nSz + nSy) * cs;
-      case PF_K_YPASS_INV6: ypass(true, 0, 6); return 6.0 * (nSz + nSy) * cs;
-      case PF_K_XPASS_FWD6: xpass(false, 0, 6, Sc); return 6.0 * (nSy + nSc) * cs;
-      case PF_K_XPASS_INV6: xpass(true, 0, 6, Sc); return 6.0 * (nSy + nSc) * cs;
+      case PF_K_YPASS_FWD6: ypass(false, 0, 6); return 6.0 * (nSzA + nSyA) * cs;
+      case PF_K_YPASS_INV6: ypass(true, 0, 6); return 6.0 * (nSzA + nSyA) * cs;
+      case PF_K_XPASS_FWD6: xpass(false, 0, 6, Sc); return 6.0 * (nSyA + nScA) * cs;
+      case PF_K_XPASS_INV6: xpass(true, 0, 6, Sc); return 6.0 * (nSyA + nScA) * cs;
       case PF_K_GREEN: {
         if (extended) {
           const int m = prm.elastic_anderson_depth;
@@ -1821,7 +1855,7 @@ struct Engine final : EngineBase {
           ga.partials = partials;
           for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
           hipLaunchKernelGGL(kop_green_aa<T>, dim3(gk), dim3(NT), 0, stream, ga);
-          return (6.0 + 3.0 + 3.0 * m + (m > 0 ? 3.0 : 0.0)) * nSc * cs + 9.0 * nSc * 16.0;
+          return (6.0 + 3.0 + 3.0 * m + (m > 0 ? 3.0 : 0.0)) * nScA * cs + 9.0 * nScA * 16.0;
         }
         GreenArgs<T> ga{};
         ga.d = d;
@@ -1838,18 +1872,18 @@ struct Engine final : EngineBase {
         ga.have_old = 1;
         ga.partials = partials;
         hipLaunchKernelGGL(kop_green<T>, dim3(gk), dim3(NT), 0, stream, ga);
-        return (12.0 + 6.0) * nSc * cs;
+        return (12.0 + 6.0) * nScA * cs;
       }
       case PF_K_ZPASS_PHIRHS: {
         typename OpPhiRhs<T, true>::Args oa{P, phi[cur], xB, trs, invN};
         launch_zpass<T, 6, 1, OpPhiRhs<T, true>>(Nz, stream, zargs(0, 6, 0, 1), oa);
-        return 7.0 * nSz * cs + 2.0 * R * 8.0 + R * (double)sizeof(T);
+        return 7.0 * nSzA * cs + 2.0 * R * 8.0 + R * (double)sizeof(T);
       }
       case PF_K_ZPASS_FLUX: {
         const double* ph = phi[cur];
         typename OpFlux<T>::Args oa{P, ph, xB, ph, phi[1 - cur], Y, dYdt, invN, 1, 0, 0, 1.0};
         launch_zpass<T, 3, 5, OpFlux<T>>(Nz, stream, zargs(0, 3, 0, 5), oa);
-        return 8.0 * nSz * cs + 6.0 * R * 8.0;
+        return 8.0 * nSzA * cs + 6.0 * R * 8.0;
       }
       default:
         return 0.0;

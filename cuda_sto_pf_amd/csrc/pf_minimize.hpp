@@ -264,6 +264,7 @@ __global__ void __launch_bounds__(NT) kop_lap(KDims d, long n, const cx<T>* in, 
     double kx, ky, kz;
     int iz;
     kvec(d, i, kx, ky, kz, iz);
+    if (iz >= d.Azr) continue;   // kz padding: left at zero
     const double k2 = kx * kx + ky * ky + kz * kz;
     out[i] = cx<T>{T(-k2 * double(in[i].re)), T(-k2 * double(in[i].im))};
   }

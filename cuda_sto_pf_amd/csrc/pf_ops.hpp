@@ -341,6 +341,7 @@ struct KDims {
   const double* ky;   // [Ay]
   const double* kz;   // [Az]
   int Nz_even;
+  int Azr;            // retained kz count (Az is the row stride; iz >= Azr is padding)
 };
 
 __device__ __forceinline__ void kvec(const KDims& d, long c, double& kx, double& ky, double& kz,
@@ -438,6 +439,7 @@ __global__ void __launch_bounds__(NT) kop_green(GreenArgs<T> ga) {
     double kx, ky, kz;
     int iz;
     kvec(ga.d, c, kx, ky, kz, iz);
+    if (iz >= ga.d.Azr) continue;   // kz padding: left at zero
     cx<double> hh[6];
 #pragma unroll
     for (int i = 0; i < 6; ++i) hh[i] = cx<double>{double(ga.h[i][c].re), double(ga.h[i][c].im)};
@@ -467,7 +469,7 @@ __global__ void __launch_bounds__(NT) kop_green(GreenArgs<T> ga) {
       // Hermitian weight of compute_elastic_displacement_residual_kernel
       // (cuda_kernels.cu:6436-6467): 1 on the kz = 0 plane, 2 elsewhere (the
       // Nyquist plane is never retained).
-      const double w = (iz == 0) ? 1.0 : 2.0;
+      const double w = (iz >= ga.d.Azr) ? 0.0 : ((iz == 0) ? 1.0 : 2.0);
       double dsum = 0.0, rsum = 0.0;
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
@@ -515,6 +517,7 @@ __global__ void __launch_bounds__(NT) kop_green_acc(GreenAccArgs<T> ga) {
     double kx, ky, kz;
     int iz;
     kvec(ga.d, c, kx, ky, kz, iz);
+    if (iz >= ga.d.Azr) continue;   // kz padding: left at zero
     cx<double> hh[6];
 #pragma unroll
     for (int i = 0; i < 6; ++i) hh[i] = cx<double>{double(ga.h[i][c].re), double(ga.h[i][c].im)};
@@ -549,7 +552,7 @@ __global__ void __launch_bounds__(NT) kop_green_acc(GreenAccArgs<T> ga) {
       ga.U[i][c] = un[i];
     }
     if (ga.partials) {
-      const double w = (iz == 0) ? 1.0 : 2.0;
+      const double w = (iz >= ga.d.Azr) ? 0.0 : ((iz == 0) ? 1.0 : 2.0);
       double dsum = 0.0, rsum = 0.0;
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
@@ -628,6 +631,7 @@ __global__ void __launch_bounds__(NT) kop_green_aa(GreenAAArgs<T> a) {
     double kx, ky, kz;
     int iz;
     kvec(a.d, c, kx, ky, kz, iz);
+    if (iz >= a.d.Azr) continue;   // kz padding: left at zero
     cx<double> hh[6];
 #pragma unroll
     for (int i = 0; i < 6; ++i) hh[i] = cx<double>{double(a.h[i][c].re), double(a.h[i][c].im)};
@@ -649,7 +653,7 @@ __global__ void __launch_bounds__(NT) kop_green_aa(GreenAAArgs<T> a) {
       }
       a.g[i][c] = gn[i];
     }
-    const double w = (iz == 0) ? 1.0 : 2.0;
+    const double w = (iz >= a.d.Azr) ? 0.0 : ((iz == 0) ? 1.0 : 2.0);
     cx<double> Fu[3] = {{U[0].re + gn[0].re, U[0].im + gn[0].im},
                         {U[1].re + gn[1].re, U[1].im + gn[1].im},
                         {U[2].re + gn[2].re, U[2].im + gn[2].im}};
@@ -711,6 +715,7 @@ __global__ void __launch_bounds__(NT) kop_aa_update(AAUpdateArgs<T> a) {
     double kx, ky, kz;
     int iz;
     kvec(a.d, c, kx, ky, kz, iz);
+    if (iz >= a.d.Azr) continue;   // kz padding: left at zero
     cx<double> dd[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -751,6 +756,7 @@ __global__ void __launch_bounds__(NT) kop_aa_final(KDims d, long n, cx<double>* 
     double kx, ky, kz;
     int iz;
     kvec(d, c, kx, ky, kz, iz);
+    if (iz >= d.Azr) continue;   // kz padding: left at zero
     cx<double> u[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -794,6 +800,7 @@ __global__ void __launch_bounds__(NT) kop_strain(StrainArgs<T, UT> sa) {
     double kx, ky, kz;
     int iz;
     kvec(sa.d, c, kx, ky, kz, iz);
+    if (iz >= sa.d.Azr) continue;   // kz padding: left at zero
     cx<double> u[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) u[i] = cx<double>{double(sa.u[i][c].re), double(sa.u[i][c].im)};
@@ -808,6 +815,7 @@ __global__ void __launch_bounds__(NT) kop_phi(KDims d, long n, cx<T>* g, double 
     double kx, ky, kz;
     int iz;
     kvec(d, c, kx, ky, kz, iz);
+    if (iz >= d.Azr) continue;   // kz padding: left at zero
     const double k2 = kx * kx + ky * ky + kz * kz;
     const double den = 1.0 + Ldt_kappa * k2;
     cx<T> v = g[c];
@@ -823,6 +831,7 @@ __global__ void __launch_bounds__(NT) kop_grad(KDims d, long n, const cx<T>* mu,
     double kx, ky, kz;
     int iz;
     kvec(d, c, kx, ky, kz, iz);
+    if (iz >= d.Azr) continue;   // kz padding: left at zero
     const double mr = double(mu[c].re), mi = double(mu[c].im);
     gx[c] = cx<T>{T(-kx * mi), T(kx * mr)};
     gy[c] = cx<T>{T(-ky * mi), T(ky * mr)};
@@ -843,6 +852,7 @@ __global__ void __launch_bounds__(NT) kop_yupdate(KDims d, long n, const cx<T>* 
     double kx, ky, kz;
     int iz;
     kvec(d, c, kx, ky, kz, iz);
+    if (iz >= d.Azr) continue;   // kz padding: left at zero
     const double k2 = kx * kx + ky * ky + kz * kz;
     // div J^ = i (kx Jx + ky Jy + kz Jz), accumulated x, y, z in order
     double dr = 0.0, di = 0.0;

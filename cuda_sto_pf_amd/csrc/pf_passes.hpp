@@ -9,9 +9,12 @@
 // reference's own floating-point expression.
 //
 // Layouts (complex, last index fastest):
-//   Sz  [Nx][Ny][Az]   after the z pass (real rows -> Az retained kz)
-//   Sy  [Nx][Ay][Az]   after the y pass
-//   Sc  [Ax][Ay][Az]   compact spectrum after the x pass
+//   Sz  [Nx][Ny][AzS]  after the z pass (real rows -> Az retained kz)
+//   Sy  [Nx][Ay][AzS]  after the y pass
+//   Sc  [Ax][Ay][AzS]  compact spectrum after the x pass
+// The kz stride AzS >= Az is rounded up to whole 128-byte lines on large
+// grids so the column tiles of the y and x passes are line-aligned; the
+// padding entries hold zeros, and every operator maps zeros to zeros.
 // Compact index c of a wrapped axis maps to the full index c (c < a) or
 // c + N - A (c >= a), A = 2a - 1.
 #pragma once
@@ -79,6 +82,7 @@ template <typename T>
 struct ZPassArgs {
   int rows;          // Nx*Ny
   int Az;            // retained kz count
+  int AzS;           // kz stride of the spectral rows (>= Az)
   const cx<T>* in[6];
   cx<T>* out[6];
   const cx<T>* tw;   // twiddles of length Nz
@@ -106,6 +110,7 @@ __global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op:
 
   const int row0 = blockIdx.x * RT;
   const int Az = za.Az;
+  const int AzS = za.AzS;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   for (int i = threadIdx.x; i < N; i += NT) tw[i] = za.tw[i];
   // real-space operands of the operator, issued before the inverse transform
@@ -136,8 +141,8 @@ __global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op:
       const int p = cp % NP;
       const int c = cp / NP;
       const int ra = row0 + 2 * p, rb = ra + 1;
-      const cx<T>* ia = za.in[c] + (size_t)ra * Az;
-      const cx<T>* ib = za.in[c] + (size_t)rb * Az;
+      const cx<T>* ia = za.in[c] + (size_t)ra * AzS;
+      const cx<T>* ib = za.in[c] + (size_t)rb * AzS;
       for (int k = lane; k < Az; k += 64) {
         cx<T> A = (ra < za.rows) ? ia[k] : cx<T>{T(0), T(0)};
         cx<T> B = (rb < za.rows) ? ib[k] : cx<T>{T(0), T(0)};
@@ -211,8 +216,8 @@ __global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op:
       const int p = cp % NP;
       const int c = cp / NP;
       const int ra = row0 + 2 * p, rb = ra + 1;
-      cx<T>* oa_ = za.out[c] + (size_t)ra * Az;
-      cx<T>* ob_ = za.out[c] + (size_t)rb * Az;
+      cx<T>* oa_ = za.out[c] + (size_t)ra * AzS;
+      cx<T>* ob_ = za.out[c] + (size_t)rb * AzS;
       for (int k = lane; k < Az; k += 64) {
         const cx<T> Zk = res[LY::idx(cp, k)];
         const cx<T> Zm = res[LY::idx(cp, k == 0 ? 0 : N - k)];
@@ -230,7 +235,7 @@ template <typename T>
 struct CPassArgs {
   const cx<T>* in;
   cx<T>* out;
-  int cols;                 // contiguous columns per j (Az for y, Ay*Az for x)
+  int cols;                 // contiguous columns per j (AzS for y, Ay*AzS for x)
   long in_pstride, out_pstride;   // plane strides (elements)
   long in_cstride, out_cstride;   // component strides (elements)
   int A, a;                 // compact count and retained half-width

@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(64) zpass_reg_kernel(ZPassArgs<T> za, typename
   const int row = blockIdx.x * G + grp;
   const bool live = row < za.rows;
   const int Az = za.Az;
-  const size_t rb = (size_t)row * Az;
+  const size_t rb = (size_t)row * za.AzS;
   cx<T> v[PM][E];
 
   // ---- inverse: Hermitian extension of the retained kz, two channels per transform
