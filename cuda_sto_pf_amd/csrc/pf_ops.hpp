@@ -660,26 +660,35 @@ __global__ void __launch_bounds__(NT) kop_green_aa(GreenAAArgs<T> a) {
     acc[0] += w * wdot(gn, gn);
     acc[1] += w * wdot(Fu, Fu);
     if (a.mode == 1) {
-      cx<T> q[3];
+      // the T-rounded residual difference (kept in registers: q0..q2)
+      const cx<T> q0 = {T(dgn[0].re), T(dgn[0].im)};
+      const cx<T> q1 = {T(dgn[1].re), T(dgn[1].im)};
+      const cx<T> q2 = {T(dgn[2].re), T(dgn[2].im)};
+      const cx<double> dq[3] = {{double(q0.re), double(q0.im)}, {double(q1.re), double(q1.im)},
+                                {double(q2.re), double(q2.im)}};
 #pragma unroll
-      for (int i = 0; i < 3; ++i) q[i] = cx<T>{T(dgn[i].re), T(dgn[i].im)};
-#pragma unroll
-      for (int j = 0; j < AA_MAX; ++j)
-        if (j == a.slot)
-#pragma unroll
-          for (int i = 0; i < 3; ++i) a.dg[j][i][c] = q[i];
+      for (int j = 0; j < AA_MAX; ++j) {
+        if (j == a.slot) {
+          a.dg[j][0][c] = q0;
+          a.dg[j][1][c] = q1;
+          a.dg[j][2][c] = q2;
+        }
+      }
 #pragma unroll
       for (int j = 0; j < AA_MAX; ++j) {
         if (j < a.nhist) {
           cx<double> v[3];
+          if (j == a.slot) {
+            v[0] = dq[0];
+            v[1] = dq[1];
+            v[2] = dq[2];
+          } else {
 #pragma unroll
-          for (int i = 0; i < 3; ++i) {
-            cx<T> t = (j == a.slot) ? q[i] : a.dg[j][i][c];
-            v[i] = cx<double>{double(t.re), double(t.im)};
+            for (int i = 0; i < 3; ++i) {
+              const cx<T> t = a.dg[j][i][c];
+              v[i] = cx<double>{double(t.re), double(t.im)};
+            }
           }
-          cx<double> dq[3] = {{double(q[0].re), double(q[0].im)},
-                              {double(q[1].re), double(q[1].im)},
-                              {double(q[2].re), double(q[2].im)}};
           acc[2 + j] += w * wdot(dq, v);
           acc[2 + AA_MAX + j] += w * wdot(gn, v);
         }
