@@ -548,6 +548,8 @@ struct Engine final : EngineBase {
     az = make_axis(Nz, p.dz, false);
     if (az.A < 1 || ax.A < 1 || ay.A < 1 || 2 * az.A > Nz + 1)
       return fail(PF_E_ARG, "degenerate retained-mode set");
+    if (3 * (az.A - 1) > Nz)   // the z pass holds at most Nz/3 + 1 retained kz per lane block
+      return fail(PF_E_ARG, "retained kz count exceeds the 2/3 rule bound");
     R = (long)Nx * Ny * Nz;
     {
       // kz stride: whole 128-byte lines on large grids (aligned column tiles

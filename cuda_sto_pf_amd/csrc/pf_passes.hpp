@@ -35,6 +35,13 @@ constexpr int zpass_rows() {
   return 2 * p;
 }
 
+// Upper bound (a multiple of 64) on the retained kz count: the 2/3 rule keeps
+// kz < N/3, so Az <= N/3 + 1.  The engine rejects grids that exceed it.
+template <int N>
+constexpr int zpass_kmax() {
+  return ((N / 3 + 1 + 63) / 64) * 64;
+}
+
 template <typename T, int N>
 constexpr int cpass_cols() {
   constexpr int per_col = (N + N / 16) * (int)sizeof(cx<T>) * (plan_has_big_prime(N) ? 2 : 1);
@@ -137,21 +144,63 @@ __global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op:
       const int k = e % N;
       if (k >= Az && N - k >= Az) buf[LY::idx(e / N, k)] = cx<T>{T(0), T(0)};
     }
-    for (int cp = wave; cp < NCI * NP; cp += NT / 64) {
+    // all retained-mode loads of a wave are issued before the LDS scatter
+    // (when they fit a register budget; small grids use the loop below)
+    constexpr int CPW0 = (NCI * NP + NT / 64 - 1) / (NT / 64);
+    constexpr int KI = zpass_kmax<N>() / 64;
+    constexpr bool BATCHED = CPW0 * KI * 2 <= 24;
+    constexpr int CPW = BATCHED ? CPW0 : 0;
+    cx<T> va[CPW > 0 ? CPW : 1][KI], vb[CPW > 0 ? CPW : 1][KI];
+#pragma unroll
+    for (int q = 0; q < CPW; ++q) {
+      const int cp = wave + q * (NT / 64);
       const int p = cp % NP;
-      const int c = cp / NP;
+      const int c = cp < NCI * NP ? cp / NP : 0;
       const int ra = row0 + 2 * p, rb = ra + 1;
+      const bool live = cp < NCI * NP;
       const cx<T>* ia = za.in[c] + (size_t)ra * AzS;
       const cx<T>* ib = za.in[c] + (size_t)rb * AzS;
-      for (int k = lane; k < Az; k += 64) {
-        cx<T> A = (ra < za.rows) ? ia[k] : cx<T>{T(0), T(0)};
-        cx<T> B = (rb < za.rows) ? ib[k] : cx<T>{T(0), T(0)};
+#pragma unroll
+      for (int i = 0; i < KI; ++i) {
+        const int k = lane + 64 * i;
+        va[q][i] = (live && k < Az && ra < za.rows) ? ia[k] : cx<T>{T(0), T(0)};
+        vb[q][i] = (live && k < Az && rb < za.rows) ? ib[k] : cx<T>{T(0), T(0)};
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < CPW; ++q) {
+      const int cp = wave + q * (NT / 64);
+      if (cp >= NCI * NP) continue;
+#pragma unroll
+      for (int i = 0; i < KI; ++i) {
+        const int k = lane + 64 * i;
+        if (k >= Az) continue;
+        cx<T> A = va[q][i], B = vb[q][i];
         if (k == 0) {
           A.im = T(0);
           B.im = T(0);
         }
         buf[LY::idx(cp, k)] = cx<T>{A.re - B.im, A.im + B.re};
         if (k > 0) buf[LY::idx(cp, N - k)] = cx<T>{A.re + B.im, B.re - A.im};
+      }
+    }
+    if constexpr (!BATCHED) {
+      for (int cp = wave; cp < NCI * NP; cp += NT / 64) {
+        const int p = cp % NP;
+        const int c = cp / NP;
+        const int ra = row0 + 2 * p, rb = ra + 1;
+        const cx<T>* ia = za.in[c] + (size_t)ra * AzS;
+        const cx<T>* ib = za.in[c] + (size_t)rb * AzS;
+        for (int k = lane; k < Az; k += 64) {
+          cx<T> A = (ra < za.rows) ? ia[k] : cx<T>{T(0), T(0)};
+          cx<T> B = (rb < za.rows) ? ib[k] : cx<T>{T(0), T(0)};
+          if (k == 0) {
+            A.im = T(0);
+            B.im = T(0);
+          }
+          buf[LY::idx(cp, k)] = cx<T>{A.re - B.im, A.im + B.re};
+          if (k > 0) buf[LY::idx(cp, N - k)] = cx<T>{A.re + B.im, B.re - A.im};
+        }
       }
     }
     __syncthreads();
@@ -258,22 +307,43 @@ __global__ void __launch_bounds__(NT) cpass_kernel(CPassArgs<T> ca) {
   const cx<T>* in = ca.in + comp * ca.in_cstride + plane * ca.in_pstride;
   cx<T>* out = ca.out + comp * ca.out_cstride + plane * ca.out_pstride;
   for (int i = threadIdx.x; i < N; i += NT) tw[i] = ca.tw[i];
+  // All of a thread's global loads are issued before the first LDS write, so
+  // they are in flight together instead of one round trip each.
+  constexpr int PER = (N * TC + NT - 1) / NT;
+  cx<T> v[PER];
   if (!INV) {
-    for (int e = threadIdx.x; e < N * TC; e += NT) {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = threadIdx.x + i * NT;
       const int b = e % TC, j = e / TC;
       const int col = col0 + b;
-      buf[LY::idx(b, j)] = (col < cols) ? in[(size_t)j * cols + col] : cx<T>{T(0), T(0)};
+      v[i] = (e < N * TC && col < cols) ? in[(size_t)j * cols + col] : cx<T>{T(0), T(0)};
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = threadIdx.x + i * NT;
+      if (e < N * TC) buf[LY::idx(e % TC, e / TC)] = v[i];
     }
   } else {
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = threadIdx.x + i * NT;
+      const int b = e % TC, jc = e / TC;
+      const int col = col0 + b;
+      v[i] = (e < A * TC && col < cols) ? in[(size_t)jc * cols + col] : cx<T>{T(0), T(0)};
+    }
     for (int e = threadIdx.x; e < N * TC; e += NT) {
       const int b = e % TC, j = e / TC;
       if (j >= a && j <= N - a) buf[LY::idx(b, j)] = cx<T>{T(0), T(0)};
     }
-    for (int e = threadIdx.x; e < A * TC; e += NT) {
-      const int b = e % TC, jc = e / TC;
-      const int col = col0 + b;
-      const int jf = jc < a ? jc : jc + N - A;
-      buf[LY::idx(b, jf)] = (col < cols) ? in[(size_t)jc * cols + col] : cx<T>{T(0), T(0)};
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      const int e = threadIdx.x + i * NT;
+      if (e < A * TC) {
+        const int b = e % TC, jc = e / TC;
+        const int jf = jc < a ? jc : jc + N - A;
+        buf[LY::idx(b, jf)] = v[i];
+      }
     }
   }
   __syncthreads();
