@@ -99,14 +99,28 @@ __global__ void __launch_bounds__(256) conv3_lds_kernel(Geom g, const uint16_t* 
   const int b = (int)(blk / nbx);
   const int x0 = bx * TBX, y0 = by * TBY, z0 = bz * TBZ;
   const uint16_t* Xb = X + (size_t)b * g.S * CH;
-  for (int c = threadIdx.x; c < TLVOX * 2; c += 256) {
-    const int h = c & 1, v = c >> 1;
-    const int lz = v % TLZ, r = v / TLZ;
-    const int ly = r % TLY, lx = r / TLY;
-    const int gx = pmod(x0 + lx - 1, g.Nx), gy = pmod(y0 + ly - 1, g.Ny), gz = pmod(z0 + lz - 1, g.Nz);
-    const uint4 val = *reinterpret_cast<const uint4*>(Xb + (((size_t)gx * g.Ny + gy) * g.Nz + gz) * CH + h * 8);
-    *reinterpret_cast<uint2*>(lds + (2 * h) * TPLANE + v * 8) = make_uint2(val.x, val.y);
-    *reinterpret_cast<uint2*>(lds + (2 * h + 1) * TPLANE + v * 8) = make_uint2(val.z, val.w);
+  // all of a thread's 16-byte tile loads are issued before the LDS writes
+  constexpr int SPER = (TLVOX * 2 + 255) / 256;
+  uint4 sv[SPER];
+#pragma unroll
+  for (int i = 0; i < SPER; ++i) {
+    const int c = threadIdx.x + i * 256;
+    if (c < TLVOX * 2) {
+      const int h = c & 1, v = c >> 1;
+      const int lz = v % TLZ, r = v / TLZ;
+      const int ly = r % TLY, lx = r / TLY;
+      const int gx = pmod(x0 + lx - 1, g.Nx), gy = pmod(y0 + ly - 1, g.Ny), gz = pmod(z0 + lz - 1, g.Nz);
+      sv[i] = *reinterpret_cast<const uint4*>(Xb + (((size_t)gx * g.Ny + gy) * g.Nz + gz) * CH + h * 8);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < SPER; ++i) {
+    const int c = threadIdx.x + i * 256;
+    if (c < TLVOX * 2) {
+      const int h = c & 1, v = c >> 1;
+      *reinterpret_cast<uint2*>(lds + (2 * h) * TPLANE + v * 8) = make_uint2(sv[i].x, sv[i].y);
+      *reinterpret_cast<uint2*>(lds + (2 * h + 1) * TPLANE + v * 8) = make_uint2(sv[i].z, sv[i].w);
+    }
   }
   v4s w[TAPS];
 #pragma unroll

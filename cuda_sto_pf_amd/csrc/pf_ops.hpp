@@ -141,14 +141,16 @@ struct OpPhiRhs {
       red[2] += (1.0 - hc) * ((1.0 - xc) * mA + xc * mB) + hc * P.mu0;
     }
     if constexpr (ELASTIC) {
-      double eps[6], e0[6], el[6], sig[6];
-#pragma unroll
-      for (int c = 0; c < 6; ++c) eps[c] = double(vin[c] * a.invN) + P.E0[c];
+      // Strain, stress and Q in the transform precision T (FP32 in mixed
+      // mode, as the reference's float stress and strain arrays), the
+      // derivative terms and energies in FP64.
+      double e0[6];
+      T el[6], sig[6];
       eigenstrain(P, h, xB, e0);
 #pragma unroll
-      for (int c = 0; c < 6; ++c) el[c] = eps[c] - e0[c];
+      for (int c = 0; c < 6; ++c) el[c] = T(double(vin[c] * a.invN) + P.E0[c] - e0[c]);
       // sigma = (C + h dC):(eps - eps0)  (cuda_kernels.cu:7381-7480)
-      voigt_apply_h<double>(P, h, el, sig);
+      voigt_apply_h<T>(P, T(h), el, sig);
       const double eps_c = xB * P.eps_iso;
       const double diag_iso = -eps_c + P.eps_iso * (xB - P.v_B);
       const double d0 = hp * (P.eps00[0] + diag_iso);
@@ -159,14 +161,14 @@ struct OpPhiRhs {
                                   2.0 * sig[4] * (hp * P.eps00[4]) +
                                   2.0 * sig[3] * (hp * P.eps00[3]));
       // Q = e~^T dC e~ (cuda_kernels.cu:167-185)
-      double q[6];
-      voigt_apply<double>(P.dC, el, q);
-      const double Q = el[0] * q[0] + el[1] * q[1] + el[2] * q[2] +
-                       2.0 * el[3] * q[3] + 2.0 * el[4] * q[4] + 2.0 * el[5] * q[5];
+      T q[6];
+      voigt_apply<T>(P.dC, el, q);
+      const double Q = double(el[0] * q[0] + el[1] * q[1] + el[2] * q[2] + T(2) * el[3] * q[3] +
+                              T(2) * el[4] * q[4] + T(2) * el[5] * q[5]);
       f += dgel_eigen + 0.5 * hp * Q;
       a.trs[g] = T(sig[0] + sig[1] + sig[2]);
-      red[0] += 0.5 * (sig[0] * el[0] + sig[1] * el[1] + sig[2] * el[2] +
-                       2.0 * (sig[5] * el[5] + sig[4] * el[4] + sig[3] * el[3]));
+      red[0] += 0.5 * double(sig[0] * el[0] + sig[1] * el[1] + sig[2] * el[2] +
+                             T(2) * (sig[5] * el[5] + sig[4] * el[4] + sig[3] * el[3]));
     }
     out[0] = T(phi - (P.L_phi * P.dt) * f);
   }
