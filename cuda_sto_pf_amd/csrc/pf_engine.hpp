@@ -163,19 +163,23 @@ __global__ void __launch_bounds__(NT) grad_energy_kernel(const double* phi, int 
                                                          double dx, double dy, double dz,
                                                          double kappa, double* partials) {
   double acc[1] = {0.0};
-  const long n = (long)Nx * Ny * Nz;
-  for (long g = blockIdx.x * (long)NT + threadIdx.x; g < n; g += (long)gridDim.x * NT) {
-    const int k = (int)(g % Nz);
-    const long rem = g / Nz;
-    const int j = (int)(rem % Ny);
-    const int i = (int)(rem / Ny);
-    const int ip = (i + 1) % Nx, im = (i - 1 + Nx) % Nx;
-    const int jp = (j + 1) % Ny, jm = (j - 1 + Ny) % Ny;
-    const int kp = (k + 1) % Nz, km = (k - 1 + Nz) % Nz;
-    const double gx = (phi[((long)ip * Ny + j) * Nz + k] - phi[((long)im * Ny + j) * Nz + k]) / (2.0 * dx);
-    const double gy = (phi[((long)i * Ny + jp) * Nz + k] - phi[((long)i * Ny + jm) * Nz + k]) / (2.0 * dy);
-    const double gz = (phi[((long)i * Ny + j) * Nz + kp] - phi[((long)i * Ny + j) * Nz + km]) / (2.0 * dz);
-    acc[0] += 0.5 * kappa * (gx * gx + gy * gy + gz * gz);
+  // one z row per block iteration: no per-element index division
+  for (int r = blockIdx.x; r < Nx * Ny; r += gridDim.x) {
+    const int i = r / Ny, j = r - (r / Ny) * Ny;
+    const int ip = i + 1 == Nx ? 0 : i + 1, im = i == 0 ? Nx - 1 : i - 1;
+    const int jp = j + 1 == Ny ? 0 : j + 1, jm = j == 0 ? Ny - 1 : j - 1;
+    const double* row = phi + ((long)i * Ny + j) * Nz;
+    const double* rxp = phi + ((long)ip * Ny + j) * Nz;
+    const double* rxm = phi + ((long)im * Ny + j) * Nz;
+    const double* ryp = phi + ((long)i * Ny + jp) * Nz;
+    const double* rym = phi + ((long)i * Ny + jm) * Nz;
+    for (int k = threadIdx.x; k < Nz; k += NT) {
+      const int kp = k + 1 == Nz ? 0 : k + 1, km = k == 0 ? Nz - 1 : k - 1;
+      const double gx = (rxp[k] - rxm[k]) / (2.0 * dx);
+      const double gy = (ryp[k] - rym[k]) / (2.0 * dy);
+      const double gz = (row[kp] - row[km]) / (2.0 * dz);
+      acc[0] += 0.5 * kappa * (gx * gx + gy * gy + gz * gz);
+    }
   }
   int kinds[1] = {RED_SUM};
   block_reduce_store<1>(acc, kinds, partials + blockIdx.x);
@@ -333,11 +337,12 @@ struct ForceMapArgs {
 };
 template <typename T>
 __global__ void __launch_bounds__(NT) kop_force_map(ForceMapArgs<T> fa) {
-  for (long c = (long)blockIdx.x * NT + threadIdx.x; c < fa.n; c += (long)gridDim.x * NT) {
+  const long nact_ = (fa.n / fa.d.Az) * fa.d.Azr;   // retained modes only (no kz padding)
+  for (long j_ = (long)blockIdx.x * NT + threadIdx.x; j_ < nact_; j_ += (long)gridDim.x * NT) {
+    const long c = kslot(fa.d, j_);
     double kx, ky, kz;
     int iz;
     kvec(fa.d, c, kx, ky, kz, iz);
-    if (iz >= fa.d.Azr) continue;   // kz padding: left at zero
     double M[6];
     bool ok = c != 0;
     if (ok) {

@@ -260,11 +260,12 @@ __global__ void __launch_bounds__(NT) min_project_kernel(double* phi, double lc,
 // cuda_kernels.cu:3528-3538)
 template <typename T>
 __global__ void __launch_bounds__(NT) kop_lap(KDims d, long n, const cx<T>* in, cx<T>* out) {
-  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
+  const long nact_ = (n / d.Az) * d.Azr;   // retained modes only (no kz padding)
+  for (long j_ = (long)blockIdx.x * NT + threadIdx.x; j_ < nact_; j_ += (long)gridDim.x * NT) {
+    const long i = kslot(d, j_);
     double kx, ky, kz;
     int iz;
     kvec(d, i, kx, ky, kz, iz);
-    if (iz >= d.Azr) continue;   // kz padding: left at zero
     const double k2 = kx * kx + ky * ky + kz * kz;
     out[i] = cx<T>{T(-k2 * double(in[i].re)), T(-k2 * double(in[i].im))};
   }

@@ -346,12 +346,22 @@ struct KDims {
   int Azr;            // retained kz count (Az is the row stride; iz >= Azr is padding)
 };
 
+// Compact index of the j-th retained mode (rows of Azr retained kz laid out
+// with stride Az).
+__device__ __forceinline__ long kslot(const KDims& d, long j) {
+  const unsigned ju = (unsigned)j;
+  const unsigned row = ju / (unsigned)d.Azr;
+  return (long)row * d.Az + (long)(ju - row * (unsigned)d.Azr);
+}
+
 __device__ __forceinline__ void kvec(const KDims& d, long c, double& kx, double& ky, double& kz,
                                      int& iz) {
-  iz = (int)(c % d.Az);
-  const long r = c / d.Az;
-  const int iy = (int)(r % d.Ay);
-  const int ix = (int)(r / d.Ay);
+  // 32-bit index arithmetic: compact spectra stay below 2^32 entries
+  const unsigned cu = (unsigned)c;
+  const unsigned r = cu / (unsigned)d.Az;
+  iz = (int)(cu - r * (unsigned)d.Az);
+  const unsigned ix = r / (unsigned)d.Ay;
+  const int iy = (int)(r - ix * (unsigned)d.Ay);
   kx = d.kx[ix];
   ky = d.ky[iy];
   kz = d.kz[iz];
@@ -437,11 +447,12 @@ struct GreenArgs {
 template <typename T>
 __global__ void __launch_bounds__(NT) kop_green(GreenArgs<T> ga) {
   double acc[2] = {0.0, 0.0};
-  for (long c = blockIdx.x * (long)NT + threadIdx.x; c < ga.n; c += (long)gridDim.x * NT) {
+  const long nact_ = (ga.n / ga.d.Az) * ga.d.Azr;   // retained modes only (no kz padding)
+  for (long j_ = blockIdx.x * (long)NT + threadIdx.x; j_ < nact_; j_ += (long)gridDim.x * NT) {
+    const long c = kslot(ga.d, j_);
     double kx, ky, kz;
     int iz;
     kvec(ga.d, c, kx, ky, kz, iz);
-    if (iz >= ga.d.Azr) continue;   // kz padding: left at zero
     cx<double> hh[6];
 #pragma unroll
     for (int i = 0; i < 6; ++i) hh[i] = cx<double>{double(ga.h[i][c].re), double(ga.h[i][c].im)};
@@ -515,11 +526,12 @@ struct GreenAccArgs {
 template <typename T>
 __global__ void __launch_bounds__(NT) kop_green_acc(GreenAccArgs<T> ga) {
   double acc[2] = {0.0, 0.0};
-  for (long c = blockIdx.x * (long)NT + threadIdx.x; c < ga.n; c += (long)gridDim.x * NT) {
+  const long nact_ = (ga.n / ga.d.Az) * ga.d.Azr;   // retained modes only (no kz padding)
+  for (long j_ = blockIdx.x * (long)NT + threadIdx.x; j_ < nact_; j_ += (long)gridDim.x * NT) {
+    const long c = kslot(ga.d, j_);
     double kx, ky, kz;
     int iz;
     kvec(ga.d, c, kx, ky, kz, iz);
-    if (iz >= ga.d.Azr) continue;   // kz padding: left at zero
     cx<double> hh[6];
 #pragma unroll
     for (int i = 0; i < 6; ++i) hh[i] = cx<double>{double(ga.h[i][c].re), double(ga.h[i][c].im)};
@@ -629,11 +641,12 @@ __global__ void __launch_bounds__(NT) kop_green_aa(GreenAAArgs<T> a) {
   double acc[AA_NRED];
 #pragma unroll
   for (int r = 0; r < AA_NRED; ++r) acc[r] = 0.0;
-  for (long c = blockIdx.x * (long)NT + threadIdx.x; c < a.n; c += (long)gridDim.x * NT) {
+  const long nact_ = (a.n / a.d.Az) * a.d.Azr;   // retained modes only (no kz padding)
+  for (long j_ = blockIdx.x * (long)NT + threadIdx.x; j_ < nact_; j_ += (long)gridDim.x * NT) {
+    const long c = kslot(a.d, j_);
     double kx, ky, kz;
     int iz;
     kvec(a.d, c, kx, ky, kz, iz);
-    if (iz >= a.d.Azr) continue;   // kz padding: left at zero
     cx<double> hh[6];
 #pragma unroll
     for (int i = 0; i < 6; ++i) hh[i] = cx<double>{double(a.h[i][c].re), double(a.h[i][c].im)};
@@ -722,11 +735,12 @@ struct AAUpdateArgs {
 
 template <typename T>
 __global__ void __launch_bounds__(NT) kop_aa_update(AAUpdateArgs<T> a) {
-  for (long c = blockIdx.x * (long)NT + threadIdx.x; c < a.n; c += (long)gridDim.x * NT) {
+  const long nact_ = (a.n / a.d.Az) * a.d.Azr;   // retained modes only (no kz padding)
+  for (long j_ = blockIdx.x * (long)NT + threadIdx.x; j_ < nact_; j_ += (long)gridDim.x * NT) {
+    const long c = kslot(a.d, j_);
     double kx, ky, kz;
     int iz;
     kvec(a.d, c, kx, ky, kz, iz);
-    if (iz >= a.d.Azr) continue;   // kz padding: left at zero
     cx<double> dd[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -763,11 +777,12 @@ __global__ void __launch_bounds__(NT) kop_aa_final(KDims d, long n, cx<double>* 
   cx<double>* U[3] = {U0, U1, U2};
   const cx<double>* g[3] = {g0, g1, g2};
   cx<T>* eps[6] = {e0, e1, e2, e3, e4, e5};
-  for (long c = blockIdx.x * (long)NT + threadIdx.x; c < n; c += (long)gridDim.x * NT) {
+  const long nact_ = (n / d.Az) * d.Azr;   // retained modes only (no kz padding)
+  for (long j_ = blockIdx.x * (long)NT + threadIdx.x; j_ < nact_; j_ += (long)gridDim.x * NT) {
+    const long c = kslot(d, j_);
     double kx, ky, kz;
     int iz;
     kvec(d, c, kx, ky, kz, iz);
-    if (iz >= d.Azr) continue;   // kz padding: left at zero
     cx<double> u[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -807,11 +822,12 @@ struct StrainArgs {
 };
 template <typename T, typename UT = T>
 __global__ void __launch_bounds__(NT) kop_strain(StrainArgs<T, UT> sa) {
-  for (long c = blockIdx.x * (long)NT + threadIdx.x; c < sa.n; c += (long)gridDim.x * NT) {
+  const long nact_ = (sa.n / sa.d.Az) * sa.d.Azr;   // retained modes only (no kz padding)
+  for (long j_ = blockIdx.x * (long)NT + threadIdx.x; j_ < nact_; j_ += (long)gridDim.x * NT) {
+    const long c = kslot(sa.d, j_);
     double kx, ky, kz;
     int iz;
     kvec(sa.d, c, kx, ky, kz, iz);
-    if (iz >= sa.d.Azr) continue;   // kz padding: left at zero
     cx<double> u[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) u[i] = cx<double>{double(sa.u[i][c].re), double(sa.u[i][c].im)};
@@ -822,11 +838,12 @@ __global__ void __launch_bounds__(NT) kop_strain(StrainArgs<T, UT> sa) {
 // phi^ <- g^ / (1 + L dt kappa k^2)
 template <typename T>
 __global__ void __launch_bounds__(NT) kop_phi(KDims d, long n, cx<T>* g, double Ldt_kappa) {
-  for (long c = blockIdx.x * (long)NT + threadIdx.x; c < n; c += (long)gridDim.x * NT) {
+  const long nact_ = (n / d.Az) * d.Azr;   // retained modes only (no kz padding)
+  for (long j_ = blockIdx.x * (long)NT + threadIdx.x; j_ < nact_; j_ += (long)gridDim.x * NT) {
+    const long c = kslot(d, j_);
     double kx, ky, kz;
     int iz;
     kvec(d, c, kx, ky, kz, iz);
-    if (iz >= d.Azr) continue;   // kz padding: left at zero
     const double k2 = kx * kx + ky * ky + kz * kz;
     const double den = 1.0 + Ldt_kappa * k2;
     cx<T> v = g[c];
@@ -838,11 +855,12 @@ __global__ void __launch_bounds__(NT) kop_phi(KDims d, long n, cx<T>* g, double 
 template <typename T>
 __global__ void __launch_bounds__(NT) kop_grad(KDims d, long n, const cx<T>* mu, cx<T>* gx,
                                                cx<T>* gy, cx<T>* gz) {
-  for (long c = blockIdx.x * (long)NT + threadIdx.x; c < n; c += (long)gridDim.x * NT) {
+  const long nact_ = (n / d.Az) * d.Azr;   // retained modes only (no kz padding)
+  for (long j_ = blockIdx.x * (long)NT + threadIdx.x; j_ < nact_; j_ += (long)gridDim.x * NT) {
+    const long c = kslot(d, j_);
     double kx, ky, kz;
     int iz;
     kvec(d, c, kx, ky, kz, iz);
-    if (iz >= d.Azr) continue;   // kz padding: left at zero
     const double mr = double(mu[c].re), mi = double(mu[c].im);
     gx[c] = cx<T>{T(-kx * mi), T(kx * mr)};
     gy[c] = cx<T>{T(-ky * mi), T(ky * mr)};
@@ -859,11 +877,12 @@ __global__ void __launch_bounds__(NT) kop_yupdate(KDims d, long n, const cx<T>* 
                                                   const cx<T>* yn, const cx<T>* s, cx<T>* out,
                                                   const double* dbar, double dt) {
   const double D = *dbar;
-  for (long c = blockIdx.x * (long)NT + threadIdx.x; c < n; c += (long)gridDim.x * NT) {
+  const long nact_ = (n / d.Az) * d.Azr;   // retained modes only (no kz padding)
+  for (long j_ = blockIdx.x * (long)NT + threadIdx.x; j_ < nact_; j_ += (long)gridDim.x * NT) {
+    const long c = kslot(d, j_);
     double kx, ky, kz;
     int iz;
     kvec(d, c, kx, ky, kz, iz);
-    if (iz >= d.Azr) continue;   // kz padding: left at zero
     const double k2 = kx * kx + ky * ky + kz * kz;
     // div J^ = i (kx Jx + ky Jy + kz Jz), accumulated x, y, z in order
     double dr = 0.0, di = 0.0;

@@ -249,7 +249,22 @@ __global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op:
       if (e < RT * N) point(e, pre[i]);
     }
   } else {
-    for (int e = threadIdx.x; e < RT * N; e += NT) point(e, nullptr);
+    // operands loaded a chunk of points at a time, all in flight together
+    constexpr int CHK = 12 / NPRE > 0 ? 12 / NPRE : 1;
+    for (int base = 0; base < PTS; base += CHK) {
+      double lp[CHK][NPRE];
+#pragma unroll
+      for (int i = 0; i < CHK; ++i) {
+        const int e = threadIdx.x + (base + i) * NT;
+        const int row = row0 + e / N;
+        if (base + i < PTS && e < RT * N && row < za.rows) Op::load(oa, (size_t)row * N + (e % N), lp[i]);
+      }
+#pragma unroll
+      for (int i = 0; i < CHK; ++i) {
+        const int e = threadIdx.x + (base + i) * NT;
+        if (base + i < PTS && e < RT * N) point(e, lp[i]);
+      }
+    }
   }
   if constexpr (Op::NRED > 0) {
     int kinds[Op::NRED > 0 ? Op::NRED : 1];
