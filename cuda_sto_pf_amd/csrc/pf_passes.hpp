@@ -118,7 +118,9 @@ __global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op:
   const int row0 = blockIdx.x * RT;
   const int Az = za.Az;
   const int AzS = za.AzS;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  // wave index as a wave-uniform (scalar) value, so per-wave component
+  // pointers come from scalar loads of the kernel arguments
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   for (int i = threadIdx.x; i < N; i += NT) tw[i] = za.tw[i];
   // real-space operands of the operator, issued before the inverse transform
   // so their latency overlaps it
@@ -151,15 +153,20 @@ __global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op:
     constexpr bool BATCHED = CPW0 * KI * 2 <= 24;
     constexpr int CPW = BATCHED ? CPW0 : 0;
     cx<T> va[CPW > 0 ? CPW : 1][KI], vb[CPW > 0 ? CPW : 1][KI];
+    const cx<T>* pin[CPW > 0 ? CPW : 1];
+#pragma unroll
+    for (int q = 0; q < CPW; ++q) {
+      const int cp = wave + q * (NT / 64);
+      pin[q] = za.in[cp < NCI * NP ? cp / NP : 0];
+    }
 #pragma unroll
     for (int q = 0; q < CPW; ++q) {
       const int cp = wave + q * (NT / 64);
       const int p = cp % NP;
-      const int c = cp < NCI * NP ? cp / NP : 0;
       const int ra = row0 + 2 * p, rb = ra + 1;
       const bool live = cp < NCI * NP;
-      const cx<T>* ia = za.in[c] + (size_t)ra * AzS;
-      const cx<T>* ib = za.in[c] + (size_t)rb * AzS;
+      const cx<T>* ia = pin[q] + (size_t)ra * AzS;
+      const cx<T>* ib = pin[q] + (size_t)rb * AzS;
 #pragma unroll
       for (int i = 0; i < KI; ++i) {
         const int k = lane + 64 * i;
