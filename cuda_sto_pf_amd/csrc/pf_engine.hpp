@@ -382,10 +382,10 @@ __global__ void __launch_bounds__(NT) kop_diff3(long n, const S* u0, const S* u1
 }
 
 // ------------------------------------------------------------ dispatch
-template <typename T, int N, int NCI, int NCO>
+template <typename T, int N, int NCI, int NCO, int NPRE>
 constexpr int zpass_rows_for() {
   constexpr int NCM = NCI > NCO ? NCI : NCO;
-  return zpass_rows<T, N, (NCM > 0 ? NCM : 1)>();
+  return zpass_rows<T, N, (NCM > 0 ? NCM : 1), NPRE>();
 }
 
 // z-pass implementation: the LDS-staged kernel (default), or the
@@ -399,14 +399,14 @@ inline bool zpass_use_reg() {
   return use;
 }
 
-template <typename T, int NCI, int NCO>
+template <typename T, int NCI, int NCO, int NPRE>
 int zpass_grid(int N, int rows) {
   switch (N) {
 #define PF_CASE_ZG(v)                                                                  \
   case v:                                                                              \
     if constexpr (RegPlan<v>::ok)                                                      \
       if (zpass_use_reg()) return (rows + 64 / RegPlan<v>::L - 1) / (64 / RegPlan<v>::L); \
-    return (rows + zpass_rows_for<T, v, NCI, NCO>() - 1) / zpass_rows_for<T, v, NCI, NCO>();
+    return (rows + zpass_rows_for<T, v, NCI, NCO, NPRE>() - 1) / zpass_rows_for<T, v, NCI, NCO, NPRE>();
     PF_SIZES(PF_CASE_ZG)
 #undef PF_CASE_ZG
     default:
@@ -414,9 +414,11 @@ int zpass_grid(int N, int rows) {
   }
 }
 
+// Returns the grid size (the number of per-block partials the operator's
+// reductions wrote).
 template <typename T, int NCI, int NCO, class Op>
-void launch_zpass(int N, hipStream_t s, const ZPassArgs<T>& za, const typename Op::Args& oa) {
-  const int grid = zpass_grid<T, NCI, NCO>(N, za.rows);
+int launch_zpass(int N, hipStream_t s, const ZPassArgs<T>& za, const typename Op::Args& oa) {
+  const int grid = zpass_grid<T, NCI, NCO, Op::NPRE>(N, za.rows);
   switch (N) {
 #define PF_CASE_Z(v)                                                                                \
   case v:                                                                                           \
@@ -433,6 +435,7 @@ void launch_zpass(int N, hipStream_t s, const ZPassArgs<T>& za, const typename O
     default:
       break;
   }
+  return grid;
 }
 
 template <typename T, bool INV>
@@ -617,12 +620,10 @@ struct Engine final : EngineBase {
       }
     }
     // partial-sum scratch: largest grid times 4 reductions
+    // (a z pass takes at least one row pair per workgroup)
     long maxg = 2048;
     const int rows = Nx * Ny;
-    maxg = std::max<long>(maxg, zpass_grid<T, 1, 1>(Nz, rows));
-    maxg = std::max<long>(maxg, zpass_grid<T, 6, 1>(Nz, rows));
-    maxg = std::max<long>(maxg, zpass_grid<T, 1, 0>(Nz, rows));
-    maxg = std::max<long>(maxg, zpass_grid<T, 0, 1>(Nz, rows));
+    maxg = std::max<long>(maxg, (rows + 1) / 2);
     npartials = (size_t)maxg * 16;   // up to 16 reductions per block (Anderson dots use AA_NRED)
     if ((rc = dalloc(&partials, npartials)) || (rc = dalloc(&dscal, 64))) return rc;
     PF_HIP(hipHostMalloc((void**)&hscal, 64 * sizeof(double), hipHostMallocDefault));
@@ -1170,13 +1171,11 @@ struct Engine final : EngineBase {
     if (P.elastic) {
       if ((rc = elastic_solve())) return rc;
       typename OpPhiRhs<T, true>::Args oa{P, phi_old, xB, trs, invN};
-      launch_zpass<T, 6, 1, OpPhiRhs<T, true>>(Nz, stream, zargs(0, 6, 0, 1), oa);
-      const int g = zpass_grid<T, 6, 1>(Nz, rows);
+      const int g = launch_zpass<T, 6, 1, OpPhiRhs<T, true>>(Nz, stream, zargs(0, 6, 0, 1), oa);
       finalize(g, 3, RedKinds{{RED_SUM, RED_SUM, RED_SUM, 0}}, dscal + 9, 1.0 / (double)R);
     } else {
       typename OpPhiRhs<T, false>::Args oa{P, phi_old, xB, trs, invN};
-      launch_zpass<T, 0, 1, OpPhiRhs<T, false>>(Nz, stream, zargs(0, 0, 0, 1), oa);
-      const int g = zpass_grid<T, 0, 1>(Nz, rows);
+      const int g = launch_zpass<T, 0, 1, OpPhiRhs<T, false>>(Nz, stream, zargs(0, 0, 0, 1), oa);
       finalize(g, 3, RedKinds{{RED_SUM, RED_SUM, RED_SUM, 0}}, dscal + 9, 1.0 / (double)R);
     }
     const KDims d = kd();
@@ -1190,8 +1189,7 @@ struct Engine final : EngineBase {
     {
       const int phiY_old = (zm || prm.enable_Y_rhs_previous_time_level) ? 1 : 0;
       typename OpPhiFinish<T>::Args oa{P, phi_new, phi_old, Y, xB, trs, invN, phiY_old};
-      launch_zpass<T, 1, 1, OpPhiFinish<T>>(Nz, stream, zargs(0, 1, 1, 1), oa);
-      const int g = zpass_grid<T, 1, 1>(Nz, rows);
+      const int g = launch_zpass<T, 1, 1, OpPhiFinish<T>>(Nz, stream, zargs(0, 1, 1, 1), oa);
       finalize(g, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 4, 1.0 / (double)R);
     }
     // Y: spectral gradient of mu, fluxes, update
@@ -1215,8 +1213,7 @@ struct Engine final : EngineBase {
     ypass(true, 5, 1);
     if (zm) {
       typename OpYStar<T>::Args oa{Ystar, invN};
-      launch_zpass<T, 1, 0, OpYStar<T>>(Nz, stream, zargs(5, 1, 0, 0), oa);
-      const int g = zpass_grid<T, 1, 0>(Nz, rows);
+      const int g = launch_zpass<T, 1, 0, OpYStar<T>>(Nz, stream, zargs(5, 1, 0, 0), oa);
       finalize(g, 2, RedKinds{{RED_MIN, RED_MAX, 0, 0}}, dscal, 1.0);
       if ((rc = fetch(13))) return rc;
       mean_DY = hscal[4];
@@ -1485,8 +1482,8 @@ struct Engine final : EngineBase {
   void min_gfull(const double* ph, int o) {
     const T invN = T(1.0 / (double)R);
     typename OpMinGfull<T>::Args oa{ph, m_gexp, m_gfull, prm.kappa_phi, invN};
-    launch_zpass<T, 1, 0, OpMinGfull<T>>(Nz, stream, zargs(1, 1, 0, 0), oa);
-    finalize(zpass_grid<T, 1, 0>(Nz, Nx * Ny), 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal + o, 1.0);
+    const int gz = launch_zpass<T, 1, 0, OpMinGfull<T>>(Nz, stream, zargs(1, 1, 0, 0), oa);
+    finalize(gz, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal + o, 1.0);
   }
   // semi-implicit phi update from the field phi - L dt (g_exp + lambda h') of ph
   void min_update(const double* ph, double lambda) {
@@ -1521,12 +1518,12 @@ struct Engine final : EngineBase {
     if (el) {
       if ((rc = elastic_solve())) return rc;
       typename OpMinRhs<T, true>::Args oa{P, phi_old, m_gexp, m_eps, m_s00, (size_t)R, invN};
-      launch_zpass<T, 6, 1, OpMinRhs<T, true>>(Nz, stream, zargs(0, 6, 0, 1), oa);
-      finalize(zpass_grid<T, 6, 1>(Nz, rows), 3, RedKinds{{RED_SUM, RED_SUM, RED_SUM, 0}}, dscal + 9, 1.0);
+      const int gz = launch_zpass<T, 6, 1, OpMinRhs<T, true>>(Nz, stream, zargs(0, 6, 0, 1), oa);
+      finalize(gz, 3, RedKinds{{RED_SUM, RED_SUM, RED_SUM, 0}}, dscal + 9, 1.0);
     } else {
       typename OpMinRhs<T, false>::Args oa{P, phi_old, m_gexp, m_eps, m_s00, (size_t)R, invN};
-      launch_zpass<T, 0, 1, OpMinRhs<T, false>>(Nz, stream, zargs(0, 0, 0, 1), oa);
-      finalize(zpass_grid<T, 0, 1>(Nz, rows), 3, RedKinds{{RED_SUM, RED_SUM, RED_SUM, 0}}, dscal + 9, 1.0);
+      const int gz = launch_zpass<T, 0, 1, OpMinRhs<T, false>>(Nz, stream, zargs(0, 0, 0, 1), oa);
+      finalize(gz, 3, RedKinds{{RED_SUM, RED_SUM, RED_SUM, 0}}, dscal + 9, 1.0);
     }
     // g_full(phi^n) and the under-relaxed multiplier (main_cuda.cu:29436-29500)
     min_forward_to_sc0();
@@ -1542,14 +1539,15 @@ struct Engine final : EngineBase {
     m_lambda = (1.0 - eta) * m_lambda + eta * lam_raw;
     // phi^{n+1}
     min_update(phi_old, m_lambda);
+    int gz;
     if (el) {
       typename OpMinFinish<T, true, true>::Args oa{P, phi_old, phi_new, m_gexp, m_eps, m_s00, (size_t)R, invN};
-      launch_zpass<T, 1, 1, OpMinFinish<T, true, true>>(Nz, stream, zargs(0, 1, 0, 1), oa);
+      gz = launch_zpass<T, 1, 1, OpMinFinish<T, true, true>>(Nz, stream, zargs(0, 1, 0, 1), oa);
     } else {
       typename OpMinFinish<T, false, true>::Args oa{P, phi_old, phi_new, m_gexp, m_eps, m_s00, (size_t)R, invN};
-      launch_zpass<T, 1, 1, OpMinFinish<T, false, true>>(Nz, stream, zargs(0, 1, 0, 1), oa);
+      gz = launch_zpass<T, 1, 1, OpMinFinish<T, false, true>>(Nz, stream, zargs(0, 1, 0, 1), oa);
     }
-    finalize(zpass_grid<T, 1, 1>(Nz, rows), 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 52, 1.0);
+    finalize(gz, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 52, 1.0);
     // KKT residual at phi^{n+1} with the exact multiplier (main_cuda.cu:29596-29728)
     min_forward_to_sc0();
     min_lap_to_sz1();

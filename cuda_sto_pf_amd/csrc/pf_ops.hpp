@@ -87,13 +87,16 @@ struct OpHij {
 // g = phi - L dt rhs (phi^ and rhs^ enter the update linearly,
 // phi_semi_implicit_update_kernel cuda_kernels.cu:821-843).
 __device__ __forceinline__ double phi_rhs_chem(const PhysParams& P, double phi, double xB,
-                                               double& hp_out, double& h_out) {
+                                               double& hp_out, double& h_out, double& muA_out,
+                                               double& muB_out) {
   const double hp = h_prime_of_phi(phi);
   const double gp = g_prime_of_phi(phi);
   const double h = h_of_phi(phi);
   double f = P.W * gp;
   const double muA = mu_A(P, xB);
   const double muB = mu_B(P, xB);
+  muA_out = muA;
+  muB_out = muB;
   const double delta_mu = P.mu0 - P.v_A * muA - P.v_B * muB - (P.elastic ? P.shift : 0.0);
   const double c_bulk = c_xB_phi(P, xB, h);
   const double mu_total = mu_tot_mix(muA, muB, xB, P.mu0, h);
@@ -131,13 +134,19 @@ struct OpPhiRhs {
     const double phi = pre[0];
     const double xraw = pre[1];
     const double xB = clamp01(xraw);
-    double hp, h;
-    double f = phi_rhs_chem(P, phi, xB, hp, h);
+    double hp, h, mA, mB;
+    double f = phi_rhs_chem(P, phi, xB, hp, h, mA, mB);
     red[1] += P.W * g_of_phi(phi);
     {
       const double hc = h_of_phi(clamp01(phi));
       const double xc = clamp_fraction_eps(xraw);
-      const double mA = mu_A(P, xc), mB = mu_B(P, xc);
+      // mu(clamp01(x)) equals mu(clamp_eps(x)) (the calphad branch clamps
+      // to [1e-12, 1 - 1e-12] itself) except on the convex continuation
+      // for x above 1 - 1e-12
+      if (P.convex && xraw > 1.0 - 1e-12) {
+        mA = mu_A(P, xc);
+        mB = mu_B(P, xc);
+      }
       red[2] += (1.0 - hc) * ((1.0 - xc) * mA + xc * mB) + hc * P.mu0;
     }
     if constexpr (ELASTIC) {

@@ -27,11 +27,22 @@ constexpr int LDS_BUDGET = 48 * 1024;   // bytes of FFT data per workgroup
 
 constexpr int ZPASS_LDS_BUDGET = 32 * 1024;   // smaller tiles: more resident workgroups
 
-template <typename T, int N, int NC>
+// Register budget (doubles per thread) for the operator's real-space operands
+// loaded ahead of the inverse transform.
+constexpr int ZPASS_PREFETCH_DOUBLES = 48;
+
+// Rows per workgroup: as many row pairs as the LDS budget holds; an operator
+// with NPRE real-space operands per point whose operands would not fit the
+// prefetch budget at that tile gets a smaller tile where they fit in half of
+// it (so their loads overlap the inverse transform instead of being issued
+// a chunk at a time after it).
+template <typename T, int N, int NC, int NPRE = 0>
 constexpr int zpass_rows() {
   constexpr int per_pair = NC * (N + N / 16) * (int)sizeof(cx<T>) * (plan_has_big_prime(N) ? 2 : 1);
   constexpr int pairs = ZPASS_LDS_BUDGET / per_pair;
-  constexpr int p = pairs < 1 ? 1 : (pairs > 32 ? 32 : pairs);
+  int p = pairs < 1 ? 1 : (pairs > 32 ? 32 : pairs);
+  if (NPRE > 0 && ((2 * p * N + NT - 1) / NT) * NPRE > ZPASS_PREFETCH_DOUBLES)
+    while (p > 1 && ((2 * p * N + NT - 1) / NT) * NPRE > ZPASS_PREFETCH_DOUBLES / 2) --p;
   return 2 * p;
 }
 
@@ -107,7 +118,7 @@ template <typename T, int N, int NCI, int NCO, class Op>
 __global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op::Args oa) {
   constexpr int NCM = NCI > NCO ? NCI : NCO;
   constexpr int NCB = NCM > 0 ? NCM : 1;
-  constexpr int RT = zpass_rows<T, N, NCB>();
+  constexpr int RT = zpass_rows<T, N, NCB, Op::NPRE>();
   constexpr int NP = RT / 2;
   constexpr bool BIG = plan_has_big_prime(N);
   using LY = Lds<N, NCB * NP, false>;   // row-major: layout independent of the batch
@@ -128,7 +139,7 @@ __global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op:
   // register budget; otherwise each point loads its operands just before use)
   constexpr int PTS = (RT * N + NT - 1) / NT;
   constexpr int NPRE = Op::NPRE > 0 ? Op::NPRE : 1;
-  constexpr bool PREFETCH = PTS * NPRE <= 48;
+  constexpr bool PREFETCH = PTS * NPRE <= ZPASS_PREFETCH_DOUBLES;
   double pre[PREFETCH ? PTS : 1][NPRE];
   if constexpr (PREFETCH) {
 #pragma unroll

@@ -8,6 +8,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include "pf_log.hpp"
 
 namespace pf {
 
@@ -53,14 +54,15 @@ PF_HD double clamp01(double v) { return v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v); }
 PF_HD double clamp_eps(double v, double eps) { return v < eps ? eps : (v > 1.0 - eps ? 1.0 - eps : v); }
 PF_HD double clamp_fraction_eps(double x) { return clamp_eps(x, 1e-12); }
 
-// thermo_utils.h:364-377 (CALPHAD regular solution, clamped x)
+// thermo_utils.h:364-377 (CALPHAD regular solution, clamped x; the
+// logarithm is pf_log, within one ulp of the correctly rounded value)
 PF_HD double mu_PbTe_calphad(const PhysParams& P, double xB) {
   double x = clamp_fraction_eps(xB);
-  return P.G_A0 + P.RT * log(1.0 - x) + P.Lparam * x * x;
+  return P.G_A0 + P.RT * pf_log(1.0 - x) + P.Lparam * x * x;
 }
 PF_HD double mu_Ag2Te_calphad(const PhysParams& P, double xB) {
   double x = clamp_fraction_eps(xB);
-  return P.G_B0 + P.RT * log(x) + P.Lparam * (1.0 - x) * (1.0 - x);
+  return P.G_B0 + P.RT * pf_log(x) + P.Lparam * (1.0 - x) * (1.0 - x);
 }
 // thermo_utils.h:400-430 (optional convex continuation above x = 0.09)
 PF_HD double mu_PbTe_raw(const PhysParams& P, double xB) {
