@@ -111,12 +111,34 @@ struct RedKinds {
 namespace {
 
 // Sums (or min/max) of block partials in a fixed order; one workgroup.
+// Each thread combines blocks b = tid, tid + NT, ... in that order; the loads
+// of FIN_U consecutive terms are issued together (one workgroup reads up to
+// ~10^5 partials, so load latency, not bandwidth, sets its duration).
+constexpr int FIN_U = 8;
+
 __global__ void __launch_bounds__(NT) finalize_kernel(const double* partials, int nblocks, int nred,
                                                       RedKinds kinds, double* out, double scale) {
   double v[4];
+#pragma unroll
   for (int r = 0; r < 4; ++r) v[r] = red_identity(kinds.k[r]);
-  for (int b = threadIdx.x; b < nblocks; b += NT)
-    for (int r = 0; r < nred; ++r) v[r] = red_combine(kinds.k[r], v[r], partials[(size_t)b * nred + r]);
+  int b = threadIdx.x;
+  for (; b + (FIN_U - 1) * NT < nblocks; b += FIN_U * NT) {
+    double t[FIN_U][4];
+#pragma unroll
+    for (int u = 0; u < FIN_U; ++u)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (r < nred) t[u][r] = partials[(size_t)(b + u * NT) * nred + r];
+#pragma unroll
+    for (int u = 0; u < FIN_U; ++u)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (r < nred) v[r] = red_combine(kinds.k[r], v[r], t[u][r]);
+  }
+  for (; b < nblocks; b += NT)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      if (r < nred) v[r] = red_combine(kinds.k[r], v[r], partials[(size_t)b * nred + r]);
   __shared__ double outv[4];
   block_reduce_store<4>(v, kinds.k, outv);
   __syncthreads();
@@ -128,9 +150,26 @@ __global__ void __launch_bounds__(NT) finalize_kernel(const double* partials, in
 __global__ void __launch_bounds__(NT) finalize_sums_kernel(const double* partials, int nblocks, int nred,
                                                            double* out) {
   double v[16];
+#pragma unroll
   for (int r = 0; r < 16; ++r) v[r] = 0.0;
-  for (int b = threadIdx.x; b < nblocks; b += NT)
-    for (int r = 0; r < nred; ++r) v[r] += partials[(size_t)b * nred + r];
+  int b = threadIdx.x;
+  for (; b + 3 * NT < nblocks; b += 4 * NT) {
+    double t[4][16];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int r = 0; r < 16; ++r)
+        if (r < nred) t[u][r] = partials[(size_t)(b + u * NT) * nred + r];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int r = 0; r < 16; ++r)
+        if (r < nred) v[r] += t[u][r];
+  }
+  for (; b < nblocks; b += NT)
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+      if (r < nred) v[r] += partials[(size_t)b * nred + r];
   __shared__ double outv[16];
   int kinds[16];
   for (int r = 0; r < 16; ++r) kinds[r] = RED_SUM;
@@ -188,21 +227,143 @@ __global__ void __launch_bounds__(NT) grad_energy_kernel(const double* phi, int 
 // Zero-mode evaluation (compute_pf_Y_zero_mode_terms_kernel, cuda_kernels.cu:3409-3420):
 // mass(lambda) = sum (1-h) sigma(Y*+lambda) + h v_B, its derivative, and the
 // count of shifted values outside [Y_lo, Y_hi] (validate_pf_Y_zero_mode_bounds_kernel 3422-3432).
-__global__ void __launch_bounds__(NT) zm_eval_kernel(const double* Ystar, const double* phi,
-                                                     double lam, double vB, double ylo, double yhi,
-                                                     long n, double* partials) {
-  double acc[3] = {0.0, 0.0, 0.0};
+//
+// The zero-mode root find (run_pf_conserved_y_zero_mode_host, main_cuda.cu:4269-4405)
+// with its decisions taken on the device, so the host waits once per step
+// instead of once per evaluation.  The three evaluations that open the search
+// (the bracket ends lo, hi and the start clamp(0, lo, hi)) share one pass over
+// the field; each later Newton/bisection step is an evaluation, its partial
+// sums and a one-thread controller that applies the host loop's rules in the
+// same FP64 operations.  Steps enqueued beyond convergence see ZS_ACTIVE == 0
+// and return at once.
+enum ZmSlot {
+  ZS_LO = 0, ZS_HI, ZS_LAM, ZS_R, ZS_DERIV, ZS_INVALID, ZS_ACTIVE, ZS_STATUS, ZS_ITERS, ZS_NEXT,
+  ZS_SUMS = 16,   // 9 sums of the opening pass, then 3 of each step
+  ZS_COUNT = 32
+};
+enum ZmStatus { ZM_OK = 0, ZM_EMPTY_BRACKET = 1, ZM_NONFINITE = 2, ZM_OUTSIDE = 3 };
+struct ZmConst {
+  double Y_clip, vB, target, tol;
+  int max_it;
+};
+
+__device__ __forceinline__ void zm_terms(double ys, double ph, double lam, double vB, double ylo, double yhi,
+                                         double* acc) {
+  const double h = h_of_phi(ph);
+  const double alpha = 1.0 - h;
+  const double y = ys + lam;
+  const double x = sigmoid_unclipped(y);
+  acc[0] += alpha * x + h * vB;
+  acc[1] += alpha * x * (1.0 - x);
+  if (!isfinite(y) || y < ylo || y > yhi) acc[2] += 1.0;
+}
+
+// opening pass: lambda = lo, hi, clamp(0, lo, hi) from the Y* extremes ymm[0..1]
+__global__ void __launch_bounds__(NT) zm_eval3_kernel(const double* Ystar, const double* phi, const double* ymm,
+                                                      double Y_clip, double vB, long n, double* partials) {
+  const double lo = -Y_clip - ymm[0];
+  const double hi = Y_clip - ymm[1];
+  const double lam0 = fmin(fmax(0.0, lo), hi);
+  double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   for (long g = blockIdx.x * (long)NT + threadIdx.x; g < n; g += (long)gridDim.x * NT) {
-    const double h = h_of_phi(phi[g]);
-    const double alpha = 1.0 - h;
-    const double y = Ystar[g] + lam;
-    const double x = sigmoid_unclipped(y);
-    acc[0] += alpha * x + h * vB;
-    acc[1] += alpha * x * (1.0 - x);
-    if (!isfinite(y) || y < ylo || y > yhi) acc[2] += 1.0;
+    const double ys = Ystar[g], ph = phi[g];
+    zm_terms(ys, ph, lo, vB, -Y_clip, Y_clip, acc);
+    zm_terms(ys, ph, hi, vB, -Y_clip, Y_clip, acc + 3);
+    zm_terms(ys, ph, lam0, vB, -Y_clip, Y_clip, acc + 6);
   }
+  int kinds[9];
+#pragma unroll
+  for (int r = 0; r < 9; ++r) kinds[r] = RED_SUM;
+  block_reduce_store<9>(acc, kinds, partials + (size_t)blockIdx.x * 9);
+}
+
+// one search step at st[ZS_NEXT]
+__global__ void __launch_bounds__(NT) zm_eval_kernel(const double* Ystar, const double* phi, const double* st,
+                                                     double Y_clip, double vB, long n, double* partials) {
+  if (st[ZS_ACTIVE] == 0.0) return;
+  const double lam = st[ZS_NEXT];
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (long g = blockIdx.x * (long)NT + threadIdx.x; g < n; g += (long)gridDim.x * NT)
+    zm_terms(Ystar[g], phi[g], lam, vB, -Y_clip, Y_clip, acc);
   int kinds[3] = {RED_SUM, RED_SUM, RED_SUM};
   block_reduce_store<3>(acc, kinds, partials + (size_t)blockIdx.x * 3);
+}
+
+// the loop condition and the next candidate (main_cuda.cu:4370-4390)
+__device__ void zm_propose(double* st, const ZmConst& c) {
+  const double r = st[ZS_R];
+  if (!(st[ZS_ITERS] < (double)c.max_it && fabs(r) > c.tol)) {
+    st[ZS_ACTIVE] = 0.0;
+    return;
+  }
+  const double lam = st[ZS_LAM];
+  double lo = st[ZS_LO], hi = st[ZS_HI];
+  if (r < 0.0)
+    lo = lam;
+  else
+    hi = lam;
+  const double deriv = st[ZS_DERIV];
+  double cand = NAN;
+  if (deriv > 0.0 && isfinite(deriv)) cand = lam - r / deriv;
+  if (!(isfinite(cand) && cand > lo && cand < hi)) cand = 0.5 * (lo + hi);
+  st[ZS_LO] = lo;
+  st[ZS_HI] = hi;
+  st[ZS_NEXT] = cand;
+  st[ZS_ACTIVE] = 1.0;
+}
+
+__global__ void zm_ctrl_open_kernel(double* st, const double* ymm, ZmConst c) {
+  if (threadIdx.x != 0) return;
+  const double* s = st + ZS_SUMS;
+  const double lo = -c.Y_clip - ymm[0];
+  const double hi = c.Y_clip - ymm[1];
+  st[ZS_ACTIVE] = 0.0;
+  st[ZS_ITERS] = 0.0;
+  st[ZS_LAM] = NAN;
+  st[ZS_R] = NAN;
+  st[ZS_INVALID] = 0.0;
+  const double roundoff = 128.0 * 2.220446049250313e-16 * fmax(fmax(fabs(lo), fabs(hi)), 1.0);
+  if (!isfinite(lo) || !isfinite(hi) || lo > hi + roundoff) {
+    st[ZS_STATUS] = ZM_EMPTY_BRACKET;
+    return;
+  }
+  if (!isfinite(s[0]) || !isfinite(s[1]) || !isfinite(s[3]) || !isfinite(s[4])) {
+    st[ZS_STATUS] = ZM_NONFINITE;
+    return;
+  }
+  const double F_lo = s[0] - c.target, F_hi = s[3] - c.target;
+  if (F_lo > c.tol || F_hi < -c.tol) {
+    st[ZS_STATUS] = ZM_OUTSIDE;
+    return;
+  }
+  if (!isfinite(s[6]) || !isfinite(s[7])) {
+    st[ZS_STATUS] = ZM_NONFINITE;
+    return;
+  }
+  st[ZS_STATUS] = ZM_OK;
+  st[ZS_LO] = lo;
+  st[ZS_HI] = hi;
+  st[ZS_LAM] = fmin(fmax(0.0, lo), hi);
+  st[ZS_R] = s[6] - c.target;
+  st[ZS_DERIV] = s[7];
+  st[ZS_INVALID] = s[8];
+  zm_propose(st, c);
+}
+
+__global__ void zm_ctrl_step_kernel(double* st, ZmConst c) {
+  if (threadIdx.x != 0 || st[ZS_ACTIVE] == 0.0) return;
+  const double* s = st + ZS_SUMS;
+  st[ZS_LAM] = st[ZS_NEXT];
+  if (!isfinite(s[0]) || !isfinite(s[1])) {
+    st[ZS_STATUS] = ZM_NONFINITE;
+    st[ZS_ACTIVE] = 0.0;
+    return;
+  }
+  st[ZS_R] = s[0] - c.target;
+  st[ZS_DERIV] = s[1];
+  st[ZS_INVALID] = s[2];
+  st[ZS_ITERS] += 1.0;
+  zm_propose(st, c);
 }
 
 // Y = Y* + lambda, x_B = sigma(Y) unclipped (apply_pf_Y_zero_mode_shift_kernel
@@ -625,8 +786,8 @@ struct Engine final : EngineBase {
     const int rows = Nx * Ny;
     maxg = std::max<long>(maxg, (rows + 1) / 2);
     npartials = (size_t)maxg * 16;   // up to 16 reductions per block (Anderson dots use AA_NRED)
-    if ((rc = dalloc(&partials, npartials)) || (rc = dalloc(&dscal, 64))) return rc;
-    PF_HIP(hipHostMalloc((void**)&hscal, 64 * sizeof(double), hipHostMallocDefault));
+    if ((rc = dalloc(&partials, npartials)) || (rc = dalloc(&dscal, 128))) return rc;
+    PF_HIP(hipHostMalloc((void**)&hscal, 128 * sizeof(double), hipHostMallocDefault));
     return PF_OK;
   }
 
@@ -1215,16 +1376,14 @@ struct Engine final : EngineBase {
       typename OpYStar<T>::Args oa{Ystar, invN};
       const int g = launch_zpass<T, 1, 0, OpYStar<T>>(Nz, stream, zargs(5, 1, 0, 0), oa);
       finalize(g, 2, RedKinds{{RED_MIN, RED_MAX, 0, 0}}, dscal, 1.0);
-      if ((rc = fetch(13))) return rc;
+      // the step's scalars come back with the zero-mode search state
+      rc = zero_mode(phi_new, 13);
       mean_DY = hscal[4];
       e_grad = hscal[8];
       e_el = hscal[9];
       e_dw = hscal[10];
       e_chem = hscal[11];
-      if ((rc = zero_mode(hscal[0], hscal[1], phi_new))) {
-        // restore the time level n state as the reference does before exiting
-        return rc;
-      }
+      if (rc) return rc;
     } else {
       typename OpYFinal<T>::Args oa{P, Y, xB, dYdt, invN, P.Y_clip};
       launch_zpass<T, 1, 0, OpYFinal<T>>(Nz, stream, zargs(5, 1, 0, 0), oa);
@@ -1240,55 +1399,48 @@ struct Engine final : EngineBase {
     return PF_OK;
   }
 
-  // run_pf_conserved_y_zero_mode_host (main_cuda.cu:4269-4405)
-  int zero_mode(double ymin, double ymax, const double* phi_np1) {
-    const double Ycap = P.Y_clip;
-    double lo = -P.Y_clip - ymin;
-    double hi = Ycap - ymax;
-    const double roundoff = 128.0 * 2.220446049250313e-16 * std::fmax(std::fmax(std::fabs(lo), std::fabs(hi)), 1.0);
-    if (!std::isfinite(lo) || !std::isfinite(hi) || lo > hi + roundoff)
-      return fail(PF_E_ZERO_MODE, "PF zero-mode rejected: empty bracket");
-    const double target = target_mass;
-    const double tol = prm.zero_mode_tolerance_relative * std::fmax(std::fabs(target), 1.0);
+  // run_pf_conserved_y_zero_mode_host (main_cuda.cu:4269-4405), decided on the
+  // device (zm_ctrl_*): the Y* extremes are in dscal[0..1]; `nfetch` more
+  // leading scalars come back with the search state in the one wait.
+  static constexpr int ZM_SLOT = 64;   // dscal offset of the search state
+  int zm_spec = 3;                     // search steps enqueued before the first wait
+  int zero_mode(const double* phi_np1, int nfetch) {
+    double* st = dscal + ZM_SLOT;
+    const ZmConst c{P.Y_clip, P.v_B, target_mass,
+                    prm.zero_mode_tolerance_relative * std::fmax(std::fabs(target_mass), 1.0),
+                    prm.zero_mode_max_iterations};
     const int g = grid_for(R);
-    double mass = 0, deriv = 0, invalid = 0;
-    auto eval = [&](double lam) -> int {
-      hipLaunchKernelGGL(zm_eval_kernel, dim3(g), dim3(NT), 0, stream, Ystar, phi_np1, lam, P.v_B, -P.Y_clip,
-                         Ycap, R, partials);
-      finalize(g, 3, RedKinds{{RED_SUM, RED_SUM, RED_SUM, 0}}, dscal + 16, 1.0);
-      PF_HIP(hipMemcpyAsync(hscal + 16, dscal + 16, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    hipLaunchKernelGGL(zm_eval3_kernel, dim3(g), dim3(NT), 0, stream, Ystar, phi_np1, (const double*)dscal,
+                       P.Y_clip, P.v_B, R, partials);
+    hipLaunchKernelGGL(finalize_sums_kernel, dim3(1), dim3(NT), 0, stream, partials, g, 9, st + ZS_SUMS);
+    hipLaunchKernelGGL(zm_ctrl_open_kernel, dim3(1), dim3(64), 0, stream, st, (const double*)dscal, c);
+    int left = std::max(prm.zero_mode_max_iterations, 0);
+    int batch = std::min(zm_spec, left);
+    for (;;) {
+      for (int k = 0; k < batch; ++k) {
+        hipLaunchKernelGGL(zm_eval_kernel, dim3(g), dim3(NT), 0, stream, Ystar, phi_np1, (const double*)st,
+                           P.Y_clip, P.v_B, R, partials);
+        hipLaunchKernelGGL(finalize_sums_kernel, dim3(1), dim3(NT), 0, stream, partials, g, 3, st + ZS_SUMS);
+        hipLaunchKernelGGL(zm_ctrl_step_kernel, dim3(1), dim3(64), 0, stream, st, c);
+      }
+      left -= batch;
+      PF_HIP(hipMemcpyAsync(hscal, dscal, nfetch * sizeof(double), hipMemcpyDeviceToHost, stream));
+      PF_HIP(hipMemcpyAsync(hscal + ZM_SLOT, st, ZS_SUMS * sizeof(double), hipMemcpyDeviceToHost, stream));
       PF_HIP(hipStreamSynchronize(stream));
-      mass = hscal[16];
-      deriv = hscal[17];
-      invalid = hscal[18];
-      if (!std::isfinite(mass) || !std::isfinite(deriv)) return fail(PF_E_ZERO_MODE, "zero-mode evaluation non-finite");
-      return PF_OK;
-    };
-    int rc;
-    if ((rc = eval(lo))) return rc;
-    const double F_lo = mass - target;
-    if ((rc = eval(hi))) return rc;
-    const double F_hi = mass - target;
-    if (F_lo > tol || F_hi < -tol) return fail(PF_E_ZERO_MODE, "PF zero-mode rejected: target outside bracket");
-    double lam = std::fmin(std::fmax(0.0, lo), hi);
-    if ((rc = eval(lam))) return rc;
-    double r = mass - target;
-    int iters = 0;
-    for (int iteration = 0; iteration < prm.zero_mode_max_iterations && std::fabs(r) > tol; ++iteration) {
-      if (r < 0.0)
-        lo = lam;
-      else
-        hi = lam;
-      double cand = NAN;
-      if (deriv > 0.0 && std::isfinite(deriv)) cand = lam - r / deriv;
-      if (!(std::isfinite(cand) && cand > lo && cand < hi)) cand = 0.5 * (lo + hi);
-      lam = cand;
-      if ((rc = eval(lam))) return rc;
-      r = mass - target;
-      iters = iteration + 1;
+      if (hscal[ZM_SLOT + ZS_ACTIVE] == 0.0 || left <= 0) break;
+      batch = std::min(4, left);
     }
+    const double* hs = hscal + ZM_SLOT;
+    const int status = (int)hs[ZS_STATUS];
+    if (status == ZM_EMPTY_BRACKET) return fail(PF_E_ZERO_MODE, "PF zero-mode rejected: empty bracket");
+    if (status == ZM_NONFINITE) return fail(PF_E_ZERO_MODE, "zero-mode evaluation non-finite");
+    if (status == ZM_OUTSIDE) return fail(PF_E_ZERO_MODE, "PF zero-mode rejected: target outside bracket");
+    const int iters = (int)hs[ZS_ITERS];
+    // enqueue one step more than this search took, next time
+    zm_spec = std::min(std::max(iters + 1, 1), 8);
+    const double r = hs[ZS_R], lam = hs[ZS_LAM], invalid = hs[ZS_INVALID];
     zm_iters = iters;
-    if (std::fabs(r) > tol || invalid > 0.0) {
+    if (std::fabs(r) > c.tol || invalid > 0.0) {
       char buf[256];
       std::snprintf(buf, sizeof buf, "PF zero-mode rejected: residual=%.17e lambda=%.17e invalid=%.0f", r, lam,
                     invalid);

@@ -214,7 +214,11 @@ struct OpPhiFinish {
     a.phi_new[g] = v;
     const double phiY = a.phiY_is_old ? pre[0] : v;
     const double Y = pre[1];
-    const double xB = sigmoid_from_logit(Y, P.Y_clip, P.xB_eps);
+    // sigmoid_from_logit(Y) evaluates the logistic at the clipped Y; the
+    // D_mix sum below needs it at Y itself: one exp unless Y was clipped
+    const double Yc = Y > P.Y_clip ? P.Y_clip : (Y < -P.Y_clip ? -P.Y_clip : Y);
+    const double sc = sigmoid_unclipped(Yc);
+    const double xB = sc < P.xB_eps ? P.xB_eps : (sc > 1.0 - P.xB_eps ? 1.0 - P.xB_eps : sc);
     a.xB[g] = xB;
     const double h = h_of_phi(phiY);
     const double muA = mu_A(P, xB);
@@ -224,7 +228,7 @@ struct OpPhiFinish {
     double mu = c_bulk * (muB - muA - c_bulk * mu_total * P.dVm);
     if (P.elastic) mu += -P.eps_iso * pre[2];
     out[0] = T(mu);
-    double x = sigmoid_unclipped(Y);
+    double x = (Yc == Y) ? sc : sigmoid_unclipped(Y);
     if (x < 1e-12) x = 1e-12;
     if (x > 1.0 - 1e-12) x = 1.0 - 1e-12;
     red[0] += D_mix(P, h) * (x * (1.0 - x));

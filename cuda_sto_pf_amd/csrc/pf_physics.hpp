@@ -90,14 +90,13 @@ PF_HD double mu_tot_mix(double muA, double muB, double xB, double mu0, double h)
   return (1.0 - h) * mu_mix + h * mu0;
 }
 
-// stable logistic (cuda_kernels.cu:2201-2208, 3391-3399)
+// stable logistic (cuda_kernels.cu:2201-2208, 3391-3399): 1/(1+e^-Y) for
+// Y >= 0, e^Y/(1+e^Y) below, written without a branch so a wave with both
+// signs evaluates one exp
 PF_HD double sigmoid_unclipped(double Y) {
-  if (Y >= 0.0) {
-    double e = exp(-Y);
-    return 1.0 / (1.0 + e);
-  }
-  double e = exp(Y);
-  return e / (1.0 + e);
+  const bool pos = Y >= 0.0;
+  const double e = exp(pos ? -Y : Y);
+  return (pos ? 1.0 : e) / (1.0 + e);
 }
 // thermo_utils.h:522-536
 PF_HD double sigmoid_from_logit(double Y, double Y_clip, double eps) {

@@ -70,6 +70,10 @@ def test_fp64_step_parity(pf, name, pd, phi, xB, cap):
         assert dphi < FP64_TOL and dY < FP64_TOL and dx < FP64_TOL
         if pd["elastic_enabled"]:
             assert sc["last_elastic_iterations"] == o.last_elastic_iterations
+        if pd["zero_mode"]:
+            # the device-side root find takes the oracle's steps to the same shift
+            assert sc["zero_mode_iterations"] == o.zero_mode_iterations
+            assert math.isclose(sc["zero_mode_lambda"], o.last_lambda, rel_tol=1e-9, abs_tol=1e-13)
     sc = sim.scalars()
     assert math.isclose(sc["mass_xBtot"], o.mass(), rel_tol=1e-12)
     if pd["zero_mode"]:
@@ -145,3 +149,35 @@ def test_mixed_extended_against_fp64_reference(pf):
     assert it_g < it_o
     sc = sim.scalars()
     assert abs(sc["mass_xBtot"] - o.target_mass) <= 1e-12 * max(abs(o.target_mass), 1.0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("max_it", [0, 1, 2, 24])
+def test_zero_mode_iteration_cap(pf, max_it):
+    """zero_mode_max_iterations caps the root find (main_cuda.cu:4360-4395):
+    the GPU step fails with PF_E_ZERO_MODE exactly when the oracle's search
+    stops short of the tolerance, and otherwise takes the same steps."""
+    from oracle.step_ref import ZeroModeRejected
+    name, pd, phi, xB, cap = CASES[2]
+    pd = dict(pd, zero_mode_max_iterations=max_it)
+    o = _oracle(pd, phi, xB, cap)
+    sim = _gpu(pf, pd, phi, xB, cap, pf.PF_FP64)
+    for s in range(3):
+        try:
+            o.step()
+            oracle_ok = True
+        except ZeroModeRejected:
+            oracle_ok = False
+        if oracle_ok:
+            sim.step()
+            sc = sim.scalars()
+            print(f"max_it={max_it} step {s + 1}: iters gpu={sc['zero_mode_iterations']} "
+                  f"cpu={o.zero_mode_iterations}", flush=True)
+            assert sc["zero_mode_iterations"] == o.zero_mode_iterations
+        else:
+            with pytest.raises(pf.PfError) as ei:
+                sim.step()
+            print(f"max_it={max_it} step {s + 1}: both rejected ({ei.value})", flush=True)
+            assert ei.value.code == -5 and "zero-mode" in str(ei.value)
+            return
+    assert max_it > 0
