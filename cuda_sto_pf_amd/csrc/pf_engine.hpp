@@ -148,7 +148,8 @@ __global__ void __launch_bounds__(NT) finalize_kernel(const double* partials, in
 
 // Up to 16 sums of block partials in a fixed order; one workgroup.
 __global__ void __launch_bounds__(NT) finalize_sums_kernel(const double* partials, int nblocks, int nred,
-                                                           double* out) {
+                                                           double* out, const double* gate) {
+  if (gate && *gate == 0.0) return;
   double v[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) v[r] = 0.0;
@@ -364,6 +365,95 @@ __global__ void zm_ctrl_step_kernel(double* st, ZmConst c) {
   st[ZS_INVALID] = s[2];
   st[ZS_ITERS] += 1.0;
   zm_propose(st, c);
+}
+
+// ---- accelerated elastic solver, decided on the device (control block AaCtl)
+struct AaConst {
+  int m, iter_min, iter_max;
+  double tol, floor;
+};
+
+__global__ void aa_init_kernel(double* ctl) {
+  for (int i = threadIdx.x; i < AC_COUNT; i += blockDim.x) ctl[i] = 0.0;
+  if (threadIdx.x == 0) {
+    ctl[AC_ACTIVE] = 1.0;
+    ctl[AC_RES] = NAN;
+  }
+}
+
+// mixing coefficients: min |g - sum gamma_j dg_j| by the normal equations
+// with a 1e-12 relative Tikhonov shift, Gaussian elimination with partial
+// pivoting
+__global__ void aa_coef_kernel(double* ctl) {
+  if (threadIdx.x != 0 || ctl[AC_ACTIVE] == 0.0) return;
+  const int nh = (int)ctl[AC_NH];
+  double gam[AA_MAX] = {0.0, 0.0, 0.0, 0.0};
+  if (nh > 0) {
+    double A[AA_MAX][AA_MAX + 1];
+    double tr = 0.0;
+    for (int i = 0; i < nh; ++i) tr += ctl[AC_H + i * AA_MAX + i];
+    for (int i = 0; i < nh; ++i) {
+      for (int j = 0; j < nh; ++j) A[i][j] = ctl[AC_H + i * AA_MAX + j] + (i == j ? 1e-12 * tr / nh : 0.0);
+      A[i][nh] = ctl[AC_B + i];
+    }
+    for (int col = 0; col < nh; ++col) {
+      int piv = col;
+      for (int r = col + 1; r < nh; ++r)
+        if (fabs(A[r][col]) > fabs(A[piv][col])) piv = r;
+      if (piv != col)
+        for (int q = 0; q <= nh; ++q) {
+          const double t = A[col][q];
+          A[col][q] = A[piv][q];
+          A[piv][q] = t;
+        }
+      if (A[col][col] == 0.0) continue;
+      for (int r = col + 1; r < nh; ++r) {
+        const double f = A[r][col] / A[col][col];
+        for (int q = col; q <= nh; ++q) A[r][q] -= f * A[col][q];
+      }
+    }
+    for (int i = nh - 1; i >= 0; --i) {
+      double v = A[i][nh];
+      for (int j = i + 1; j < nh; ++j) v -= A[i][j] * gam[j];
+      gam[i] = (A[i][i] != 0.0) ? v / A[i][i] : 0.0;
+    }
+  }
+  for (int j = 0; j < AA_MAX; ++j) ctl[AC_GAM + j] = gam[j];
+}
+
+// after a Green application: residual, history Gram matrix, stopping rule
+__global__ void aa_ctrl_kernel(double* ctl, int mode, AaConst c) {
+  if (threadIdx.x != 0 || ctl[AC_ACTIVE] == 0.0) return;
+  const double* s = ctl + AC_SUMS;
+  const int it = (int)ctl[AC_IT] + 1;
+  ctl[AC_IT] = it;
+  const double gg = s[0], ff = s[1];
+  if (!isfinite(gg) || !isfinite(ff) || gg < 0.0 || ff < 0.0) {
+    ctl[AC_STATUS] = 1.0;
+    ctl[AC_ACTIVE] = 0.0;
+    return;
+  }
+  const double res = sqrt(gg / fmax(ff, c.floor));
+  ctl[AC_RES] = res;
+  int nh = (int)ctl[AC_NH];
+  if (mode == 1 && c.m > 0) {
+    const int next = (int)ctl[AC_NEXT];
+    const int nh_after = nh + 1 < c.m ? nh + 1 : c.m;
+    for (int j = 0; j < nh_after; ++j) {
+      ctl[AC_H + next * AA_MAX + j] = s[2 + j];
+      ctl[AC_H + j * AA_MAX + next] = s[2 + j];
+    }
+    nh = nh_after;
+    ctl[AC_NH] = nh;
+    ctl[AC_NEXT] = (next + 1) % c.m;
+  }
+  for (int j = 0; j < nh; ++j) ctl[AC_B + j] = s[2 + AA_MAX + j];
+  if (it >= c.iter_min && res <= c.tol) {
+    ctl[AC_CONV] = 1.0;
+    ctl[AC_ACTIVE] = 0.0;
+  } else if (it >= c.iter_max) {
+    ctl[AC_ACTIVE] = 0.0;
+  }
 }
 
 // Y = Y* + lambda, x_B = sigma(Y) unclipped (apply_pf_Y_zero_mode_shift_kernel
@@ -786,8 +876,8 @@ struct Engine final : EngineBase {
     const int rows = Nx * Ny;
     maxg = std::max<long>(maxg, (rows + 1) / 2);
     npartials = (size_t)maxg * 16;   // up to 16 reductions per block (Anderson dots use AA_NRED)
-    if ((rc = dalloc(&partials, npartials)) || (rc = dalloc(&dscal, 128))) return rc;
-    PF_HIP(hipHostMalloc((void**)&hscal, 128 * sizeof(double), hipHostMallocDefault));
+    if ((rc = dalloc(&partials, npartials)) || (rc = dalloc(&dscal, 256))) return rc;
+    PF_HIP(hipHostMalloc((void**)&hscal, 256 * sizeof(double), hipHostMallocDefault));
     return PF_OK;
   }
 
@@ -806,6 +896,7 @@ struct Engine final : EngineBase {
     for (int c = 0; c < nco; ++c) za.out[c] = sz(c_out0 + c);
     za.tw = twz;
     za.partials = partials;
+    za.gate = gate_;
     return za;
   }
 
@@ -816,6 +907,7 @@ struct Engine final : EngineBase {
     ca.A = ay.A;
     ca.a = ay.a;
     ca.tw = twy;
+    ca.gate = gate_;
     if (!inv) {
       ca.in = sz(c0);
       ca.out = sy(c0);
@@ -841,6 +933,7 @@ struct Engine final : EngineBase {
     ca.A = ax.A;
     ca.a = ax.a;
     ca.tw = twx;
+    ca.gate = gate_;
     ca.in_pstride = ca.out_pstride = 0;
     if (!inv) {
       ca.in = sy(c0);
@@ -975,106 +1068,80 @@ struct Engine final : EngineBase {
         hipLaunchKernelGGL((kop_strain<T, double>), dim3(gk), dim3(NT), 0, stream, sa);
       }
     }
-    double H[AA_MAX][AA_MAX] = {};
-    double bvec[AA_MAX] = {};
-    int nh = 0, next = 0, it = 0;
-    bool converged = false;
-    double res = NAN;
-    while (true) {
-      const int mode = (it == 0) ? 0 : 1;
-      if (mode == 1) {
-        // Anderson coefficients: min | g - sum gamma_j dg_j |
-        double gam[AA_MAX] = {};
-        if (nh > 0) {
-          double A[AA_MAX][AA_MAX + 1];
-          double tr = 0.0;
-          for (int i = 0; i < nh; ++i) tr += H[i][i];
-          for (int i = 0; i < nh; ++i) {
-            for (int j = 0; j < nh; ++j) A[i][j] = H[i][j] + (i == j ? 1e-12 * tr / nh : 0.0);
-            A[i][nh] = bvec[i];
-          }
-          for (int col = 0; col < nh; ++col) {   // Gaussian elimination, partial pivoting
-            int piv = col;
-            for (int r = col + 1; r < nh; ++r)
-              if (std::fabs(A[r][col]) > std::fabs(A[piv][col])) piv = r;
-            if (piv != col)
-              for (int q = 0; q <= nh; ++q) std::swap(A[col][q], A[piv][q]);
-            if (A[col][col] == 0.0) continue;
-            for (int r = col + 1; r < nh; ++r) {
-              const double f = A[r][col] / A[col][col];
-              for (int q = col; q <= nh; ++q) A[r][q] -= f * A[col][q];
+    // The iteration's decisions (mixing coefficients, history bookkeeping,
+    // stopping rule) are taken on the device in the control block; the host
+    // enqueues as many iterations as the previous solve took and waits once,
+    // enqueuing more only if the solve is still running.
+    double* ctl = dscal + AA_SLOT;
+    const AaConst ac{m, prm.elastic_iter_min, prm.elastic_iter_max, prm.elastic_residual_tolerance,
+                     prm.elastic_residual_absolute_floor};
+    hipLaunchKernelGGL(aa_init_kernel, dim3(1), dim3(64), 0, stream, ctl);
+    const int kmax = std::max(prm.elastic_iter_max, 1);
+    int k = 0;
+    int batch = std::max(aa_spec, 1);
+    gate_ = ctl + AC_ACTIVE;
+    for (;;) {
+      for (int b = 0; b < batch && k < kmax; ++b, ++k) {
+        const int mode = (k == 0) ? 0 : 1;
+        if (mode == 1) {
+          hipLaunchKernelGGL(aa_coef_kernel, dim3(1), dim3(64), 0, stream, ctl);
+          AAUpdateArgs<T> ua{};
+          ua.d = d;
+          ua.n = nSc;
+          for (int i = 0; i < 3; ++i) {
+            ua.g[i] = Gd + (size_t)i * nSc;
+            ua.U[i] = Ud + (size_t)i * nSc;
+            ua.dinc[i] = dinc + (size_t)i * nSc;
+            for (int j = 0; j < AA_MAX; ++j) {
+              ua.du[j][i] = du_slot(j, i);
+              ua.dg[j][i] = dg_slot(j, i);
             }
           }
-          for (int i = nh - 1; i >= 0; --i) {
-            double v = A[i][nh];
-            for (int j = i + 1; j < nh; ++j) v -= A[i][j] * gam[j];
-            gam[i] = (A[i][i] != 0.0) ? v / A[i][i] : 0.0;
-          }
+          for (int i = 0; i < 6; ++i) ua.eps[i] = sc(i);
+          ua.ctl = ctl;
+          ua.m = m;
+          hipLaunchKernelGGL(kop_aa_update<T>, dim3(gk), dim3(NT), 0, stream, ua);
         }
-        AAUpdateArgs<T> ua{};
-        ua.d = d;
-        ua.n = nSc;
+        xpass(true, 0, 6, Sc);
+        ypass(true, 0, 6);
+        typename OpHij<T>::Args oa{P, phi[cur], xB_el(), invN, mode == 1 ? 1 : 0};
+        launch_zpass<T, 6, 6, OpHij<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
+        ypass(false, 0, 6);
+        xpass(false, 0, 6, Sc);
+        GreenAAArgs<T> ga{};
+        ga.d = d;
+        ga.n = nSc;
+        for (int i = 0; i < 6; ++i) ga.h[i] = sc(i);
         for (int i = 0; i < 3; ++i) {
-          ua.g[i] = Gd + (size_t)i * nSc;
-          ua.U[i] = Ud + (size_t)i * nSc;
-          ua.dinc[i] = dinc + (size_t)i * nSc;
-          for (int j = 0; j < AA_MAX; ++j) {
-            ua.du[j][i] = du_slot(j, i);
-            ua.dg[j][i] = dg_slot(j, i);
-          }
+          ga.U[i] = Ud + (size_t)i * nSc;
+          ga.g[i] = Gd + (size_t)i * nSc;
+          ga.dinc[i] = dinc + (size_t)i * nSc;
+          for (int j = 0; j < AA_MAX; ++j) ga.dg[j][i] = dg_slot(j, i);
         }
-        for (int i = 0; i < 6; ++i) ua.eps[i] = sc(i);
-        for (int j = 0; j < AA_MAX; ++j) ua.gamma[j] = gam[j];
-        ua.nhist = nh;
-        ua.slot = (m > 0) ? next : -1;
-        hipLaunchKernelGGL(kop_aa_update<T>, dim3(gk), dim3(NT), 0, stream, ua);
+        ga.mode = mode;
+        ga.partials = partials;
+        for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
+        ga.ctl = ctl;
+        ga.m = m;
+        hipLaunchKernelGGL(kop_green_aa<T>, dim3(gk), dim3(NT), 0, stream, ga);
+        hipLaunchKernelGGL(finalize_sums_kernel, dim3(1), dim3(NT), 0, stream, partials, gk, AA_NRED,
+                           ctl + AC_SUMS, (const double*)(ctl + AC_ACTIVE));
+        hipLaunchKernelGGL(aa_ctrl_kernel, dim3(1), dim3(64), 0, stream, ctl, mode, ac);
       }
-      xpass(true, 0, 6, Sc);
-      ypass(true, 0, 6);
-      typename OpHij<T>::Args oa{P, phi[cur], xB_el(), invN, mode == 1 ? 1 : 0};
-      launch_zpass<T, 6, 6, OpHij<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
-      ypass(false, 0, 6);
-      xpass(false, 0, 6, Sc);
-      const int nh_after = (mode == 1 && m > 0) ? std::min(nh + 1, m) : nh;
-      GreenAAArgs<T> ga{};
-      ga.d = d;
-      ga.n = nSc;
-      for (int i = 0; i < 6; ++i) ga.h[i] = sc(i);
-      for (int i = 0; i < 3; ++i) {
-        ga.U[i] = Ud + (size_t)i * nSc;
-        ga.g[i] = Gd + (size_t)i * nSc;
-        ga.dinc[i] = dinc + (size_t)i * nSc;
-        for (int j = 0; j < AA_MAX; ++j) ga.dg[j][i] = dg_slot(j, i);
-      }
-      ga.nhist = nh_after;
-      ga.slot = (mode == 1 && m > 0) ? next : -1;
-      ga.mode = mode;
-      ga.partials = partials;
-      for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
-      hipLaunchKernelGGL(kop_green_aa<T>, dim3(gk), dim3(NT), 0, stream, ga);
-      hipLaunchKernelGGL(finalize_sums_kernel, dim3(1), dim3(NT), 0, stream, partials, gk, AA_NRED, dscal + 32);
-      PF_HIP(hipMemcpyAsync(hscal + 32, dscal + 32, AA_NRED * sizeof(double), hipMemcpyDeviceToHost, stream));
+      PF_HIP(hipMemcpyAsync(hscal + AA_SLOT, ctl, AC_GAM * sizeof(double), hipMemcpyDeviceToHost, stream));
       PF_HIP(hipStreamSynchronize(stream));
-      ++it;
-      const double gg = hscal[32], ff = hscal[33];
-      if (!std::isfinite(gg) || !std::isfinite(ff) || gg < 0.0 || ff < 0.0) {
-        last_iters = it;
-        return fail(PF_E_NONFINITE, "elastic residual is non-finite at step " + std::to_string(steps_done + 1));
-      }
-      res = std::sqrt(gg / std::fmax(ff, prm.elastic_residual_absolute_floor));
-      if (mode == 1 && m > 0) {
-        for (int j = 0; j < nh_after; ++j) {
-          H[next][j] = H[j][next] = hscal[34 + j];
-        }
-        nh = nh_after;
-        next = (next + 1) % m;
-      }
-      for (int j = 0; j < nh; ++j) bvec[j] = hscal[34 + AA_MAX + j];
-      if (it >= prm.elastic_iter_min && res <= prm.elastic_residual_tolerance) {
-        converged = true;
-        break;
-      }
-      if (it >= prm.elastic_iter_max) break;
+      if (hscal[AA_SLOT + AC_ACTIVE] == 0.0 || k >= kmax) break;
+      batch = 2;
+    }
+    gate_ = nullptr;
+    const double* hc = hscal + AA_SLOT;
+    const int it = (int)hc[AC_IT];
+    const double res = hc[AC_RES];
+    const bool converged = hc[AC_CONV] != 0.0;
+    aa_spec = std::max(it, 1);
+    if (hc[AC_STATUS] != 0.0) {
+      last_iters = it;
+      return fail(PF_E_NONFINITE, "elastic residual is non-finite at step " + std::to_string(steps_done + 1));
     }
     hipLaunchKernelGGL(kop_aa_final<T>, dim3(gk), dim3(NT), 0, stream, d, nSc, Ud, Ud + nSc, Ud + 2 * nSc, Gd,
                        Gd + nSc, Gd + 2 * nSc, sc(0), sc(1), sc(2), sc(3), sc(4), sc(5));
@@ -1404,6 +1471,9 @@ struct Engine final : EngineBase {
   // leading scalars come back with the search state in the one wait.
   static constexpr int ZM_SLOT = 64;   // dscal offset of the search state
   int zm_spec = 3;                     // search steps enqueued before the first wait
+  static constexpr int AA_SLOT = 128;  // dscal offset of the accelerated solver's control block
+  int aa_spec = 4;                     // iterations enqueued before the solver's first wait
+  const double* gate_ = nullptr;       // passes do nothing while *gate_ == 0 (set during the solve)
   int zero_mode(const double* phi_np1, int nfetch) {
     double* st = dscal + ZM_SLOT;
     const ZmConst c{P.Y_clip, P.v_B, target_mass,
@@ -1412,7 +1482,8 @@ struct Engine final : EngineBase {
     const int g = grid_for(R);
     hipLaunchKernelGGL(zm_eval3_kernel, dim3(g), dim3(NT), 0, stream, Ystar, phi_np1, (const double*)dscal,
                        P.Y_clip, P.v_B, R, partials);
-    hipLaunchKernelGGL(finalize_sums_kernel, dim3(1), dim3(NT), 0, stream, partials, g, 9, st + ZS_SUMS);
+    hipLaunchKernelGGL(finalize_sums_kernel, dim3(1), dim3(NT), 0, stream, partials, g, 9, st + ZS_SUMS,
+                       (const double*)nullptr);
     hipLaunchKernelGGL(zm_ctrl_open_kernel, dim3(1), dim3(64), 0, stream, st, (const double*)dscal, c);
     int left = std::max(prm.zero_mode_max_iterations, 0);
     int batch = std::min(zm_spec, left);
@@ -1420,7 +1491,8 @@ struct Engine final : EngineBase {
       for (int k = 0; k < batch; ++k) {
         hipLaunchKernelGGL(zm_eval_kernel, dim3(g), dim3(NT), 0, stream, Ystar, phi_np1, (const double*)st,
                            P.Y_clip, P.v_B, R, partials);
-        hipLaunchKernelGGL(finalize_sums_kernel, dim3(1), dim3(NT), 0, stream, partials, g, 3, st + ZS_SUMS);
+        hipLaunchKernelGGL(finalize_sums_kernel, dim3(1), dim3(NT), 0, stream, partials, g, 3, st + ZS_SUMS,
+                           (const double*)(st + ZS_ACTIVE));
         hipLaunchKernelGGL(zm_ctrl_step_kernel, dim3(1), dim3(64), 0, stream, st, c);
       }
       left -= batch;

@@ -608,6 +608,19 @@ __global__ void __launch_bounds__(NT) kop_green_acc(GreenAccArgs<T> ga) {
 constexpr int AA_MAX = 4;
 constexpr int AA_NRED = 2 + 2 * AA_MAX;
 
+// Device-resident control block of the accelerated solver (doubles), written
+// by the one-thread kernels that take the loop's decisions; the iteration's
+// kernels read their history bookkeeping from it and return at once when
+// AC_ACTIVE is 0 (iterations enqueued past convergence).
+enum AaCtl {
+  AC_ACTIVE = 0, AC_IT, AC_NH, AC_NEXT, AC_RES, AC_CONV, AC_STATUS,
+  AC_GAM = 8,                          // AA_MAX mixing coefficients
+  AC_H = AC_GAM + AA_MAX,              // AA_MAX x AA_MAX Gram matrix of the dg history
+  AC_B = AC_H + AA_MAX * AA_MAX,       // AA_MAX right-hand side <g, dg_j>
+  AC_SUMS = AC_B + AA_MAX,             // AA_NRED sums of the last Green application
+  AC_COUNT = AC_SUMS + AA_NRED
+};
+
 template <typename T>
 struct GreenAAArgs {
   KDims d;
@@ -622,6 +635,8 @@ struct GreenAAArgs {
   int mode;
   double* partials;
   double C[21];
+  const double* ctl;   // control block: nhist/slot from it (null: the fields above)
+  int m;               // history depth (with ctl)
 };
 
 __device__ __forceinline__ double wdot(const cx<double>* a, const cx<double>* b) {
@@ -651,6 +666,14 @@ __device__ __forceinline__ void green_apply_point(const double* C, double kx, do
 
 template <typename T>
 __global__ void __launch_bounds__(NT) kop_green_aa(GreenAAArgs<T> a) {
+  int nhist = a.nhist, slot = a.slot;
+  if (a.ctl) {
+    if (a.ctl[AC_ACTIVE] == 0.0) return;
+    const int nh = (int)a.ctl[AC_NH];
+    const bool grow = a.mode == 1 && a.m > 0;
+    nhist = grow ? (nh + 1 < a.m ? nh + 1 : a.m) : nh;
+    slot = grow ? (int)a.ctl[AC_NEXT] : -1;
+  }
   double acc[AA_NRED];
 #pragma unroll
   for (int r = 0; r < AA_NRED; ++r) acc[r] = 0.0;
@@ -696,7 +719,7 @@ __global__ void __launch_bounds__(NT) kop_green_aa(GreenAAArgs<T> a) {
                                 {double(q2.re), double(q2.im)}};
 #pragma unroll
       for (int j = 0; j < AA_MAX; ++j) {
-        if (j == a.slot) {
+        if (j == slot) {
           a.dg[j][0][c] = q0;
           a.dg[j][1][c] = q1;
           a.dg[j][2][c] = q2;
@@ -704,9 +727,9 @@ __global__ void __launch_bounds__(NT) kop_green_aa(GreenAAArgs<T> a) {
       }
 #pragma unroll
       for (int j = 0; j < AA_MAX; ++j) {
-        if (j < a.nhist) {
+        if (j < nhist) {
           cx<double> v[3];
-          if (j == a.slot) {
+          if (j == slot) {
             v[0] = dq[0];
             v[1] = dq[1];
             v[2] = dq[2];
@@ -744,10 +767,23 @@ struct AAUpdateArgs {
   double gamma[AA_MAX];
   int nhist;
   int slot;   // du slot receiving d (or -1 when no history is kept)
+  const double* ctl;   // control block: gamma/nhist/slot from it (null: the fields above)
+  int m;               // history depth (with ctl)
 };
 
 template <typename T>
 __global__ void __launch_bounds__(NT) kop_aa_update(AAUpdateArgs<T> a) {
+  int nhist = a.nhist, slot = a.slot;
+  double gamma[AA_MAX];
+#pragma unroll
+  for (int j = 0; j < AA_MAX; ++j) gamma[j] = a.gamma[j];
+  if (a.ctl) {
+    if (a.ctl[AC_ACTIVE] == 0.0) return;
+    nhist = (int)a.ctl[AC_NH];
+    slot = a.m > 0 ? (int)a.ctl[AC_NEXT] : -1;
+#pragma unroll
+    for (int j = 0; j < AA_MAX; ++j) gamma[j] = a.ctl[AC_GAM + j];
+  }
   const long nact_ = (a.n / a.d.Az) * a.d.Azr;   // retained modes only (no kz padding)
   for (long j_ = blockIdx.x * (long)NT + threadIdx.x; j_ < nact_; j_ += (long)gridDim.x * NT) {
     const long c = kslot(a.d, j_);
@@ -760,11 +796,11 @@ __global__ void __launch_bounds__(NT) kop_aa_update(AAUpdateArgs<T> a) {
       cx<double> v = a.g[i][c];
 #pragma unroll
       for (int j = 0; j < AA_MAX; ++j) {
-        if (j < a.nhist) {
+        if (j < nhist) {
           const cx<T> u1 = a.du[j][i][c];
           const cx<T> g1 = a.dg[j][i][c];
-          v.re -= a.gamma[j] * (double(u1.re) + double(g1.re));
-          v.im -= a.gamma[j] * (double(u1.im) + double(g1.im));
+          v.re -= gamma[j] * (double(u1.re) + double(g1.re));
+          v.im -= gamma[j] * (double(u1.im) + double(g1.im));
         }
       }
       const cx<T> q = {T(v.re), T(v.im)};
@@ -772,7 +808,7 @@ __global__ void __launch_bounds__(NT) kop_aa_update(AAUpdateArgs<T> a) {
       a.dinc[i][c] = q;
 #pragma unroll
       for (int j = 0; j < AA_MAX; ++j)
-        if (j == a.slot) a.du[j][i][c] = q;
+        if (j == slot) a.du[j][i][c] = q;
       cx<double> uo = a.U[i][c];
       a.U[i][c] = cx<double>{uo.re + dd[i].re, uo.im + dd[i].im};
     }

@@ -105,6 +105,7 @@ struct ZPassArgs {
   cx<T>* out[6];
   const cx<T>* tw;   // twiddles of length Nz
   double* partials;  // [gridDim.x][Op::NRED]
+  const double* gate;   // when set, the launch does nothing while *gate == 0
 };
 
 // One workgroup: RT consecutive rows, all components.
@@ -116,6 +117,7 @@ struct ZPassArgs {
 //     spectra, store the Az retained kz.
 template <typename T, int N, int NCI, int NCO, class Op>
 __global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op::Args oa) {
+  if (za.gate && *za.gate == 0.0) return;
   constexpr int NCM = NCI > NCO ? NCI : NCO;
   constexpr int NCB = NCM > 0 ? NCM : 1;
   constexpr int RT = zpass_rows<T, N, NCB, Op::NPRE>();
@@ -322,10 +324,12 @@ struct CPassArgs {
   long in_cstride, out_cstride;   // component strides (elements)
   int A, a;                 // compact count and retained half-width
   const cx<T>* tw;
+  const double* gate;       // when set, the launch does nothing while *gate == 0
 };
 
 template <typename T, int N, bool INV>
 __global__ void __launch_bounds__(NT) cpass_kernel(CPassArgs<T> ca) {
+  if (ca.gate && *ca.gate == 0.0) return;
   constexpr int TC = cpass_cols<T, N>();
   constexpr bool BIG = plan_has_big_prime(N);
   using LY = Lds<N, TC, true>;

@@ -60,6 +60,7 @@ struct RegPlan {
 
 template <typename T, int N, int NCI, int NCO, class Op>
 __global__ void __launch_bounds__(64) zpass_reg_kernel(ZPassArgs<T> za, typename Op::Args oa) {
+  if (za.gate && *za.gate == 0.0) return;
   constexpr int L = RegPlan<N>::L;
   constexpr int E = N / L;
   constexpr int G = 64 / L;
