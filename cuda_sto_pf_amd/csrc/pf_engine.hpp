@@ -197,28 +197,58 @@ __global__ void __launch_bounds__(NT) mass_vf_kernel(const double* phi, const do
   block_reduce_store<4>(acc, kinds, partials + (size_t)blockIdx.x * 4);
 }
 
+// Logical block index with consecutive indices on one XCD: the 8 XCDs take
+// workgroups round-robin, so block b runs on XCD b % 8.  Giving XCD x the
+// contiguous run [x nb/8, (x+1) nb/8) keeps neighbouring rows (stencil
+// halos) in one L2.  Identity when the grid is not a multiple of 8.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  return (nb & 7) ? b : (b & 7) * (nb >> 3) + (b >> 3);
+}
+
 // 0.5 kappa |grad phi|^2 by periodic central differences
-// (compute_grad_energy_density_kernel, cuda_kernels.cu:773-809)
+// (compute_grad_energy_density_kernel, cuda_kernels.cu:773-809).
+// Each block takes a contiguous run of z rows; the points of GE_U rows are
+// loaded together before they are combined.
+constexpr int GE_U = 4;
 __global__ void __launch_bounds__(NT) grad_energy_kernel(const double* phi, int Nx, int Ny, int Nz,
                                                          double dx, double dy, double dz,
                                                          double kappa, double* partials) {
   double acc[1] = {0.0};
-  // one z row per block iteration: no per-element index division
-  for (int r = blockIdx.x; r < Nx * Ny; r += gridDim.x) {
-    const int i = r / Ny, j = r - (r / Ny) * Ny;
-    const int ip = i + 1 == Nx ? 0 : i + 1, im = i == 0 ? Nx - 1 : i - 1;
-    const int jp = j + 1 == Ny ? 0 : j + 1, jm = j == 0 ? Ny - 1 : j - 1;
-    const double* row = phi + ((long)i * Ny + j) * Nz;
-    const double* rxp = phi + ((long)ip * Ny + j) * Nz;
-    const double* rxm = phi + ((long)im * Ny + j) * Nz;
-    const double* ryp = phi + ((long)i * Ny + jp) * Nz;
-    const double* rym = phi + ((long)i * Ny + jm) * Nz;
-    for (int k = threadIdx.x; k < Nz; k += NT) {
-      const int kp = k + 1 == Nz ? 0 : k + 1, km = k == 0 ? Nz - 1 : k - 1;
-      const double gx = (rxp[k] - rxm[k]) / (2.0 * dx);
-      const double gy = (ryp[k] - rym[k]) / (2.0 * dy);
-      const double gz = (row[kp] - row[km]) / (2.0 * dz);
-      acc[0] += 0.5 * kappa * (gx * gx + gy * gy + gz * gz);
+  const int rows = Nx * Ny;
+  const int L = xcd_block(blockIdx.x, gridDim.x);
+  const int rb = (rows + gridDim.x - 1) / gridDim.x;
+  const int r0 = L * rb;
+  const int r1 = r0 + rb < rows ? r0 + rb : rows;
+  const int npt = r1 > r0 ? (r1 - r0) * Nz : 0;   // a block's run: rows/grid + 1 rows
+  for (int e0 = threadIdx.x; e0 < npt; e0 += GE_U * NT) {
+    double v[GE_U][6];
+#pragma unroll
+    for (int q = 0; q < GE_U; ++q) {
+      const int e = e0 + q * NT;
+      if (e < npt) {
+        const int rr = e / Nz;
+        const int r = r0 + rr, k = e - rr * Nz;
+        const int i = r / Ny, j = r - i * Ny;
+        const int ip = i + 1 == Nx ? 0 : i + 1, im = i == 0 ? Nx - 1 : i - 1;
+        const int jp = j + 1 == Ny ? 0 : j + 1, jm = j == 0 ? Ny - 1 : j - 1;
+        const int kp = k + 1 == Nz ? 0 : k + 1, km = k == 0 ? Nz - 1 : k - 1;
+        const double* row = phi + (long)r * Nz;
+        v[q][0] = phi[((long)ip * Ny + j) * Nz + k];
+        v[q][1] = phi[((long)im * Ny + j) * Nz + k];
+        v[q][2] = phi[((long)i * Ny + jp) * Nz + k];
+        v[q][3] = phi[((long)i * Ny + jm) * Nz + k];
+        v[q][4] = row[kp];
+        v[q][5] = row[km];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < GE_U; ++q) {
+      if (e0 + q * NT < npt) {
+        const double gx = (v[q][0] - v[q][1]) / (2.0 * dx);
+        const double gy = (v[q][2] - v[q][3]) / (2.0 * dy);
+        const double gz = (v[q][4] - v[q][5]) / (2.0 * dz);
+        acc[0] += 0.5 * kappa * (gx * gx + gy * gy + gz * gz);
+      }
     }
   }
   int kinds[1] = {RED_SUM};
@@ -731,6 +761,7 @@ struct Engine final : EngineBase {
   C *Sz = nullptr, *Sy = nullptr, *Sc = nullptr, *u = nullptr, *un = nullptr;
   cx<double>* Ud = nullptr;   // FP64 accumulated displacement (accelerated solver)
   cx<double>* Gd = nullptr;   // FP64 fixed-point residual g = F(U) - U
+  T* hcache = nullptr;         // h(phi) per voxel for the strain-increment passes of a solve
   cx<double>* Hist = nullptr; // last three converged U (warm-start extrapolation)
   C* dinc = nullptr;          // current increment
   C* dU = nullptr;            // Anderson history of increments [AA_MAX][3]
@@ -837,8 +868,11 @@ struct Engine final : EngineBase {
       if ((kMixed || extended) && (rc = dalloc(&Ud, 3 * nSc))) return rc;
       if (extended) {
         if ((rc = dalloc(&Gd, 3 * nSc)) || (rc = dalloc(&Hist, 9 * nSc)) || (rc = dalloc(&dinc, 3 * nSc)) ||
-            (rc = dalloc(&dU, (size_t)AA_MAX * 3 * nSc)) || (rc = dalloc(&dG, (size_t)AA_MAX * 3 * nSc)))
+            (rc = dalloc(&dU, (size_t)AA_MAX * 3 * nSc)) || (rc = dalloc(&dG, (size_t)AA_MAX * 3 * nSc)) ||
+            (rc = dalloc(&hcache, R)))
           return rc;
+        for (int k = 0; k < 3; ++k) hbuf[k] = Hist + (size_t)k * 3 * nSc;
+        PF_HIP(hipMemset(hcache, 0, R * sizeof(T)));
       }
     }
     auto wx = twiddles<T>(Nx), wy = twiddles<T>(Ny), wz = twiddles<T>(Nz);
@@ -1013,7 +1047,15 @@ struct Engine final : EngineBase {
     return kMixed ? elastic_solve_mixed() : elastic_solve_plain();
   }
 
-  cx<double>* hist_slot(int i) { return Hist + (size_t)i * 3 * nSc; }
+  // History ring of the last converged states: three buffers, rotated with
+  // Ud by pointer (the solve leaves its result in Ud and swaps it into the
+  // ring; the buffer it gets back is overwritten by the next warm start).
+  cx<double>* hbuf[3] = {nullptr, nullptr, nullptr};   // set with the Hist allocation
+  cx<double>* hist_slot(int i) { return hbuf[i]; }
+  // the last converged displacement (FP64 state)
+  const cx<double>* U_last() {
+    return (extended && hist_count > 0) ? hist_slot((hist_head + 2) % 3) : Ud;
+  }
   C* du_slot(int j, int i) { return dU + ((size_t)j * 3 + i) * nSc; }
   C* dg_slot(int j, int i) { return dG + ((size_t)j * 3 + i) * nSc; }
 
@@ -1032,12 +1074,18 @@ struct Engine final : EngineBase {
       auto h_at = [&](int back) -> const cx<double>* {   // back = 0: latest
         return back < order ? hist_slot((hist_head - 1 - back + 3) % 3) : nullptr;
       };
-      hipLaunchKernelGGL(kop_extrapolate<0>, dim3(grid_for(3 * nSc)), dim3(NT), 0, stream, 3 * nSc, Ud, h_at(0),
-                         h_at(1), h_at(2), coef[order - 1][0], coef[order - 1][1], coef[order - 1][2]);
+      // extrapolated start and its strain in one pass over the retained modes
+      // (the kz padding of Ud stays zero)
       StrainArgs<T, double> sa{};
       sa.d = d;
       sa.n = nSc;
-      for (int i = 0; i < 3; ++i) sa.u[i] = Ud + (size_t)i * nSc;
+      for (int i = 0; i < 3; ++i) {
+        sa.u[i] = Ud + (size_t)i * nSc;
+        sa.Uout[i] = Ud + (size_t)i * nSc;
+        for (int k = 0; k < order; ++k) sa.H[k][i] = h_at(k) + (size_t)i * nSc;
+      }
+      for (int k = 0; k < order; ++k) sa.cf[k] = coef[order - 1][k];
+      sa.nh = order;
       for (int i = 0; i < 6; ++i) sa.eps[i] = sc(i);
       hipLaunchKernelGGL((kop_strain<T, double>), dim3(gk), dim3(NT), 0, stream, sa);
     } else {
@@ -1104,8 +1152,13 @@ struct Engine final : EngineBase {
         }
         xpass(true, 0, 6, Sc);
         ypass(true, 0, 6);
-        typename OpHij<T>::Args oa{P, phi[cur], xB_el(), invN, mode == 1 ? 1 : 0};
-        launch_zpass<T, 6, 6, OpHij<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
+        if (mode == 0) {
+          typename OpHij<T>::Args oa{P, phi[cur], xB_el(), invN, 0, hcache};
+          launch_zpass<T, 6, 6, OpHij<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
+        } else {
+          typename OpHijInc<T>::Args oa{P, hcache, invN};
+          launch_zpass<T, 6, 6, OpHijInc<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
+        }
         ypass(false, 0, 6);
         xpass(false, 0, 6, Sc);
         GreenAAArgs<T> ga{};
@@ -1116,7 +1169,10 @@ struct Engine final : EngineBase {
           ga.U[i] = Ud + (size_t)i * nSc;
           ga.g[i] = Gd + (size_t)i * nSc;
           ga.dinc[i] = dinc + (size_t)i * nSc;
-          for (int j = 0; j < AA_MAX; ++j) ga.dg[j][i] = dg_slot(j, i);
+          for (int j = 0; j < AA_MAX; ++j) {
+            ga.dg[j][i] = dg_slot(j, i);
+            ga.du[j][i] = du_slot(j, i);
+          }
         }
         ga.mode = mode;
         ga.partials = partials;
@@ -1147,7 +1203,7 @@ struct Engine final : EngineBase {
                        Gd + nSc, Gd + 2 * nSc, sc(0), sc(1), sc(2), sc(3), sc(4), sc(5));
     xpass(true, 0, 6, Sc);
     ypass(true, 0, 6);
-    PF_HIP(hipMemcpyAsync(hist_slot(hist_head), Ud, 3 * nSc * sizeof(cx<double>), hipMemcpyDeviceToDevice, stream));
+    std::swap(hbuf[hist_head], Ud);
     hist_head = (hist_head + 1) % 3;
     hist_count = std::min(hist_count + 1, 3);
     last_iters = it;
@@ -1651,8 +1707,8 @@ struct Engine final : EngineBase {
     ga.have_old = 0;
     hipLaunchKernelGGL(kop_green<T>, dim3(gk), dim3(NT), 0, stream, ga);
     if (state_fp64())
-      hipLaunchKernelGGL((kop_diff3<T, cx<double>>), dim3(gk), dim3(NT), 0, stream, nSc, (const cx<double>*)Ud,
-                         (const cx<double>*)(Ud + nSc), (const cx<double>*)(Ud + 2 * nSc), (const C*)uu(un, 0),
+      hipLaunchKernelGGL((kop_diff3<T, cx<double>>), dim3(gk), dim3(NT), 0, stream, nSc, U_last(),
+                         U_last() + nSc, U_last() + 2 * nSc, (const C*)uu(un, 0),
                          (const C*)uu(un, 1), (const C*)uu(un, 2), sc(0), sc(1), sc(2));
     else
       hipLaunchKernelGGL((kop_diff3<T, C>), dim3(gk), dim3(NT), 0, stream, nSc, (const C*)uu(u, 0),
@@ -1960,7 +2016,7 @@ struct Engine final : EngineBase {
   int get_elastic_state(double* out) override {
     PF_HIP(hipStreamSynchronize(stream));
     if (kMixed || extended) {
-      PF_HIP(hipMemcpy(out, Ud, 3 * nSc * sizeof(cx<double>), hipMemcpyDeviceToHost));
+      PF_HIP(hipMemcpy(out, U_last(), 3 * nSc * sizeof(cx<double>), hipMemcpyDeviceToHost));
       return PF_OK;
     }
     std::vector<C> t(3 * nSc);
@@ -2054,12 +2110,18 @@ struct Engine final : EngineBase {
     switch (id) {
       case PF_K_ZPASS_HIJ: {
         // the form the solve repeats: increments through the linear part of
-        // the map (phi only) in the mixed and accelerated solvers, the full
-        // map (phi and x_B) in the reference iteration
+        // the map in the mixed and accelerated solvers (h(phi) from the
+        // per-solve cache in the accelerated one, phi in the mixed one), the
+        // full map (phi and x_B) in the reference iteration
         const int inc = (kMixed || extended) ? 1 : 0;
-        typename OpHij<T>::Args oa{P, phi[cur], xB, invN, inc};
-        launch_zpass<T, 6, 6, OpHij<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
-        return 12.0 * nSzA * cs + (inc ? 1.0 : 2.0) * R * 8.0;
+        if (extended) {
+          typename OpHijInc<T>::Args oa{P, hcache, invN};
+          launch_zpass<T, 6, 6, OpHijInc<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
+        } else {
+          typename OpHij<T>::Args oa{P, phi[cur], xB, invN, inc, nullptr};
+          launch_zpass<T, 6, 6, OpHij<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
+        }
+        return 12.0 * nSzA * cs + (inc ? (extended ? (double)sizeof(T) : 8.0) : 16.0) * R;
       }
       case PF_K_YPASS_FWD6: ypass(false, 0, 6); return 6.0 * (nSzA + nSyA) * cs;
       case PF_K_YPASS_INV6: ypass(true, 0, 6); return 6.0 * (nSzA + nSyA) * cs;

@@ -55,6 +55,9 @@ struct OpHij {
     const double* xB;
     T invN;
     int increment;
+    // optional h(phi) per voxel in T, written by the full map for the
+    // increments of the same solve (OpHijInc; T(h) is all they use)
+    T* hc;
   };
   static constexpr int NRED = 0;
   static constexpr int kinds[1] = {0};
@@ -68,6 +71,7 @@ struct OpHij {
 #pragma unroll
     for (int c = 0; c < 6; ++c) eps[c] = vin[c] * a.invN;
     const double h = h_of_phi(pre[0]);
+    if (!a.increment && a.hc) a.hc[g] = T(h);
     if (a.increment) {
       hij_increment_point<T>(a.P, T(h), eps, out);
     } else {
@@ -78,6 +82,28 @@ struct OpHij {
       for (int c = 0; c < 6; ++c) e0t[c] = T(e0[c]);
       hij_point<T>(a.P, T(h), e0t, eps, out);
     }
+  }
+};
+
+// The linear part of the map on an increment, -h dC : eps(d), with T(h) read
+// from the per-voxel cache the full map of the same solve wrote (4 bytes per
+// voxel in mixed precision instead of phi's 8).
+template <typename T>
+struct OpHijInc {
+  struct Args {
+    PhysParams P;
+    const T* hc;
+    T invN;
+  };
+  static constexpr int NRED = 0;
+  static constexpr int kinds[1] = {0};
+  static constexpr int NPRE = 1;
+  __device__ static void load(const Args& a, size_t g, double* pre) { pre[0] = double(a.hc[g]); }
+  __device__ static void apply(const Args& a, size_t, const double* pre, const T* vin, T* out, double*) {
+    T eps[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) eps[c] = vin[c] * a.invN;
+    hij_increment_point<T>(a.P, T(pre[0]), eps, out);
   }
 };
 
@@ -637,6 +663,7 @@ struct GreenAAArgs {
   double C[21];
   const double* ctl;   // control block: nhist/slot from it (null: the fields above)
   int m;               // history depth (with ctl)
+  const cx<T>* du[AA_MAX][3];   // with ctl and m > 0: the increment is read from du[slot]
 };
 
 __device__ __forceinline__ double wdot(const cx<double>* a, const cx<double>* b) {
@@ -674,6 +701,14 @@ __global__ void __launch_bounds__(NT) kop_green_aa(GreenAAArgs<T> a) {
     nhist = grow ? (nh + 1 < a.m ? nh + 1 : a.m) : nh;
     slot = grow ? (int)a.ctl[AC_NEXT] : -1;
   }
+  // the increment just applied: the kop_aa_update copy in du[slot] when a
+  // history is kept under the control block, dinc otherwise
+  const cx<T>* const* dsrc = a.dinc;
+  if (a.ctl && slot >= 0) {
+#pragma unroll
+    for (int j = 0; j < AA_MAX; ++j)
+      if (j == slot) dsrc = a.du[j];
+  }
   double acc[AA_NRED];
 #pragma unroll
   for (int r = 0; r < AA_NRED; ++r) acc[r] = 0.0;
@@ -698,7 +733,8 @@ __global__ void __launch_bounds__(NT) kop_green_aa(GreenAAArgs<T> a) {
         dgn[i] = cx<double>{0.0, 0.0};
       } else {
         const cx<double> go = a.g[i][c];
-        const cx<double> di = {double(a.dinc[i][c].re), double(a.dinc[i][c].im)};
+        const cx<T> dt = dsrc[i][c];
+        const cx<double> di = {double(dt.re), double(dt.im)};
         gn[i] = cx<double>{go.re + Gh[i].re - di.re, go.im + Gh[i].im - di.im};
         dgn[i] = cx<double>{gn[i].re - go.re, gn[i].im - go.im};
       }
@@ -805,7 +841,7 @@ __global__ void __launch_bounds__(NT) kop_aa_update(AAUpdateArgs<T> a) {
       }
       const cx<T> q = {T(v.re), T(v.im)};
       dd[i] = cx<double>{double(q.re), double(q.im)};
-      a.dinc[i][c] = q;
+      if (!(a.ctl && slot >= 0)) a.dinc[i][c] = q;   // with a history, du[slot] holds it
 #pragma unroll
       for (int j = 0; j < AA_MAX; ++j)
         if (j == slot) a.du[j][i][c] = q;
@@ -843,31 +879,18 @@ __global__ void __launch_bounds__(NT) kop_aa_final(KDims d, long n, cx<double>* 
   }
 }
 
-// U = c1 H1 + c2 H2 + c3 H3 (polynomial extrapolation of the last converged states)
-template <int UNUSED = 0>
-__global__ void __launch_bounds__(NT) kop_extrapolate(long n3, cx<double>* U, const cx<double>* H1,
-                                                      const cx<double>* H2, const cx<double>* H3,
-                                                      double c1, double c2, double c3) {
-  for (long c = blockIdx.x * (long)NT + threadIdx.x; c < n3; c += (long)gridDim.x * NT) {
-    cx<double> v = {c1 * H1[c].re, c1 * H1[c].im};
-    if (H2) {
-      v.re += c2 * H2[c].re;
-      v.im += c2 * H2[c].im;
-    }
-    if (H3) {
-      v.re += c3 * H3[c].re;
-      v.im += c3 * H3[c].im;
-    }
-    U[c] = v;
-  }
-}
-
 template <typename T, typename UT = T>
 struct StrainArgs {
   KDims d;
   long n;
   const cx<UT>* u[3];
   cx<T>* eps[6];
+  // optional warm start in the same pass (FP64 state): U = sum_k cf[k] H[k]
+  // over the nh latest converged states, written to Uout and differentiated
+  cx<double>* Uout[3];
+  const cx<double>* H[3][3];   // [k][component]
+  double cf[3];
+  int nh;
 };
 template <typename T, typename UT = T>
 __global__ void __launch_bounds__(NT) kop_strain(StrainArgs<T, UT> sa) {
@@ -878,8 +901,28 @@ __global__ void __launch_bounds__(NT) kop_strain(StrainArgs<T, UT> sa) {
     int iz;
     kvec(sa.d, c, kx, ky, kz, iz);
     cx<double> u[3];
+    if (sa.nh > 0) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i) u[i] = cx<double>{double(sa.u[i][c].re), double(sa.u[i][c].im)};
+      for (int i = 0; i < 3; ++i) {
+        const cx<double> h1 = sa.H[0][i][c];
+        cx<double> v = {sa.cf[0] * h1.re, sa.cf[0] * h1.im};
+        if (sa.nh > 1) {
+          const cx<double> h2 = sa.H[1][i][c];
+          v.re += sa.cf[1] * h2.re;
+          v.im += sa.cf[1] * h2.im;
+        }
+        if (sa.nh > 2) {
+          const cx<double> h3 = sa.H[2][i][c];
+          v.re += sa.cf[2] * h3.re;
+          v.im += sa.cf[2] * h3.im;
+        }
+        sa.Uout[i][c] = v;
+        u[i] = v;
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) u[i] = cx<double>{double(sa.u[i][c].re), double(sa.u[i][c].im)};
+    }
     strain_of<T>(kx, ky, kz, u, sa.eps, c);
   }
 }

@@ -181,3 +181,26 @@ def test_zero_mode_iteration_cap(pf, max_it):
             assert ei.value.code == -5 and "zero-mode" in str(ei.value)
             return
     assert max_it > 0
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_elastic_state_is_last_converged_solution(pf, prec):
+    """After a step of the accelerated solver the elastic state is the
+    displacement that step converged to (the history ring rotates buffers
+    rather than copying them): a cold reference-iteration solve at the same
+    phi and x_B reaches the same field to the solver tolerance."""
+    pd, phi, xB, meta = config(2, 16)
+    cap = meta["xB_max_safe"]
+    a = _gpu(pf, _ext(pd), phi, xB, cap, pf.PF_FP64 if prec == "fp64" else pf.PF_MIXED)
+    for _ in range(5):
+        a.step()
+    phi_n, xB_n = a.field(pf.PF_PHI), a.field(pf.PF_XB)
+    a.step()   # its solve is at (phi_n, xB_n)
+    ua = a.elastic_state()
+    b = _gpu(pf, pd, phi_n, xB_n, float("inf"), pf.PF_FP64)
+    b.step()
+    ub = b.elastic_state()
+    rel = np.abs(ua - ub).max() / np.abs(ub).max()
+    print(f"{prec}: |u_accel - u_cold| / |u| = {rel:.2e}", flush=True)
+    assert rel < 1e-6
