@@ -18,7 +18,7 @@ LIB = os.path.join(HERE, "libpfcore.so")
 CLI = os.path.join(HERE, "bin", "main_pf")
 
 HIP_SOURCES = ["pf_engine_f64.hip", "pf_engine_f32.hip", "pf_runtime.hip", "pf_nn.hip"]
-CXX_SOURCES = ["pf_params.cpp", "pf_batch.cpp"]
+CXX_SOURCES = ["pf_params.cpp", "pf_batch.cpp", "pf_checkpoint.cpp"]
 CLI_SOURCES = ["main_pf.cpp"]
 
 ARCH = os.environ.get("PF_OFFLOAD_ARCH", "gfx950")

@@ -783,6 +783,9 @@ struct Engine final : EngineBase {
   long long steps_done = 0, elastic_total = 0;
   int last_iters = 0, zm_iters = 0;
   double last_res = NAN, zm_lambda = 0.0, mean_DY = 0.0;
+  // zero-mode runtime of the last accepted step (main_cuda.cu:30880-30888)
+  double zm_res = 0.0, zm_deriv = 0.0;
+  unsigned long long zm_accepted = 0;
   double e_el = 0.0, e_dw = 0.0, e_chem = 0.0, e_grad = 0.0;
   // phi-only minimize state
   bool minimizing = false;
@@ -1025,6 +1028,9 @@ struct Engine final : EngineBase {
     int rc = mass_vf(&m, &vf);
     if (rc) return rc;
     target_mass = m;
+    zm_lambda = zm_res = zm_deriv = 0.0;
+    zm_iters = 0;
+    zm_accepted = 0;
     return PF_OK;
   }
 
@@ -1575,6 +1581,9 @@ struct Engine final : EngineBase {
       return fail(PF_E_ZERO_MODE, buf);
     }
     zm_lambda = lam;
+    zm_res = r;
+    zm_deriv = hs[ZS_DERIV];
+    ++zm_accepted;
     hipLaunchKernelGGL(zm_apply_kernel, dim3(g), dim3(NT), 0, stream, Ystar, lam, Y, xB, dYdt, prm.dt, R);
     return PF_OK;
   }
@@ -2041,6 +2050,201 @@ struct Engine final : EngineBase {
     std::vector<C> t(3 * nSc);
     for (long i = 0; i < 3 * nSc; ++i) t[i] = C{T(in[2 * i]), T(in[2 * i + 1])};
     PF_HIP(hipMemcpy(u, t.data(), 3 * nSc * sizeof(C), hipMemcpyHostToDevice));
+    warm_valid = true;
+    return PF_OK;
+  }
+
+  // ---------------------------------------------------------------- checkpoint / restart
+  // The warm state after the last accepted step, FP64, compact layout (3 x nSc).
+  int warm_state_host(std::vector<cx<double>>& v) {
+    v.resize(3 * nSc);
+    if (state_fp64()) {
+      PF_HIP(hipMemcpy(v.data(), U_last(), 3 * nSc * sizeof(cx<double>), hipMemcpyDeviceToHost));
+    } else {
+      std::vector<C> t(3 * nSc);
+      PF_HIP(hipMemcpy(t.data(), u, 3 * nSc * sizeof(C), hipMemcpyDeviceToHost));
+      for (long i = 0; i < 3 * nSc; ++i) v[i] = cx<double>{double(t[i].re), double(t[i].im)};
+    }
+    return PF_OK;
+  }
+  // compact index <-> full half-spectrum index [Nx][Ny][Nz/2+1] of a retained mode
+  template <typename F>
+  void for_retained(F f) const {
+    const long kfull = (long)Nx * Ny * (Nz / 2 + 1);
+    (void)kfull;
+    for (int cx_ = 0; cx_ < ax.A; ++cx_) {
+      const int ix = cx_ < ax.a ? cx_ : cx_ + Nx - ax.A;
+      for (int cy = 0; cy < ay.A; ++cy) {
+        const int iy = cy < ay.a ? cy : cy + Ny - ay.A;
+        for (int cz = 0; cz < az.A; ++cz)
+          f(((long)cx_ * ay.A + cy) * AzS + cz, ((long)ix * Ny + iy) * (Nz / 2 + 1) + cz);
+      }
+    }
+  }
+
+  int checkpoint_write(const char* path, const pf_checkpoint_provenance* q) override {
+    if (!prm.zero_mode)
+      return fail(PF_E_ARG, "PF checkpoint/restart provenance is available only with PF_CONSERVED_Y_ZERO_MODE_V1");
+    if (minimizing) return fail(PF_E_ARG, "pf_checkpoint_write: the handle is in minimize mode");
+    pf_checkpoint_provenance def;
+    if (!q) {
+      pf_checkpoint_provenance_default(&prm, &def);
+      q = &def;
+    }
+    PF_HIP(hipStreamSynchronize(stream));
+    ckpt::Checkpoint c;
+    c.accepted_step = (std::uint64_t)steps_done;
+    c.nx = Nx;
+    c.ny = Ny;
+    c.nz = Nz;
+    c.dt_code = prm.dt;
+    c.temperature_K = prm.temperature_C + 273.15;
+    c.prov = provenance_from_abi(q);
+    c.target_mass_code = target_mass;
+    c.last_lambda = zm_lambda;
+    c.last_residual_code = zm_res;
+    c.last_derivative_code = zm_deriv;
+    c.last_iterations = (std::uint64_t)zm_iters;
+    c.accepted_zero_mode_steps = zm_accepted;
+    c.phi.resize(R);
+    c.Y.resize(R);
+    c.xB.resize(R);
+    c.dY_dt_prev.resize(R);
+    PF_HIP(hipMemcpy(c.phi.data(), phi[cur], R * sizeof(double), hipMemcpyDeviceToHost));
+    PF_HIP(hipMemcpy(c.Y.data(), Y, R * sizeof(double), hipMemcpyDeviceToHost));
+    PF_HIP(hipMemcpy(c.xB.data(), xB, R * sizeof(double), hipMemcpyDeviceToHost));
+    PF_HIP(hipMemcpy(c.dY_dt_prev.data(), dYdt, R * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<cx<double>> w;
+    if (elastic_state_required(&prm)) {
+      // main_cuda.cu:33509-33533: the converged state of the last accepted step
+      if (!warm_valid || steps_done < 1)
+        return fail(PF_E_ARG, "refusing to checkpoint an invalid elastic warm state at step " +
+                                  std::to_string(steps_done));
+      c.elastic_present = true;
+      c.elastic_source_field_step = (std::uint64_t)steps_done - 1;
+      c.elastic_last_iterations = (std::uint64_t)last_iters;
+      c.elastic_last_relative_residual = last_res;
+      int rc = warm_state_host(w);
+      if (rc) return rc;
+      const long kfull = (long)Nx * Ny * (Nz / 2 + 1);
+      c.displacement_k.assign(6 * kfull, 0.0f);
+      for (int i = 0; i < 3; ++i) {
+        float* d = c.displacement_k.data() + 2 * kfull * i;
+        const cx<double>* src = w.data() + (size_t)i * nSc;
+        for_retained([&](long cc, long fi) {
+          d[2 * fi] = (float)src[cc].re;
+          d[2 * fi + 1] = (float)src[cc].im;
+        });
+      }
+    }
+    std::string err;
+    if (!ckpt::write(path, c, &err))
+      return fail(PF_E_IO, "PF zero-mode checkpoint write failed at step " + std::to_string(steps_done) + ": " + err);
+    if (c.elastic_present) {
+      // exact-resume companion: the FP64 warm state and the extrapolation history
+      ckpt::ExactState x;
+      x.bound_checksum = c.payload_checksum;
+      x.accepted_step = c.accepted_step;
+      x.precision = kMixed ? PF_MIXED : PF_FP64;
+      x.n = (std::uint64_t)nSc;
+      const int nh = extended ? hist_count : 0;
+      x.nstates = 1 + nh;
+      x.states.resize((size_t)x.nstates * 6 * nSc);
+      std::memcpy(x.states.data(), w.data(), 3 * nSc * sizeof(cx<double>));
+      for (int k = 0; k < nh; ++k)
+        PF_HIP(hipMemcpy(x.states.data() + (size_t)(1 + k) * 6 * nSc, hist_slot((hist_head + 2 - k) % 3),
+                         3 * nSc * sizeof(cx<double>), hipMemcpyDeviceToHost));
+      if (!ckpt::write_exact(std::string(path) + ".state", x, &err)) return fail(PF_E_IO, err);
+    }
+    return PF_OK;
+  }
+
+  int checkpoint_read(const char* path, const pf_checkpoint_provenance* q) override {
+    if (!prm.zero_mode)
+      return fail(PF_E_ARG, "PF checkpoint/restart provenance is available only with PF_CONSERVED_Y_ZERO_MODE_V1");
+    pf_checkpoint_provenance def;
+    if (!q) {
+      pf_checkpoint_provenance_default(&prm, &def);
+      q = &def;
+    }
+    ckpt::Checkpoint c;
+    std::string err;
+    if (!ckpt::read(path, provenance_from_abi(q), &c, &err))
+      return fail(PF_E_IO, "PF zero-mode restart rejected: " + err);
+    // main_cuda.cu:27350-27358 (exact comparisons, as there)
+    if (c.nx != Nx || c.ny != Ny || c.nz != Nz || c.dt_code != prm.dt || c.temperature_K != prm.temperature_C + 273.15)
+      return fail(PF_E_IO, "PF zero-mode restart grid/dt/temperature mismatch");
+    const bool el = elastic_state_required(&prm);
+    if (el && !c.elastic_present) return fail(PF_E_IO, "accelerated elastic restart lacks the registered V4 warm state");
+    PF_HIP(hipStreamSynchronize(stream));
+    cur = 0;
+    PF_HIP(hipMemcpy(phi[0], c.phi.data(), R * sizeof(double), hipMemcpyHostToDevice));
+    PF_HIP(hipMemcpy(Y, c.Y.data(), R * sizeof(double), hipMemcpyHostToDevice));
+    PF_HIP(hipMemcpy(xB, c.xB.data(), R * sizeof(double), hipMemcpyHostToDevice));
+    PF_HIP(hipMemcpy(dYdt, c.dY_dt_prev.data(), R * sizeof(double), hipMemcpyHostToDevice));
+    // main_cuda.cu:28081-28105: the field ledger must match the frozen target
+    double m, vf;
+    int rc = mass_vf(&m, &vf);
+    if (rc) return rc;
+    const double tolm = 1.0e-12 * std::fmax(std::fabs(c.target_mass_code), 1.0);
+    if (!std::isfinite(m) || std::fabs(m - c.target_mass_code) > tolm) {
+      char buf[256];
+      std::snprintf(buf, sizeof buf,
+                    "checkpoint field ledger does not match frozen zero-mode target: current=%.17e target=%.17e "
+                    "residual=%.17e",
+                    m, c.target_mass_code, m - c.target_mass_code);
+      return fail(PF_E_ZERO_MODE, buf);
+    }
+    target_mass = c.target_mass_code;
+    zm_lambda = c.last_lambda;
+    zm_res = c.last_residual_code;
+    zm_deriv = c.last_derivative_code;
+    zm_iters = (int)c.last_iterations;
+    zm_accepted = c.accepted_zero_mode_steps;
+    steps_done = (long long)c.accepted_step;
+    elastic_total = 0;
+    minimizing = false;
+    warm_valid = false;
+    hist_count = 0;
+    hist_head = 0;
+    if (!el) return PF_OK;
+    last_iters = (int)c.elastic_last_iterations;
+    last_res = c.elastic_last_relative_residual;
+    ckpt::ExactState x;
+    const bool have_x = ckpt::read_exact(std::string(path) + ".state", &x, &err);
+    if (!have_x && !err.empty()) return fail(PF_E_IO, err);
+    std::vector<cx<double>> cur_state(3 * nSc);
+    int nh = 0;
+    if (have_x && x.bound_checksum == c.payload_checksum && x.accepted_step == c.accepted_step &&
+        x.n == (std::uint64_t)nSc && x.precision == (kMixed ? PF_MIXED : PF_FP64)) {
+      std::memcpy(cur_state.data(), x.states.data(), 3 * nSc * sizeof(cx<double>));
+      nh = extended ? std::min(x.nstates - 1, 3) : 0;
+      for (int k = 0; k < nh; ++k)   // newest first -> ring slots nh-1, ..., 0
+        PF_HIP(hipMemcpy(hist_slot(nh - 1 - k), x.states.data() + (size_t)(1 + k) * 6 * nSc,
+                         3 * nSc * sizeof(cx<double>), hipMemcpyHostToDevice));
+    } else {
+      // the reference's restart: the FP32 warm state of the V4 file
+      std::fill(cur_state.begin(), cur_state.end(), cx<double>{0.0, 0.0});
+      const long kfull = (long)Nx * Ny * (Nz / 2 + 1);
+      for (int i = 0; i < 3; ++i) {
+        const float* s = c.displacement_k.data() + 2 * kfull * i;
+        cx<double>* d = cur_state.data() + (size_t)i * nSc;
+        for_retained([&](long cc, long fi) { d[cc] = cx<double>{(double)s[2 * fi], (double)s[2 * fi + 1]}; });
+      }
+      if (extended) {
+        PF_HIP(hipMemcpy(hist_slot(0), cur_state.data(), 3 * nSc * sizeof(cx<double>), hipMemcpyHostToDevice));
+        nh = 1;
+      }
+    }
+    if (state_fp64()) {
+      PF_HIP(hipMemcpy(Ud, cur_state.data(), 3 * nSc * sizeof(cx<double>), hipMemcpyHostToDevice));
+    } else {
+      std::vector<C> t(3 * nSc);
+      for (long i = 0; i < 3 * nSc; ++i) t[i] = C{T(cur_state[i].re), T(cur_state[i].im)};
+      PF_HIP(hipMemcpy(u, t.data(), 3 * nSc * sizeof(C), hipMemcpyHostToDevice));
+    }
+    hist_count = nh;
+    hist_head = nh % 3;
     warm_valid = true;
     return PF_OK;
   }

@@ -3,12 +3,15 @@
 #include <memory>
 
 #include "../../include/pfcore.h"
+#include "pf_checkpoint.hpp"
 #include "pf_nn_api.hpp"
 #include "pf_physics.hpp"
 
 namespace pf {
 
 PhysParams make_phys(const pf_params& p);
+ckpt::Provenance provenance_from_abi(const pf_checkpoint_provenance* q);
+bool elastic_state_required(const pf_params* p);
 
 struct EngineBase {
   virtual ~EngineBase() = default;
@@ -32,6 +35,8 @@ struct EngineBase {
   virtual int nn_collect(nn::Net* net) = 0;
   virtual int eta_update(const pf_eta_params& q, const double* eta, const double* phi, const double* xa,
                          const double* trs, double* out) = 0;
+  virtual int checkpoint_write(const char* path, const pf_checkpoint_provenance* prov) = 0;
+  virtual int checkpoint_read(const char* path, const pf_checkpoint_provenance* prov) = 0;
 };
 
 std::unique_ptr<EngineBase> make_engine_f64();

@@ -302,6 +302,16 @@ int pf_sync(pf_handle* h) {
   return h->eng->sync();
 }
 
+int pf_checkpoint_write(pf_handle* h, const char* path, const pf_checkpoint_provenance* prov) {
+  if (!h || !path) return pf::fail(PF_E_ARG, "pf_checkpoint_write: null");
+  return h->eng->checkpoint_write(path, prov);
+}
+
+int pf_checkpoint_read(pf_handle* h, const char* path, const pf_checkpoint_provenance* prov) {
+  if (!h || !path) return pf::fail(PF_E_ARG, "pf_checkpoint_read: null");
+  return h->eng->checkpoint_read(path, prov);
+}
+
 int pf_destroy(pf_handle* h) {
   delete h;
   return PF_OK;

@@ -20,7 +20,7 @@ KERNELS = {"zpass_hij": 0, "ypass_fwd6": 1, "ypass_inv6": 2, "xpass_fwd6": 3, "x
            "green": 5, "zpass_phirhs": 6, "zpass_flux": 7}
 
 STATUS = {0: "PF_OK", -1: "PF_E_ARG", -2: "PF_E_PARAMS", -3: "PF_E_HIP", -4: "PF_E_NONCONVERGED",
-          -5: "PF_E_ZERO_MODE", -6: "PF_E_NONFINITE"}
+          -5: "PF_E_ZERO_MODE", -6: "PF_E_NONFINITE", -7: "PF_E_IO"}
 
 VOIGT21 = ["11", "12", "13", "14", "15", "16", "22", "23", "24", "25", "26", "33", "34", "35",
            "36", "44", "45", "46", "55", "56", "66"]
@@ -109,6 +109,42 @@ class PfEtaParams(C.Structure):
     ]
 
 
+class PfCheckpointProvenance(C.Structure):
+    """pf_checkpoint_provenance: the restart identity of a checkpoint
+    (pf_zero_mode_checkpoint.h:24-37)."""
+    _fields_ = [("zero_mode", C.c_char * 64), ("backend", C.c_char * 64),
+                ("composition_mode", C.c_char * 64), ("y_update_mode", C.c_char * 64),
+                ("explicit_context", C.c_char * 64), ("reaction_discretization", C.c_char * 64),
+                ("elastic_solver_mode", C.c_char * 64), ("initial_state_class", C.c_char * 96),
+                ("fixture_manifest_sha256", C.c_char * 96),
+                ("profile_library_manifest_sha256", C.c_char * 96),
+                ("parameter_fingerprint", C.c_ulonglong), ("elastic_solver_fingerprint", C.c_ulonglong)]
+
+    TEXT = ("zero_mode", "backend", "composition_mode", "y_update_mode", "explicit_context",
+            "reaction_discretization", "elastic_solver_mode", "initial_state_class",
+            "fixture_manifest_sha256", "profile_library_manifest_sha256")
+
+    def as_dict(self):
+        d = {k: getattr(self, k).decode() for k in self.TEXT}
+        d["parameter_fingerprint"] = int(self.parameter_fingerprint)
+        d["elastic_solver_fingerprint"] = int(self.elastic_solver_fingerprint)
+        return d
+
+
+class PfCheckpointHeader(C.Structure):
+    _fields_ = [("accepted_step", C.c_ulonglong), ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("version", C.c_int), ("dt_code", C.c_double), ("temperature_K", C.c_double),
+                ("target_mass_code", C.c_double), ("last_lambda", C.c_double),
+                ("last_residual_code", C.c_double), ("last_derivative_code", C.c_double),
+                ("last_iterations", C.c_ulonglong), ("accepted_zero_mode_steps", C.c_ulonglong),
+                ("elastic_present", C.c_int), ("elastic_source_field_step", C.c_ulonglong),
+                ("elastic_last_iterations", C.c_ulonglong), ("elastic_last_relative_residual", C.c_double),
+                ("payload_checksum", C.c_ulonglong)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 EXPORTS = {
     "pf_version": (C.c_int, []),
     "pf_last_error": (C.c_int, [C.c_char_p, C.c_size_t]),
@@ -158,6 +194,17 @@ EXPORTS = {
     "pf_destroy": (C.c_int, [C.c_void_p]),
     "pf_fft_forward": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "pf_fft_inverse": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
+    "pf_parameter_fingerprint": (C.c_ulonglong, [C.POINTER(PfParams)]),
+    "pf_elastic_solver_fingerprint": (C.c_ulonglong, [C.POINTER(PfParams)]),
+    "pf_checkpoint_provenance_default": (C.c_int, [C.POINTER(PfParams), C.POINTER(PfCheckpointProvenance)]),
+    "pf_checkpoint_file_write": (C.c_int, [C.c_char_p, C.POINTER(PfCheckpointProvenance),
+                                           C.POINTER(PfCheckpointHeader), C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.c_void_p, C.c_void_p]),
+    "pf_checkpoint_file_read": (C.c_int, [C.c_char_p, C.POINTER(PfCheckpointProvenance),
+                                          C.POINTER(PfCheckpointHeader), C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_void_p]),
+    "pf_checkpoint_write": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(PfCheckpointProvenance)]),
+    "pf_checkpoint_read": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(PfCheckpointProvenance)]),
 }
 
 
@@ -286,6 +333,14 @@ class Simulation:
 
     def sync(self):
         _check(lib().pf_sync(self._h))
+
+    def checkpoint_write(self, path, prov=None):
+        """Zero-mode checkpoint (V4 with the elastic warm state, V3 without)
+        plus the exact-resume companion <path>.state."""
+        _check(lib().pf_checkpoint_write(self._h, str(path).encode(), None if prov is None else C.byref(prov)))
+
+    def checkpoint_read(self, path, prov=None):
+        _check(lib().pf_checkpoint_read(self._h, str(path).encode(), None if prov is None else C.byref(prov)))
 
     def minimize_begin(self, opts):
         self._opts = opts
@@ -483,3 +538,54 @@ def fft_inverse(ak, shape, precision=PF_FP64):
     out = np.empty(shape, dtype=np.float64)
     _check(lib().pf_fft_inverse(nx, ny, nz, int(precision), _ptr(ak), _ptr(out)))
     return out
+
+
+# ---------------------------------------------------------------- checkpoint files
+def _as_params(params):
+    return params_from_dict(params) if isinstance(params, dict) else params
+
+
+def parameter_fingerprint(params):
+    return int(lib().pf_parameter_fingerprint(C.byref(_as_params(params))))
+
+
+def elastic_solver_fingerprint(params):
+    return int(lib().pf_elastic_solver_fingerprint(C.byref(_as_params(params))))
+
+
+def checkpoint_provenance(params=None, **overrides):
+    """The provenance a reference run with `params` writes, with optional
+    field overrides (strings or fingerprints)."""
+    q = PfCheckpointProvenance()
+    if params is not None:
+        _check(lib().pf_checkpoint_provenance_default(C.byref(_as_params(params)), C.byref(q)))
+    for k, v in overrides.items():
+        setattr(q, k, v.encode() if isinstance(v, str) else v)
+    return q
+
+
+def checkpoint_file_write(path, prov, header, phi, Y, xB, dYdt, displacement_k=None):
+    """header: dict of PfCheckpointHeader fields; arrays [nx][ny][nz] float64;
+    displacement_k: float32 of 6 * nx*ny*(nz/2+1) values when elastic_present."""
+    h = PfCheckpointHeader()
+    for k, v in header.items():
+        setattr(h, k, v)
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (phi, Y, xB, dYdt)]
+    dk = None if displacement_k is None else np.ascontiguousarray(displacement_k, dtype=np.float32)
+    _check(lib().pf_checkpoint_file_write(str(path).encode(), C.byref(prov), C.byref(h),
+                                          *[_ptr(a) for a in arrs], None if dk is None else _ptr(dk)))
+
+
+def checkpoint_file_read(path, expected):
+    """Returns (header dict, {phi, Y, xB, dYdt, displacement_k})."""
+    h = PfCheckpointHeader()
+    _check(lib().pf_checkpoint_file_read(str(path).encode(), C.byref(expected), C.byref(h),
+                                         None, None, None, None, None))
+    shape = (h.nx, h.ny, h.nz)
+    out = {k: np.empty(shape, dtype=np.float64) for k in ("phi", "Y", "xB", "dYdt")}
+    dk = np.empty(6 * h.nx * h.ny * (h.nz // 2 + 1), dtype=np.float32) if h.elastic_present else None
+    _check(lib().pf_checkpoint_file_read(str(path).encode(), C.byref(expected), C.byref(h),
+                                         _ptr(out["phi"]), _ptr(out["Y"]), _ptr(out["xB"]), _ptr(out["dYdt"]),
+                                         None if dk is None else _ptr(dk)))
+    out["displacement_k"] = dk
+    return h.as_dict(), out

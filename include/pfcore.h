@@ -36,7 +36,8 @@ enum pf_status {
   PF_E_HIP = -3,            /* HIP runtime failure */
   PF_E_NONCONVERGED = -4,   /* elastic fixed point did not reach the tolerance */
   PF_E_ZERO_MODE = -5,      /* zero-mode shift rejected */
-  PF_E_NONFINITE = -6       /* non-finite residual or field */
+  PF_E_NONFINITE = -6,      /* non-finite residual or field */
+  PF_E_IO = -7              /* checkpoint file: I/O, format, provenance or checksum */
 };
 
 enum pf_precision {
@@ -321,6 +322,73 @@ int pf_batch_step(pf_batch* b, int nsteps);
 /* pf_minimize on every instance with opts[i]; stats (may be NULL) gets n entries */
 int pf_batch_minimize(pf_batch* b, const pf_minimize_opts* opts, pf_minimize_stats* stats);
 int pf_batch_destroy(pf_batch* b);
+
+/* ---------------------------------------------------------------------------
+ * Zero-mode checkpoint / restart in the reference's file format
+ * (pf_zero_mode_checkpoint.h:9-96, pf_zero_mode_checkpoint.cpp:1-763):
+ * V4 with the elastic warm state (904-byte header), V3 without (800 bytes),
+ * V2 read-only; FNV-1a 64 payload checksum; written to <path>.tmp and
+ * renamed.  A file is read only when the caller's expected provenance
+ * matches it exactly.
+ *
+ * Provenance strings are NUL-padded; the fingerprints are those of
+ * main_cuda.cu:4187-4267. */
+typedef struct pf_checkpoint_provenance {
+  char zero_mode[64], backend[64], composition_mode[64], y_update_mode[64];
+  char explicit_context[64], reaction_discretization[64], elastic_solver_mode[64];
+  char initial_state_class[96], fixture_manifest_sha256[96], profile_library_manifest_sha256[96];
+  unsigned long long parameter_fingerprint, elastic_solver_fingerprint;
+} pf_checkpoint_provenance;
+
+/* The scalar part of a checkpoint (Checkpoint minus the arrays,
+ * pf_zero_mode_checkpoint.h:39-76).  version is set by the reader. */
+typedef struct pf_checkpoint_header {
+  unsigned long long accepted_step;
+  int nx, ny, nz, version;
+  double dt_code, temperature_K;
+  double target_mass_code, last_lambda, last_residual_code, last_derivative_code;
+  unsigned long long last_iterations, accepted_zero_mode_steps;
+  int elastic_present;
+  unsigned long long elastic_source_field_step, elastic_last_iterations;
+  double elastic_last_relative_residual;
+  unsigned long long payload_checksum;
+} pf_checkpoint_header;
+
+/* pf_zero_mode_parameter_fingerprint, main_cuda.cu:4187-4221 */
+unsigned long long pf_parameter_fingerprint(const pf_params* p);
+/* elastic_solver_parameter_fingerprint, main_cuda.cu:4223-4267 */
+unsigned long long pf_elastic_solver_fingerprint(const pf_params* p);
+/* The provenance a reference run with these parameters writes
+ * (main_cuda.cu:24721-24743 with the default selectors of 22478-22500):
+ * zero mode V1, host Newton/bisection backend, "legacy" / "lagged_rhs",
+ * LEGACY_UNSPECIFIED identities, and the elastic warm-state mode and
+ * fingerprint when elastic_enabled and (warm start or residual control). */
+int pf_checkpoint_provenance_default(const pf_params* p, pf_checkpoint_provenance* out);
+
+/* File level: write_checkpoint / read_checkpoint (pf_zero_mode_checkpoint.h:79-91).
+ * Arrays are [nx][ny][nz] doubles; displacement_k is ux, uy, uz over the full
+ * half spectrum [nx][ny][nz/2+1] as interleaved float (re, im), present when
+ * hdr->elastic_present.  The reader may be given NULL arrays. */
+int pf_checkpoint_file_write(const char* path, const pf_checkpoint_provenance* prov,
+                             const pf_checkpoint_header* hdr, const double* phi, const double* Y,
+                             const double* xB, const double* dYdt, const float* displacement_k);
+int pf_checkpoint_file_read(const char* path, const pf_checkpoint_provenance* expected,
+                            pf_checkpoint_header* hdr, double* phi, double* Y, double* xB, double* dYdt,
+                            float* displacement_k);
+
+/* Handle level: the periodic checkpoint of main_cuda.cu:33494-33600 and the
+ * restart of 27340-27395, 27997-28030, 28066-28105.  Requires the zero mode.
+ * Write: phi, Y, x_B, dY/dt, the zero-mode runtime and, when the reference
+ * would keep one, the elastic warm state after the last accepted step.  It
+ * also writes <path>.state, this library's exact-resume companion (the FP64
+ * warm state and extrapolation history), bound to the V4 file's checksum.
+ * Read: checks grid, dt and temperature against the handle, restores the
+ * fields as stored (no clamping), the zero-mode target (the field mass must
+ * match it to 1e-12 relative) and the step count; the warm state comes from
+ * <path>.state when it belongs to this file, else from the V4 FP32 state.
+ * prov NULL means pf_checkpoint_provenance_default of the handle's params. */
+int pf_checkpoint_write(pf_handle* h, const char* path, const pf_checkpoint_provenance* prov);
+int pf_checkpoint_read(pf_handle* h, const char* path, const pf_checkpoint_provenance* prov);
 
 /* Synchronise the handle's stream. */
 int pf_sync(pf_handle* h);
