@@ -152,6 +152,12 @@ void usage() {
       "  --init-dY-dt-prev-raw <f>\n"
       "  --pf-zero-mode PF_CONSERVED_Y_ZERO_MODE_V1|OFF\n"
       "  --init-case-tag <tag>\n"
+      "  --pf-checkpoint-every <n>            0 disables periodic checkpoints\n"
+      "  --pf-checkpoint-path <path>          atomic full-state checkpoint (+ <path>.state)\n"
+      "  --pf-restart-from <path>             provenance-strict restart\n"
+      "  --pf-initial-state-class <name>      checkpoint-pinned initial-state policy\n"
+      "  --pf-fixture-manifest-sha256 <hex>   checkpoint-pinned fixture identity\n"
+      "  --pf-profile-library-manifest-sha256 <hex> checkpoint-pinned profile-library identity\n"
       "  --precision fp64|mixed   (default fp64)\n");
 }
 
@@ -226,6 +232,10 @@ int main(int argc, char** argv) {
   const char* param_file = nullptr;
   std::string mode = "dynamics", init_mode, phi_raw, xB_raw, meta_path, dydt_raw, case_tag;
   std::string zero_mode_sel = "OFF";
+  // checkpoint / restart (main_cuda.cu:22478-22500, 22720-22760)
+  std::string ckpt_path, restart_from, init_class = "LEGACY_UNSPECIFIED", fixture_sha = "LEGACY_UNSPECIFIED",
+                                       library_sha = "LEGACY_UNSPECIFIED";
+  int ckpt_every = 0;
   int precision = PF_FP64;
   pf_minimize_opts mo;
   pf_minimize_opts_default(&mo);
@@ -260,6 +270,20 @@ int main(int argc, char** argv) {
       if (!next(&zero_mode_sel)) return fatal("--pf-zero-mode needs a value");
     } else if (a == "--init-case-tag") {
       if (!next(&case_tag)) return fatal("--init-case-tag needs a value");
+    } else if (a == "--pf-checkpoint-every" || a == "--pf_checkpoint_every") {
+      std::string v;
+      if (!next(&v)) return fatal(a + " needs a value");
+      ckpt_every = std::atoi(v.c_str());
+    } else if (a == "--pf-checkpoint-path" || a == "--pf_checkpoint_path") {
+      if (!next(&ckpt_path)) return fatal(a + " needs a path");
+    } else if (a == "--pf-restart-from" || a == "--pf_restart_from") {
+      if (!next(&restart_from)) return fatal(a + " needs a path");
+    } else if (a == "--pf-initial-state-class" || a == "--pf_initial_state_class") {
+      if (!next(&init_class)) return fatal(a + " needs a value");
+    } else if (a == "--pf-fixture-manifest-sha256" || a == "--pf_fixture_manifest_sha256") {
+      if (!next(&fixture_sha)) return fatal(a + " needs a value");
+    } else if (a == "--pf-profile-library-manifest-sha256" || a == "--pf_profile_library_manifest_sha256") {
+      if (!next(&library_sha)) return fatal(a + " needs a value");
     } else if (a == "--precision") {
       std::string v;
       if (!next(&v)) return fatal("--precision needs a value");
@@ -348,6 +372,25 @@ int main(int argc, char** argv) {
     P.zero_mode = 0;
   else
     return fatal("unknown --pf-zero-mode selector: " + zero_mode_sel);
+  // main_cuda.cu:24476-24491
+  if (ckpt_every < 0) return fatal("--pf-checkpoint-every must be >= 0");
+  if ((ckpt_every > 0 || !restart_from.empty()) && !P.zero_mode)
+    return fatal("PF checkpoint/restart provenance is available only with PF_CONSERVED_Y_ZERO_MODE_V1");
+  if (ckpt_every > 0 && ckpt_path.empty())
+    return fatal("--pf-checkpoint-path is required when periodic checkpointing is enabled");
+  if (minimize && (ckpt_every > 0 || !restart_from.empty()))
+    return fatal("checkpoint/restart applies to dynamics runs");
+  pf_checkpoint_provenance prov;
+  pf_checkpoint_provenance_default(&P, &prov);
+  auto set_identity = [](char* dst, size_t width, const std::string& v) {
+    std::memset(dst, 0, width);
+    std::strncpy(dst, v.c_str(), width - 1);
+    return v.size() < width;
+  };
+  if (!set_identity(prov.initial_state_class, sizeof prov.initial_state_class, init_class) ||
+      !set_identity(prov.fixture_manifest_sha256, sizeof prov.fixture_manifest_sha256, fixture_sha) ||
+      !set_identity(prov.profile_library_manifest_sha256, sizeof prov.profile_library_manifest_sha256, library_sha))
+    return fatal("checkpoint identity value is too long");
   if (P.out_every > P.nsteps) P.out_every = P.nsteps;
   if (P.csv_out_every > P.nsteps) P.csv_out_every = P.nsteps;
   if (P.out_every < 1) P.out_every = 1;
@@ -381,6 +424,19 @@ int main(int argc, char** argv) {
   if (pf_create(&P, precision, &h) != PF_OK) return fatal(lib_error());
   if (pf_set_fields(h, phi.data(), xB.data(), dydt.empty() ? nullptr : dydt.data(), xB_cap) != PF_OK)
     return fatal(lib_error());
+  // restart: the checkpoint replaces the raw initial state (main_cuda.cu:27340-27395)
+  long long first_step = 1;
+  if (!restart_from.empty()) {
+    if (pf_checkpoint_read(h, restart_from.c_str(), &prov) != PF_OK) return fatal(lib_error());
+    pf_scalars sr;
+    pf_scalars_get(h, &sr);
+    if (sr.steps_done >= P.nsteps)
+      return fatal("restart accepted_step=" + std::to_string(sr.steps_done) +
+                   " must be less than requested final nsteps=" + std::to_string(P.nsteps));
+    first_step = sr.steps_done + 1;
+    std::printf("PF_ZERO_MODE_RESTART_LOADED checkpoint=%s accepted_step=%lld\n", restart_from.c_str(),
+                sr.steps_done);
+  }
 
   // Results tree (main_cuda.cu:25270-25462)
   const char* root_env = std::getenv("CUDA_STO_RESULTS_ROOT");
@@ -413,13 +469,23 @@ int main(int argc, char** argv) {
   FILE* csv = std::fopen((case_dir + "/vf_precip_vs_time_" + case_tag + ".csv").c_str(), "w");
   if (csv) std::fprintf(csv, "step,t_code,t_real_s,vf_precip,R_avg\n");
   const bool accel = P.elastic_enabled && (P.elastic_warm_start_enabled || P.elastic_residual_control_enabled);
-  FILE* etrace = accel ? std::fopen((case_dir + "/elastic_solver_trace.csv").c_str(), "w") : nullptr;
-  if (etrace)
+  // a restart appends to the traces of the run it continues (main_cuda.cu:25620-25650)
+  const bool restarting = !restart_from.empty();
+  auto open_trace = [&](const std::string& path, bool* fresh) -> FILE* {
+    struct stat st;
+    const bool empty = stat(path.c_str(), &st) != 0 || st.st_size == 0;
+    std::FILE* f = std::fopen(path.c_str(), restarting ? "a" : "w");
+    *fresh = f && (!restarting || empty);   // a header only at the start of a file
+    return f;
+  };
+  bool etrace_fresh = false, ztrace_fresh = false;
+  FILE* etrace = accel ? open_trace(case_dir + "/elastic_solver_trace.csv", &etrace_fresh) : nullptr;
+  if (etrace && etrace_fresh)
     std::fprintf(etrace,
                  "step,warm_start_used,iterations,relative_residual,converged,iter_max,tolerance,"
                  "elastic_solver_wall_s,source_field_step\n");
-  FILE* ztrace = P.zero_mode ? std::fopen((case_dir + "/pf_zero_mode_trace.csv").c_str(), "w") : nullptr;
-  if (ztrace)
+  FILE* ztrace = P.zero_mode ? open_trace(case_dir + "/pf_zero_mode_trace.csv", &ztrace_fresh) : nullptr;
+  if (ztrace && ztrace_fresh)
     std::fprintf(ztrace, "step,target_mass_code,lambda,iterations,mass_code\n");
 
   pf_scalars s0;
@@ -439,7 +505,7 @@ int main(int argc, char** argv) {
     std::printf("[3D] step=%d <x_Btot>=%.10f vf_precip=%.6e R_avg=%.4f\n", step, sc.mean_xBtot,
                 sc.volume_fraction, R_avg);
   };
-  write_row(0, s0);
+  if (!restarting) write_row(0, s0);
   auto dump_vtk = [&](int step, bool final_) {
     std::vector<double> f(N);
     const std::string sfx = final_ ? "final" : std::to_string(step);
@@ -453,7 +519,7 @@ int main(int argc, char** argv) {
 
   std::vector<double> wall;
   wall.reserve(P.nsteps);
-  for (int step = 1; step <= P.nsteps; ++step) {
+  for (int step = (int)first_step; step <= P.nsteps; ++step) {
     const double t0 = now_s();
     const int rc = pf_step(h, 1);
     pf_sync(h);
@@ -476,6 +542,14 @@ int main(int argc, char** argv) {
                    sc.mass_xBtot);
     if (step % P.csv_out_every == 0 || step == P.nsteps) write_row(step, sc);
     if (step % P.out_every == 0) dump_vtk(step, false);
+    // periodic checkpoint (main_cuda.cu:33494-33600)
+    if (ckpt_every > 0 && (step % ckpt_every == 0 || step == P.nsteps)) {
+      if (pf_checkpoint_write(h, ckpt_path.c_str(), &prov) != PF_OK) return fatal(lib_error());
+      std::printf("PF_ZERO_MODE_CHECKPOINT_WRITTEN step=%d path=%s target_mass_code=%.17e lambda=%.17e "
+                  "initial_state_class=%s fixture_manifest_sha256=%s profile_library_manifest_sha256=%s\n",
+                  step, ckpt_path.c_str(), target, sc.zero_mode_lambda, prov.initial_state_class,
+                  prov.fixture_manifest_sha256, prov.profile_library_manifest_sha256);
+    }
   }
   dump_vtk(P.nsteps, true);
   if (csv) std::fclose(csv);

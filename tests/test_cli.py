@@ -51,10 +51,10 @@ def test_incomplete_param_file_is_fatal(tmp_path):
     assert r.returncode == 2 and "S_p_" in r.stderr
 
 
-def test_minimize_mode_is_refused(tmp_path):
+def test_unknown_mode_is_refused(tmp_path):
     args, *_ = _bundle(tmp_path)
-    r = _run(args + ["--mode=minimize"], tmp_path)
-    assert r.returncode == 2 and "[fatal]" in r.stderr
+    r = _run(args + ["--mode=nucleation"], tmp_path)
+    assert r.returncode == 2 and "--mode=nucleation is not implemented" in r.stderr
 
 
 @pytest.mark.gpu
@@ -118,3 +118,49 @@ def test_cli_minimize_matches_library(pf, tmp_path):
         assert int(cols[0]) == i + 1
         assert abs(float(cols[2]) - st["mean_h"]) <= 1e-8 * abs(st["mean_h"])
         assert abs(float(cols[4]) - st["lambda_vol"]) <= 1e-8 * abs(st["lambda_vol"])
+
+
+def test_checkpoint_flags_fail_closed(tmp_path):
+    """main_cuda.cu:24476-24491: checkpoints need the zero mode, a path with a
+    period, and a non-negative period."""
+    args, *_ = _bundle(tmp_path, zero_mode=False)
+    r = _run(args + ["--pf-checkpoint-every", "2", "--pf-checkpoint-path", str(tmp_path / "c.chk")], tmp_path)
+    assert r.returncode == 2 and "only with PF_CONSERVED_Y_ZERO_MODE_V1" in r.stderr
+    args, *_ = _bundle(tmp_path)
+    r = _run(args + ["--pf-checkpoint-every", "2"], tmp_path)
+    assert r.returncode == 2 and "--pf-checkpoint-path is required" in r.stderr
+    r = _run(args + ["--pf-checkpoint-every", "-1"], tmp_path)
+    assert r.returncode == 2 and "must be >= 0" in r.stderr
+
+
+@pytest.mark.gpu
+def test_cli_restart_reproduces_continuous_run(pf, tmp_path):
+    """The reference's restart qualification through the binary: 6 steps in
+    one run and 3 + restart + 3 end in byte-identical checkpoints, and the
+    restart appends to the elastic and zero-mode traces."""
+    ident = ["--pf-initial-state-class", "MASS_CONSERVING_LIBRARY_ASSEMBLED_CONDITIONAL_HANDOFF_V1",
+             "--pf-fixture-manifest-sha256", "ab" * 32]
+    args, *_ = _bundle(tmp_path, steps=6)
+    cont = str(tmp_path / "continuous.chk")
+    r = _run(args + ident + ["--pf-checkpoint-every", "6", "--pf-checkpoint-path", cont,
+                             "--init-case-tag", "cont"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "PF_ZERO_MODE_CHECKPOINT_WRITTEN step=6" in r.stdout
+    args3, *_ = _bundle(tmp_path, steps=3)
+    mid = str(tmp_path / "mid.chk")
+    r = _run(args3 + ident + ["--pf-checkpoint-every", "3", "--pf-checkpoint-path", mid,
+                              "--init-case-tag", "half"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    fin = str(tmp_path / "restart.chk")
+    r = _run(args + ident + ["--pf-restart-from", mid, "--pf-checkpoint-every", "6", "--pf-checkpoint-path", fin,
+                             "--init-case-tag", "half"], tmp_path)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0, r.stderr
+    assert "PF_ZERO_MODE_RESTART_LOADED" in r.stdout and "accepted_step=3" in r.stdout
+    assert open(cont, "rb").read() == open(fin, "rb").read()
+    # a restart without the identity it was written with is refused
+    r = _run(args + ["--pf-restart-from", mid, "--init-case-tag", "bad"], tmp_path)
+    assert r.returncode == 2 and "provenance mismatch" in r.stderr
+    runs = [d for d in os.listdir(tmp_path / "Results") if "_steps6_" in d]
+    trace = (tmp_path / "Results" / runs[0] / "half" / "elastic_solver_trace.csv").read_text().splitlines()
+    assert trace[0].startswith("step,") and [ln.split(",")[0] for ln in trace[1:]] == ["4", "5", "6"]
