@@ -663,7 +663,9 @@ struct GreenAAArgs {
   double C[21];
   const double* ctl;   // control block: nhist/slot from it (null: the fields above)
   int m;               // history depth (with ctl)
-  const cx<T>* du[AA_MAX][3];   // with ctl and m > 0: the increment is read from du[slot]
+  // with ctl and m > 0: the increment is read from du[slot], which then
+  // receives du + dg (the only combination the Anderson step uses)
+  cx<T>* du[AA_MAX][3];
 };
 
 __device__ __forceinline__ double wdot(const cx<double>* a, const cx<double>* b) {
@@ -703,8 +705,9 @@ __global__ void __launch_bounds__(NT) kop_green_aa(GreenAAArgs<T> a) {
   }
   // the increment just applied: the kop_aa_update copy in du[slot] when a
   // history is kept under the control block, dinc otherwise
+  const bool hist = a.ctl && slot >= 0;
   const cx<T>* const* dsrc = a.dinc;
-  if (a.ctl && slot >= 0) {
+  if (hist) {
 #pragma unroll
     for (int j = 0; j < AA_MAX; ++j)
       if (j == slot) dsrc = a.du[j];
@@ -725,15 +728,18 @@ __global__ void __launch_bounds__(NT) kop_green_aa(GreenAAArgs<T> a) {
     if (c != 0) green_apply_point<T>(a.C, kx, ky, kz, hh, Gh);
     else Gh[0] = Gh[1] = Gh[2] = cx<double>{0.0, 0.0};
     cx<double> U[3], gn[3], dgn[3];
+    cx<T> dT[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
       U[i] = a.U[i][c];
       if (a.mode == 0) {
         gn[i] = cx<double>{Gh[i].re - U[i].re, Gh[i].im - U[i].im};
         dgn[i] = cx<double>{0.0, 0.0};
+        dT[i] = cx<T>{T(0), T(0)};
       } else {
         const cx<double> go = a.g[i][c];
         const cx<T> dt = dsrc[i][c];
+        dT[i] = dt;
         const cx<double> di = {double(dt.re), double(dt.im)};
         gn[i] = cx<double>{go.re + Gh[i].re - di.re, go.im + Gh[i].im - di.im};
         dgn[i] = cx<double>{gn[i].re - go.re, gn[i].im - go.im};
@@ -759,6 +765,11 @@ __global__ void __launch_bounds__(NT) kop_green_aa(GreenAAArgs<T> a) {
           a.dg[j][0][c] = q0;
           a.dg[j][1][c] = q1;
           a.dg[j][2][c] = q2;
+          if (hist) {
+            a.du[j][0][c] = cx<T>{T(double(dT[0].re) + double(q0.re)), T(double(dT[0].im) + double(q0.im))};
+            a.du[j][1][c] = cx<T>{T(double(dT[1].re) + double(q1.re)), T(double(dT[1].im) + double(q1.im))};
+            a.du[j][2][c] = cx<T>{T(double(dT[2].re) + double(q2.re)), T(double(dT[2].im) + double(q2.im))};
+          }
         }
       }
 #pragma unroll
@@ -788,7 +799,9 @@ __global__ void __launch_bounds__(NT) kop_green_aa(GreenAAArgs<T> a) {
   block_reduce_store<AA_NRED>(acc, kinds, a.partials + (size_t)blockIdx.x * AA_NRED);
 }
 
-// d = g - sum_j gamma_j (du_j + dg_j);  U += d (the T-rounded d);  du[slot] = d;
+// d = g - sum_j gamma_j (du_j + dg_j);  U += d (the T-rounded d);  du[slot] = d
+// (under the control block the history slots hold du_j + dg_j, summed by
+// kop_green_aa once dg_j is known, so one array is read per history entry);
 // eps = strain(d) for the next linear-map application.
 template <typename T>
 struct AAUpdateArgs {
@@ -833,10 +846,17 @@ __global__ void __launch_bounds__(NT) kop_aa_update(AAUpdateArgs<T> a) {
 #pragma unroll
       for (int j = 0; j < AA_MAX; ++j) {
         if (j < nhist) {
-          const cx<T> u1 = a.du[j][i][c];
-          const cx<T> g1 = a.dg[j][i][c];
-          v.re -= gamma[j] * (double(u1.re) + double(g1.re));
-          v.im -= gamma[j] * (double(u1.im) + double(g1.im));
+          if (a.ctl) {
+            // the history slot holds du + dg (kop_green_aa)
+            const cx<T> e1 = a.du[j][i][c];
+            v.re -= gamma[j] * double(e1.re);
+            v.im -= gamma[j] * double(e1.im);
+          } else {
+            const cx<T> u1 = a.du[j][i][c];
+            const cx<T> g1 = a.dg[j][i][c];
+            v.re -= gamma[j] * (double(u1.re) + double(g1.re));
+            v.im -= gamma[j] * (double(u1.im) + double(g1.im));
+          }
         }
       }
       const cx<T> q = {T(v.re), T(v.im)};
