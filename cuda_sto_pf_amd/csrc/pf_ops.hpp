@@ -652,7 +652,7 @@ struct GreenAAArgs {
   KDims d;
   long n;
   const cx<T>* h[6];
-  const cx<double>* U[3];
+  cx<double>* U[3];   // with ctl, mode 1: U += d happens here (kop_aa_update leaves U alone)
   cx<double>* g[3];
   const cx<T>* dinc[3];
   cx<T>* dg[AA_MAX][3];
@@ -741,6 +741,11 @@ __global__ void __launch_bounds__(NT) kop_green_aa(GreenAAArgs<T> a) {
         const cx<T> dt = dsrc[i][c];
         dT[i] = dt;
         const cx<double> di = {double(dt.re), double(dt.im)};
+        if (a.ctl) {
+          // the increment applied by this iteration's kop_aa_update
+          U[i] = cx<double>{U[i].re + di.re, U[i].im + di.im};
+          a.U[i][c] = U[i];
+        }
         gn[i] = cx<double>{go.re + Gh[i].re - di.re, go.im + Gh[i].im - di.im};
         dgn[i] = cx<double>{gn[i].re - go.re, gn[i].im - go.im};
       }
@@ -799,10 +804,11 @@ __global__ void __launch_bounds__(NT) kop_green_aa(GreenAAArgs<T> a) {
   block_reduce_store<AA_NRED>(acc, kinds, a.partials + (size_t)blockIdx.x * AA_NRED);
 }
 
-// d = g - sum_j gamma_j (du_j + dg_j);  U += d (the T-rounded d);  du[slot] = d
-// (under the control block the history slots hold du_j + dg_j, summed by
-// kop_green_aa once dg_j is known, so one array is read per history entry);
+// d = g - sum_j gamma_j (du_j + dg_j);  U += d (the T-rounded d);  du[slot] = d;
 // eps = strain(d) for the next linear-map application.
+// Under the control block the history slots hold du_j + dg_j (summed by
+// kop_green_aa once dg_j is known), so one array is read per history entry,
+// and U += d is left to the Green update, which reads U and d anyway.
 template <typename T>
 struct AAUpdateArgs {
   KDims d;
@@ -865,8 +871,10 @@ __global__ void __launch_bounds__(NT) kop_aa_update(AAUpdateArgs<T> a) {
 #pragma unroll
       for (int j = 0; j < AA_MAX; ++j)
         if (j == slot) a.du[j][i][c] = q;
-      cx<double> uo = a.U[i][c];
-      a.U[i][c] = cx<double>{uo.re + dd[i].re, uo.im + dd[i].im};
+      if (!a.ctl) {   // under the control block kop_green_aa adds d to U
+        cx<double> uo = a.U[i][c];
+        a.U[i][c] = cx<double>{uo.re + dd[i].re, uo.im + dd[i].im};
+      }
     }
     strain_of<T>(kx, ky, kz, dd, a.eps, c);
   }
