@@ -207,8 +207,11 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 
 // 0.5 kappa |grad phi|^2 by periodic central differences
 // (compute_grad_energy_density_kernel, cuda_kernels.cu:773-809).
-// Each block takes a contiguous run of z rows; the points of GE_U rows are
-// loaded together before they are combined.
+// Each block takes a contiguous run of z rows, GE_U rows at a time with all
+// their loads issued together; row indices are wave-uniform (scalar unit) and
+// the central differences multiply by 1/(2 d), which is exact for the unit
+// spacing of the workloads and within one rounding of the reference's
+// division otherwise.
 constexpr int GE_U = 4;
 __global__ void __launch_bounds__(NT) grad_energy_kernel(const double* phi, int Nx, int Ny, int Nz,
                                                          double dx, double dy, double dz,
@@ -219,35 +222,35 @@ __global__ void __launch_bounds__(NT) grad_energy_kernel(const double* phi, int 
   const int rb = (rows + gridDim.x - 1) / gridDim.x;
   const int r0 = L * rb;
   const int r1 = r0 + rb < rows ? r0 + rb : rows;
-  const int npt = r1 > r0 ? (r1 - r0) * Nz : 0;   // a block's run: rows/grid + 1 rows
-  for (int e0 = threadIdx.x; e0 < npt; e0 += GE_U * NT) {
-    double v[GE_U][6];
+  const double hx = 1.0 / (2.0 * dx), hy = 1.0 / (2.0 * dy), hz = 1.0 / (2.0 * dz);
+  for (int r = r0; r < r1; r += GE_U) {
+    for (int k = threadIdx.x; k < Nz; k += NT) {
+      const int kp = k + 1 == Nz ? 0 : k + 1, km = k == 0 ? Nz - 1 : k - 1;
+      double v[GE_U][6];
 #pragma unroll
-    for (int q = 0; q < GE_U; ++q) {
-      const int e = e0 + q * NT;
-      if (e < npt) {
-        const int rr = e / Nz;
-        const int r = r0 + rr, k = e - rr * Nz;
-        const int i = r / Ny, j = r - i * Ny;
-        const int ip = i + 1 == Nx ? 0 : i + 1, im = i == 0 ? Nx - 1 : i - 1;
-        const int jp = j + 1 == Ny ? 0 : j + 1, jm = j == 0 ? Ny - 1 : j - 1;
-        const int kp = k + 1 == Nz ? 0 : k + 1, km = k == 0 ? Nz - 1 : k - 1;
-        const double* row = phi + (long)r * Nz;
-        v[q][0] = phi[((long)ip * Ny + j) * Nz + k];
-        v[q][1] = phi[((long)im * Ny + j) * Nz + k];
-        v[q][2] = phi[((long)i * Ny + jp) * Nz + k];
-        v[q][3] = phi[((long)i * Ny + jm) * Nz + k];
-        v[q][4] = row[kp];
-        v[q][5] = row[km];
+      for (int q = 0; q < GE_U; ++q) {
+        const int rr = r + q;
+        if (rr < r1) {
+          const int i = rr / Ny, j = rr - i * Ny;
+          const int ip = i + 1 == Nx ? 0 : i + 1, im = i == 0 ? Nx - 1 : i - 1;
+          const int jp = j + 1 == Ny ? 0 : j + 1, jm = j == 0 ? Ny - 1 : j - 1;
+          const double* row = phi + (long)rr * Nz;
+          v[q][0] = phi[((long)ip * Ny + j) * Nz + k];
+          v[q][1] = phi[((long)im * Ny + j) * Nz + k];
+          v[q][2] = phi[((long)i * Ny + jp) * Nz + k];
+          v[q][3] = phi[((long)i * Ny + jm) * Nz + k];
+          v[q][4] = row[kp];
+          v[q][5] = row[km];
+        }
       }
-    }
 #pragma unroll
-    for (int q = 0; q < GE_U; ++q) {
-      if (e0 + q * NT < npt) {
-        const double gx = (v[q][0] - v[q][1]) / (2.0 * dx);
-        const double gy = (v[q][2] - v[q][3]) / (2.0 * dy);
-        const double gz = (v[q][4] - v[q][5]) / (2.0 * dz);
-        acc[0] += 0.5 * kappa * (gx * gx + gy * gy + gz * gz);
+      for (int q = 0; q < GE_U; ++q) {
+        if (r + q < r1) {
+          const double gx = (v[q][0] - v[q][1]) * hx;
+          const double gy = (v[q][2] - v[q][3]) * hy;
+          const double gz = (v[q][4] - v[q][5]) * hz;
+          acc[0] += 0.5 * kappa * (gx * gx + gy * gy + gz * gz);
+        }
       }
     }
   }
@@ -787,6 +790,7 @@ struct Engine final : EngineBase {
   double zm_res = 0.0, zm_deriv = 0.0;
   unsigned long long zm_accepted = 0;
   double e_el = 0.0, e_dw = 0.0, e_chem = 0.0, e_grad = 0.0;
+  bool egrad_pending = false;   // e_grad of the last step not evaluated yet
   // phi-only minimize state
   bool minimizing = false;
   pf_minimize_opts mopt{};
@@ -1028,6 +1032,8 @@ struct Engine final : EngineBase {
     int rc = mass_vf(&m, &vf);
     if (rc) return rc;
     target_mass = m;
+    egrad_pending = false;
+    e_grad = 0.0;
     zm_lambda = zm_res = zm_deriv = 0.0;
     zm_iters = 0;
     zm_accepted = 0;
@@ -1451,13 +1457,12 @@ struct Engine final : EngineBase {
     double* phi_old = phi[cur];
     double* phi_new = phi[1 - cur];
     int rc;
-    // energies at the start of the step: gradient term of phi^n
-    {
-      const int g = grid_for(R);
-      hipLaunchKernelGGL(grad_energy_kernel, dim3(g), dim3(NT), 0, stream, phi_old, Nx, Ny, Nz, prm.dx,
-                         prm.dy, prm.dz, prm.kappa_phi, partials);
-      finalize(g, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 8, 1.0 / (double)R);
-    }
+    // The gradient energy of phi^n is a diagnostic (the reference's dynamics
+    // loop does not evaluate it, main_cuda.cu:29261-29290 is minimize only):
+    // it is computed when the scalars are read, from phi^n, which stays in
+    // the other ping-pong buffer until the next step (egrad_pending, set
+    // when the step is accepted).
+    egrad_pending = false;
     if (P.elastic) {
       if ((rc = elastic_solve())) return rc;
       typename OpPhiRhs<T, true>::Args oa{P, phi_old, xB, trs, invN};
@@ -1508,7 +1513,6 @@ struct Engine final : EngineBase {
       // the step's scalars come back with the zero-mode search state
       rc = zero_mode(phi_new, 13);
       mean_DY = hscal[4];
-      e_grad = hscal[8];
       e_el = hscal[9];
       e_dw = hscal[10];
       e_chem = hscal[11];
@@ -1518,12 +1522,12 @@ struct Engine final : EngineBase {
       launch_zpass<T, 1, 0, OpYFinal<T>>(Nz, stream, zargs(5, 1, 0, 0), oa);
       if ((rc = fetch(13))) return rc;
       mean_DY = hscal[4];
-      e_grad = hscal[8];
       e_el = hscal[9];
       e_dw = hscal[10];
       e_chem = hscal[11];
     }
     cur = 1 - cur;
+    egrad_pending = true;
     ++steps_done;
     return PF_OK;
   }
@@ -1998,6 +2002,17 @@ struct Engine final : EngineBase {
 
   int scalars(pf_scalars* s) override {
     std::memset(s, 0, sizeof(*s));
+    if (egrad_pending) {
+      // phi^n of the last step is in the buffer the step did not end in
+      const int g = grid_for(R);
+      hipLaunchKernelGGL(grad_energy_kernel, dim3(g), dim3(NT), 0, stream, (const double*)phi[1 - cur], Nx, Ny,
+                         Nz, prm.dx, prm.dy, prm.dz, prm.kappa_phi, partials);
+      finalize(g, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 8, 1.0 / (double)R);
+      PF_HIP(hipMemcpyAsync(hscal + 8, dscal + 8, sizeof(double), hipMemcpyDeviceToHost, stream));
+      PF_HIP(hipStreamSynchronize(stream));
+      e_grad = hscal[8];
+      egrad_pending = false;
+    }
     double m, vf, nin, nif;
     int rc = mass_vf(&m, &vf, &nin, &nif);
     if (rc) return rc;
@@ -2203,6 +2218,7 @@ struct Engine final : EngineBase {
     zm_accepted = c.accepted_zero_mode_steps;
     steps_done = (long long)c.accepted_step;
     elastic_total = 0;
+    egrad_pending = false;
     minimizing = false;
     warm_valid = false;
     hist_count = 0;
