@@ -164,3 +164,26 @@ def test_cli_restart_reproduces_continuous_run(pf, tmp_path):
     runs = [d for d in os.listdir(tmp_path / "Results") if "_steps6_" in d]
     trace = (tmp_path / "Results" / runs[0] / "half" / "elastic_solver_trace.csv").read_text().splitlines()
     assert trace[0].startswith("step,") and [ln.split(",")[0] for ln in trace[1:]] == ["4", "5", "6"]
+
+
+@pytest.mark.gpu
+def test_front_end_to_run(pf, tmp_path):
+    """Physical inputs -> .params (units.py) and a uniform matrix bundle
+    (raw_init.py) drive main_pf: a matrix-only run keeps phi = 0 and the
+    composition uniform, with the mass held by the zero mode."""
+    from cuda_sto_pf_amd import raw_init, units
+    prm = str(tmp_path / "phys.params")
+    units.write_params(prm, units.load_physical_inputs(None, gamma=0.168, lambda_sm=4e-9, pf_dx=1e-9,
+                                                       phys_dx_ref=1e-9, dt=0.02))
+    b = raw_init.uniform(tmp_path / "bundle", 16, 16, 16, 1.0, 4.0, 0.03)
+    args = [CLI, "16", "16", "16", "0.02", "3", "3", "1", "1", "--pf-param-file", prm, "--init-mode", "raw_fields",
+            "--init-phi-raw", str(b / "phi_init.raw"), "--init-xB-raw", str(b / "xB_init.raw"),
+            "--init-meta", str(b / "init_meta.json"), "--pf-zero-mode", "PF_CONSERVED_Y_ZERO_MODE_V1",
+            "--init-case-tag", "uniform"]
+    r = _run(args, tmp_path, {"CUDA_STO_SUPPRESS_VTK_OUTPUT": "1"})
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0, r.stderr
+    runs = os.listdir(tmp_path / "Results")
+    assert len(runs) == 1 and runs[0].startswith("chel_T380_cuda_16x16x16_dt0.02_steps3_xB0.030")
+    rows = (tmp_path / "Results" / runs[0] / "uniform" / "vf_precip_vs_time_uniform.csv").read_text().splitlines()
+    assert len(rows) == 1 + 4 and all(float(r.split(",")[3]) == 0.0 for r in rows[1:])

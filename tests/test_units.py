@@ -78,3 +78,19 @@ def test_physics_invariants():
     with pytest.raises(ValueError, match="orthogonal"):
         units.main_cuda_overrides(units.load_physical_inputs(None, eigenstrain_rotation_matrix=[[1, 1, 0], [0, 1, 0],
                                                                                                 [0, 0, 1]]))
+
+
+def test_uniform_raw_bundle(tmp_path):
+    """The matrix-only bundle of tools/analysis/write_uniform_matrix_raw_init.py:
+    float32 zeros and a constant x_B, and the same meta keys and values."""
+    import numpy as np
+    from cuda_sto_pf_amd import raw_init
+    raw_init.main(["uniform", "--out-dir", str(tmp_path / "b"), "--nx", "4", "--ny", "3", "--nz", "2",
+                   "--dx-nm", "1", "--interface-width-nm", "4", "--xb-out", "0.03"])
+    d = tmp_path / "b"
+    assert np.array_equal(np.fromfile(d / "phi_init.raw", dtype=np.float32), np.zeros(24, np.float32))
+    assert np.array_equal(np.fromfile(d / "xB_init.raw", dtype=np.float32), np.full(24, 0.03, np.float32))
+    meta = json.loads((d / "init_meta.json").read_text())
+    assert meta == {"Nx": 4, "Ny": 3, "Nz": 2, "dx_nm": 1.0, "interface_width_nm": 4.0, "dtype": "float32",
+                    "order": "C", "dt_recommended": None, "mean_xBtot": 0.03, "xB_max_safe": 0.03,
+                    "reference_type": "matrix_only_same_strain", "mean_phi": 0.0, "mean_h": 0.0, "voxel_count": 0}
