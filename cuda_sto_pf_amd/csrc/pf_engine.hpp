@@ -2349,6 +2349,7 @@ struct Engine final : EngineBase {
       case PF_K_XPASS_INV6: xpass(true, 0, 6, Sc); return 6.0 * (nSyA + nScA) * cs;
       case PF_K_GREEN: {
         if (extended) {
+          // the solve's form: control block with a full history (set by time_kernel)
           const int m = prm.elastic_anderson_depth;
           GreenAAArgs<T> ga{};
           ga.d = d;
@@ -2358,15 +2359,20 @@ struct Engine final : EngineBase {
             ga.U[i] = Ud + (size_t)i * nSc;
             ga.g[i] = Gd + (size_t)i * nSc;
             ga.dinc[i] = dinc + (size_t)i * nSc;
-            for (int j = 0; j < AA_MAX; ++j) ga.dg[j][i] = dg_slot(j, i);
+            for (int j = 0; j < AA_MAX; ++j) {
+              ga.dg[j][i] = dg_slot(j, i);
+              ga.du[j][i] = du_slot(j, i);
+            }
           }
-          ga.nhist = m;
-          ga.slot = m > 0 ? 0 : -1;
           ga.mode = 1;
           ga.partials = partials;
           for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
+          ga.ctl = dscal + AA_SLOT;
+          ga.m = m;
           hipLaunchKernelGGL(kop_green_aa<T>, dim3(gk), dim3(NT), 0, stream, ga);
-          return (6.0 + 3.0 + 3.0 * m + (m > 0 ? 3.0 : 0.0)) * nScA * cs + 9.0 * nScA * 16.0;
+          // h read; U, g read and written; with a history: du[slot] read and
+          // written, dg[slot] written, the other m-1 dg slots read
+          return (m > 0 ? 12.0 + 3.0 * m : 9.0) * nScA * cs + 12.0 * nScA * 16.0;
         }
         GreenArgs<T> ga{};
         ga.d = d;
@@ -2385,6 +2391,28 @@ struct Engine final : EngineBase {
         hipLaunchKernelGGL(kop_green<T>, dim3(gk), dim3(NT), 0, stream, ga);
         return (12.0 + 6.0) * nScA * cs;
       }
+      case PF_K_AA_UPDATE: {
+        const int m = prm.elastic_anderson_depth;
+        if (!extended || m == 0) return 0.0;
+        AAUpdateArgs<T> ua{};
+        ua.d = d;
+        ua.n = nSc;
+        for (int i = 0; i < 3; ++i) {
+          ua.g[i] = Gd + (size_t)i * nSc;
+          ua.U[i] = Ud + (size_t)i * nSc;
+          ua.dinc[i] = dinc + (size_t)i * nSc;
+          for (int j = 0; j < AA_MAX; ++j) {
+            ua.du[j][i] = du_slot(j, i);
+            ua.dg[j][i] = dg_slot(j, i);
+          }
+        }
+        for (int i = 0; i < 6; ++i) ua.eps[i] = sc(i);
+        ua.ctl = dscal + AA_SLOT;
+        ua.m = m;
+        hipLaunchKernelGGL(kop_aa_update<T>, dim3(gk), dim3(NT), 0, stream, ua);
+        // g read; the m history slots read; du[slot] and the 6 strains written
+        return (3.0 * m + 9.0) * nScA * cs + 3.0 * nScA * 16.0;
+      }
       case PF_K_ZPASS_PHIRHS: {
         typename OpPhiRhs<T, true>::Args oa{P, phi[cur], xB, trs, invN};
         launch_zpass<T, 6, 1, OpPhiRhs<T, true>>(Nz, stream, zargs(0, 6, 0, 1), oa);
@@ -2402,6 +2430,17 @@ struct Engine final : EngineBase {
   }
 
   int time_kernel(int id, int reps, double* us, double* bytes) override {
+    if (extended && (id == PF_K_GREEN || id == PF_K_AA_UPDATE)) {
+      // the accelerated solver's control block in its steady state: running,
+      // a full history, slot 0 next, fixed mixing coefficients
+      double* hc = hscal + AA_SLOT;
+      for (int i = 0; i < AC_COUNT; ++i) hc[i] = 0.0;
+      hc[AC_ACTIVE] = 1.0;
+      hc[AC_NH] = (double)prm.elastic_anderson_depth;
+      for (int j = 0; j < AA_MAX; ++j) hc[AC_GAM + j] = 0.25;
+      PF_HIP(hipMemcpyAsync(dscal + AA_SLOT, hc, AC_COUNT * sizeof(double), hipMemcpyHostToDevice, stream));
+      PF_HIP(hipStreamSynchronize(stream));
+    }
     hipEvent_t e0, e1;
     PF_HIP(hipEventCreate(&e0));
     PF_HIP(hipEventCreate(&e1));

@@ -17,7 +17,7 @@ PF_FP64 = 0
 PF_MIXED = 1
 PF_PHI, PF_XB, PF_Y, PF_DYDT, PF_XBTOT, PF_TRSIGMA = range(6)
 KERNELS = {"zpass_hij": 0, "ypass_fwd6": 1, "ypass_inv6": 2, "xpass_fwd6": 3, "xpass_inv6": 4,
-           "green": 5, "zpass_phirhs": 6, "zpass_flux": 7}
+           "green": 5, "zpass_phirhs": 6, "zpass_flux": 7, "aa_update": 8}
 
 STATUS = {0: "PF_OK", -1: "PF_E_ARG", -2: "PF_E_PARAMS", -3: "PF_E_HIP", -4: "PF_E_NONCONVERGED",
           -5: "PF_E_ZERO_MODE", -6: "PF_E_NONFINITE", -7: "PF_E_IO"}

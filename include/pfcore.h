@@ -159,10 +159,14 @@ enum pf_kernel_id {
   PF_K_YPASS_INV6 = 2,     /* y pass, inverse, 6 components */
   PF_K_XPASS_FWD6 = 3,     /* x pass, forward, 6 components */
   PF_K_XPASS_INV6 = 4,     /* x pass, inverse, 6 components */
-  PF_K_GREEN = 5,          /* k-space Green update + residual + strain */
+  PF_K_GREEN = 5,          /* k-space Green update + residual (+ the Anderson history in the
+                              accelerated solver, with a full history) */
   PF_K_ZPASS_PHIRHS = 6,   /* z pass: inverse 6 strains -> stress, dF/dphi -> forward 1 */
   PF_K_ZPASS_FLUX = 7,     /* z pass: inverse 3 gradients -> fluxes -> forward 5 */
-  PF_K_COUNT = 8
+  PF_K_AA_UPDATE = 8,      /* Anderson mixing update + strain of the increment (accelerated
+                              solver with a history only; otherwise nothing is launched and
+                              the bytes are 0) */
+  PF_K_COUNT = 9
 };
 
 /* Time `reps` back-to-back launches of one kernel of the step on the
