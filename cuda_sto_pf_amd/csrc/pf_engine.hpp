@@ -146,10 +146,9 @@ __global__ void __launch_bounds__(NT) finalize_kernel(const double* partials, in
     for (int r = 0; r < nred; ++r) out[r] = kinds.k[r] == RED_SUM ? outv[r] * scale : outv[r];
 }
 
-// Up to 16 sums of block partials in a fixed order; one workgroup.
-__global__ void __launch_bounds__(NT) finalize_sums_kernel(const double* partials, int nblocks, int nred,
-                                                           double* out, const double* gate) {
-  if (gate && *gate == 0.0) return;
+// Up to 16 sums of block partials in a fixed order, into the workgroup's
+// shared outv[16] (valid after the call for every thread).
+__device__ __forceinline__ void sum_partials16(const double* partials, int nblocks, int nred, double* outv) {
   double v[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) v[r] = 0.0;
@@ -171,11 +170,18 @@ __global__ void __launch_bounds__(NT) finalize_sums_kernel(const double* partial
 #pragma unroll
     for (int r = 0; r < 16; ++r)
       if (r < nred) v[r] += partials[(size_t)b * nred + r];
-  __shared__ double outv[16];
   int kinds[16];
   for (int r = 0; r < 16; ++r) kinds[r] = RED_SUM;
   block_reduce_store<16>(v, kinds, outv);
   __syncthreads();
+}
+
+// Up to 16 sums of block partials in a fixed order; one workgroup.
+__global__ void __launch_bounds__(NT) finalize_sums_kernel(const double* partials, int nblocks, int nred,
+                                                           double* out, const double* gate) {
+  if (gate && *gate == 0.0) return;
+  __shared__ double outv[16];
+  sum_partials16(partials, nblocks, nred, outv);
   if (threadIdx.x == 0)
     for (int r = 0; r < nred; ++r) out[r] = outv[r];
 }
@@ -346,8 +352,7 @@ __device__ void zm_propose(double* st, const ZmConst& c) {
   st[ZS_ACTIVE] = 1.0;
 }
 
-__global__ void zm_ctrl_open_kernel(double* st, const double* ymm, ZmConst c) {
-  if (threadIdx.x != 0) return;
+__device__ void zm_ctrl_open(double* st, const double* ymm, ZmConst c) {
   const double* s = st + ZS_SUMS;
   const double lo = -c.Y_clip - ymm[0];
   const double hi = c.Y_clip - ymm[1];
@@ -384,8 +389,7 @@ __global__ void zm_ctrl_open_kernel(double* st, const double* ymm, ZmConst c) {
   zm_propose(st, c);
 }
 
-__global__ void zm_ctrl_step_kernel(double* st, ZmConst c) {
-  if (threadIdx.x != 0 || st[ZS_ACTIVE] == 0.0) return;
+__device__ void zm_ctrl_step(double* st, ZmConst c) {
   const double* s = st + ZS_SUMS;
   st[ZS_LAM] = st[ZS_NEXT];
   if (!isfinite(s[0]) || !isfinite(s[1])) {
@@ -398,6 +402,29 @@ __global__ void zm_ctrl_step_kernel(double* st, ZmConst c) {
   st[ZS_INVALID] = s[2];
   st[ZS_ITERS] += 1.0;
   zm_propose(st, c);
+}
+
+// The evaluation's sums into the search state, then the bracket set-up
+// (open) or one safeguarded step (step); one workgroup.  A step launched
+// after the search has ended does nothing.
+__global__ void __launch_bounds__(NT) zm_open_end_kernel(const double* partials, int nblocks, double* st,
+                                                         const double* ymm, ZmConst c) {
+  __shared__ double outv[16];
+  sum_partials16(partials, nblocks, 9, outv);
+  if (threadIdx.x == 0) {
+    for (int r = 0; r < 9; ++r) st[ZS_SUMS + r] = outv[r];
+    zm_ctrl_open(st, ymm, c);
+  }
+}
+__global__ void __launch_bounds__(NT) zm_step_end_kernel(const double* partials, int nblocks, double* st,
+                                                         ZmConst c) {
+  if (st[ZS_ACTIVE] == 0.0) return;
+  __shared__ double outv[16];
+  sum_partials16(partials, nblocks, 3, outv);
+  if (threadIdx.x == 0) {
+    for (int r = 0; r < 3; ++r) st[ZS_SUMS + r] = outv[r];
+    zm_ctrl_step(st, c);
+  }
 }
 
 // ---- accelerated elastic solver, decided on the device (control block AaCtl)
@@ -417,8 +444,8 @@ __global__ void aa_init_kernel(double* ctl) {
 // mixing coefficients: min |g - sum gamma_j dg_j| by the normal equations
 // with a 1e-12 relative Tikhonov shift, Gaussian elimination with partial
 // pivoting
-__global__ void aa_coef_kernel(double* ctl) {
-  if (threadIdx.x != 0 || ctl[AC_ACTIVE] == 0.0) return;
+__device__ void aa_coef(double* ctl) {
+  if (ctl[AC_ACTIVE] == 0.0) return;
   const int nh = (int)ctl[AC_NH];
   double gam[AA_MAX] = {0.0, 0.0, 0.0, 0.0};
   if (nh > 0) {
@@ -455,8 +482,8 @@ __global__ void aa_coef_kernel(double* ctl) {
 }
 
 // after a Green application: residual, history Gram matrix, stopping rule
-__global__ void aa_ctrl_kernel(double* ctl, int mode, AaConst c) {
-  if (threadIdx.x != 0 || ctl[AC_ACTIVE] == 0.0) return;
+__device__ void aa_ctrl(double* ctl, int mode, AaConst c) {
+  if (ctl[AC_ACTIVE] == 0.0) return;
   const double* s = ctl + AC_SUMS;
   const int it = (int)ctl[AC_IT] + 1;
   ctl[AC_IT] = it;
@@ -486,6 +513,21 @@ __global__ void aa_ctrl_kernel(double* ctl, int mode, AaConst c) {
     ctl[AC_ACTIVE] = 0.0;
   } else if (it >= c.iter_max) {
     ctl[AC_ACTIVE] = 0.0;
+  }
+}
+
+// End of an iteration, one workgroup: the Green update's sums into the
+// control block, the residual / history / stopping decision, and the next
+// iteration's mixing coefficients (unused when the solve has stopped).
+__global__ void __launch_bounds__(NT) aa_end_kernel(const double* partials, int nblocks, double* ctl, int mode,
+                                                    AaConst c) {
+  if (ctl[AC_ACTIVE] == 0.0) return;
+  __shared__ double outv[16];
+  sum_partials16(partials, nblocks, AA_NRED, outv);
+  if (threadIdx.x == 0) {
+    for (int r = 0; r < AA_NRED; ++r) ctl[AC_SUMS + r] = outv[r];
+    aa_ctrl(ctl, mode, c);
+    aa_coef(ctl);
   }
 }
 
@@ -1144,7 +1186,7 @@ struct Engine final : EngineBase {
       for (int b = 0; b < batch && k < kmax; ++b, ++k) {
         const int mode = (k == 0) ? 0 : 1;
         if (mode == 1) {
-          hipLaunchKernelGGL(aa_coef_kernel, dim3(1), dim3(64), 0, stream, ctl);
+          // mixing coefficients: from the previous iteration's aa_end_kernel
           AAUpdateArgs<T> ua{};
           ua.d = d;
           ua.n = nSc;
@@ -1192,9 +1234,7 @@ struct Engine final : EngineBase {
         ga.ctl = ctl;
         ga.m = m;
         hipLaunchKernelGGL(kop_green_aa<T>, dim3(gk), dim3(NT), 0, stream, ga);
-        hipLaunchKernelGGL(finalize_sums_kernel, dim3(1), dim3(NT), 0, stream, partials, gk, AA_NRED,
-                           ctl + AC_SUMS, (const double*)(ctl + AC_ACTIVE));
-        hipLaunchKernelGGL(aa_ctrl_kernel, dim3(1), dim3(64), 0, stream, ctl, mode, ac);
+        hipLaunchKernelGGL(aa_end_kernel, dim3(1), dim3(NT), 0, stream, partials, gk, ctl, mode, ac);
       }
       PF_HIP(hipMemcpyAsync(hscal + AA_SLOT, ctl, AC_GAM * sizeof(double), hipMemcpyDeviceToHost, stream));
       PF_HIP(hipStreamSynchronize(stream));
@@ -1548,18 +1588,15 @@ struct Engine final : EngineBase {
     const int g = grid_for(R);
     hipLaunchKernelGGL(zm_eval3_kernel, dim3(g), dim3(NT), 0, stream, Ystar, phi_np1, (const double*)dscal,
                        P.Y_clip, P.v_B, R, partials);
-    hipLaunchKernelGGL(finalize_sums_kernel, dim3(1), dim3(NT), 0, stream, partials, g, 9, st + ZS_SUMS,
-                       (const double*)nullptr);
-    hipLaunchKernelGGL(zm_ctrl_open_kernel, dim3(1), dim3(64), 0, stream, st, (const double*)dscal, c);
+    hipLaunchKernelGGL(zm_open_end_kernel, dim3(1), dim3(NT), 0, stream, partials, g, st, (const double*)dscal,
+                       c);
     int left = std::max(prm.zero_mode_max_iterations, 0);
     int batch = std::min(zm_spec, left);
     for (;;) {
       for (int k = 0; k < batch; ++k) {
         hipLaunchKernelGGL(zm_eval_kernel, dim3(g), dim3(NT), 0, stream, Ystar, phi_np1, (const double*)st,
                            P.Y_clip, P.v_B, R, partials);
-        hipLaunchKernelGGL(finalize_sums_kernel, dim3(1), dim3(NT), 0, stream, partials, g, 3, st + ZS_SUMS,
-                           (const double*)(st + ZS_ACTIVE));
-        hipLaunchKernelGGL(zm_ctrl_step_kernel, dim3(1), dim3(64), 0, stream, st, c);
+        hipLaunchKernelGGL(zm_step_end_kernel, dim3(1), dim3(NT), 0, stream, partials, g, st, c);
       }
       left -= batch;
       PF_HIP(hipMemcpyAsync(hscal, dscal, nfetch * sizeof(double), hipMemcpyDeviceToHost, stream));
