@@ -410,21 +410,29 @@ __device__ void zm_ctrl_step(double* st, ZmConst c) {
 __global__ void __launch_bounds__(NT) zm_open_end_kernel(const double* partials, int nblocks, double* st,
                                                          const double* ymm, ZmConst c) {
   __shared__ double outv[16];
+  __shared__ double ss[ZS_COUNT];   // the search state, worked on in LDS by one thread
+  for (int i = threadIdx.x; i < ZS_COUNT; i += NT) ss[i] = st[i];
   sum_partials16(partials, nblocks, 9, outv);
   if (threadIdx.x == 0) {
-    for (int r = 0; r < 9; ++r) st[ZS_SUMS + r] = outv[r];
-    zm_ctrl_open(st, ymm, c);
+    for (int r = 0; r < 9; ++r) ss[ZS_SUMS + r] = outv[r];
+    zm_ctrl_open(ss, ymm, c);
   }
+  __syncthreads();
+  for (int i = threadIdx.x; i < ZS_COUNT; i += NT) st[i] = ss[i];
 }
 __global__ void __launch_bounds__(NT) zm_step_end_kernel(const double* partials, int nblocks, double* st,
                                                          ZmConst c) {
   if (st[ZS_ACTIVE] == 0.0) return;
   __shared__ double outv[16];
+  __shared__ double ss[ZS_COUNT];
+  for (int i = threadIdx.x; i < ZS_COUNT; i += NT) ss[i] = st[i];
   sum_partials16(partials, nblocks, 3, outv);
   if (threadIdx.x == 0) {
-    for (int r = 0; r < 3; ++r) st[ZS_SUMS + r] = outv[r];
-    zm_ctrl_step(st, c);
+    for (int r = 0; r < 3; ++r) ss[ZS_SUMS + r] = outv[r];
+    zm_ctrl_step(ss, c);
   }
+  __syncthreads();
+  for (int i = threadIdx.x; i < ZS_COUNT; i += NT) st[i] = ss[i];
 }
 
 // ---- accelerated elastic solver, decided on the device (control block AaCtl)
@@ -523,12 +531,16 @@ __global__ void __launch_bounds__(NT) aa_end_kernel(const double* partials, int 
                                                     AaConst c) {
   if (ctl[AC_ACTIVE] == 0.0) return;
   __shared__ double outv[16];
+  __shared__ double sc[AC_COUNT];   // the control block, worked on in LDS by one thread
+  for (int i = threadIdx.x; i < AC_COUNT; i += NT) sc[i] = ctl[i];
   sum_partials16(partials, nblocks, AA_NRED, outv);
   if (threadIdx.x == 0) {
-    for (int r = 0; r < AA_NRED; ++r) ctl[AC_SUMS + r] = outv[r];
-    aa_ctrl(ctl, mode, c);
-    aa_coef(ctl);
+    for (int r = 0; r < AA_NRED; ++r) sc[AC_SUMS + r] = outv[r];
+    aa_ctrl(sc, mode, c);
+    aa_coef(sc);
   }
+  __syncthreads();
+  for (int i = threadIdx.x; i < AC_COUNT; i += NT) ctl[i] = sc[i];
 }
 
 // Y = Y* + lambda, x_B = sigma(Y) unclipped (apply_pf_Y_zero_mode_shift_kernel
