@@ -153,15 +153,15 @@ __device__ __forceinline__ void sum_partials16(const double* partials, int nbloc
 #pragma unroll
   for (int r = 0; r < 16; ++r) v[r] = 0.0;
   int b = threadIdx.x;
-  for (; b + 3 * NT < nblocks; b += 4 * NT) {
-    double t[4][16];
+  for (; b + (FIN_U - 1) * NT < nblocks; b += FIN_U * NT) {
+    double t[FIN_U][16];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < FIN_U; ++u)
 #pragma unroll
       for (int r = 0; r < 16; ++r)
         if (r < nred) t[u][r] = partials[(size_t)(b + u * NT) * nred + r];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < FIN_U; ++u)
 #pragma unroll
       for (int r = 0; r < 16; ++r)
         if (r < nred) v[r] += t[u][r];
