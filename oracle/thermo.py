@@ -6,6 +6,14 @@ may import this package; the product path never does.
 
 Each function follows the named reference source line by line in its
 arithmetic, vectorised over numpy arrays (float64).
+
+Pinning (tests/test_oracle_pins.py): the finite-difference Gamma against the
+analytic derivative, x_eq against the tangent condition, h and g against
+their closed-form values and the logit/sigmoid round trip; the
+SGTE/CALPHAD Gibbs energies (which have no known-answer value in the
+reference) against an independent restatement of the reference's
+physical-unit converter (Unit_Psedobinary.py:441-485), on both sides of every
+breakpoint (tests/test_units.py).
 """
 import math
 

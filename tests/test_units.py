@@ -94,3 +94,24 @@ def test_uniform_raw_bundle(tmp_path):
     assert meta == {"Nx": 4, "Ny": 3, "Nz": 2, "dx_nm": 1.0, "interface_width_nm": 4.0, "dtype": "float32",
                     "order": "C", "dt_recommended": None, "mean_xBtot": 0.03, "xB_max_safe": 0.03,
                     "reference_type": "matrix_only_same_strain", "mean_phi": 0.0, "mean_h": 0.0, "voxel_count": 0}
+
+
+def test_calphad_functions_agree_with_the_step_thermodynamics():
+    """The converter's SGTE/CALPHAD functions (restated from
+    Unit_Psedobinary.py:441-485) and the time step's (oracle/thermo.py, from
+    thermo_utils.h:316-377) are two independent transcriptions of two
+    reference sources: they agree on both sides of every SGTE breakpoint
+    (600.61 K Pb, 722.66 K Te, 1234.93 K Ag), so a dropped term or a wrong
+    coefficient in either fails here."""
+    import numpy as np
+    from oracle import thermo
+    for T in (400.0, 600.0, 600.62, 653.15, 700.0, 722.65, 722.67, 900.0, 1234.92, 1234.94, 1400.0):
+        assert units._ghser_pb(T) == pytest.approx(thermo.GHSER_Pb(T), rel=1e-14, abs=1e-9)
+        assert units._ghser_ag(T) == pytest.approx(thermo.GHSER_Ag(T), rel=1e-14, abs=1e-9)
+        assert units._ghser_te(T) == pytest.approx(thermo.GHSER_Te(T), rel=1e-14, abs=1e-9)
+        assert units._g_pbte(T) == pytest.approx(thermo.G_PbTe_Solid(T), rel=1e-14)
+        assert units._g_ag2te(T) == pytest.approx(thermo.G_Ag2Te_Solid(T), rel=1e-14)
+        assert units._l0(T) == pytest.approx(thermo.get_L_param(T), rel=1e-15)
+        for x in (1e-6, 0.004665, 0.03, 0.5, 0.97):
+            assert units.mu_pbte(T, x) == pytest.approx(float(thermo.mu_PbTe_calphad(T, np.float64(x))), rel=1e-13)
+            assert units.mu_ag2te(T, x) == pytest.approx(float(thermo.mu_Ag2Te_calphad(T, np.float64(x))), rel=1e-13)
