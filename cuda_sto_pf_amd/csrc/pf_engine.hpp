@@ -176,16 +176,6 @@ __device__ __forceinline__ void sum_partials16(const double* partials, int nbloc
   __syncthreads();
 }
 
-// Up to 16 sums of block partials in a fixed order; one workgroup.
-__global__ void __launch_bounds__(NT) finalize_sums_kernel(const double* partials, int nblocks, int nred,
-                                                           double* out, const double* gate) {
-  if (gate && *gate == 0.0) return;
-  __shared__ double outv[16];
-  sum_partials16(partials, nblocks, nred, outv);
-  if (threadIdx.x == 0)
-    for (int r = 0; r < nred; ++r) out[r] = outv[r];
-}
-
 // sum (1-h) x + v_B h, sum h(clamp01 phi), count(phi > 0.5), count(0.12 < phi < 0.88)
 // (cuda_kernels.cu:3792-3828, 5580-5607, 3746-3790)
 __global__ void __launch_bounds__(NT) mass_vf_kernel(const double* phi, const double* xB, double vB,
@@ -1585,8 +1575,9 @@ struct Engine final : EngineBase {
   }
 
   // run_pf_conserved_y_zero_mode_host (main_cuda.cu:4269-4405), decided on the
-  // device (zm_ctrl_*): the Y* extremes are in dscal[0..1]; `nfetch` more
-  // leading scalars come back with the search state in the one wait.
+  // device (zm_open_end_kernel, zm_step_end_kernel): the Y* extremes are in
+  // dscal[0..1]; `nfetch` more leading scalars come back with the search
+  // state in the one wait.
   static constexpr int ZM_SLOT = 64;   // dscal offset of the search state
   int zm_spec = 3;                     // search steps enqueued before the first wait
   static constexpr int AA_SLOT = 128;  // dscal offset of the accelerated solver's control block

@@ -40,6 +40,8 @@ def _cases():
     out.append(("rect_hetero_zm",) + _rect_case())
     out.append(("rect_hetero_nozm",) + _rect_case(zero_mode=False))
     out.append(("rect_noelastic",) + _rect_case(elastic=False))
+    # Nz = 128 keeps 43 kz: spectral rows padded to whole 128-B lines
+    out.append(("rect_kz_padded",) + _rect_case(shape=(24, 20, 128)))
     return out
 
 
@@ -107,7 +109,8 @@ def _ext(pd, order=3, depth=3):
 
 
 EXT_CASES = [("cfg2_16_ext",) + (lambda c: (_ext(c[0]), c[1], c[2], c[3]["xB_max_safe"]))(config(2, 16)),
-             ("rect_ext_m2",) + (lambda c: (_ext(c[0], 2, 2), c[1], c[2], c[3]))(_rect_case())]
+             ("rect_ext_m2",) + (lambda c: (_ext(c[0], 2, 2), c[1], c[2], c[3]))(_rect_case()),
+             ("rect_kz_padded_ext",) + (lambda c: (_ext(c[0]), c[1], c[2], c[3]))(_rect_case(shape=(24, 20, 128)))]
 
 
 @pytest.mark.gpu
