@@ -164,13 +164,25 @@ class KSpace:
         self.herm_w = np.broadcast_to(w[None, None, :], (p.Nx, p.Ny, nzc))
 
 
+# Threads for the transforms (None: numpy's single-threaded pocketfft, the
+# default everywhere the oracle checks results; an integer runs scipy.fft with
+# that many workers, used only to time a multi-threaded CPU baseline).
+FFT_WORKERS = None
+
+
 def fwd(a, ks):
     """Unnormalised R2C (cuFFT D2Z/R2C) followed by dealias."""
+    if FFT_WORKERS:
+        import scipy.fft
+        return scipy.fft.rfftn(a, workers=FFT_WORKERS) * ks.mask
     return np.fft.rfftn(a) * ks.mask
 
 
 def inv(ak, shape):
     """Z2D/C2R followed by the explicit x(1/N)."""
+    if FFT_WORKERS:
+        import scipy.fft
+        return scipy.fft.irfftn(ak, s=shape, axes=(0, 1, 2), workers=FFT_WORKERS)
     return np.fft.irfftn(ak, s=shape, axes=(0, 1, 2))
 
 
