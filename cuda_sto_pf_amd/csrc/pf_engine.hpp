@@ -835,6 +835,8 @@ struct Engine final : EngineBase {
   unsigned long long zm_accepted = 0;
   double e_el = 0.0, e_dw = 0.0, e_chem = 0.0, e_grad = 0.0;
   bool egrad_pending = false;   // e_grad of the last step not evaluated yet
+  T* sig6 = nullptr;            // stress components of the last step (keep_stress)
+  bool keep_sig6 = false, sig6_valid = false;
   // phi-only minimize state
   bool minimizing = false;
   pf_minimize_opts mopt{};
@@ -1077,6 +1079,7 @@ struct Engine final : EngineBase {
     if (rc) return rc;
     target_mass = m;
     egrad_pending = false;
+    sig6_valid = false;
     e_grad = 0.0;
     zm_lambda = zm_res = zm_deriv = 0.0;
     zm_iters = 0;
@@ -1505,9 +1508,10 @@ struct Engine final : EngineBase {
     // the other ping-pong buffer until the next step (egrad_pending, set
     // when the step is accepted).
     egrad_pending = false;
+    sig6_valid = false;
     if (P.elastic) {
       if ((rc = elastic_solve())) return rc;
-      typename OpPhiRhs<T, true>::Args oa{P, phi_old, xB, trs, invN};
+      typename OpPhiRhs<T, true>::Args oa{P, phi_old, xB, trs, invN, keep_sig6 ? sig6 : nullptr, (size_t)R};
       const int g = launch_zpass<T, 6, 1, OpPhiRhs<T, true>>(Nz, stream, zargs(0, 6, 0, 1), oa);
       finalize(g, 3, RedKinds{{RED_SUM, RED_SUM, RED_SUM, 0}}, dscal + 9, 1.0 / (double)R);
     } else {
@@ -1570,6 +1574,7 @@ struct Engine final : EngineBase {
     }
     cur = 1 - cur;
     egrad_pending = true;
+    sig6_valid = keep_sig6 && P.elastic;
     ++steps_done;
     return PF_OK;
   }
@@ -2009,6 +2014,20 @@ struct Engine final : EngineBase {
   }
 
   // ---------------------------------------------------------------- queries
+  // The 6 stress components of the step's elastic solution at the
+  // start-of-step fields (the reference's sigma arrays) are written by the
+  // phi right-hand-side pass while enabled: 6 R values of the transform
+  // precision, allocated on the first enable and kept until the handle goes.
+  int keep_stress(int enable) override {
+    if (enable && !sig6) {
+      int rc = dalloc(&sig6, (size_t)6 * R);
+      if (rc) return rc;
+    }
+    keep_sig6 = enable != 0;
+    if (!keep_sig6) sig6_valid = false;
+    return PF_OK;
+  }
+
   int get_field(int which, double* out) override {
     PF_HIP(hipStreamSynchronize(stream));
     const double* src = nullptr;
@@ -2030,6 +2049,16 @@ struct Engine final : EngineBase {
       case PF_TRSIGMA: {
         std::vector<T> t(R);
         PF_HIP(hipMemcpy(t.data(), trs, R * sizeof(T), hipMemcpyDeviceToHost));
+        for (long i = 0; i < R; ++i) out[i] = double(t[i]);
+        return PF_OK;
+      }
+      case PF_SIGMA_XX: case PF_SIGMA_YY: case PF_SIGMA_ZZ:
+      case PF_SIGMA_YZ: case PF_SIGMA_XZ: case PF_SIGMA_XY: {
+        if (!sig6_valid)
+          return fail(PF_E_ARG, "stress components are kept only for steps taken after pf_keep_stress(h, 1)");
+        std::vector<T> t(R);
+        PF_HIP(hipMemcpy(t.data(), sig6 + (size_t)(which - PF_SIGMA_XX) * R, R * sizeof(T),
+                         hipMemcpyDeviceToHost));
         for (long i = 0; i < R; ++i) out[i] = double(t[i]);
         return PF_OK;
       }
@@ -2259,6 +2288,7 @@ struct Engine final : EngineBase {
     steps_done = (long long)c.accepted_step;
     elastic_total = 0;
     egrad_pending = false;
+    sig6_valid = false;
     minimizing = false;
     warm_valid = false;
     hist_count = 0;

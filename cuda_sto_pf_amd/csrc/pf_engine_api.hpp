@@ -19,6 +19,7 @@ struct EngineBase {
   virtual int set_fields(const double* phi, const double* xB, const double* dYdt, double xB_cap) = 0;
   virtual int step() = 0;
   virtual int get_field(int which, double* out) = 0;
+  virtual int keep_stress(int enable) = 0;
   virtual int scalars(pf_scalars* out) = 0;
   virtual long long elastic_state_size() = 0;
   virtual int get_elastic_state(double* out) = 0;

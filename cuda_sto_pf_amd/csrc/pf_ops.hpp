@@ -147,6 +147,8 @@ struct OpPhiRhs {
     const double* xB;
     T* trs;        // out: tr(sigma) per voxel (ELASTIC)
     T invN;
+    T* sig6 = nullptr;   // optional out: the 6 stress components (xx,yy,zz,yz,xz,xy), stride n
+    size_t n = 0;
   };
   static constexpr int NRED = 3;
   static constexpr int kinds[3] = {RED_SUM, RED_SUM, RED_SUM};
@@ -202,6 +204,10 @@ struct OpPhiRhs {
                               T(2) * el[4] * q[4] + T(2) * el[5] * q[5]);
       f += dgel_eigen + 0.5 * hp * Q;
       a.trs[g] = T(sig[0] + sig[1] + sig[2]);
+      if (a.sig6) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) a.sig6[c * a.n + g] = sig[c];
+      }
       red[0] += 0.5 * double(sig[0] * el[0] + sig[1] * el[1] + sig[2] * el[2] +
                              T(2) * (sig[5] * el[5] + sig[4] * el[4] + sig[3] * el[3]));
     }

@@ -197,6 +197,11 @@ int pf_get_field(pf_handle* h, int which, double* out) {
   return h->eng->get_field(which, out);
 }
 
+int pf_keep_stress(pf_handle* h, int enable) {
+  if (!h) return pf::fail(PF_E_ARG, "pf_keep_stress: null");
+  return h->eng->keep_stress(enable);
+}
+
 int pf_scalars_get(pf_handle* h, pf_scalars* out) {
   if (!h || !out) return pf::fail(PF_E_ARG, "pf_scalars_get: null");
   return h->eng->scalars(out);

@@ -16,6 +16,9 @@ LIB_PATH = os.environ.get("PFCORE_LIB", os.path.join(HERE, "libpfcore.so"))
 PF_FP64 = 0
 PF_MIXED = 1
 PF_PHI, PF_XB, PF_Y, PF_DYDT, PF_XBTOT, PF_TRSIGMA = range(6)
+# stress components xx, yy, zz, yz, xz, xy (kept while keep_stress is on)
+PF_SIGMA_XX, PF_SIGMA_YY, PF_SIGMA_ZZ, PF_SIGMA_YZ, PF_SIGMA_XZ, PF_SIGMA_XY = range(6, 12)
+PF_SIGMA = (PF_SIGMA_XX, PF_SIGMA_YY, PF_SIGMA_ZZ, PF_SIGMA_YZ, PF_SIGMA_XZ, PF_SIGMA_XY)
 KERNELS = {"zpass_hij": 0, "ypass_fwd6": 1, "ypass_inv6": 2, "xpass_fwd6": 3, "xpass_inv6": 4,
            "green": 5, "zpass_phirhs": 6, "zpass_flux": 7, "aa_update": 8}
 
@@ -155,6 +158,7 @@ EXPORTS = {
     "pf_set_fields": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double]),
     "pf_step": (C.c_int, [C.c_void_p, C.c_int]),
     "pf_get_field": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    "pf_keep_stress": (C.c_int, [C.c_void_p, C.c_int]),
     "pf_scalars_get": (C.c_int, [C.c_void_p, C.POINTER(PfScalars)]),
     "pf_elastic_state_size": (C.c_longlong, [C.c_void_p]),
     "pf_get_elastic_state": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -315,6 +319,10 @@ class Simulation:
         out = np.empty(self.shape, dtype=np.float64)
         _check(lib().pf_get_field(self._h, int(which), _ptr(out)))
         return out
+
+    def keep_stress(self, enable=True):
+        """Keep the 6 stress components of each following step (PF_SIGMA_*)."""
+        _check(lib().pf_keep_stress(self._h, 1 if enable else 0))
 
     def scalars(self):
         s = PfScalars()

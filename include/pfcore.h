@@ -46,7 +46,11 @@ enum pf_precision {
 };
 
 enum pf_field {
-  PF_PHI = 0, PF_XB = 1, PF_Y = 2, PF_DYDT = 3, PF_XBTOT = 4, PF_TRSIGMA = 5
+  PF_PHI = 0, PF_XB = 1, PF_Y = 2, PF_DYDT = 3, PF_XBTOT = 4, PF_TRSIGMA = 5,
+  /* stress components of the last step's elastic solution at its start-of-step
+     fields (the reference's d_sigma_*_r arrays, main_cuda.cu:29138-29166);
+     available for steps taken after pf_keep_stress(h, 1) */
+  PF_SIGMA_XX = 6, PF_SIGMA_YY = 7, PF_SIGMA_ZZ = 8, PF_SIGMA_YZ = 9, PF_SIGMA_XZ = 10, PF_SIGMA_XY = 11
 };
 
 /* Code-unit parameters the time step reads (subset of PFParams, pf_params.h:6-702).
@@ -138,8 +142,17 @@ int pf_set_fields(pf_handle* h, const double* phi, const double* xB, const doubl
 /* Advance nsteps time steps (the loop body main_cuda.cu:28427-31623). */
 int pf_step(pf_handle* h, int nsteps);
 
-/* Copy a real field to the host (N doubles). */
+/* Copy a real field to the host (N doubles).  PF_SIGMA_* fail with PF_E_ARG
+ * unless the last step was taken with pf_keep_stress enabled. */
 int pf_get_field(pf_handle* h, int which, double* out);
+
+/* Keep (enable = 1) or stop keeping (0) the 6 stress components of each step
+ * for pf_get_field; they are written by the pass that already evaluates the
+ * stress (6 N values of the transform precision per step, allocated on the
+ * first enable and held until pf_destroy).  Replaces reading the reference's
+ * device stress arrays after a step (main_cuda.cu:29138-29166, the stress
+ * that cuda_kernels.cu:7381-7480 evaluates). */
+int pf_keep_stress(pf_handle* h, int enable);
 
 /* Mass, volume fraction, energy densities and solver statistics. */
 int pf_scalars_get(pf_handle* h, pf_scalars* out);

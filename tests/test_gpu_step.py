@@ -207,3 +207,37 @@ def test_elastic_state_is_last_converged_solution(pf, prec):
     rel = np.abs(ua - ub).max() / np.abs(ub).max()
     print(f"{prec}: |u_accel - u_cold| / |u| = {rel:.2e}", flush=True)
     assert rel < 1e-6
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,pd,phi,xB,cap", [CASES[1], CASES[2]], ids=[CASES[1][0], CASES[2][0]])
+def test_stress_components(pf, name, pd, phi, xB, cap):
+    """PF_SIGMA_*: the step's stress at its start-of-step fields, against the
+    oracle's sigma of the same step (FP64 to 1e-10 of the stress scale; the
+    mixed path's FP32 stress to 1e-5 of it); unavailable until kept."""
+    o = _oracle(pd, phi, xB, cap)
+    sim = _gpu(pf, pd, phi, xB, cap, pf.PF_FP64)
+    mixed = _gpu(pf, pd, phi, xB, cap, pf.PF_MIXED)
+    with pytest.raises(pf.PfError, match="pf_keep_stress"):
+        sim.field(pf.PF_SIGMA_XX)
+    sim.keep_stress()
+    mixed.keep_stress()
+    for s in range(2):
+        o.step()
+        sim.step()
+        mixed.step()
+    scale = max(float(np.abs(c).max()) for c in o.sigma)
+    assert scale > 0.0
+    tr = np.zeros(o.shape)
+    for i, which in enumerate(pf.PF_SIGMA):
+        g = sim.field(which)
+        err = float(np.abs(g - o.sigma[i]).max())
+        errm = float(np.abs(mixed.field(which) - o.sigma[i]).max())
+        print(f"{name} sigma[{i}]: fp64 {err / scale:.2e}  mixed {errm / scale:.2e} (of {scale:.3e})", flush=True)
+        assert err <= 1e-10 * scale and errm <= 1e-5 * scale
+        tr += g if i < 3 else 0.0
+    assert np.allclose(tr, sim.field(pf.PF_TRSIGMA), rtol=0, atol=1e-12 * scale)
+    sim.keep_stress(False)
+    sim.step()
+    with pytest.raises(pf.PfError, match="pf_keep_stress"):
+        sim.field(pf.PF_SIGMA_XY)
