@@ -873,6 +873,11 @@ struct Engine final : EngineBase {
     return PF_OK;
   }
 
+  // Grid of the k-space kernels without reductions: one retained mode per
+  // thread (measured faster than a 2048-workgroup grid-stride loop at 256^3;
+  // the kernels with block partials keep grid_for so their sums keep one order)
+  unsigned gmodes() const { return (unsigned)((nScA + NT - 1) / NT); }
+
   KDims kd() const {
     return KDims{ax.A, ay.A, AzS, kx, ky, kz, (Nz % 2) == 0, az.A};
   }
@@ -1146,7 +1151,7 @@ struct Engine final : EngineBase {
       for (int k = 0; k < order; ++k) sa.cf[k] = coef[order - 1][k];
       sa.nh = order;
       for (int i = 0; i < 6; ++i) sa.eps[i] = sc(i);
-      hipLaunchKernelGGL((kop_strain<T, double>), dim3(gk), dim3(NT), 0, stream, sa);
+      hipLaunchKernelGGL((kop_strain<T, double>), dim3(gmodes()), dim3(NT), 0, stream, sa);
     } else {
       typename OpElasticCold<T>::Args oa{P, phi[cur], xB_el()};
       launch_zpass<T, 0, 6, OpElasticCold<T>>(Nz, stream, zargs(0, 0, 0, 6), oa);
@@ -1172,7 +1177,7 @@ struct Engine final : EngineBase {
         sa.n = nSc;
         for (int i = 0; i < 3; ++i) sa.u[i] = Ud + (size_t)i * nSc;
         for (int i = 0; i < 6; ++i) sa.eps[i] = sc(i);
-        hipLaunchKernelGGL((kop_strain<T, double>), dim3(gk), dim3(NT), 0, stream, sa);
+        hipLaunchKernelGGL((kop_strain<T, double>), dim3(gmodes()), dim3(NT), 0, stream, sa);
       }
     }
     // The iteration's decisions (mixing coefficients, history bookkeeping,
@@ -1207,7 +1212,7 @@ struct Engine final : EngineBase {
           for (int i = 0; i < 6; ++i) ua.eps[i] = sc(i);
           ua.ctl = ctl;
           ua.m = m;
-          hipLaunchKernelGGL(kop_aa_update<T>, dim3(gk), dim3(NT), 0, stream, ua);
+          hipLaunchKernelGGL(kop_aa_update<T>, dim3(gmodes()), dim3(NT), 0, stream, ua);
         }
         xpass(true, 0, 6, Sc);
         ypass(true, 0, 6);
@@ -1256,8 +1261,8 @@ struct Engine final : EngineBase {
       last_iters = it;
       return fail(PF_E_NONFINITE, "elastic residual is non-finite at step " + std::to_string(steps_done + 1));
     }
-    hipLaunchKernelGGL(kop_aa_final<T>, dim3(gk), dim3(NT), 0, stream, d, nSc, Ud, Ud + nSc, Ud + 2 * nSc, Gd,
-                       Gd + nSc, Gd + 2 * nSc, sc(0), sc(1), sc(2), sc(3), sc(4), sc(5));
+    hipLaunchKernelGGL(kop_aa_final<T>, dim3(gmodes()), dim3(NT), 0, stream, d, nSc, Ud, Ud + nSc, Ud + 2 * nSc,
+                       Gd, Gd + nSc, Gd + 2 * nSc, sc(0), sc(1), sc(2), sc(3), sc(4), sc(5));
     xpass(true, 0, 6, Sc);
     ypass(true, 0, 6);
     std::swap(hbuf[hist_head], Ud);
@@ -1307,7 +1312,7 @@ struct Engine final : EngineBase {
       sa.n = nSc;
       for (int i = 0; i < 3; ++i) sa.u[i] = Ud + (size_t)i * nSc;
       for (int i = 0; i < 6; ++i) sa.eps[i] = sc(i);
-      hipLaunchKernelGGL((kop_strain<T, double>), dim3(gk), dim3(NT), 0, stream, sa);
+      hipLaunchKernelGGL((kop_strain<T, double>), dim3(gmodes()), dim3(NT), 0, stream, sa);
     };
     if (!warm) {
       typename OpElasticCold<T>::Args oa{P, phi[cur], xB_el()};
@@ -1414,7 +1419,7 @@ struct Engine final : EngineBase {
         sa.n = nSc;
         for (int i = 0; i < 3; ++i) sa.u[i] = uu(u, i);
         for (int i = 0; i < 6; ++i) sa.eps[i] = sc(i);
-        hipLaunchKernelGGL(kop_strain<T>, dim3(gk), dim3(NT), 0, stream, sa);
+        hipLaunchKernelGGL(kop_strain<T>, dim3(gmodes()), dim3(NT), 0, stream, sa);
       }
     } else {
       StrainArgs<T> sa{};
@@ -1422,7 +1427,7 @@ struct Engine final : EngineBase {
       sa.n = nSc;
       for (int i = 0; i < 3; ++i) sa.u[i] = uu(u, i);
       for (int i = 0; i < 6; ++i) sa.eps[i] = sc(i);
-      hipLaunchKernelGGL(kop_strain<T>, dim3(gk), dim3(NT), 0, stream, sa);
+      hipLaunchKernelGGL(kop_strain<T>, dim3(gmodes()), dim3(NT), 0, stream, sa);
     }
     xpass(true, 0, 6, Sc);
     ypass(true, 0, 6);
@@ -1524,7 +1529,8 @@ struct Engine final : EngineBase {
     // phi semi-implicit update
     ypass(false, 0, 1);
     xpass(false, 0, 1, Sc);
-    hipLaunchKernelGGL(kop_phi<T>, dim3(gk), dim3(NT), 0, stream, d, nSc, sc(0), prm.L_phi * prm.dt * prm.kappa_phi);
+    hipLaunchKernelGGL(kop_phi<T>, dim3(gmodes()), dim3(NT), 0, stream, d, nSc, sc(0),
+                       prm.L_phi * prm.dt * prm.kappa_phi);
     xpass(true, 0, 1, Sc);
     ypass(true, 0, 1);
     {
@@ -1536,7 +1542,7 @@ struct Engine final : EngineBase {
     // Y: spectral gradient of mu, fluxes, update
     ypass(false, 1, 1);
     xpass(false, 1, 1, sc(1));
-    hipLaunchKernelGGL(kop_grad<T>, dim3(gk), dim3(NT), 0, stream, d, nSc, sc(1), sc(2), sc(3), sc(4));
+    hipLaunchKernelGGL(kop_grad<T>, dim3(gmodes()), dim3(NT), 0, stream, d, nSc, sc(1), sc(2), sc(3), sc(4));
     xpass(true, 2, 3, sc(2));
     ypass(true, 2, 3);
     {
@@ -1548,7 +1554,7 @@ struct Engine final : EngineBase {
     }
     ypass(false, 0, 5);
     xpass(false, 0, 5, Sc);
-    hipLaunchKernelGGL(kop_yupdate<T>, dim3(gk), dim3(NT), 0, stream, d, nSc, sc(0), sc(1), sc(2), sc(3),
+    hipLaunchKernelGGL(kop_yupdate<T>, dim3(gmodes()), dim3(NT), 0, stream, d, nSc, sc(0), sc(1), sc(2), sc(3),
                        sc(4), sc(5), (const double*)(dscal + 4), prm.dt);
     xpass(true, 5, 1, sc(5));
     ypass(true, 5, 1);
@@ -2479,7 +2485,7 @@ struct Engine final : EngineBase {
         for (int i = 0; i < 6; ++i) ua.eps[i] = sc(i);
         ua.ctl = dscal + AA_SLOT;
         ua.m = m;
-        hipLaunchKernelGGL(kop_aa_update<T>, dim3(gk), dim3(NT), 0, stream, ua);
+        hipLaunchKernelGGL(kop_aa_update<T>, dim3(gmodes()), dim3(NT), 0, stream, ua);
         // g read; the m history slots read; du[slot] and the 6 strains written
         return (3.0 * m + 9.0) * nScA * cs + 3.0 * nScA * 16.0;
       }
