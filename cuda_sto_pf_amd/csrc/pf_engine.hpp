@@ -277,6 +277,8 @@ struct ZmConst {
   int max_it;
 };
 
+constexpr int ZM_U = 4;   // grid strides per iteration of the opening zero-mode evaluation
+
 __device__ __forceinline__ void zm_terms(double ys, double ph, double lam, double vB, double ylo, double yhi,
                                          double* acc) {
   const double h = h_of_phi(ph);
@@ -295,11 +297,24 @@ __global__ void __launch_bounds__(NT) zm_eval3_kernel(const double* Ystar, const
   const double hi = Y_clip - ymm[1];
   const double lam0 = fmin(fmax(0.0, lo), hi);
   double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (long g = blockIdx.x * (long)NT + threadIdx.x; g < n; g += (long)gridDim.x * NT) {
-    const double ys = Ystar[g], ph = phi[g];
-    zm_terms(ys, ph, lo, vB, -Y_clip, Y_clip, acc);
-    zm_terms(ys, ph, hi, vB, -Y_clip, Y_clip, acc + 3);
-    zm_terms(ys, ph, lam0, vB, -Y_clip, Y_clip, acc + 6);
+  // ZM_U grid strides' loads issued together; the terms are added in the
+  // order of the plain grid-stride loop
+  const long stride = (long)gridDim.x * NT;
+  for (long g0 = blockIdx.x * (long)NT + threadIdx.x; g0 < n; g0 += ZM_U * stride) {
+    double ys[ZM_U], ph[ZM_U];
+#pragma unroll
+    for (int u = 0; u < ZM_U; ++u) {
+      const long g = g0 + u * stride;
+      ys[u] = g < n ? Ystar[g] : 0.0;
+      ph[u] = g < n ? phi[g] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < ZM_U; ++u) {
+      if (g0 + u * stride >= n) break;
+      zm_terms(ys[u], ph[u], lo, vB, -Y_clip, Y_clip, acc);
+      zm_terms(ys[u], ph[u], hi, vB, -Y_clip, Y_clip, acc + 3);
+      zm_terms(ys[u], ph[u], lam0, vB, -Y_clip, Y_clip, acc + 6);
+    }
   }
   int kinds[9];
 #pragma unroll
@@ -1639,7 +1654,8 @@ struct Engine final : EngineBase {
     zm_res = r;
     zm_deriv = hs[ZS_DERIV];
     ++zm_accepted;
-    hipLaunchKernelGGL(zm_apply_kernel, dim3(g), dim3(NT), 0, stream, Ystar, lam, Y, xB, dYdt, prm.dt, R);
+    hipLaunchKernelGGL(zm_apply_kernel, dim3((unsigned)((R + NT - 1) / NT)), dim3(NT), 0, stream, Ystar, lam, Y,
+                       xB, dYdt, prm.dt, R);
     return PF_OK;
   }
 
