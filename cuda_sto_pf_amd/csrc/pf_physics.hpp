@@ -126,9 +126,16 @@ PF_HD double gamma_thermo(const PhysParams& P, double xB, double h) {
   const double dx = 1e-5;
   double xp = clamp_fraction_eps(xc + dx);
   double xm = clamp_fraction_eps(xc - dx);
-  double dmuA = (mu_A(P, xp) - mu_A(P, xm)) / (xp - xm);
-  double dmuB = (mu_B(P, xp) - mu_B(P, xm)) / (xp - xm);
-  return c_bulk * (dmuB - dmuA);
+  // The reference's difference quotient of mu_B - mu_A, evaluated in closed
+  // form: both points lie on the CALPHAD branch (below the convex
+  // continuation at 0.09), G(T) cancels and the logarithm differences combine,
+  //   [RT (ln(xp/xm) - ln((1-xp)/(1-xm))) + L ((1-xp)^2 - (1-xm)^2 - xp^2 + xm^2)] / d
+  //     = RT log1p(d / (xm (1 - xp))) / d - 2 L,   d = xp - xm.
+  // One logarithm instead of four, without the cancellation of the literal
+  // differences (whose rounding noise is ~1e-12 of Gamma).
+  const double d = xp - xm;
+  const double g = P.RT * log1p(d / (xm * (1.0 - xp))) / d - 2.0 * P.Lparam;
+  return c_bulk * (g / P.mu_scale);
 }
 // cuda_kernels.cu:30-38
 PF_HD double stabilized_meff(double Dm, double gamma) {

@@ -73,6 +73,29 @@ def test_gamma_fd_matches_analytic_derivative():
     assert np.all(np.abs(G - ana) <= 1.05 * bound + 1e-9 * np.abs(ana))
 
 
+def test_gamma_closed_form_equals_the_difference_quotient():
+    """The device evaluates the reference's difference quotient of mu_B - mu_A
+    in closed form, c_bulk (RT log1p(d / (xm (1 - xp))) / d - 2 L) / scale
+    with d = xp - xm (G(T) cancels, the logarithm differences combine).  It
+    must equal the literal quotient to that form's own rounding noise, over
+    the whole clamped range, including the clamped ends."""
+    T = 653.15
+    scale = 20668.536
+    R = th.R_GAS
+    L = th.get_L_param(T)
+    x = np.concatenate([np.logspace(-13, -1, 60), np.linspace(0.1, 0.9, 41), 1.0 - np.logspace(-1, -13, 60)])
+    h = np.linspace(0.0, 1.0, x.size)
+    lit = th.gamma_thermo_nonlinear(x, h, 1.0, 0.1, 1.2, T, scale)
+    c_bulk = th.c_xB_phi(x, 1.0, 0.1, 1.2, h)
+    xp = th.clamp_fraction_eps(x + 1e-5)
+    xm = th.clamp_fraction_eps(x - 1e-5)
+    d = xp - xm
+    closed = c_bulk * ((R * T) * np.log1p(d / (xm * (1.0 - xp))) / d - 2.0 * L) / scale
+    # literal form: mu values O(|G|/scale) differenced over d ~ 2e-5
+    noise = 2 * np.finfo(float).eps * (abs(th.G_Ag2Te_Solid(T)) + abs(th.G_PbTe_Solid(T)) + 60 * R * T) / scale / d
+    assert np.all(np.abs(closed - lit) <= noise * c_bulk + 1e-13 * np.abs(lit))
+
+
 def test_x_eq_solves_the_tangent_condition():
     T = 653.15
     x = th.solve_x_eq(T)
