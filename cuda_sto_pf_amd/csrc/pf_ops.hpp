@@ -253,11 +253,20 @@ struct OpPhiFinish {
     const double xB = sc < P.xB_eps ? P.xB_eps : (sc > 1.0 - P.xB_eps ? 1.0 - P.xB_eps : sc);
     a.xB[g] = xB;
     const double h = h_of_phi(phiY);
-    const double muA = mu_A(P, xB);
-    const double muB = mu_B(P, xB);
     const double c_bulk = c_xB_phi(P, xB, h);
-    const double mu_total = mu_tot_mix(muA, muB, xB, P.mu0, h);
-    double mu = c_bulk * (muB - muA - c_bulk * mu_total * P.dVm);
+    double mu;
+    if (P.dVm == 0.0 && (!P.convex || xB <= 0.09)) {
+      // the volume term vanishes and both potentials are on the CALPHAD
+      // branch: mu_B - mu_A with G(T) differenced once and the logarithms
+      // combined, (G_B0 - G_A0 + RT ln(x/(1-x)) + L (1-2x)) / scale
+      const double x = clamp_fraction_eps(xB);
+      mu = c_bulk * ((P.G_B0 - P.G_A0 + P.RT * pf_log(x / (1.0 - x)) + P.Lparam * (1.0 - 2.0 * x)) / P.mu_scale);
+    } else {
+      const double muA = mu_A(P, xB);
+      const double muB = mu_B(P, xB);
+      const double mu_total = mu_tot_mix(muA, muB, xB, P.mu0, h);
+      mu = c_bulk * (muB - muA - c_bulk * mu_total * P.dVm);
+    }
     if (P.elastic) mu += -P.eps_iso * pre[2];
     out[0] = T(mu);
     double x = (Yc == Y) ? sc : sigmoid_unclipped(Y);

@@ -96,6 +96,23 @@ def test_gamma_closed_form_equals_the_difference_quotient():
     assert np.all(np.abs(closed - lit) <= noise * c_bulk + 1e-13 * np.abs(lit))
 
 
+def test_potential_difference_closed_form():
+    """The phi finish evaluates mu_B - mu_A on the CALPHAD branch as
+    (G_B - G_A + RT ln(x/(1-x)) + L (1 - 2x)) / scale (one logarithm).  It
+    must equal the difference of the oracle's two potentials to their
+    rounding (a few ulps of |G|/scale), over the clamped range."""
+    T = 653.15
+    scale = 20668.536
+    x = np.concatenate([np.logspace(-12, -1, 50), np.linspace(0.1, 0.9, 41), 1.0 - np.logspace(-1, -12, 50)])
+    lit = th.mu_B_dimless(x, T, scale) - th.mu_A_dimless(x, T, scale)
+    xc = th.clamp_fraction_eps(x)
+    closed = (th.G_Ag2Te_Solid(T) - th.G_PbTe_Solid(T) + th.R_GAS * T * np.log(xc / (1.0 - xc))
+              + th.get_L_param(T) * (1.0 - 2.0 * xc)) / scale
+    tol = 8 * np.finfo(float).eps * (abs(th.G_Ag2Te_Solid(T)) + abs(th.G_PbTe_Solid(T))
+                                     + th.R_GAS * T * 28.0) / scale
+    assert np.all(np.abs(closed - lit) <= tol)
+
+
 def test_x_eq_solves_the_tangent_condition():
     T = 653.15
     x = th.solve_x_eq(T)
