@@ -187,3 +187,22 @@ def test_front_end_to_run(pf, tmp_path):
     assert len(runs) == 1 and runs[0].startswith("chel_T380_cuda_16x16x16_dt0.02_steps3_xB0.030")
     rows = (tmp_path / "Results" / runs[0] / "uniform" / "vf_precip_vs_time_uniform.csv").read_text().splitlines()
     assert len(rows) == 1 + 4 and all(float(r.split(",")[3]) == 0.0 for r in rows[1:])
+
+
+@pytest.mark.gpu
+def test_cli_nonconvergence_is_fatal(pf, tmp_path):
+    """A step whose elastic solve stops at elastic_iter_max above tolerance
+    ends the run with [fatal] and status 2, as the reference does."""
+    args, *_ = _bundle(tmp_path, steps=2)
+    prm = args[args.index("--pf-param-file") + 1]
+    lines = []
+    for ln in open(prm):
+        if ln.startswith("elastic_iter_max"):
+            ln = "elastic_iter_max = 3\n"
+        elif ln.startswith("elastic_residual_tolerance"):
+            ln = "elastic_residual_tolerance = 1e-15\n"
+        lines.append(ln)
+    open(prm, "w").writelines(lines)
+    r = _run(args, tmp_path, {"CUDA_STO_SUPPRESS_VTK_OUTPUT": "1"})
+    print(r.stdout[-800:], r.stderr[-800:], flush=True)
+    assert r.returncode == 2 and "[fatal]" in r.stderr and "did not converge" in r.stderr

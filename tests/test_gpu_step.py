@@ -241,3 +241,26 @@ def test_stress_components(pf, name, pd, phi, xB, cap):
     sim.step()
     with pytest.raises(pf.PfError, match="pf_keep_stress"):
         sim.field(pf.PF_SIGMA_XY)
+
+
+SOLVERS = [("fp64_reference", "PF_FP64", {}), ("mixed", "PF_MIXED", {}),
+           ("fp64_accelerated", "PF_FP64", {"elastic_warm_start_order": 3, "elastic_anderson_depth": 3})]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,prec,extra", SOLVERS, ids=[c[0] for c in SOLVERS])
+def test_elastic_nonconvergence_is_fail_closed(pf, name, prec, extra):
+    """An elastic solve that reaches elastic_iter_max above tolerance fails the
+    step with PF_E_NONCONVERGED when elastic_fail_on_nonconvergence is set
+    (main_cuda.cu:16273-16279), and is accepted at the cap otherwise."""
+    pd, phi, xB, meta = config(2, 16)
+    pd = dict(pd, elastic_iter_max=3, elastic_residual_tolerance=1e-15, **extra)
+    s = _gpu(pf, dict(pd, elastic_fail_on_nonconvergence=1), phi, xB, meta["xB_max_safe"], getattr(pf, prec))
+    with pytest.raises(pf.PfError, match="did not converge") as ei:
+        s.step()
+    assert ei.value.code == -4
+    t = _gpu(pf, dict(pd, elastic_fail_on_nonconvergence=0), phi, xB, meta["xB_max_safe"], getattr(pf, prec))
+    t.step(2)
+    sc = t.scalars()
+    assert sc["last_elastic_iterations"] == 3 and sc["last_elastic_residual"] > 1e-15
+    assert np.isfinite(t.field(pf.PF_XB)).all()
