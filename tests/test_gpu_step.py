@@ -264,3 +264,25 @@ def test_elastic_nonconvergence_is_fail_closed(pf, name, prec, extra):
     sc = t.scalars()
     assert sc["last_elastic_iterations"] == 3 and sc["last_elastic_residual"] > 1e-15
     assert np.isfinite(t.field(pf.PF_XB)).all()
+
+
+@pytest.mark.gpu
+def test_nonfinite_initial_fields_are_refused(pf):
+    """NaN/Inf in phi, x_B or dY/dt is refused with PF_E_NONFINITE before any
+    device state changes (load_raw_init_fields, main_cuda.cu:2905-2965); the
+    handle then accepts valid fields."""
+    pd, phi, xB, meta = config(2, 16)
+    s = pf.Simulation(pd, pf.PF_FP64)
+    for bad_phi, bad_xB in ((True, False), (False, True)):
+        p = phi.copy()
+        x = xB.copy()
+        if bad_phi:
+            p[3, 4, 5] = np.nan
+        if bad_xB:
+            x[0, 0, 1] = np.inf
+        with pytest.raises(pf.PfError, match="NaN/Inf") as ei:
+            s.set_fields(p, x, xB_cap=meta["xB_max_safe"])
+        assert ei.value.code == -6
+    s.set_fields(phi, xB, xB_cap=meta["xB_max_safe"])
+    s.step()
+    assert np.isfinite(s.field(pf.PF_XB)).all()
