@@ -28,9 +28,10 @@ Pinning: the FFT is checked against direct DFT sums; dealias against brute
 force; the semi-implicit updates and the elastic Green operator against closed
 forms (single Fourier mode, affine fixed point, homogeneous one-step
 convergence, div(sigma)=0); the zero mode against bisection; the thermo
-against analytic derivatives.  The whole lagged-RHS step has no reference
-golden vector (the reference's own tests pin none of the step arithmetic), so
-the composed step is "parity unpinned" beyond those component checks.
+against analytic derivatives.  The reference's own tests pin none of the
+step arithmetic, so the composed lagged-RHS composition update is pinned by
+the closed linear recurrence it reduces to for a small single Fourier mode on
+a uniform matrix (tests/test_oracle_pins.py), on top of the component checks.
 """
 import math
 from dataclasses import dataclass, field

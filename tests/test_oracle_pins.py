@@ -285,3 +285,46 @@ def test_mass_drifts_without_zero_mode():
     m0 = o.mass()
     o.step(3)
     assert abs(o.mass() - m0) > 1e-12 * m0
+
+
+def test_composed_y_update_single_mode_recurrence():
+    """The whole lagged-RHS composition update (Oracle.Y_update, main_cuda.cu:
+    30283-30806: mu_x, the flux M_eff grad mu with the FD Gamma, its
+    divergence, the D-bar Laplacian stabiliser, the lagged gamma_local dY/dt
+    term, the semi-implicit solve and the dY/dt history) reduces, for a
+    small single Fourier mode of Y on a uniform matrix (phi = 0, no
+    elasticity, no zero mode), to a closed linear recurrence: M_eff Gamma = D
+    makes div J = D q lap Y with q = x(1-x), which the stabiliser cancels, so
+        a_1 (1 + dt Db k^2) = a_0,
+        a_{n+1} (1 + dt Db k^2) = a_n + (1 - q)(a_n - a_{n-1}),   Db = D q.
+    dt = 2 makes dt Db k^2 = 0.14, so a dropped stabiliser, a wrong sign or
+    level of the lagged term, or a wrong time step fails by percents."""
+    from cuda_sto_pf_amd.configs import base_params
+    shape = (16, 4, 4)
+    pd = base_params(shape, dt=2.0, hetero=False, zero_mode=False, elastic=False)
+    p = Params(**pd)
+    x0, eps, m = 0.03, 1e-7, 4
+    Y0 = math.log(x0 / (1.0 - x0))
+    cosx = np.cos(2.0 * np.pi * m * np.arange(shape[0]) / shape[0])
+    Y = Y0 + eps * np.broadcast_to(cosx[:, None, None], shape)
+    o = Oracle(p, np.zeros(shape), th.sigmoid_unclipped(Y))
+
+    def amp(f):
+        return 2.0 * float(np.mean(f * cosx[:, None, None]))
+
+    a = [amp(o.Y)]
+    for _ in range(6):
+        o.step()
+        a.append(amp(o.Y))
+        assert np.abs(o.phi).max() == 0.0
+    q = x0 * (1.0 - x0)
+    k2 = (2.0 * np.pi * m / shape[0]) ** 2
+    den = 1.0 + p.dt * p.D_alpha * q * k2
+    assert 0.1 < p.dt * p.D_alpha * q * k2 < 0.2
+    b = [a[0], a[0] / den]
+    for n in range(1, 6):
+        b.append((b[n] + (1.0 - q) * (b[n] - b[n - 1])) / den)
+    for n in range(7):
+        assert abs(a[n] - b[n]) <= 1e-6 * abs(a[0]), (n, a[n], b[n])
+    # the lagged term matters here: without it the third amplitude differs by ~1e-2
+    assert abs(b[2] - b[1] / den) > 1e-3 * abs(a[0])
