@@ -286,3 +286,35 @@ def test_nonfinite_initial_fields_are_refused(pf):
     s.set_fields(phi, xB, xB_cap=meta["xB_max_safe"])
     s.step()
     assert np.isfinite(s.field(pf.PF_XB)).all()
+
+
+@pytest.mark.gpu
+def test_y_update_single_mode_recurrence(pf):
+    """The HIP composition update on a small single Fourier mode of Y over a
+    uniform matrix follows the closed recurrence of the lagged-RHS scheme,
+    a_{n+1} (1 + dt Db k^2) = a_n + (1 - q)(a_n - a_{n-1}), Db = D q,
+    q = x(1-x) (FP64 path; the oracle test states the derivation)."""
+    shape = (16, 4, 4)
+    pd = base_params(shape, dt=2.0, hetero=False, zero_mode=False, elastic=False)
+    x0, eps, m = 0.03, 1e-7, 4
+    Y0 = math.log(x0 / (1.0 - x0))
+    cosx = np.cos(2.0 * np.pi * m * np.arange(shape[0]) / shape[0])
+    Y = Y0 + eps * np.broadcast_to(cosx[:, None, None], shape)
+    s = pf.Simulation(pd, pf.PF_FP64)
+    s.set_fields(np.zeros(shape), 1.0 / (1.0 + np.exp(-Y)))
+
+    def amp(f):
+        return 2.0 * float(np.mean(f * cosx[:, None, None]))
+
+    a = [amp(s.field(pf.PF_Y))]
+    for _ in range(6):
+        s.step()
+        a.append(amp(s.field(pf.PF_Y)))
+    q = x0 * (1.0 - x0)
+    den = 1.0 + pd["dt"] * pd["D_alpha"] * q * (2.0 * np.pi * m / shape[0]) ** 2
+    b = [a[0], a[0] / den]
+    for n in range(1, 6):
+        b.append((b[n] + (1.0 - q) * (b[n] - b[n - 1])) / den)
+    err = max(abs(a[n] - b[n]) for n in range(7)) / abs(a[0])
+    print(f"single-mode Y recurrence: max deviation {err:.2e} of the amplitude", flush=True)
+    assert err <= 1e-6
