@@ -725,10 +725,10 @@ __global__ void __launch_bounds__(NT) kop_diff3(long n, const S* u0, const S* u1
 }
 
 // ------------------------------------------------------------ dispatch
-template <typename T, int N, int NCI, int NCO, int NPRE>
+template <typename T, int N, int NCI, int NCO, int NPRE, int BUDGET = ZPASS_LDS_BUDGET>
 constexpr int zpass_rows_for() {
   constexpr int NCM = NCI > NCO ? NCI : NCO;
-  return zpass_rows<T, N, (NCM > 0 ? NCM : 1), NPRE>();
+  return zpass_rows<T, N, (NCM > 0 ? NCM : 1), NPRE, BUDGET>();
 }
 
 // z-pass implementation: the LDS-staged kernel (default), or the
@@ -742,14 +742,15 @@ inline bool zpass_use_reg() {
   return use;
 }
 
-template <typename T, int NCI, int NCO, int NPRE>
+template <typename T, int NCI, int NCO, int NPRE, int BUDGET = ZPASS_LDS_BUDGET>
 int zpass_grid(int N, int rows) {
   switch (N) {
 #define PF_CASE_ZG(v)                                                                  \
   case v:                                                                              \
     if constexpr (RegPlan<v>::ok)                                                      \
       if (zpass_use_reg()) return (rows + 64 / RegPlan<v>::L - 1) / (64 / RegPlan<v>::L); \
-    return (rows + zpass_rows_for<T, v, NCI, NCO, NPRE>() - 1) / zpass_rows_for<T, v, NCI, NCO, NPRE>();
+    return (rows + zpass_rows_for<T, v, NCI, NCO, NPRE, BUDGET>() - 1) /                        \
+           zpass_rows_for<T, v, NCI, NCO, NPRE, BUDGET>();
     PF_SIZES(PF_CASE_ZG)
 #undef PF_CASE_ZG
     default:
@@ -761,7 +762,7 @@ int zpass_grid(int N, int rows) {
 // reductions wrote).
 template <typename T, int NCI, int NCO, class Op>
 int launch_zpass(int N, hipStream_t s, const ZPassArgs<T>& za, const typename Op::Args& oa) {
-  const int grid = zpass_grid<T, NCI, NCO, Op::NPRE>(N, za.rows);
+  const int grid = zpass_grid<T, NCI, NCO, Op::NPRE, ZBudget<Op>::value>(N, za.rows);
   switch (N) {
 #define PF_CASE_Z(v)                                                                                \
   case v:                                                                                           \

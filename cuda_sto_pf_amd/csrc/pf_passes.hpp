@@ -18,6 +18,8 @@
 // Compact index c of a wrapped axis maps to the full index c (c < a) or
 // c + N - A (c >= a), A = 2a - 1.
 #pragma once
+#include <type_traits>
+
 #include "pf_fft.hpp"
 
 namespace pf {
@@ -36,10 +38,10 @@ constexpr int ZPASS_PREFETCH_DOUBLES = 48;
 // prefetch budget at that tile gets a smaller tile where they fit in half of
 // it (so their loads overlap the inverse transform instead of being issued
 // a chunk at a time after it).
-template <typename T, int N, int NC, int NPRE = 0>
+template <typename T, int N, int NC, int NPRE = 0, int BUDGET = ZPASS_LDS_BUDGET>
 constexpr int zpass_rows() {
   constexpr int per_pair = NC * (N + N / 16) * (int)sizeof(cx<T>) * (plan_has_big_prime(N) ? 2 : 1);
-  constexpr int pairs = ZPASS_LDS_BUDGET / per_pair;
+  constexpr int pairs = BUDGET / per_pair;
   int p = pairs < 1 ? 1 : (pairs > 32 ? 32 : pairs);
   if (NPRE > 0 && ((2 * p * N + NT - 1) / NT) * NPRE > ZPASS_PREFETCH_DOUBLES)
     while (p > 1 && ((2 * p * N + NT - 1) / NT) * NPRE > ZPASS_PREFETCH_DOUBLES / 2) --p;
@@ -115,12 +117,22 @@ struct ZPassArgs {
 //     may read/write real fields and accumulate reductions.
 //  3. NCO>0: forward complex FFT of row pairs, split into two real-row
 //     spectra, store the Az retained kz.
+// An operator may set its own LDS budget (static constexpr int LDS_BUDGET).
+template <class Op, class = void>
+struct ZBudget {
+  static constexpr int value = ZPASS_LDS_BUDGET;
+};
+template <class Op>
+struct ZBudget<Op, std::void_t<decltype(Op::LDS_BUDGET)>> {
+  static constexpr int value = Op::LDS_BUDGET;
+};
+
 template <typename T, int N, int NCI, int NCO, class Op>
 __global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op::Args oa) {
   if (za.gate && *za.gate == 0.0) return;
   constexpr int NCM = NCI > NCO ? NCI : NCO;
   constexpr int NCB = NCM > 0 ? NCM : 1;
-  constexpr int RT = zpass_rows<T, N, NCB, Op::NPRE>();
+  constexpr int RT = zpass_rows<T, N, NCB, Op::NPRE, ZBudget<Op>::value>();
   constexpr int NP = RT / 2;
   constexpr bool BIG = plan_has_big_prime(N);
   using LY = Lds<N, NCB * NP, false>;   // row-major: layout independent of the batch
