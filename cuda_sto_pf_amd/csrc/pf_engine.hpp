@@ -850,6 +850,12 @@ struct Engine final : EngineBase {
   double zm_res = 0.0, zm_deriv = 0.0;
   unsigned long long zm_accepted = 0;
   double e_el = 0.0, e_dw = 0.0, e_chem = 0.0, e_grad = 0.0;
+  // The phi right-hand-side pass leaves its energy sums as block partials in
+  // epartials; they are summed when the scalars are read (nothing in the step
+  // uses them), eng_g blocks of 3 values.
+  double* epartials = nullptr;
+  int eng_g = 0;
+  bool eng_pending = false;
   bool egrad_pending = false;   // e_grad of the last step not evaluated yet
   T* sig6 = nullptr;            // stress components of the last step (keep_stress)
   bool keep_sig6 = false, sig6_valid = false;
@@ -984,7 +990,9 @@ struct Engine final : EngineBase {
     const int rows = Nx * Ny;
     maxg = std::max<long>(maxg, (rows + 1) / 2);
     npartials = (size_t)maxg * 16;   // up to 16 reductions per block (Anderson dots use AA_NRED)
-    if ((rc = dalloc(&partials, npartials)) || (rc = dalloc(&dscal, 256))) return rc;
+    if ((rc = dalloc(&partials, npartials)) || (rc = dalloc(&dscal, 256)) ||
+        (rc = dalloc(&epartials, (size_t)maxg * 3)))
+      return rc;
     PF_HIP(hipHostMalloc((void**)&hscal, 256 * sizeof(double), hipHostMallocDefault));
     return PF_OK;
   }
@@ -1100,6 +1108,7 @@ struct Engine final : EngineBase {
     if (rc) return rc;
     target_mass = m;
     egrad_pending = false;
+    eng_pending = false;
     sig6_valid = false;
     e_grad = 0.0;
     zm_lambda = zm_res = zm_deriv = 0.0;
@@ -1529,16 +1538,19 @@ struct Engine final : EngineBase {
     // the other ping-pong buffer until the next step (egrad_pending, set
     // when the step is accepted).
     egrad_pending = false;
+    eng_pending = false;
     sig6_valid = false;
     if (P.elastic) {
       if ((rc = elastic_solve())) return rc;
       typename OpPhiRhs<T, true>::Args oa{P, phi_old, xB, trs, invN, keep_sig6 ? sig6 : nullptr, (size_t)R};
-      const int g = launch_zpass<T, 6, 1, OpPhiRhs<T, true>>(Nz, stream, zargs(0, 6, 0, 1), oa);
-      finalize(g, 3, RedKinds{{RED_SUM, RED_SUM, RED_SUM, 0}}, dscal + 9, 1.0 / (double)R);
+      ZPassArgs<T> za = zargs(0, 6, 0, 1);
+      za.partials = epartials;
+      eng_g = launch_zpass<T, 6, 1, OpPhiRhs<T, true>>(Nz, stream, za, oa);
     } else {
       typename OpPhiRhs<T, false>::Args oa{P, phi_old, xB, trs, invN};
-      const int g = launch_zpass<T, 0, 1, OpPhiRhs<T, false>>(Nz, stream, zargs(0, 0, 0, 1), oa);
-      finalize(g, 3, RedKinds{{RED_SUM, RED_SUM, RED_SUM, 0}}, dscal + 9, 1.0 / (double)R);
+      ZPassArgs<T> za = zargs(0, 0, 0, 1);
+      za.partials = epartials;
+      eng_g = launch_zpass<T, 0, 1, OpPhiRhs<T, false>>(Nz, stream, za, oa);
     }
     const KDims d = kd();
     const int gk = grid_for(nSc);
@@ -1581,21 +1593,16 @@ struct Engine final : EngineBase {
       // the step's scalars come back with the zero-mode search state
       rc = zero_mode(phi_new, 13);
       mean_DY = hscal[4];
-      e_el = hscal[9];
-      e_dw = hscal[10];
-      e_chem = hscal[11];
       if (rc) return rc;
     } else {
       typename OpYFinal<T>::Args oa{P, Y, xB, dYdt, invN, P.Y_clip};
       launch_zpass<T, 1, 0, OpYFinal<T>>(Nz, stream, zargs(5, 1, 0, 0), oa);
       if ((rc = fetch(13))) return rc;
       mean_DY = hscal[4];
-      e_el = hscal[9];
-      e_dw = hscal[10];
-      e_chem = hscal[11];
     }
     cur = 1 - cur;
     egrad_pending = true;
+    eng_pending = true;
     sig6_valid = keep_sig6 && P.elastic;
     ++steps_done;
     return PF_OK;
@@ -1971,6 +1978,7 @@ struct Engine final : EngineBase {
     e_dw = dw_sum / (double)R;
     e_el = gel_sum / (double)R;
     e_chem = 0.0;
+    eng_pending = false;
     ++steps_done;
     if (st) {
       st->iteration = m_iter;
@@ -2104,6 +2112,16 @@ struct Engine final : EngineBase {
       PF_HIP(hipStreamSynchronize(stream));
       e_grad = hscal[8];
       egrad_pending = false;
+    }
+    if (eng_pending) {
+      hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(NT), 0, stream, (const double*)epartials, eng_g, 3,
+                         RedKinds{{RED_SUM, RED_SUM, RED_SUM, 0}}, dscal + 9, 1.0 / (double)R);
+      PF_HIP(hipMemcpyAsync(hscal + 9, dscal + 9, 3 * sizeof(double), hipMemcpyDeviceToHost, stream));
+      PF_HIP(hipStreamSynchronize(stream));
+      e_el = hscal[9];
+      e_dw = hscal[10];
+      e_chem = hscal[11];
+      eng_pending = false;
     }
     double m, vf, nin, nif;
     int rc = mass_vf(&m, &vf, &nin, &nif);
@@ -2311,6 +2329,7 @@ struct Engine final : EngineBase {
     steps_done = (long long)c.accepted_step;
     elastic_total = 0;
     egrad_pending = false;
+    eng_pending = false;
     sig6_valid = false;
     minimizing = false;
     warm_valid = false;
