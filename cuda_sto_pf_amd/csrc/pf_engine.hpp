@@ -462,34 +462,68 @@ __device__ void aa_coef(double* ctl) {
   const int nh = (int)ctl[AC_NH];
   double gam[AA_MAX] = {0.0, 0.0, 0.0, 0.0};
   if (nh > 0) {
-    double A[AA_MAX][AA_MAX + 1];
+    // Every index is a compile-time constant (loops over AA_MAX with nh
+    // guards, the pivot row found by value), so the system stays in registers
+    // instead of private memory; the operations are those of the plain
+    // elimination over nh rows, in the same order.
+    double A[AA_MAX][AA_MAX], b[AA_MAX];
     double tr = 0.0;
-    for (int i = 0; i < nh; ++i) tr += ctl[AC_H + i * AA_MAX + i];
-    for (int i = 0; i < nh; ++i) {
-      for (int j = 0; j < nh; ++j) A[i][j] = ctl[AC_H + i * AA_MAX + j] + (i == j ? 1e-12 * tr / nh : 0.0);
-      A[i][nh] = ctl[AC_B + i];
+#pragma unroll
+    for (int i = 0; i < AA_MAX; ++i)
+      if (i < nh) tr += ctl[AC_H + i * AA_MAX + i];
+#pragma unroll
+    for (int i = 0; i < AA_MAX; ++i) {
+#pragma unroll
+      for (int j = 0; j < AA_MAX; ++j)
+        A[i][j] = (i < nh && j < nh) ? ctl[AC_H + i * AA_MAX + j] + (i == j ? 1e-12 * tr / nh : 0.0) : 0.0;
+      b[i] = i < nh ? ctl[AC_B + i] : 0.0;
     }
-    for (int col = 0; col < nh; ++col) {
-      int piv = col;
-      for (int r = col + 1; r < nh; ++r)
-        if (fabs(A[r][col]) > fabs(A[piv][col])) piv = r;
-      if (piv != col)
-        for (int q = 0; q <= nh; ++q) {
-          const double t = A[col][q];
-          A[col][q] = A[piv][q];
-          A[piv][q] = t;
+#pragma unroll
+    for (int col = 0; col < AA_MAX; ++col) {
+      if (col < nh) {
+        int piv = col;
+        double pv = fabs(A[col][col]);
+#pragma unroll
+        for (int r = col + 1; r < AA_MAX; ++r)
+          if (r < nh && fabs(A[r][col]) > pv) {
+            piv = r;
+            pv = fabs(A[r][col]);
+          }
+#pragma unroll
+        for (int r = col + 1; r < AA_MAX; ++r)
+          if (r == piv) {
+#pragma unroll
+            for (int q = 0; q < AA_MAX; ++q) {
+              const double t = A[col][q];
+              A[col][q] = A[r][q];
+              A[r][q] = t;
+            }
+            const double t = b[col];
+            b[col] = b[r];
+            b[r] = t;
+          }
+        if (A[col][col] != 0.0) {
+#pragma unroll
+          for (int r = col + 1; r < AA_MAX; ++r)
+            if (r < nh) {
+              const double f = A[r][col] / A[col][col];
+#pragma unroll
+              for (int q = col; q < AA_MAX; ++q)
+                if (q < nh) A[r][q] -= f * A[col][q];
+              b[r] -= f * b[col];
+            }
         }
-      if (A[col][col] == 0.0) continue;
-      for (int r = col + 1; r < nh; ++r) {
-        const double f = A[r][col] / A[col][col];
-        for (int q = col; q <= nh; ++q) A[r][q] -= f * A[col][q];
       }
     }
-    for (int i = nh - 1; i >= 0; --i) {
-      double v = A[i][nh];
-      for (int j = i + 1; j < nh; ++j) v -= A[i][j] * gam[j];
-      gam[i] = (A[i][i] != 0.0) ? v / A[i][i] : 0.0;
-    }
+#pragma unroll
+    for (int i = AA_MAX - 1; i >= 0; --i)
+      if (i < nh) {
+        double v = b[i];
+#pragma unroll
+        for (int j = i + 1; j < AA_MAX; ++j)
+          if (j < nh) v -= A[i][j] * gam[j];
+        gam[i] = (A[i][i] != 0.0) ? v / A[i][i] : 0.0;
+      }
   }
   for (int j = 0; j < AA_MAX; ++j) ctl[AC_GAM + j] = gam[j];
 }
