@@ -56,7 +56,7 @@ def base_params(n, dt=0.01, hetero=False, tol=1e-8, zero_mode=True, elastic=True
         elastic_enabled=1 if elastic else 0,
         C=C, dC=dC, eps00=EPS00, E0=(0.0,) * 6,
         elastic_warm_start_enabled=1, elastic_residual_control_enabled=1,
-        elastic_iter_min=2, elastic_iter_max=64, elastic_residual_tolerance=tol,
+        elastic_iter_min=2, elastic_iter_max=32, elastic_residual_tolerance=tol,
         elastic_residual_absolute_floor=1e-30, elastic_fail_on_nonconvergence=1,
         zero_mode=1 if zero_mode else 0,
     )

@@ -5,8 +5,10 @@ import pytest
 
 from oracle.step_ref import KSpace, Params
 
+# 256 and 512 run two radix-16 stages (the second with 16x16 twiddles), 192 and
+# 384 mix radix 16 with 3 and 8; 256 along each axis is the headline grid's plan
 SHAPES = [(8, 8, 8), (16, 12, 8), (12, 20, 24), (32, 32, 32), (6, 8, 246), (246, 4, 6), (64, 48, 40),
-          (96, 16, 128)]
+          (96, 16, 128), (256, 6, 8), (8, 256, 6), (6, 8, 256), (4, 6, 512), (192, 4, 384)]
 
 
 def _mask(shape):

@@ -42,6 +42,10 @@ def _cases():
     out.append(("rect_noelastic",) + _rect_case(elastic=False))
     # Nz = 128 keeps 43 kz: spectral rows padded to whole 128-B lines
     out.append(("rect_kz_padded",) + _rect_case(shape=(24, 20, 128)))
+    # length-256 axes: the plan of the 256^3 grid (two radix-16 stages, 16-column
+    # y/x tiles, 86 retained kz in 96-wide padded rows) along z and along x
+    out.append(("rect_z256",) + _rect_case(shape=(16, 16, 256)))
+    out.append(("rect_x256",) + _rect_case(shape=(256, 12, 16)))
     return out
 
 
