@@ -380,6 +380,40 @@ int main(int argc, char** argv) {
     return fatal("--pf-checkpoint-path is required when periodic checkpointing is enabled");
   if (minimize && (ckpt_every > 0 || !restart_from.empty()))
     return fatal("checkpoint/restart applies to dynamics runs");
+  // initial-state identity (main_cuda.cu:24515-24555): either the complete
+  // legacy triple, or the conditional-handoff class with two lowercase
+  // SHA-256 manifests, the latter only for zero-mode dynamics from raw_fields
+  // or a restart
+  {
+    auto is_lowercase_sha256 = [](const std::string& v) {
+      if (v.size() != 64) return false;
+      for (char c : v)
+        if (!((c >= '0' && c <= '9') || (c >= 'a' && c <= 'f'))) return false;
+      return true;
+    };
+    const std::string legacy = "LEGACY_UNSPECIFIED";
+    const std::string handoff = "MASS_CONSERVING_LIBRARY_ASSEMBLED_CONDITIONAL_HANDOFF_V1";
+    const bool legacy_id = init_class == legacy && fixture_sha == legacy && library_sha == legacy;
+    const bool handoff_id = init_class == handoff && is_lowercase_sha256(fixture_sha) &&
+                            is_lowercase_sha256(library_sha);
+    if (!legacy_id && !handoff_id)
+      return fatal("PF initial-state provenance must be either the complete LEGACY_UNSPECIFIED identity or " +
+                   handoff + " with two lowercase SHA-256 identities");
+    if (handoff_id) {
+      if (!P.zero_mode || minimize)
+        return fatal(handoff + " requires dynamics with PF_CONSERVED_Y_ZERO_MODE_V1 enabled");
+      if (restart_from.empty() && init_mode != "raw_fields")
+        return fatal("fresh " + handoff +
+                     " runs require --init-mode raw_fields; continuation must use a provenance-matched checkpoint");
+    }
+  }
+  // main_cuda.cu:24826-24841
+  if (!(P.dt > 0.0)) return fatal("Invalid dt. Time step must be > 0.");
+  if (P.nsteps <= 0) return fatal("Invalid nsteps=" + std::to_string(P.nsteps) + ". Total step count must be > 0.");
+  if (P.out_every <= 0)
+    return fatal("Invalid out_every=" + std::to_string(P.out_every) + ". VTK output interval must be > 0.");
+  if (P.csv_out_every <= 0)
+    return fatal("Invalid csv_out_every=" + std::to_string(P.csv_out_every) + ". CSV output interval must be > 0.");
   pf_checkpoint_provenance prov;
   pf_checkpoint_provenance_default(&P, &prov);
   auto set_identity = [](char* dst, size_t width, const std::string& v) {
@@ -391,10 +425,17 @@ int main(int argc, char** argv) {
       !set_identity(prov.fixture_manifest_sha256, sizeof prov.fixture_manifest_sha256, fixture_sha) ||
       !set_identity(prov.profile_library_manifest_sha256, sizeof prov.profile_library_manifest_sha256, library_sha))
     return fatal("checkpoint identity value is too long");
-  if (P.out_every > P.nsteps) P.out_every = P.nsteps;
-  if (P.csv_out_every > P.nsteps) P.csv_out_every = P.nsteps;
-  if (P.out_every < 1) P.out_every = 1;
-  if (P.csv_out_every < 1) P.csv_out_every = 1;
+  // main_cuda.cu:24871-24878
+  if (P.out_every > P.nsteps) {
+    std::fprintf(stderr, "[warn] out_every (%d) exceeds nsteps (%d); output at the last step\n", P.out_every,
+                 P.nsteps);
+    P.out_every = P.nsteps;
+  }
+  if (P.csv_out_every > P.nsteps) {
+    std::fprintf(stderr, "[warn] csv_out_every (%d) exceeds nsteps (%d); output at the last step\n",
+                 P.csv_out_every, P.nsteps);
+    P.csv_out_every = P.nsteps;
+  }
 
   const size_t N = (size_t)P.Nx * P.Ny * P.Nz;
   std::vector<double> phi, xB, dydt;
@@ -490,7 +531,9 @@ int main(int argc, char** argv) {
 
   pf_scalars s0;
   pf_scalars_get(h, &s0);
-  const double target = s0.mass_xBtot;
+  // the engine's frozen zero-mode target (after a restart, the checkpoint's
+  // target_mass_code rather than the restored fields' own sum)
+  const double target = P.zero_mode ? s0.zero_mode_target_mass : s0.mass_xBtot;
   auto write_row = [&](int step, const pf_scalars& sc) {
     double R_avg = 0.0;
     if (sc.n_if > 0.0) {

@@ -875,6 +875,19 @@ struct Engine final : EngineBase {
   std::vector<void*> allocations;
   int cur = 0;
   bool warm_valid = false;
+  // The device fields form a consistent state: set by pf_set_fields and an
+  // accepted restart, cleared while a step, minimize iteration or restart is
+  // in progress (kept cleared when one fails part-way, so the handle refuses
+  // further steps until it is restored) and by the measurement entry points
+  // that overwrite the buffers.
+  bool state_ok = false;
+  int require_state(const char* what) const {
+    if (state_ok) return PF_OK;
+    return fail(PF_E_ARG, std::string(what) +
+                              ": the handle holds no valid state (no fields yet, or a step, minimize iteration, "
+                              "restart or kernel timing left it incomplete); call pf_set_fields or "
+                              "pf_checkpoint_read");
+  }
   double target_mass = 0.0;
   // statistics
   long long steps_done = 0, elastic_total = 0;
@@ -1126,6 +1139,7 @@ struct Engine final : EngineBase {
       yy[i] = logit_from_fraction(x, P.xB_eps, P.Y_clip);
       if (dYdt0) dd[i] = dYdt0[i];
     }
+    state_ok = false;
     cur = 0;
     PF_HIP(hipMemcpy(phi[0], ph.data(), R * sizeof(double), hipMemcpyHostToDevice));
     PF_HIP(hipMemcpy(xB, xb.data(), R * sizeof(double), hipMemcpyHostToDevice));
@@ -1148,6 +1162,7 @@ struct Engine final : EngineBase {
     zm_lambda = zm_res = zm_deriv = 0.0;
     zm_iters = 0;
     zm_accepted = 0;
+    state_ok = true;
     return PF_OK;
   }
 
@@ -1559,13 +1574,15 @@ struct Engine final : EngineBase {
 
   // ---------------------------------------------------------------- step
   int step() override {
+    int rc = require_state("pf_step");
+    if (rc) return rc;
+    state_ok = false;
     minimizing = false;
     const T invN = T(1.0 / (double)R);
     const int rows = Nx * Ny;
     const bool zm = prm.zero_mode != 0;
     double* phi_old = phi[cur];
     double* phi_new = phi[1 - cur];
-    int rc;
     // The gradient energy of phi^n is a diagnostic (the reference's dynamics
     // loop does not evaluate it, main_cuda.cu:29261-29290 is minimize only):
     // it is computed when the scalars are read, from phi^n, which stays in
@@ -1639,6 +1656,7 @@ struct Engine final : EngineBase {
     eng_pending = true;
     sig6_valid = keep_sig6 && P.elastic;
     ++steps_done;
+    state_ok = true;
     return PF_OK;
   }
 
@@ -1804,6 +1822,7 @@ struct Engine final : EngineBase {
     int rc;
     if ((rc = nn::dataset_slot(net, Nx, Ny, Nz, &feat, &tgt))) return rc;
     if (!P.elastic) return fail(PF_E_ARG, "pf_nn_collect: elasticity is disabled");
+    if ((rc = require_state("pf_nn_collect"))) return rc;
     if ((rc = elastic_solve())) return rc;
     double sp, st, so;
     nn::scales(net, &sp, &st, &so);
@@ -1848,7 +1867,8 @@ struct Engine final : EngineBase {
 
   // ---------------------------------------------------------------- minimize
   int minimize_begin(const pf_minimize_opts& o) override {
-    int rc;
+    int rc = require_state("pf_minimize_begin");
+    if (rc) return rc;
     if (!m_gexp) {
       if ((rc = dalloc(&m_gexp, R)) || (rc = dalloc(&m_gfull, R)) || (rc = dalloc(&m_s00, R)) ||
           (rc = dalloc(&m_xzero, R)) || (rc = dalloc(&m_eps, 6 * (size_t)R)))
@@ -1902,6 +1922,8 @@ struct Engine final : EngineBase {
 
   int minimize_iterate(pf_minimize_stats* st) override {
     if (!minimizing) return fail(PF_E_ARG, "pf_minimize_begin has not been called on this handle");
+    if (int rs = require_state("pf_minimize_iterate")) return rs;
+    state_ok = false;
     const T invN = T(1.0 / (double)R);
     const int rows = Nx * Ny;
     const bool el = P.elastic != 0;
@@ -2034,6 +2056,7 @@ struct Engine final : EngineBase {
     }
     if (!std::isfinite(mean_h) || !std::isfinite(m_lambda))
       return fail(PF_E_NONFINITE, "minimize produced a non-finite phi");
+    state_ok = true;
     return PF_OK;
   }
 
@@ -2177,6 +2200,7 @@ struct Engine final : EngineBase {
     s->zero_mode_iterations = zm_iters;
     s->steps_done = steps_done;
     s->elastic_iterations_total = elastic_total;
+    s->zero_mode_target_mass = prm.zero_mode ? target_mass : NAN;
     return PF_OK;
   }
 
@@ -2335,7 +2359,17 @@ struct Engine final : EngineBase {
       return fail(PF_E_IO, "PF zero-mode restart grid/dt/temperature mismatch");
     const bool el = elastic_state_required(&prm);
     if (el && !c.elastic_present) return fail(PF_E_IO, "accelerated elastic restart lacks the registered V4 warm state");
+    // the exact-resume companion is read (and a broken one refused) before any
+    // device state changes
+    ckpt::ExactState x;
+    bool have_x = false;
+    if (el) {
+      have_x = ckpt::read_exact(std::string(path) + ".state", &x, &err);
+      if (!have_x && !err.empty()) return fail(PF_E_IO, err);
+    }
     PF_HIP(hipStreamSynchronize(stream));
+    // from here on a rejection leaves the handle without a valid state
+    state_ok = false;
     cur = 0;
     PF_HIP(hipMemcpy(phi[0], c.phi.data(), R * sizeof(double), hipMemcpyHostToDevice));
     PF_HIP(hipMemcpy(Y, c.Y.data(), R * sizeof(double), hipMemcpyHostToDevice));
@@ -2369,12 +2403,12 @@ struct Engine final : EngineBase {
     warm_valid = false;
     hist_count = 0;
     hist_head = 0;
-    if (!el) return PF_OK;
+    if (!el) {
+      state_ok = true;
+      return PF_OK;
+    }
     last_iters = (int)c.elastic_last_iterations;
     last_res = c.elastic_last_relative_residual;
-    ckpt::ExactState x;
-    const bool have_x = ckpt::read_exact(std::string(path) + ".state", &x, &err);
-    if (!have_x && !err.empty()) return fail(PF_E_IO, err);
     std::vector<cx<double>> cur_state(3 * nSc);
     int nh = 0;
     if (have_x && x.bound_checksum == c.payload_checksum && x.accepted_step == c.accepted_step &&
@@ -2408,6 +2442,7 @@ struct Engine final : EngineBase {
     hist_count = nh;
     hist_head = nh % 3;
     warm_valid = true;
+    state_ok = true;
     return PF_OK;
   }
 
@@ -2576,6 +2611,7 @@ struct Engine final : EngineBase {
   }
 
   int time_kernel(int id, int reps, double* us, double* bytes) override {
+    state_ok = false;   // the timed launches overwrite the simulation buffers
     if (extended && (id == PF_K_GREEN || id == PF_K_AA_UPDATE)) {
       // the accelerated solver's control block in its steady state: running,
       // a full history, slot 0 next, fixed mixing coefficients

@@ -153,6 +153,17 @@ static int validate(const pf_params* p) {
                     "accelerated elastic solver requires elastic_warm_start_enabled=1 and "
                     "elastic_residual_control_enabled=1");
   if (p->elastic_iter_max < 1) return pf::fail(PF_E_PARAMS, "elastic_iter_max must be >= 1");
+  // accelerated elastic iteration/residual contract (main_cuda.cu:24492-24510)
+  if (p->elastic_enabled && accel &&
+      (p->elastic_iter_min <= 0 || p->elastic_iter_max < p->elastic_iter_min ||
+       !(p->elastic_residual_tolerance > 0.0) || !std::isfinite(p->elastic_residual_tolerance) ||
+       !(p->elastic_residual_absolute_floor > 0.0) || !std::isfinite(p->elastic_residual_absolute_floor) ||
+       (p->elastic_fail_on_nonconvergence != 0 && p->elastic_fail_on_nonconvergence != 1)))
+    return pf::fail(PF_E_PARAMS, "accelerated elastic iteration/residual contract is invalid");
+  // zero-mode tolerance/iteration contract (main_cuda.cu:24468-24475)
+  if (p->zero_mode && (!(p->zero_mode_tolerance_relative > 0.0) || !std::isfinite(p->zero_mode_tolerance_relative) ||
+                       p->zero_mode_max_iterations <= 0))
+    return pf::fail(PF_E_PARAMS, "PF zero-mode tolerance/iteration contract is invalid");
   if (p->elastic_warm_start_order < 1 || p->elastic_warm_start_order > 3)
     return pf::fail(PF_E_PARAMS, "elastic_warm_start_order must be 1, 2 or 3");
   if (p->elastic_anderson_depth < 0 || p->elastic_anderson_depth > 4)

@@ -105,6 +105,9 @@ typedef struct pf_scalars {
   long long elastic_iterations_total;
   double n_in;                /* count of phi > 0.5          (reduce_sum_N_in_N_if_kernel) */
   double n_if;                /* count of 0.12 < phi < 0.88 */
+  double zero_mode_target_mass; /* the frozen target of the zero mode (NaN without it): the mass at
+                                   pf_set_fields, or the restarted checkpoint's target_mass_code
+                                   (main_cuda.cu:28080-28105) */
 } pf_scalars;
 
 typedef struct pf_handle pf_handle;

@@ -72,3 +72,26 @@ def test_unknown_key_is_ignored(pf):
     assert pf.lib().pf_params_set(C.byref(p), b"no_such_key", b"1") == 0
     assert pf.lib().pf_params_set(C.byref(p), b"S_p_46", b"2.5") == 1
     assert p.S_p[17] == 2.5
+
+
+@pytest.mark.parametrize("key,value,match", [
+    ("elastic_iter_min", 0, "accelerated elastic iteration/residual contract"),
+    ("elastic_iter_max", 1, "accelerated elastic iteration/residual contract"),   # below iter_min = 2
+    ("elastic_residual_tolerance", 0.0, "accelerated elastic iteration/residual contract"),
+    ("elastic_residual_tolerance", float("inf"), "accelerated elastic iteration/residual contract"),
+    ("elastic_residual_absolute_floor", 0.0, "accelerated elastic iteration/residual contract"),
+    ("elastic_fail_on_nonconvergence", 2, "accelerated elastic iteration/residual contract"),
+    ("zero_mode_tolerance_relative", -1e-12, "zero-mode tolerance/iteration contract"),
+    ("zero_mode_max_iterations", 0, "zero-mode tolerance/iteration contract"),
+])
+def test_create_refuses_invalid_solver_contracts(pf, key, value, match):
+    """pf_create fails with PF_E_PARAMS before any device call on the
+    reference's contracts (main_cuda.cu:24468-24475, 24492-24510)."""
+    pd = dict(base_params(16, hetero=True), **{key: value})
+    p = pf.params_from_dict(pd)
+    h = C.c_void_p()
+    rc = pf.lib().pf_create(C.byref(p), 0, C.byref(h))
+    assert rc == -2
+    buf = C.create_string_buffer(1024)
+    pf.lib().pf_last_error(buf, 1024)
+    assert match in buf.value.decode()

@@ -139,7 +139,7 @@ def test_cli_restart_reproduces_continuous_run(pf, tmp_path):
     one run and 3 + restart + 3 end in byte-identical checkpoints, and the
     restart appends to the elastic and zero-mode traces."""
     ident = ["--pf-initial-state-class", "MASS_CONSERVING_LIBRARY_ASSEMBLED_CONDITIONAL_HANDOFF_V1",
-             "--pf-fixture-manifest-sha256", "ab" * 32]
+             "--pf-fixture-manifest-sha256", "ab" * 32, "--pf-profile-library-manifest-sha256", "cd" * 32]
     args, *_ = _bundle(tmp_path, steps=6)
     cont = str(tmp_path / "continuous.chk")
     r = _run(args + ident + ["--pf-checkpoint-every", "6", "--pf-checkpoint-path", cont,
@@ -164,6 +164,14 @@ def test_cli_restart_reproduces_continuous_run(pf, tmp_path):
     runs = [d for d in os.listdir(tmp_path / "Results") if "_steps6_" in d]
     trace = (tmp_path / "Results" / runs[0] / "half" / "elastic_solver_trace.csv").read_text().splitlines()
     assert trace[0].startswith("step,") and [ln.split(",")[0] for ln in trace[1:]] == ["4", "5", "6"]
+    # the zero-mode trace of the continued run carries the checkpoint's frozen
+    # target, the value the first half's rows carry
+    half = [d for d in os.listdir(tmp_path / "Results") if "_steps3_" in d]
+    z1 = (tmp_path / "Results" / half[0] / "half" / "pf_zero_mode_trace.csv").read_text().splitlines()
+    z2 = (tmp_path / "Results" / runs[0] / "half" / "pf_zero_mode_trace.csv").read_text().splitlines()
+    assert z1[0].startswith("step,target_mass_code")
+    targets = {ln.split(",")[1] for ln in z1[1:] + z2[1:]}
+    assert len(targets) == 1, targets
 
 
 @pytest.mark.gpu
@@ -206,3 +214,52 @@ def test_cli_nonconvergence_is_fatal(pf, tmp_path):
     r = _run(args, tmp_path, {"CUDA_STO_SUPPRESS_VTK_OUTPUT": "1"})
     print(r.stdout[-800:], r.stderr[-800:], flush=True)
     assert r.returncode == 2 and "[fatal]" in r.stderr and "did not converge" in r.stderr
+
+
+HANDOFF = "MASS_CONSERVING_LIBRARY_ASSEMBLED_CONDITIONAL_HANDOFF_V1"
+
+
+@pytest.mark.parametrize("flags,match", [
+    # a mixed triple: legacy class with a real manifest (main_cuda.cu:24527-24545)
+    (["--pf-initial-state-class", "LEGACY_UNSPECIFIED", "--pf-fixture-manifest-sha256", "a" * 64],
+     "complete LEGACY_UNSPECIFIED identity"),
+    # handoff class with an upper-case or a short digest
+    (["--pf-initial-state-class", HANDOFF, "--pf-fixture-manifest-sha256", "A" * 64,
+      "--pf-profile-library-manifest-sha256", "b" * 64], "two lowercase SHA-256"),
+    (["--pf-initial-state-class", HANDOFF, "--pf-fixture-manifest-sha256", "a" * 63,
+      "--pf-profile-library-manifest-sha256", "b" * 64], "two lowercase SHA-256"),
+    (["--pf-initial-state-class", "SOMETHING_ELSE"], "complete LEGACY_UNSPECIFIED identity"),
+])
+def test_checkpoint_identity_rule_is_fail_closed(tmp_path, flags, match):
+    args, *_ = _bundle(tmp_path)
+    r = _run(args + flags, tmp_path)
+    assert r.returncode == 2 and match in r.stderr, r.stderr
+
+
+def test_handoff_identity_needs_zero_mode_dynamics_and_raw_fields(tmp_path):
+    """A well-formed handoff identity is accepted only for zero-mode dynamics
+    (main_cuda.cu:24546-24553) started from raw_fields or a restart (24554-24561)."""
+    ident = ["--pf-initial-state-class", HANDOFF, "--pf-fixture-manifest-sha256", "a" * 64,
+             "--pf-profile-library-manifest-sha256", "b" * 64]
+    args, *_ = _bundle(tmp_path, zero_mode=False)
+    r = _run(args + ident, tmp_path)
+    assert r.returncode == 2 and "requires dynamics with PF_CONSERVED_Y_ZERO_MODE_V1" in r.stderr
+    args, *_ = _bundle(tmp_path)
+    r = _run(args + ident + ["--mode=minimize"], tmp_path)
+    assert r.returncode == 2 and "requires dynamics" in r.stderr
+    i = args.index("--init-mode")
+    no_raw = args[:i] + args[i + 2:]
+    r = _run(no_raw + ident, tmp_path)
+    assert r.returncode == 2 and "require --init-mode raw_fields" in r.stderr
+
+
+@pytest.mark.parametrize("pos,value,match", [(5, "0", "Invalid out_every=0"), (6, "0", "Invalid csv_out_every=0"),
+                                             (4, "0", "Invalid nsteps=0")])
+def test_nonpositive_intervals_are_fatal(tmp_path, pos, value, match):
+    """out_every, csv_out_every and nsteps <= 0 exit [fatal] (main_cuda.cu:24830-24841)
+    rather than being clamped."""
+    args, *_ = _bundle(tmp_path)
+    args = list(args)
+    args[pos + 1] = value
+    r = _run(args, tmp_path)
+    assert r.returncode == 2 and match in r.stderr, r.stderr

@@ -105,9 +105,12 @@ def test_restart_from_v4_alone(pf, tmp_path, prec):
 @pytest.mark.gpu
 def test_checkpoint_contents_and_reference_reader(pf, tmp_path):
     """The V4 file holds the handle's fields, the zero-mode runtime and the
-    warm state at the retained modes (zero elsewhere); when the reference
-    driver is present, the reference reader accepts the file and its writer
-    reproduces it byte for byte."""
+    warm state at the retained modes (zero elsewhere).  Its bytes equal what
+    the host writer (pf_checkpoint_file_write) makes of the same content, and
+    that writer is byte-identical to the reference's own writer built from
+    the reference sources (tests/test_checkpoint.py, CPU), so the GPU-written
+    file is the reference's file.  Where that build is present here too, the
+    reference reader and writer also round-trip the file directly."""
     pd, phi, xB, cap = _case(True)
     s = _sim(pf, pd, phi, xB, cap, pf.PF_FP64)
     s.step(3)
@@ -130,6 +133,12 @@ def test_checkpoint_contents_and_reference_reader(pf, tmp_path):
     assert nz_full <= np.count_nonzero(u)          # only retained modes are populated
     assert np.allclose(np.sort(np.abs(d[..., 0] + 1j * d[..., 1]).ravel())[-nz_full:],
                        np.sort(np.abs(u).astype(np.float32).ravel())[-nz_full:], rtol=1e-6)
+    # the chain to the reference writer: re-serialise the content with the host writer
+    hdr = {k: v for k, v in h.items() if k not in ("version", "payload_checksum")}
+    again = str(tmp_path / "again.chk")
+    pf.checkpoint_file_write(again, prov, hdr, data["phi"], data["Y"], data["xB"], data["dYdt"],
+                             data["displacement_k"])
+    assert open(path, "rb").read() == open(again, "rb").read()
     if os.path.exists(TOOL):
         args = [prov.as_dict()[k] for k in pf.PfCheckpointProvenance.TEXT]
         args += [format(prov.parameter_fingerprint, "x"), format(prov.elastic_solver_fingerprint, "x")]

@@ -263,6 +263,13 @@ def test_elastic_nonconvergence_is_fail_closed(pf, name, prec, extra):
     with pytest.raises(pf.PfError, match="did not converge") as ei:
         s.step()
     assert ei.value.code == -4
+    # the failed step left the handle part-way: it refuses to step until restored
+    with pytest.raises(pf.PfError, match="holds no valid state") as ei:
+        s.step()
+    assert ei.value.code == -1
+    s.set_fields(phi, xB, xB_cap=meta["xB_max_safe"])
+    with pytest.raises(pf.PfError, match="did not converge"):
+        s.step()
     t = _gpu(pf, dict(pd, elastic_fail_on_nonconvergence=0), phi, xB, meta["xB_max_safe"], getattr(pf, prec))
     t.step(2)
     sc = t.scalars()
