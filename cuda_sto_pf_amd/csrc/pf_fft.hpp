@@ -4,7 +4,7 @@
 // creates at main_cuda.cu:28158-28214.  A workgroup holds BATCH transforms of
 // compile-time length N in LDS and runs a Stockham auto-sort sequence of
 // radix-4/2/3/5 stages (in place, butterflies staged through registers);
-// any other prime factor p (e.g. the 41 of 246 = 2*3*41) runs as a direct
+// any other prime factor p (e.g. the 41 of 246 = 2*3*41) runs as a symmetric
 // p-point DFT stage out of place through a second LDS buffer.
 //
 // Element (b, j) of the batch lives at  COLMAJ ? j*BATCH + b : b*N + j.
@@ -48,6 +48,18 @@ constexpr bool plan_has_big_prime(int n) {
     int r = pick_radix(n);
     if (r > 5 && !is_pow2_radix(r)) return true;
     n /= r;
+  }
+  return false;
+}
+// A prime stage needs the second LDS buffer unless it is the last stage
+// (NS * R == N): there each butterfly's outputs land on its own inputs' slots
+// and the stage runs in place.
+constexpr bool plan_needs_alt(int n) {
+  int ns = 1;
+  while (ns < n) {
+    const int r = pick_radix(n / ns);
+    if (r > 5 && !is_pow2_radix(r) && ns * r != n) return true;
+    ns *= r;
   }
   return false;
 }
@@ -267,17 +279,32 @@ __device__ __forceinline__ void stage_inplace(cx<T>* buf, const cx<T>* tw) {
   __syncthreads();
 }
 
-// Out-of-place direct DFT stage for a prime radix R > 5.
+// DFT stage for an odd prime radix R > 5 (the 41 of 246), out of place
+// (src -> dst) or, as the last stage of a plan, in place (dst == src: a
+// butterfly's R outputs occupy its R input slots, and its lanes are one
+// wave's, which read all inputs before any output is written).
+//  1. The inter-stage twiddles W_N^(jm s TSTEP) are applied to the R-1
+//     non-zero inputs of every butterfly, in place in `src`.
+//  2. Each output pair (X_k, X_{R-k}) comes from the symmetric sums
+//        A = x_0 + sum_s (x_s + x_{R-s}) cos(2 pi s k / R)
+//        B =       sum_s (x_s - x_{R-s}) (-/+) sin(2 pi s k / R),  s = 1..(R-1)/2,
+//     X_k = A + iB, X_{R-k} = A - iB (a quarter of the multiplies of the
+//     direct sum).  One lane per pair, (R+1)/2 lanes per butterfly (lane 0 of
+//     the group gives X_0); the group's lanes read x_s and x_{R-s} at the same
+//     LDS addresses in the same iteration (broadcast reads), and each lane's
+//     cos/sin of s*k mod R comes from the length-N twiddle table.
 template <typename T, int N, int R, int NS, int BATCH, bool COLMAJ, bool INV, int NT>
-__device__ __forceinline__ void stage_prime(const cx<T>* src, cx<T>* dst, const cx<T>* tw) {
+__device__ __forceinline__ void stage_prime(cx<T>* src, cx<T>* dst, const cx<T>* tw) {
   constexpr int NB = N / R;
   constexpr int TSTEP = N / (NS * R);
   constexpr int WSTEP = N / R;   // W_R^m = W_N^(m*N/R)
-  constexpr int TOT = NB * BATCH * R;   // one output element per work item
-  for (int t = threadIdx.x; t < TOT; t += NT) {
-    const int qo = t % R;
-    const int bt = t / R;
-    int b, j;
+  constexpr int NBF = NB * BATCH;   // butterflies in the workgroup
+  constexpr int H = (R - 1) / 2;
+  constexpr int G = H + 1;          // lanes per butterfly
+  static_assert(R % 2 == 1 && G <= 64, "odd prime radix up to 127");
+  constexpr int GPW = 64 / G;       // butterflies per wave and pass
+  constexpr int GROUPS = (NT / 64) * GPW;
+  auto split = [](int bt, int& b, int& j) {
     if (COLMAJ) {
       b = bt % BATCH;
       j = bt / BATCH;
@@ -285,24 +312,53 @@ __device__ __forceinline__ void stage_prime(const cx<T>* src, cx<T>* dst, const 
       j = bt % NB;
       b = bt / NB;
     }
-    const int jm = j % NS;
-    cx<T> acc = {T(0), T(0)};
-    int m = 0;   // (qo * s) mod R
-    for (int s = 0; s < R; ++s) {
-      cx<T> a = src[lds_index<N, BATCH, COLMAJ>(b, j + s * NB)];
-      if (NS > 1 && s > 0) {
-        cx<T> w = tw[jm * s * TSTEP];
+  };
+  if constexpr (NS > 1) {
+    constexpr int TOT = NBF * (R - 1);
+    for (int t = threadIdx.x; t < TOT; t += NT) {
+      const int s = 1 + t % (R - 1);
+      int b, j;
+      split(t / (R - 1), b, j);
+      const int jm = j % NS;
+      if (jm != 0) {
+        const int at = lds_index<N, BATCH, COLMAJ>(b, j + s * NB);
+        cx<T> w = tw[jm * s * TSTEP];   // jm s TSTEP < NS R TSTEP = N
         if (INV) w.im = -w.im;
-        a = cmul(a, w);
+        src[at] = cmul(src[at], w);
       }
-      cx<T> w2 = tw[m * WSTEP];
-      if (INV) w2.im = -w2.im;
-      acc = cadd(acc, cmul(a, w2));
-      m += qo;
-      if (m >= R) m -= R;
     }
-    const int d = (j / NS) * NS * R + jm;
-    dst[lds_index<N, BATCH, COLMAJ>(b, d + qo * NS)] = acc;
+    __syncthreads();
+  }
+  const int lane = threadIdx.x & 63;
+  const int g = lane / G;
+  const int k = lane - g * G;
+  const int slot = (threadIdx.x >> 6) * GPW + g;
+  for (int base = 0; base < NBF; base += GROUPS) {
+    const int bt = base + slot;
+    if (g < GPW && bt < NBF) {
+      int b, j;
+      split(bt, b, j);
+      const cx<T> x0 = src[lds_index<N, BATCH, COLMAJ>(b, j)];
+      cx<T> A = x0, B = {T(0), T(0)};
+      int m = 0;   // s k mod R
+#pragma unroll 4
+      for (int s = 1; s <= H; ++s) {
+        m += k;
+        if (m >= R) m -= R;
+        const cx<T> a = src[lds_index<N, BATCH, COLMAJ>(b, j + s * NB)];
+        const cx<T> c = src[lds_index<N, BATCH, COLMAJ>(b, j + (R - s) * NB)];
+        const cx<T> w = tw[m * WSTEP];   // (cos, -sin) of 2 pi m / R
+        const T wi = INV ? -w.im : w.im;
+        A.re += (a.re + c.re) * w.re;
+        A.im += (a.im + c.im) * w.re;
+        B.re += (a.re - c.re) * wi;
+        B.im += (a.im - c.im) * wi;
+      }
+      const int jm = j % NS;
+      const int d = (j / NS) * NS * R + jm;
+      dst[lds_index<N, BATCH, COLMAJ>(b, d + k * NS)] = cx<T>{A.re - B.im, A.im + B.re};
+      if (k > 0) dst[lds_index<N, BATCH, COLMAJ>(b, d + (R - k) * NS)] = cx<T>{A.re + B.im, A.im - B.re};
+    }
   }
   __syncthreads();
 }
@@ -318,6 +374,9 @@ __device__ __forceinline__ cx<T>* fft_stages(cx<T>* buf, cx<T>* alt, const cx<T>
     if constexpr (R <= 5 || is_pow2_radix(R)) {
       stage_inplace<T, N, R, NS, BATCH, COLMAJ, INV, NT>(buf, tw);
       return fft_stages<T, N, NS * R, BATCH, COLMAJ, INV, NT>(buf, alt, tw);
+    } else if constexpr (NS * R == N) {
+      stage_prime<T, N, R, NS, BATCH, COLMAJ, INV, NT>(buf, buf, tw);
+      return buf;
     } else {
       stage_prime<T, N, R, NS, BATCH, COLMAJ, INV, NT>(buf, alt, tw);
       return fft_stages<T, N, NS * R, BATCH, COLMAJ, INV, NT>(alt, buf, tw);

@@ -40,7 +40,7 @@ constexpr int ZPASS_PREFETCH_DOUBLES = 48;
 // a chunk at a time after it).
 template <typename T, int N, int NC, int NPRE = 0, int BUDGET = ZPASS_LDS_BUDGET>
 constexpr int zpass_rows() {
-  constexpr int per_pair = NC * (N + N / 16) * (int)sizeof(cx<T>) * (plan_has_big_prime(N) ? 2 : 1);
+  constexpr int per_pair = NC * (N + N / 16) * (int)sizeof(cx<T>) * (plan_needs_alt(N) ? 2 : 1);
   constexpr int pairs = BUDGET / per_pair;
   int p = pairs < 1 ? 1 : (pairs > 32 ? 32 : pairs);
   if (NPRE > 0 && ((2 * p * N + NT - 1) / NT) * NPRE > ZPASS_PREFETCH_DOUBLES)
@@ -57,7 +57,7 @@ constexpr int zpass_kmax() {
 
 template <typename T, int N>
 constexpr int cpass_cols() {
-  constexpr int per_col = (N + N / 16) * (int)sizeof(cx<T>) * (plan_has_big_prime(N) ? 2 : 1);
+  constexpr int per_col = (N + N / 16) * (int)sizeof(cx<T>) * (plan_needs_alt(N) ? 2 : 1);
   constexpr int c = (LDS_BUDGET / per_col) - 1;
   return c >= 64 ? 64 : c >= 32 ? 32 : c >= 16 ? 16 : c >= 8 ? 8 : c >= 4 ? 4 : c >= 2 ? 2 : 1;
 }
@@ -134,7 +134,7 @@ __global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op:
   constexpr int NCB = NCM > 0 ? NCM : 1;
   constexpr int RT = zpass_rows<T, N, NCB, Op::NPRE, ZBudget<Op>::value>();
   constexpr int NP = RT / 2;
-  constexpr bool BIG = plan_has_big_prime(N);
+  constexpr bool BIG = plan_needs_alt(N);
   using LY = Lds<N, NCB * NP, false>;   // row-major: layout independent of the batch
   __shared__ cx<T> buf[LY::size];
   __shared__ cx<T> alt[BIG ? LY::size : 1];
@@ -343,7 +343,7 @@ template <typename T, int N, bool INV>
 __global__ void __launch_bounds__(NT) cpass_kernel(CPassArgs<T> ca) {
   if (ca.gate && *ca.gate == 0.0) return;
   constexpr int TC = cpass_cols<T, N>();
-  constexpr bool BIG = plan_has_big_prime(N);
+  constexpr bool BIG = plan_needs_alt(N);
   using LY = Lds<N, TC, true>;
   __shared__ cx<T> buf[LY::size];
   __shared__ cx<T> alt[BIG ? LY::size : 1];
