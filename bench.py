@@ -12,12 +12,17 @@ state/reductions/zero mode/residual, the learned network at the cold start of
 the elastic fixed point) as ``value``; the plain FP64 path on the
 same inputs; max |x_B mixed - x_B FP64| after the same steps; elastic updates
 per step; the roofline of the dominant kernel (HIP events inside the library);
-the transforms and the whole step against a fixed byte-accounting model; the
-cold-start gain of the
-learned warm start; and the CPU FP64 oracle timed on a bounded sample as
+the step's timed kernels on their own bytes; the cold-start gain of the
+learned warm start; and the CPU FP64 oracle timed on one warm 256^3 step as
 ``cpu_baseline``.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+       python bench.py --accuracy --workload 2|3|5 [--steps K]
+The second form is the long-horizon accuracy run: the FP64 path with the
+reference iteration and the mixed + accelerated path side by side on the
+same inputs for the workload's stated number of steps (500 at 128^3, 2000 at
+246^3), reporting max |dx_B|, max |dphi|, the mass drift of both and the
+differences of the free-energy and elastic-energy densities every 100 steps.
 """
 import argparse
 import json
@@ -103,36 +108,19 @@ def learned_cold_start(pf, learned, net, pd, pd_fast, phi, xB):
     return out
 
 
-def fft_byte_model(table, n):
-    """Fixed accounting of an FP32 R2C/C2R 3-D transform (each of the three
-    axis passes reads and writes its array once: 4R + 40C bytes, C =
-    N*N*(N/2+1)) against the measured per-update transform time of the
-    fused passes (two 6-component transforms per elastic update)."""
-    R = n ** 3
-    Cc = n * n * (n // 2 + 1)
-    model = 12 * (4 * R + 40 * Cc)
-    us = sum(table[k]["us"] for k in ("zpass_hij", "ypass_fwd6", "ypass_inv6", "xpass_fwd6", "xpass_inv6"))
-    ach = model / us * 1e-3
-    return {"model_bytes_per_update": model, "us_per_update": us, "achieved_GBps": ach, "peak_GBps": HBM_PEAK_GBS,
-            "frac": ach / HBM_PEAK_GBS,
-            "note": "12 FP32 transforms of the byte model per elastic update vs the measured zpass_hij + y/x "
-                    "passes (the z pass includes the h_ij operator)"}
-
-
-def step_byte_model(n, iters, ms_per_step, fp64):
-    """Fixed accounting of one step's HBM bytes with I elastic updates: every
-    transform's three axis passes read and write their array once, and every
-    fused point-wise stage reads each distinct input field once and writes
-    each output once (dealias, 1/N and k-vectors free).  That is
-    1712 R + 837 R I bytes with FP64 transforms and 908 R + 418 R I with FP32
-    transforms (phi 3, Y 12 and elastic 12 I + 6 transforms per step).  The
-    fused, pruned passes move less than this model, so the fraction can
-    exceed 1."""
-    R = float(n) ** 3
-    b = (1712.0 + 837.0 * iters) * R if fp64 else (908.0 + 418.0 * iters) * R
-    ach = b / (ms_per_step * 1e-3) * 1e-9
-    return {"bytes_per_step": b, "elastic_updates_per_step": iters, "ms_per_step": ms_per_step,
-            "achieved_GBps": ach, "peak_GBps": HBM_PEAK_GBS, "frac": ach / HBM_PEAK_GBS}
+def step_kernels(table, ms_per_step):
+    """The step's HBM accounting on the kernels' own bytes: each timed kernel's
+    algorithmic bytes per launch (every input read once, every output written
+    once, only the retained modes the pruned passes move) times its launches
+    per step.  `achieved_GBps` is those bytes over the time those launches
+    take; `share_of_step` is that time over the measured step time (the rest
+    is the first full map of each solve, the one-component phi and Y
+    transforms, the finish and zero-mode kernels and the gaps)."""
+    b = sum(v["bytes"] * v["per_step"] for v in table.values())
+    us = sum(v["us_per_step"] for v in table.values())
+    ach = b / us * 1e-3
+    return {"bytes_per_step": b, "us_per_step": us, "achieved_GBps": ach, "peak_GBps": HBM_PEAK_GBS,
+            "frac": ach / HBM_PEAK_GBS, "share_of_step": us / (ms_per_step * 1e3)}
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_256cube_mixed.json")
@@ -179,13 +167,60 @@ def _oracle_step_times(n, workers):
     return cold, warm, o.last_elastic_iterations
 
 
-def cpu_baseline(workers=16):
-    """The FP64 numpy oracle on a bounded sample of the same workload family:
-    the 40-particle geometry generated at 80^3; one cold step (untimed) then
-    one timed warm step; extrapolated to 256^3 by R log R.  Timed single-
-    threaded (numpy pocketfft) and with the transforms on `workers` scipy.fft
-    threads (the point-wise numpy work stays on one thread); the faster is
-    the reported value."""
+def gpu_state_snapshot(pf, sim):
+    """The FP64 handle's state after its timed steps: fields, the converged
+    displacement of the last solve (compact retained spectrum, expanded to
+    the oracle's masked half spectrum) and the frozen zero-mode target."""
+    from oracle.step_ref import KSpace, Params, dealias_axis_mask
+    p = sim.params
+    nx, ny, nz = p.Nx, p.Ny, p.Nz
+    ix = np.nonzero(dealias_axis_mask(nx, p.dx))[0]
+    iy = np.nonzero(dealias_axis_mask(ny, p.dy))[0]
+    iz = np.nonzero(dealias_axis_mask(nz, p.dz, wrap=False))[0]
+    u = sim.elastic_state()
+    azs = u.shape[1] // (ix.size * iy.size)
+    u = u.reshape(3, ix.size, iy.size, azs)[..., :iz.size]
+    full = []
+    for c in range(3):
+        f = np.zeros((nx, ny, nz // 2 + 1), dtype=np.complex128)
+        f[np.ix_(ix, iy, iz)] = u[c]
+        full.append(f)
+    sc = sim.scalars()
+    return {"phi": sim.field(pf.PF_PHI), "Y": sim.field(pf.PF_Y), "xB": sim.field(pf.PF_XB),
+            "dYdt": sim.field(pf.PF_DYDT), "u_k": full, "target_mass": sc["zero_mode_target_mass"],
+            "steps_done": sc["steps_done"]}
+
+
+def cpu_baseline_256(pd, cap, snap, workers):
+    """One warm step of the FP64 numpy oracle at the full 256^3 shape, started
+    from the state the GPU's FP64 reference iteration reached (the same
+    fields and converged displacement the oracle's own previous step would
+    hold), with the transforms on `workers` scipy.fft threads and the
+    point-wise numpy work on one thread."""
+    from oracle import step_ref
+    step_ref.FFT_WORKERS = workers
+    try:
+        o = step_ref.Oracle(step_ref.Params(**pd), snap["phi"], snap["xB"], dYdt0=snap["dYdt"], xB_cap=cap)
+        o.Y = snap["Y"].copy()
+        o.xB = snap["xB"].copy()
+        o.u_k = [c.copy() for c in snap["u_k"]]
+        o.warm_valid = True
+        o.target_mass = snap["target_mass"]
+        t0 = time.perf_counter()
+        o.step()
+        dt = time.perf_counter() - t0
+    finally:
+        step_ref.FFT_WORKERS = None
+    return dt, o.last_elastic_iterations
+
+
+def cpu_baseline(pd=None, cap=None, snap=None, workers=None):
+    """The FP64 numpy oracle timed on the GPU box's host.  Headline: one warm
+    step at 256^3 from the FP64 GPU state (cpu_baseline_256).  Secondary: the
+    40-particle geometry generated at 80^3, one cold step (untimed) then one
+    timed warm step, single-threaded and with the transforms on `workers`
+    scipy.fft threads, extrapolated to 256^3 by R log R."""
+    workers = workers or min(16, os.cpu_count() or 1)
     n = 80
     r_s, r_t = n ** 3, 256 ** 3
     scale = (r_t * math.log(r_t)) / (r_s * math.log(r_s))
@@ -194,24 +229,89 @@ def cpu_baseline(workers=16):
     single = {"value": 1.0 / (w1 * scale), "cores": 1, "warm_step_s": w1, "cold_step_s": c1}
     threaded = {"value": 1.0 / (ww * scale), "cores": workers, "warm_step_s": ww, "cold_step_s": cw}
     best = threaded if threaded["value"] > single["value"] else single
-    return {"value": best["value"], "unit": "timesteps/s", "cores": best["cores"], "kind": "port",
-            "sample": f"oracle/step_ref.py FP64 numpy, 1 warm step at {n}^3 of the same geometry "
-                      f"({best['warm_step_s']:.2f} s, {it1} elastic updates; cold step {best['cold_step_s']:.2f} s "
-                      f"untimed), extrapolated to 256^3 by R log R (x{scale:.1f}); transforms on "
-                      f"{best['cores']} thread(s)",
-            "single_thread": single, "fft_threads": threaded,
-            "nproc": os.cpu_count(), "cpu_model": _cpu_model()}
+    small = {"value": best["value"], "cores": best["cores"],
+             "sample": f"oracle/step_ref.py FP64 numpy, 1 warm step at {n}^3 of the same geometry "
+                       f"({best['warm_step_s']:.2f} s, {it1} elastic updates; cold step {best['cold_step_s']:.2f} s "
+                       f"untimed), extrapolated to 256^3 by R log R (x{scale:.1f}); transforms on "
+                       f"{best['cores']} thread(s)",
+             "single_thread": single, "fft_threads": threaded}
+    out = {"unit": "timesteps/s", "kind": "port", "nproc": os.cpu_count(), "cpu_model": _cpu_model(),
+           "extrapolated_80cube": small}
+    if snap is not None:
+        t, it = cpu_baseline_256(pd, cap, snap, workers)
+        out.update({"value": 1.0 / t, "cores": workers, "warm_step_s": t, "elastic_updates": it,
+                    "sample": f"oracle/step_ref.py FP64 numpy, 1 warm step at 256^3 of the workload from the "
+                              f"FP64 GPU state after {snap['steps_done']} steps ({t:.1f} s, {it} elastic "
+                              f"updates); transforms on {workers} scipy.fft threads, point-wise work on one"})
+    else:
+        out.update({"value": small["value"], "cores": small["cores"], "sample": small["sample"]})
+    return out
+
+
+def accuracy_run(pf, idx, steps):
+    """Both paths in lockstep over `steps` steps of workload `idx`."""
+    from cuda_sto_pf_amd.configs import config
+    pd, phi, xB, meta = config(idx)
+    steps = steps or meta["steps"]
+    cap = meta["xB_max_safe"]
+    ref = pf.Simulation(pd, pf.PF_FP64)
+    fast = pf.Simulation(dict(pd, **ACCEL), pf.PF_MIXED)
+    ref.set_fields(phi, xB, xB_cap=cap)
+    fast.set_fields(phi, xB, xB_cap=cap)
+    m0 = ref.scalars()["mass_xBtot"]
+    rows = []
+    it_r = it_f = 0
+    t_r = t_f = 0.0
+    every = 100 if steps >= 100 else max(1, steps // 5)
+    for s in range(1, steps + 1):
+        t0 = time.perf_counter()
+        ref.step()
+        ref.sync()
+        t1 = time.perf_counter()
+        fast.step()
+        fast.sync()
+        t_r += t1 - t0
+        t_f += time.perf_counter() - t1
+        sr, sf = ref.scalars(), fast.scalars()
+        it_r += sr["last_elastic_iterations"]
+        it_f += sf["last_elastic_iterations"]
+        if s % every == 0 or s == steps:
+            xr, xf = ref.field(pf.PF_XB), fast.field(pf.PF_XB)
+            pr, pm = ref.field(pf.PF_PHI), fast.field(pf.PF_PHI)
+            row = {"step": s, "max_abs_dxB": float(np.abs(xr - xf).max()),
+                   "max_abs_dphi": float(np.abs(pr - pm).max()),
+                   "mass_drift_rel_fp64": abs(sr["mass_xBtot"] / m0 - 1.0),
+                   "mass_drift_rel_mixed": abs(sf["mass_xBtot"] / m0 - 1.0),
+                   "f_total_fp64": sr["f_total"], "f_total_rel_diff": abs(sf["f_total"] / sr["f_total"] - 1.0),
+                   "f_elastic_fp64": sr["f_elastic"],
+                   "f_elastic_rel_diff": abs(sf["f_elastic"] / sr["f_elastic"] - 1.0),
+                   "volume_fraction": sr["volume_fraction"],
+                   "elastic_updates_fp64": it_r / s, "elastic_updates_mixed": it_f / s}
+            rows.append(row)
+            print(f"[accuracy] workload {idx} step {s}: max|dxB|={row['max_abs_dxB']:.2e} "
+                  f"max|dphi|={row['max_abs_dphi']:.2e} mass drift {row['mass_drift_rel_fp64']:.1e}/"
+                  f"{row['mass_drift_rel_mixed']:.1e} dF/F={row['f_total_rel_diff']:.1e} "
+                  f"dFel/Fel={row['f_elastic_rel_diff']:.1e}", file=sys.stderr, flush=True)
+    return {"workload": meta["workload"], "grid": list(phi.shape), "steps": steps,
+            "fp64_reference_iteration": {"steps_per_s": steps / t_r, "elastic_updates_per_step": it_r / steps},
+            "mixed_accelerated": {"steps_per_s": steps / t_f, "elastic_updates_per_step": it_f / steps},
+            "gate_max_abs_dxB": 1e-5, "max_abs_dxB": max(r["max_abs_dxB"] for r in rows), "rows": rows}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 20; with --accuracy the workload's stated horizon)")
     ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-learned-eval", action="store_true",
                     help="skip the cold-start comparison with/without the network (profiling runs)")
+    ap.add_argument("--workload", type=int, default=5,
+                    help="workload of the run (5: the 256^3 headline; 2 and 3 for --accuracy and profiling)")
+    ap.add_argument("--accuracy", action="store_true",
+                    help="long-horizon FP64 vs mixed comparison of --workload (no throughput line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: measures one GPU only (--gpus {args.gpus} requested); "
@@ -221,7 +321,12 @@ def main():
     from cuda_sto_pf_amd import pfcore as pf
     from cuda_sto_pf_amd.configs import config
 
-    pd, phi, xB, meta = config(5)
+    if args.accuracy:
+        print(json.dumps(accuracy_run(pf, args.workload, args.steps)))
+        return
+    if args.steps is None:
+        args.steps = 20
+    pd, phi, xB, meta = config(args.workload)
     cap = meta["xB_max_safe"]
     print(f"[bench] workload {meta['workload']} {phi.shape} warmup={args.warmup} steps={args.steps}",
           file=sys.stderr, flush=True)
@@ -238,6 +343,7 @@ def main():
     if not args.no_fp64:
         sim_d, t_d, it_d, sc_d = run_path(pf, pd, phi, xB, cap, pf.PF_FP64, args.warmup, args.steps)
         xb_d = sim_d.field(pf.PF_XB)
+        snap = None if args.no_cpu else gpu_state_snapshot(pf, sim_d)
         max_dc = float(np.abs(xb_m - xb_d).max())
         result_fp64 = {"steps_per_s": args.steps / t_d, "ms_per_step": 1e3 * t_d / args.steps,
                        "elastic_updates_per_step": it_d,
@@ -268,7 +374,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline()
+        cpu = cpu_baseline(pd, cap, None if args.no_fp64 else snap)
         print(f"[bench] cpu baseline {cpu['value']:.4g} steps/s ({cpu['sample']})", file=sys.stderr, flush=True)
 
     value = args.steps / t_m
@@ -285,7 +391,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 transforms / f64 state, reductions, zero mode, residual",
         "data": "synthetic",
-        "config": {"workload": "256^3, 40 particles R~U(5,12), hetero elastic tol 1e-8, zero mode, dt 0.01",
+        "config": {"workload": ("256^3, 40 particles R~U(5,12), hetero elastic tol 1e-8, zero mode, dt 0.01"
+                                if args.workload == 5 else f"{meta['workload']} {phi.shape[0]}^3"),
                    "grid": list(phi.shape), "learned_warm_start": net is not None,
                    "path": "mixed precision, learned cold start + extrapolated warm start (order 3) + "
                            "Anderson depth 3"},
@@ -297,11 +404,9 @@ def main():
         "roofline": {"kernel": dom_name, "bound": "hbm", "achieved": dom["GBps"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": dom["GBps"] / HBM_PEAK_GBS, "traffic": pmc_traffic(dom_name),
                      "bytes_per_launch": dom["bytes"], "us_per_launch": dom["us"]},
-        "fft_byte_model": fft_byte_model(table, phi.shape[0]),
-        "step_byte_model": {
-            "mixed": step_byte_model(phi.shape[0], it_m, 1e3 * t_m / args.steps, False),
-            "fp64_plain": (step_byte_model(phi.shape[0], result_fp64["elastic_updates_per_step"],
-                                           result_fp64["ms_per_step"], True) if result_fp64 else None)},
+        "step_kernels": {
+            "mixed": step_kernels(table, 1e3 * t_m / args.steps),
+            "fp64_plain": step_kernels(table_d, result_fp64["ms_per_step"]) if result_fp64 else None},
         "learned_warm_start": learned_stats,
         "kernels_mixed": table,
         "kernels_fp64": table_d if not args.no_fp64 else None,

@@ -346,28 +346,38 @@ bool read(const std::string& path, const Provenance& ex, Checkpoint* out, std::s
 // ------------------------------------------------------------ exact-resume companion
 namespace {
 constexpr char kExactMagic[8] = {'P', 'F', 'A', 'M', 'D', 'X', 'S', '1'};
-constexpr std::uint32_t kExactHeader = 48;
+constexpr std::uint32_t kExactHeader = 48;    // version 1: warm state and history
+constexpr std::uint32_t kExactHeader2 = 56;   // version 2: + count of the phi* block
 }  // namespace
 
 bool write_exact(const std::string& path, const ExactState& s, std::string* err) {
   if (s.states.size() != (size_t)s.nstates * 6 * s.n) return set_err(err, "exact-resume state size mismatch");
+  const bool v2 = !s.phi_star.empty();
+  if (v2 && s.phi_spec.size() != 2 * s.n) return set_err(err, "exact-resume phi* spectrum size mismatch");
   Image h;
   h.put_text(std::string(kExactMagic, 8), 8);
-  h.put<std::uint32_t>(1);
-  h.put<std::uint32_t>(kExactHeader);
+  h.put<std::uint32_t>(v2 ? 2 : 1);
+  h.put<std::uint32_t>(v2 ? kExactHeader2 : kExactHeader);
   h.put<std::uint64_t>(s.bound_checksum);
   h.put<std::uint64_t>(s.accepted_step);
   h.put<std::int32_t>(s.precision);
   h.put<std::int32_t>(s.nstates);
   h.put<std::uint64_t>(s.n);
+  if (v2) h.put<std::uint64_t>(s.phi_star.size());
   std::uint64_t sum = fnv1a64(h.b.data(), h.b.size());
   sum = fnv1a64(s.states.data(), s.states.size() * sizeof(double), sum);
+  if (v2) {
+    sum = fnv1a64(s.phi_star.data(), s.phi_star.size() * sizeof(double), sum);
+    sum = fnv1a64(s.phi_spec.data(), s.phi_spec.size() * sizeof(double), sum);
+  }
   const std::string tmp = path + ".tmp";
   FILE* f = std::fopen(tmp.c_str(), "wb");
   if (!f) return set_err(err, "cannot open exact-resume temporary file: " + std::string(std::strerror(errno)));
-  bool ok = write_all(f, h.b.data(), h.b.size()) &&
-            write_all(f, s.states.data(), s.states.size() * sizeof(double)) && write_all(f, &sum, sizeof(sum)) &&
-            std::fflush(f) == 0;
+  bool ok = write_all(f, h.b.data(), h.b.size()) && write_all(f, s.states.data(), s.states.size() * sizeof(double));
+  if (v2)
+    ok = ok && write_all(f, s.phi_star.data(), s.phi_star.size() * sizeof(double)) &&
+         write_all(f, s.phi_spec.data(), s.phi_spec.size() * sizeof(double));
+  ok = ok && write_all(f, &sum, sizeof(sum)) && std::fflush(f) == 0;
   if (std::fclose(f) != 0) ok = false;
   if (!ok || std::rename(tmp.c_str(), path.c_str()) != 0) {
     std::remove(tmp.c_str());
@@ -382,26 +392,44 @@ bool read_exact(const std::string& path, ExactState* s, std::string* err) {
     if (err) err->clear();
     return false;
   }
-  unsigned char hb[kExactHeader];
-  bool ok = read_all(f, hb, kExactHeader) && std::memcmp(hb, kExactMagic, 8) == 0;
-  ExactState t;
+  unsigned char hb[kExactHeader2];
+  bool ok = read_all(f, hb, 16) && std::memcmp(hb, kExactMagic, 8) == 0;
+  std::uint32_t version = 0, hsize = 0;
   if (ok) {
     Cursor r{hb + 8};
-    ok = r.get<std::uint32_t>() == 1 && r.get<std::uint32_t>() == kExactHeader;
+    version = r.get<std::uint32_t>();
+    hsize = r.get<std::uint32_t>();
+    ok = (version == 1 && hsize == kExactHeader) || (version == 2 && hsize == kExactHeader2);
+  }
+  ok = ok && read_all(f, hb + 16, hsize - 16);
+  ExactState t;
+  std::uint64_t nphi = 0;
+  if (ok) {
+    Cursor r{hb + 16};
     t.bound_checksum = r.get<std::uint64_t>();
     t.accepted_step = r.get<std::uint64_t>();
     t.precision = r.get<std::int32_t>();
     t.nstates = r.get<std::int32_t>();
     t.n = r.get<std::uint64_t>();
-    ok = ok && t.nstates >= 1 && t.nstates <= 4 && t.n > 0 && t.n < (1ull << 40);
+    if (version == 2) nphi = r.get<std::uint64_t>();
+    ok = t.nstates >= 0 && t.nstates <= 4 && (t.nstates > 0 || nphi > 0) && t.n > 0 && t.n < (1ull << 40) &&
+         nphi < (1ull << 40);
   }
   std::uint64_t sum = 0, stored = 0;
   if (ok) {
     t.states.resize((size_t)t.nstates * 6 * t.n);
-    ok = read_all(f, t.states.data(), t.states.size() * sizeof(double)) && read_all(f, &stored, sizeof(stored));
+    t.phi_star.resize(nphi);
+    t.phi_spec.resize(nphi ? 2 * t.n : 0);
+    ok = read_all(f, t.states.data(), t.states.size() * sizeof(double)) &&
+         read_all(f, t.phi_star.data(), t.phi_star.size() * sizeof(double)) &&
+         read_all(f, t.phi_spec.data(), t.phi_spec.size() * sizeof(double)) && read_all(f, &stored, sizeof(stored));
     unsigned char extra;
     if (ok && std::fread(&extra, 1, 1, f) != 0) ok = false;
-    sum = fnv1a64(t.states.data(), t.states.size() * sizeof(double), fnv1a64(hb, kExactHeader));
+    sum = fnv1a64(t.states.data(), t.states.size() * sizeof(double), fnv1a64(hb, hsize));
+    if (nphi) {
+      sum = fnv1a64(t.phi_star.data(), t.phi_star.size() * sizeof(double), sum);
+      sum = fnv1a64(t.phi_spec.data(), t.phi_spec.size() * sizeof(double), sum);
+    }
   }
   std::fclose(f);
   if (!ok || sum != stored) return set_err(err, "exact-resume state is corrupt");

@@ -77,6 +77,9 @@ struct ExactState {
   std::int32_t nstates = 0;           // 1 + history length
   std::uint64_t n = 0;                // complex values per component (compact layout)
   std::vector<double> states;         // nstates * 3 * n complex (re, im): current, then history newest first
+  // mixed precision: the unclamped phi* of the phi update ([Nx][Ny][Nz]) and
+  // its retained spectrum (n complex), or empty
+  std::vector<double> phi_star, phi_spec;
 };
 bool write_exact(const std::string& path, const ExactState& s, std::string* err);
 // false with an empty error when the file does not exist

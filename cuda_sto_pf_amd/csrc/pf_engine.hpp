@@ -854,6 +854,11 @@ struct Engine final : EngineBase {
   int AzS = 0;
   // device memory
   double *phi[2] = {nullptr, nullptr}, *Y = nullptr, *Ystar = nullptr, *xB = nullptr, *dYdt = nullptr;
+  // mixed precision: the unclamped band-limited phi* (real, FP64) and its
+  // retained spectrum (compact, FP64), so that the FP32 transforms of the
+  // phi update carry its increment only (kop_phi_state)
+  double* phis = nullptr;
+  cx<double>* Phik = nullptr;
   T* trs = nullptr;
   C *Sz = nullptr, *Sy = nullptr, *Sc = nullptr, *u = nullptr, *un = nullptr;
   cx<double>* Ud = nullptr;   // FP64 accumulated displacement (accelerated solver)
@@ -881,6 +886,13 @@ struct Engine final : EngineBase {
   // further steps until it is restored) and by the measurement entry points
   // that overwrite the buffers.
   bool state_ok = false;
+  // Y is band-limited: it is the inverse transform of a dealiased spectrum
+  // plus the zero-mode shift (false for fields from pf_set_fields)
+  bool y_banded = false;
+  // phis / Phik hold the unclamped phi* of the current phi (mixed precision;
+  // false for fields from pf_set_fields, after minimize and after a restart
+  // without the exact-resume companion's phi* block)
+  bool phi_banded = false;
   int require_state(const char* what) const {
     if (state_ok) return PF_OK;
     return fail(PF_E_ARG, std::string(what) +
@@ -993,6 +1005,7 @@ struct Engine final : EngineBase {
       const bool accel = p.elastic_enabled && (p.elastic_warm_start_enabled || p.elastic_residual_control_enabled);
       extended = accel && (kMixed || p.elastic_warm_start_order > 1 || p.elastic_anderson_depth > 0);
       if ((kMixed || extended) && (rc = dalloc(&Ud, 3 * nSc))) return rc;
+      if (kMixed && ((rc = dalloc(&phis, R)) || (rc = dalloc(&Phik, nSc)))) return rc;
       if (extended) {
         if ((rc = dalloc(&Gd, 3 * nSc)) || (rc = dalloc(&Hist, 9 * nSc)) || (rc = dalloc(&dinc, 3 * nSc)) ||
             (rc = dalloc(&dU, (size_t)AA_MAX * 3 * nSc)) || (rc = dalloc(&dG, (size_t)AA_MAX * 3 * nSc)) ||
@@ -1140,6 +1153,8 @@ struct Engine final : EngineBase {
       if (dYdt0) dd[i] = dYdt0[i];
     }
     state_ok = false;
+    y_banded = false;
+    phi_banded = false;
     cur = 0;
     PF_HIP(hipMemcpy(phi[0], ph.data(), R * sizeof(double), hipMemcpyHostToDevice));
     PF_HIP(hipMemcpy(xB, xb.data(), R * sizeof(double), hipMemcpyHostToDevice));
@@ -1591,14 +1606,18 @@ struct Engine final : EngineBase {
     egrad_pending = false;
     eng_pending = false;
     sig6_valid = false;
+    // mixed precision: 1 = first update from new fields (direct, sets phi*),
+    // 2 = increment form; 0 in FP64
+    const int phi_mode = kMixed ? (phi_banded ? 2 : 1) : 0;
     if (P.elastic) {
       if ((rc = elastic_solve())) return rc;
-      typename OpPhiRhs<T, true>::Args oa{P, phi_old, xB, trs, invN, keep_sig6 ? sig6 : nullptr, (size_t)R};
+      typename OpPhiRhs<T, true>::Args oa{P, phi_old, xB, trs, invN, keep_sig6 ? sig6 : nullptr, (size_t)R,
+                                          phi_mode == 2 ? phis : nullptr};
       ZPassArgs<T> za = zargs(0, 6, 0, 1);
       za.partials = epartials;
       eng_g = launch_zpass<T, 6, 1, OpPhiRhs<T, true>>(Nz, stream, za, oa);
     } else {
-      typename OpPhiRhs<T, false>::Args oa{P, phi_old, xB, trs, invN};
+      typename OpPhiRhs<T, false>::Args oa{P, phi_old, xB, trs, invN, nullptr, 0, phi_mode == 2 ? phis : nullptr};
       ZPassArgs<T> za = zargs(0, 0, 0, 1);
       za.partials = epartials;
       eng_g = launch_zpass<T, 0, 1, OpPhiRhs<T, false>>(Nz, stream, za, oa);
@@ -1608,13 +1627,17 @@ struct Engine final : EngineBase {
     // phi semi-implicit update
     ypass(false, 0, 1);
     xpass(false, 0, 1, Sc);
-    hipLaunchKernelGGL(kop_phi<T>, dim3(gmodes()), dim3(NT), 0, stream, d, nSc, sc(0),
-                       prm.L_phi * prm.dt * prm.kappa_phi);
+    if (phi_mode)
+      hipLaunchKernelGGL(kop_phi_state<T>, dim3(gmodes()), dim3(NT), 0, stream, d, nSc, sc(0),
+                         prm.L_phi * prm.dt * prm.kappa_phi, Phik, phi_mode);
+    else
+      hipLaunchKernelGGL(kop_phi<T>, dim3(gmodes()), dim3(NT), 0, stream, d, nSc, sc(0),
+                         prm.L_phi * prm.dt * prm.kappa_phi);
     xpass(true, 0, 1, Sc);
     ypass(true, 0, 1);
     {
       const int phiY_old = (zm || prm.enable_Y_rhs_previous_time_level) ? 1 : 0;
-      typename OpPhiFinish<T>::Args oa{P, phi_new, phi_old, Y, xB, trs, invN, phiY_old};
+      typename OpPhiFinish<T>::Args oa{P, phi_new, phi_old, Y, xB, trs, invN, phiY_old, phis, phi_mode};
       const int g = launch_zpass<T, 1, 1, OpPhiFinish<T>>(Nz, stream, zargs(0, 1, 1, 1), oa);
       finalize(g, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 4, 1.0 / (double)R);
     }
@@ -1624,21 +1647,36 @@ struct Engine final : EngineBase {
     hipLaunchKernelGGL(kop_grad<T>, dim3(gmodes()), dim3(NT), 0, stream, d, nSc, sc(1), sc(2), sc(3), sc(4));
     xpass(true, 2, 3, sc(2));
     ypass(true, 2, 3);
+    // Mixed precision with the zero mode updates Y by its increment once Y^n
+    // is band-limited (after the first step): the FP32 transforms then carry
+    // dt (div J - s) / (1 + dt D-bar k^2) rather than Y itself, so their
+    // rounding scales with the increment (kop_yupdate), and Y^n needs no
+    // transform.
+    const bool yinc = kMixed && zm && y_banded;
     {
       const double* phiY = (zm || prm.enable_Y_rhs_previous_time_level) ? phi_old : phi_new;
       typename OpFlux<T>::Args oa{P,    phiY, xB,  phi_old, phi_new, Y, dYdt, invN, zm ? 1 : 0,
                                   prm.enable_Y_rhs_previous_time_level, prm.disable_Y_rhs_gamma_term,
-                                  prm.Y_rhs_term_h_scale};
-      launch_zpass<T, 3, 5, OpFlux<T>>(Nz, stream, zargs(2, 3, 0, 5), oa);
+                                  prm.Y_rhs_term_h_scale, yinc ? 1 : 0};
+      if constexpr (kMixed) {
+        if (yinc)
+          launch_zpass<T, 3, 4, OpFlux<T>>(Nz, stream, zargs(2, 3, 0, 4), oa);
+        else
+          launch_zpass<T, 3, 5, OpFlux<T>>(Nz, stream, zargs(2, 3, 0, 5), oa);
+      } else {
+        launch_zpass<T, 3, 5, OpFlux<T>>(Nz, stream, zargs(2, 3, 0, 5), oa);
+      }
     }
-    ypass(false, 0, 5);
-    xpass(false, 0, 5, Sc);
-    hipLaunchKernelGGL(kop_yupdate<T>, dim3(gmodes()), dim3(NT), 0, stream, d, nSc, sc(0), sc(1), sc(2), sc(3),
-                       sc(4), sc(5), (const double*)(dscal + 4), prm.dt);
+    const int nyc = yinc ? 4 : 5;
+    ypass(false, 0, nyc);
+    xpass(false, 0, nyc, Sc);
+    hipLaunchKernelGGL(kop_yupdate<T>, dim3(gmodes()), dim3(NT), 0, stream, d, nSc, sc(0), sc(1), sc(2),
+                       yinc ? (const C*)nullptr : (const C*)sc(3), yinc ? sc(3) : sc(4), sc(5),
+                       (const double*)(dscal + 4), prm.dt);
     xpass(true, 5, 1, sc(5));
     ypass(true, 5, 1);
     if (zm) {
-      typename OpYStar<T>::Args oa{Ystar, invN};
+      typename OpYStar<T>::Args oa{Ystar, invN, yinc ? (const double*)Y : nullptr};
       const int g = launch_zpass<T, 1, 0, OpYStar<T>>(Nz, stream, zargs(5, 1, 0, 0), oa);
       finalize(g, 2, RedKinds{{RED_MIN, RED_MAX, 0, 0}}, dscal, 1.0);
       // the step's scalars come back with the zero-mode search state
@@ -1656,6 +1694,8 @@ struct Engine final : EngineBase {
     eng_pending = true;
     sig6_valid = keep_sig6 && P.elastic;
     ++steps_done;
+    y_banded = true;
+    phi_banded = kMixed;
     state_ok = true;
     return PF_OK;
   }
@@ -1877,6 +1917,7 @@ struct Engine final : EngineBase {
     }
     mopt = o;
     minimizing = true;
+    phi_banded = false;   // minimize moves phi outside the dynamics update
     m_lambda = 0.0;
     m_V0 = o.V0;
     m_iter = 0;
@@ -2323,20 +2364,30 @@ struct Engine final : EngineBase {
     std::string err;
     if (!ckpt::write(path, c, &err))
       return fail(PF_E_IO, "PF zero-mode checkpoint write failed at step " + std::to_string(steps_done) + ": " + err);
-    if (c.elastic_present) {
-      // exact-resume companion: the FP64 warm state and the extrapolation history
+    const bool phi_block = kMixed && phi_banded;
+    if (c.elastic_present || phi_block) {
+      // exact-resume companion: the FP64 warm state and the extrapolation
+      // history, and in mixed precision the unclamped phi* with its spectrum
       ckpt::ExactState x;
       x.bound_checksum = c.payload_checksum;
       x.accepted_step = c.accepted_step;
       x.precision = kMixed ? PF_MIXED : PF_FP64;
       x.n = (std::uint64_t)nSc;
       const int nh = extended ? hist_count : 0;
-      x.nstates = 1 + nh;
+      x.nstates = c.elastic_present ? 1 + nh : 0;
       x.states.resize((size_t)x.nstates * 6 * nSc);
-      std::memcpy(x.states.data(), w.data(), 3 * nSc * sizeof(cx<double>));
-      for (int k = 0; k < nh; ++k)
-        PF_HIP(hipMemcpy(x.states.data() + (size_t)(1 + k) * 6 * nSc, hist_slot((hist_head + 2 - k) % 3),
-                         3 * nSc * sizeof(cx<double>), hipMemcpyDeviceToHost));
+      if (c.elastic_present) {
+        std::memcpy(x.states.data(), w.data(), 3 * nSc * sizeof(cx<double>));
+        for (int k = 0; k < nh; ++k)
+          PF_HIP(hipMemcpy(x.states.data() + (size_t)(1 + k) * 6 * nSc, hist_slot((hist_head + 2 - k) % 3),
+                           3 * nSc * sizeof(cx<double>), hipMemcpyDeviceToHost));
+      }
+      if (phi_block) {
+        x.phi_star.resize(R);
+        x.phi_spec.resize(2 * (size_t)nSc);
+        PF_HIP(hipMemcpy(x.phi_star.data(), phis, R * sizeof(double), hipMemcpyDeviceToHost));
+        PF_HIP(hipMemcpy(x.phi_spec.data(), Phik, nSc * sizeof(cx<double>), hipMemcpyDeviceToHost));
+      }
       if (!ckpt::write_exact(std::string(path) + ".state", x, &err)) return fail(PF_E_IO, err);
     }
     return PF_OK;
@@ -2363,10 +2414,13 @@ struct Engine final : EngineBase {
     // device state changes
     ckpt::ExactState x;
     bool have_x = false;
-    if (el) {
+    if (el || kMixed) {
       have_x = ckpt::read_exact(std::string(path) + ".state", &x, &err);
       if (!have_x && !err.empty()) return fail(PF_E_IO, err);
     }
+    // the companion belongs to this file and this handle
+    const bool x_ok = have_x && x.bound_checksum == c.payload_checksum && x.accepted_step == c.accepted_step &&
+                      x.n == (std::uint64_t)nSc && x.precision == (kMixed ? PF_MIXED : PF_FP64);
     PF_HIP(hipStreamSynchronize(stream));
     // from here on a rejection leaves the handle without a valid state
     state_ok = false;
@@ -2395,6 +2449,7 @@ struct Engine final : EngineBase {
     zm_iters = (int)c.last_iterations;
     zm_accepted = c.accepted_zero_mode_steps;
     steps_done = (long long)c.accepted_step;
+    y_banded = c.accepted_step > 0;   // the state of an accepted step
     elastic_total = 0;
     egrad_pending = false;
     eng_pending = false;
@@ -2403,6 +2458,14 @@ struct Engine final : EngineBase {
     warm_valid = false;
     hist_count = 0;
     hist_head = 0;
+    // mixed precision: phi* and its spectrum from the companion continue the
+    // increment form bit for bit; without them the next update starts it anew
+    phi_banded = false;
+    if (kMixed && x_ok && x.phi_star.size() == (size_t)R && x.phi_spec.size() == 2 * (size_t)nSc) {
+      PF_HIP(hipMemcpy(phis, x.phi_star.data(), R * sizeof(double), hipMemcpyHostToDevice));
+      PF_HIP(hipMemcpy(Phik, x.phi_spec.data(), nSc * sizeof(cx<double>), hipMemcpyHostToDevice));
+      phi_banded = true;
+    }
     if (!el) {
       state_ok = true;
       return PF_OK;
@@ -2411,8 +2474,7 @@ struct Engine final : EngineBase {
     last_res = c.elastic_last_relative_residual;
     std::vector<cx<double>> cur_state(3 * nSc);
     int nh = 0;
-    if (have_x && x.bound_checksum == c.payload_checksum && x.accepted_step == c.accepted_step &&
-        x.n == (std::uint64_t)nSc && x.precision == (kMixed ? PF_MIXED : PF_FP64)) {
+    if (x_ok && x.nstates >= 1) {
       std::memcpy(cur_state.data(), x.states.data(), 3 * nSc * sizeof(cx<double>));
       nh = extended ? std::min(x.nstates - 1, 3) : 0;
       for (int k = 0; k < nh; ++k)   // newest first -> ring slots nh-1, ..., 0
@@ -2595,13 +2657,23 @@ struct Engine final : EngineBase {
         return (3.0 * m + 9.0) * nScA * cs + 3.0 * nScA * 16.0;
       }
       case PF_K_ZPASS_PHIRHS: {
-        typename OpPhiRhs<T, true>::Args oa{P, phi[cur], xB, trs, invN};
+        // the form the step runs: in mixed precision phi* is read as well
+        typename OpPhiRhs<T, true>::Args oa{P, phi[cur], xB, trs, invN, nullptr, 0, kMixed ? phis : nullptr};
         launch_zpass<T, 6, 1, OpPhiRhs<T, true>>(Nz, stream, zargs(0, 6, 0, 1), oa);
-        return 7.0 * nSzA * cs + 2.0 * R * 8.0 + R * (double)sizeof(T);
+        return 7.0 * nSzA * cs + (kMixed ? 3.0 : 2.0) * R * 8.0 + R * (double)sizeof(T);
       }
       case PF_K_ZPASS_FLUX: {
         const double* ph = phi[cur];
-        typename OpFlux<T>::Args oa{P, ph, xB, ph, phi[1 - cur], Y, dYdt, invN, 1, 0, 0, 1.0};
+        // the form the step runs: (J, s) in mixed precision with the zero mode
+        // after the first step, (J, Y^n, s) otherwise
+        const bool inc = kMixed && prm.zero_mode;
+        typename OpFlux<T>::Args oa{P, ph, xB, ph, phi[1 - cur], Y, dYdt, invN, 1, 0, 0, 1.0, inc ? 1 : 0};
+        if constexpr (kMixed) {
+          if (inc) {
+            launch_zpass<T, 3, 4, OpFlux<T>>(Nz, stream, zargs(0, 3, 0, 4), oa);
+            return 7.0 * nSzA * cs + 6.0 * R * 8.0;
+          }
+        }
         launch_zpass<T, 3, 5, OpFlux<T>>(Nz, stream, zargs(0, 3, 0, 5), oa);
         return 8.0 * nSzA * cs + 6.0 * R * 8.0;
       }

@@ -149,13 +149,18 @@ struct OpPhiRhs {
     T invN;
     T* sig6 = nullptr;   // optional out: the 6 stress components (xx,yy,zz,yz,xz,xy), stride n
     size_t n = 0;
+    // mixed precision, increment form: the unclamped band-limited phi* of the
+    // last update; the transform then carries the clamp correction phi - phi*
+    // minus L dt (dF/dphi) (null: phi - L dt (dF/dphi))
+    const double* phistar = nullptr;
   };
   static constexpr int NRED = 3;
   static constexpr int kinds[3] = {RED_SUM, RED_SUM, RED_SUM};
-  static constexpr int NPRE = 2;
+  static constexpr int NPRE = sizeof(T) == 4 ? 3 : 2;
   __device__ static void load(const Args& a, size_t g, double* pre) {
     pre[0] = a.phi[g];
     pre[1] = a.xB[g];
+    if constexpr (NPRE > 2) pre[2] = a.phistar ? a.phistar[g] : 0.0;
   }
   __device__ static void apply(const Args& a, size_t g, const double* pre, const T* vin, T* out, double* red) {
     const PhysParams& P = a.P;
@@ -211,6 +216,12 @@ struct OpPhiRhs {
       red[0] += 0.5 * double(sig[0] * el[0] + sig[1] * el[1] + sig[2] * el[2] +
                              T(2) * (sig[5] * el[5] + sig[4] * el[4] + sig[3] * el[3]));
     }
+    if constexpr (NPRE > 2) {
+      if (a.phistar) {
+        out[0] = T((phi - pre[2]) - (P.L_phi * P.dt) * f);
+        return;
+      }
+    }
     out[0] = T(phi - (P.L_phi * P.dt) * f);
   }
 };
@@ -229,18 +240,27 @@ struct OpPhiFinish {
     const T* trs;
     T invN;
     int phiY_is_old;   // zero mode / previous-time-level: mu uses phi^n
+    // mixed precision: the unclamped phi* (1: set it to this update's value;
+    // 2: the transform holds the increment, phi* += inv/N)
+    double* phistar = nullptr;
+    int phi_state = 0;
   };
   static constexpr int NRED = 1;
   static constexpr int kinds[1] = {RED_SUM};
-  static constexpr int NPRE = 3;
+  static constexpr int NPRE = sizeof(T) == 4 ? 4 : 3;
   __device__ static void load(const Args& a, size_t g, double* pre) {
     pre[0] = a.phiY_is_old ? a.phi_old[g] : 0.0;
     pre[1] = a.Y[g];
     pre[2] = a.P.elastic ? double(a.trs[g]) : 0.0;
+    if constexpr (NPRE > 3) pre[3] = a.phi_state == 2 ? a.phistar[g] : 0.0;
   }
   __device__ static void apply(const Args& a, size_t g, const double* pre, const T* vin, T* out, double* red) {
     const PhysParams& P = a.P;
     double v = double(vin[0]) * double(a.invN);
+    if constexpr (NPRE > 3) {
+      if (a.phi_state == 2) v += pre[3];
+      if (a.phi_state) a.phistar[g] = v;
+    }
     if (v < -1e-6) v = -1e-6;
     if (v > 1.0 + 1e-6) v = 1.0 + 1e-6;
     a.phi_new[g] = v;
@@ -295,6 +315,7 @@ struct OpFlux {
     int prev_level;      // enable_Y_rhs_previous_time_level
     int no_gamma;        // disable_Y_rhs_gamma_term
     double term_h_scale;
+    int increment;       // emit (J, s) only: the update forms the increment of Y (NCO = 4)
   };
   static constexpr int NRED = 0;
   static constexpr int kinds[1] = {0};
@@ -338,25 +359,31 @@ struct OpFlux {
       const double term_gamma = a.no_gamma ? 0.0 : gamma_local * pre[5];
       s = term_h + term_gamma;
     }
-    out[3] = T(Y);
-    out[4] = T(s);
+    if (a.increment) {
+      out[3] = T(s);
+    } else {
+      out[3] = T(Y);
+      out[4] = T(s);
+    }
   }
 };
 
 // Y* = inv/N before the zero-mode shift (copy_scaled_pf_Y_kernel
-// cuda_kernels.cu:3401-3407), with its min/max for the bracket.
+// cuda_kernels.cu:3401-3407), with its min/max for the bracket.  With Ybase
+// the transform holds the increment of the step and Y* = Y^n + inv/N.
 template <typename T>
 struct OpYStar {
   struct Args {
     double* Ystar;
     T invN;
+    const double* Ybase;   // Y^n (increment form) or null
   };
   static constexpr int NRED = 2;
   static constexpr int kinds[2] = {RED_MIN, RED_MAX};
-  static constexpr int NPRE = 0;
-  __device__ static void load(const Args&, size_t, double*) {}
-  __device__ static void apply(const Args& a, size_t g, const double*, const T* vin, T*, double* red) {
-    const double y = double(vin[0]) * double(a.invN);
+  static constexpr int NPRE = 1;
+  __device__ static void load(const Args& a, size_t g, double* pre) { pre[0] = a.Ybase ? a.Ybase[g] : 0.0; }
+  __device__ static void apply(const Args& a, size_t g, const double* pre, const T* vin, T*, double* red) {
+    const double y = pre[0] + double(vin[0]) * double(a.invN);
     a.Ystar[g] = y;
     red[0] = fmin(red[0], y);
     red[1] = fmax(red[1], y);
@@ -971,6 +998,39 @@ __global__ void __launch_bounds__(NT) kop_strain(StrainArgs<T, UT> sa) {
   }
 }
 
+// Mixed-precision phi update with the FP64 spectrum Phi of the unclamped,
+// band-limited phi*: mode 1 (first update after new fields) phi^ = g^ / den
+// and Phi = phi^; mode 2 g^ holds the transform of the clamp correction
+// minus L dt (dF/dphi), and the increment
+//   d^ = (g^ - L dt kappa k^2 Phi) / den,   den = 1 + L dt kappa k^2
+// is written (Phi += d^).  With phi* = inverse(Phi) that is the reference
+// update (phi_semi_implicit_update_kernel, cuda_kernels.cu:821-843), and the
+// transforms carry only the increment.
+template <typename T>
+__global__ void __launch_bounds__(NT) kop_phi_state(KDims d, long n, cx<T>* g, double Ldt_kappa, cx<double>* Phi,
+                                                    int mode) {
+  const long nact_ = (n / d.Az) * d.Azr;   // retained modes only (no kz padding)
+  for (long j_ = blockIdx.x * (long)NT + threadIdx.x; j_ < nact_; j_ += (long)gridDim.x * NT) {
+    const long c = kslot(d, j_);
+    double kx, ky, kz;
+    int iz;
+    kvec(d, c, kx, ky, kz, iz);
+    const double k2 = kx * kx + ky * ky + kz * kz;
+    const double den = 1.0 + Ldt_kappa * k2;
+    const cx<T> v = g[c];
+    if (mode == 2) {
+      const cx<double> F = Phi[c];
+      const cx<double> dd = {(double(v.re) - Ldt_kappa * k2 * F.re) / den, (double(v.im) - Ldt_kappa * k2 * F.im) / den};
+      Phi[c] = cx<double>{F.re + dd.re, F.im + dd.im};
+      g[c] = cx<T>{T(dd.re), T(dd.im)};
+    } else {
+      const cx<double> w = {double(v.re) / den, double(v.im) / den};
+      Phi[c] = w;
+      g[c] = cx<T>{T(w.re), T(w.im)};
+    }
+  }
+}
+
 // phi^ <- g^ / (1 + L dt kappa k^2)
 template <typename T>
 __global__ void __launch_bounds__(NT) kop_phi(KDims d, long n, cx<T>* g, double Ldt_kappa) {
@@ -1007,6 +1067,13 @@ __global__ void __launch_bounds__(NT) kop_grad(KDims d, long n, const cx<T>* mu,
 // Y^ <- (Y^n + dt f^)/(1 + dt Dbar k^2),  f^ = i k.J^ + Dbar k^2 Y^n - s^
 // (Y_semi_implicit_update_kernel 2528-2550 with divJ_accumulate 2112-2137
 // and the spectral Laplacian 3528-3538 folded in)
+// Y_semi_implicit_update_kernel (cuda_kernels.cu:2528-2550) with the
+// right-hand side f = div J + D-bar k^2 Y^n - s assembled in k space.  With
+// yn == null it writes the increment Y^(n+1) - Y^n instead, in which the
+// stabiliser cancels: dt (div J - s) / (1 + dt D-bar k^2).  For a band-limited
+// Y^n (every state after the first step; the zero-mode shift is the k = 0
+// mode) that is the same update, and its FP32 rounding scales with the
+// increment instead of with Y.
 template <typename T>
 __global__ void __launch_bounds__(NT) kop_yupdate(KDims d, long n, const cx<T>* jx,
                                                   const cx<T>* jy, const cx<T>* jz,
@@ -1028,10 +1095,14 @@ __global__ void __launch_bounds__(NT) kop_yupdate(KDims d, long n, const cx<T>* 
     di += ky * double(jy[c].re);
     dr += -kz * double(jz[c].im);
     di += kz * double(jz[c].re);
+    const double den = 1.0 + dt * D * k2;
+    if (!yn) {
+      out[c] = cx<T>{T(dt * (dr - double(s[c].re)) / den), T(dt * (di - double(s[c].im)) / den)};
+      continue;
+    }
     const double ynr = double(yn[c].re), yni = double(yn[c].im);
     const double fr = dr + D * k2 * ynr - double(s[c].re);
     const double fi = di + D * k2 * yni - double(s[c].im);
-    const double den = 1.0 + dt * D * k2;
     out[c] = cx<T>{T((ynr + dt * fr) / den), T((yni + dt * fi) / den)};
   }
 }

@@ -7,8 +7,11 @@
   cfg2_16_10.npz    workload 2 geometry at 16^3 (heterogeneous, tol 1e-8,
                     zero mode), 10 steps: full phi, Y, x_B, dY/dt fields and the
                     per-step elastic iteration counts.
+  cfg2_32_100.npz   workload 2 geometry at 32^3 (heterogeneous, tol 1e-8, zero
+                    mode), 100 steps: full phi and Y fields, x_B checksums, the
+                    per-step elastic iteration counts, mass and energy densities.
 
-Run:  python tests/golden/make_golden.py   (about two minutes)
+Run:  python tests/golden/make_golden.py [names...]   (a few minutes)
 """
 import os
 import sys
@@ -63,6 +66,22 @@ def cfg2_small():
     print("cfg2_16_10 iterations", iters)
 
 
+def cfg2_32():
+    pd, phi, xB, meta = config(2, 32)
+    o = Oracle(Params(**pd), phi, xB, xB_cap=meta["xB_max_safe"])
+    iters = []
+    t = time.time()
+    for s in range(100):
+        o.step()
+        iters.append(o.last_elastic_iterations)
+    np.savez_compressed(os.path.join(HERE, "cfg2_32_100.npz"), phi=o.phi, Y=o.Y, xB_sums=checksums(o.xB),
+                        iters=np.array(iters), mass=o.mass(), target_mass=o.target_mass,
+                        energies=np.array([o.last_energies[k] for k in ("grad", "dw", "elastic", "chem")]))
+    print(f"cfg2_32_100: {time.time() - t:.1f}s, iterations {iters[:5]}...{iters[-3:]}")
+
+
+JOBS = {"cfg2_16_10": cfg2_small, "cfg1_64_100": cfg1, "cfg2_32_100": cfg2_32}
+
 if __name__ == "__main__":
-    cfg2_small()
-    cfg1()
+    for name in (sys.argv[1:] or list(JOBS)):
+        JOBS[name]()

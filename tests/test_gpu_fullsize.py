@@ -47,3 +47,31 @@ def test_fullsize_mixed_vs_fp64(pf, idx, steps):
     assert abs(mr - m0) <= 1e-12 * m0
     assert abs(mf - m0) <= 1e-12 * m0
     assert it_f < it_r
+
+
+@pytest.mark.gpu
+def test_workload2_stated_horizon(pf):
+    """Workload 2 over its stated 500 steps at 128^3: the mixed accelerated path
+    stays within the 1e-5 gate of the FP64 reference iteration in x_B, both
+    hold the mass to rounding, and the free-energy and elastic-energy
+    densities agree (the mixed path updates Y by its increment, so FP32
+    rounding does not accumulate in Y)."""
+    pd, phi, xB, meta = config(2)
+    cap = meta["xB_max_safe"]
+    ref = pf.Simulation(pd, pf.PF_FP64)
+    fast = pf.Simulation(dict(pd, **ACCEL), pf.PF_MIXED)
+    ref.set_fields(phi, xB, xB_cap=cap)
+    fast.set_fields(phi, xB, xB_cap=cap)
+    m0 = ref.scalars()["mass_xBtot"]
+    for block in range(5):
+        ref.step(100)
+        fast.step(100)
+        dx = np.abs(ref.field(pf.PF_XB) - fast.field(pf.PF_XB)).max()
+        sr, sf = ref.scalars(), fast.scalars()
+        print(f"workload 2 step {100 * (block + 1)}: max|dxB|={dx:.2e} "
+              f"dF/F={abs(sf['f_total'] / sr['f_total'] - 1):.1e} "
+              f"dFel/Fel={abs(sf['f_elastic'] / sr['f_elastic'] - 1):.1e}", flush=True)
+        assert dx < 1e-5
+    assert abs(sr["mass_xBtot"] - m0) <= 1e-12 * m0 and abs(sf["mass_xBtot"] - m0) <= 1e-12 * m0
+    assert abs(sf["f_total"] / sr["f_total"] - 1) < 1e-6
+    assert abs(sf["f_elastic"] / sr["f_elastic"] - 1) < 1e-3

@@ -60,3 +60,31 @@ def test_cfg2_16_10_steps(pf):
     for name, fid in (("phi", pf.PF_PHI), ("Y", pf.PF_Y), ("xB", pf.PF_XB), ("dYdt", pf.PF_DYDT)):
         d = np.abs(sim.field(fid) - g[name]).max()
         assert d < TOL * (1e2 if name == "dYdt" else 1.0), (name, d)
+
+
+@pytest.mark.gpu
+def test_cfg2_32_100_steps(pf):
+    """Heterogeneous elasticity over 100 steps at 32^3 (workload 2 geometry):
+    the full phi and Y fields within the FP64 gate of 1e-10, identical
+    elastic iteration counts every step, x_B checksums, mass to rounding and
+    the energy densities of the last step."""
+    g = np.load(os.path.join(HERE, "cfg2_32_100.npz"))
+    pd, phi, xB, meta = config(2, 32)
+    sim = pf.Simulation(pd, pf.PF_FP64)
+    sim.set_fields(phi, xB, xB_cap=meta["xB_max_safe"])
+    iters = []
+    for s in range(100):
+        sim.step()
+        iters.append(sim.scalars()["last_elastic_iterations"])
+    assert iters == list(g["iters"])
+    dphi = np.abs(sim.field(pf.PF_PHI) - g["phi"]).max()
+    dY = np.abs(sim.field(pf.PF_Y) - g["Y"]).max()
+    print(f"cfg2 32^3 x100: max|dphi|={dphi:.3e} max|dY|={dY:.3e}", flush=True)
+    assert dphi < TOL and dY < TOL
+    x = sim.field(pf.PF_XB)
+    sums = np.array([x.sum(), (x * x).sum(), x.min(), x.max()])
+    assert np.all(np.abs(sums - g["xB_sums"]) <= TOL * np.maximum(1.0, np.abs(g["xB_sums"])) * 8), (sums, g["xB_sums"])
+    sc = sim.scalars()
+    assert abs(sc["mass_xBtot"] - float(g["target_mass"])) <= 1e-12 * abs(float(g["target_mass"]))
+    e = np.array([sc["f_grad"], sc["f_dw"], sc["f_elastic"], sc["f_chem"]])
+    assert np.allclose(e, g["energies"], rtol=1e-10, atol=1e-14)
