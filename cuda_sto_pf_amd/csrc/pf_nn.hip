@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -25,6 +26,32 @@ namespace nn {
 static_assert(P_TOTAL == PF_NN_PARAMS, "parameter count of the C ABI");
 
 // Activations of one forward/backward pass over a batch of grids.
+// Static scratch of pf_nn_forward_noalloc: [3][NOALLOC_MAX_VOXELS][CH] BF16
+// (packed input, then the two activation planes), 24 MiB held by each GPU the
+// library runs on until the process ends.  Host code uses the per-device
+// address from hipGetSymbolAddress, never the array's name.
+constexpr long NOALLOC_MAX_VOXELS = 64L * 64 * 64;
+static __device__ __align__(256) uint16_t scratch[3 * NOALLOC_MAX_VOXELS * CH];
+
+constexpr int MAX_DEVICES = 64;
+static std::mutex scratch_mu;
+static uint16_t* scratch_ptr[MAX_DEVICES] = {};
+
+// The device address of `scratch` on `device` (which must be current),
+// looked up once per device.
+static int scratch_address(int device, uint16_t** out) {
+  if (device < 0 || device >= MAX_DEVICES) return fail(PF_E_ARG, "pf_nn: device index out of range");
+  std::lock_guard<std::mutex> lk(scratch_mu);
+  if (!scratch_ptr[device]) {
+    void* p = nullptr;
+    NN_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(PF_E_HIP, "pf_nn: no device address for the static scratch");
+    scratch_ptr[device] = static_cast<uint16_t*>(p);
+  }
+  *out = scratch_ptr[device];
+  return PF_OK;
+}
+
 struct Buffers {
   Geom g{};
   uint16_t* act[LAYERS] = {};   // act[0] input features, act[L] output of layer L-1
@@ -76,6 +103,7 @@ struct Net {
   uint16_t* feat = nullptr;  // dataset [capacity][S][16]
   float* tgt = nullptr;      // dataset [capacity][3][S]
   long adam_t = 0;
+  int device = 0;            // the GPU the network lives on
   hipStream_t stream = nullptr;
   Buffers train;
   std::vector<void*> mem;
@@ -238,6 +266,7 @@ int pf_nn_create(int Nx, int Ny, int Nz, int max_batch, int capacity, pf_nn** ou
     delete h;
     return code;
   };
+  if (hipGetDevice(&n.device) != hipSuccess) return bad(fail(PF_E_HIP, "pf_nn_create: device"));
   if (hipStreamCreateWithFlags(&n.stream, hipStreamNonBlocking) != hipSuccess)
     return bad(fail(PF_E_HIP, "pf_nn_create: stream"));
   if ((rc = n.alloc(&n.master, P_TOTAL)) || (rc = n.alloc(&n.m, P_TOTAL)) || (rc = n.alloc(&n.v, P_TOTAL)) ||
@@ -393,6 +422,35 @@ int pf_nn_time_inference(pf_nn* net, int Nx, int Ny, int Nz, int reps, double* u
   (void)hipEventDestroy(e1);
   *us = 1000.0 * ms / reps;
   return PF_OK;
+}
+
+// Inference that allocates nothing (no counterpart in the reference; for
+// callers that cannot allocate device memory).  The packed input and the two
+// ping-pong activation planes live in one file-scope device array of
+// NOALLOC_MAX_VOXELS voxels; every call on a GPU shares that GPU's copy.
+int pf_nn_forward_noalloc(pf_nn* net, int Nx, int Ny, int Nz, const float* in7_dev, float* out3_dev) {
+  if (!net || !in7_dev || !out3_dev || Nx < 1 || Ny < 1 || Nz < 1)
+    return fail(PF_E_ARG, "pf_nn_forward_noalloc: bad argument");
+  const long S = (long)Nx * Ny * Nz;
+  if (S > pf::nn::NOALLOC_MAX_VOXELS)
+    return fail(PF_E_ARG, "pf_nn_forward_noalloc: " + std::to_string(S) + " voxels exceed the " +
+                              std::to_string(pf::nn::NOALLOC_MAX_VOXELS) + " (64^3) of its static scratch");
+  Net& n = net->net;
+  int prev = 0;
+  NN_HIP(hipGetDevice(&prev));
+  if (prev != n.device) NN_HIP(hipSetDevice(n.device));
+  uint16_t* scr = nullptr;
+  int rc = pf::nn::scratch_address(n.device, &scr);
+  if (!rc) {
+    uint16_t* X0 = scr;
+    uint16_t* a = scr + (size_t)pf::nn::NOALLOC_MAX_VOXELS * CH;
+    uint16_t* b = scr + 2 * (size_t)pf::nn::NOALLOC_MAX_VOXELS * CH;
+    hipLaunchKernelGGL(pack_input_kernel, dim3(512), dim3(256), 0, n.stream, in7_dev, S, X0);
+    rc = infer(&n, n.stream, Nx, Ny, Nz, X0, a, b, out3_dev);
+    if (!rc && hipStreamSynchronize(n.stream) != hipSuccess) rc = fail(PF_E_HIP, "pf_nn_forward_noalloc: sync");
+  }
+  if (prev != n.device) (void)hipSetDevice(prev);
+  return rc;
 }
 
 }  // extern "C"

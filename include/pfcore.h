@@ -329,6 +329,19 @@ int pf_nn_time_inference(pf_nn* net, int Nx, int Ny, int Nz, int reps, double* u
  * detaches).  The network must outlive the attachment. */
 int pf_attach_learned_warm_start(pf_handle* h, pf_nn* net);
 
+/* pf_nn_forward for one grid that allocates nothing.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * in7_dev [Nx*Ny*Nz][7] and out3_dev [3][Nx*Ny*Nz] are device arrays of the
+ * caller (FP32, the layouts of pf_nn_forward); the packed input and the
+ * activations live in a static device array of the library holding grids of
+ * up to 262144 voxels (64^3; 24 MiB per GPU, held until the process ends).
+ * Larger grids fail with PF_E_ARG.  The result equals pf_nn_forward's bit for
+ * bit.  Every call on one GPU shares that array, so calls that overlap on one
+ * GPU (from several threads or networks) race; the call returns after the
+ * network's stream has finished. */
+int pf_nn_forward_noalloc(pf_nn* net, int Nx, int Ny, int Nz, const float* in7_dev, float* out3_dev);
+
 /* ---------------------------------------------------------------------------
  * Batched sweep: many independent instances (e.g. the 48 minimize runs of
  * uneven size of workload 4) driven concurrently.  Each handle keeps its own
