@@ -11,7 +11,8 @@
 // pf_input.params, vf_precip_vs_time_<tag>.csv, elastic_solver_trace.csv,
 // pf_zero_mode_trace.csv, performance_summary.{csv,json} and phi/xB VTKs
 // (io_vtk_cuda.h:35-131; CUDA_STO_SUPPRESS_VTK_OUTPUT=1 suppresses them).
-// --mode=minimize runs phi-only minimize (main_cuda.cu:28341-32600) and writes
+// --mode=minimize runs phi-only or (--minimize-full-model) full-model minimize
+// (main_cuda.cu:28341-32600) and writes
 // energy_minimize_T<T>_r<r>_xB<xB>_<tag>.csv and phi_<iter>_<tag>.vtk into
 // <prefix>_T<T>_cuda_<Nx>x<Ny>x<Nz>_dt<dt>_steps<n>_r<R>nm_xB<xB>/<tag>/.
 // Errors print "[fatal] ..." and exit 2, as the reference does.
@@ -143,7 +144,9 @@ void usage() {
       "    dt from --pf-param-file takes precedence over the positional dt.\n"
       "Flags:\n"
       "  --pf-param-file <path>   required: complete physical PF inputs (key=value)\n"
-      "  --mode=dynamics|minimize (phi-only minimize; the full-model and continue modes are not built)\n"
+      "  --mode=dynamics|minimize (the continue modes are not built)\n"
+      "  --minimize-full-model    chemical drive, composition and volume constraint (no zero mode)\n"
+      "  --minimize-rms-dY-threshold <v> --minimize-energy-diff-rel-threshold <v> --minimize-xB-max-safe <x>\n"
       "  --minimize-max-iter <n> --minimize-dt <dt> --V0 <v> --eta-lambda-vol <v>\n"
       "  --minimize-post-projection-iters <n> --minimize-rms-dphi-threshold <v>\n"
       "  --minimize-rms-res-threshold <v> --minimize-vol-err-rel-threshold <v>\n"
@@ -161,7 +164,7 @@ void usage() {
       "  --precision fp64|mixed   (default fp64)\n");
 }
 
-// Phi-only minimize loop with the reference's energy log (main_cuda.cu:27440-27466,
+// Minimize loop with the reference's energy log (main_cuda.cu:27440-27466,
 // 32455-32489), VTK outputs (build_case_vtk_path, main_cuda.cu:126-170) and
 // convergence report (main_cuda.cu:32546-32600).
 int run_minimize(pf_handle* h, const pf_params& P, const pf_minimize_opts& mo, const std::string& case_dir,
@@ -196,15 +199,19 @@ int run_minimize(pf_handle* h, const pf_params& P, const pf_minimize_opts& mo, c
       std::fprintf(efp,
                    "%d,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,"
                    "%.17e,%.17e,%.17e,%.17e,%d\n",
-                   it, mo.dt, st.mean_h, st.V0, st.lambda_vol, st.F_surf_hat, st.F_el_hat, nan, nan, nan, nan,
-                   st.interface_sum, st.el_core_sum, st.rms_res, st.rms_dphi, nan, nan, st.vol_err_rel, nan, nan,
-                   nan, nan, mo.post_projection_iters);
+                   it, mo.dt, st.mean_h, st.V0, st.lambda_vol, st.F_surf_hat, st.F_el_hat, st.F_chem_excess_hat,
+                   st.F_total_excess_hat, st.F_chem_CNT_hat, st.F_total_CNT_hat, st.interface_sum, st.el_core_sum,
+                   st.rms_res, st.rms_dphi, st.rms_dY, st.energy_diff_rel, st.vol_err_rel, nan, nan, nan, nan,
+                   mo.post_projection_iters);
       std::fflush(efp);
     }
     if (P.out_every > 0 && it % P.out_every == 0) dump(std::to_string(it), it);
     if (st.converged) {
-      std::printf("[minimize] converged iter=%d trigger=condA rms_res=%.3e rms_dphi=%.3e vol_err_rel=%.3e\n", it,
-                  st.rms_res, st.rms_dphi, st.vol_err_rel);
+      std::printf("[minimize] converged iter=%d rms_res=%.3e rms_dphi=%.3e vol_err_rel=%.3e\n", it, st.rms_res,
+                  st.rms_dphi, st.vol_err_rel);
+      if (mo.full_model)
+        std::printf("[minimize] rms_dY=%.3e energy_diff_rel=%.3e F_total_excess_hat=%.6e\n", st.rms_dY,
+                    st.energy_diff_rel, st.F_total_excess_hat);
       break;
     }
   }
@@ -297,7 +304,9 @@ int main(int argc, char** argv) {
                a == "--eta_lambda_vol" || a == "--minimize-post-projection-iters" ||
                a == "--minimize-rms-dphi-threshold" || a == "--minimize-rms-res-threshold" ||
                a == "--minimize-vol-err-rel-threshold" || a == "--minimize-convergence-steps" ||
-               a == "--minimize-min-pseudo-time" || a == "--radius") {
+               a == "--minimize-min-pseudo-time" || a == "--radius" || a == "--minimize-rms-dY-threshold" ||
+               a == "--minimize-rms-dy-threshold" || a == "--minimize-energy-diff-rel-threshold" ||
+               a == "--minimize-xB-max-safe") {
       // main_cuda.cu:23762-23862
       std::string v;
       if (!next(&v)) return fatal(a + " needs a value");
@@ -324,12 +333,20 @@ int main(int argc, char** argv) {
         mo.convergence_steps = std::atoi(v.c_str());
       } else if (a == "--minimize-min-pseudo-time") {
         mo.min_pseudo_time = x;
+      } else if (a == "--minimize-rms-dY-threshold" || a == "--minimize-rms-dy-threshold") {
+        mo.rms_dY_threshold = x;
+      } else if (a == "--minimize-energy-diff-rel-threshold") {
+        mo.energy_diff_rel_threshold = x;
+      } else if (a == "--minimize-xB-max-safe") {
+        mo.xB_max_safe = x;
       } else {
         seed_radius = x;
       }
-    } else if (a == "--minimize-full-model" || a == "--minimize-full" || a == "--minimize-mass-constraint" ||
-               a == "--minimize-freeze-phi" || a == "--minimize-component-volume-constraints") {
-      return fatal(a + " is not implemented (phi-only minimize only)");
+    } else if (a == "--minimize-full-model" || a == "--minimize-full") {
+      mo.full_model = 1;   // main_cuda.cu:23726-23730
+    } else if (a == "--minimize-mass-constraint" || a == "--minimize-freeze-phi" ||
+               a == "--minimize-component-volume-constraints") {
+      return fatal(a + " is not implemented (phi-only and full-model minimize without constraints)");
     } else if (starts_with(a.c_str(), "--")) {
       std::fprintf(stderr, "[warn] flag not handled by main_pf: %s\n", a.c_str());
     }
@@ -446,6 +463,7 @@ int main(int argc, char** argv) {
     if (meta_path.empty() || !read_meta(meta_path, &m)) return fatal("cannot read init meta: " + meta_path);
     if (m.Nx != P.Nx || m.Ny != P.Ny || m.Nz != P.Nz) return fatal("init meta grid does not match the run grid");
     if (!read_raw(phi_raw, m.dtype, N, &phi)) return fatal("cannot read phi raw field: " + phi_raw);
+    if (minimize && mo.full_model && xB_raw.empty()) return fatal("full-model minimize needs --init-xB-raw");
     if (minimize && xB_raw.empty()) {
       xB.assign(N, 0.0);   // phi-only minimize allocates no composition (main_cuda.cu:2609)
     } else if (!read_raw(xB_raw, m.dtype, N, &xB)) {
@@ -453,7 +471,7 @@ int main(int argc, char** argv) {
     }
     if (!dydt_raw.empty() && !read_raw(dydt_raw, m.dtype, N, &dydt))
       return fatal("cannot read dY_dt_prev raw field: " + dydt_raw);
-    xB_cap = std::isfinite(m.xB_max_safe) ? m.xB_max_safe : 0.07;   // minimize_xB_max_safe default
+    xB_cap = std::isfinite(m.xB_max_safe) ? m.xB_max_safe : mo.xB_max_safe;   // main_cuda.cu:2940
     if (case_tag.empty()) case_tag = "raw_fields_relax";
   } else {
     return fatal("main_pf initialises from --init-mode raw_fields only");

@@ -918,7 +918,7 @@ struct Engine final : EngineBase {
   bool egrad_pending = false;   // e_grad of the last step not evaluated yet
   T* sig6 = nullptr;            // stress components of the last step (keep_stress)
   bool keep_sig6 = false, sig6_valid = false;
-  // phi-only minimize state
+  // minimize state (phi-only and full model)
   bool minimizing = false;
   pf_minimize_opts mopt{};
   double* m_gexp = nullptr;   // g_exp of the current phi
@@ -929,7 +929,11 @@ struct Engine final : EngineBase {
   double m_lambda = 0.0, m_V0 = 0.0;
   int m_iter = 0, m_conv = 0;
   bool m_converged = false;
-  const double* xB_el() const { return minimizing ? m_xzero : xB; }
+  // full model: chemical baseline x_ref and g_bulk0 = mu_mix(x_ref)
+  // (main_cuda.cu:27400-27432), the Y cap of minimize_xB_max_safe
+  // (main_cuda.cu:30639-30646) and the previous iteration's excess energy
+  double m_xref = 0.0, m_gbulk0 = 0.0, m_ycap = 0.0, m_Fprev = NAN;
+  const double* xB_el() const { return (minimizing && !mopt.full_model) ? m_xzero : xB; }
   // cold-start hooks: learned correction and caller callback
   nn::Net* nn_net = nullptr;
   uint16_t *nn_x0 = nullptr, *nn_a = nullptr, *nn_b = nullptr;
@@ -1641,21 +1645,44 @@ struct Engine final : EngineBase {
       const int g = launch_zpass<T, 1, 1, OpPhiFinish<T>>(Nz, stream, zargs(0, 1, 1, 1), oa);
       finalize(g, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 4, 1.0 / (double)R);
     }
-    // Y: spectral gradient of mu, fluxes, update
+    // Y: spectral gradient of mu, fluxes, update.  Mixed precision with the
+    // zero mode updates Y by its increment once Y^n is band-limited (after
+    // the first step): the FP32 transforms then carry dt (div J - s) /
+    // (1 + dt D-bar k^2) rather than Y itself, so their rounding scales with
+    // the increment (kop_yupdate), and Y^n needs no transform.
+    const bool yinc = kMixed && zm && y_banded;
+    if ((rc = y_chain(P, phi_old, phi_new, yinc, P.Y_clip, 13))) return rc;
+    cur = 1 - cur;
+    egrad_pending = true;
+    eng_pending = true;
+    sig6_valid = keep_sig6 && P.elastic;
+    ++steps_done;
+    y_banded = true;
+    phi_banded = kMixed;
+    state_ok = true;
+    return PF_OK;
+  }
+
+  // The composition update from the mu spectrum in Sz(1) and D-bar in
+  // dscal[4] (main_cuda.cu:30283-30911): the spectral gradient of mu, the
+  // fluxes, the lagged right-hand side, the semi-implicit update, then the
+  // zero mode or, without it, the clamp of Y to [-Y_clip, y_cap].  PY holds
+  // the time step of the update; yinc selects the increment form; the first
+  // nfetch device scalars come back with the result.
+  int y_chain(const PhysParams& PY, const double* phi_old, const double* phi_new, bool yinc, double y_cap,
+              int nfetch) {
+    const T invN = T(1.0 / (double)R);
+    const KDims d = kd();
+    const bool zm = prm.zero_mode != 0;
+    int rc = PF_OK;
     ypass(false, 1, 1);
     xpass(false, 1, 1, sc(1));
     hipLaunchKernelGGL(kop_grad<T>, dim3(gmodes()), dim3(NT), 0, stream, d, nSc, sc(1), sc(2), sc(3), sc(4));
     xpass(true, 2, 3, sc(2));
     ypass(true, 2, 3);
-    // Mixed precision with the zero mode updates Y by its increment once Y^n
-    // is band-limited (after the first step): the FP32 transforms then carry
-    // dt (div J - s) / (1 + dt D-bar k^2) rather than Y itself, so their
-    // rounding scales with the increment (kop_yupdate), and Y^n needs no
-    // transform.
-    const bool yinc = kMixed && zm && y_banded;
     {
       const double* phiY = (zm || prm.enable_Y_rhs_previous_time_level) ? phi_old : phi_new;
-      typename OpFlux<T>::Args oa{P,    phiY, xB,  phi_old, phi_new, Y, dYdt, invN, zm ? 1 : 0,
+      typename OpFlux<T>::Args oa{PY,   phiY, xB,  phi_old, phi_new, Y, dYdt, invN, zm ? 1 : 0,
                                   prm.enable_Y_rhs_previous_time_level, prm.disable_Y_rhs_gamma_term,
                                   prm.Y_rhs_term_h_scale, yinc ? 1 : 0};
       if constexpr (kMixed) {
@@ -1672,7 +1699,7 @@ struct Engine final : EngineBase {
     xpass(false, 0, nyc, Sc);
     hipLaunchKernelGGL(kop_yupdate<T>, dim3(gmodes()), dim3(NT), 0, stream, d, nSc, sc(0), sc(1), sc(2),
                        yinc ? (const C*)nullptr : (const C*)sc(3), yinc ? sc(3) : sc(4), sc(5),
-                       (const double*)(dscal + 4), prm.dt);
+                       (const double*)(dscal + 4), PY.dt);
     xpass(true, 5, 1, sc(5));
     ypass(true, 5, 1);
     if (zm) {
@@ -1680,24 +1707,16 @@ struct Engine final : EngineBase {
       const int g = launch_zpass<T, 1, 0, OpYStar<T>>(Nz, stream, zargs(5, 1, 0, 0), oa);
       finalize(g, 2, RedKinds{{RED_MIN, RED_MAX, 0, 0}}, dscal, 1.0);
       // the step's scalars come back with the zero-mode search state
-      rc = zero_mode(phi_new, 13);
+      rc = zero_mode(phi_new, nfetch);
       mean_DY = hscal[4];
       if (rc) return rc;
     } else {
-      typename OpYFinal<T>::Args oa{P, Y, xB, dYdt, invN, P.Y_clip};
+      typename OpYFinal<T>::Args oa{PY, Y, xB, dYdt, invN, y_cap};
       launch_zpass<T, 1, 0, OpYFinal<T>>(Nz, stream, zargs(5, 1, 0, 0), oa);
-      if ((rc = fetch(13))) return rc;
+      if ((rc = fetch(nfetch))) return rc;
       mean_DY = hscal[4];
     }
-    cur = 1 - cur;
-    egrad_pending = true;
-    eng_pending = true;
-    sig6_valid = keep_sig6 && P.elastic;
-    ++steps_done;
-    y_banded = true;
-    phi_banded = kMixed;
-    state_ok = true;
-    return PF_OK;
+    return rc;
   }
 
   // run_pf_conserved_y_zero_mode_host (main_cuda.cu:4269-4405), decided on the
@@ -1926,6 +1945,29 @@ struct Engine final : EngineBase {
     warm_valid = false;   // the eigenstrain differs from the dynamics one
     hist_count = 0;
     hist_head = 0;
+    if (o.full_model) {
+      if (prm.zero_mode)
+        return fail(PF_E_ARG,
+                    "full-model minimize runs the composition update with the clamp; with the zero mode it is "
+                    "not built (its root find has symmetric Y bounds, the minimize Y cap is not)");
+      y_banded = false;
+      // chemical baseline (main_cuda.cu:27400-27432): ic_23d_xB_out, else <x_B>
+      double xr = prm.ic_23d_xB_out;
+      if (!(xr > 0.0)) {
+        const int g = grid_for(R);
+        hipLaunchKernelGGL(sum_sq_kernel, dim3(g), dim3(NT), 0, stream, (const double*)xB, R, partials);
+        finalize(g, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal + 58, 1.0);
+        if ((rc = fetch(60))) return rc;
+        xr = hscal[58] / (double)R;
+      }
+      m_xref = xr;
+      m_gbulk0 = (1.0 - xr) * mu_A(P, xr) + xr * mu_B(P, xr);
+      double c = o.xB_max_safe;
+      if (c < prm.xB_eps) c = prm.xB_eps;
+      if (c > 1.0 - prm.xB_eps) c = 1.0 - prm.xB_eps;
+      m_ycap = std::fmin(prm.Y_clip, std::log(c / (1.0 - c)));
+      m_Fprev = NAN;
+    }
     return PF_OK;
   }
 
@@ -1965,6 +2007,7 @@ struct Engine final : EngineBase {
     if (!minimizing) return fail(PF_E_ARG, "pf_minimize_begin has not been called on this handle");
     if (int rs = require_state("pf_minimize_iterate")) return rs;
     state_ok = false;
+    if (mopt.full_model) return minimize_iterate_full(st);
     const T invN = T(1.0 / (double)R);
     const int rows = Nx * Ny;
     const bool el = P.elastic != 0;
@@ -2094,9 +2137,184 @@ struct Engine final : EngineBase {
       st->vol_err_rel = vol_err_rel;
       st->lambda_raw = lam_raw;
       st->lambda_kkt = lam_kkt;
+      st->F_chem_excess_hat = st->F_total_excess_hat = st->F_chem_CNT_hat = st->F_total_CNT_hat = NAN;
+      st->rms_dY = st->energy_diff_rel = NAN;
     }
     if (!std::isfinite(mean_h) || !std::isfinite(m_lambda))
       return fail(PF_E_NONFINITE, "minimize produced a non-finite phi");
+    state_ok = true;
+    return PF_OK;
+  }
+
+  // One full-model iteration (main_cuda.cu:28427-32600 with P.mode == 1 and
+  // minimize_full_model == 1): the solve with the composition eigenstrain;
+  // energies of phi^n with the chemical excess; x_B refreshed from Y and the
+  // dynamics phi right-hand side as g_exp; the multiplier step; the KKT
+  // residual at phi^{n+1}; the composition update at the pseudo-time step
+  // with Y <= logit(minimize_xB_max_safe); the post-projection sub-steps with
+  // the new x_B; the volume projection; rms_dY and the full-model condA/condB.
+  int minimize_iterate_full(pf_minimize_stats* st) {
+    const T invN = T(1.0 / (double)R);
+    const bool el = P.elastic != 0;
+    double* phi_old = phi[cur];
+    double* phi_new = phi[1 - cur];
+    const int gR = grid_for(R);
+    int rc;
+    ++m_iter;
+    last_iters = 0;
+    hipLaunchKernelGGL(grad_energy_kernel, dim3(gR), dim3(NT), 0, stream, phi_old, Nx, Ny, Nz, prm.dx, prm.dy,
+                       prm.dz, prm.kappa_phi, partials);
+    finalize(gR, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 8, 1.0 / (double)R);
+    hipLaunchKernelGGL(min_volume_kernel, dim3(gR), dim3(NT), 0, stream, phi_old, R, partials);
+    finalize(gR, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal + 48, 1.0);
+    if (el) {
+      if ((rc = elastic_solve())) return rc;
+      typename OpMinRhsFull<T, true>::Args oa{P, phi_old, Y, xB, m_gexp, m_eps, m_s00, trs, (size_t)R, invN};
+      const int gz = launch_zpass<T, 6, 1, OpMinRhsFull<T, true>>(Nz, stream, zargs(0, 6, 0, 1), oa);
+      finalize(gz, 4, RedKinds{{RED_SUM, RED_SUM, RED_SUM, RED_SUM}}, dscal + 9, 1.0);
+    } else {
+      typename OpMinRhsFull<T, false>::Args oa{P, phi_old, Y, xB, m_gexp, m_eps, m_s00, trs, (size_t)R, invN};
+      const int gz = launch_zpass<T, 0, 1, OpMinRhsFull<T, false>>(Nz, stream, zargs(0, 0, 0, 1), oa);
+      finalize(gz, 4, RedKinds{{RED_SUM, RED_SUM, RED_SUM, RED_SUM}}, dscal + 9, 1.0);
+    }
+    min_forward_to_sc0();
+    min_lap_to_sz1();
+    min_gfull(phi_old, 50);
+    if ((rc = fetch(58))) return rc;
+    const double grad_mean = hscal[8];
+    const double gel_sum = hscal[9], dw_sum = hscal[10], core_sum = hscal[11], chem_sum = hscal[12];
+    const double mean_h0 = hscal[48] / (double)R;
+    if (m_iter == 1 && m_V0 <= 0.0) m_V0 = mean_h0;
+    const double F_surf = grad_mean + dw_sum / (double)R;
+    const double F_el = gel_sum / (double)R;
+    const double F_chem_excess = chem_sum / (double)R - m_gbulk0;
+    const double F_total_excess = F_chem_excess + F_surf + F_el;
+    double F_chem_cnt = NAN, F_total_cnt = NAN;
+    if (prm.ic_23d_xB_out > 0.0 && mean_h0 > 0.0) {
+      F_chem_cnt = mean_h0 * (P.mu0 - mu_B(P, prm.ic_23d_xB_out));
+      F_total_cnt = F_surf + F_el + F_chem_cnt;
+    }
+    const double F_prev = m_Fprev;
+    m_Fprev = F_total_excess;
+    const double lam_raw = min_multiplier(hscal[50], hscal[51]);
+    const double eta = std::fmin(std::fmax(mopt.eta_lambda_vol, 0.0), 1.0);
+    m_lambda = (1.0 - eta) * m_lambda + eta * lam_raw;
+    // phi^{n+1}, and g_exp there for the KKT residual (x_B unchanged: Y is)
+    min_update(phi_old, m_lambda);
+    int gz;
+    if (el) {
+      typename OpMinFinishFull<T, true>::Args oa{P, phi_old, phi_new, m_gexp, m_eps, m_s00, trs, xB, (size_t)R,
+                                                 invN};
+      gz = launch_zpass<T, 1, 1, OpMinFinishFull<T, true>>(Nz, stream, zargs(0, 1, 0, 1), oa);
+    } else {
+      typename OpMinFinishFull<T, false>::Args oa{P, phi_old, phi_new, m_gexp, m_eps, m_s00, trs, xB, (size_t)R,
+                                                  invN};
+      gz = launch_zpass<T, 1, 1, OpMinFinishFull<T, false>>(Nz, stream, zargs(0, 1, 0, 1), oa);
+    }
+    finalize(gz, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 52, 1.0);
+    min_forward_to_sc0();
+    min_lap_to_sz1();
+    min_gfull(phi_new, 53);
+    if ((rc = fetch(58))) return rc;
+    const double rms_dphi = std::sqrt(hscal[52] / (double)R) / mopt.dt;
+    const double lam_kkt = min_multiplier(hscal[53], hscal[54]);
+    hipLaunchKernelGGL(min_kkt_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, m_gfull, lam_kkt, R, partials);
+    finalize(gR, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 55, 1.0);
+    // the composition update at the pseudo-time step (P.dt is the minimize dt)
+    {
+      const double* phiY = prm.enable_Y_rhs_previous_time_level ? phi_old : phi_new;
+      typename OpMuX<T>::Args oa{P, Y, phiY, xB, trs};
+      const int g = launch_zpass<T, 0, 1, OpMuX<T>>(Nz, stream, zargs(0, 0, 1, 1), oa);
+      finalize(g, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 4, 1.0 / (double)R);
+      PhysParams PY = P;
+      PY.dt = mopt.dt;
+      if ((rc = y_chain(PY, phi_old, phi_new, false, m_ycap, 13))) return rc;
+    }
+    // post-projection sub-steps with the updated x_B, elastic state frozen
+    for (int k = 0; k < mopt.post_projection_iters; ++k) {
+      if (el) {
+        typename OpMinEvalFull<T, true>::Args oa{P, phi_new, m_gexp, m_eps, m_s00, trs, xB, (size_t)R};
+        launch_zpass<T, 0, 1, OpMinEvalFull<T, true>>(Nz, stream, zargs(0, 0, 0, 1), oa);
+      } else {
+        typename OpMinEvalFull<T, false>::Args oa{P, phi_new, m_gexp, m_eps, m_s00, trs, xB, (size_t)R};
+        launch_zpass<T, 0, 1, OpMinEvalFull<T, false>>(Nz, stream, zargs(0, 0, 0, 1), oa);
+      }
+      min_forward_to_sc0();
+      min_lap_to_sz1();
+      min_gfull(phi_new, 56);
+      if ((rc = fetch(58))) return rc;
+      m_lambda = min_multiplier(hscal[56], hscal[57]);
+      min_update(phi_new, m_lambda);
+      typename OpMinFinish<T, false, false>::Args oa{P, phi_new, phi_new, m_gexp, m_eps, m_s00, (size_t)R, invN};
+      launch_zpass<T, 1, 0, OpMinFinish<T, false, false>>(Nz, stream, zargs(0, 1, 0, 0), oa);
+    }
+    // first-order volume projection, then rms_dY from dY/dt = (Y^{n+1} - Y^n)/dt
+    hipLaunchKernelGGL(min_volume_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, R, partials);
+    finalize(gR, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal + 48, 1.0);
+    hipLaunchKernelGGL(sum_sq_kernel, dim3(gR), dim3(NT), 0, stream, (const double*)dYdt, R, partials);
+    finalize(gR, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal + 58, 1.0);
+    if ((rc = fetch(60))) return rc;
+    const double rms_res = std::sqrt(hscal[55] / (double)R);
+    const double rms_dY = std::sqrt(hscal[59] / (double)R);
+    double mean_h = hscal[48] / (double)R;
+    if (m_V0 > 1e-30 && std::isfinite(mean_h)) {
+      const double vol_err = m_V0 - mean_h;
+      if (std::fabs(vol_err) > 1e-14) {
+        const double mean_hp2 = hscal[49] / (double)R;
+        const double lc = mean_hp2 > 1e-30 ? vol_err / mean_hp2 : 0.0;
+        hipLaunchKernelGGL(min_project_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, lc, R, partials);
+        finalize(gR, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 48, 1.0);
+        if ((rc = fetch(58))) return rc;
+        mean_h = hscal[48] / (double)R;
+      }
+    }
+    cur = 1 - cur;
+    const double energy_diff_rel =
+        std::isfinite(F_prev) ? std::fabs(F_total_excess - F_prev) / std::fmax(std::fabs(F_prev), 1.0e-20) : NAN;
+    const double vol_err_rel = m_V0 > 1e-30 ? std::fabs(mean_h - m_V0) / (m_V0 + 1e-30) : 0.0;
+    auto fin = [](double v) { return std::isfinite(v); };
+    const bool vol_ok = fin(vol_err_rel) && vol_err_rel < mopt.vol_err_rel_threshold;
+    const bool res_ok = fin(rms_res) && rms_res < mopt.rms_res_threshold;
+    const bool condA = fin(rms_dphi) && fin(rms_dY) && rms_dphi < mopt.rms_dphi_threshold &&
+                       rms_dY < mopt.rms_dY_threshold && res_ok && vol_ok;
+    const bool condB = fin(energy_diff_rel) && energy_diff_rel < mopt.energy_diff_rel_threshold && fin(rms_dphi) &&
+                       fin(rms_dY) && rms_dphi < 5.0 * mopt.rms_dphi_threshold &&
+                       rms_dY < 5.0 * mopt.rms_dY_threshold && res_ok && vol_ok;
+    const bool t_ok = (double)m_iter * mopt.dt + 1e-14 >= mopt.min_pseudo_time;
+    m_conv = ((condA || condB) && t_ok) ? m_conv + 1 : 0;
+    if (m_conv >= mopt.convergence_steps) m_converged = true;
+    e_grad = grad_mean;
+    e_dw = dw_sum / (double)R;
+    e_el = gel_sum / (double)R;
+    e_chem = chem_sum / (double)R;
+    eng_pending = false;
+    ++steps_done;
+    if (st) {
+      st->iteration = m_iter;
+      st->converged = m_converged ? 1 : 0;
+      st->convergence_count = m_conv;
+      st->elastic_iterations = el ? last_iters : 0;
+      st->mean_h = mean_h;
+      st->V0 = m_V0;
+      st->lambda_vol = m_lambda;
+      st->F_surf_hat = F_surf;
+      st->F_el_hat = F_el;
+      st->interface_sum = grad_mean * (double)R + dw_sum;
+      st->el_core_sum = core_sum;
+      st->rms_res = rms_res;
+      st->rms_dphi = rms_dphi;
+      st->vol_err_rel = vol_err_rel;
+      st->lambda_raw = lam_raw;
+      st->lambda_kkt = lam_kkt;
+      st->F_chem_excess_hat = F_chem_excess;
+      st->F_total_excess_hat = F_total_excess;
+      st->F_chem_CNT_hat = F_chem_cnt;
+      st->F_total_CNT_hat = F_total_cnt;
+      st->rms_dY = rms_dY;
+      st->energy_diff_rel = energy_diff_rel;
+    }
+    if (!std::isfinite(mean_h) || !std::isfinite(m_lambda) || !std::isfinite(rms_dY))
+      return fail(PF_E_NONFINITE, "minimize produced a non-finite phi or Y");
     state_ok = true;
     return PF_OK;
   }

@@ -1,4 +1,4 @@
-// Phi-only minimize mode (main_cuda --mode=minimize without
+// Minimize mode (main_cuda --mode=minimize, phi-only and
 // --minimize-full-model): point-wise operators fused into the z passes and
 // the element-wise reductions of the volume constraint.
 //
@@ -208,6 +208,176 @@ struct OpMinEval {
   }
 };
 
+// ---------------------------------------------------------------- full model
+// --minimize-full-model evaluates the dynamics phi right-hand side
+// (compute_phi_rhs_kernel, cuda_kernels.cu:390-552: chemical drive, W g' and
+// the elastic terms with the composition eigenstrain) in every g_exp, with
+// the stress and total strain of the iteration's solve held fixed.  Those are
+// kept as the 6 total strains, s00 = sigma : eps00 (engineering shear) and
+// tr(sigma), which is all that term needs at a new phi and x_B: with
+// d eps0_ii/d phi = h' (eps00_ii - x e_iso + e_iso (x - v_B)) and shears h' eps00_ij,
+//   -sigma : d eps0/d phi = -h' (s00 + (-x e_iso + e_iso (x - v_B)) tr sigma),
+// plus 1/2 h' Q(eps - eps0(phi, x)).
+PF_HD double min_dgel_full(const PhysParams& P, double phi, double xB, const double* eps, double s00,
+                           double trs) {
+  const double hp = h_prime_of_phi(phi);
+  const double h = h_of_phi(phi);
+  const double x = clamp01(xB);
+  const double diag_iso = -(x * P.eps_iso) + P.eps_iso * (x - P.v_B);
+  const double dgel_eigen = -hp * (s00 + diag_iso * trs);
+  double e0[6], el[6], q[6];
+  eigenstrain(P, h, x, e0);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) el[c] = eps[c] - e0[c];
+  voigt_apply<double>(P.dC, el, q);
+  const double Q = el[0] * q[0] + el[1] * q[1] + el[2] * q[2] + 2.0 * el[3] * q[3] + 2.0 * el[4] * q[4] +
+                   2.0 * el[5] * q[5];
+  return dgel_eigen + 0.5 * hp * Q;
+}
+
+// g_exp of the full model at (phi, x_B) with the frozen elastic state
+template <bool ELASTIC, typename T>
+__device__ __forceinline__ double min_gexp_full(const PhysParams& P, double phi, double xB, const T* eps_s,
+                                                size_t R, size_t g, const double* s00, const T* trs) {
+  double hp, h, mA, mB;
+  double f = phi_rhs_chem(P, phi, clamp01(xB), hp, h, mA, mB);
+  if constexpr (ELASTIC) {
+    double eps[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) eps[c] = double(eps_s[c * R + g]);
+    f += min_dgel_full(P, phi, xB, eps, s00[g], double(trs[g]));
+  }
+  return f;
+}
+
+// After the solve: inverse z of the 6 strains -> stress at phi^n with the
+// solve's x_B, kept as (eps, s00, tr sigma); x_B refreshed from Y
+// (compute_mu_x_kernel); g_exp(phi^n) at the refreshed x_B; energies with the
+// solve's x_B; forward z of phi^n.
+//   red[0] sum 0.5 sigma:eps_el, red[1] sum W g(phi), red[2] sum 0.5 sigma:eps_el h(clamp01 phi),
+//   red[3] sum (1 - h) mu_mix + h mu0 (compute_gbulk_excess_hat_kernel, cuda_kernels.cu:725-759,
+//          before its baseline shift; clamp01 phi, x clamped to [1e-12, 1 - 1e-12])
+template <typename T, bool ELASTIC>
+struct OpMinRhsFull {
+  struct Args {
+    PhysParams P;
+    const double* phi;
+    const double* Y;
+    double* xB;
+    double* gexp;
+    T* eps;            // 6 components, stride R
+    double* s00;
+    T* trs;
+    size_t R;
+    T invN;
+  };
+  static constexpr int NRED = 4;
+  static constexpr int kinds[4] = {RED_SUM, RED_SUM, RED_SUM, RED_SUM};
+  static constexpr int NPRE = 3;
+  __device__ static void load(const Args& a, size_t g, double* pre) {
+    pre[0] = a.phi[g];
+    pre[1] = a.xB[g];
+    pre[2] = a.Y[g];
+  }
+  __device__ static void apply(const Args& a, size_t g, const double* pre, const T* vin, T* out, double* red) {
+    const PhysParams& P = a.P;
+    const double phi = pre[0];
+    const double xs = pre[1];
+    const double hc = h_of_phi(clamp01(phi));
+    {
+      const double xc = clamp_fraction_eps(xs);
+      red[3] += (1.0 - hc) * ((1.0 - xc) * mu_A(P, xc) + xc * mu_B(P, xc)) + hc * P.mu0;
+    }
+    red[1] += P.W * g_of_phi(phi);
+    const double xr = sigmoid_from_logit(pre[2], P.Y_clip, P.xB_eps);
+    a.xB[g] = xr;
+    if constexpr (ELASTIC) {
+      double eps[6], e0[6], el[6], sig[6];
+      const double h = h_of_phi(phi);
+      eigenstrain(P, h, clamp01(xs), e0);
+#pragma unroll
+      for (int c = 0; c < 6; ++c) {
+        eps[c] = double(vin[c] * a.invN) + P.E0[c];
+        el[c] = eps[c] - e0[c];
+      }
+      voigt_apply_h<double>(P, h, el, sig);
+      const double s00 = sig[0] * P.eps00[0] + sig[1] * P.eps00[1] + sig[2] * P.eps00[2] +
+                         2.0 * (sig[3] * P.eps00[3] + sig[4] * P.eps00[4] + sig[5] * P.eps00[5]);
+      const double gel = 0.5 * (sig[0] * el[0] + sig[1] * el[1] + sig[2] * el[2] +
+                                2.0 * (sig[5] * el[5] + sig[4] * el[4] + sig[3] * el[3]));
+      red[0] += gel;
+      red[2] += gel * hc;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) a.eps[c * a.R + g] = T(eps[c]);
+      a.s00[g] = s00;
+      a.trs[g] = T(sig[0] + sig[1] + sig[2]);
+    }
+    a.gexp[g] = min_gexp_full<ELASTIC, T>(P, phi, xr, a.eps, a.R, g, a.s00, a.trs);
+    out[0] = T(phi);
+  }
+};
+
+// inverse z of the updated phi^: phi = clamp(v/N, -1e-6, 1+1e-6); red[0] sum
+// (phi - phi_ref)^2; g_exp at the new phi and the current x_B; forward z of phi
+template <typename T, bool ELASTIC>
+struct OpMinFinishFull {
+  struct Args {
+    PhysParams P;
+    const double* phi_ref;
+    double* phi_out;
+    double* gexp;
+    const T* eps;
+    const double* s00;
+    const T* trs;
+    const double* xB;
+    size_t R;
+    T invN;
+  };
+  static constexpr int NRED = 1;
+  static constexpr int kinds[1] = {RED_SUM};
+  static constexpr int NPRE = 2;
+  __device__ static void load(const Args& a, size_t g, double* pre) {
+    pre[0] = a.phi_ref[g];
+    pre[1] = a.xB[g];
+  }
+  __device__ static void apply(const Args& a, size_t g, const double* pre, const T* vin, T* out, double* red) {
+    double v = double(vin[0]) * double(a.invN);
+    if (v < -1e-6) v = -1e-6;
+    if (v > 1.0 + 1e-6) v = 1.0 + 1e-6;
+    a.phi_out[g] = v;
+    const double d = v - pre[0];
+    red[0] += d * d;
+    a.gexp[g] = min_gexp_full<ELASTIC, T>(a.P, v, pre[1], a.eps, a.R, g, a.s00, a.trs);
+    out[0] = T(v);
+  }
+};
+
+// g_exp at the current phi and x_B (frozen elastic state) and forward z of phi
+template <typename T, bool ELASTIC>
+struct OpMinEvalFull {
+  struct Args {
+    PhysParams P;
+    const double* phi;
+    double* gexp;
+    const T* eps;
+    const double* s00;
+    const T* trs;
+    const double* xB;
+    size_t R;
+  };
+  static constexpr int NRED = 0;
+  static constexpr int kinds[1] = {0};
+  static constexpr int NPRE = 2;
+  __device__ static void load(const Args& a, size_t g, double* pre) {
+    pre[0] = a.phi[g];
+    pre[1] = a.xB[g];
+  }
+  __device__ static void apply(const Args& a, size_t g, const double* pre, const T*, T* out, double*) {
+    a.gexp[g] = min_gexp_full<ELASTIC, T>(a.P, pre[0], pre[1], a.eps, a.R, g, a.s00, a.trs);
+    out[0] = T(pre[0]);
+  }
+};
+
 // ---------------------------------------------------------------- element-wise
 namespace {
 // res = g_full + lambda h'(phi), zeroed where it only points out of [0,1]
@@ -235,6 +405,18 @@ __global__ void __launch_bounds__(NT) min_volume_kernel(const double* phi, long 
     const double hp = h_prime_of_phi(p);
     v[0] += h_of_phi(clamp01(p));
     v[1] += hp * hp;
+  }
+  const int kinds[2] = {RED_SUM, RED_SUM};
+  block_reduce_store<2>(v, kinds, partials + 2 * (size_t)blockIdx.x);
+}
+
+// red[0] sum a, red[1] sum a^2
+__global__ void __launch_bounds__(NT) sum_sq_kernel(const double* a, long n, double* partials) {
+  double v[2] = {0.0, 0.0};
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
+    const double x = a[i];
+    v[0] += x;
+    v[1] += x * x;
   }
   const int kinds[2] = {RED_SUM, RED_SUM};
   block_reduce_store<2>(v, kinds, partials + 2 * (size_t)blockIdx.x);

@@ -226,6 +226,42 @@ struct OpPhiRhs {
   }
 };
 
+// compute_mu_x_kernel (cuda_kernels.cu:1266-1324) at one voxel: x_B =
+// sigma_clip(Y) (into xB_out), the composition potential
+// mu = c (mu_B - mu_A - c mu_tot dV) - (eps_iso / v_B) tr(sigma) at phiY, and the
+// D_mix(h) x (1 - x) term of reduce_sum_DY_kernel (cuda_kernels.cu:3870-3921,
+// x the logistic of Y clamped to [1e-12, 1 - 1e-12]) into dsum.
+__device__ __forceinline__ double mu_x_point(const PhysParams& P, double Y, double phiY, double trs,
+                                             double& xB_out, double& dsum) {
+  // sigmoid_from_logit(Y) evaluates the logistic at the clipped Y; the
+  // D_mix sum below needs it at Y itself: one exp unless Y was clipped
+  const double Yc = Y > P.Y_clip ? P.Y_clip : (Y < -P.Y_clip ? -P.Y_clip : Y);
+  const double sc = sigmoid_unclipped(Yc);
+  const double xB = sc < P.xB_eps ? P.xB_eps : (sc > 1.0 - P.xB_eps ? 1.0 - P.xB_eps : sc);
+  xB_out = xB;
+  const double h = h_of_phi(phiY);
+  const double c_bulk = c_xB_phi(P, xB, h);
+  double mu;
+  if (P.dVm == 0.0 && (!P.convex || xB <= 0.09)) {
+    // the volume term vanishes and both potentials are on the CALPHAD
+    // branch: mu_B - mu_A with G(T) differenced once and the logarithms
+    // combined, (G_B0 - G_A0 + RT ln(x/(1-x)) + L (1-2x)) / scale
+    const double x = clamp_fraction_eps(xB);
+    mu = c_bulk * ((P.G_B0 - P.G_A0 + P.RT * pf_log(x / (1.0 - x)) + P.Lparam * (1.0 - 2.0 * x)) / P.mu_scale);
+  } else {
+    const double muA = mu_A(P, xB);
+    const double muB = mu_B(P, xB);
+    const double mu_total = mu_tot_mix(muA, muB, xB, P.mu0, h);
+    mu = c_bulk * (muB - muA - c_bulk * mu_total * P.dVm);
+  }
+  if (P.elastic) mu += -P.eps_iso * trs;
+  double x = (Yc == Y) ? sc : sigmoid_unclipped(Y);
+  if (x < 1e-12) x = 1e-12;
+  if (x > 1.0 - 1e-12) x = 1.0 - 1e-12;
+  dsum = D_mix(P, h) * (x * (1.0 - x));
+  return mu;
+}
+
 // phi^{n+1} = clamp(inv/N, -1e-6, 1+1e-6) (cuda_kernels.cu:1071-1084), then the
 // mu_x evaluation (cuda_kernels.cu:1266-1324) and the D_mix x(1-x) sum of
 // reduce_sum_DY_kernel (cuda_kernels.cu:3870-3921).
@@ -265,34 +301,40 @@ struct OpPhiFinish {
     if (v > 1.0 + 1e-6) v = 1.0 + 1e-6;
     a.phi_new[g] = v;
     const double phiY = a.phiY_is_old ? pre[0] : v;
-    const double Y = pre[1];
-    // sigmoid_from_logit(Y) evaluates the logistic at the clipped Y; the
-    // D_mix sum below needs it at Y itself: one exp unless Y was clipped
-    const double Yc = Y > P.Y_clip ? P.Y_clip : (Y < -P.Y_clip ? -P.Y_clip : Y);
-    const double sc = sigmoid_unclipped(Yc);
-    const double xB = sc < P.xB_eps ? P.xB_eps : (sc > 1.0 - P.xB_eps ? 1.0 - P.xB_eps : sc);
+    double xB, dsum;
+    out[0] = T(mu_x_point(P, pre[1], phiY, pre[2], xB, dsum));
     a.xB[g] = xB;
-    const double h = h_of_phi(phiY);
-    const double c_bulk = c_xB_phi(P, xB, h);
-    double mu;
-    if (P.dVm == 0.0 && (!P.convex || xB <= 0.09)) {
-      // the volume term vanishes and both potentials are on the CALPHAD
-      // branch: mu_B - mu_A with G(T) differenced once and the logarithms
-      // combined, (G_B0 - G_A0 + RT ln(x/(1-x)) + L (1-2x)) / scale
-      const double x = clamp_fraction_eps(xB);
-      mu = c_bulk * ((P.G_B0 - P.G_A0 + P.RT * pf_log(x / (1.0 - x)) + P.Lparam * (1.0 - 2.0 * x)) / P.mu_scale);
-    } else {
-      const double muA = mu_A(P, xB);
-      const double muB = mu_B(P, xB);
-      const double mu_total = mu_tot_mix(muA, muB, xB, P.mu0, h);
-      mu = c_bulk * (muB - muA - c_bulk * mu_total * P.dVm);
-    }
-    if (P.elastic) mu += -P.eps_iso * pre[2];
-    out[0] = T(mu);
-    double x = (Yc == Y) ? sc : sigmoid_unclipped(Y);
-    if (x < 1e-12) x = 1e-12;
-    if (x > 1.0 - 1e-12) x = 1.0 - 1e-12;
-    red[0] += D_mix(P, h) * (x * (1.0 - x));
+    red[0] += dsum;
+  }
+};
+
+// The composition potential alone (compute_mu_x_kernel, cuda_kernels.cu:1266-1324)
+// at phiY, x_B = sigma_clip(Y) written, and the D_mix x(1 - x) sum of
+// reduce_sum_DY_kernel; the mu field goes to the forward transform.  Used
+// where the composition update does not follow a phi update in the same pass
+// (full-model minimize).
+template <typename T>
+struct OpMuX {
+  struct Args {
+    PhysParams P;
+    const double* Y;
+    const double* phiY;
+    double* xB;
+    const T* trs;
+  };
+  static constexpr int NRED = 1;
+  static constexpr int kinds[1] = {RED_SUM};
+  static constexpr int NPRE = 3;
+  __device__ static void load(const Args& a, size_t g, double* pre) {
+    pre[0] = a.Y[g];
+    pre[1] = a.phiY[g];
+    pre[2] = a.P.elastic ? double(a.trs[g]) : 0.0;
+  }
+  __device__ static void apply(const Args& a, size_t g, const double* pre, const T*, T* out, double* red) {
+    double xB, dsum;
+    out[0] = T(mu_x_point(a.P, pre[0], pre[1], pre[2], xB, dsum));
+    a.xB[g] = xB;
+    red[0] += dsum;
   }
 };
 

@@ -133,6 +133,7 @@ extern "C" int pf_params_default(pf_params* p) {
   p->mode = 0;
   p->elastic_warm_start_order = 1;
   p->elastic_anderson_depth = 0;
+  p->ic_23d_xB_out = 0.03;
   return PF_OK;
 }
 
@@ -175,6 +176,7 @@ extern "C" int pf_params_set(pf_params* p, const char* key, const char* value) {
                {"ic_vf_init_phi", &p->ic_vf_init_phi},
                {"ic_vf_target_phi", &p->ic_vf_target_phi},
                {"Vm_alpha_0_phys_m3mol", &p->Vm_alpha_0_phys_m3mol},
+               {"ic_23d_xB_out", &p->ic_23d_xB_out},
                {nullptr, nullptr}};
   for (int i = 0; dkeys[i].name; ++i) {
     if (k == dkeys[i].name) {

@@ -248,6 +248,10 @@ int pf_minimize_opts_default(pf_minimize_opts* o) {
   o->vol_err_rel_threshold = 1.0e-4;
   o->convergence_steps = 10;
   o->min_pseudo_time = 0.0;
+  o->full_model = 0;
+  o->rms_dY_threshold = 5.0e-5;
+  o->energy_diff_rel_threshold = 1.0e-9;
+  o->xB_max_safe = 0.07;
   return PF_OK;
 }
 

@@ -59,6 +59,7 @@ class PfParams(C.Structure):
         ("Vm_alpha_0_phys_m3mol", C.c_double),
         ("mode", C.c_int),
         ("elastic_warm_start_order", C.c_int), ("elastic_anderson_depth", C.c_int),
+        ("ic_23d_xB_out", C.c_double),
     ]
 
 
@@ -84,6 +85,8 @@ class PfMinimizeOpts(C.Structure):
         ("post_projection_iters", C.c_int), ("rms_dphi_threshold", C.c_double),
         ("rms_res_threshold", C.c_double), ("vol_err_rel_threshold", C.c_double),
         ("convergence_steps", C.c_int), ("min_pseudo_time", C.c_double),
+        ("full_model", C.c_int), ("rms_dY_threshold", C.c_double), ("energy_diff_rel_threshold", C.c_double),
+        ("xB_max_safe", C.c_double),
     ]
 
 
@@ -96,6 +99,9 @@ class PfMinimizeStats(C.Structure):
         ("interface_sum", C.c_double), ("el_core_sum", C.c_double),
         ("rms_res", C.c_double), ("rms_dphi", C.c_double), ("vol_err_rel", C.c_double),
         ("lambda_raw", C.c_double), ("lambda_kkt", C.c_double),
+        ("F_chem_excess_hat", C.c_double), ("F_total_excess_hat", C.c_double),
+        ("F_chem_CNT_hat", C.c_double), ("F_total_CNT_hat", C.c_double),
+        ("rms_dY", C.c_double), ("energy_diff_rel", C.c_double),
     ]
 
     def as_dict(self):

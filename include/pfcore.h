@@ -89,6 +89,11 @@ typedef struct pf_params {
    * The map, the residual and the tolerance test are unchanged. */
   int elastic_warm_start_order;
   int elastic_anderson_depth;
+  /* far-field composition of the initial condition (key ic_23d_xB_out,
+   * default 0.03, main_cuda.cu:16215); in full-model minimize the chemical
+   * baseline x_ref when positive, else <x_B> of the initial fields
+   * (main_cuda.cu:27400-27432) */
+  double ic_23d_xB_out;
 } pf_params;
 
 typedef struct pf_scalars {
@@ -192,7 +197,7 @@ enum pf_kernel_id {
  * the simulation state is not valid afterwards (call pf_set_fields). */
 int pf_time_kernel(pf_handle* h, int kernel_id, int reps, double* us_per_launch, double* bytes_per_launch);
 
-/* Phi-only minimize mode (--mode=minimize without --minimize-full-model).
+/* Minimize mode (--mode=minimize, phi-only or with --minimize-full-model).
  * Defaults of params_default (main_cuda.cu:16319-16347); CLI flags
  * main_cuda.cu:23762-23851.  Composition is absent: the eigenstrain is
  * h(phi) eps00 (eigenstrain_phi_only_point, cuda_kernels.cu:363-383). */
@@ -207,6 +212,13 @@ typedef struct pf_minimize_opts {
   double vol_err_rel_threshold;  /* --minimize-vol-err-rel-threshold (1e-4) */
   int convergence_steps;         /* --minimize-convergence-steps (10) */
   double min_pseudo_time;        /* minimize_min_pseudo_time (0) */
+  /* --minimize-full-model (main_cuda.cu:23729): the chemical drive, the
+   * composition eigenstrain and the composition update at the pseudo-time
+   * step join the iteration (main_cuda.cu:29357-29700, 30283-30911) */
+  int full_model;
+  double rms_dY_threshold;          /* --minimize-rms-dY-threshold (5e-5) */
+  double energy_diff_rel_threshold; /* --minimize-energy-diff-rel-threshold (1e-9) */
+  double xB_max_safe;               /* --minimize-xB-max-safe (0.07): Y <= logit(xB_max_safe) */
 } pf_minimize_opts;
 
 /* One row of energy_minimize_*.csv (main_cuda.cu:32455-32489) for phi-only
@@ -221,6 +233,12 @@ typedef struct pf_minimize_stats {
   double interface_sum, el_core_sum;
   double rms_res, rms_dphi, vol_err_rel;
   double lambda_raw, lambda_kkt;
+  /* full model (NaN in phi-only minimize): <g_bulk> - mu_mix(x_ref), the
+   * total excess F_chem_excess + F_surf + F_el, the CNT summaries
+   * <h>(mu0 - mu_B(x_out)) and F_surf + F_el + that (when ic_23d_xB_out > 0),
+   * rms(Y^{n+1} - Y^n)/dt and the relative change of the total excess */
+  double F_chem_excess_hat, F_total_excess_hat, F_chem_CNT_hat, F_total_CNT_hat;
+  double rms_dY, energy_diff_rel;
 } pf_minimize_stats;
 
 int pf_minimize_opts_default(pf_minimize_opts* o);

@@ -38,12 +38,12 @@ as an exact stationary point; monotone decrease of the free energy for a
 small pseudo-time step without elasticity.
 """
 import math
-from dataclasses import dataclass
+from dataclasses import dataclass, replace
 
 import numpy as np
 
 from . import thermo as th
-from .step_ref import Oracle, Params, fwd, inv, voigt_apply
+from .step_ref import Oracle, Params, eigenstrain, fwd, inv, voigt_apply
 
 
 @dataclass
@@ -59,6 +59,11 @@ class MinimizeOptions:
     vol_err_rel_threshold: float = 1.0e-4
     convergence_steps: int = 10
     min_pseudo_time: float = 0.0
+    # full-model minimize (main_cuda.cu:16336-16345)
+    rms_dY_threshold: float = 5.0e-5
+    energy_diff_rel_threshold: float = 1.0e-9
+    xB_max_safe: float = 0.07
+    xB_ref: float = 0.03         # ic_23d_xB_out (default 0.03, main_cuda.cu:16215); <= 0: <x_B> of the initial fields
 
 
 class MinimizeOracle:
@@ -205,3 +210,163 @@ class MinimizeOracle:
         while self.iteration < m and not self.converged:
             self.iterate()
         return self
+
+
+class MinimizeFullOracle(MinimizeOracle):
+    """``--mode=minimize --minimize-full-model`` (chemical drive, composition and
+    the volume constraint), one iteration after ``main_cuda.cu:28427-32600``
+    with ``P.mode == 1`` and ``P.minimize_full_model == 1``:
+
+    1. elastic fixed point with the composition eigenstrain of the current x_B;
+    2. energies of phi^n: F_surf, F_el (with the composition eigenstrain), the
+       bulk chemical excess <g_bulk> - g_bulk0 with g_bulk0 = mu_mix(x_ref)
+       (``compute_gbulk_excess_hat_kernel``, ``cuda_kernels.cu:725-759``;
+       baseline ``main_cuda.cu:27400-27432``) and
+       F_total_excess = F_chem_excess + F_surf + F_el (``main_cuda.cu:29334-29346``);
+    3. x_B refreshed from Y (``compute_mu_x_kernel``), then the dynamics phi
+       right-hand side (``compute_phi_rhs_kernel``, chemical drive included)
+       as g_exp, and the multiplier update and semi-implicit step of the
+       phi-only mode (``main_cuda.cu:29357-29560``);
+    4. the KKT residual at phi^{n+1} with the same right-hand side, the stress
+       and total strain frozen (``main_cuda.cu:29577-29728``);
+    5. the dynamics composition update at the pseudo-time step (``P.dt`` is
+       the minimize dt), Y capped at min(Y_clip, logit(minimize_xB_max_safe))
+       (``main_cuda.cu:30283-30911``, cap ``30639-30646``);
+    6. post-projection sub-steps with the x_B of the updated Y
+       (``main_cuda.cu:32104-32190``), the first-order volume projection;
+    7. rms_dY = rms(Y^{n+1} - Y^n) / dt, the relative change of F_total_excess
+       against the previous iteration, and the full-model condA / condB
+       (``main_cuda.cu:32400-32575``).
+    """
+
+    def __init__(self, p: Params, phi0, xB0, opts: MinimizeOptions):
+        p = replace(p, dt=opts.dt)
+        self.p = p
+        self.opts = opts
+        shape = (p.Nx, p.Ny, p.Nz)
+        self.shape = shape
+        # raw_fields load with the minimize cap (main_cuda.cu:2940-2965)
+        self.o = Oracle(p, phi0, xB0, xB_cap=opts.xB_max_safe)
+        o = self.o
+        self.x_ref = opts.xB_ref if opts.xB_ref > 0.0 else math.fsum(o.xB.ravel()) / o.xB.size
+        muA = float(th.mu_A_dimless(self.x_ref, o.T, p.mu_reference_scale))
+        muB = float(th.mu_B_dimless(self.x_ref, o.T, p.mu_reference_scale))
+        self.g_bulk0 = (1.0 - self.x_ref) * muA + self.x_ref * muB
+        c = min(max(opts.xB_max_safe, p.xB_eps), 1.0 - p.xB_eps)
+        self.y_cap = min(p.Y_clip, math.log(c / (1.0 - c)))
+        self.lambda_vol = 0.0
+        self.V0_target = opts.V0
+        self.iteration = 0
+        self.conv_count = 0
+        self.converged = False
+        self.F_prev = float("nan")
+        self.stats = {}
+
+    def g_explicit(self, phi):
+        """compute_phi_rhs_kernel at phi with the current x_B and the stress and
+        total strain of the last solve held fixed."""
+        o = self.o
+        keep = o.phi
+        o.phi = phi
+        try:
+            return o.phi_rhs()
+        finally:
+            o.phi = keep
+
+    def refresh_xB(self):
+        """x_B = sigma_clip(Y) (compute_mu_x_kernel, cuda_kernels.cu:1266-1324)."""
+        self.o.xB = th.sigmoid_from_logit(self.o.Y, self.p.Y_clip, self.p.xB_eps)
+
+    def energies(self):
+        p, o = self.p, self.o
+        e = o.energies()
+        out = {"F_surf": e["grad"] + e["dw"], "F_el": e["elastic"],
+               "interface_sum": (e["grad"] + e["dw"]) * o.phi.size, "el_core_sum": 0.0,
+               "F_chem_excess": e["chem"] - self.g_bulk0}
+        if p.elastic_enabled:
+            e0 = eigenstrain(p, o.phi, o.xB)
+            el = [o.eps[i] - e0[i] for i in range(6)]
+            s = o.sigma
+            gel = 0.5 * (s[0] * el[0] + s[1] * el[1] + s[2] * el[2]
+                         + 2.0 * (s[5] * el[5] + s[4] * el[4] + s[3] * el[3]))
+            out["el_core_sum"] = math.fsum((gel * th.h_of_phi(th.clamp01(o.phi))).ravel())
+        out["F_total_excess"] = out["F_chem_excess"] + out["F_surf"] + out["F_el"]
+        return out
+
+    def iterate(self):
+        p, opts, o = self.p, self.opts, self.o
+        self.iteration += 1
+        n = o.phi.size
+        phi_n = o.phi.copy()
+        Y_n = o.Y.copy()
+        if p.elastic_enabled:
+            o.elastic_solve()
+        mean_h0 = self.mean_h(phi_n)
+        if self.iteration == 1 and self.V0_target <= 0.0:
+            self.V0_target = mean_h0
+        en = self.energies()
+        F_prev = self.F_prev
+        self.F_prev = en["F_total_excess"]
+        # phi with the under-relaxed multiplier
+        self.refresh_xB()
+        gexp = self.g_explicit(phi_n)
+        phik = fwd(phi_n, o.ks)
+        gfull = gexp - p.kappa_phi * self.laplacian(phik)
+        lam_raw = self.multiplier(phi_n, gfull)
+        eta = min(max(opts.eta_lambda_vol, 0.0), 1.0)
+        self.lambda_vol = (1.0 - eta) * self.lambda_vol + eta * lam_raw
+        phi = self.semi_implicit(phik, gexp + self.lambda_vol * th.h_prime_of_phi(phi_n))
+        # KKT residual at phi^{n+1}
+        self.refresh_xB()
+        gexp1 = self.g_explicit(phi)
+        gfull1 = gexp1 - p.kappa_phi * self.laplacian(fwd(phi, o.ks))
+        lam_kkt = self.multiplier(phi, gfull1)
+        res = gfull1 + lam_kkt * th.h_prime_of_phi(phi)
+        res = np.where(((phi <= 1e-12) & (res > 0.0)) | ((phi >= 1.0 - 1e-12) & (res < 0.0)), 0.0, res)
+        rms_res = math.sqrt(math.fsum((res * res).ravel()) / n)
+        rms_dphi = math.sqrt(math.fsum(((phi - phi_n) ** 2).ravel()) / n) / opts.dt
+        # composition at the pseudo-time step
+        o.phi = phi
+        o.Y_update(phi_n, Y_n, y_cap=self.y_cap)
+        o.dYdt = (o.Y - Y_n) / p.dt
+        # post-projection with the updated composition, stress frozen
+        for _ in range(max(opts.post_projection_iters, 0)):
+            g_k = self.g_explicit(phi)
+            phik_k = fwd(phi, o.ks)
+            gf_k = g_k - p.kappa_phi * self.laplacian(phik_k)
+            self.lambda_vol = self.multiplier(phi, gf_k)
+            phi = self.semi_implicit(phik_k, g_k + self.lambda_vol * th.h_prime_of_phi(phi))
+        mean_h = self.mean_h(phi)
+        if self.V0_target > 1e-30 and math.isfinite(mean_h):
+            vol_err = self.V0_target - mean_h
+            if abs(vol_err) > 1e-14:
+                hp = th.h_prime_of_phi(phi)
+                mean_hp2 = math.fsum((hp * hp).ravel()) / n
+                lc = vol_err / mean_hp2 if mean_hp2 > 1e-30 else 0.0
+                phi = th.clamp01(phi + lc * hp)
+                mean_h = self.mean_h(phi)
+        o.phi = phi
+        rms_dY = math.sqrt(math.fsum(((o.Y - Y_n) ** 2).ravel()) / n) / p.dt
+        energy_diff_rel = (abs(en["F_total_excess"] - F_prev) / max(abs(F_prev), 1.0e-20)
+                           if math.isfinite(F_prev) else float("nan"))
+        vol_err_rel = (abs(mean_h - self.V0_target) / (self.V0_target + 1e-30)
+                       if self.V0_target > 1e-30 else 0.0)
+        fin = math.isfinite
+        vol_ok = fin(vol_err_rel) and vol_err_rel < opts.vol_err_rel_threshold
+        res_ok = fin(rms_res) and rms_res < opts.rms_res_threshold
+        condA = (fin(rms_dphi) and fin(rms_dY) and rms_dphi < opts.rms_dphi_threshold
+                 and rms_dY < opts.rms_dY_threshold and res_ok and vol_ok)
+        condB = (fin(energy_diff_rel) and energy_diff_rel < opts.energy_diff_rel_threshold
+                 and fin(rms_dphi) and fin(rms_dY) and rms_dphi < 5.0 * opts.rms_dphi_threshold
+                 and rms_dY < 5.0 * opts.rms_dY_threshold and res_ok and vol_ok)
+        t_ok = self.iteration * opts.dt + 1e-14 >= opts.min_pseudo_time
+        self.conv_count = self.conv_count + 1 if ((condA or condB) and t_ok) else 0
+        if self.conv_count >= opts.convergence_steps:
+            self.converged = True
+        self.stats = dict(mean_h=mean_h, V0=self.V0_target, lambda_vol=self.lambda_vol,
+                          F_surf=en["F_surf"], F_el=en["F_el"], interface_sum=en["interface_sum"],
+                          el_core_sum=en["el_core_sum"], F_chem_excess=en["F_chem_excess"],
+                          F_total_excess=en["F_total_excess"], rms_res=rms_res, rms_dphi=rms_dphi,
+                          rms_dY=rms_dY, energy_diff_rel=energy_diff_rel, vol_err_rel=vol_err_rel,
+                          lambda_raw=lam_raw, lambda_kkt=lam_kkt, elastic_iterations=o.last_elastic_iterations)
+        return self.stats

@@ -585,8 +585,10 @@ class Oracle:
             mu = mu + (-p.eps_iso_over_vB * (s[0] + s[1] + s[2]))
         return mu
 
-    def Y_update(self, phi_n, Y_n):
-        """main_cuda.cu:30283-30806 (lagged RHS) and the zero mode."""
+    def Y_update(self, phi_n, Y_n, y_cap=None):
+        """main_cuda.cu:30283-30806 (lagged RHS) and the zero mode.  y_cap is the
+        upper Y bound (Y_clip in dynamics; min(Y_clip, logit(minimize_xB_max_safe))
+        in full-model minimize, main_cuda.cu:30639-30646)."""
         p, ks, shape = self.p, self.ks, self.shape
         zm = bool(p.zero_mode)
         phi_np1 = self.phi
@@ -634,20 +636,21 @@ class Oracle:
         fk = fwd(f, ks)
         Yk = (Ynk + p.dt * fk) / (1.0 + p.dt * mean_DY * ks.k2) * ks.mask
         Ynew = inv(Yk, shape)
+        cap = p.Y_clip if y_cap is None else y_cap
         if zm:
-            self.zero_mode(Ynew)
+            self.zero_mode(Ynew, cap)
         else:
-            Ynew = np.clip(np.minimum(Ynew, p.Y_clip), -p.Y_clip, None)
+            Ynew = np.clip(np.minimum(Ynew, cap), -p.Y_clip, None)
             self.Y = Ynew
             self.xB = th.sigmoid_from_logit(Ynew, p.Y_clip, p.xB_eps)
         self.mean_DY = mean_DY
 
-    def zero_mode(self, Ystar):
+    def zero_mode(self, Ystar, Y_cap=None):
         """``run_pf_conserved_y_zero_mode_host`` (main_cuda.cu:4269-4405)."""
         p = self.p
         h = th.h_of_phi(self.phi)
         alpha = 1.0 - h
-        Y_cap = p.Y_clip
+        Y_cap = p.Y_clip if Y_cap is None else Y_cap
         ymin, ymax = float(Ystar.min()), float(Ystar.max())
         lo = -p.Y_clip - ymin
         hi = Y_cap - ymax

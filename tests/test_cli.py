@@ -82,10 +82,11 @@ def test_cli_raw_fields_run_matches_library(pf, tmp_path):
     assert np.abs(vals.transpose(2, 1, 0) - ref).max() <= 6e-6
 
 
-def test_minimize_full_model_is_refused(tmp_path):
+def test_minimize_constraints_are_refused(tmp_path):
     args, *_ = _bundle(tmp_path)
-    r = _run(args + ["--mode=minimize", "--minimize-full-model"], tmp_path)
-    assert r.returncode == 2 and "[fatal]" in r.stderr
+    for flag in ("--minimize-mass-constraint", "--minimize-component-volume-constraints"):
+        r = _run(args + ["--mode=minimize", "--minimize-full-model", flag], tmp_path)
+        assert r.returncode == 2 and "[fatal]" in r.stderr and flag in r.stderr
 
 
 @pytest.mark.gpu

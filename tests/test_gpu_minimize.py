@@ -117,3 +117,60 @@ def test_fp64_minimize_two_post_projections(pf):
         print(f"pp2 iter {it + 1}: dphi={dphi:.2e} lambda {sg['lambda_vol']:.12e}/{so['lambda_vol']:.12e}", flush=True)
         assert dphi < FP64_TOL
         assert _close(sg["lambda_vol"], so["lambda_vol"], 1e-9, 1e-12)
+
+
+# ---------------------------------------------------------------- full model
+def _full_case(n=16, elastic=True, hetero=True):
+    from cuda_sto_pf_amd.configs import config
+    pd, phi, xB, meta = config(2, n)
+    pd = dict(pd, zero_mode=0, elastic_enabled=1 if elastic else 0)
+    if not hetero:
+        pd["dC"] = np.zeros((6, 6))
+    return pd, phi, xB
+
+
+FULL_CASES = [("cfg2geom16_hetero",) + _full_case(), ("cfg2geom16_noelastic",) + _full_case(elastic=False)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,pd,phi,xB", FULL_CASES, ids=[c[0] for c in FULL_CASES])
+def test_fp64_full_model_minimize_parity(pf, name, pd, phi, xB):
+    """--minimize-full-model: per iteration phi and Y to 1e-10, the multiplier,
+    residuals, rms_dY, the excess energies and the elastic iteration counts."""
+    from oracle.minimize_ref import MinimizeFullOracle
+    opts = dict(dt=0.05, V0=0.0)
+    o = MinimizeFullOracle(Params(**pd), phi, xB, MinimizeOptions(**opts))
+    sim = pf.Simulation(dict(pd, dt=0.05), pf.PF_FP64)
+    sim.set_fields(phi, xB, xB_cap=0.07)
+    sim.minimize_begin(pf.minimize_opts(full_model=1, **opts))
+    for it in range(6):
+        so = o.iterate()
+        sg = sim.minimize_iterate()
+        dphi = np.abs(sim.field(pf.PF_PHI) - o.phi).max()
+        dY = np.abs(sim.field(pf.PF_Y) - o.o.Y).max()
+        print(f"{name} iter {it + 1}: dphi={dphi:.2e} dY={dY:.2e} F_tot {sg['F_total_excess_hat']:.12e}/"
+              f"{so['F_total_excess']:.12e} rms_dY {sg['rms_dY']:.6e}/{so['rms_dY']:.6e} "
+              f"el iters {sg['elastic_iterations']}/{so['elastic_iterations']}", flush=True)
+        assert dphi < FP64_TOL and dY < FP64_TOL
+        assert sg["elastic_iterations"] == so["elastic_iterations"]
+        for k, kg, rel, ab in (("lambda_vol", "lambda_vol", 1e-9, 1e-12), ("lambda_kkt", "lambda_kkt", 1e-9, 1e-12),
+                               ("rms_res", "rms_res", 1e-8, 1e-12), ("rms_dphi", "rms_dphi", 1e-6, 1e-9),
+                               ("rms_dY", "rms_dY", 1e-7, 1e-10), ("mean_h", "mean_h", 1e-12, 1e-15),
+                               ("F_surf", "F_surf_hat", 1e-10, 1e-15), ("F_el", "F_el_hat", 1e-10, 1e-15),
+                               ("F_chem_excess", "F_chem_excess_hat", 1e-9, 1e-13),
+                               ("F_total_excess", "F_total_excess_hat", 1e-9, 1e-13),
+                               ("el_core_sum", "el_core_sum", 1e-10, 1e-13)):
+            assert _close(sg[kg], so[k], rel, ab), (k, sg[kg], so[k])
+        if it > 0:
+            assert _close(sg["energy_diff_rel"], so["energy_diff_rel"], 1e-5, 1e-13)
+        else:
+            assert math.isnan(sg["energy_diff_rel"])
+
+
+@pytest.mark.gpu
+def test_full_model_minimize_refuses_the_zero_mode(pf):
+    pd, phi, xB = _full_case()
+    sim = pf.Simulation(dict(pd, zero_mode=1), pf.PF_FP64)
+    sim.set_fields(phi, xB, xB_cap=0.07)
+    with pytest.raises(pf.PfError, match="zero mode"):
+        sim.minimize_begin(pf.minimize_opts(full_model=1, dt=0.05))
