@@ -248,14 +248,17 @@ def cpu_baseline(pd=None, cap=None, snap=None, workers=None):
     return out
 
 
-def accuracy_run(pf, idx, steps):
-    """Both paths in lockstep over `steps` steps of workload `idx`."""
+def accuracy_run(pf, idx, steps, pair="mixed"):
+    """Both paths in lockstep over `steps` steps of workload `idx`: the FP64
+    reference iteration against the mixed accelerated path (pair "mixed") or
+    against the FP64 accelerated path (pair "fp64-accel", which separates the
+    solver tolerance's share of the difference from the precision's)."""
     from cuda_sto_pf_amd.configs import config
     pd, phi, xB, meta = config(idx)
     steps = steps or meta["steps"]
     cap = meta["xB_max_safe"]
     ref = pf.Simulation(pd, pf.PF_FP64)
-    fast = pf.Simulation(dict(pd, **ACCEL), pf.PF_MIXED)
+    fast = pf.Simulation(dict(pd, **ACCEL), pf.PF_MIXED if pair == "mixed" else pf.PF_FP64)
     ref.set_fields(phi, xB, xB_cap=cap)
     fast.set_fields(phi, xB, xB_cap=cap)
     m0 = ref.scalars()["mass_xBtot"]
@@ -278,7 +281,11 @@ def accuracy_run(pf, idx, steps):
         if s % every == 0 or s == steps:
             xr, xf = ref.field(pf.PF_XB), fast.field(pf.PF_XB)
             pr, pm = ref.field(pf.PF_PHI), fast.field(pf.PF_PHI)
-            row = {"step": s, "max_abs_dxB": float(np.abs(xr - xf).max()),
+            dxa = np.abs(xr - xf)
+            at = np.unravel_index(int(np.argmax(dxa)), dxa.shape)
+            row = {"step": s, "max_abs_dxB": float(dxa.max()),
+                   "p999_abs_dxB": float(np.quantile(dxa, 0.999)),
+                   "max_at": [int(v) for v in at], "phi_at_max": float(pr[at]), "xB_at_max": float(xr[at]),
                    "max_abs_dphi": float(np.abs(pr - pm).max()),
                    "mass_drift_rel_fp64": abs(sr["mass_xBtot"] / m0 - 1.0),
                    "mass_drift_rel_mixed": abs(sf["mass_xBtot"] / m0 - 1.0),
@@ -292,7 +299,7 @@ def accuracy_run(pf, idx, steps):
                   f"max|dphi|={row['max_abs_dphi']:.2e} mass drift {row['mass_drift_rel_fp64']:.1e}/"
                   f"{row['mass_drift_rel_mixed']:.1e} dF/F={row['f_total_rel_diff']:.1e} "
                   f"dFel/Fel={row['f_elastic_rel_diff']:.1e}", file=sys.stderr, flush=True)
-    return {"workload": meta["workload"], "grid": list(phi.shape), "steps": steps,
+    return {"workload": meta["workload"], "grid": list(phi.shape), "steps": steps, "pair": pair,
             "fp64_reference_iteration": {"steps_per_s": steps / t_r, "elastic_updates_per_step": it_r / steps},
             "mixed_accelerated": {"steps_per_s": steps / t_f, "elastic_updates_per_step": it_f / steps},
             "gate_max_abs_dxB": 1e-5, "max_abs_dxB": max(r["max_abs_dxB"] for r in rows), "rows": rows}
@@ -312,6 +319,8 @@ def main():
                     help="workload of the run (5: the 256^3 headline; 2 and 3 for --accuracy and profiling)")
     ap.add_argument("--accuracy", action="store_true",
                     help="long-horizon FP64 vs mixed comparison of --workload (no throughput line)")
+    ap.add_argument("--pair", choices=("mixed", "fp64-accel"), default="mixed",
+                    help="with --accuracy: the path compared with the FP64 reference iteration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: measures one GPU only (--gpus {args.gpus} requested); "
@@ -322,7 +331,7 @@ def main():
     from cuda_sto_pf_amd.configs import config
 
     if args.accuracy:
-        print(json.dumps(accuracy_run(pf, args.workload, args.steps)))
+        print(json.dumps(accuracy_run(pf, args.workload, args.steps, args.pair)))
         return
     if args.steps is None:
         args.steps = 20
