@@ -181,6 +181,7 @@ def config(idx: int, n=None):
     else:
         raise ValueError(f"unknown workload {idx}")
     meta["xB_max_safe"] = 0.5
+    meta["centres"] = [tuple(float(v) for v in c) for c in (centres if idx != 1 else [c])]
     return p, phi, xB, meta
 
 

@@ -19,8 +19,10 @@
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <cctype>
 #include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -147,6 +149,10 @@ void usage() {
       "  --mode=dynamics|minimize (the continue modes are not built)\n"
       "  --minimize-full-model    chemical drive, composition and volume constraint (no zero mode)\n"
       "  --minimize-rms-dY-threshold <v> --minimize-energy-diff-rel-threshold <v> --minimize-xB-max-safe <x>\n"
+      "  --minimize-mass-constraint [--minimize-target-mass-code <m>] [--minimize-mass-tolerance-relative <v>]\n"
+      "  [--minimize-mass-max-iterations <n>] --minimize-freeze-phi\n"
+      "  --minimize-component-volume-constraints --minimize-component-count <K>\n"
+      "  --minimize-component-label-raw <int32 raw> --minimize-component-target-h-sums <v0,v1,...>\n"
       "  --minimize-max-iter <n> --minimize-dt <dt> --V0 <v> --eta-lambda-vol <v>\n"
       "  --minimize-post-projection-iters <n> --minimize-rms-dphi-threshold <v>\n"
       "  --minimize-rms-res-threshold <v> --minimize-vol-err-rel-threshold <v>\n"
@@ -164,6 +170,68 @@ void usage() {
       "  --precision fp64|mixed   (default fp64)\n");
 }
 
+// "v0,v1,..." with exactly `count` positive finite entries
+// (parse_component_target_h_sums, main_cuda.cu:2835-2873)
+bool parse_target_h_sums(const std::string& text, int count, std::vector<double>* out) {
+  out->clear();
+  const char* p = text.c_str();
+  for (int i = 0; i < count; ++i) {
+    while (*p && std::isspace((unsigned char)*p)) ++p;
+    char* end = nullptr;
+    const double v = std::strtod(p, &end);
+    if (end == p || !std::isfinite(v) || v <= 0.0) {
+      std::fprintf(stderr, "[fatal] invalid component target h sum at entry %d in '%s'\n", i, text.c_str());
+      return false;
+    }
+    out->push_back(v);
+    p = end;
+    while (*p && std::isspace((unsigned char)*p)) ++p;
+    if (i + 1 < count) {
+      if (*p != ',') {
+        std::fprintf(stderr, "[fatal] expected comma after component target h sum %d in '%s'\n", i,
+                     text.c_str());
+        return false;
+      }
+      ++p;
+    }
+  }
+  while (*p && std::isspace((unsigned char)*p)) ++p;
+  if (*p) {
+    std::fprintf(stderr, "[fatal] component target h sums has extra entries: '%s'\n", text.c_str());
+    return false;
+  }
+  return true;
+}
+
+// exactly n little-endian int32 labels in [0, count)
+// (read_raw_component_labels_i32, main_cuda.cu:2805-2833)
+bool read_labels_i32(const std::string& path, size_t n, int count, std::vector<int>* out) {
+  FILE* fp = std::fopen(path.c_str(), "rb");
+  if (!fp) {
+    std::fprintf(stderr, "[fatal] cannot open component label raw field: %s\n", path.c_str());
+    return false;
+  }
+  std::vector<int32_t> tmp(n);
+  const size_t got = std::fread(tmp.data(), sizeof(int32_t), n, fp);
+  const int extra = std::fgetc(fp);
+  std::fclose(fp);
+  if (got != n || extra != EOF) {
+    std::fprintf(stderr, "[fatal] component label raw size mismatch: got %zu int32 values, expected exactly %zu\n",
+                 got, n);
+    return false;
+  }
+  out->resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    if (tmp[i] < 0 || tmp[i] >= count) {
+      std::fprintf(stderr, "[fatal] component label out of range at idx=%zu: %d not in [0,%d)\n", i, (int)tmp[i],
+                   count);
+      return false;
+    }
+    (*out)[i] = tmp[i];
+  }
+  return true;
+}
+
 // Minimize loop with the reference's energy log (main_cuda.cu:27440-27466,
 // 32455-32489), VTK outputs (build_case_vtk_path, main_cuda.cu:126-170) and
 // convergence report (main_cuda.cu:32546-32600).
@@ -174,6 +242,24 @@ int run_minimize(pf_handle* h, const pf_params& P, const pf_minimize_opts& mo, c
   std::snprintf(name, sizeof name, "/energy_minimize_T%.0f_r%.2f_xB%.3f_%s.csv", P.temperature_C, seed_radius,
                 xB_out, tag.c_str());
   FILE* efp = std::fopen((case_dir + name).c_str(), "w");
+  // main_cuda.cu:25667-25690
+  FILE* tfp = nullptr;
+  if (mo.mass_constraint) {
+    const std::string tp = case_dir + "/minimize_mass_constraint_trace.csv";
+    if (!(tfp = std::fopen(tp.c_str(), "w"))) {
+      if (efp) std::fclose(efp);
+      return fatal("cannot open minimize mass-constraint trace: " + tp);
+    }
+    std::fprintf(tfp,
+                 "iter,target_mass_code,current_mass_code,lambda,residual_code,residual_relative,derivative_code,"
+                 "iterations,newton_steps,bisection_steps,Y_star_min,Y_star_max,lambda_lower,lambda_upper,"
+                 "accepted_constraint_steps\n");
+  }
+  auto close_all = [&]() {
+    if (efp) std::fclose(efp);
+    if (tfp) std::fclose(tfp);
+    efp = tfp = nullptr;
+  };
   if (efp)
     std::fprintf(efp,
                  "iter,dt,mean_h,V0,lambda,F_surf_hat,F_el_hat,F_chem_excess_hat,F_total_excess_hat,"
@@ -186,24 +272,34 @@ int run_minimize(pf_handle* h, const pf_params& P, const pf_minimize_opts& mo, c
     write_vtk(case_dir + "/phi_" + sfx + "_" + tag + ".vtk", f, P.Nx, P.Ny, P.Nz, "phi", step);
   };
   dump("init", 0);
-  if (pf_minimize_begin(h, &mo) != PF_OK) return fatal(lib_error());
+  if (pf_minimize_begin(h, &mo) != PF_OK) {
+    close_all();
+    return fatal(lib_error());
+  }
   pf_minimize_stats st{};
   const double t0 = now_s();
   for (int it = 1; it <= mo.max_iter; ++it) {
     if (pf_minimize_iterate(h, &st) != PF_OK) {
-      if (efp) std::fclose(efp);
+      close_all();
       return fatal(lib_error() + " (minimize iteration " + std::to_string(it) + ")");
     }
     if (efp) {
-      const double nan = NAN;
       std::fprintf(efp,
                    "%d,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,"
                    "%.17e,%.17e,%.17e,%.17e,%d\n",
                    it, mo.dt, st.mean_h, st.V0, st.lambda_vol, st.F_surf_hat, st.F_el_hat, st.F_chem_excess_hat,
                    st.F_total_excess_hat, st.F_chem_CNT_hat, st.F_total_CNT_hat, st.interface_sum, st.el_core_sum,
-                   st.rms_res, st.rms_dphi, st.rms_dY, st.energy_diff_rel, st.vol_err_rel, nan, nan, nan, nan,
-                   mo.post_projection_iters);
+                   st.rms_res, st.rms_dphi, st.rms_dY, st.energy_diff_rel, st.vol_err_rel, st.mass_target,
+                   st.mass_current, st.mass_err_rel, st.mass_lambda, mo.post_projection_iters);
       std::fflush(efp);
+    }
+    if (tfp) {
+      std::fprintf(tfp, "%d,%.17e,%.17e,%.17e,%.17e,%.17e,%.17e,%d,%d,%d,%.17e,%.17e,%.17e,%.17e,%d\n", it,
+                   st.mass_target, st.mass_current, st.mass_lambda, st.mass_residual,
+                   st.mass_residual / std::fmax(std::fabs(st.mass_target), 1.0), st.mass_derivative,
+                   st.mass_iterations, st.mass_newton_steps, st.mass_bisection_steps, st.mass_Ystar_min,
+                   st.mass_Ystar_max, st.mass_lambda_lower, st.mass_lambda_upper, it);
+      std::fflush(tfp);
     }
     if (P.out_every > 0 && it % P.out_every == 0) dump(std::to_string(it), it);
     if (st.converged) {
@@ -219,7 +315,7 @@ int run_minimize(pf_handle* h, const pf_params& P, const pf_minimize_opts& mo, c
     std::printf("[minimize] reached max_iter=%d without convergence (rms_res=%.3e rms_dphi=%.3e)\n", mo.max_iter,
                 st.rms_res, st.rms_dphi);
   dump("final", st.iteration);
-  if (efp) std::fclose(efp);
+  close_all();
   FILE* pj = std::fopen((case_dir + "/minimize_summary.json").c_str(), "w");
   if (pj) {
     std::fprintf(pj,
@@ -247,6 +343,10 @@ int main(int argc, char** argv) {
   pf_minimize_opts mo;
   pf_minimize_opts_default(&mo);
   bool minimize_dt_set = false, max_iter_set = false;
+  // V5 component constraints (main_cuda.cu:23740-23760)
+  bool comp_enabled = false;
+  int comp_count = 0;
+  std::string comp_label_raw, comp_targets_text;
   double seed_radius = 10.0;   // ic_phi_seed_radius default (main_cuda.cu:16221)
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
@@ -306,7 +406,8 @@ int main(int argc, char** argv) {
                a == "--minimize-vol-err-rel-threshold" || a == "--minimize-convergence-steps" ||
                a == "--minimize-min-pseudo-time" || a == "--radius" || a == "--minimize-rms-dY-threshold" ||
                a == "--minimize-rms-dy-threshold" || a == "--minimize-energy-diff-rel-threshold" ||
-               a == "--minimize-xB-max-safe") {
+               a == "--minimize-xB-max-safe" || a == "--minimize-target-mass-code" ||
+               a == "--minimize-mass-tolerance-relative" || a == "--minimize-mass-max-iterations") {
       // main_cuda.cu:23762-23862
       std::string v;
       if (!next(&v)) return fatal(a + " needs a value");
@@ -339,14 +440,31 @@ int main(int argc, char** argv) {
         mo.energy_diff_rel_threshold = x;
       } else if (a == "--minimize-xB-max-safe") {
         mo.xB_max_safe = x;
+      } else if (a == "--minimize-target-mass-code") {
+        mo.target_mass_code = x;
+      } else if (a == "--minimize-mass-tolerance-relative") {
+        mo.mass_tolerance_relative = x;
+      } else if (a == "--minimize-mass-max-iterations") {
+        mo.mass_max_iterations = std::atoi(v.c_str());
       } else {
         seed_radius = x;
       }
     } else if (a == "--minimize-full-model" || a == "--minimize-full") {
       mo.full_model = 1;   // main_cuda.cu:23726-23730
-    } else if (a == "--minimize-mass-constraint" || a == "--minimize-freeze-phi" ||
-               a == "--minimize-component-volume-constraints") {
-      return fatal(a + " is not implemented (phi-only and full-model minimize without constraints)");
+    } else if (a == "--minimize-mass-constraint") {
+      mo.mass_constraint = 1;   // main_cuda.cu:23732-23735
+    } else if (a == "--minimize-freeze-phi") {
+      mo.freeze_phi = 1;
+    } else if (a == "--minimize-component-volume-constraints") {
+      comp_enabled = true;
+    } else if (a == "--minimize-component-count") {
+      std::string v;
+      if (!next(&v)) return fatal(a + " needs a value");
+      comp_count = std::atoi(v.c_str());
+    } else if (a == "--minimize-component-label-raw") {
+      if (!next(&comp_label_raw)) return fatal(a + " needs a value");
+    } else if (a == "--minimize-component-target-h-sums") {
+      if (!next(&comp_targets_text)) return fatal(a + " needs a value");
     } else if (starts_with(a.c_str(), "--")) {
       std::fprintf(stderr, "[warn] flag not handled by main_pf: %s\n", a.c_str());
     }
@@ -356,6 +474,28 @@ int main(int argc, char** argv) {
   if (mode != "dynamics" && mode != "minimize")
     return fatal("--mode=" + mode + " is not implemented (dynamics and phi-only minimize)");
   const bool minimize = mode == "minimize";
+  // main_cuda.cu:24562-24623
+  if (mo.mass_constraint && !(minimize && mo.full_model))
+    return fatal("--minimize-mass-constraint requires --mode=minimize --minimize-full-model");
+  if (mo.freeze_phi && !(minimize && mo.full_model && mo.mass_constraint))
+    return fatal("--minimize-freeze-phi requires --mode=minimize --minimize-full-model --minimize-mass-constraint");
+  std::vector<double> comp_targets;
+  if (comp_enabled && minimize && mo.full_model) {
+    // main_cuda.cu:24638-24670
+    if (!mo.mass_constraint || mo.freeze_phi)
+      return fatal("--minimize-component-volume-constraints requires full-model constrained minimize with phi "
+                   "evolution enabled");
+    if (comp_count < 2 || comp_count > 64 || comp_label_raw.empty() || comp_targets_text.empty())
+      return fatal("V5 component constraints require component count in [2,64], a label raw path, and exactly one "
+                   "positive target h sum per component");
+    if (mo.post_projection_iters != 0 || mo.V0 > 0.0)
+      return fatal("V5 component constraints own the phi volume projection; require "
+                   "--minimize-post-projection-iters 0 and V0<=0");
+    if (!parse_target_h_sums(comp_targets_text, comp_count, &comp_targets)) return 2;
+  } else if (comp_enabled) {
+    return fatal("--minimize-component-volume-constraints requires full-model constrained minimize with phi "
+                 "evolution enabled");
+  }
 
   pf_params P;
   pf_params_default(&P);
@@ -479,6 +619,13 @@ int main(int argc, char** argv) {
   for (size_t i = 0; i < N; ++i) xB_out += xB[i];
   xB_out /= (double)N;
 
+  std::vector<int> comp_labels;
+  if (!comp_targets.empty()) {
+    if (!read_labels_i32(comp_label_raw, N, comp_count, &comp_labels)) return 2;
+    mo.component_count = comp_count;
+    mo.component_labels = comp_labels.data();
+    mo.component_target_h_sums = comp_targets.data();
+  }
   pf_handle* h = nullptr;
   if (pf_create(&P, precision, &h) != PF_OK) return fatal(lib_error());
   if (pf_set_fields(h, phi.data(), xB.data(), dydt.empty() ? nullptr : dydt.data(), xB_cap) != PF_OK)

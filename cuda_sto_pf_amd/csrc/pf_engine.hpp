@@ -267,14 +267,23 @@ __global__ void __launch_bounds__(NT) grad_energy_kernel(const double* phi, int 
 // same FP64 operations.  Steps enqueued beyond convergence see ZS_ACTIVE == 0
 // and return at once.
 enum ZmSlot {
-  ZS_LO = 0, ZS_HI, ZS_LAM, ZS_R, ZS_DERIV, ZS_INVALID, ZS_ACTIVE, ZS_STATUS, ZS_ITERS, ZS_NEXT,
+  ZS_LO = 0, ZS_HI, ZS_LAM, ZS_R, ZS_DERIV, ZS_INVALID, ZS_ACTIVE, ZS_STATUS, ZS_ITERS, ZS_NEXT, ZS_NEWTON,
+  ZS_BISECT,
   ZS_SUMS = 16,   // 9 sums of the opening pass, then 3 of each step
   ZS_COUNT = 32
 };
 enum ZmStatus { ZM_OK = 0, ZM_EMPTY_BRACKET = 1, ZM_NONFINITE = 2, ZM_OUTSIDE = 3 };
+// Y* + lambda is bounded to [-Y_clip, Y_cap]: Y_cap = Y_clip in the dynamics,
+// the minimize cap logit(xB_max_safe) under the minimize mass constraint
 struct ZmConst {
-  double Y_clip, vB, target, tol;
+  double Y_clip, Y_cap, vB, target, tol;
   int max_it;
+};
+// an accepted (or the last rejected) search: shift, residual, d mass/d lambda, steps
+struct ZmResultT {
+  double lam = NAN, r = NAN, deriv = NAN;
+  int iters = 0, newton = 0, bisect = 0;
+  double ymin = NAN, ymax = NAN, lo = NAN, hi = NAN;   // Y* extremes, final bracket
 };
 
 constexpr int ZM_U = 4;   // grid strides per iteration of the opening zero-mode evaluation
@@ -292,9 +301,10 @@ __device__ __forceinline__ void zm_terms(double ys, double ph, double lam, doubl
 
 // opening pass: lambda = lo, hi, clamp(0, lo, hi) from the Y* extremes ymm[0..1]
 __global__ void __launch_bounds__(NT) zm_eval3_kernel(const double* Ystar, const double* phi, const double* ymm,
-                                                      double Y_clip, double vB, long n, double* partials) {
+                                                      double Y_clip, double Y_cap, double vB, long n,
+                                                      double* partials) {
   const double lo = -Y_clip - ymm[0];
-  const double hi = Y_clip - ymm[1];
+  const double hi = Y_cap - ymm[1];
   const double lam0 = fmin(fmax(0.0, lo), hi);
   double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   // ZM_U grid strides' loads issued together; the terms are added in the
@@ -311,9 +321,9 @@ __global__ void __launch_bounds__(NT) zm_eval3_kernel(const double* Ystar, const
 #pragma unroll
     for (int u = 0; u < ZM_U; ++u) {
       if (g0 + u * stride >= n) break;
-      zm_terms(ys[u], ph[u], lo, vB, -Y_clip, Y_clip, acc);
-      zm_terms(ys[u], ph[u], hi, vB, -Y_clip, Y_clip, acc + 3);
-      zm_terms(ys[u], ph[u], lam0, vB, -Y_clip, Y_clip, acc + 6);
+      zm_terms(ys[u], ph[u], lo, vB, -Y_clip, Y_cap, acc);
+      zm_terms(ys[u], ph[u], hi, vB, -Y_clip, Y_cap, acc + 3);
+      zm_terms(ys[u], ph[u], lam0, vB, -Y_clip, Y_cap, acc + 6);
     }
   }
   int kinds[9];
@@ -324,12 +334,13 @@ __global__ void __launch_bounds__(NT) zm_eval3_kernel(const double* Ystar, const
 
 // one search step at st[ZS_NEXT]
 __global__ void __launch_bounds__(NT) zm_eval_kernel(const double* Ystar, const double* phi, const double* st,
-                                                     double Y_clip, double vB, long n, double* partials) {
+                                                     double Y_clip, double Y_cap, double vB, long n,
+                                                     double* partials) {
   if (st[ZS_ACTIVE] == 0.0) return;
   const double lam = st[ZS_NEXT];
   double acc[3] = {0.0, 0.0, 0.0};
   for (long g = blockIdx.x * (long)NT + threadIdx.x; g < n; g += (long)gridDim.x * NT)
-    zm_terms(Ystar[g], phi[g], lam, vB, -Y_clip, Y_clip, acc);
+    zm_terms(Ystar[g], phi[g], lam, vB, -Y_clip, Y_cap, acc);
   int kinds[3] = {RED_SUM, RED_SUM, RED_SUM};
   block_reduce_store<3>(acc, kinds, partials + (size_t)blockIdx.x * 3);
 }
@@ -350,7 +361,12 @@ __device__ void zm_propose(double* st, const ZmConst& c) {
   const double deriv = st[ZS_DERIV];
   double cand = NAN;
   if (deriv > 0.0 && isfinite(deriv)) cand = lam - r / deriv;
-  if (!(isfinite(cand) && cand > lo && cand < hi)) cand = 0.5 * (lo + hi);
+  if (isfinite(cand) && cand > lo && cand < hi) {
+    st[ZS_NEWTON] += 1.0;
+  } else {
+    cand = 0.5 * (lo + hi);
+    st[ZS_BISECT] += 1.0;
+  }
   st[ZS_LO] = lo;
   st[ZS_HI] = hi;
   st[ZS_NEXT] = cand;
@@ -360,9 +376,13 @@ __device__ void zm_propose(double* st, const ZmConst& c) {
 __device__ void zm_ctrl_open(double* st, const double* ymm, ZmConst c) {
   const double* s = st + ZS_SUMS;
   const double lo = -c.Y_clip - ymm[0];
-  const double hi = c.Y_clip - ymm[1];
+  const double hi = c.Y_cap - ymm[1];
   st[ZS_ACTIVE] = 0.0;
   st[ZS_ITERS] = 0.0;
+  st[ZS_NEWTON] = 0.0;
+  st[ZS_BISECT] = 0.0;
+  st[ZS_LO] = lo;
+  st[ZS_HI] = hi;
   st[ZS_LAM] = NAN;
   st[ZS_R] = NAN;
   st[ZS_INVALID] = 0.0;
@@ -583,15 +603,16 @@ __global__ void __launch_bounds__(NT) aa_end_kernel(const double* partials, int 
 }
 
 // Y = Y* + lambda, x_B = sigma(Y) unclipped (apply_pf_Y_zero_mode_shift_kernel
-// 3434-3443), dY/dt = (Y - Y^n)/dt (update_dY_dt_prev_kernel 3545-3556).
+// 3434-3443), dY/dt = (Y - Y^n)/dt (update_dY_dt_prev_kernel 3545-3556) when
+// dYdt is given; the minimize mass constraint keeps the dY/dt of its clamp
+// (main_cuda.cu:31622 runs before the constraint at 32297).
 __global__ void __launch_bounds__(NT) zm_apply_kernel(const double* Ystar, double lam, double* Y,
                                                       double* xB, double* dYdt, double dt, long n) {
   for (long g = blockIdx.x * (long)NT + threadIdx.x; g < n; g += (long)gridDim.x * NT) {
     const double y = Ystar[g] + lam;
-    const double yn = Y[g];
+    if (dYdt) dYdt[g] = (y - Y[g]) / dt;
     Y[g] = y;
     xB[g] = sigmoid_unclipped(y);
-    dYdt[g] = (y - yn) / dt;
   }
 }
 
@@ -933,6 +954,16 @@ struct Engine final : EngineBase {
   // (main_cuda.cu:27400-27432), the Y cap of minimize_xB_max_safe
   // (main_cuda.cu:30639-30646) and the previous iteration's excess energy
   double m_xref = 0.0, m_gbulk0 = 0.0, m_ycap = 0.0, m_Fprev = NAN;
+  double m_mass_target = NAN;   // minimize mass constraint: the conserved sum of x_Btot
+  ZmResultT m_mass{};           // its last accepted search
+  // V5 component constraints: labels, device multipliers [0, 64), KKT
+  // multipliers / projection corrections [64, 128), per-label sums [128,
+  // 128 + 3*64), then the per-block partials
+  int m_ncomp = 0;
+  int* m_labels = nullptr;
+  double* m_cbuf = nullptr;
+  std::vector<double> m_ctarget, m_clambda, m_csum;
+  double m_cerr = NAN;
   const double* xB_el() const { return (minimizing && !mopt.full_model) ? m_xzero : xB; }
   // cold-start hooks: learned correction and caller callback
   nn::Net* nn_net = nullptr;
@@ -1729,21 +1760,37 @@ struct Engine final : EngineBase {
   int aa_spec = 4;                     // iterations enqueued before the solver's first wait
   const double* gate_ = nullptr;       // passes do nothing while *gate_ == 0 (set during the solve)
   int zero_mode(const double* phi_np1, int nfetch) {
-    double* st = dscal + ZM_SLOT;
-    const ZmConst c{P.Y_clip, P.v_B, target_mass,
+    const ZmConst c{P.Y_clip, P.Y_clip, P.v_B, target_mass,
                     prm.zero_mode_tolerance_relative * std::fmax(std::fabs(target_mass), 1.0),
                     prm.zero_mode_max_iterations};
+    ZmResult res;
+    int rc = zero_mode_solve(phi_np1, nfetch, c, "PF zero-mode", &res);
+    zm_iters = res.iters;
+    if (rc) return rc;
+    zm_lambda = res.lam;
+    zm_res = res.r;
+    zm_deriv = res.deriv;
+    ++zm_accepted;
+    hipLaunchKernelGGL(zm_apply_kernel, dim3((unsigned)((R + NT - 1) / NT)), dim3(NT), 0, stream, Ystar, res.lam,
+                       Y, xB, dYdt, prm.dt, R);
+    return PF_OK;
+  }
+  // the search on Y* (member Ystar) with the extremes in dscal[0..1]; on
+  // success res holds the accepted shift, nothing is applied yet
+  using ZmResult = ZmResultT;
+  int zero_mode_solve(const double* phi_np1, int nfetch, const ZmConst& c, const char* what, ZmResult* res) {
+    double* st = dscal + ZM_SLOT;
     const int g = grid_for(R);
     hipLaunchKernelGGL(zm_eval3_kernel, dim3(g), dim3(NT), 0, stream, Ystar, phi_np1, (const double*)dscal,
-                       P.Y_clip, P.v_B, R, partials);
+                       c.Y_clip, c.Y_cap, c.vB, R, partials);
     hipLaunchKernelGGL(zm_open_end_kernel, dim3(1), dim3(NT), 0, stream, partials, g, st, (const double*)dscal,
                        c);
-    int left = std::max(prm.zero_mode_max_iterations, 0);
+    int left = std::max(c.max_it, 0);
     int batch = std::min(zm_spec, left);
     for (;;) {
       for (int k = 0; k < batch; ++k) {
         hipLaunchKernelGGL(zm_eval_kernel, dim3(g), dim3(NT), 0, stream, Ystar, phi_np1, (const double*)st,
-                           P.Y_clip, P.v_B, R, partials);
+                           c.Y_clip, c.Y_cap, c.vB, R, partials);
         hipLaunchKernelGGL(zm_step_end_kernel, dim3(1), dim3(NT), 0, stream, partials, g, st, c);
       }
       left -= batch;
@@ -1755,26 +1802,33 @@ struct Engine final : EngineBase {
     }
     const double* hs = hscal + ZM_SLOT;
     const int status = (int)hs[ZS_STATUS];
-    if (status == ZM_EMPTY_BRACKET) return fail(PF_E_ZERO_MODE, "PF zero-mode rejected: empty bracket");
-    if (status == ZM_NONFINITE) return fail(PF_E_ZERO_MODE, "zero-mode evaluation non-finite");
-    if (status == ZM_OUTSIDE) return fail(PF_E_ZERO_MODE, "PF zero-mode rejected: target outside bracket");
+    res->ymin = hscal[0];
+    res->ymax = hscal[1];
+    res->lo = hs[ZS_LO];
+    res->hi = hs[ZS_HI];
+    char buf[256];
+    if (status != ZM_OK) {
+      std::snprintf(buf, sizeof buf, "%s rejected: %s", what,
+                    status == ZM_EMPTY_BRACKET ? "empty bracket"
+                    : status == ZM_NONFINITE   ? "evaluation non-finite"
+                                               : "target outside bracket");
+      return fail(PF_E_ZERO_MODE, buf);
+    }
     const int iters = (int)hs[ZS_ITERS];
     // enqueue one step more than this search took, next time
     zm_spec = std::min(std::max(iters + 1, 1), 8);
     const double r = hs[ZS_R], lam = hs[ZS_LAM], invalid = hs[ZS_INVALID];
-    zm_iters = iters;
+    res->iters = iters;
+    res->newton = (int)hs[ZS_NEWTON];
+    res->bisect = (int)hs[ZS_BISECT];
+    res->lam = lam;
+    res->r = r;
+    res->deriv = hs[ZS_DERIV];
     if (std::fabs(r) > c.tol || invalid > 0.0) {
-      char buf[256];
-      std::snprintf(buf, sizeof buf, "PF zero-mode rejected: residual=%.17e lambda=%.17e invalid=%.0f", r, lam,
+      std::snprintf(buf, sizeof buf, "%s rejected: residual=%.17e lambda=%.17e invalid=%.0f", what, r, lam,
                     invalid);
       return fail(PF_E_ZERO_MODE, buf);
     }
-    zm_lambda = lam;
-    zm_res = r;
-    zm_deriv = hs[ZS_DERIV];
-    ++zm_accepted;
-    hipLaunchKernelGGL(zm_apply_kernel, dim3((unsigned)((R + NT - 1) / NT)), dim3(NT), 0, stream, Ystar, lam, Y,
-                       xB, dYdt, prm.dt, R);
     return PF_OK;
   }
 
@@ -1948,14 +2002,14 @@ struct Engine final : EngineBase {
     if (o.full_model) {
       if (prm.zero_mode)
         return fail(PF_E_ARG,
-                    "full-model minimize runs the composition update with the clamp; with the zero mode it is "
-                    "not built (its root find has symmetric Y bounds, the minimize Y cap is not)");
+                    "full-model minimize with the dynamics zero mode in its composition update is not built; "
+                    "the minimize mass constraint conserves the composition instead");
       y_banded = false;
       // chemical baseline (main_cuda.cu:27400-27432): ic_23d_xB_out, else <x_B>
       double xr = prm.ic_23d_xB_out;
       if (!(xr > 0.0)) {
         const int g = grid_for(R);
-        hipLaunchKernelGGL(sum_sq_kernel, dim3(g), dim3(NT), 0, stream, (const double*)xB, R, partials);
+        hipLaunchKernelGGL(sum_sq_kernel, dim3(g), dim3(NT), 0, stream, (const double*)xB, 0.0, R, partials);
         finalize(g, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal + 58, 1.0);
         if ((rc = fetch(60))) return rc;
         xr = hscal[58] / (double)R;
@@ -1967,7 +2021,140 @@ struct Engine final : EngineBase {
       if (c > 1.0 - prm.xB_eps) c = 1.0 - prm.xB_eps;
       m_ycap = std::fmin(prm.Y_clip, std::log(c / (1.0 - c)));
       m_Fprev = NAN;
+      m_mass = ZmResultT{};
+      m_mass_target = NAN;
+      if (o.mass_constraint) {
+        // main_cuda.cu:24599-24614: the constraint owns the Y zero mode
+        if (prm.enable_Y_rhs_previous_time_level || prm.disable_Y_rhs_gamma_term || prm.Y_rhs_term_h_scale != 1.0)
+          return fail(PF_E_ARG,
+                      "the minimize mass constraint requires the legacy Y update: previous-time-level rhs off, "
+                      "gamma term on, Y_rhs_term_h_scale 1");
+        // the exact initial ledger, or an explicit target that must match it
+        // (main_cuda.cu:28117-28146)
+        double m, vf;
+        if ((rc = mass_vf(&m, &vf))) return rc;
+        const double t = o.target_mass_code > 0.0 ? o.target_mass_code : m;
+        const double tol = o.mass_tolerance_relative * std::fmax(std::fabs(t), 1.0);
+        if (!std::isfinite(m) || std::fabs(m - t) > tol) {
+          char buf[256];
+          std::snprintf(buf, sizeof buf,
+                        "explicit minimize target mass does not match the initial ledger: current=%.17e "
+                        "target=%.17e residual=%.17e tolerance=%.17e",
+                        m, t, m - t, tol);
+          return fail(PF_E_ARG, buf);
+        }
+        m_mass_target = t;
+      }
+      m_ncomp = 0;
+      if (o.component_count > 0) {
+        // ownership labels (read_raw_component_labels_i32, main_cuda.cu:2805-2833,
+        // 27262-27290): every label in range and owning a cell
+        const int K = o.component_count;
+        std::vector<long> owned(K, 0);
+        for (long i = 0; i < R; ++i) {
+          const int c = o.component_labels[i];
+          if (c < 0 || c >= K) {
+            char buf[160];
+            std::snprintf(buf, sizeof buf, "component label out of range at idx=%ld: %d not in [0,%d)", i, c, K);
+            return fail(PF_E_ARG, buf);
+          }
+          ++owned[c];
+        }
+        for (int c = 0; c < K; ++c)
+          if (owned[c] == 0) {
+            char buf[96];
+            std::snprintf(buf, sizeof buf, "V5 component %d owns no grid cells", c);
+            return fail(PF_E_ARG, buf);
+          }
+        const int nb = comp_blocks();
+        if (!m_labels && ((rc = dalloc(&m_labels, R)) ||
+                          (rc = dalloc(&m_cbuf, 128 + 3 * COMP_MAX + (size_t)nb * 3 * COMP_MAX))))
+          return rc;
+        PF_HIP(hipMemcpy(m_labels, o.component_labels, R * sizeof(int), hipMemcpyHostToDevice));
+        m_ctarget.assign(o.component_target_h_sums, o.component_target_h_sums + K);
+        m_clambda.assign(K, 0.0);
+        m_csum.assign(3 * K, 0.0);
+        m_cerr = NAN;
+        m_ncomp = K;
+      }
+    } else if (o.mass_constraint || o.freeze_phi || o.component_count) {
+      return fail(PF_E_ARG, "the mass constraint, freeze_phi and the component constraints belong to the "
+                            "full-model minimize");
     }
+    return PF_OK;
+  }
+
+  int comp_blocks() const { return (int)((R + (long)NT * COMP_VPT - 1) / ((long)NT * COMP_VPT)); }
+  double* comp_lambda_dev() { return m_cbuf; }
+  double* comp_aux_dev() { return m_cbuf + 64; }
+  // per-label sums of h' g (g may be null), h'^2 and h(clamp01 phi) into m_csum
+  int comp_moments(const double* ph, const double* g) {
+    const int nb = comp_blocks();
+    double* sums = m_cbuf + 128;
+    double* part = sums + 3 * COMP_MAX;
+    hipLaunchKernelGGL(comp_moments_kernel, dim3(nb), dim3(NT), 0, stream, ph, g, (const int*)m_labels, m_ncomp,
+                       R, part);
+    hipLaunchKernelGGL(comp_sum_kernel, dim3(1), dim3(NT), 0, stream, (const double*)part, nb, 3 * m_ncomp, sums);
+    PF_HIP(hipMemcpyAsync(m_csum.data(), sums, 3 * m_ncomp * sizeof(double), hipMemcpyDeviceToHost, stream));
+    PF_HIP(hipStreamSynchronize(stream));
+    return PF_OK;
+  }
+  // -sum h' g / sum h'^2 per label (main_cuda.cu:29468-29478, 29694-29700)
+  void comp_multipliers(std::vector<double>* lam) {
+    lam->resize(m_ncomp);
+    for (int c = 0; c < m_ncomp; ++c) {
+      const double d = m_csum[3 * c + 1];
+      (*lam)[c] = std::fabs(d) > 1.0e-30 ? -m_csum[3 * c] / d : 0.0;
+    }
+  }
+  int comp_upload(double* dst, const std::vector<double>& v) {
+    PF_HIP(hipMemcpyAsync(dst, v.data(), v.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+    PF_HIP(hipStreamSynchronize(stream));   // v is pageable host memory reused by the caller
+    return PF_OK;
+  }
+  // the per-label projection: up to three first-order corrections, then the
+  // largest relative error of the label sums (main_cuda.cu:32207-32275)
+  int comp_project(double* ph) {
+    int rc;
+    std::vector<double> corr(m_ncomp);
+    for (int k = 0; k < 3; ++k) {
+      if ((rc = comp_moments(ph, nullptr))) return rc;
+      bool need = false;
+      for (int c = 0; c < m_ncomp; ++c) {
+        const double delta = m_ctarget[c] - m_csum[3 * c + 2];
+        corr[c] = m_csum[3 * c + 1] > 1.0e-30 ? delta / m_csum[3 * c + 1] : 0.0;
+        if (std::fabs(delta) > 1.0e-14) need = true;
+      }
+      if (!need) break;
+      if ((rc = comp_upload(comp_aux_dev(), corr))) return rc;
+      hipLaunchKernelGGL(comp_project_kernel, dim3(grid_for(R)), dim3(NT), 0, stream, ph, (const int*)m_labels,
+                         (const double*)comp_aux_dev(), R);
+    }
+    if ((rc = comp_moments(ph, nullptr))) return rc;
+    m_cerr = 0.0;
+    for (int c = 0; c < m_ncomp; ++c)
+      m_cerr = std::fmax(m_cerr, std::fabs(m_csum[3 * c + 2] - m_ctarget[c]) / std::fmax(m_ctarget[c], 1.0e-30));
+    return PF_OK;
+  }
+
+  // The minimize mass constraint (main_cuda.cu:32297-32366): the zero-mode
+  // search on the clamped Y of this iteration at the final phi, with Y bounded
+  // to [-Y_clip, logit(xB_max_safe)]; Y and x_B take the shift, dY/dt keeps
+  // the clamp's.  m_mass holds the search.
+  int min_mass_constraint(const double* ph) {
+    const int gR = grid_for(R);
+    hipLaunchKernelGGL(copy_minmax_kernel, dim3(gR), dim3(NT), 0, stream, (const double*)Y, Ystar, R, partials);
+    finalize(gR, 2, RedKinds{{RED_MIN, RED_MAX, 0, 0}}, dscal, 1.0);
+    const ZmConst c{P.Y_clip, m_ycap, P.v_B, m_mass_target,
+                    mopt.mass_tolerance_relative * std::fmax(std::fabs(m_mass_target), 1.0),
+                    mopt.mass_max_iterations};
+    char what[96];
+    std::snprintf(what, sizeof what, "minimize mass constraint (iteration %d)", m_iter);
+    m_mass = ZmResultT{};
+    int rc = zero_mode_solve(ph, 2, c, what, &m_mass);
+    if (rc) return rc;
+    hipLaunchKernelGGL(zm_apply_kernel, dim3((unsigned)((R + NT - 1) / NT)), dim3(NT), 0, stream, Ystar,
+                       m_mass.lam, Y, xB, (double*)nullptr, mopt.dt, R);
     return PF_OK;
   }
 
@@ -1991,9 +2178,9 @@ struct Engine final : EngineBase {
     finalize(gz, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal + o, 1.0);
   }
   // semi-implicit phi update from the field phi - L dt (g_exp + lambda h') of ph
-  void min_update(const double* ph, double lambda) {
+  void min_update(const double* ph, double lambda, const int* labels = nullptr, const double* lambdas = nullptr) {
     const double Ldt = prm.L_phi * mopt.dt;
-    typename OpMinUpd<T>::Args oa{ph, m_gexp, lambda, Ldt};
+    typename OpMinUpd<T>::Args oa{ph, m_gexp, lambda, Ldt, labels, lambdas};
     launch_zpass<T, 0, 1, OpMinUpd<T>>(Nz, stream, zargs(0, 0, 0, 1), oa);
     min_forward_to_sc0();
     hipLaunchKernelGGL(kop_phi<T>, dim3(grid_for(nSc)), dim3(NT), 0, stream, kd(), nSc, sc(0),
@@ -2063,7 +2250,8 @@ struct Engine final : EngineBase {
     if ((rc = fetch(58))) return rc;
     const double rms_dphi = std::sqrt(hscal[52] / (double)R) / mopt.dt;
     const double lam_kkt = min_multiplier(hscal[53], hscal[54]);
-    hipLaunchKernelGGL(min_kkt_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, m_gfull, lam_kkt, R, partials);
+    hipLaunchKernelGGL(min_kkt_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, m_gfull, lam_kkt,
+                         (const int*)nullptr, (const double*)nullptr, R, partials);
     finalize(gR, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 55, 1.0);
     // post-projection sub-steps, eta = 1, stress frozen (main_cuda.cu:32104-32197)
     double lam_pp = lam_kkt;
@@ -2139,6 +2327,10 @@ struct Engine final : EngineBase {
       st->lambda_kkt = lam_kkt;
       st->F_chem_excess_hat = st->F_total_excess_hat = st->F_chem_CNT_hat = st->F_total_CNT_hat = NAN;
       st->rms_dY = st->energy_diff_rel = NAN;
+      st->mass_target = st->mass_current = st->mass_err_rel = st->mass_lambda = NAN;
+      st->mass_residual = st->mass_derivative = st->mass_Ystar_min = st->mass_Ystar_max = NAN;
+      st->mass_lambda_lower = st->mass_lambda_upper = NAN;
+      st->mass_iterations = st->mass_newton_steps = st->mass_bisection_steps = 0;
     }
     if (!std::isfinite(mean_h) || !std::isfinite(m_lambda))
       return fail(PF_E_NONFINITE, "minimize produced a non-finite phi");
@@ -2196,42 +2388,70 @@ struct Engine final : EngineBase {
     }
     const double F_prev = m_Fprev;
     m_Fprev = F_total_excess;
-    const double lam_raw = min_multiplier(hscal[50], hscal[51]);
-    const double eta = std::fmin(std::fmax(mopt.eta_lambda_vol, 0.0), 1.0);
-    m_lambda = (1.0 - eta) * m_lambda + eta * lam_raw;
-    // phi^{n+1}, and g_exp there for the KKT residual (x_B unchanged: Y is)
-    min_update(phi_old, m_lambda);
-    int gz;
-    if (el) {
-      typename OpMinFinishFull<T, true>::Args oa{P, phi_old, phi_new, m_gexp, m_eps, m_s00, trs, xB, (size_t)R,
-                                                 invN};
-      gz = launch_zpass<T, 1, 1, OpMinFinishFull<T, true>>(Nz, stream, zargs(0, 1, 0, 1), oa);
-    } else {
-      typename OpMinFinishFull<T, false>::Args oa{P, phi_old, phi_new, m_gexp, m_eps, m_s00, trs, xB, (size_t)R,
-                                                  invN};
-      gz = launch_zpass<T, 1, 1, OpMinFinishFull<T, false>>(Nz, stream, zargs(0, 1, 0, 1), oa);
+    // with freeze_phi, phi stays as it is (main_cuda.cu:29357, 32108, 32192, 32276)
+    const bool frozen = mopt.freeze_phi != 0;
+    const double* phi_end = frozen ? phi_old : phi_new;
+    double lam_raw = NAN, lam_kkt = NAN, rms_dphi = NAN, rms_res = NAN;
+    const bool comp = m_ncomp > 0;
+    if (!frozen) {
+      const double eta = std::fmin(std::fmax(mopt.eta_lambda_vol, 0.0), 1.0);
+      if (comp) {
+        // one multiplier per label (main_cuda.cu:29446-29510)
+        std::vector<double> raw;
+        if ((rc = comp_moments(phi_old, m_gfull))) return rc;
+        comp_multipliers(&raw);
+        for (int c = 0; c < m_ncomp; ++c) m_clambda[c] = (1.0 - eta) * m_clambda[c] + eta * raw[c];
+        if ((rc = comp_upload(comp_lambda_dev(), m_clambda))) return rc;
+        min_update(phi_old, 0.0, m_labels, comp_lambda_dev());
+      } else {
+        lam_raw = min_multiplier(hscal[50], hscal[51]);
+        m_lambda = (1.0 - eta) * m_lambda + eta * lam_raw;
+        // phi^{n+1}, and g_exp there for the KKT residual (x_B unchanged: Y is)
+        min_update(phi_old, m_lambda);
+      }
+      int gz;
+      if (el) {
+        typename OpMinFinishFull<T, true>::Args oa{P, phi_old, phi_new, m_gexp, m_eps, m_s00, trs, xB, (size_t)R,
+                                                   invN};
+        gz = launch_zpass<T, 1, 1, OpMinFinishFull<T, true>>(Nz, stream, zargs(0, 1, 0, 1), oa);
+      } else {
+        typename OpMinFinishFull<T, false>::Args oa{P, phi_old, phi_new, m_gexp, m_eps, m_s00, trs, xB, (size_t)R,
+                                                    invN};
+        gz = launch_zpass<T, 1, 1, OpMinFinishFull<T, false>>(Nz, stream, zargs(0, 1, 0, 1), oa);
+      }
+      finalize(gz, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 52, 1.0);
+      min_forward_to_sc0();
+      min_lap_to_sz1();
+      min_gfull(phi_new, 53);
+      if ((rc = fetch(58))) return rc;
+      rms_dphi = std::sqrt(hscal[52] / (double)R) / mopt.dt;
+      if (comp) {
+        // the exact multiplier of every label at phi^{n+1} (main_cuda.cu:29660-29728)
+        std::vector<double> kkt;
+        if ((rc = comp_moments(phi_new, m_gfull))) return rc;
+        comp_multipliers(&kkt);
+        if ((rc = comp_upload(comp_aux_dev(), kkt))) return rc;
+        hipLaunchKernelGGL(min_kkt_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, m_gfull, 0.0,
+                           (const int*)m_labels, (const double*)comp_aux_dev(), R, partials);
+      } else {
+        lam_kkt = min_multiplier(hscal[53], hscal[54]);
+        hipLaunchKernelGGL(min_kkt_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, m_gfull, lam_kkt,
+                           (const int*)nullptr, (const double*)nullptr, R, partials);
+      }
+      finalize(gR, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 55, 1.0);
     }
-    finalize(gz, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 52, 1.0);
-    min_forward_to_sc0();
-    min_lap_to_sz1();
-    min_gfull(phi_new, 53);
-    if ((rc = fetch(58))) return rc;
-    const double rms_dphi = std::sqrt(hscal[52] / (double)R) / mopt.dt;
-    const double lam_kkt = min_multiplier(hscal[53], hscal[54]);
-    hipLaunchKernelGGL(min_kkt_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, m_gfull, lam_kkt, R, partials);
-    finalize(gR, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 55, 1.0);
     // the composition update at the pseudo-time step (P.dt is the minimize dt)
     {
-      const double* phiY = prm.enable_Y_rhs_previous_time_level ? phi_old : phi_new;
+      const double* phiY = prm.enable_Y_rhs_previous_time_level ? phi_old : phi_end;
       typename OpMuX<T>::Args oa{P, Y, phiY, xB, trs};
       const int g = launch_zpass<T, 0, 1, OpMuX<T>>(Nz, stream, zargs(0, 0, 1, 1), oa);
       finalize(g, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 4, 1.0 / (double)R);
       PhysParams PY = P;
       PY.dt = mopt.dt;
-      if ((rc = y_chain(PY, phi_old, phi_new, false, m_ycap, 13))) return rc;
+      if ((rc = y_chain(PY, phi_old, phi_end, false, m_ycap, 13))) return rc;
     }
     // post-projection sub-steps with the updated x_B, elastic state frozen
-    for (int k = 0; k < mopt.post_projection_iters; ++k) {
+    for (int k = 0; !frozen && k < mopt.post_projection_iters; ++k) {
       if (el) {
         typename OpMinEvalFull<T, true>::Args oa{P, phi_new, m_gexp, m_eps, m_s00, trs, xB, (size_t)R};
         launch_zpass<T, 0, 1, OpMinEvalFull<T, true>>(Nz, stream, zargs(0, 0, 0, 1), oa);
@@ -2248,16 +2468,19 @@ struct Engine final : EngineBase {
       typename OpMinFinish<T, false, false>::Args oa{P, phi_new, phi_new, m_gexp, m_eps, m_s00, (size_t)R, invN};
       launch_zpass<T, 1, 0, OpMinFinish<T, false, false>>(Nz, stream, zargs(0, 1, 0, 0), oa);
     }
-    // first-order volume projection, then rms_dY from dY/dt = (Y^{n+1} - Y^n)/dt
-    hipLaunchKernelGGL(min_volume_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, R, partials);
+    // first-order volume projection
+    hipLaunchKernelGGL(min_volume_kernel, dim3(gR), dim3(NT), 0, stream, phi_end, R, partials);
     finalize(gR, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal + 48, 1.0);
-    hipLaunchKernelGGL(sum_sq_kernel, dim3(gR), dim3(NT), 0, stream, (const double*)dYdt, R, partials);
-    finalize(gR, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal + 58, 1.0);
-    if ((rc = fetch(60))) return rc;
-    const double rms_res = std::sqrt(hscal[55] / (double)R);
-    const double rms_dY = std::sqrt(hscal[59] / (double)R);
+    if ((rc = fetch(58))) return rc;
+    if (!frozen) rms_res = std::sqrt(hscal[55] / (double)R);
     double mean_h = hscal[48] / (double)R;
-    if (m_V0 > 1e-30 && std::isfinite(mean_h)) {
+    if (comp) {
+      if ((rc = comp_project(phi_new))) return rc;
+      hipLaunchKernelGGL(min_volume_kernel, dim3(gR), dim3(NT), 0, stream, (const double*)phi_new, R, partials);
+      finalize(gR, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal + 48, 1.0);
+      if ((rc = fetch(58))) return rc;
+      mean_h = hscal[48] / (double)R;
+    } else if (!frozen && m_V0 > 1e-30 && std::isfinite(mean_h)) {
       const double vol_err = m_V0 - mean_h;
       if (std::fabs(vol_err) > 1e-14) {
         const double mean_hp2 = hscal[49] / (double)R;
@@ -2268,18 +2491,43 @@ struct Engine final : EngineBase {
         mean_h = hscal[48] / (double)R;
       }
     }
-    cur = 1 - cur;
+    if (!frozen) cur = 1 - cur;
+    // the exact mass after every phi correction (main_cuda.cu:32291-32296),
+    // then rms_dY = rms(Y^{n+1} - Y^n)/dt with the shift included
+    double mass_cur = NAN, mass_err_rel = NAN, lam_dt = 0.0;
+    if (mopt.mass_constraint) {
+      if ((rc = min_mass_constraint(phi[cur]))) return rc;
+      double vf;
+      if ((rc = mass_vf(&mass_cur, &vf))) return rc;
+      mass_err_rel = std::fabs(mass_cur - m_mass_target) / std::fmax(std::fabs(m_mass_target), 1.0);
+      lam_dt = m_mass.lam / mopt.dt;
+    }
+    hipLaunchKernelGGL(sum_sq_kernel, dim3(gR), dim3(NT), 0, stream, (const double*)dYdt, lam_dt, R, partials);
+    finalize(gR, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal + 58, 1.0);
+    if ((rc = fetch(60))) return rc;
+    const double rms_dY = std::sqrt(hscal[59] / (double)R);
     const double energy_diff_rel =
         std::isfinite(F_prev) ? std::fabs(F_total_excess - F_prev) / std::fmax(std::fabs(F_prev), 1.0e-20) : NAN;
-    const double vol_err_rel = m_V0 > 1e-30 ? std::fabs(mean_h - m_V0) / (m_V0 + 1e-30) : 0.0;
+    const double vol_err_rel =
+        comp ? m_cerr : (m_V0 > 1e-30 ? std::fabs(mean_h - m_V0) / (m_V0 + 1e-30) : 0.0);
     auto fin = [](double v) { return std::isfinite(v); };
     const bool vol_ok = fin(vol_err_rel) && vol_err_rel < mopt.vol_err_rel_threshold;
     const bool res_ok = fin(rms_res) && rms_res < mopt.rms_res_threshold;
-    const bool condA = fin(rms_dphi) && fin(rms_dY) && rms_dphi < mopt.rms_dphi_threshold &&
-                       rms_dY < mopt.rms_dY_threshold && res_ok && vol_ok;
-    const bool condB = fin(energy_diff_rel) && energy_diff_rel < mopt.energy_diff_rel_threshold && fin(rms_dphi) &&
-                       fin(rms_dY) && rms_dphi < 5.0 * mopt.rms_dphi_threshold &&
-                       rms_dY < 5.0 * mopt.rms_dY_threshold && res_ok && vol_ok;
+    const bool mass_ok =
+        !mopt.mass_constraint || (fin(mass_err_rel) && mass_err_rel < mopt.mass_tolerance_relative);
+    bool condA, condB;
+    if (frozen) {
+      // main_cuda.cu:32508-32522: no phi residual, no rms_dphi
+      condA = fin(rms_dY) && rms_dY < mopt.rms_dY_threshold && vol_ok && mass_ok;
+      condB = fin(energy_diff_rel) && energy_diff_rel < mopt.energy_diff_rel_threshold && fin(rms_dY) &&
+              rms_dY < 5.0 * mopt.rms_dY_threshold && vol_ok && mass_ok;
+    } else {
+      condA = fin(rms_dphi) && fin(rms_dY) && rms_dphi < mopt.rms_dphi_threshold &&
+              rms_dY < mopt.rms_dY_threshold && res_ok && vol_ok && mass_ok;
+      condB = fin(energy_diff_rel) && energy_diff_rel < mopt.energy_diff_rel_threshold && fin(rms_dphi) &&
+              fin(rms_dY) && rms_dphi < 5.0 * mopt.rms_dphi_threshold && rms_dY < 5.0 * mopt.rms_dY_threshold &&
+              res_ok && vol_ok && mass_ok;
+    }
     const bool t_ok = (double)m_iter * mopt.dt + 1e-14 >= mopt.min_pseudo_time;
     m_conv = ((condA || condB) && t_ok) ? m_conv + 1 : 0;
     if (m_conv >= mopt.convergence_steps) m_converged = true;
@@ -2312,6 +2560,19 @@ struct Engine final : EngineBase {
       st->F_total_CNT_hat = F_total_cnt;
       st->rms_dY = rms_dY;
       st->energy_diff_rel = energy_diff_rel;
+      st->mass_target = m_mass_target;
+      st->mass_current = mass_cur;
+      st->mass_err_rel = mass_err_rel;
+      st->mass_lambda = mopt.mass_constraint ? m_mass.lam : NAN;
+      st->mass_iterations = mopt.mass_constraint ? m_mass.iters : 0;
+      st->mass_residual = mopt.mass_constraint ? m_mass.r : NAN;
+      st->mass_derivative = mopt.mass_constraint ? m_mass.deriv : NAN;
+      st->mass_Ystar_min = mopt.mass_constraint ? m_mass.ymin : NAN;
+      st->mass_Ystar_max = mopt.mass_constraint ? m_mass.ymax : NAN;
+      st->mass_lambda_lower = mopt.mass_constraint ? m_mass.lo : NAN;
+      st->mass_lambda_upper = mopt.mass_constraint ? m_mass.hi : NAN;
+      st->mass_newton_steps = mopt.mass_constraint ? m_mass.newton : 0;
+      st->mass_bisection_steps = mopt.mass_constraint ? m_mass.bisect : 0;
     }
     if (!std::isfinite(mean_h) || !std::isfinite(m_lambda) || !std::isfinite(rms_dY))
       return fail(PF_E_NONFINITE, "minimize produced a non-finite phi or Y");

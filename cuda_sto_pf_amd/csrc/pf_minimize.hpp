@@ -123,17 +123,20 @@ struct OpMinUpd {
     const double* gexp;
     double lambda;
     double Ldt;
+    const int* labels;       // per-label multipliers lambdas[labels[i]] when set
+    const double* lambdas;   // (add_component_volume_constraint_kernel, cuda_kernels.cu:606-615)
   };
   static constexpr int NRED = 0;
   static constexpr int kinds[1] = {0};
-  static constexpr int NPRE = 2;
+  static constexpr int NPRE = 3;
   __device__ static void load(const Args& a, size_t g, double* pre) {
     pre[0] = a.phi[g];
     pre[1] = a.gexp[g];
+    pre[2] = a.labels ? a.lambdas[a.labels[g]] : a.lambda;
   }
   __device__ static void apply(const Args& a, size_t, const double* pre, const T*, T* out, double*) {
     const double phi = pre[0];
-    out[0] = T(phi - a.Ldt * (pre[1] + a.lambda * h_prime_of_phi(phi)));
+    out[0] = T(phi - a.Ldt * (pre[1] + pre[2] * h_prime_of_phi(phi)));
   }
 };
 
@@ -384,11 +387,12 @@ namespace {
 // (compute_projected_phi_kkt_residual_sq_kernel, cuda_kernels.cu:684-706);
 //   red[0] sum res^2
 __global__ void __launch_bounds__(NT) min_kkt_kernel(const double* phi, const double* gfull, double lambda,
-                                                     long n, double* partials) {
+                                                     const int* labels, const double* lambdas, long n,
+                                                     double* partials) {
   double s = 0.0;
   for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
     const double p = phi[i];
-    double r = gfull[i] + lambda * h_prime_of_phi(p);
+    double r = gfull[i] + (labels ? lambdas[labels[i]] : lambda) * h_prime_of_phi(p);
     if ((p <= 1e-12 && r > 0.0) || (p >= 1.0 - 1e-12 && r < 0.0)) r = 0.0;
     s += r * r;
   }
@@ -410,15 +414,107 @@ __global__ void __launch_bounds__(NT) min_volume_kernel(const double* phi, long 
   block_reduce_store<2>(v, kinds, partials + 2 * (size_t)blockIdx.x);
 }
 
-// red[0] sum a, red[1] sum a^2
-__global__ void __launch_bounds__(NT) sum_sq_kernel(const double* a, long n, double* partials) {
+// red[0] sum (a + shift), red[1] sum (a + shift)^2
+__global__ void __launch_bounds__(NT) sum_sq_kernel(const double* a, double shift, long n, double* partials) {
   double v[2] = {0.0, 0.0};
   for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
-    const double x = a[i];
+    const double x = a[i] + shift;
     v[0] += x;
     v[1] += x * x;
   }
   const int kinds[2] = {RED_SUM, RED_SUM};
+  block_reduce_store<2>(v, kinds, partials + 2 * (size_t)blockIdx.x);
+}
+
+// ---------------------------------------------------------------- V5 labels
+// Per-label sums over the ownership labels 0..ncomp-1 (at most COMP_MAX):
+// sum h'(phi) g, sum h'(phi)^2, sum h(clamp01 phi)
+// (compute_component_constraint_moments_kernel, cuda_kernels.cu:590-604).
+// A block owns COMP_VPT * NT consecutive cells, so it meets the few labels
+// of a compact region; it reduces the labels in its [min, max] range and
+// writes zeros for the rest.  partials: 3 ncomp per block.
+constexpr int COMP_MAX = 64;
+constexpr int COMP_VPT = 8;
+__global__ void __launch_bounds__(NT) comp_moments_kernel(const double* phi, const double* g, const int* labels,
+                                                          int ncomp, long n, double* partials) {
+  __shared__ int range[2];
+  const long base = (long)blockIdx.x * NT * COMP_VPT;
+  double v[COMP_VPT][3];
+  int lab[COMP_VPT];
+  int lo = ncomp, hi = -1;
+#pragma unroll
+  for (int u = 0; u < COMP_VPT; ++u) {
+    const long i = base + (long)u * NT + threadIdx.x;
+    lab[u] = -1;
+    v[u][0] = v[u][1] = v[u][2] = 0.0;
+    if (i < n) {
+      const double p = phi[i];
+      const double hp = h_prime_of_phi(p);
+      lab[u] = labels[i];
+      v[u][0] = g ? hp * g[i] : 0.0;
+      v[u][1] = hp * hp;
+      v[u][2] = h_of_phi(clamp01(p));
+      lo = min(lo, lab[u]);
+      hi = max(hi, lab[u]);
+    }
+  }
+  if (threadIdx.x == 0) {
+    range[0] = ncomp;
+    range[1] = -1;
+  }
+  __syncthreads();
+  atomicMin(&range[0], lo);
+  atomicMax(&range[1], hi);
+  __syncthreads();
+  const int c0 = range[0], c1 = range[1];
+  double* out = partials + (size_t)blockIdx.x * 3 * ncomp;
+  for (int c = threadIdx.x; c < ncomp; c += NT)
+    if (c < c0 || c > c1) out[3 * c] = out[3 * c + 1] = out[3 * c + 2] = 0.0;
+  const int kinds[3] = {RED_SUM, RED_SUM, RED_SUM};
+  for (int c = c0; c <= c1; ++c) {
+    double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int u = 0; u < COMP_VPT; ++u)
+      if (lab[u] == c) {
+        acc[0] += v[u][0];
+        acc[1] += v[u][1];
+        acc[2] += v[u][2];
+      }
+    block_reduce_store<3>(acc, kinds, out + 3 * c);
+    __syncthreads();   // the reduction's LDS is used again by the next label
+  }
+}
+
+// out[j] = sum over blocks of partials[b * nval + j], in block order
+__global__ void __launch_bounds__(NT) comp_sum_kernel(const double* partials, int nblocks, int nval,
+                                                      double* out) {
+  for (int j = threadIdx.x; j < nval; j += NT) {
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += partials[(size_t)b * nval + j];
+    out[j] = s;
+  }
+}
+
+// phi <- clamp01(phi + corr[label] h'(phi)) (apply_component_volume_projection_kernel,
+// cuda_kernels.cu:617-627)
+__global__ void __launch_bounds__(NT) comp_project_kernel(double* phi, const int* labels, const double* corr,
+                                                          long n) {
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
+    const double p = phi[i];
+    phi[i] = clamp01(p + corr[labels[i]] * h_prime_of_phi(p));
+  }
+}
+
+// b = a, red[0] min a, red[1] max a
+__global__ void __launch_bounds__(NT) copy_minmax_kernel(const double* a, double* b, long n, double* partials) {
+  double v[2] = {1.0e300, -1.0e300};
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
+    const double x = a[i];
+    b[i] = x;
+    v[0] = fmin(v[0], x);
+    v[1] = fmax(v[1], x);
+  }
+  const int kinds[2] = {RED_MIN, RED_MAX};
   block_reduce_store<2>(v, kinds, partials + 2 * (size_t)blockIdx.x);
 }
 

@@ -252,6 +252,14 @@ int pf_minimize_opts_default(pf_minimize_opts* o) {
   o->rms_dY_threshold = 5.0e-5;
   o->energy_diff_rel_threshold = 1.0e-9;
   o->xB_max_safe = 0.07;
+  o->mass_constraint = 0;
+  o->mass_tolerance_relative = 1.0e-12;
+  o->mass_max_iterations = 64;
+  o->target_mass_code = 0.0;
+  o->freeze_phi = 0;
+  o->component_count = 0;
+  o->component_labels = nullptr;
+  o->component_target_h_sums = nullptr;
   return PF_OK;
 }
 
@@ -260,6 +268,37 @@ int pf_minimize_begin(pf_handle* h, const pf_minimize_opts* o) {
   if (!(o->dt > 0.0) || !std::isfinite(o->dt)) return pf::fail(PF_E_ARG, "minimize dt must be positive");
   if (o->max_iter < 0 || o->post_projection_iters < 0 || o->convergence_steps < 1)
     return pf::fail(PF_E_ARG, "minimize: bad iteration counts");
+  // main_cuda.cu:24562-24636
+  if (o->mass_constraint) {
+    if (!o->full_model)
+      return pf::fail(PF_E_ARG, "the minimize mass constraint requires the full-model minimize");
+    if (!(o->mass_tolerance_relative > 0.0) || !std::isfinite(o->mass_tolerance_relative) ||
+        o->mass_max_iterations <= 0 || !std::isfinite(o->target_mass_code))
+      return pf::fail(PF_E_ARG, "minimize mass-constraint tolerance, iteration, or target contract is invalid");
+  }
+  if (o->freeze_phi) {
+    if (!o->full_model || !o->mass_constraint)
+      return pf::fail(PF_E_ARG, "freeze_phi requires the full-model minimize with the mass constraint");
+    if (o->post_projection_iters != 0)
+      return pf::fail(PF_E_ARG, "freeze_phi requires post_projection_iters 0 (phi stays byte-identical)");
+    if (o->V0 > 0.0) return pf::fail(PF_E_ARG, "freeze_phi requires V0 <= 0 (the input h-volume is kept)");
+  }
+  if (o->component_count != 0) {
+    if (!o->full_model || !o->mass_constraint || o->freeze_phi)
+      return pf::fail(PF_E_ARG,
+                      "component volume constraints require full-model constrained minimize with phi evolution "
+                      "enabled");
+    if (o->component_count < 2 || o->component_count > 64 || !o->component_labels || !o->component_target_h_sums)
+      return pf::fail(PF_E_ARG,
+                      "component constraints require a component count in [2,64], labels and one positive "
+                      "target h sum per component");
+    if (o->post_projection_iters != 0 || o->V0 > 0.0)
+      return pf::fail(PF_E_ARG,
+                      "component constraints own the phi volume projection: post_projection_iters 0 and V0 <= 0");
+    for (int c = 0; c < o->component_count; ++c)
+      if (!(o->component_target_h_sums[c] > 0.0) || !std::isfinite(o->component_target_h_sums[c]))
+        return pf::fail(PF_E_ARG, "invalid component target h sum");
+  }
   return h->eng->minimize_begin(*o);
 }
 

@@ -87,6 +87,10 @@ class PfMinimizeOpts(C.Structure):
         ("convergence_steps", C.c_int), ("min_pseudo_time", C.c_double),
         ("full_model", C.c_int), ("rms_dY_threshold", C.c_double), ("energy_diff_rel_threshold", C.c_double),
         ("xB_max_safe", C.c_double),
+        ("mass_constraint", C.c_int), ("mass_tolerance_relative", C.c_double), ("mass_max_iterations", C.c_int),
+        ("target_mass_code", C.c_double), ("freeze_phi", C.c_int),
+        ("component_count", C.c_int), ("component_labels", C.POINTER(C.c_int)),
+        ("component_target_h_sums", C.POINTER(C.c_double)),
     ]
 
 
@@ -102,6 +106,11 @@ class PfMinimizeStats(C.Structure):
         ("F_chem_excess_hat", C.c_double), ("F_total_excess_hat", C.c_double),
         ("F_chem_CNT_hat", C.c_double), ("F_total_CNT_hat", C.c_double),
         ("rms_dY", C.c_double), ("energy_diff_rel", C.c_double),
+        ("mass_target", C.c_double), ("mass_current", C.c_double), ("mass_err_rel", C.c_double),
+        ("mass_lambda", C.c_double), ("mass_iterations", C.c_int), ("mass_newton_steps", C.c_int),
+        ("mass_bisection_steps", C.c_int), ("mass_residual", C.c_double), ("mass_derivative", C.c_double),
+        ("mass_Ystar_min", C.c_double), ("mass_Ystar_max", C.c_double), ("mass_lambda_lower", C.c_double),
+        ("mass_lambda_upper", C.c_double),
     ]
 
     def as_dict(self):

@@ -36,6 +36,33 @@ def write_bundle(out_dir, phi, xB, dx_nm, interface_width_nm, dtype="float64", *
     return out
 
 
+def ownership_labels(shape, centres):
+    """Periodic nearest-centre ownership (the V5 ownership map of
+    scripts/materialize_pf_elastic_multi_particle_component_constraint_v5.py:
+    every cell goes to the closest centre on the periodic grid, ties to the
+    lower index); centres in grid units.  int32, C order."""
+    coords = np.indices(shape, dtype=np.float64)
+    best = np.full(shape, np.inf)
+    owner = np.zeros(shape, dtype=np.int32)
+    for k, c in enumerate(centres):
+        d2 = np.zeros(shape)
+        for ax, n in enumerate(shape):
+            d = (coords[ax] - c[ax] + 0.5 * n) % n - 0.5 * n
+            d2 += d * d
+        closer = d2 < best
+        owner[closer] = k
+        best[closer] = d2[closer]
+    return owner
+
+
+def component_h_sums(phi, labels, count):
+    """The per-label sums of h(phi) = phi^3 (6 phi^2 - 15 phi + 10) over
+    clamp01(phi) that --minimize-component-target-h-sums takes."""
+    p = np.clip(phi, 0.0, 1.0)
+    h = p * p * p * (6.0 * p * p - 15.0 * p + 10.0)
+    return np.bincount(labels.ravel(), weights=h.ravel(), minlength=count)
+
+
 def uniform(out_dir, nx, ny, nz, dx_nm, interface_width_nm, xb_out):
     """Matrix-only bundle (write_uniform_matrix_raw_init.py:28-57)."""
     shape = (nx, ny, nz)

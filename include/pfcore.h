@@ -219,6 +219,32 @@ typedef struct pf_minimize_opts {
   double rms_dY_threshold;          /* --minimize-rms-dY-threshold (5e-5) */
   double energy_diff_rel_threshold; /* --minimize-energy-diff-rel-threshold (1e-9) */
   double xB_max_safe;               /* --minimize-xB-max-safe (0.07): Y <= logit(xB_max_safe) */
+  /* --minimize-mass-constraint (main_cuda.cu:32297-32366, full model only):
+   * after the phi projections, the zero-mode root find shifts the clamped Y
+   * of the iteration so that sum (1-h) x_B + h v_B equals the target, with Y
+   * bounded to [-Y_clip, logit(xB_max_safe)]; a rejected search fails the
+   * iteration with PF_E_ZERO_MODE.  The target is target_mass_code when
+   * positive (it must match the initial fields), else the initial sum. */
+  int mass_constraint;
+  double mass_tolerance_relative;   /* --minimize-mass-tolerance-relative (1e-12) */
+  int mass_max_iterations;          /* --minimize-mass-max-iterations (64) */
+  double target_mass_code;          /* --minimize-target-mass-code (0: the initial ledger) */
+  /* --minimize-freeze-phi (V4, main_cuda.cu:29357, 32508-32522): phi is left
+   * untouched; only the composition update and the mass constraint run.
+   * Needs mass_constraint, post_projection_iters 0 and V0 <= 0. */
+  int freeze_phi;
+  /* --minimize-component-volume-constraints (V5, main_cuda.cu:24638-24670,
+   * 29446-29510, 29660-29728, 32207-32275): one h-volume constraint per
+   * ownership label instead of the global one.  component_labels holds
+   * Nx*Ny*Nz labels in 0..component_count-1 (the C-order int32 raw of
+   * --minimize-component-label-raw, every label owning a cell) and
+   * component_target_h_sums the target sum of h(phi) per label; both are
+   * host arrays read by pf_minimize_begin.  Needs the mass constraint, phi
+   * evolving, post_projection_iters 0 and V0 <= 0; component_count in
+   * [2, 64], 0 = off.  vol_err_rel reports the largest relative error. */
+  int component_count;
+  const int* component_labels;
+  const double* component_target_h_sums;
 } pf_minimize_opts;
 
 /* One row of energy_minimize_*.csv (main_cuda.cu:32455-32489) for phi-only
@@ -239,6 +265,14 @@ typedef struct pf_minimize_stats {
    * rms(Y^{n+1} - Y^n)/dt and the relative change of the total excess */
   double F_chem_excess_hat, F_total_excess_hat, F_chem_CNT_hat, F_total_CNT_hat;
   double rms_dY, energy_diff_rel;
+  /* mass constraint (NaN and 0 without it): target, sum after the shift,
+   * |sum - target| / max(|target|, 1), the shift and its search steps; the
+   * search's residual and d sum/d lambda at the shift, the extremes of the
+   * clamped Y it shifts and the final bracket (the columns of
+   * minimize_mass_constraint_trace.csv, main_cuda.cu:25681-25685) */
+  double mass_target, mass_current, mass_err_rel, mass_lambda;
+  int mass_iterations, mass_newton_steps, mass_bisection_steps;
+  double mass_residual, mass_derivative, mass_Ystar_min, mass_Ystar_max, mass_lambda_lower, mass_lambda_upper;
 } pf_minimize_stats;
 
 int pf_minimize_opts_default(pf_minimize_opts* o);

@@ -64,6 +64,17 @@ class MinimizeOptions:
     energy_diff_rel_threshold: float = 1.0e-9
     xB_max_safe: float = 0.07
     xB_ref: float = 0.03         # ic_23d_xB_out (default 0.03, main_cuda.cu:16215); <= 0: <x_B> of the initial fields
+    # constraints of the full model (main_cuda.cu:16321-16328)
+    mass_constraint: int = 0     # --minimize-mass-constraint
+    freeze_phi: int = 0          # --minimize-freeze-phi (V4, needs the mass constraint)
+    mass_tolerance_relative: float = 1.0e-12
+    mass_max_iterations: int = 64
+    target_mass: float = 0.0     # --minimize-target-mass-code; <= 0: the initial mass
+    # V5 (--minimize-component-volume-constraints): one h-volume constraint per
+    # ownership label (int array of the grid's shape, values 0..K-1) with the
+    # target sums of h(clamp01 phi) per label (main_cuda.cu:24638-24670)
+    component_labels: object = None
+    component_targets: object = None
 
 
 class MinimizeOracle:
@@ -261,6 +272,31 @@ class MinimizeFullOracle(MinimizeOracle):
         self.converged = False
         self.F_prev = float("nan")
         self.stats = {}
+        if opts.freeze_phi and (not opts.mass_constraint or opts.post_projection_iters != 0 or opts.V0 > 0.0):
+            raise ValueError("--minimize-freeze-phi needs the mass constraint, no post-projection and V0 <= 0 "
+                             "(main_cuda.cu:24597-24617)")
+        self.labels = None
+        if opts.component_labels is not None:
+            lab = np.asarray(opts.component_labels)
+            tg = np.asarray(opts.component_targets, dtype=np.float64)
+            K = tg.size
+            if (not opts.mass_constraint or opts.freeze_phi or opts.post_projection_iters != 0 or opts.V0 > 0.0
+                    or K < 2 or K > 64 or lab.shape != self.shape or lab.min() < 0 or lab.max() >= K
+                    or np.bincount(lab.ravel(), minlength=K).min() == 0 or not np.all(tg > 0.0)
+                    or not np.all(np.isfinite(tg))):
+                raise ValueError("V5 component constraints: bad labels, targets or options "
+                                 "(main_cuda.cu:24638-24670, 27262-27290)")
+            self.labels = lab.astype(np.int64)
+            self.targets = tg
+            self.comp_lambda = np.zeros(K)
+            self.comp_err = float("nan")
+        self.mass_target = None
+        if opts.mass_constraint:
+            # main_cuda.cu:28117-28146: the initial ledger, or a matching explicit target
+            m0 = o.mass()
+            self.mass_target = opts.target_mass if opts.target_mass > 0.0 else m0
+            if abs(m0 - self.mass_target) > opts.mass_tolerance_relative * max(abs(self.mass_target), 1.0):
+                raise ValueError("explicit minimize target mass does not match the initial ledger")
 
     def g_explicit(self, phi):
         """compute_phi_rhs_kernel at phi with the current x_B and the stress and
@@ -307,21 +343,33 @@ class MinimizeFullOracle(MinimizeOracle):
         en = self.energies()
         F_prev = self.F_prev
         self.F_prev = en["F_total_excess"]
-        # phi with the under-relaxed multiplier
+        if opts.freeze_phi:
+            return self._iterate_fixed_phi(phi_n, Y_n, en, F_prev)
+        # phi with the under-relaxed multiplier (one per label under V5,
+        # main_cuda.cu:29446-29510)
         self.refresh_xB()
         gexp = self.g_explicit(phi_n)
         phik = fwd(phi_n, o.ks)
         gfull = gexp - p.kappa_phi * self.laplacian(phik)
-        lam_raw = self.multiplier(phi_n, gfull)
         eta = min(max(opts.eta_lambda_vol, 0.0), 1.0)
-        self.lambda_vol = (1.0 - eta) * self.lambda_vol + eta * lam_raw
-        phi = self.semi_implicit(phik, gexp + self.lambda_vol * th.h_prime_of_phi(phi_n))
-        # KKT residual at phi^{n+1}
+        if self.labels is not None:
+            lam_raw = float("nan")
+            self.comp_lambda = (1.0 - eta) * self.comp_lambda + eta * self.comp_multipliers(phi_n, gfull)
+            phi = self.semi_implicit(phik, gexp + self.comp_lambda[self.labels] * th.h_prime_of_phi(phi_n))
+        else:
+            lam_raw = self.multiplier(phi_n, gfull)
+            self.lambda_vol = (1.0 - eta) * self.lambda_vol + eta * lam_raw
+            phi = self.semi_implicit(phik, gexp + self.lambda_vol * th.h_prime_of_phi(phi_n))
+        # KKT residual at phi^{n+1} (the exact multipliers there, main_cuda.cu:29660-29728)
         self.refresh_xB()
         gexp1 = self.g_explicit(phi)
         gfull1 = gexp1 - p.kappa_phi * self.laplacian(fwd(phi, o.ks))
-        lam_kkt = self.multiplier(phi, gfull1)
-        res = gfull1 + lam_kkt * th.h_prime_of_phi(phi)
+        if self.labels is not None:
+            lam_kkt = float("nan")
+            res = gfull1 + self.comp_multipliers(phi, gfull1)[self.labels] * th.h_prime_of_phi(phi)
+        else:
+            lam_kkt = self.multiplier(phi, gfull1)
+            res = gfull1 + lam_kkt * th.h_prime_of_phi(phi)
         res = np.where(((phi <= 1e-12) & (res > 0.0)) | ((phi >= 1.0 - 1e-12) & (res < 0.0)), 0.0, res)
         rms_res = math.sqrt(math.fsum((res * res).ravel()) / n)
         rms_dphi = math.sqrt(math.fsum(((phi - phi_n) ** 2).ravel()) / n) / opts.dt
@@ -337,7 +385,10 @@ class MinimizeFullOracle(MinimizeOracle):
             self.lambda_vol = self.multiplier(phi, gf_k)
             phi = self.semi_implicit(phik_k, g_k + self.lambda_vol * th.h_prime_of_phi(phi))
         mean_h = self.mean_h(phi)
-        if self.V0_target > 1e-30 and math.isfinite(mean_h):
+        if self.labels is not None:
+            phi = self.comp_project(phi)
+            mean_h = self.mean_h(phi)
+        elif self.V0_target > 1e-30 and math.isfinite(mean_h):
             vol_err = self.V0_target - mean_h
             if abs(vol_err) > 1e-14:
                 hp = th.h_prime_of_phi(phi)
@@ -346,19 +397,24 @@ class MinimizeFullOracle(MinimizeOracle):
                 phi = th.clamp01(phi + lc * hp)
                 mean_h = self.mean_h(phi)
         o.phi = phi
+        mass_err_rel = self._mass_constraint()
         rms_dY = math.sqrt(math.fsum(((o.Y - Y_n) ** 2).ravel()) / n) / p.dt
         energy_diff_rel = (abs(en["F_total_excess"] - F_prev) / max(abs(F_prev), 1.0e-20)
                            if math.isfinite(F_prev) else float("nan"))
-        vol_err_rel = (abs(mean_h - self.V0_target) / (self.V0_target + 1e-30)
-                       if self.V0_target > 1e-30 else 0.0)
+        if self.labels is not None:
+            vol_err_rel = self.comp_err
+        else:
+            vol_err_rel = (abs(mean_h - self.V0_target) / (self.V0_target + 1e-30)
+                           if self.V0_target > 1e-30 else 0.0)
         fin = math.isfinite
         vol_ok = fin(vol_err_rel) and vol_err_rel < opts.vol_err_rel_threshold
         res_ok = fin(rms_res) and rms_res < opts.rms_res_threshold
+        mass_ok = mass_err_rel is None or (fin(mass_err_rel) and mass_err_rel < opts.mass_tolerance_relative)
         condA = (fin(rms_dphi) and fin(rms_dY) and rms_dphi < opts.rms_dphi_threshold
-                 and rms_dY < opts.rms_dY_threshold and res_ok and vol_ok)
+                 and rms_dY < opts.rms_dY_threshold and res_ok and vol_ok and mass_ok)
         condB = (fin(energy_diff_rel) and energy_diff_rel < opts.energy_diff_rel_threshold
                  and fin(rms_dphi) and fin(rms_dY) and rms_dphi < 5.0 * opts.rms_dphi_threshold
-                 and rms_dY < 5.0 * opts.rms_dY_threshold and res_ok and vol_ok)
+                 and rms_dY < 5.0 * opts.rms_dY_threshold and res_ok and vol_ok and mass_ok)
         t_ok = self.iteration * opts.dt + 1e-14 >= opts.min_pseudo_time
         self.conv_count = self.conv_count + 1 if ((condA or condB) and t_ok) else 0
         if self.conv_count >= opts.convergence_steps:
@@ -368,5 +424,93 @@ class MinimizeFullOracle(MinimizeOracle):
                           el_core_sum=en["el_core_sum"], F_chem_excess=en["F_chem_excess"],
                           F_total_excess=en["F_total_excess"], rms_res=rms_res, rms_dphi=rms_dphi,
                           rms_dY=rms_dY, energy_diff_rel=energy_diff_rel, vol_err_rel=vol_err_rel,
-                          lambda_raw=lam_raw, lambda_kkt=lam_kkt, elastic_iterations=o.last_elastic_iterations)
+                          lambda_raw=lam_raw, lambda_kkt=lam_kkt, elastic_iterations=o.last_elastic_iterations,
+                          **self._mass_stats(mass_err_rel))
+        return self.stats
+
+    def comp_sums(self, phi, w):
+        """Per-label sums of w (compute_component_constraint_moments_kernel,
+        cuda_kernels.cu:590-604)."""
+        flat = self.labels.ravel()
+        w = w.ravel()
+        return np.array([math.fsum(w[flat == c]) for c in range(self.targets.size)])
+
+    def comp_multipliers(self, phi, g):
+        """-sum h' g / sum h'^2 over each label (main_cuda.cu:29468-29478)."""
+        hp = th.h_prime_of_phi(phi)
+        s_hg = self.comp_sums(phi, hp * g)
+        s_h2 = self.comp_sums(phi, hp * hp)
+        return np.where(np.abs(s_h2) > 1e-30, -s_hg / np.where(s_h2 == 0.0, 1.0, s_h2), 0.0)
+
+    def comp_project(self, phi):
+        """Up to three first-order corrections phi <- clamp01(phi + c_k h'(phi))
+        with c_k = (T_k - sum_k h) / sum_k h'^2 per label, skipped once every
+        |T_k - sum_k h| <= 1e-14; then the largest relative error
+        (main_cuda.cu:32207-32275, apply_component_volume_projection_kernel
+        cuda_kernels.cu:617-627)."""
+        for _ in range(3):
+            hp = th.h_prime_of_phi(phi)
+            s_h2 = self.comp_sums(phi, hp * hp)
+            s_h = self.comp_sums(phi, th.h_of_phi(th.clamp01(phi)))
+            delta = self.targets - s_h
+            corr = np.where(s_h2 > 1e-30, delta / np.where(s_h2 == 0.0, 1.0, s_h2), 0.0)
+            if not np.any(np.abs(delta) > 1e-14):
+                break
+            phi = th.clamp01(phi + corr[self.labels] * hp)
+        s_h = self.comp_sums(phi, th.h_of_phi(th.clamp01(phi)))
+        self.comp_err = float(np.max(np.abs(s_h - self.targets) / np.maximum(self.targets, 1e-30)))
+        return phi
+
+    def _mass_constraint(self):
+        """main_cuda.cu:32297-32366: the zero-mode root find on the clamped Y of
+        this iteration, at the current phi, with the minimize Y cap, tolerance
+        and iteration cap; the relative mass error after it (32443-32452)."""
+        if self.mass_target is None:
+            return None
+        o, opts = self.o, self.opts
+        o.zero_mode(o.Y.copy(), self.y_cap, target=self.mass_target, tol_rel=opts.mass_tolerance_relative,
+                    max_it=opts.mass_max_iterations)
+        self.mass_lambda = o.last_lambda
+        return abs(o.mass() - self.mass_target) / max(abs(self.mass_target), 1.0)
+
+    def _mass_stats(self, mass_err_rel):
+        if mass_err_rel is None:
+            return {}
+        return dict(mass_target=self.mass_target, mass_current=self.o.mass(), mass_err_rel=mass_err_rel,
+                    mass_lambda=self.mass_lambda)
+
+    def _iterate_fixed_phi(self, phi_n, Y_n, en, F_prev):
+        """V4 (--minimize-freeze-phi): phi is held fixed; only the composition
+        update and the mass constraint run, and the stopping rule looks at Y,
+        the energy, the volume and the mass (main_cuda.cu:32108, 32192-32195,
+        32276, 32514-32526)."""
+        p, opts, o = self.p, self.opts, self.o
+        n = o.phi.size
+        o.Y_update(phi_n, Y_n, y_cap=self.y_cap)
+        o.dYdt = (o.Y - Y_n) / p.dt
+        mean_h = self.mean_h(o.phi)
+        mass_err_rel = self._mass_constraint()
+        rms_dY = math.sqrt(math.fsum(((o.Y - Y_n) ** 2).ravel()) / n) / p.dt
+        energy_diff_rel = (abs(en["F_total_excess"] - F_prev) / max(abs(F_prev), 1.0e-20)
+                           if math.isfinite(F_prev) else float("nan"))
+        vol_err_rel = (abs(mean_h - self.V0_target) / (self.V0_target + 1e-30)
+                       if self.V0_target > 1e-30 else 0.0)
+        fin = math.isfinite
+        vol_ok = fin(vol_err_rel) and vol_err_rel < opts.vol_err_rel_threshold
+        mass_ok = fin(mass_err_rel) and mass_err_rel < opts.mass_tolerance_relative
+        condA = fin(rms_dY) and rms_dY < opts.rms_dY_threshold and vol_ok and mass_ok
+        condB = (fin(energy_diff_rel) and energy_diff_rel < opts.energy_diff_rel_threshold
+                 and fin(rms_dY) and rms_dY < 5.0 * opts.rms_dY_threshold and vol_ok and mass_ok)
+        t_ok = self.iteration * opts.dt + 1e-14 >= opts.min_pseudo_time
+        self.conv_count = self.conv_count + 1 if ((condA or condB) and t_ok) else 0
+        if self.conv_count >= opts.convergence_steps:
+            self.converged = True
+        nan = float("nan")
+        self.stats = dict(mean_h=mean_h, V0=self.V0_target, lambda_vol=self.lambda_vol,
+                          F_surf=en["F_surf"], F_el=en["F_el"], interface_sum=en["interface_sum"],
+                          el_core_sum=en["el_core_sum"], F_chem_excess=en["F_chem_excess"],
+                          F_total_excess=en["F_total_excess"], rms_res=nan, rms_dphi=nan,
+                          rms_dY=rms_dY, energy_diff_rel=energy_diff_rel, vol_err_rel=vol_err_rel,
+                          lambda_raw=nan, lambda_kkt=nan, elastic_iterations=o.last_elastic_iterations,
+                          **self._mass_stats(mass_err_rel))
         return self.stats

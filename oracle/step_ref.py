@@ -645,8 +645,10 @@ class Oracle:
             self.xB = th.sigmoid_from_logit(Ynew, p.Y_clip, p.xB_eps)
         self.mean_DY = mean_DY
 
-    def zero_mode(self, Ystar, Y_cap=None):
-        """``run_pf_conserved_y_zero_mode_host`` (main_cuda.cu:4269-4405)."""
+    def zero_mode(self, Ystar, Y_cap=None, target=None, tol_rel=None, max_it=None):
+        """``run_pf_conserved_y_zero_mode_host`` (main_cuda.cu:4269-4405).  The
+        minimize mass constraint calls it with its own target, tolerance and
+        iteration cap (main_cuda.cu:32297-32366)."""
         p = self.p
         h = th.h_of_phi(self.phi)
         alpha = 1.0 - h
@@ -657,8 +659,10 @@ class Oracle:
         roundoff = 128.0 * np.finfo(float).eps * max(abs(lo), abs(hi), 1.0)
         if lo > hi + roundoff:
             raise ZeroModeRejected("empty zero-mode bracket")
-        target = self.target_mass
-        tol = p.zero_mode_tolerance_relative * max(abs(target), 1.0)
+        target = self.target_mass if target is None else target
+        tol_rel = p.zero_mode_tolerance_relative if tol_rel is None else tol_rel
+        max_it = p.zero_mode_max_iterations if max_it is None else max_it
+        tol = tol_rel * max(abs(target), 1.0)
         hv = h * p.v_B
 
         def evaluate(lam):
@@ -674,7 +678,7 @@ class Oracle:
         m, dm = evaluate(lam)
         r = m - target
         it = 0
-        while it < p.zero_mode_max_iterations and abs(r) > tol:
+        while it < max_it and abs(r) > tol:
             if r < 0.0:
                 lo = lam
             else:

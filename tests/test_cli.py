@@ -82,11 +82,51 @@ def test_cli_raw_fields_run_matches_library(pf, tmp_path):
     assert np.abs(vals.transpose(2, 1, 0) - ref).max() <= 6e-6
 
 
-def test_minimize_constraints_are_refused(tmp_path):
-    args, *_ = _bundle(tmp_path)
-    for flag in ("--minimize-mass-constraint", "--minimize-component-volume-constraints"):
-        r = _run(args + ["--mode=minimize", "--minimize-full-model", flag], tmp_path)
-        assert r.returncode == 2 and "[fatal]" in r.stderr and flag in r.stderr
+def test_minimize_constraint_flags_fail_closed(tmp_path):
+    """main_cuda.cu:24562-24623: the mass constraint needs the full-model
+    minimize, freeze-phi needs the mass constraint; the per-component volume
+    constraints are not built and refused."""
+    args, *_ = _bundle(tmp_path, zero_mode=False)
+    for extra, flag in ((["--minimize-mass-constraint"], "--minimize-mass-constraint"),
+                        (["--mode=minimize", "--minimize-mass-constraint"], "--minimize-mass-constraint"),
+                        (["--mode=minimize", "--minimize-full-model", "--minimize-freeze-phi"],
+                         "--minimize-freeze-phi"),
+                        (["--mode=minimize", "--minimize-full-model", "--minimize-component-volume-constraints"],
+                         "--minimize-component-volume-constraints")):
+        r = _run(args + extra, tmp_path)
+        assert r.returncode == 2 and "[fatal]" in r.stderr and flag in r.stderr, (extra, r.stderr)
+
+
+@pytest.mark.gpu
+def test_cli_mass_constrained_minimize_matches_library(pf, tmp_path):
+    """--minimize-full-model --minimize-mass-constraint: the energy log's mass
+    columns and minimize_mass_constraint_trace.csv (main_cuda.cu:25667-25690,
+    32368-32398) equal the library's own run."""
+    args, pd, phi, xB, meta = _bundle(tmp_path, zero_mode=False)
+    args = args + ["--mode=minimize", "--minimize-full-model", "--minimize-mass-constraint", "--minimize-max-iter",
+                   "4", "--minimize-dt", "0.05", "--init-case-tag", "masscase"]
+    r = _run(args, tmp_path)
+    print(r.stdout[-2000:], r.stderr[-2000:], flush=True)
+    assert r.returncode == 0, r.stderr
+    runs = os.listdir(tmp_path / "Results")
+    case = tmp_path / "Results" / runs[0] / "masscase"
+    logs = [f for f in os.listdir(case) if f.startswith("energy_minimize_")]
+    rows = (case / logs[0]).read_text().splitlines()
+    head = rows[0].split(",")
+    trace = (case / "minimize_mass_constraint_trace.csv").read_text().splitlines()
+    assert trace[0].startswith("iter,target_mass_code,current_mass_code,lambda,") and len(trace) == 5
+    sim = pf.Simulation(dict(pd, dt=0.05, zero_mode=0), pf.PF_FP64)
+    sim.set_fields(phi, xB, xB_cap=meta["xB_max_safe"])
+    # the fields load under the meta cap, Y under the minimize cap (0.07)
+    sim.minimize_begin(pf.minimize_opts(dt=0.05, max_iter=4, full_model=1, mass_constraint=1))
+    for i in range(4):
+        st = sim.minimize_iterate()
+        cols = dict(zip(head, rows[1 + i].split(",")))
+        tr = trace[1 + i].split(",")
+        assert float(cols["mass_target_code"]) == st["mass_target"] == float(tr[1])
+        assert abs(float(cols["mass_lambda"]) - st["mass_lambda"]) <= 1e-14 + 1e-12 * abs(st["mass_lambda"])
+        assert float(cols["mass_err_rel"]) < 1e-12
+        assert int(tr[7]) == st["mass_iterations"] and int(tr[8]) + int(tr[9]) == int(tr[7])
 
 
 @pytest.mark.gpu
@@ -264,3 +304,41 @@ def test_nonpositive_intervals_are_fatal(tmp_path, pos, value, match):
     args[pos + 1] = value
     r = _run(args, tmp_path)
     assert r.returncode == 2 and match in r.stderr, r.stderr
+
+
+def test_component_constraint_flags_fail_closed(tmp_path):
+    """V5 command line (main_cuda.cu:2805-2873, 24638-24670): count in [2,64],
+    exactly one positive target per component, an int32 label raw of the grid's
+    size with labels in range, the mass constraint on and no post-projection."""
+    args, pd, phi, *_ = _bundle(tmp_path, zero_mode=False)
+    lab = np.zeros(phi.shape, dtype=np.int32)
+    lab[8:] = 1
+    lab.tofile(tmp_path / "labels.raw")
+    lab[:2].tofile(tmp_path / "short.raw")
+    bad = lab.copy()
+    bad[0, 0, 0] = 2
+    bad.tofile(tmp_path / "range.raw")
+    base = args + ["--mode=minimize", "--minimize-full-model", "--minimize-post-projection-iters", "0",
+                   "--minimize-component-volume-constraints"]
+    good = ["--minimize-component-count", "2", "--minimize-component-label-raw", str(tmp_path / "labels.raw"),
+            "--minimize-component-target-h-sums", "10.0,12.5"]
+    cases = [
+        ([], "requires full-model constrained minimize"),
+        (["--minimize-mass-constraint", "--minimize-component-count", "1"] + good[2:], "component count in [2,64]"),
+        (["--minimize-mass-constraint"] + good[:4] + ["--minimize-component-target-h-sums", "10.0"],
+         "expected comma"),
+        (["--minimize-mass-constraint"] + good[:4] + ["--minimize-component-target-h-sums", "10.0,12.5,3"],
+         "extra entries"),
+        (["--minimize-mass-constraint"] + good[:4] + ["--minimize-component-target-h-sums", "10.0,-1"],
+         "invalid component target"),
+        (["--minimize-mass-constraint", "--minimize-component-count", "2", "--minimize-component-label-raw",
+          str(tmp_path / "short.raw")] + good[4:], "size mismatch"),
+        (["--minimize-mass-constraint", "--minimize-component-count", "2", "--minimize-component-label-raw",
+          str(tmp_path / "range.raw")] + good[4:], "out of range"),
+    ]
+    for extra, match in cases:
+        r = _run(base + extra, tmp_path)
+        assert r.returncode == 2 and "[fatal]" in r.stderr and match in r.stderr, (extra, r.stderr)
+    r = _run(args + ["--mode=minimize", "--minimize-full-model", "--minimize-mass-constraint",
+                     "--minimize-component-volume-constraints"] + good, tmp_path)
+    assert r.returncode == 2 and "post-projection-iters 0" in r.stderr
