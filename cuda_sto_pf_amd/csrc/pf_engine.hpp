@@ -1018,10 +1018,9 @@ struct Engine final : EngineBase {
     {
       // kz stride: whole 128-byte lines on large grids (aligned column tiles
       // in the y and x passes), the retained count on small ones
-      const int line = 128 / (int)sizeof(C);
-      AzS = az.A >= 40 ? (az.A + line - 1) / line * line : az.A;
-      if (const char* e = std::getenv("PF_KZ_PAD"))
-        if (e[0] == '0') AzS = az.A;
+      int line = 128 / (int)sizeof(C);
+      if (const char* e = std::getenv("PF_KZ_PAD")) line = std::atoi(e);   // elements; 0: no padding
+      AzS = (az.A >= 40 && line > 0) ? (az.A + line - 1) / line * line : az.A;
     }
     nSz = (long)Nx * Ny * AzS;
     nSy = (long)Nx * ay.A * AzS;
