@@ -146,7 +146,9 @@ void usage() {
       "    dt from --pf-param-file takes precedence over the positional dt.\n"
       "Flags:\n"
       "  --pf-param-file <path>   required: complete physical PF inputs (key=value)\n"
-      "  --mode=dynamics|minimize (the continue modes are not built)\n"
+      "  --mode=dynamics|dynamics-continue|minimize|minimize-continue\n"
+      "  --continue-phi-vtk <f> [--continue-xB-vtk <f>]   continue from VTK outputs (no zero mode)\n"
+      "  --ic-23d-xB-out <x>      far-field x_B (run directory name, continuation rebuild, minimize baseline)\n"
       "  --minimize-full-model    chemical drive, composition and volume constraint (no zero mode)\n"
       "  --minimize-rms-dY-threshold <v> --minimize-energy-diff-rel-threshold <v> --minimize-xB-max-safe <x>\n"
       "  --minimize-mass-constraint [--minimize-target-mass-code <m>] [--minimize-mass-tolerance-relative <v>]\n"
@@ -168,6 +170,180 @@ void usage() {
       "  --pf-fixture-manifest-sha256 <hex>   checkpoint-pinned fixture identity\n"
       "  --pf-profile-library-manifest-sha256 <hex> checkpoint-pinned profile-library identity\n"
       "  --precision fp64|mixed   (default fp64)\n");
+}
+
+// ---------------------------------------------------------------- VTK continuation
+// The path up to its last '/', or "" (derive_continue_case_output_dir,
+// main_cuda.cu:277-292); twice gives the run directory (derive_continue_output_root, 257-275).
+std::string parent_dir(const std::string& p) {
+  const size_t k = p.rfind('/');
+  return k == std::string::npos ? std::string() : p.substr(0, k);
+}
+std::string base_name(const std::string& p) {
+  const size_t k = p.rfind('/');
+  return k == std::string::npos ? p : p.substr(k + 1);
+}
+bool file_nonempty(const std::string& p) {
+  struct stat st;
+  return stat(p.c_str(), &st) == 0 && st.st_size > 0;
+}
+
+// continue_dyn_<n+1> / continue_min_<n+1> with n the largest index already in
+// the path (derive_next_continue_case_tag, main_cuda.cu:2492-2521)
+std::string next_continue_tag(const std::string& vtk, bool minimize) {
+  const std::string prefix = minimize ? "continue_min_" : "continue_dyn_";
+  long best = 0;
+  for (size_t k = vtk.find(prefix); k != std::string::npos; k = vtk.find(prefix, k + 1)) {
+    size_t d = k + prefix.size();
+    if (d < vtk.size() && std::isdigit((unsigned char)vtk[d])) {
+      long v = 0;
+      while (d < vtk.size() && std::isdigit((unsigned char)vtk[d])) v = v * 10 + (vtk[d++] - '0');
+      best = std::max(best, v);
+    }
+  }
+  return prefix + std::to_string(best + 1);
+}
+
+// An ASCII STRUCTURED_POINTS scalar file as main_pf writes it: the grid from
+// DIMENSIONS, values after LOOKUP_TABLE in z-y-x order; a file of one y layer
+// fills both layers of an Ny = 2 run (read_vtk_ascii_to_host, io_vtk_cuda.h:131-216).
+bool read_vtk_scalars(const std::string& path, int Nx, int Ny, int Nz, std::vector<double>* out, const char* what) {
+  FILE* fp = std::fopen(path.c_str(), "r");
+  if (!fp) {
+    std::fprintf(stderr, "[fatal] cannot open VTK file %s for reading\n", path.c_str());
+    return false;
+  }
+  char line[4096];
+  int vx = -1, vy = -1, vz = -1;
+  bool dims = false, table = false;
+  while (std::fgets(line, sizeof line, fp)) {
+    if (!dims && std::strncmp(line, "DIMENSIONS", 10) == 0) {
+      dims = std::sscanf(line, "DIMENSIONS %d %d %d", &vx, &vy, &vz) == 3;
+      continue;
+    }
+    if (std::strncmp(line, "LOOKUP_TABLE", 12) == 0) {
+      table = true;
+      break;
+    }
+  }
+  if (!dims || !table) {
+    std::fprintf(stderr, "[fatal] invalid VTK file %s: missing DIMENSIONS or LOOKUP_TABLE header\n", path.c_str());
+    std::fclose(fp);
+    return false;
+  }
+  const bool flat = vy == 1 && Ny == 2;
+  if (!(vx == Nx && vz == Nz && (vy == Ny || flat))) {
+    std::fprintf(stderr, "[fatal] VTK grid mismatch for %s (%s): file=%dx%dx%d expected=%dx%dx%d\n", path.c_str(),
+                 what, vx, vy, vz, Nx, Ny, Nz);
+    std::fclose(fp);
+    return false;
+  }
+  out->assign((size_t)Nx * Ny * Nz, 0.0);
+  const int ny_file = flat ? 1 : Ny;
+  for (int k = 0; k < Nz; ++k)
+    for (int j = 0; j < ny_file; ++j)
+      for (int i = 0; i < Nx; ++i) {
+        double v = 0.0;
+        if (std::fscanf(fp, "%lf", &v) != 1) {
+          std::fprintf(stderr, "[fatal] unexpected end of VTK data in %s\n", path.c_str());
+          std::fclose(fp);
+          return false;
+        }
+        for (int jj = flat ? 0 : j; jj < (flat ? Ny : j + 1); ++jj) (*out)[((size_t)i * Ny + jj) * Nz + k] = v;
+      }
+  std::fclose(fp);
+  return true;
+}
+
+// matrix solubility: Newton on RT ln x + L (1-x)^2 = 0 with L = 41212.9 - 18.05 T
+// (solve_x_eq_device, thermo_utils.h:453-481)
+double matrix_x_eq(double T) {
+  const double L = 41212.9 - 18.05 * T;
+  const double RT = 8.31446261815324 * T;
+  double x = std::exp(-L / RT);
+  if (x < 1e-9) x = 1e-9;
+  if (x > 0.99) x = 0.5;
+  for (int i = 0; i < 20; ++i) {
+    const double f = RT * std::log(x) + L * (1.0 - x) * (1.0 - x);
+    const double df = RT / x - 2.0 * L * (1.0 - x);
+    const double delta = f / df;
+    x -= delta;
+    if (x < 1e-10) x = 1e-10;
+    if (x > 0.999) x = 0.999;
+    if (std::fabs(delta) < 1e-8) break;
+  }
+  return x;
+}
+
+double h_quintic(double p) { return p * p * p * (6.0 * p * p - 15.0 * p + 10.0); }
+double clamp_frac(double x, double e) { return x < e ? e : (x > 1.0 - e ? 1.0 - e : x); }
+
+// x_B of a continued dynamics or full-model minimize run without an x_B VTK:
+// x_eq inside the particles and the far field ic_23d_xB_out outside, or,
+// with ic_23d_xB_out <= 0, the far field that gives the seed-radius (or
+// ic_vf_target_phi) composition budget (rebuild_full_model_composition_from_phi,
+// main_cuda.cu:2523-2600; compute_effective_vf_target, 2458-2490)
+void rebuild_composition(const pf_params& P, double seed_radius, std::vector<double>& phi, std::vector<double>* xB) {
+  const size_t n = phi.size();
+  const double xe = matrix_x_eq(P.temperature_C + 273.15);
+  double sh = 0.0, sh2 = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    const double h = h_quintic(phi[i]);
+    sh += h;
+    sh2 += h * h;
+  }
+  const double mh = sh / (double)n, mh2 = sh2 / (double)n;
+  double xout;
+  if (P.ic_23d_xB_out > 0.0) {
+    xout = clamp_frac(P.ic_23d_xB_out, P.xB_eps);
+  } else {
+    double vf = std::fmin(0.999, std::fmax(1e-6, P.ic_vf_target_phi));
+    if (seed_radius > 0.0) {
+      const bool three_d = P.Ny > 2;
+      const double R = seed_radius;
+      const double f = three_d ? (4.0 * M_PI / 3.0) * R * R * R / (P.Nx * P.dx * P.Ny * P.dy * P.Nz * P.dz)
+                               : M_PI * R * R / (P.Nx * P.dx * P.Nz * P.dz);
+      vf = std::fmin(0.999, std::fmax(1e-6, f));
+    }
+    const double A = 1.0 - 2.0 * mh + mh2, B = mh - mh2, C = P.v_B * mh;
+    const double target = P.v_B * vf + (1.0 - vf) * xe;
+    if (std::fabs(A) < 1e-12) {
+      const double den = std::fmax(1.0 - mh, 1e-12);
+      xout = ((vf - mh) * P.v_B + (1.0 - vf) * xe) / den;
+    } else {
+      xout = (target - B * xe - C) / A;
+    }
+    xout = clamp_frac(xout, P.xB_eps);
+  }
+  xB->resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    const double h = h_quintic(phi[i]);
+    (*xB)[i] = clamp_frac((1.0 - h) * xout + h * xe, P.xB_eps);
+  }
+  std::printf("[continue] x_B rebuilt from phi: x_eq=%.6e x_out=%.6e <h>=%.6f\n", xe, xout, mh);
+}
+
+// the last data row's leading integer and two reals of a CSV, if any
+// (read_last_vf_csv_state / read_last_energy_iter, main_cuda.cu:311-352)
+bool last_csv_row(const std::string& path, int* step, double* t0, double* t1) {
+  FILE* fp = std::fopen(path.c_str(), "r");
+  if (!fp) return false;
+  char line[4096];
+  bool found = false;
+  while (std::fgets(line, sizeof line, fp)) {
+    if (!std::isdigit((unsigned char)line[0]) && line[0] != '-') continue;
+    int st = 0;
+    double a = 0.0, b = 0.0;
+    const int got = std::sscanf(line, "%d,%lf,%lf", &st, &a, &b);
+    if (got >= 1 && (t0 == nullptr || got == 3)) {
+      found = true;
+      *step = st;
+      if (t0) *t0 = a;
+      if (t1) *t1 = b;
+    }
+  }
+  std::fclose(fp);
+  return found;
 }
 
 // "v0,v1,..." with exactly `count` positive finite entries
@@ -236,12 +412,17 @@ bool read_labels_i32(const std::string& path, size_t n, int count, std::vector<i
 // 32455-32489), VTK outputs (build_case_vtk_path, main_cuda.cu:126-170) and
 // convergence report (main_cuda.cu:32546-32600).
 int run_minimize(pf_handle* h, const pf_params& P, const pf_minimize_opts& mo, const std::string& case_dir,
-                 const std::string& tag, double seed_radius, double xB_out) {
+                 const std::string& tag, const std::string& csv_tag, bool continuing, double seed_radius) {
   const size_t N = (size_t)P.Nx * P.Ny * P.Nz;
   char name[512];
   std::snprintf(name, sizeof name, "/energy_minimize_T%.0f_r%.2f_xB%.3f_%s.csv", P.temperature_C, seed_radius,
-                xB_out, tag.c_str());
-  FILE* efp = std::fopen((case_dir + name).c_str(), "w");
+                P.ic_23d_xB_out, csv_tag.c_str());
+  // a continuation appends to an existing log of its case, numbering on
+  // (main_cuda.cu:27436-27447)
+  int iter_offset = 0;
+  const bool append = continuing && file_nonempty(case_dir + name) &&
+                      last_csv_row(case_dir + name, &iter_offset, nullptr, nullptr);
+  FILE* efp = std::fopen((case_dir + name).c_str(), append ? "a" : "w");
   // main_cuda.cu:25667-25690
   FILE* tfp = nullptr;
   if (mo.mass_constraint) {
@@ -260,7 +441,7 @@ int run_minimize(pf_handle* h, const pf_params& P, const pf_minimize_opts& mo, c
     if (tfp) std::fclose(tfp);
     efp = tfp = nullptr;
   };
-  if (efp)
+  if (efp && !append)
     std::fprintf(efp,
                  "iter,dt,mean_h,V0,lambda,F_surf_hat,F_el_hat,F_chem_excess_hat,F_total_excess_hat,"
                  "F_chem_CNT_hat,F_total_CNT_hat,total_interface_sum,total_el_core_sum,"
@@ -287,7 +468,8 @@ int run_minimize(pf_handle* h, const pf_params& P, const pf_minimize_opts& mo, c
       std::fprintf(efp,
                    "%d,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,%.8e,"
                    "%.17e,%.17e,%.17e,%.17e,%d\n",
-                   it, mo.dt, st.mean_h, st.V0, st.lambda_vol, st.F_surf_hat, st.F_el_hat, st.F_chem_excess_hat,
+                   it + iter_offset, mo.dt, st.mean_h, st.V0, st.lambda_vol, st.F_surf_hat, st.F_el_hat,
+                   st.F_chem_excess_hat,
                    st.F_total_excess_hat, st.F_chem_CNT_hat, st.F_total_CNT_hat, st.interface_sum, st.el_core_sum,
                    st.rms_res, st.rms_dphi, st.rms_dY, st.energy_diff_rel, st.vol_err_rel, st.mass_target,
                    st.mass_current, st.mass_err_rel, st.mass_lambda, mo.post_projection_iters);
@@ -348,6 +530,10 @@ int main(int argc, char** argv) {
   int comp_count = 0;
   std::string comp_label_raw, comp_targets_text;
   double seed_radius = 10.0;   // ic_phi_seed_radius default (main_cuda.cu:16221)
+  // VTK continuation (main_cuda.cu:22761-22771, 23770-23782)
+  bool continuing = false;
+  std::string continue_phi_vtk, continue_xB_vtk;
+  double xB_out_flag = NAN;   // --ic-23d-xB-out
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
     auto next = [&](std::string* dst) {
@@ -363,6 +549,19 @@ int main(int argc, char** argv) {
       param_file = argv[++i];
     } else if (starts_with(a.c_str(), "--mode=")) {
       mode = a.substr(7);
+      if (mode == "minimize-continue" || mode == "dynamics-continue") {
+        continuing = true;
+        mode = mode == "minimize-continue" ? "minimize" : "dynamics";
+      }
+    } else if (a == "--continue-phi-vtk") {
+      if (!next(&continue_phi_vtk)) return fatal(a + " needs a path");
+      continuing = true;
+    } else if (a == "--continue-xB-vtk") {
+      if (!next(&continue_xB_vtk)) return fatal(a + " needs a path");
+    } else if (a == "--ic-23d-xB-out") {
+      std::string v;
+      if (!next(&v)) return fatal(a + " needs a value");
+      xB_out_flag = std::atof(v.c_str());
     } else if (a == "--init-mode") {
       if (!next(&init_mode)) return fatal("--init-mode needs a value");
     } else if (a == "--init-phi-raw") {
@@ -469,10 +668,31 @@ int main(int argc, char** argv) {
       std::fprintf(stderr, "[warn] flag not handled by main_pf: %s\n", a.c_str());
     }
   }
+  // a continuation without --pf-param-file takes the parameters stored next to
+  // the VTK it continues (main_cuda.cu:22543-22565)
+  std::string stored_params;
+  if (continuing && !continue_phi_vtk.empty() && !parent_dir(continue_phi_vtk).empty()) {
+    const std::string cand = parent_dir(continue_phi_vtk) + "/pf_input.params";
+    struct stat st;
+    if (stat(cand.c_str(), &st) == 0) {
+      if (!param_file) {
+        stored_params = cand;
+        param_file = stored_params.c_str();
+        std::printf("[continue-param] loaded stored PF params from %s\n", cand.c_str());
+      } else {
+        std::printf("[continue-param] explicit --pf-param-file takes precedence; ignoring stored PF params at %s\n",
+                    cand.c_str());
+      }
+    }
+  }
   if (!param_file)
     return fatal("physical parameters have no built-in defaults: pass --pf-param-file <path>");
   if (mode != "dynamics" && mode != "minimize")
-    return fatal("--mode=" + mode + " is not implemented (dynamics and phi-only minimize)");
+    return fatal("--mode=" + mode + " is not implemented (dynamics, minimize and their -continue forms)");
+  // main_cuda.cu:24752-24770
+  if (continuing && continue_phi_vtk.empty()) return fatal("continuation needs --continue-phi-vtk <path>");
+  if (continuing && !init_mode.empty())
+    return fatal("--init-mode raw_fields cannot be combined with --continue-phi-vtk/--continue-xB-vtk");
   const bool minimize = mode == "minimize";
   // main_cuda.cu:24562-24623
   if (mo.mass_constraint && !(minimize && mo.full_model))
@@ -511,6 +731,7 @@ int main(int argc, char** argv) {
   if (pos(7)) P.csv_out_every = std::atoi(pos(7));
   if (pos(8)) P.elastic_enabled = std::atoi(pos(8));
   if (pf_params_load(param_file, &P) != PF_OK) return fatal(lib_error());
+  if (std::isfinite(xB_out_flag)) P.ic_23d_xB_out = xB_out_flag;
   if (P.pf_params_schema_version != 1)
     return fatal("pf_params_schema_version mismatch: file=" + std::to_string(P.pf_params_schema_version) +
                  " expected=1");
@@ -537,6 +758,10 @@ int main(int argc, char** argv) {
     return fatal("--pf-checkpoint-path is required when periodic checkpointing is enabled");
   if (minimize && (ckpt_every > 0 || !restart_from.empty()))
     return fatal("checkpoint/restart applies to dynamics runs");
+  // main_cuda.cu:24706-24710
+  if (continuing && P.zero_mode)
+    return fatal("zero-mode restart cannot be mixed with VTK continuation; use --pf-restart-from for exact state "
+                 "restoration");
   // initial-state identity (main_cuda.cu:24515-24555): either the complete
   // legacy triple, or the conditional-handoff class with two lowercase
   // SHA-256 manifests, the latter only for zero-mode dynamics from raw_fields
@@ -597,7 +822,6 @@ int main(int argc, char** argv) {
   const size_t N = (size_t)P.Nx * P.Ny * P.Nz;
   std::vector<double> phi, xB, dydt;
   double xB_cap = NAN;
-  double xB_out = 0.0;
   if (init_mode == "raw_fields") {
     Meta m;
     if (meta_path.empty() || !read_meta(meta_path, &m)) return fatal("cannot read init meta: " + meta_path);
@@ -613,11 +837,28 @@ int main(int argc, char** argv) {
       return fatal("cannot read dY_dt_prev raw field: " + dydt_raw);
     xB_cap = std::isfinite(m.xB_max_safe) ? m.xB_max_safe : mo.xB_max_safe;   // main_cuda.cu:2940
     if (case_tag.empty()) case_tag = "raw_fields_relax";
+  } else if (continuing) {
+    // load_continue_fields_from_vtk (main_cuda.cu:2602-2655): phi from its VTK;
+    // with a composition (dynamics, full-model minimize) x_B from its VTK when
+    // it exists, else rebuilt from phi
+    if (!read_vtk_scalars(continue_phi_vtk, P.Nx, P.Ny, P.Nz, &phi, "phi")) return 2;
+    for (double& v : phi) v = v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
+    const bool needs_x = !minimize || mo.full_model;
+    if (!needs_x) {
+      xB.assign(N, 0.0);
+    } else if (!continue_xB_vtk.empty() && file_nonempty(continue_xB_vtk)) {
+      if (!read_vtk_scalars(continue_xB_vtk, P.Nx, P.Ny, P.Nz, &xB, "xB")) return 2;
+      for (double& v : xB) v = clamp_frac(v, P.xB_eps);
+    } else {
+      if (!continue_xB_vtk.empty())
+        std::fprintf(stderr, "[warn] continuation x_B VTK missing or unreadable, rebuilding x_B from phi: %s\n",
+                     continue_xB_vtk.c_str());
+      rebuild_composition(P, seed_radius, phi, &xB);
+    }
+    if (case_tag.empty()) case_tag = next_continue_tag(continue_phi_vtk, minimize);
   } else {
-    return fatal("main_pf initialises from --init-mode raw_fields only");
+    return fatal("main_pf initialises from --init-mode raw_fields or a VTK continuation only");
   }
-  for (size_t i = 0; i < N; ++i) xB_out += xB[i];
-  xB_out /= (double)N;
 
   std::vector<int> comp_labels;
   if (!comp_targets.empty()) {
@@ -648,32 +889,57 @@ int main(int argc, char** argv) {
   const char* root_env = std::getenv("CUDA_STO_RESULTS_ROOT");
   const std::string root = (root_env && root_env[0]) ? root_env : "Results";
   char run_dir[512];
-  if (minimize) {
+  std::string out_dir, csv_tag = case_tag;
+  if (continuing) {
+    // the continued run's own tree (main_cuda.cu:25363-25386, 25443-25456):
+    // Results root + the source run directory's name when the root is set,
+    // else the source run directory itself; CSVs keep the source case's tag
+    const std::string src_case = parent_dir(continue_phi_vtk);
+    const std::string src_run = parent_dir(src_case);
+    if (root_env && root_env[0] && !base_name(src_run).empty() && !src_run.empty())
+      out_dir = root + "/" + base_name(src_run);
+    else if (!src_run.empty())
+      out_dir = src_run;
+    else
+      out_dir = root;
+    if (!src_case.empty()) csv_tag = base_name(src_case);
+  }
+  if (continuing) {
+  } else if (minimize) {
     // the seed radius in nm from the physical dx (main_cuda.cu:25283-25396)
     const double dx_m = P.ic_phi_iface_w > 1e-30 ? P.lambda_sm_m / (2.0 * P.ic_phi_iface_w) : P.dx * 1e-9;
     const double unit_to_m = std::fabs(P.dx) > 1e-30 ? dx_m / P.dx : 1e-9;
     std::snprintf(run_dir, sizeof run_dir, "%s_T%.0f_cuda_%dx%dx%d_dt%.3g_steps%d_r%.3fnm_xB%.3f",
                   P.elastic_enabled ? "chel" : "ch", P.temperature_C, P.Nx, P.Ny, P.Nz, P.dt, P.nsteps,
-                  seed_radius * unit_to_m * 1e9, xB_out);
+                  seed_radius * unit_to_m * 1e9, P.ic_23d_xB_out);
   } else {
     std::snprintf(run_dir, sizeof run_dir, "%s_T%.0f_cuda_%dx%dx%d_dt%.3g_steps%d_xB%.3f",
-                  P.elastic_enabled ? "chel" : "ch", P.temperature_C, P.Nx, P.Ny, P.Nz, P.dt, P.nsteps, xB_out);
+                  P.elastic_enabled ? "chel" : "ch", P.temperature_C, P.Nx, P.Ny, P.Nz, P.dt, P.nsteps,
+                  P.ic_23d_xB_out);
   }
-  const std::string out_dir = root + "/" + run_dir;
+  if (!continuing) out_dir = root + "/" + run_dir;
   const std::string case_dir = out_dir + "/" + case_tag;
   mkdir(root.c_str(), 0755);
   mkdir(out_dir.c_str(), 0755);
   mkdir(case_dir.c_str(), 0755);
-  if (!minimize) copy_text_file(param_file, out_dir + "/pf_input.params");
+  if (!minimize && !continuing) copy_text_file(param_file, out_dir + "/pf_input.params");
   copy_text_file(param_file, case_dir + "/pf_input.params");
   if (minimize) {
-    const int rc = run_minimize(h, P, mo, case_dir, case_tag, seed_radius, xB_out);
+    const int rc = run_minimize(h, P, mo, case_dir, case_tag, csv_tag, continuing, seed_radius);
     pf_destroy(h);
     return rc;
   }
 
-  FILE* csv = std::fopen((case_dir + "/vf_precip_vs_time_" + case_tag + ".csv").c_str(), "w");
-  if (csv) std::fprintf(csv, "step,t_code,t_real_s,vf_precip,R_avg\n");
+  // a continuation appends to an existing log of its case, numbering on
+  // (main_cuda.cu:25571-25590)
+  const std::string csv_path = case_dir + "/vf_precip_vs_time_" + csv_tag + ".csv";
+  int csv_step0 = 0;
+  double csv_t0 = 0.0, csv_treal0 = 0.0;
+  const bool csv_append = continuing && file_nonempty(csv_path) && last_csv_row(csv_path, &csv_step0, &csv_t0,
+                                                                                 &csv_treal0);
+  if (!csv_append) csv_step0 = 0;
+  FILE* csv = std::fopen(csv_path.c_str(), csv_append ? "a" : "w");
+  if (csv && !csv_append) std::fprintf(csv, "step,t_code,t_real_s,vf_precip,R_avg\n");
   const bool accel = P.elastic_enabled && (P.elastic_warm_start_enabled || P.elastic_residual_control_enabled);
   // a restart appends to the traces of the run it continues (main_cuda.cu:25620-25650)
   const bool restarting = !restart_from.empty();
@@ -706,14 +972,14 @@ int main(int argc, char** argv) {
       R_avg = (P.Ny == 2 ? 2.0 : 3.0) * P.dx * ratio;
     }
     if (csv) {
-      std::fprintf(csv, "%d,%.8f,%.8e,%.8e,%.8e\n", step, step * P.dt, step * P.dt * P.t_real_unit,
-                   sc.volume_fraction, R_avg);
+      std::fprintf(csv, "%d,%.8f,%.8e,%.8e,%.8e\n", step + csv_step0, step * P.dt + csv_t0,
+                   step * P.dt * P.t_real_unit + csv_treal0, sc.volume_fraction, R_avg);
       std::fflush(csv);
     }
     std::printf("[3D] step=%d <x_Btot>=%.10f vf_precip=%.6e R_avg=%.4f\n", step, sc.mean_xBtot,
                 sc.volume_fraction, R_avg);
   };
-  if (!restarting) write_row(0, s0);
+  if (!restarting && !continuing) write_row(0, s0);   // main_cuda.cu:28309
   auto dump_vtk = [&](int step, bool final_) {
     std::vector<double> f(N);
     const std::string sfx = final_ ? "final" : std::to_string(step);

@@ -342,3 +342,105 @@ def test_component_constraint_flags_fail_closed(tmp_path):
     r = _run(args + ["--mode=minimize", "--minimize-full-model", "--minimize-mass-constraint",
                      "--minimize-component-volume-constraints"] + good, tmp_path)
     assert r.returncode == 2 and "post-projection-iters 0" in r.stderr
+
+
+def _read_vtk(path, shape):
+    txt = path.read_text().split("LOOKUP_TABLE default\n")[1].split()
+    return np.array([float(v) for v in txt]).reshape(shape[2], shape[1], shape[0]).transpose(2, 1, 0)
+
+
+def _write_vtk(path, f, name="phi"):
+    nx, ny, nz = f.shape
+    head = ("# vtk DataFile Version 3.0\nPhase Field Simulation\nASCII\nDATASET STRUCTURED_POINTS\n"
+            f"DIMENSIONS {nx} {ny} {nz}\nASPECT_RATIO 1 1 1\nORIGIN 0 0 0\nPOINT_DATA {nx * ny * nz}\n"
+            f"SCALARS {name} double 1\nLOOKUP_TABLE default\n")
+    path.write_text(head + " ".join(f"{v:.5f}" for v in f.transpose(2, 1, 0).ravel()) + "\n")
+
+
+def test_continuation_flags_fail_closed(tmp_path):
+    """main_cuda.cu:24706-24770: a continuation needs --continue-phi-vtk,
+    cannot start from raw fields, refuses the zero mode, and checks the VTK
+    grid; without --pf-param-file it takes the stored pf_input.params next
+    to the VTK (main_cuda.cu:22543-22565)."""
+    args, pd, phi, *_ = _bundle(tmp_path, zero_mode=False)
+    prm = args[args.index("--pf-param-file") + 1]
+    pos = args[:9]
+    src = tmp_path / "Results" / "run" / "case"
+    src.mkdir(parents=True)
+    _write_vtk(src / "phi_final.vtk", phi)
+    _write_vtk(tmp_path / "small.vtk", phi[:8])
+    cases = [(pos + ["--pf-param-file", prm, "--mode=dynamics-continue"], "--continue-phi-vtk"),
+             (args + ["--mode=dynamics-continue", "--continue-phi-vtk", str(src / "phi_final.vtk")], "raw_fields"),
+             (pos + ["--pf-param-file", prm, "--mode=dynamics-continue", "--continue-phi-vtk",
+                     str(src / "phi_final.vtk"), "--pf-zero-mode", "PF_CONSERVED_Y_ZERO_MODE_V1"], "zero-mode"),
+             (pos + ["--pf-param-file", prm, "--mode=dynamics-continue", "--continue-phi-vtk",
+                     str(tmp_path / "small.vtk")], "VTK grid mismatch")]
+    for a, match in cases:
+        r = _run(a, tmp_path)
+        assert r.returncode == 2 and match in r.stderr, (a[-3:], r.stderr)
+    # no parameter file at all, then the one stored beside the VTK
+    r = _run(pos + ["--mode=dynamics-continue", "--continue-phi-vtk", str(src / "phi_final.vtk")], tmp_path)
+    assert r.returncode == 2 and "no built-in defaults" in r.stderr
+    import shutil
+    shutil.copy(prm, src / "pf_input.params")
+    r = _run(pos + ["--mode=dynamics-continue", "--continue-phi-vtk", str(src / "phi_final.vtk"),
+                    "--pf-zero-mode", "PF_CONSERVED_Y_ZERO_MODE_V1"], tmp_path)
+    assert "[continue-param] loaded stored PF params from" in r.stdout and "zero-mode" in r.stderr
+
+
+@pytest.mark.gpu
+def test_cli_vtk_continuation_matches_library(pf, tmp_path):
+    """--mode=dynamics-continue from the VTKs of a finished run (main_cuda.cu:
+    2602-2655, 25363-25456): the continued case is continue_dyn_1 beside the
+    source case, its CSV carries the source case's tag, and its fields equal
+    a library run from the same %.5f fields; without an x_B VTK the
+    composition is rebuilt from phi with x_eq(T) inside and ic_23d_xB_out
+    outside; --mode=minimize-continue logs under the source tag."""
+    from oracle import thermo as th
+    args, pd, phi, xB, meta = _bundle(tmp_path, steps=4, zero_mode=False)
+    env = {"CUDA_STO_SUPPRESS_VTK_OUTPUT": "0"}
+    r = _run(args, tmp_path, env)
+    assert r.returncode == 0, r.stderr
+    run = tmp_path / "Results" / os.listdir(tmp_path / "Results")[0]
+    src = run / "raw_fields_relax"
+    pos = args[:9]
+    pos[5] = "3"
+    prm = args[args.index("--pf-param-file") + 1]
+    cont = pos + ["--pf-param-file", prm, "--mode=dynamics-continue", "--continue-phi-vtk",
+                  str(src / "phi_final.vtk"), "--continue-xB-vtk", str(src / "xB_final.vtk")]
+    r = _run(cont, tmp_path, env)
+    print(r.stdout[-1500:], r.stderr[-1500:], flush=True)
+    assert r.returncode == 0, r.stderr
+    case = run / "continue_dyn_1"
+    rows = (case / "vf_precip_vs_time_raw_fields_relax.csv").read_text().splitlines()
+    assert rows[0].startswith("step,") and [int(x.split(",")[0]) for x in rows[1:]] == [1, 2, 3]
+    shape = phi.shape
+    p0 = np.clip(_read_vtk(src / "phi_final.vtk", shape), 0.0, 1.0)
+    x0 = np.clip(_read_vtk(src / "xB_final.vtk", shape), pd["xB_eps"], 1.0 - pd["xB_eps"])
+    sim = pf.Simulation(dict(pd, zero_mode=0), pf.PF_FP64)
+    sim.set_fields(p0, x0)
+    sim.step(3)
+    assert np.abs(_read_vtk(case / "phi_final.vtk", shape) - sim.field(pf.PF_PHI)).max() <= 6e-6
+    assert np.abs(_read_vtk(case / "xB_final.vtk", shape) - sim.field(pf.PF_XB)).max() <= 6e-6
+    # a second continuation numbers on
+    r = _run(pos + ["--pf-param-file", prm, "--mode=dynamics-continue", "--continue-phi-vtk",
+                    str(case / "phi_final.vtk")], tmp_path, env)
+    assert r.returncode == 0, r.stderr
+    case2 = run / "continue_dyn_2"
+    assert (case2 / "vf_precip_vs_time_continue_dyn_1.csv").exists()
+    # that one rebuilt x_B from phi: x_eq inside, the far field outside
+    p1 = np.clip(_read_vtk(case / "phi_final.vtk", shape), 0.0, 1.0)
+    h = th.h_of_phi(p1)
+    xe = th.solve_x_eq(pd.get("temperature_C", 380.0) + 273.15)
+    xr = np.clip((1.0 - h) * 0.03 + h * xe, pd["xB_eps"], 1.0 - pd["xB_eps"])
+    sim2 = pf.Simulation(dict(pd, zero_mode=0), pf.PF_FP64)
+    sim2.set_fields(p1, xr)
+    sim2.step(3)
+    assert np.abs(_read_vtk(case2 / "xB_final.vtk", shape) - sim2.field(pf.PF_XB)).max() <= 6e-6
+    # phi-only minimize continued from the VTK
+    r = _run(pos + ["--pf-param-file", prm, "--mode=minimize-continue", "--continue-phi-vtk",
+                    str(src / "phi_final.vtk"), "--minimize-max-iter", "3", "--minimize-dt", "0.05"], tmp_path, env)
+    assert r.returncode == 0, r.stderr
+    logs = [f for f in os.listdir(run / "continue_min_1") if f.startswith("energy_minimize_")]
+    assert len(logs) == 1 and logs[0].endswith("_raw_fields_relax.csv")
+    assert len((run / "continue_min_1" / logs[0]).read_text().splitlines()) == 1 + 3
