@@ -406,6 +406,7 @@ def main():
         "config": {"workload": ("256^3, 40 particles R~U(5,12), hetero elastic tol 1e-8, zero mode, dt 0.01"
                                 if args.workload == 5 else f"{meta['workload']} {phi.shape[0]}^3"),
                    "grid": list(phi.shape), "learned_warm_start": net is not None,
+                   "kz_stride": sc_m["kz_stride"],
                    "path": "mixed precision, learned cold start + extrapolated warm start (order 3) + "
                            "Anderson depth 3"},
         "elastic_updates_per_step": it_m,

@@ -1016,11 +1016,20 @@ struct Engine final : EngineBase {
       return fail(PF_E_ARG, "retained kz count exceeds the 2/3 rule bound");
     R = (long)Nx * Ny * Nz;
     {
-      // kz stride: whole 128-byte lines on large grids (aligned column tiles
-      // in the y and x passes), the retained count on small ones
-      int line = 128 / (int)sizeof(C);
-      if (const char* e = std::getenv("PF_KZ_PAD")) line = std::atoi(e);   // elements; 0: no padding
-      AzS = (az.A >= 40 && line > 0) ? (az.A + line - 1) / line * line : az.A;
+      // kz stride candidates: whole 128- or 64-byte lines on large grids
+      // (aligned column tiles in the y and x passes, partial lines in the
+      // mode-by-mode k-space kernels) or none; the plan tuner picks one at
+      // the end of init, PF_KZ_PAD (elements, 0: none) fixes it
+      auto pad = [&](int line) { return (az.A >= 40 && line > 0) ? (az.A + line - 1) / line * line : az.A; };
+      const int line = 128 / (int)sizeof(C);
+      kz_cands.clear();
+      if (const char* e = std::getenv("PF_KZ_PAD")) {
+        kz_cands.push_back(pad(std::atoi(e)));
+      } else {
+        for (int c : {pad(line), pad(line / 2), az.A})
+          if (std::find(kz_cands.begin(), kz_cands.end(), c) == kz_cands.end()) kz_cands.push_back(c);
+      }
+      AzS = *std::max_element(kz_cands.begin(), kz_cands.end());   // allocations hold the widest
     }
     nSz = (long)Nx * Ny * AzS;
     nSy = (long)Nx * ay.A * AzS;
@@ -1088,6 +1097,73 @@ struct Engine final : EngineBase {
         (rc = dalloc(&epartials, (size_t)maxg * 3)))
       return rc;
     PF_HIP(hipHostMalloc((void**)&hscal, 256 * sizeof(double), hipHostMallocDefault));
+    if (kz_cands.size() > 1 && (rc = tune_kz_stride())) return rc;
+    return PF_OK;
+  }
+
+  // ---------------------------------------------------------------- plan tuner
+  // The kz stride changes only where spectra sit in memory, not any
+  // arithmetic or summation order, so every candidate gives bitwise the same
+  // results; the tuner times one elastic update's kernels (z pass, y and x
+  // passes both ways, the Green update and the Anderson update) on each and
+  // keeps the fastest.  Buffers are cleared afterwards.
+  std::vector<int> kz_cands;
+  std::vector<double> kz_times;   // microseconds per candidate, in kz_cands order
+  void set_kz_stride(int azs) {
+    AzS = azs;
+    nSz = (long)Nx * Ny * AzS;
+    nSy = (long)Nx * ay.A * AzS;
+    nSc = (long)ax.A * ay.A * AzS;
+    if (Hist)
+      for (int k = 0; k < 3; ++k) hbuf[k] = Hist + (size_t)k * 3 * nSc;
+  }
+  int clear_spectra() {
+    const size_t cs = sizeof(C);
+    const long wz = (long)Nx * Ny * AzS, wy = (long)Nx * ay.A * AzS, wc = (long)ax.A * ay.A * AzS;
+    PF_HIP(hipMemset(Sz, 0, 6 * wz * cs));
+    PF_HIP(hipMemset(Sy, 0, 6 * wy * cs));
+    PF_HIP(hipMemset(Sc, 0, 6 * wc * cs));
+    PF_HIP(hipMemset(u, 0, 3 * wc * cs));
+    PF_HIP(hipMemset(un, 0, 3 * wc * cs));
+    if (Ud) PF_HIP(hipMemset(Ud, 0, 3 * wc * sizeof(cx<double>)));
+    if (Phik) PF_HIP(hipMemset(Phik, 0, wc * sizeof(cx<double>)));
+    if (extended) {
+      PF_HIP(hipMemset(Gd, 0, 3 * wc * sizeof(cx<double>)));
+      PF_HIP(hipMemset(Hist, 0, 9 * wc * sizeof(cx<double>)));
+      PF_HIP(hipMemset(dinc, 0, 3 * wc * cs));
+      PF_HIP(hipMemset(dU, 0, (size_t)AA_MAX * 3 * wc * cs));
+      PF_HIP(hipMemset(dG, 0, (size_t)AA_MAX * 3 * wc * cs));
+      PF_HIP(hipMemset(hcache, 0, R * sizeof(T)));
+    }
+    return PF_OK;
+  }
+  int tune_kz_stride() {
+    const int widest = AzS;
+    PF_HIP(hipMemset(phi[0], 0, R * sizeof(double)));
+    PF_HIP(hipMemset(phi[1], 0, R * sizeof(double)));
+    PF_HIP(hipMemset(xB, 0, R * sizeof(double)));
+    const int ids[] = {PF_K_ZPASS_HIJ, PF_K_YPASS_FWD6, PF_K_XPASS_FWD6, PF_K_GREEN,
+                       PF_K_XPASS_INV6, PF_K_YPASS_INV6, PF_K_AA_UPDATE};
+    kz_times.assign(kz_cands.size(), 0.0);
+    size_t best = 0;
+    for (size_t c = 0; c < kz_cands.size(); ++c) {
+      set_kz_stride(kz_cands[c]);
+      for (int id : ids) {
+        if (!P.elastic && id != PF_K_YPASS_FWD6 && id != PF_K_XPASS_FWD6 && id != PF_K_XPASS_INV6 &&
+            id != PF_K_YPASS_INV6)
+          continue;
+        double us = 0.0, b = 0.0;
+        int rc = time_kernel(id, 3, &us, &b);
+        if (rc) return rc;
+        kz_times[c] += us;
+      }
+      if (kz_times[c] < kz_times[best]) best = c;
+    }
+    set_kz_stride(widest);
+    int rc = clear_spectra();
+    if (rc) return rc;
+    set_kz_stride(kz_cands[best]);
+    PF_HIP(hipStreamSynchronize(stream));
     return PF_OK;
   }
 
@@ -2720,6 +2796,7 @@ struct Engine final : EngineBase {
     s->steps_done = steps_done;
     s->elastic_iterations_total = elastic_total;
     s->zero_mode_target_mass = prm.zero_mode ? target_mass : NAN;
+    s->kz_stride = AzS;
     return PF_OK;
   }
 

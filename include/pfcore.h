@@ -113,6 +113,8 @@ typedef struct pf_scalars {
   double zero_mode_target_mass; /* the frozen target of the zero mode (NaN without it): the mass at
                                    pf_set_fields, or the restarted checkpoint's target_mass_code
                                    (main_cuda.cu:28080-28105) */
+  int kz_stride;              /* kz stride of the spectral arrays the plan tuner chose at pf_create
+                                 (whole 128- or 64-byte lines or none; bitwise-neutral) */
 } pf_scalars;
 
 typedef struct pf_handle pf_handle;
