@@ -706,7 +706,8 @@ struct OpStoreF3 {
 };
 
 // u_i += d_i on the compact spectrum, k = 0 excluded (the Green operator's
-// zero mode stays 0)
+// zero mode stays 0); kop_add3_state adds into solver state (retained-mode
+// index) from k-space slots
 template <typename T, typename S>
 __global__ void __launch_bounds__(NT) kop_add3(long n, const cx<T>* d0, const cx<T>* d1, const cx<T>* d2, S* u0,
                                                S* u1, S* u2) {
@@ -718,6 +719,22 @@ __global__ void __launch_bounds__(NT) kop_add3(long n, const cx<T>* d0, const cx
     u1[c].im += d1[c].im;
     u2[c].re += d2[c].re;
     u2[c].im += d2[c].im;
+  }
+}
+template <typename T>
+__global__ void __launch_bounds__(NT) kop_add3_state(KDims d, long n, const cx<T>* d0, const cx<T>* d1,
+                                                     const cx<T>* d2, cx<double>* u0, cx<double>* u1,
+                                                     cx<double>* u2) {
+  const long nact_ = (n / d.Az) * d.Azr;
+  for (long j_ = (long)blockIdx.x * NT + threadIdx.x; j_ < nact_; j_ += (long)gridDim.x * NT) {
+    if (j_ == 0) continue;
+    const long c = kslot(d, j_);
+    u0[j_].re += d0[c].re;
+    u0[j_].im += d0[c].im;
+    u1[j_].re += d1[c].re;
+    u1[j_].im += d1[c].im;
+    u2[j_].re += d2[c].re;
+    u2[j_].im += d2[c].im;
   }
 }
 // The network's 3 channels are a force density f whose Green response is the
@@ -776,6 +793,20 @@ __global__ void __launch_bounds__(NT) kop_diff3(long n, const S* u0, const S* u1
     d0[c] = cx<T>{T(double(u0[c].re) - double(v0[c].re)), T(double(u0[c].im) - double(v0[c].im))};
     d1[c] = cx<T>{T(double(u1[c].re) - double(v1[c].re)), T(double(u1[c].im) - double(v1[c].im))};
     d2[c] = cx<T>{T(double(u2[c].re) - double(v2[c].re)), T(double(u2[c].im) - double(v2[c].im))};
+  }
+}
+
+// d = u - v with u solver state (retained-mode index), v and d at k-space slots
+template <typename T>
+__global__ void __launch_bounds__(NT) kop_diff3_state(KDims dm, long n, const cx<double>* u0, const cx<double>* u1,
+                                                      const cx<double>* u2, const cx<T>* v0, const cx<T>* v1,
+                                                      const cx<T>* v2, cx<T>* d0, cx<T>* d1, cx<T>* d2) {
+  const long nact_ = (n / dm.Az) * dm.Azr;
+  for (long j_ = (long)blockIdx.x * NT + threadIdx.x; j_ < nact_; j_ += (long)gridDim.x * NT) {
+    const long c = kslot(dm, j_);
+    d0[c] = cx<T>{T(u0[j_].re - double(v0[c].re)), T(u0[j_].im - double(v0[c].im))};
+    d1[c] = cx<T>{T(u1[j_].re - double(v1[c].re)), T(u1[j_].im - double(v1[c].im))};
+    d2[c] = cx<T>{T(u2[j_].re - double(v2[c].re)), T(u2[j_].im - double(v2[c].im))};
   }
 }
 
@@ -1047,14 +1078,15 @@ struct Engine final : EngineBase {
     {
       const bool accel = p.elastic_enabled && (p.elastic_warm_start_enabled || p.elastic_residual_control_enabled);
       extended = accel && (kMixed || p.elastic_warm_start_order > 1 || p.elastic_anderson_depth > 0);
-      if ((kMixed || extended) && (rc = dalloc(&Ud, 3 * nSc))) return rc;
-      if (kMixed && ((rc = dalloc(&phis, R)) || (rc = dalloc(&Phik, nSc)))) return rc;
+      // solver state by retained-mode index (nScA entries per component)
+      if ((kMixed || extended) && (rc = dalloc(&Ud, 3 * nScA))) return rc;
+      if (kMixed && ((rc = dalloc(&phis, R)) || (rc = dalloc(&Phik, nScA)))) return rc;
       if (extended) {
-        if ((rc = dalloc(&Gd, 3 * nSc)) || (rc = dalloc(&Hist, 9 * nSc)) || (rc = dalloc(&dinc, 3 * nSc)) ||
-            (rc = dalloc(&dU, (size_t)AA_MAX * 3 * nSc)) || (rc = dalloc(&dG, (size_t)AA_MAX * 3 * nSc)) ||
+        if ((rc = dalloc(&Gd, 3 * nScA)) || (rc = dalloc(&Hist, 9 * nScA)) || (rc = dalloc(&dinc, 3 * nScA)) ||
+            (rc = dalloc(&dU, (size_t)AA_MAX * 3 * nScA)) || (rc = dalloc(&dG, (size_t)AA_MAX * 3 * nScA)) ||
             (rc = dalloc(&hcache, R)))
           return rc;
-        for (int k = 0; k < 3; ++k) hbuf[k] = Hist + (size_t)k * 3 * nSc;
+        for (int k = 0; k < 3; ++k) hbuf[k] = Hist + (size_t)k * 3 * nScA;
         PF_HIP(hipMemset(hcache, 0, R * sizeof(T)));
       }
     }
@@ -1078,13 +1110,14 @@ struct Engine final : EngineBase {
           {Sz, 6 * nSz * sizeof(C)}, {Sy, 6 * nSy * sizeof(C)}, {Sc, 6 * nSc * sizeof(C)},
           {u, 3 * nSc * sizeof(C)},  {un, 3 * nSc * sizeof(C)}};
       for (const auto& z : zero) PF_HIP(hipMemset(z.first, 0, z.second));
-      if (Ud) PF_HIP(hipMemset(Ud, 0, 3 * nSc * sizeof(cx<double>)));
+      if (Ud) PF_HIP(hipMemset(Ud, 0, 3 * nScA * sizeof(cx<double>)));
+      if (Phik) PF_HIP(hipMemset(Phik, 0, nScA * sizeof(cx<double>)));
       if (extended) {
-        PF_HIP(hipMemset(Gd, 0, 3 * nSc * sizeof(cx<double>)));
-        PF_HIP(hipMemset(Hist, 0, 9 * nSc * sizeof(cx<double>)));
-        PF_HIP(hipMemset(dinc, 0, 3 * nSc * sizeof(C)));
-        PF_HIP(hipMemset(dU, 0, (size_t)AA_MAX * 3 * nSc * sizeof(C)));
-        PF_HIP(hipMemset(dG, 0, (size_t)AA_MAX * 3 * nSc * sizeof(C)));
+        PF_HIP(hipMemset(Gd, 0, 3 * nScA * sizeof(cx<double>)));
+        PF_HIP(hipMemset(Hist, 0, 9 * nScA * sizeof(cx<double>)));
+        PF_HIP(hipMemset(dinc, 0, 3 * nScA * sizeof(C)));
+        PF_HIP(hipMemset(dU, 0, (size_t)AA_MAX * 3 * nScA * sizeof(C)));
+        PF_HIP(hipMemset(dG, 0, (size_t)AA_MAX * 3 * nScA * sizeof(C)));
       }
     }
     // partial-sum scratch: largest grid times 4 reductions
@@ -1114,8 +1147,6 @@ struct Engine final : EngineBase {
     nSz = (long)Nx * Ny * AzS;
     nSy = (long)Nx * ay.A * AzS;
     nSc = (long)ax.A * ay.A * AzS;
-    if (Hist)
-      for (int k = 0; k < 3; ++k) hbuf[k] = Hist + (size_t)k * 3 * nSc;
   }
   int clear_spectra() {
     const size_t cs = sizeof(C);
@@ -1125,14 +1156,14 @@ struct Engine final : EngineBase {
     PF_HIP(hipMemset(Sc, 0, 6 * wc * cs));
     PF_HIP(hipMemset(u, 0, 3 * wc * cs));
     PF_HIP(hipMemset(un, 0, 3 * wc * cs));
-    if (Ud) PF_HIP(hipMemset(Ud, 0, 3 * wc * sizeof(cx<double>)));
-    if (Phik) PF_HIP(hipMemset(Phik, 0, wc * sizeof(cx<double>)));
+    if (Ud) PF_HIP(hipMemset(Ud, 0, 3 * nScA * sizeof(cx<double>)));
+    if (Phik) PF_HIP(hipMemset(Phik, 0, nScA * sizeof(cx<double>)));
     if (extended) {
-      PF_HIP(hipMemset(Gd, 0, 3 * wc * sizeof(cx<double>)));
-      PF_HIP(hipMemset(Hist, 0, 9 * wc * sizeof(cx<double>)));
-      PF_HIP(hipMemset(dinc, 0, 3 * wc * cs));
-      PF_HIP(hipMemset(dU, 0, (size_t)AA_MAX * 3 * wc * cs));
-      PF_HIP(hipMemset(dG, 0, (size_t)AA_MAX * 3 * wc * cs));
+      PF_HIP(hipMemset(Gd, 0, 3 * nScA * sizeof(cx<double>)));
+      PF_HIP(hipMemset(Hist, 0, 9 * nScA * sizeof(cx<double>)));
+      PF_HIP(hipMemset(dinc, 0, 3 * nScA * cs));
+      PF_HIP(hipMemset(dU, 0, (size_t)AA_MAX * 3 * nScA * cs));
+      PF_HIP(hipMemset(dG, 0, (size_t)AA_MAX * 3 * nScA * cs));
       PF_HIP(hipMemset(hcache, 0, R * sizeof(T)));
     }
     return PF_OK;
@@ -1319,8 +1350,8 @@ struct Engine final : EngineBase {
   const cx<double>* U_last() {
     return (extended && hist_count > 0) ? hist_slot((hist_head + 2) % 3) : Ud;
   }
-  C* du_slot(int j, int i) { return dU + ((size_t)j * 3 + i) * nSc; }
-  C* dg_slot(int j, int i) { return dG + ((size_t)j * 3 + i) * nSc; }
+  C* du_slot(int j, int i) { return dU + ((size_t)j * 3 + i) * nScA; }
+  C* dg_slot(int j, int i) { return dG + ((size_t)j * 3 + i) * nScA; }
 
   // Accelerated fixed point (same map F, residual |F(U)-U|/|F(U)| and test as
   // main_cuda.cu:28887-29046): warm start by polynomial extrapolation of the
@@ -1340,12 +1371,13 @@ struct Engine final : EngineBase {
       // extrapolated start and its strain in one pass over the retained modes
       // (the kz padding of Ud stays zero)
       StrainArgs<T, double> sa{};
+      sa.ustate = 1;
       sa.d = d;
       sa.n = nSc;
       for (int i = 0; i < 3; ++i) {
-        sa.u[i] = Ud + (size_t)i * nSc;
-        sa.Uout[i] = Ud + (size_t)i * nSc;
-        for (int k = 0; k < order; ++k) sa.H[k][i] = h_at(k) + (size_t)i * nSc;
+        sa.u[i] = Ud + (size_t)i * nScA;
+        sa.Uout[i] = Ud + (size_t)i * nScA;
+        for (int k = 0; k < order; ++k) sa.H[k][i] = h_at(k) + (size_t)i * nScA;
       }
       for (int k = 0; k < order; ++k) sa.cf[k] = coef[order - 1][k];
       sa.nh = order;
@@ -1363,7 +1395,7 @@ struct Engine final : EngineBase {
         ga.h[i] = sc(i);
         ga.eps[i] = sc(i);
       }
-      for (int i = 0; i < 3; ++i) ga.U[i] = Ud + (size_t)i * nSc;
+      for (int i = 0; i < 3; ++i) ga.U[i] = Ud + (size_t)i * nScA;
       ga.partials = nullptr;
       for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
       ga.mode = 0;
@@ -1372,9 +1404,10 @@ struct Engine final : EngineBase {
         int rc = cold_hook();
         if (rc) return rc;
         StrainArgs<T, double> sa{};
+      sa.ustate = 1;
         sa.d = d;
         sa.n = nSc;
-        for (int i = 0; i < 3; ++i) sa.u[i] = Ud + (size_t)i * nSc;
+        for (int i = 0; i < 3; ++i) sa.u[i] = Ud + (size_t)i * nScA;
         for (int i = 0; i < 6; ++i) sa.eps[i] = sc(i);
         hipLaunchKernelGGL((kop_strain<T, double>), dim3(gmodes()), dim3(NT), 0, stream, sa);
       }
@@ -1400,9 +1433,9 @@ struct Engine final : EngineBase {
           ua.d = d;
           ua.n = nSc;
           for (int i = 0; i < 3; ++i) {
-            ua.g[i] = Gd + (size_t)i * nSc;
-            ua.U[i] = Ud + (size_t)i * nSc;
-            ua.dinc[i] = dinc + (size_t)i * nSc;
+            ua.g[i] = Gd + (size_t)i * nScA;
+            ua.U[i] = Ud + (size_t)i * nScA;
+            ua.dinc[i] = dinc + (size_t)i * nScA;
             for (int j = 0; j < AA_MAX; ++j) {
               ua.du[j][i] = du_slot(j, i);
               ua.dg[j][i] = dg_slot(j, i);
@@ -1429,9 +1462,9 @@ struct Engine final : EngineBase {
         ga.n = nSc;
         for (int i = 0; i < 6; ++i) ga.h[i] = sc(i);
         for (int i = 0; i < 3; ++i) {
-          ga.U[i] = Ud + (size_t)i * nSc;
-          ga.g[i] = Gd + (size_t)i * nSc;
-          ga.dinc[i] = dinc + (size_t)i * nSc;
+          ga.U[i] = Ud + (size_t)i * nScA;
+          ga.g[i] = Gd + (size_t)i * nScA;
+          ga.dinc[i] = dinc + (size_t)i * nScA;
           for (int j = 0; j < AA_MAX; ++j) {
             ga.dg[j][i] = dg_slot(j, i);
             ga.du[j][i] = du_slot(j, i);
@@ -1460,8 +1493,8 @@ struct Engine final : EngineBase {
       last_iters = it;
       return fail(PF_E_NONFINITE, "elastic residual is non-finite at step " + std::to_string(steps_done + 1));
     }
-    hipLaunchKernelGGL(kop_aa_final<T>, dim3(gmodes()), dim3(NT), 0, stream, d, nSc, Ud, Ud + nSc, Ud + 2 * nSc,
-                       Gd, Gd + nSc, Gd + 2 * nSc, sc(0), sc(1), sc(2), sc(3), sc(4), sc(5));
+    hipLaunchKernelGGL(kop_aa_final<T>, dim3(gmodes()), dim3(NT), 0, stream, d, nSc, Ud, Ud + nScA,
+                       Ud + 2 * nScA, Gd, Gd + nScA, Gd + 2 * nScA, sc(0), sc(1), sc(2), sc(3), sc(4), sc(5));
     xpass(true, 0, 6, Sc);
     ypass(true, 0, 6);
     std::swap(hbuf[hist_head], Ud);
@@ -1499,7 +1532,7 @@ struct Engine final : EngineBase {
         ga.h[i] = sc(i);
         ga.eps[i] = sc(i);
       }
-      for (int i = 0; i < 3; ++i) ga.U[i] = Ud + (size_t)i * nSc;
+      for (int i = 0; i < 3; ++i) ga.U[i] = Ud + (size_t)i * nScA;
       ga.partials = resid ? partials : nullptr;
       for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
       ga.mode = mode;
@@ -1507,9 +1540,10 @@ struct Engine final : EngineBase {
     };
     auto strain_of_U = [&]() {
       StrainArgs<T, double> sa{};
+      sa.ustate = 1;
       sa.d = d;
       sa.n = nSc;
-      for (int i = 0; i < 3; ++i) sa.u[i] = Ud + (size_t)i * nSc;
+      for (int i = 0; i < 3; ++i) sa.u[i] = Ud + (size_t)i * nScA;
       for (int i = 0; i < 6; ++i) sa.eps[i] = sc(i);
       hipLaunchKernelGGL((kop_strain<T, double>), dim3(gmodes()), dim3(NT), 0, stream, sa);
     };
@@ -1952,8 +1986,8 @@ struct Engine final : EngineBase {
       if (rc) return rc;
       force_map(false);
       if (state_fp64())
-        hipLaunchKernelGGL((kop_add3<T, cx<double>>), dim3(gk), dim3(NT), 0, stream, nSc, (const C*)sc(0),
-                           (const C*)sc(1), (const C*)sc(2), Ud, Ud + nSc, Ud + 2 * nSc);
+        hipLaunchKernelGGL(kop_add3_state<T>, dim3(gmodes()), dim3(NT), 0, stream, kd(), nSc, (const C*)sc(0),
+                           (const C*)sc(1), (const C*)sc(2), Ud, Ud + nScA, Ud + 2 * nScA);
       else
         hipLaunchKernelGGL((kop_add3<T, C>), dim3(gk), dim3(NT), 0, stream, nSc, (const C*)sc(0), (const C*)sc(1),
                            (const C*)sc(2), uu(u, 0), uu(u, 1), uu(u, 2));
@@ -1979,10 +2013,10 @@ struct Engine final : EngineBase {
     if (!in_cold_hook) return set_elastic_state(in);
     // inside a cold start: replace the start of this solve only
     if (state_fp64()) {
-      PF_HIP(hipMemcpyAsync(Ud, in, 3 * nSc * sizeof(cx<double>), hipMemcpyHostToDevice, stream));
+      PF_HIP(hipMemcpyAsync(Ud, in, 3 * nScA * sizeof(cx<double>), hipMemcpyHostToDevice, stream));
     } else {
-      std::vector<C> t(3 * nSc);
-      for (long i = 0; i < 3 * nSc; ++i) t[i] = C{T(in[2 * i]), T(in[2 * i + 1])};
+      std::vector<C> t;
+      state_to_slots(in, &t);
       PF_HIP(hipMemcpyAsync(u, t.data(), 3 * nSc * sizeof(C), hipMemcpyHostToDevice, stream));
       PF_HIP(hipStreamSynchronize(stream));
     }
@@ -2036,9 +2070,9 @@ struct Engine final : EngineBase {
     ga.have_old = 0;
     hipLaunchKernelGGL(kop_green<T>, dim3(gk), dim3(NT), 0, stream, ga);
     if (state_fp64())
-      hipLaunchKernelGGL((kop_diff3<T, cx<double>>), dim3(gk), dim3(NT), 0, stream, nSc, U_last(),
-                         U_last() + nSc, U_last() + 2 * nSc, (const C*)uu(un, 0),
-                         (const C*)uu(un, 1), (const C*)uu(un, 2), sc(0), sc(1), sc(2));
+      hipLaunchKernelGGL(kop_diff3_state<T>, dim3(gmodes()), dim3(NT), 0, stream, d, nSc, U_last(),
+                         U_last() + nScA, U_last() + 2 * nScA, (const C*)uu(un, 0), (const C*)uu(un, 1),
+                         (const C*)uu(un, 2), sc(0), sc(1), sc(2));
     else
       hipLaunchKernelGGL((kop_diff3<T, C>), dim3(gk), dim3(NT), 0, stream, nSc, (const C*)uu(u, 0),
                          (const C*)uu(u, 1), (const C*)uu(u, 2), (const C*)uu(un, 0), (const C*)uu(un, 1),
@@ -2800,53 +2834,69 @@ struct Engine final : EngineBase {
     return PF_OK;
   }
 
-  long long elastic_state_size() override { return nSc; }
+  // The external warm state is 3 components over the retained modes,
+  // [Ax][Ay][Az] with no kz padding, whatever stride the spectra use; the
+  // plain solver keeps its state at k-space slots and is converted here.
+  long slot_of(long j) const { return (j / az.A) * AzS + j % az.A; }
+  void state_to_slots(const double* in, std::vector<C>* t) const {
+    t->assign(3 * nSc, C{T(0), T(0)});
+    for (int i = 0; i < 3; ++i)
+      for (long j = 0; j < nScA; ++j) {
+        const long q = (long)i * nScA + j;
+        (*t)[(size_t)i * nSc + slot_of(j)] = C{T(in[2 * q]), T(in[2 * q + 1])};
+      }
+  }
+  void slots_to_state(const std::vector<C>& t, cx<double>* out) const {
+    for (int i = 0; i < 3; ++i)
+      for (long j = 0; j < nScA; ++j) {
+        const C v = t[(size_t)i * nSc + slot_of(j)];
+        out[(size_t)i * nScA + j] = cx<double>{double(v.re), double(v.im)};
+      }
+  }
+  long long elastic_state_size() override { return nScA; }
   int get_elastic_state(double* out) override {
     PF_HIP(hipStreamSynchronize(stream));
     if (kMixed || extended) {
-      PF_HIP(hipMemcpy(out, U_last(), 3 * nSc * sizeof(cx<double>), hipMemcpyDeviceToHost));
+      PF_HIP(hipMemcpy(out, U_last(), 3 * nScA * sizeof(cx<double>), hipMemcpyDeviceToHost));
       return PF_OK;
     }
     std::vector<C> t(3 * nSc);
     PF_HIP(hipMemcpy(t.data(), u, 3 * nSc * sizeof(C), hipMemcpyDeviceToHost));
-    for (long i = 0; i < 3 * nSc; ++i) {
-      out[2 * i] = double(t[i].re);
-      out[2 * i + 1] = double(t[i].im);
-    }
+    slots_to_state(t, reinterpret_cast<cx<double>*>(out));
     return PF_OK;
   }
   int set_elastic_state(const double* in) override {
     if (kMixed || extended) {
-      PF_HIP(hipMemcpy(Ud, in, 3 * nSc * sizeof(cx<double>), hipMemcpyHostToDevice));
+      PF_HIP(hipMemcpy(Ud, in, 3 * nScA * sizeof(cx<double>), hipMemcpyHostToDevice));
       if (extended) {
-        PF_HIP(hipMemcpy(hist_slot(0), Ud, 3 * nSc * sizeof(cx<double>), hipMemcpyDeviceToDevice));
+        PF_HIP(hipMemcpy(hist_slot(0), Ud, 3 * nScA * sizeof(cx<double>), hipMemcpyDeviceToDevice));
         hist_head = 1;
         hist_count = 1;
       }
       warm_valid = true;
       return PF_OK;
     }
-    std::vector<C> t(3 * nSc);
-    for (long i = 0; i < 3 * nSc; ++i) t[i] = C{T(in[2 * i]), T(in[2 * i + 1])};
+    std::vector<C> t;
+    state_to_slots(in, &t);
     PF_HIP(hipMemcpy(u, t.data(), 3 * nSc * sizeof(C), hipMemcpyHostToDevice));
     warm_valid = true;
     return PF_OK;
   }
 
   // ---------------------------------------------------------------- checkpoint / restart
-  // The warm state after the last accepted step, FP64, compact layout (3 x nSc).
+  // The warm state after the last accepted step, FP64, retained modes (3 x nScA).
   int warm_state_host(std::vector<cx<double>>& v) {
-    v.resize(3 * nSc);
+    v.resize(3 * nScA);
     if (state_fp64()) {
-      PF_HIP(hipMemcpy(v.data(), U_last(), 3 * nSc * sizeof(cx<double>), hipMemcpyDeviceToHost));
+      PF_HIP(hipMemcpy(v.data(), U_last(), 3 * nScA * sizeof(cx<double>), hipMemcpyDeviceToHost));
     } else {
       std::vector<C> t(3 * nSc);
       PF_HIP(hipMemcpy(t.data(), u, 3 * nSc * sizeof(C), hipMemcpyDeviceToHost));
-      for (long i = 0; i < 3 * nSc; ++i) v[i] = cx<double>{double(t[i].re), double(t[i].im)};
+      slots_to_state(t, v.data());
     }
     return PF_OK;
   }
-  // compact index <-> full half-spectrum index [Nx][Ny][Nz/2+1] of a retained mode
+  // retained-mode index <-> full half-spectrum index [Nx][Ny][Nz/2+1]
   template <typename F>
   void for_retained(F f) const {
     const long kfull = (long)Nx * Ny * (Nz / 2 + 1);
@@ -2856,7 +2906,7 @@ struct Engine final : EngineBase {
       for (int cy = 0; cy < ay.A; ++cy) {
         const int iy = cy < ay.a ? cy : cy + Ny - ay.A;
         for (int cz = 0; cz < az.A; ++cz)
-          f(((long)cx_ * ay.A + cy) * AzS + cz, ((long)ix * Ny + iy) * (Nz / 2 + 1) + cz);
+          f(((long)cx_ * ay.A + cy) * az.A + cz, ((long)ix * Ny + iy) * (Nz / 2 + 1) + cz);
       }
     }
   }
@@ -2909,7 +2959,7 @@ struct Engine final : EngineBase {
       c.displacement_k.assign(6 * kfull, 0.0f);
       for (int i = 0; i < 3; ++i) {
         float* d = c.displacement_k.data() + 2 * kfull * i;
-        const cx<double>* src = w.data() + (size_t)i * nSc;
+        const cx<double>* src = w.data() + (size_t)i * nScA;
         for_retained([&](long cc, long fi) {
           d[2 * fi] = (float)src[cc].re;
           d[2 * fi + 1] = (float)src[cc].im;
@@ -2927,21 +2977,21 @@ struct Engine final : EngineBase {
       x.bound_checksum = c.payload_checksum;
       x.accepted_step = c.accepted_step;
       x.precision = kMixed ? PF_MIXED : PF_FP64;
-      x.n = (std::uint64_t)nSc;
+      x.n = (std::uint64_t)nScA;   // retained modes: independent of the tuned kz stride
       const int nh = extended ? hist_count : 0;
       x.nstates = c.elastic_present ? 1 + nh : 0;
-      x.states.resize((size_t)x.nstates * 6 * nSc);
+      x.states.resize((size_t)x.nstates * 6 * nScA);
       if (c.elastic_present) {
-        std::memcpy(x.states.data(), w.data(), 3 * nSc * sizeof(cx<double>));
+        std::memcpy(x.states.data(), w.data(), 3 * nScA * sizeof(cx<double>));
         for (int k = 0; k < nh; ++k)
-          PF_HIP(hipMemcpy(x.states.data() + (size_t)(1 + k) * 6 * nSc, hist_slot((hist_head + 2 - k) % 3),
-                           3 * nSc * sizeof(cx<double>), hipMemcpyDeviceToHost));
+          PF_HIP(hipMemcpy(x.states.data() + (size_t)(1 + k) * 6 * nScA, hist_slot((hist_head + 2 - k) % 3),
+                           3 * nScA * sizeof(cx<double>), hipMemcpyDeviceToHost));
       }
       if (phi_block) {
         x.phi_star.resize(R);
-        x.phi_spec.resize(2 * (size_t)nSc);
+        x.phi_spec.resize(2 * (size_t)nScA);
         PF_HIP(hipMemcpy(x.phi_star.data(), phis, R * sizeof(double), hipMemcpyDeviceToHost));
-        PF_HIP(hipMemcpy(x.phi_spec.data(), Phik, nSc * sizeof(cx<double>), hipMemcpyDeviceToHost));
+        PF_HIP(hipMemcpy(x.phi_spec.data(), Phik, nScA * sizeof(cx<double>), hipMemcpyDeviceToHost));
       }
       if (!ckpt::write_exact(std::string(path) + ".state", x, &err)) return fail(PF_E_IO, err);
     }
@@ -2975,7 +3025,7 @@ struct Engine final : EngineBase {
     }
     // the companion belongs to this file and this handle
     const bool x_ok = have_x && x.bound_checksum == c.payload_checksum && x.accepted_step == c.accepted_step &&
-                      x.n == (std::uint64_t)nSc && x.precision == (kMixed ? PF_MIXED : PF_FP64);
+                      x.n == (std::uint64_t)nScA && x.precision == (kMixed ? PF_MIXED : PF_FP64);
     PF_HIP(hipStreamSynchronize(stream));
     // from here on a rejection leaves the handle without a valid state
     state_ok = false;
@@ -3016,9 +3066,9 @@ struct Engine final : EngineBase {
     // mixed precision: phi* and its spectrum from the companion continue the
     // increment form bit for bit; without them the next update starts it anew
     phi_banded = false;
-    if (kMixed && x_ok && x.phi_star.size() == (size_t)R && x.phi_spec.size() == 2 * (size_t)nSc) {
+    if (kMixed && x_ok && x.phi_star.size() == (size_t)R && x.phi_spec.size() == 2 * (size_t)nScA) {
       PF_HIP(hipMemcpy(phis, x.phi_star.data(), R * sizeof(double), hipMemcpyHostToDevice));
-      PF_HIP(hipMemcpy(Phik, x.phi_spec.data(), nSc * sizeof(cx<double>), hipMemcpyHostToDevice));
+      PF_HIP(hipMemcpy(Phik, x.phi_spec.data(), nScA * sizeof(cx<double>), hipMemcpyHostToDevice));
       phi_banded = true;
     }
     if (!el) {
@@ -3027,33 +3077,33 @@ struct Engine final : EngineBase {
     }
     last_iters = (int)c.elastic_last_iterations;
     last_res = c.elastic_last_relative_residual;
-    std::vector<cx<double>> cur_state(3 * nSc);
+    std::vector<cx<double>> cur_state(3 * nScA);
     int nh = 0;
     if (x_ok && x.nstates >= 1) {
-      std::memcpy(cur_state.data(), x.states.data(), 3 * nSc * sizeof(cx<double>));
+      std::memcpy(cur_state.data(), x.states.data(), 3 * nScA * sizeof(cx<double>));
       nh = extended ? std::min(x.nstates - 1, 3) : 0;
       for (int k = 0; k < nh; ++k)   // newest first -> ring slots nh-1, ..., 0
-        PF_HIP(hipMemcpy(hist_slot(nh - 1 - k), x.states.data() + (size_t)(1 + k) * 6 * nSc,
-                         3 * nSc * sizeof(cx<double>), hipMemcpyHostToDevice));
+        PF_HIP(hipMemcpy(hist_slot(nh - 1 - k), x.states.data() + (size_t)(1 + k) * 6 * nScA,
+                         3 * nScA * sizeof(cx<double>), hipMemcpyHostToDevice));
     } else {
       // the reference's restart: the FP32 warm state of the V4 file
       std::fill(cur_state.begin(), cur_state.end(), cx<double>{0.0, 0.0});
       const long kfull = (long)Nx * Ny * (Nz / 2 + 1);
       for (int i = 0; i < 3; ++i) {
         const float* s = c.displacement_k.data() + 2 * kfull * i;
-        cx<double>* d = cur_state.data() + (size_t)i * nSc;
+        cx<double>* d = cur_state.data() + (size_t)i * nScA;
         for_retained([&](long cc, long fi) { d[cc] = cx<double>{(double)s[2 * fi], (double)s[2 * fi + 1]}; });
       }
       if (extended) {
-        PF_HIP(hipMemcpy(hist_slot(0), cur_state.data(), 3 * nSc * sizeof(cx<double>), hipMemcpyHostToDevice));
+        PF_HIP(hipMemcpy(hist_slot(0), cur_state.data(), 3 * nScA * sizeof(cx<double>), hipMemcpyHostToDevice));
         nh = 1;
       }
     }
     if (state_fp64()) {
-      PF_HIP(hipMemcpy(Ud, cur_state.data(), 3 * nSc * sizeof(cx<double>), hipMemcpyHostToDevice));
+      PF_HIP(hipMemcpy(Ud, cur_state.data(), 3 * nScA * sizeof(cx<double>), hipMemcpyHostToDevice));
     } else {
-      std::vector<C> t(3 * nSc);
-      for (long i = 0; i < 3 * nSc; ++i) t[i] = C{T(cur_state[i].re), T(cur_state[i].im)};
+      std::vector<C> t;
+      state_to_slots(reinterpret_cast<const double*>(cur_state.data()), &t);
       PF_HIP(hipMemcpy(u, t.data(), 3 * nSc * sizeof(C), hipMemcpyHostToDevice));
     }
     hist_count = nh;
@@ -3154,9 +3204,9 @@ struct Engine final : EngineBase {
           ga.n = nSc;
           for (int i = 0; i < 6; ++i) ga.h[i] = sc(i);
           for (int i = 0; i < 3; ++i) {
-            ga.U[i] = Ud + (size_t)i * nSc;
-            ga.g[i] = Gd + (size_t)i * nSc;
-            ga.dinc[i] = dinc + (size_t)i * nSc;
+            ga.U[i] = Ud + (size_t)i * nScA;
+            ga.g[i] = Gd + (size_t)i * nScA;
+            ga.dinc[i] = dinc + (size_t)i * nScA;
             for (int j = 0; j < AA_MAX; ++j) {
               ga.dg[j][i] = dg_slot(j, i);
               ga.du[j][i] = du_slot(j, i);
@@ -3196,9 +3246,9 @@ struct Engine final : EngineBase {
         ua.d = d;
         ua.n = nSc;
         for (int i = 0; i < 3; ++i) {
-          ua.g[i] = Gd + (size_t)i * nSc;
-          ua.U[i] = Ud + (size_t)i * nSc;
-          ua.dinc[i] = dinc + (size_t)i * nSc;
+          ua.g[i] = Gd + (size_t)i * nScA;
+          ua.U[i] = Ud + (size_t)i * nScA;
+          ua.dinc[i] = dinc + (size_t)i * nScA;
           for (int j = 0; j < AA_MAX; ++j) {
             ua.du[j][i] = du_slot(j, i);
             ua.dg[j][i] = dg_slot(j, i);

@@ -471,7 +471,10 @@ struct KDims {
 };
 
 // Compact index of the j-th retained mode (rows of Azr retained kz laid out
-// with stride Az).
+// with stride Az).  Arrays the x pass reads or writes live at these slots;
+// solver state that only k-space kernels touch (the FP64 displacement and
+// residual, the Anderson increments and history, the FP64 phi spectrum) is
+// stored by j itself, with no kz padding to read around.
 __device__ __forceinline__ long kslot(const KDims& d, long j) {
   const unsigned ju = (unsigned)j;
   const unsigned row = ju / (unsigned)d.Azr;
@@ -676,7 +679,7 @@ __global__ void __launch_bounds__(NT) kop_green_acc(GreenAccArgs<T> ga) {
     cx<double> dlt[3], un[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      cx<double> uo = ga.mode == 0 ? cx<double>{0.0, 0.0} : ga.U[i][c];
+      cx<double> uo = ga.mode == 0 ? cx<double>{0.0, 0.0} : ga.U[i][j_];
       if (ga.mode == 0) {
         un[i] = g[i];
         dlt[i] = g[i];
@@ -687,7 +690,7 @@ __global__ void __launch_bounds__(NT) kop_green_acc(GreenAccArgs<T> ga) {
         dlt[i] = cx<double>{g[i].re - uo.re, g[i].im - uo.im};
         un[i] = g[i];
       }
-      ga.U[i][c] = un[i];
+      ga.U[i][j_] = un[i];
     }
     if (ga.partials) {
       const double w = (iz >= ga.d.Azr) ? 0.0 : ((iz == 0) ? 1.0 : 2.0);
@@ -816,25 +819,25 @@ __global__ void __launch_bounds__(NT) kop_green_aa(GreenAAArgs<T> a) {
     cx<T> dT[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      U[i] = a.U[i][c];
+      U[i] = a.U[i][j_];
       if (a.mode == 0) {
         gn[i] = cx<double>{Gh[i].re - U[i].re, Gh[i].im - U[i].im};
         dgn[i] = cx<double>{0.0, 0.0};
         dT[i] = cx<T>{T(0), T(0)};
       } else {
-        const cx<double> go = a.g[i][c];
-        const cx<T> dt = dsrc[i][c];
+        const cx<double> go = a.g[i][j_];
+        const cx<T> dt = dsrc[i][j_];
         dT[i] = dt;
         const cx<double> di = {double(dt.re), double(dt.im)};
         if (a.ctl) {
           // the increment applied by this iteration's kop_aa_update
           U[i] = cx<double>{U[i].re + di.re, U[i].im + di.im};
-          a.U[i][c] = U[i];
+          a.U[i][j_] = U[i];
         }
         gn[i] = cx<double>{go.re + Gh[i].re - di.re, go.im + Gh[i].im - di.im};
         dgn[i] = cx<double>{gn[i].re - go.re, gn[i].im - go.im};
       }
-      a.g[i][c] = gn[i];
+      a.g[i][j_] = gn[i];
     }
     const double w = (iz >= a.d.Azr) ? 0.0 : ((iz == 0) ? 1.0 : 2.0);
     cx<double> Fu[3] = {{U[0].re + gn[0].re, U[0].im + gn[0].im},
@@ -852,13 +855,13 @@ __global__ void __launch_bounds__(NT) kop_green_aa(GreenAAArgs<T> a) {
 #pragma unroll
       for (int j = 0; j < AA_MAX; ++j) {
         if (j == slot) {
-          a.dg[j][0][c] = q0;
-          a.dg[j][1][c] = q1;
-          a.dg[j][2][c] = q2;
+          a.dg[j][0][j_] = q0;
+          a.dg[j][1][j_] = q1;
+          a.dg[j][2][j_] = q2;
           if (hist) {
-            a.du[j][0][c] = cx<T>{T(double(dT[0].re) + double(q0.re)), T(double(dT[0].im) + double(q0.im))};
-            a.du[j][1][c] = cx<T>{T(double(dT[1].re) + double(q1.re)), T(double(dT[1].im) + double(q1.im))};
-            a.du[j][2][c] = cx<T>{T(double(dT[2].re) + double(q2.re)), T(double(dT[2].im) + double(q2.im))};
+            a.du[j][0][j_] = cx<T>{T(double(dT[0].re) + double(q0.re)), T(double(dT[0].im) + double(q0.im))};
+            a.du[j][1][j_] = cx<T>{T(double(dT[1].re) + double(q1.re)), T(double(dT[1].im) + double(q1.im))};
+            a.du[j][2][j_] = cx<T>{T(double(dT[2].re) + double(q2.re)), T(double(dT[2].im) + double(q2.im))};
           }
         }
       }
@@ -873,7 +876,7 @@ __global__ void __launch_bounds__(NT) kop_green_aa(GreenAAArgs<T> a) {
           } else {
 #pragma unroll
             for (int i = 0; i < 3; ++i) {
-              const cx<T> t = a.dg[j][i][c];
+              const cx<T> t = a.dg[j][i][j_];
               v[i] = cx<double>{double(t.re), double(t.im)};
             }
           }
@@ -933,18 +936,18 @@ __global__ void __launch_bounds__(NT) kop_aa_update(AAUpdateArgs<T> a) {
     cx<double> dd[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      cx<double> v = a.g[i][c];
+      cx<double> v = a.g[i][j_];
 #pragma unroll
       for (int j = 0; j < AA_MAX; ++j) {
         if (j < nhist) {
           if (a.ctl) {
             // the history slot holds du + dg (kop_green_aa)
-            const cx<T> e1 = a.du[j][i][c];
+            const cx<T> e1 = a.du[j][i][j_];
             v.re -= gamma[j] * double(e1.re);
             v.im -= gamma[j] * double(e1.im);
           } else {
-            const cx<T> u1 = a.du[j][i][c];
-            const cx<T> g1 = a.dg[j][i][c];
+            const cx<T> u1 = a.du[j][i][j_];
+            const cx<T> g1 = a.dg[j][i][j_];
             v.re -= gamma[j] * (double(u1.re) + double(g1.re));
             v.im -= gamma[j] * (double(u1.im) + double(g1.im));
           }
@@ -952,13 +955,13 @@ __global__ void __launch_bounds__(NT) kop_aa_update(AAUpdateArgs<T> a) {
       }
       const cx<T> q = {T(v.re), T(v.im)};
       dd[i] = cx<double>{double(q.re), double(q.im)};
-      if (!(a.ctl && slot >= 0)) a.dinc[i][c] = q;   // with a history, du[slot] holds it
+      if (!(a.ctl && slot >= 0)) a.dinc[i][j_] = q;   // with a history, du[slot] holds it
 #pragma unroll
       for (int j = 0; j < AA_MAX; ++j)
-        if (j == slot) a.du[j][i][c] = q;
+        if (j == slot) a.du[j][i][j_] = q;
       if (!a.ctl) {   // under the control block kop_green_aa adds d to U
-        cx<double> uo = a.U[i][c];
-        a.U[i][c] = cx<double>{uo.re + dd[i].re, uo.im + dd[i].im};
+        cx<double> uo = a.U[i][j_];
+        a.U[i][j_] = cx<double>{uo.re + dd[i].re, uo.im + dd[i].im};
       }
     }
     strain_of<T>(kx, ky, kz, dd, a.eps, c);
@@ -984,9 +987,9 @@ __global__ void __launch_bounds__(NT) kop_aa_final(KDims d, long n, cx<double>* 
     cx<double> u[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      const cx<double> a = U[i][c], b = g[i][c];
+      const cx<double> a = U[i][j_], b = g[i][j_];
       u[i] = cx<double>{a.re + b.re, a.im + b.im};
-      U[i][c] = u[i];
+      U[i][j_] = u[i];
     }
     strain_of<T>(kx, ky, kz, u, eps, c);
   }
@@ -1004,6 +1007,7 @@ struct StrainArgs {
   const cx<double>* H[3][3];   // [k][component]
   double cf[3];
   int nh;
+  int ustate;   // u is solver state (retained-mode index), not a k-space array
 };
 template <typename T, typename UT = T>
 __global__ void __launch_bounds__(NT) kop_strain(StrainArgs<T, UT> sa) {
@@ -1017,24 +1021,27 @@ __global__ void __launch_bounds__(NT) kop_strain(StrainArgs<T, UT> sa) {
     if (sa.nh > 0) {
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        const cx<double> h1 = sa.H[0][i][c];
+        const cx<double> h1 = sa.H[0][i][j_];
         cx<double> v = {sa.cf[0] * h1.re, sa.cf[0] * h1.im};
         if (sa.nh > 1) {
-          const cx<double> h2 = sa.H[1][i][c];
+          const cx<double> h2 = sa.H[1][i][j_];
           v.re += sa.cf[1] * h2.re;
           v.im += sa.cf[1] * h2.im;
         }
         if (sa.nh > 2) {
-          const cx<double> h3 = sa.H[2][i][c];
+          const cx<double> h3 = sa.H[2][i][j_];
           v.re += sa.cf[2] * h3.re;
           v.im += sa.cf[2] * h3.im;
         }
-        sa.Uout[i][c] = v;
+        sa.Uout[i][j_] = v;
         u[i] = v;
       }
     } else {
 #pragma unroll
-      for (int i = 0; i < 3; ++i) u[i] = cx<double>{double(sa.u[i][c].re), double(sa.u[i][c].im)};
+      for (int i = 0; i < 3; ++i) {
+        const long q = sa.ustate ? j_ : c;
+        u[i] = cx<double>{double(sa.u[i][q].re), double(sa.u[i][q].im)};
+      }
     }
     strain_of<T>(kx, ky, kz, u, sa.eps, c);
   }
@@ -1061,13 +1068,13 @@ __global__ void __launch_bounds__(NT) kop_phi_state(KDims d, long n, cx<T>* g, d
     const double den = 1.0 + Ldt_kappa * k2;
     const cx<T> v = g[c];
     if (mode == 2) {
-      const cx<double> F = Phi[c];
+      const cx<double> F = Phi[j_];
       const cx<double> dd = {(double(v.re) - Ldt_kappa * k2 * F.re) / den, (double(v.im) - Ldt_kappa * k2 * F.im) / den};
-      Phi[c] = cx<double>{F.re + dd.re, F.im + dd.im};
+      Phi[j_] = cx<double>{F.re + dd.re, F.im + dd.im};
       g[c] = cx<T>{T(dd.re), T(dd.im)};
     } else {
       const cx<double> w = {double(v.re) / den, double(v.im) / den};
-      Phi[c] = w;
+      Phi[j_] = w;
       g[c] = cx<T>{T(w.re), T(w.im)};
     }
   }

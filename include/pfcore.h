@@ -167,9 +167,9 @@ int pf_keep_stress(pf_handle* h, int enable);
 /* Mass, volume fraction, energy densities and solver statistics. */
 int pf_scalars_get(pf_handle* h, pf_scalars* out);
 
-/* Elastic warm state u(k): 3 components of the compact retained spectrum,
- * interleaved (re,im) doubles.  pf_elastic_state_size gives the count of
- * complex values per component. */
+/* Elastic warm state u(k): 3 components over the retained (2/3-rule) modes,
+ * [Ax][Ay][Az] in C order with no padding, interleaved (re,im) doubles.
+ * pf_elastic_state_size gives the count of complex values per component. */
 long long pf_elastic_state_size(pf_handle* h);
 int pf_get_elastic_state(pf_handle* h, double* out);
 int pf_set_elastic_state(pf_handle* h, const double* in);
