@@ -20,6 +20,7 @@ CLI = os.path.join(HERE, "bin", "main_pf")
 HIP_SOURCES = ["pf_engine_f64.hip", "pf_engine_f32.hip", "pf_runtime.hip", "pf_nn.hip"]
 CXX_SOURCES = ["pf_params.cpp", "pf_batch.cpp", "pf_checkpoint.cpp"]
 CLI_SOURCES = ["main_pf.cpp"]
+PASS_GROUPS = 7   # z-pass operator groups 0-4 and 6, the column passes (5), pf_pass_inst.hpp
 
 ARCH = os.environ.get("PF_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -47,6 +48,14 @@ def build(jobs=None, verbose=True):
     for src in HIP_SOURCES:
         obj = os.path.join(OBJ, src + ".o")
         steps.append((obj, [HIPCC, *COMMON, *HIP_FLAGS, "-c", "-o", obj, os.path.join(CSRC, src)]))
+    # the pass launchers of each group and precision compile in parallel
+    # with the engine (pf_pass_inst.hpp)
+    for real in ("float", "double"):
+        for g in range(PASS_GROUPS):
+            obj = os.path.join(OBJ, f"pf_pass_inst_{real}_{g}.o")
+            steps.append((obj, [HIPCC, *COMMON, *HIP_FLAGS, f"-DPF_PASS_GROUP={g}",
+                                f"-DPF_PASS_REAL_IS_FLOAT={1 if real == 'float' else 0}", "-c", "-o", obj,
+                                os.path.join(CSRC, "pf_pass_inst.hip")]))
     for src in CXX_SOURCES:
         obj = os.path.join(OBJ, src + ".o")
         steps.append((obj, ["g++", *COMMON, "-pthread", "-c", "-o", obj, os.path.join(CSRC, src)]))

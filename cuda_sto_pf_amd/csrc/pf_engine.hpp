@@ -885,6 +885,10 @@ void launch_cpass(int N, hipStream_t s, const CPassArgs<T>& ca, int planes, int 
   }
 }
 
+}  // namespace pf
+#include "pf_pass_inst.hpp"
+namespace pf {
+
 inline int grid_for(long n) {
   long g = (n + NT - 1) / NT;
   if (g > 2048) g = 2048;
@@ -1126,10 +1130,10 @@ struct Engine final : EngineBase {
     const int rows = Nx * Ny;
     maxg = std::max<long>(maxg, (rows + 1) / 2);
     npartials = (size_t)maxg * 16;   // up to 16 reductions per block (Anderson dots use AA_NRED)
-    if ((rc = dalloc(&partials, npartials)) || (rc = dalloc(&dscal, 256)) ||
+    if ((rc = dalloc(&partials, npartials)) || (rc = dalloc(&dscal, NSCAL)) ||
         (rc = dalloc(&epartials, (size_t)maxg * 3)))
       return rc;
-    PF_HIP(hipHostMalloc((void**)&hscal, 256 * sizeof(double), hipHostMallocDefault));
+    PF_HIP(hipHostMalloc((void**)&hscal, NSCAL * sizeof(double), hipHostMallocDefault));
     if (kz_cands.size() > 1 && (rc = tune_kz_stride())) return rc;
     return PF_OK;
   }
@@ -1866,6 +1870,8 @@ struct Engine final : EngineBase {
   static constexpr int ZM_SLOT = 64;   // dscal offset of the search state
   int zm_spec = 3;                     // search steps enqueued before the first wait
   static constexpr int AA_SLOT = 128;  // dscal offset of the accelerated solver's control block
+  static constexpr int MIN_SLOT = 256; // dscal offset of the phi-only minimize control block (MinCtlSlot)
+  static constexpr int NSCAL = 512;    // device scalars (and their pinned host mirror)
   int aa_spec = 4;                     // iterations enqueued before the solver's first wait
   const double* gate_ = nullptr;       // passes do nothing while *gate_ == 0 (set during the solve)
   int zero_mode(const double* phi_np1, int nfetch) {
@@ -2103,6 +2109,7 @@ struct Engine final : EngineBase {
     m_lambda = 0.0;
     m_V0 = o.V0;
     m_iter = 0;
+    min_ctl(MCS_INIT);
     m_conv = 0;
     m_converged = false;
     warm_valid = false;   // the eigenstrain differs from the dynamics one
@@ -2287,9 +2294,10 @@ struct Engine final : EngineBase {
     finalize(gz, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal + o, 1.0);
   }
   // semi-implicit phi update from the field phi - L dt (g_exp + lambda h') of ph
-  void min_update(const double* ph, double lambda, const int* labels = nullptr, const double* lambdas = nullptr) {
+  void min_update(const double* ph, double lambda, const int* labels = nullptr, const double* lambdas = nullptr,
+                  const double* lambda_p = nullptr) {
     const double Ldt = prm.L_phi * mopt.dt;
-    typename OpMinUpd<T>::Args oa{ph, m_gexp, lambda, Ldt, labels, lambdas};
+    typename OpMinUpd<T>::Args oa{ph, m_gexp, lambda, Ldt, labels, lambdas, lambda_p};
     launch_zpass<T, 0, 1, OpMinUpd<T>>(Nz, stream, zargs(0, 0, 0, 1), oa);
     min_forward_to_sc0();
     hipLaunchKernelGGL(kop_phi<T>, dim3(grid_for(nSc)), dim3(NT), 0, stream, kd(), nSc, sc(0),
@@ -2299,17 +2307,29 @@ struct Engine final : EngineBase {
   }
   static double min_multiplier(double s_hg, double s_h2) { return std::fabs(s_h2) > 1e-30 ? -s_hg / s_h2 : 0.0; }
 
+  // the phi-only iteration's control decisions (min_ctl_kernel)
+  void min_ctl(int stage) {
+    const MinCtlConst c{(double)R, std::fmin(std::fmax(mopt.eta_lambda_vol, 0.0), 1.0), m_V0, m_iter == 1 ? 1 : 0};
+    hipLaunchKernelGGL(min_ctl_kernel, dim3(1), dim3(64), 0, stream, (const double*)dscal, dscal + MIN_SLOT, stage,
+                       c);
+  }
+
+  // One phi-only iteration.  Every decision between the passes (the
+  // multipliers, V0 from the first iteration, whether and how far to
+  // project) is taken on the device (min_ctl_kernel), so the iteration waits
+  // on the host only inside the elastic solve and once at its end, for the
+  // scalars of the stopping rule.
   int minimize_iterate(pf_minimize_stats* st) override {
     if (!minimizing) return fail(PF_E_ARG, "pf_minimize_begin has not been called on this handle");
     if (int rs = require_state("pf_minimize_iterate")) return rs;
     state_ok = false;
     if (mopt.full_model) return minimize_iterate_full(st);
     const T invN = T(1.0 / (double)R);
-    const int rows = Nx * Ny;
     const bool el = P.elastic != 0;
     double* phi_old = phi[cur];
     double* phi_new = phi[1 - cur];
     const int gR = grid_for(R);
+    double* mc = dscal + MIN_SLOT;
     int rc;
     ++m_iter;
     last_iters = 0;
@@ -2318,7 +2338,7 @@ struct Engine final : EngineBase {
                        prm.dz, prm.kappa_phi, partials);
     finalize(gR, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 8, 1.0 / (double)R);
     hipLaunchKernelGGL(min_volume_kernel, dim3(gR), dim3(NT), 0, stream, phi_old, R, partials);
-    finalize(gR, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal + 48, 1.0);
+    finalize(gR, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal + MS_VOL, 1.0);
     if (el) {
       if ((rc = elastic_solve())) return rc;
       typename OpMinRhs<T, true>::Args oa{P, phi_old, m_gexp, m_eps, m_s00, (size_t)R, invN};
@@ -2332,17 +2352,10 @@ struct Engine final : EngineBase {
     // g_full(phi^n) and the under-relaxed multiplier (main_cuda.cu:29436-29500)
     min_forward_to_sc0();
     min_lap_to_sz1();
-    min_gfull(phi_old, 50);
-    if ((rc = fetch(58))) return rc;
-    const double grad_mean = hscal[8];
-    const double gel_sum = hscal[9], dw_sum = hscal[10], core_sum = hscal[11];
-    const double mean_h0 = hscal[48] / (double)R;
-    if (m_iter == 1 && m_V0 <= 0.0) m_V0 = mean_h0;
-    const double lam_raw = min_multiplier(hscal[50], hscal[51]);
-    const double eta = std::fmin(std::fmax(mopt.eta_lambda_vol, 0.0), 1.0);
-    m_lambda = (1.0 - eta) * m_lambda + eta * lam_raw;
+    min_gfull(phi_old, MS_RAW);
+    min_ctl(MCS_RAW);
     // phi^{n+1}
-    min_update(phi_old, m_lambda);
+    min_update(phi_old, 0.0, nullptr, nullptr, mc + MC_LAMBDA);
     int gz;
     if (el) {
       typename OpMinFinish<T, true, true>::Args oa{P, phi_old, phi_new, m_gexp, m_eps, m_s00, (size_t)R, invN};
@@ -2355,15 +2368,12 @@ struct Engine final : EngineBase {
     // KKT residual at phi^{n+1} with the exact multiplier (main_cuda.cu:29596-29728)
     min_forward_to_sc0();
     min_lap_to_sz1();
-    min_gfull(phi_new, 53);
-    if ((rc = fetch(58))) return rc;
-    const double rms_dphi = std::sqrt(hscal[52] / (double)R) / mopt.dt;
-    const double lam_kkt = min_multiplier(hscal[53], hscal[54]);
-    hipLaunchKernelGGL(min_kkt_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, m_gfull, lam_kkt,
-                         (const int*)nullptr, (const double*)nullptr, R, partials);
+    min_gfull(phi_new, MS_KKT);
+    min_ctl(MCS_KKT);
+    hipLaunchKernelGGL(min_kkt_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, m_gfull, 0.0, (const int*)nullptr,
+                       (const double*)nullptr, R, partials, (const double*)(mc + MC_LAM_KKT));
     finalize(gR, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 55, 1.0);
     // post-projection sub-steps, eta = 1, stress frozen (main_cuda.cu:32104-32197)
-    double lam_pp = lam_kkt;
     for (int k = 0; k < mopt.post_projection_iters; ++k) {
       if (k > 0) {
         if (el) {
@@ -2375,32 +2385,35 @@ struct Engine final : EngineBase {
         }
         min_forward_to_sc0();
         min_lap_to_sz1();
-        min_gfull(phi_new, 56);
-        if ((rc = fetch(58))) return rc;
-        lam_pp = min_multiplier(hscal[56], hscal[57]);
+        min_gfull(phi_new, MS_PP);
+        min_ctl(MCS_PP);
       }
-      m_lambda = lam_pp;
-      min_update(phi_new, lam_pp);
+      min_ctl(MCS_PP_COMMIT);
+      min_update(phi_new, 0.0, nullptr, nullptr, mc + MC_LAMBDA);
       typename OpMinFinish<T, false, false>::Args oa{P, phi_new, phi_new, m_gexp, m_eps, m_s00, (size_t)R, invN};
       launch_zpass<T, 1, 0, OpMinFinish<T, false, false>>(Nz, stream, zargs(0, 1, 0, 0), oa);
     }
     // first-order volume projection (main_cuda.cu:32274-32290)
     hipLaunchKernelGGL(min_volume_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, R, partials);
-    finalize(gR, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal + 48, 1.0);
-    if ((rc = fetch(58))) return rc;
+    finalize(gR, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal + MS_VOL, 1.0);
+    min_ctl(MCS_VOLUME);
+    hipLaunchKernelGGL(min_project_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, 0.0, R, partials,
+                       (const double*)mc);
+    finalize(gR, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + MS_PROJ, 1.0);
+    min_ctl(MCS_FINAL);
+    // the one wait of the iteration: the scalars of the stopping rule
+    PF_HIP(hipMemcpyAsync(hscal, dscal, 64 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    PF_HIP(hipMemcpyAsync(hscal + MIN_SLOT, mc, MC_COUNT * sizeof(double), hipMemcpyDeviceToHost, stream));
+    PF_HIP(hipStreamSynchronize(stream));
+    const double* hm = hscal + MIN_SLOT;
+    const double grad_mean = hscal[8];
+    const double gel_sum = hscal[9], dw_sum = hscal[10], core_sum = hscal[11];
+    const double rms_dphi = std::sqrt(hscal[52] / (double)R) / mopt.dt;
     const double rms_res = std::sqrt(hscal[55] / (double)R);
-    double mean_h = hscal[48] / (double)R;
-    if (m_V0 > 1e-30 && std::isfinite(mean_h)) {
-      const double vol_err = m_V0 - mean_h;
-      if (std::fabs(vol_err) > 1e-14) {
-        const double mean_hp2 = hscal[49] / (double)R;
-        const double lc = mean_hp2 > 1e-30 ? vol_err / mean_hp2 : 0.0;
-        hipLaunchKernelGGL(min_project_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, lc, R, partials);
-        finalize(gR, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 48, 1.0);
-        if ((rc = fetch(58))) return rc;
-        mean_h = hscal[48] / (double)R;
-      }
-    }
+    const double lam_raw = hm[MC_LAM_RAW], lam_kkt = hm[MC_LAM_KKT];
+    const double mean_h = hm[MC_MEAN_H];
+    m_lambda = hm[MC_LAMBDA];
+    m_V0 = hm[MC_V0];
     cur = 1 - cur;
     // convergence bookkeeping (main_cuda.cu:32400-32560); condB needs the
     // excess free energy, which phi-only minimize leaves undefined
@@ -2541,11 +2554,12 @@ struct Engine final : EngineBase {
         comp_multipliers(&kkt);
         if ((rc = comp_upload(comp_aux_dev(), kkt))) return rc;
         hipLaunchKernelGGL(min_kkt_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, m_gfull, 0.0,
-                           (const int*)m_labels, (const double*)comp_aux_dev(), R, partials);
+                           (const int*)m_labels, (const double*)comp_aux_dev(), R, partials,
+                           (const double*)nullptr);
       } else {
         lam_kkt = min_multiplier(hscal[53], hscal[54]);
         hipLaunchKernelGGL(min_kkt_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, m_gfull, lam_kkt,
-                           (const int*)nullptr, (const double*)nullptr, R, partials);
+                           (const int*)nullptr, (const double*)nullptr, R, partials, (const double*)nullptr);
       }
       finalize(gR, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 55, 1.0);
     }
@@ -2594,7 +2608,8 @@ struct Engine final : EngineBase {
       if (std::fabs(vol_err) > 1e-14) {
         const double mean_hp2 = hscal[49] / (double)R;
         const double lc = mean_hp2 > 1e-30 ? vol_err / mean_hp2 : 0.0;
-        hipLaunchKernelGGL(min_project_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, lc, R, partials);
+        hipLaunchKernelGGL(min_project_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, lc, R, partials,
+                           (const double*)nullptr);
         finalize(gR, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 48, 1.0);
         if ((rc = fetch(58))) return rc;
         mean_h = hscal[48] / (double)R;

@@ -125,6 +125,7 @@ struct OpMinUpd {
     double Ldt;
     const int* labels;       // per-label multipliers lambdas[labels[i]] when set
     const double* lambdas;   // (add_component_volume_constraint_kernel, cuda_kernels.cu:606-615)
+    const double* lambda_p;  // when set, the multiplier is read here (decided on the device)
   };
   static constexpr int NRED = 0;
   static constexpr int kinds[1] = {0};
@@ -132,7 +133,7 @@ struct OpMinUpd {
   __device__ static void load(const Args& a, size_t g, double* pre) {
     pre[0] = a.phi[g];
     pre[1] = a.gexp[g];
-    pre[2] = a.labels ? a.lambdas[a.labels[g]] : a.lambda;
+    pre[2] = a.labels ? a.lambdas[a.labels[g]] : (a.lambda_p ? *a.lambda_p : a.lambda);
   }
   __device__ static void apply(const Args& a, size_t, const double* pre, const T*, T* out, double*) {
     const double phi = pre[0];
@@ -388,7 +389,8 @@ namespace {
 //   red[0] sum res^2
 __global__ void __launch_bounds__(NT) min_kkt_kernel(const double* phi, const double* gfull, double lambda,
                                                      const int* labels, const double* lambdas, long n,
-                                                     double* partials) {
+                                                     double* partials, const double* lambda_p) {
+  if (lambda_p) lambda = *lambda_p;
   double s = 0.0;
   for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
     const double p = phi[i];
@@ -519,10 +521,103 @@ __global__ void __launch_bounds__(NT) copy_minmax_kernel(const double* a, double
 }
 
 // phi <- clamp01(phi + lc h'(phi)) (apply_volume_projection_kernel,
-// cuda_kernels.cu:639-647), then red[0] sum h(clamp01 phi)
-__global__ void __launch_bounds__(NT) min_project_kernel(double* phi, double lc, long n, double* partials) {
+// cuda_kernels.cu:639-647), then red[0] sum h(clamp01 phi).  With mc set,
+// lc and whether to project at all come from the minimize control block
+// (MC_LC, MC_PROJ); without a projection phi is left alone and the sum is 0.
+__global__ void __launch_bounds__(NT) min_project_kernel(double* phi, double lc, long n, double* partials,
+                                                         const double* mc);
+
+// ---------------------------------------------------------------- control
+// The phi-only iteration's scalar decisions, taken by one thread on the
+// device between the passes (the host reads the block once per iteration):
+// the under-relaxed and exact multipliers (main_cuda.cu:29468-29500,
+// 29596-29610, 32120-32140) and the first-order volume projection
+// (32274-32290).  Sums come from the scalar slots the reductions write.
+enum MinCtlSlot {
+  MC_LAMBDA = 0,   // lambda_vol, carried across iterations
+  MC_V0,           // target <h>, set from <h(phi^0)> at the first iteration when not given
+  MC_LAM_RAW,      // -sum h' g_full / sum h'^2 at phi^n
+  MC_LAM_KKT,      // the same at phi^{n+1}
+  MC_LAM_PP,       // the multiplier of the next post-projection sub-step
+  MC_LC,           // volume projection coefficient
+  MC_PROJ,         // 1 when the projection runs
+  MC_MEAN_H,       // <h> after the iteration
+  MC_COUNT
+};
+// scalar slots of the sums: 48/49 sum h, sum h'^2; 50/51, 53/54, 56/57
+// sum h' g_full, sum h'^2 at phi^n, phi^{n+1} and the post-projection phi;
+// 59 sum h after the projection
+enum MinSumSlot { MS_VOL = 48, MS_RAW = 50, MS_KKT = 53, MS_PP = 56, MS_PROJ = 59 };
+enum MinCtlStage { MCS_RAW = 0, MCS_KKT, MCS_PP, MCS_PP_COMMIT, MCS_VOLUME, MCS_FINAL, MCS_INIT };
+struct MinCtlConst {
+  double R;        // voxel count
+  double eta;      // under-relaxation of lambda, clamped to [0, 1]
+  double V0;       // MCS_INIT: the given target (<= 0: from the first iteration)
+  int first;       // MCS_RAW of the first iteration
+};
+
+PF_HD inline double min_multiplier_of(double s_hg, double s_h2) {
+  return fabs(s_h2) > 1e-30 ? -s_hg / s_h2 : 0.0;
+}
+
+__global__ void min_ctl_kernel(const double* s, double* mc, int stage, MinCtlConst c) {
+#pragma clang fp contract(off)
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  switch (stage) {
+    case MCS_INIT:
+      for (int i = 0; i < MC_COUNT; ++i) mc[i] = 0.0;
+      mc[MC_V0] = c.V0;
+      break;
+    case MCS_RAW: {
+      const double mean_h0 = s[MS_VOL] / c.R;
+      if (c.first && mc[MC_V0] <= 0.0) mc[MC_V0] = mean_h0;
+      const double lam_raw = min_multiplier_of(s[MS_RAW], s[MS_RAW + 1]);
+      mc[MC_LAM_RAW] = lam_raw;
+      mc[MC_LAMBDA] = (1.0 - c.eta) * mc[MC_LAMBDA] + c.eta * lam_raw;
+      break;
+    }
+    case MCS_KKT:
+      mc[MC_LAM_KKT] = min_multiplier_of(s[MS_KKT], s[MS_KKT + 1]);
+      mc[MC_LAM_PP] = mc[MC_LAM_KKT];
+      break;
+    case MCS_PP:
+      mc[MC_LAM_PP] = min_multiplier_of(s[MS_PP], s[MS_PP + 1]);
+      break;
+    case MCS_PP_COMMIT:
+      mc[MC_LAMBDA] = mc[MC_LAM_PP];
+      break;
+    case MCS_VOLUME: {
+      const double mean_h = s[MS_VOL] / c.R;
+      const double V0 = mc[MC_V0];
+      double proj = 0.0, lc = 0.0;
+      if (V0 > 1e-30 && isfinite(mean_h)) {
+        const double vol_err = V0 - mean_h;
+        if (fabs(vol_err) > 1e-14) {
+          const double mean_hp2 = s[MS_VOL + 1] / c.R;
+          lc = mean_hp2 > 1e-30 ? vol_err / mean_hp2 : 0.0;
+          proj = 1.0;
+        }
+      }
+      mc[MC_LC] = lc;
+      mc[MC_PROJ] = proj;
+      mc[MC_MEAN_H] = mean_h;
+      break;
+    }
+    case MCS_FINAL:
+      if (mc[MC_PROJ] != 0.0) mc[MC_MEAN_H] = s[MS_PROJ] / c.R;
+      break;
+  }
+}
+
+__global__ void __launch_bounds__(NT) min_project_kernel(double* phi, double lc, long n, double* partials,
+                                                         const double* mc) {
+  bool doit = true;
+  if (mc) {
+    doit = mc[MC_PROJ] != 0.0;
+    lc = mc[MC_LC];
+  }
   double s = 0.0;
-  for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
+  for (long i = (long)blockIdx.x * NT + threadIdx.x; doit && i < n; i += (long)gridDim.x * NT) {
     const double p = phi[i];
     const double q = clamp01(p + lc * h_prime_of_phi(p));
     phi[i] = q;
