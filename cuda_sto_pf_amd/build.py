@@ -17,8 +17,8 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpfcore.so")
 CLI = os.path.join(HERE, "bin", "main_pf")
 
-HIP_SOURCES = ["pf_engine_f64.hip", "pf_engine_f32.hip", "pf_runtime.hip", "pf_nn.hip"]
-CXX_SOURCES = ["pf_params.cpp", "pf_batch.cpp", "pf_checkpoint.cpp"]
+HIP_SOURCES = ["pf_engine_f64.hip", "pf_engine_f32.hip", "pf_runtime.hip", "pf_nn.hip", "pf_batch.hip"]
+CXX_SOURCES = ["pf_params.cpp", "pf_checkpoint.cpp"]
 CLI_SOURCES = ["main_pf.cpp"]
 PASS_GROUPS = 7   # z-pass operator groups 0-4 and 6, the column passes (5), pf_pass_inst.hpp
 

@@ -116,8 +116,7 @@ namespace {
 // ~10^5 partials, so load latency, not bandwidth, sets its duration).
 constexpr int FIN_U = 8;
 
-__global__ void __launch_bounds__(NT) finalize_kernel(const double* partials, int nblocks, int nred,
-                                                      RedKinds kinds, double* out, double scale) {
+__device__ __forceinline__ void finalize_kernel_body(const double* partials, int nblocks, int nred, RedKinds kinds, double* out, double scale, const B3 blockIdx, const B3 gridDim) {
   double v[4];
 #pragma unroll
   for (int r = 0; r < 4; ++r) v[r] = red_identity(kinds.k[r]);
@@ -145,6 +144,45 @@ __global__ void __launch_bounds__(NT) finalize_kernel(const double* partials, in
   if (threadIdx.x == 0)
     for (int r = 0; r < nred; ++r) out[r] = kinds.k[r] == RED_SUM ? outv[r] * scale : outv[r];
 }
+__global__ void __launch_bounds__(NT) finalize_kernel(const double* partials, int nblocks, int nred, RedKinds kinds, double* out, double scale) {
+  finalize_kernel_body(partials, nblocks, nred, kinds, out, scale, PF_BIDX, PF_GDIM);
+}
+struct KFinalize {
+  struct Args {
+    const double* partials;
+    int nblocks;
+    int nred;
+    RedKinds kinds;
+    double* out;
+    double scale;
+  };
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3 b, B3 g) { finalize_kernel_body(a.partials, a.nblocks, a.nred, a.kinds, a.out, a.scale, b, g); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL(finalize_kernel, dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.partials, a.nblocks, a.nred, a.kinds, a.out, a.scale);
+  }
+};
+
+
+// dst[0..n) = src[0..n): the device scalars of a handle into a batch
+// runner's gather buffer
+__global__ void __launch_bounds__(NT) gather_kernel(const double* src, double* dst, int n) {
+  for (int i = threadIdx.x; i < n; i += NT) dst[i] = src[i];
+}
+struct KGather {
+  struct Args {
+    const double* src;
+    double* dst;
+    int n;
+  };
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3, B3) {
+    for (int i = threadIdx.x; i < a.n; i += NT) a.dst[i] = a.src[i];
+  }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL(gather_kernel, dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.src, a.dst, a.n);
+  }
+};
 
 // Up to 16 sums of block partials in a fixed order, into the workgroup's
 // shared outv[16] (valid after the call for every thread).
@@ -209,9 +247,7 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 // spacing of the workloads and within one rounding of the reference's
 // division otherwise.
 constexpr int GE_U = 4;
-__global__ void __launch_bounds__(NT) grad_energy_kernel(const double* phi, int Nx, int Ny, int Nz,
-                                                         double dx, double dy, double dz,
-                                                         double kappa, double* partials) {
+__device__ __forceinline__ void grad_energy_kernel_body(const double* phi, int Nx, int Ny, int Nz, double dx, double dy, double dz, double kappa, double* partials, const B3 blockIdx, const B3 gridDim) {
   double acc[1] = {0.0};
   const int rows = Nx * Ny;
   const int L = xcd_block(blockIdx.x, gridDim.x);
@@ -253,6 +289,28 @@ __global__ void __launch_bounds__(NT) grad_energy_kernel(const double* phi, int 
   int kinds[1] = {RED_SUM};
   block_reduce_store<1>(acc, kinds, partials + blockIdx.x);
 }
+__global__ void __launch_bounds__(NT) grad_energy_kernel(const double* phi, int Nx, int Ny, int Nz, double dx, double dy, double dz, double kappa, double* partials) {
+  grad_energy_kernel_body(phi, Nx, Ny, Nz, dx, dy, dz, kappa, partials, PF_BIDX, PF_GDIM);
+}
+struct KGradEnergy {
+  struct Args {
+    const double* phi;
+    int Nx;
+    int Ny;
+    int Nz;
+    double dx;
+    double dy;
+    double dz;
+    double kappa;
+    double* partials;
+  };
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3 b, B3 g) { grad_energy_kernel_body(a.phi, a.Nx, a.Ny, a.Nz, a.dx, a.dy, a.dz, a.kappa, a.partials, b, g); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL(grad_energy_kernel, dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.phi, a.Nx, a.Ny, a.Nz, a.dx, a.dy, a.dz, a.kappa, a.partials);
+  }
+};
+
 
 // Zero-mode evaluation (compute_pf_Y_zero_mode_terms_kernel, cuda_kernels.cu:3409-3420):
 // mass(lambda) = sum (1-h) sigma(Y*+lambda) + h v_B, its derivative, and the
@@ -466,13 +524,27 @@ struct AaConst {
   double tol, floor;
 };
 
-__global__ void aa_init_kernel(double* ctl) {
+__device__ __forceinline__ void aa_init_kernel_body(double* ctl, const B3 blockIdx, const B3 gridDim) {
   for (int i = threadIdx.x; i < AC_COUNT; i += blockDim.x) ctl[i] = 0.0;
   if (threadIdx.x == 0) {
     ctl[AC_ACTIVE] = 1.0;
     ctl[AC_RES] = NAN;
   }
 }
+__global__ void __launch_bounds__(64) aa_init_kernel(double* ctl) {
+  aa_init_kernel_body(ctl, PF_BIDX, PF_GDIM);
+}
+struct KAaInit {
+  struct Args {
+    double* ctl;
+  };
+  static constexpr int BLOCK = 64;
+  __device__ static void body(const Args& a, B3 b, B3 g) { aa_init_kernel_body(a.ctl, b, g); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL(aa_init_kernel, dim3(g.x, g.y, g.z), dim3(64), 0, s, a.ctl);
+  }
+};
+
 
 // mixing coefficients: min |g - sum gamma_j dg_j| by the normal equations
 // with a 1e-12 relative Tikhonov shift, Gaussian elimination with partial
@@ -586,8 +658,7 @@ __device__ void aa_ctrl(double* ctl, int mode, AaConst c) {
 // End of an iteration, one workgroup: the Green update's sums into the
 // control block, the residual / history / stopping decision, and the next
 // iteration's mixing coefficients (unused when the solve has stopped).
-__global__ void __launch_bounds__(NT) aa_end_kernel(const double* partials, int nblocks, double* ctl, int mode,
-                                                    AaConst c) {
+__device__ __forceinline__ void aa_end_kernel_body(const double* partials, int nblocks, double* ctl, int mode, AaConst c, const B3 blockIdx, const B3 gridDim) {
   if (ctl[AC_ACTIVE] == 0.0) return;
   __shared__ double outv[16];
   __shared__ double sc[AC_COUNT];   // the control block, worked on in LDS by one thread
@@ -601,6 +672,24 @@ __global__ void __launch_bounds__(NT) aa_end_kernel(const double* partials, int 
   __syncthreads();
   for (int i = threadIdx.x; i < AC_COUNT; i += NT) ctl[i] = sc[i];
 }
+__global__ void __launch_bounds__(NT) aa_end_kernel(const double* partials, int nblocks, double* ctl, int mode, AaConst c) {
+  aa_end_kernel_body(partials, nblocks, ctl, mode, c, PF_BIDX, PF_GDIM);
+}
+struct KAaEnd {
+  struct Args {
+    const double* partials;
+    int nblocks;
+    double* ctl;
+    int mode;
+    AaConst c;
+  };
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3 b, B3 g) { aa_end_kernel_body(a.partials, a.nblocks, a.ctl, a.mode, a.c, b, g); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL(aa_end_kernel, dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.partials, a.nblocks, a.ctl, a.mode, a.c);
+  }
+};
+
 
 // Y = Y* + lambda, x_B = sigma(Y) unclipped (apply_pf_Y_zero_mode_shift_kernel
 // 3434-3443), dY/dt = (Y - Y^n)/dt (update_dY_dt_prev_kernel 3545-3556) when
@@ -858,7 +947,7 @@ int launch_zpass(int N, hipStream_t s, const ZPassArgs<T>& za, const typename Op
         break;                                                                                      \
       }                                                                                             \
     }                                                                                               \
-    hipLaunchKernelGGL((zpass_kernel<T, v, NCI, NCO, Op>), dim3(grid), dim3(NT), 0, s, za, oa);   \
+    launch<KZPass<T, v, NCI, NCO, Op>>(s, dim3(grid), {za, oa});                                  \
     break;
     PF_SIZES(PF_CASE_Z)
 #undef PF_CASE_Z
@@ -875,7 +964,7 @@ void launch_cpass(int N, hipStream_t s, const CPassArgs<T>& ca, int planes, int 
   case v: {                                                                                     \
     constexpr int TC = cpass_cols<T, v>();                                                      \
     dim3 grid((ca.cols + TC - 1) / TC, planes, comps);                                          \
-    hipLaunchKernelGGL((cpass_kernel<T, v, INV>), grid, dim3(NT), 0, s, ca);                    \
+    launch<KCPass<T, v, INV>>(s, grid, ca);                                                    \
     break;                                                                                      \
   }
     PF_SIZES(PF_CASE_C)
@@ -1272,8 +1361,7 @@ struct Engine final : EngineBase {
   }
 
   void finalize(int nblocks, int nred, RedKinds k, double* out, double scale) {
-    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(NT), 0, stream, partials, nblocks, nred, k, out,
-                       scale);
+    launch<KFinalize>(stream, dim3(1), {partials, nblocks, nred, k, out, scale});
   }
 
   int fetch(int count) {
@@ -1361,11 +1449,37 @@ struct Engine final : EngineBase {
   // main_cuda.cu:28887-29046): warm start by polynomial extrapolation of the
   // last converged displacements, FP64 accumulation of U and g, increments in
   // T through the linear part of the map, and Anderson mixing of depth m.
+  // Accelerated fixed point (same map F, residual |F(U)-U|/|F(U)| and test as
+  // main_cuda.cu:28887-29046): warm start by polynomial extrapolation of the
+  // last converged displacements, FP64 accumulation of U and g, increments in
+  // T through the linear part of the map, and Anderson mixing of depth m.
+  // The iteration's decisions (mixing coefficients, history bookkeeping,
+  // stopping rule) are taken on the device in the control block; the host
+  // enqueues as many iterations as the previous solve took and waits once,
+  // enqueuing more only if the solve is still running.  The solve is split
+  // into accel_begin (start and the first iterations), accel_iters and
+  // accel_finish so that a batch runner can drive many handles' solves
+  // through the same waits.
+  int acc_k = 0, acc_kmax = 0;   // iterations enqueued in the running solve
   int elastic_solve_accel() {
-    const int m = prm.elastic_anderson_depth;
+    int rc = accel_begin();
+    if (rc) return rc;
+    for (;;) {
+      PF_HIP(hipMemcpyAsync(hscal + AA_SLOT, dscal + AA_SLOT, AC_GAM * sizeof(double), hipMemcpyDeviceToHost,
+                            stream));
+      PF_HIP(hipStreamSynchronize(stream));
+      if (!accel_more()) break;
+      accel_iters(2);
+    }
+    return accel_finish();
+  }
+  // the start of the solve (extrapolated warm start or cold start) and its
+  // first aa_spec iterations
+  int accel_begin() {
+    if (tl_rec && cold_hooks() && !(warm_valid && hist_count > 0))
+      return fail(PF_E_ARG, "a recorded solve cannot run a cold-start hook");
     const KDims d = kd();
     const int gk = grid_for(nSc);
-    const T invN = T(1.0 / (double)R);
     if (warm_valid && hist_count > 0) {
       const int order = std::min(prm.elastic_warm_start_order, hist_count);
       const double coef[3][3] = {{1.0, 0.0, 0.0}, {2.0, -1.0, 0.0}, {3.0, -3.0, 1.0}};
@@ -1386,7 +1500,7 @@ struct Engine final : EngineBase {
       for (int k = 0; k < order; ++k) sa.cf[k] = coef[order - 1][k];
       sa.nh = order;
       for (int i = 0; i < 6; ++i) sa.eps[i] = sc(i);
-      hipLaunchKernelGGL((kop_strain<T, double>), dim3(gmodes()), dim3(NT), 0, stream, sa);
+      launch<KStrain<T, double>>(stream, dim3(gmodes()), {sa});
     } else {
       typename OpElasticCold<T>::Args oa{P, phi[cur], xB_el()};
       launch_zpass<T, 0, 6, OpElasticCold<T>>(Nz, stream, zargs(0, 0, 0, 6), oa);
@@ -1403,90 +1517,96 @@ struct Engine final : EngineBase {
       ga.partials = nullptr;
       for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
       ga.mode = 0;
-      hipLaunchKernelGGL(kop_green_acc<T>, dim3(gk), dim3(NT), 0, stream, ga);
+      launch<KGreenAcc<T>>(stream, dim3(gk), {ga});
       if (cold_hooks()) {
         int rc = cold_hook();
         if (rc) return rc;
         StrainArgs<T, double> sa{};
-      sa.ustate = 1;
+        sa.ustate = 1;
         sa.d = d;
         sa.n = nSc;
         for (int i = 0; i < 3; ++i) sa.u[i] = Ud + (size_t)i * nScA;
         for (int i = 0; i < 6; ++i) sa.eps[i] = sc(i);
-        hipLaunchKernelGGL((kop_strain<T, double>), dim3(gmodes()), dim3(NT), 0, stream, sa);
+        launch<KStrain<T, double>>(stream, dim3(gmodes()), {sa});
       }
     }
-    // The iteration's decisions (mixing coefficients, history bookkeeping,
-    // stopping rule) are taken on the device in the control block; the host
-    // enqueues as many iterations as the previous solve took and waits once,
-    // enqueuing more only if the solve is still running.
+    double* ctl = dscal + AA_SLOT;
+    launch<KAaInit>(stream, dim3(1), {ctl});
+    acc_kmax = std::max(prm.elastic_iter_max, 1);
+    acc_k = 0;
+    gate_ = ctl + AC_ACTIVE;
+    accel_iters(std::max(aa_spec, 1));
+    return PF_OK;
+  }
+  // true while the solve runs and may take more iterations (control block
+  // mirrored into hscal)
+  bool accel_more() const { return hscal[AA_SLOT + AC_ACTIVE] != 0.0 && acc_k < acc_kmax; }
+  // enqueue up to `batch` more iterations (each does nothing once the solve has stopped)
+  void accel_iters(int batch) {
+    const int m = prm.elastic_anderson_depth;
+    const KDims d = kd();
+    const int gk = grid_for(nSc);
+    const T invN = T(1.0 / (double)R);
     double* ctl = dscal + AA_SLOT;
     const AaConst ac{m, prm.elastic_iter_min, prm.elastic_iter_max, prm.elastic_residual_tolerance,
                      prm.elastic_residual_absolute_floor};
-    hipLaunchKernelGGL(aa_init_kernel, dim3(1), dim3(64), 0, stream, ctl);
-    const int kmax = std::max(prm.elastic_iter_max, 1);
-    int k = 0;
-    int batch = std::max(aa_spec, 1);
-    gate_ = ctl + AC_ACTIVE;
-    for (;;) {
-      for (int b = 0; b < batch && k < kmax; ++b, ++k) {
-        const int mode = (k == 0) ? 0 : 1;
-        if (mode == 1) {
-          // mixing coefficients: from the previous iteration's aa_end_kernel
-          AAUpdateArgs<T> ua{};
-          ua.d = d;
-          ua.n = nSc;
-          for (int i = 0; i < 3; ++i) {
-            ua.g[i] = Gd + (size_t)i * nScA;
-            ua.U[i] = Ud + (size_t)i * nScA;
-            ua.dinc[i] = dinc + (size_t)i * nScA;
-            for (int j = 0; j < AA_MAX; ++j) {
-              ua.du[j][i] = du_slot(j, i);
-              ua.dg[j][i] = dg_slot(j, i);
-            }
-          }
-          for (int i = 0; i < 6; ++i) ua.eps[i] = sc(i);
-          ua.ctl = ctl;
-          ua.m = m;
-          hipLaunchKernelGGL(kop_aa_update<T>, dim3(gmodes()), dim3(NT), 0, stream, ua);
-        }
-        xpass(true, 0, 6, Sc);
-        ypass(true, 0, 6);
-        if (mode == 0) {
-          typename OpHij<T>::Args oa{P, phi[cur], xB_el(), invN, 0, hcache};
-          launch_zpass<T, 6, 6, OpHij<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
-        } else {
-          typename OpHijInc<T>::Args oa{P, hcache, invN};
-          launch_zpass<T, 6, 6, OpHijInc<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
-        }
-        ypass(false, 0, 6);
-        xpass(false, 0, 6, Sc);
-        GreenAAArgs<T> ga{};
-        ga.d = d;
-        ga.n = nSc;
-        for (int i = 0; i < 6; ++i) ga.h[i] = sc(i);
+    for (int b = 0; b < batch && acc_k < acc_kmax; ++b, ++acc_k) {
+      const int mode = (acc_k == 0) ? 0 : 1;
+      if (mode == 1) {
+        // mixing coefficients: from the previous iteration's aa_end_kernel
+        AAUpdateArgs<T> ua{};
+        ua.d = d;
+        ua.n = nSc;
         for (int i = 0; i < 3; ++i) {
-          ga.U[i] = Ud + (size_t)i * nScA;
-          ga.g[i] = Gd + (size_t)i * nScA;
-          ga.dinc[i] = dinc + (size_t)i * nScA;
+          ua.g[i] = Gd + (size_t)i * nScA;
+          ua.U[i] = Ud + (size_t)i * nScA;
+          ua.dinc[i] = dinc + (size_t)i * nScA;
           for (int j = 0; j < AA_MAX; ++j) {
-            ga.dg[j][i] = dg_slot(j, i);
-            ga.du[j][i] = du_slot(j, i);
+            ua.du[j][i] = du_slot(j, i);
+            ua.dg[j][i] = dg_slot(j, i);
           }
         }
-        ga.mode = mode;
-        ga.partials = partials;
-        for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
-        ga.ctl = ctl;
-        ga.m = m;
-        hipLaunchKernelGGL(kop_green_aa<T>, dim3(gk), dim3(NT), 0, stream, ga);
-        hipLaunchKernelGGL(aa_end_kernel, dim3(1), dim3(NT), 0, stream, partials, gk, ctl, mode, ac);
+        for (int i = 0; i < 6; ++i) ua.eps[i] = sc(i);
+        ua.ctl = ctl;
+        ua.m = m;
+        launch<KAaUpdate<T>>(stream, dim3(gmodes()), {ua});
       }
-      PF_HIP(hipMemcpyAsync(hscal + AA_SLOT, ctl, AC_GAM * sizeof(double), hipMemcpyDeviceToHost, stream));
-      PF_HIP(hipStreamSynchronize(stream));
-      if (hscal[AA_SLOT + AC_ACTIVE] == 0.0 || k >= kmax) break;
-      batch = 2;
+      xpass(true, 0, 6, Sc);
+      ypass(true, 0, 6);
+      if (mode == 0) {
+        typename OpHij<T>::Args oa{P, phi[cur], xB_el(), invN, 0, hcache};
+        launch_zpass<T, 6, 6, OpHij<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
+      } else {
+        typename OpHijInc<T>::Args oa{P, hcache, invN};
+        launch_zpass<T, 6, 6, OpHijInc<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
+      }
+      ypass(false, 0, 6);
+      xpass(false, 0, 6, Sc);
+      GreenAAArgs<T> ga{};
+      ga.d = d;
+      ga.n = nSc;
+      for (int i = 0; i < 6; ++i) ga.h[i] = sc(i);
+      for (int i = 0; i < 3; ++i) {
+        ga.U[i] = Ud + (size_t)i * nScA;
+        ga.g[i] = Gd + (size_t)i * nScA;
+        ga.dinc[i] = dinc + (size_t)i * nScA;
+        for (int j = 0; j < AA_MAX; ++j) {
+          ga.dg[j][i] = dg_slot(j, i);
+          ga.du[j][i] = du_slot(j, i);
+        }
+      }
+      ga.mode = mode;
+      ga.partials = partials;
+      for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
+      ga.ctl = ctl;
+      ga.m = m;
+      launch<KGreenAA<T>>(stream, dim3(gk), {ga});
+      launch<KAaEnd>(stream, dim3(1), {partials, gk, ctl, mode, ac});
     }
+  }
+  // the accepted result: host checks, U <- U + g and its strain, history rotation
+  int accel_finish() {
+    const KDims d = kd();
     gate_ = nullptr;
     const double* hc = hscal + AA_SLOT;
     const int it = (int)hc[AC_IT];
@@ -1497,8 +1617,8 @@ struct Engine final : EngineBase {
       last_iters = it;
       return fail(PF_E_NONFINITE, "elastic residual is non-finite at step " + std::to_string(steps_done + 1));
     }
-    hipLaunchKernelGGL(kop_aa_final<T>, dim3(gmodes()), dim3(NT), 0, stream, d, nSc, Ud, Ud + nScA,
-                       Ud + 2 * nScA, Gd, Gd + nScA, Gd + 2 * nScA, sc(0), sc(1), sc(2), sc(3), sc(4), sc(5));
+    launch<KAaFinal<T>>(stream, dim3(gmodes()), {d, nSc, Ud, Ud + nScA, Ud + 2 * nScA, Gd, Gd + nScA,
+                                                  Gd + 2 * nScA, sc(0), sc(1), sc(2), sc(3), sc(4), sc(5)});
     xpass(true, 0, 6, Sc);
     ypass(true, 0, 6);
     std::swap(hbuf[hist_head], Ud);
@@ -2282,7 +2402,7 @@ struct Engine final : EngineBase {
   // Lap^ of sc(0) -> inverse x, y passes into Sz(1)
   void min_lap_to_sz1() {
     const KDims d = kd();
-    hipLaunchKernelGGL(kop_lap<T>, dim3(grid_for(nSc)), dim3(NT), 0, stream, d, nSc, (const C*)sc(0), sc(1));
+    launch<KLap<T>>(stream, dim3(grid_for(nSc)), {d, nSc, (const C*)sc(0), sc(1)});
     xpass(true, 1, 1, sc(1));
     ypass(true, 1, 1);
   }
@@ -2300,8 +2420,7 @@ struct Engine final : EngineBase {
     typename OpMinUpd<T>::Args oa{ph, m_gexp, lambda, Ldt, labels, lambdas, lambda_p};
     launch_zpass<T, 0, 1, OpMinUpd<T>>(Nz, stream, zargs(0, 0, 0, 1), oa);
     min_forward_to_sc0();
-    hipLaunchKernelGGL(kop_phi<T>, dim3(grid_for(nSc)), dim3(NT), 0, stream, kd(), nSc, sc(0),
-                       Ldt * prm.kappa_phi);
+    launch<KPhi<T>>(stream, dim3(grid_for(nSc)), {kd(), nSc, sc(0), Ldt * prm.kappa_phi});
     xpass(true, 0, 1, Sc);
     ypass(true, 0, 1);
   }
@@ -2310,8 +2429,7 @@ struct Engine final : EngineBase {
   // the phi-only iteration's control decisions (min_ctl_kernel)
   void min_ctl(int stage) {
     const MinCtlConst c{(double)R, std::fmin(std::fmax(mopt.eta_lambda_vol, 0.0), 1.0), m_V0, m_iter == 1 ? 1 : 0};
-    hipLaunchKernelGGL(min_ctl_kernel, dim3(1), dim3(64), 0, stream, (const double*)dscal, dscal + MIN_SLOT, stage,
-                       c);
+    launch<KMinCtl>(stream, dim3(1), {(const double*)dscal, dscal + MIN_SLOT, stage, c});
   }
 
   // One phi-only iteration.  Every decision between the passes (the
@@ -2319,28 +2437,70 @@ struct Engine final : EngineBase {
   // project) is taken on the device (min_ctl_kernel), so the iteration waits
   // on the host only inside the elastic solve and once at its end, for the
   // scalars of the stopping rule.
+  // One phi-only iteration.  Every decision between the passes (the
+  // multipliers, V0 from the first iteration, whether and how far to
+  // project) is taken on the device (min_ctl_kernel), so the iteration waits
+  // on the host only inside the elastic solve and once at its end, for the
+  // scalars of the stopping rule.  It runs in three phases so that a batch
+  // runner can drive many handles through the same waits:
+  //   min_phase_a  energies and <h> of phi^n, the start of the elastic solve;
+  //   (the solve's remaining iterations: accel_more / accel_iters)
+  //   min_phase_b  the solve's result and everything up to the final scalars;
+  //   min_phase_c  the stopping rule and the statistics, from the scalars.
   int minimize_iterate(pf_minimize_stats* st) override {
     if (!minimizing) return fail(PF_E_ARG, "pf_minimize_begin has not been called on this handle");
     if (int rs = require_state("pf_minimize_iterate")) return rs;
     state_ok = false;
     if (mopt.full_model) return minimize_iterate_full(st);
+    int rc = min_phase_a();
+    if (rc) return rc;
+    if (P.elastic && extended) {
+      for (;;) {
+        PF_HIP(hipMemcpyAsync(hscal + AA_SLOT, dscal + AA_SLOT, AC_GAM * sizeof(double), hipMemcpyDeviceToHost,
+                              stream));
+        PF_HIP(hipStreamSynchronize(stream));
+        if (!accel_more()) break;
+        accel_iters(2);
+      }
+    }
+    if ((rc = min_phase_b())) return rc;
+    PF_HIP(hipMemcpyAsync(hscal, dscal, 64 * sizeof(double), hipMemcpyDeviceToHost, stream));
+    PF_HIP(hipMemcpyAsync(hscal + MIN_SLOT, dscal + MIN_SLOT, MC_COUNT * sizeof(double), hipMemcpyDeviceToHost,
+                          stream));
+    PF_HIP(hipStreamSynchronize(stream));
+    return min_phase_c(st);
+  }
+
+  int min_phase_a() {
+    const double* phi_old = phi[cur];
+    const int gR = grid_for(R);
+    ++m_iter;
+    last_iters = 0;
+    // energies and <h> of phi^n (main_cuda.cu:29264-29346)
+    launch<KGradEnergy>(stream, dim3(gR), {phi_old, Nx, Ny, Nz, prm.dx, prm.dy, prm.dz, prm.kappa_phi, partials});
+    finalize(gR, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 8, 1.0 / (double)R);
+    launch<KMinVolume>(stream, dim3(gR), {phi_old, R, partials});
+    finalize(gR, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal + MS_VOL, 1.0);
+    if (P.elastic) {
+      // the accelerated solve is driven by the caller; the others run here
+      const int rc = extended ? accel_begin() : elastic_solve();
+      if (rc) return rc;
+    }
+    return PF_OK;
+  }
+
+  int min_phase_b() {
     const T invN = T(1.0 / (double)R);
     const bool el = P.elastic != 0;
     double* phi_old = phi[cur];
     double* phi_new = phi[1 - cur];
     const int gR = grid_for(R);
     double* mc = dscal + MIN_SLOT;
-    int rc;
-    ++m_iter;
-    last_iters = 0;
-    // energies and <h> of phi^n (main_cuda.cu:29264-29346)
-    hipLaunchKernelGGL(grad_energy_kernel, dim3(gR), dim3(NT), 0, stream, phi_old, Nx, Ny, Nz, prm.dx, prm.dy,
-                       prm.dz, prm.kappa_phi, partials);
-    finalize(gR, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 8, 1.0 / (double)R);
-    hipLaunchKernelGGL(min_volume_kernel, dim3(gR), dim3(NT), 0, stream, phi_old, R, partials);
-    finalize(gR, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal + MS_VOL, 1.0);
+    if (el && extended) {
+      const int rc = accel_finish();
+      if (rc) return rc;
+    }
     if (el) {
-      if ((rc = elastic_solve())) return rc;
       typename OpMinRhs<T, true>::Args oa{P, phi_old, m_gexp, m_eps, m_s00, (size_t)R, invN};
       const int gz = launch_zpass<T, 6, 1, OpMinRhs<T, true>>(Nz, stream, zargs(0, 6, 0, 1), oa);
       finalize(gz, 3, RedKinds{{RED_SUM, RED_SUM, RED_SUM, 0}}, dscal + 9, 1.0);
@@ -2370,8 +2530,7 @@ struct Engine final : EngineBase {
     min_lap_to_sz1();
     min_gfull(phi_new, MS_KKT);
     min_ctl(MCS_KKT);
-    hipLaunchKernelGGL(min_kkt_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, m_gfull, 0.0, (const int*)nullptr,
-                       (const double*)nullptr, R, partials, (const double*)(mc + MC_LAM_KKT));
+    launch<KMinKkt>(stream, dim3(gR), {phi_new, m_gfull, 0.0, (const int*)nullptr, (const double*)nullptr, R, partials, (const double*)(mc + MC_LAM_KKT)});
     finalize(gR, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + 55, 1.0);
     // post-projection sub-steps, eta = 1, stress frozen (main_cuda.cu:32104-32197)
     for (int k = 0; k < mopt.post_projection_iters; ++k) {
@@ -2394,17 +2553,17 @@ struct Engine final : EngineBase {
       launch_zpass<T, 1, 0, OpMinFinish<T, false, false>>(Nz, stream, zargs(0, 1, 0, 0), oa);
     }
     // first-order volume projection (main_cuda.cu:32274-32290)
-    hipLaunchKernelGGL(min_volume_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, R, partials);
+    launch<KMinVolume>(stream, dim3(gR), {phi_new, R, partials});
     finalize(gR, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal + MS_VOL, 1.0);
     min_ctl(MCS_VOLUME);
-    hipLaunchKernelGGL(min_project_kernel, dim3(gR), dim3(NT), 0, stream, phi_new, 0.0, R, partials,
-                       (const double*)mc);
+    launch<KMinProject>(stream, dim3(gR), {phi_new, 0.0, R, partials, (const double*)mc});
     finalize(gR, 1, RedKinds{{RED_SUM, 0, 0, 0}}, dscal + MS_PROJ, 1.0);
     min_ctl(MCS_FINAL);
-    // the one wait of the iteration: the scalars of the stopping rule
-    PF_HIP(hipMemcpyAsync(hscal, dscal, 64 * sizeof(double), hipMemcpyDeviceToHost, stream));
-    PF_HIP(hipMemcpyAsync(hscal + MIN_SLOT, mc, MC_COUNT * sizeof(double), hipMemcpyDeviceToHost, stream));
-    PF_HIP(hipStreamSynchronize(stream));
+    return PF_OK;
+  }
+
+  int min_phase_c(pf_minimize_stats* st) {
+    const bool el = P.elastic != 0;
     const double* hm = hscal + MIN_SLOT;
     const double grad_mean = hscal[8];
     const double gel_sum = hscal[9], dw_sum = hscal[10], core_sum = hscal[11];
@@ -3183,6 +3342,40 @@ struct Engine final : EngineBase {
     PF_HIP(hipStreamSynchronize(stream));
     return PF_OK;
   }
+
+  // ---------------------------------------------------------------- fused batch
+  // The phases of a phi-only minimize iteration with the accelerated solver,
+  // recorded by the fused batch runner (pf_batch_minimize_fused).  Between
+  // phases the runner mirrors every handle's device scalars into its host
+  // mirror (mb_gather / mb_scatter) with one copy for all handles.
+  int mb_check() override {
+    if (!minimizing) return fail(PF_E_ARG, "pf_minimize_begin has not been called on this handle");
+    if (mopt.full_model) return fail(PF_E_ARG, "fused batches run phi-only minimize");
+    if (P.elastic && !extended) return fail(PF_E_ARG, "fused batches need the accelerated elastic solver");
+    if (cold_hooks()) return fail(PF_E_ARG, "fused batches run no cold-start hooks");
+    if (zpass_use_reg()) return fail(PF_E_ARG, "fused batches use the LDS z pass");
+    return PF_OK;
+  }
+  int mb_spec() override { return std::max(aa_spec, 1); }
+  int mb_phase_a(int spec) override {
+    if (int rs = require_state("pf_minimize_iterate")) return rs;
+    state_ok = false;
+    aa_spec = std::max(spec, 1);
+    return min_phase_a();
+  }
+  int mb_solve_more(int* more) override {
+    *more = 0;
+    if (!(P.elastic && extended) || !accel_more()) return PF_OK;
+    accel_iters(2);
+    *more = 1;
+    return PF_OK;
+  }
+  int mb_phase_b() override { return min_phase_b(); }
+  int mb_phase_c(pf_minimize_stats* st) override { return min_phase_c(st); }
+  int mb_converged() override { return m_converged ? 1 : 0; }
+  int mb_nscal() override { return NSCAL; }
+  void mb_gather(double* dst) override { launch<KGather>(stream, dim3(1), {dscal, dst, NSCAL}); }
+  void mb_scatter(const double* src) override { std::memcpy(hscal, src, NSCAL * sizeof(double)); }
 
   // One launch of kernel `id` with the current buffers; returns its bytes.
   double launch_one(int id) {

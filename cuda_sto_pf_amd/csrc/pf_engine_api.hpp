@@ -38,6 +38,19 @@ struct EngineBase {
                          const double* trs, double* out) = 0;
   virtual int checkpoint_write(const char* path, const pf_checkpoint_provenance* prov) = 0;
   virtual int checkpoint_read(const char* path, const pf_checkpoint_provenance* prov) = 0;
+  // fused batch runner (pf_batch_minimize_fused): the phases of a phi-only
+  // minimize iteration, recorded for many handles and replayed as batched
+  // launches; the runner mirrors the device scalars between phases
+  virtual int mb_check() = 0;                  // PF_OK when the handle can run recorded iterations
+  virtual int mb_spec() = 0;                   // solver iterations the next solve enqueues first
+  virtual int mb_phase_a(int spec) = 0;        // energies, <h>, the solve's start and `spec` iterations
+  virtual int mb_solve_more(int* more) = 0;    // two more iterations while the solve runs (*more = 1)
+  virtual int mb_phase_b() = 0;                // the solve's result through the final scalars
+  virtual int mb_phase_c(pf_minimize_stats* st) = 0;   // stopping rule and statistics (host)
+  virtual int mb_converged() = 0;
+  virtual int mb_nscal() = 0;                  // doubles of device scalars
+  virtual void mb_gather(double* dst) = 0;     // enqueue device scalars -> dst (device)
+  virtual void mb_scatter(const double* src) = 0;   // host mirror <- src
 };
 
 std::unique_ptr<EngineBase> make_engine_f64();

@@ -387,9 +387,7 @@ namespace {
 // res = g_full + lambda h'(phi), zeroed where it only points out of [0,1]
 // (compute_projected_phi_kkt_residual_sq_kernel, cuda_kernels.cu:684-706);
 //   red[0] sum res^2
-__global__ void __launch_bounds__(NT) min_kkt_kernel(const double* phi, const double* gfull, double lambda,
-                                                     const int* labels, const double* lambdas, long n,
-                                                     double* partials, const double* lambda_p) {
+__device__ __forceinline__ void min_kkt_kernel_body(const double* phi, const double* gfull, double lambda, const int* labels, const double* lambdas, long n, double* partials, const double* lambda_p, const B3 blockIdx, const B3 gridDim) {
   if (lambda_p) lambda = *lambda_p;
   double s = 0.0;
   for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
@@ -401,10 +399,31 @@ __global__ void __launch_bounds__(NT) min_kkt_kernel(const double* phi, const do
   const int kinds[1] = {RED_SUM};
   block_reduce_store<1>(&s, kinds, partials + blockIdx.x);
 }
+__global__ void __launch_bounds__(NT) min_kkt_kernel(const double* phi, const double* gfull, double lambda, const int* labels, const double* lambdas, long n, double* partials, const double* lambda_p) {
+  min_kkt_kernel_body(phi, gfull, lambda, labels, lambdas, n, partials, lambda_p, PF_BIDX, PF_GDIM);
+}
+struct KMinKkt {
+  struct Args {
+    const double* phi;
+    const double* gfull;
+    double lambda;
+    const int* labels;
+    const double* lambdas;
+    long n;
+    double* partials;
+    const double* lambda_p;
+  };
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3 b, B3 g) { min_kkt_kernel_body(a.phi, a.gfull, a.lambda, a.labels, a.lambdas, a.n, a.partials, a.lambda_p, b, g); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL(min_kkt_kernel, dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.phi, a.gfull, a.lambda, a.labels, a.lambdas, a.n, a.partials, a.lambda_p);
+  }
+};
+
 
 // red[0] sum h(clamp01 phi) (reduce_sum_h_kernel, cuda_kernels.cu:5580-5607),
 // red[1] sum h'(phi)^2
-__global__ void __launch_bounds__(NT) min_volume_kernel(const double* phi, long n, double* partials) {
+__device__ __forceinline__ void min_volume_kernel_body(const double* phi, long n, double* partials, const B3 blockIdx, const B3 gridDim) {
   double v[2] = {0.0, 0.0};
   for (long i = (long)blockIdx.x * NT + threadIdx.x; i < n; i += (long)gridDim.x * NT) {
     const double p = phi[i];
@@ -415,6 +434,22 @@ __global__ void __launch_bounds__(NT) min_volume_kernel(const double* phi, long 
   const int kinds[2] = {RED_SUM, RED_SUM};
   block_reduce_store<2>(v, kinds, partials + 2 * (size_t)blockIdx.x);
 }
+__global__ void __launch_bounds__(NT) min_volume_kernel(const double* phi, long n, double* partials) {
+  min_volume_kernel_body(phi, n, partials, PF_BIDX, PF_GDIM);
+}
+struct KMinVolume {
+  struct Args {
+    const double* phi;
+    long n;
+    double* partials;
+  };
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3 b, B3 g) { min_volume_kernel_body(a.phi, a.n, a.partials, b, g); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL(min_volume_kernel, dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.phi, a.n, a.partials);
+  }
+};
+
 
 // red[0] sum (a + shift), red[1] sum (a + shift)^2
 __global__ void __launch_bounds__(NT) sum_sq_kernel(const double* a, double shift, long n, double* partials) {
@@ -560,7 +595,7 @@ PF_HD inline double min_multiplier_of(double s_hg, double s_h2) {
   return fabs(s_h2) > 1e-30 ? -s_hg / s_h2 : 0.0;
 }
 
-__global__ void min_ctl_kernel(const double* s, double* mc, int stage, MinCtlConst c) {
+__device__ __forceinline__ void min_ctl_kernel_body(const double* s, double* mc, int stage, MinCtlConst c, const B3 blockIdx, const B3 gridDim) {
 #pragma clang fp contract(off)
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   switch (stage) {
@@ -608,9 +643,25 @@ __global__ void min_ctl_kernel(const double* s, double* mc, int stage, MinCtlCon
       break;
   }
 }
+__global__ void __launch_bounds__(64) min_ctl_kernel(const double* s, double* mc, int stage, MinCtlConst c) {
+  min_ctl_kernel_body(s, mc, stage, c, PF_BIDX, PF_GDIM);
+}
+struct KMinCtl {
+  struct Args {
+    const double* s;
+    double* mc;
+    int stage;
+    MinCtlConst c;
+  };
+  static constexpr int BLOCK = 64;
+  __device__ static void body(const Args& a, B3 b, B3 g) { min_ctl_kernel_body(a.s, a.mc, a.stage, a.c, b, g); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL(min_ctl_kernel, dim3(g.x, g.y, g.z), dim3(64), 0, s, a.s, a.mc, a.stage, a.c);
+  }
+};
 
-__global__ void __launch_bounds__(NT) min_project_kernel(double* phi, double lc, long n, double* partials,
-                                                         const double* mc) {
+
+__device__ __forceinline__ void min_project_kernel_body(double* phi, double lc, long n, double* partials, const double* mc, const B3 blockIdx, const B3 gridDim) {
   bool doit = true;
   if (mc) {
     doit = mc[MC_PROJ] != 0.0;
@@ -626,13 +677,31 @@ __global__ void __launch_bounds__(NT) min_project_kernel(double* phi, double lc,
   const int kinds[1] = {RED_SUM};
   block_reduce_store<1>(&s, kinds, partials + blockIdx.x);
 }
+__global__ void __launch_bounds__(NT) min_project_kernel(double* phi, double lc, long n, double* partials, const double* mc) {
+  min_project_kernel_body(phi, lc, n, partials, mc, PF_BIDX, PF_GDIM);
+}
+struct KMinProject {
+  struct Args {
+    double* phi;
+    double lc;
+    long n;
+    double* partials;
+    const double* mc;
+  };
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3 b, B3 g) { min_project_kernel_body(a.phi, a.lc, a.n, a.partials, a.mc, b, g); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL(min_project_kernel, dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.phi, a.lc, a.n, a.partials, a.mc);
+  }
+};
+
 
 }  // namespace
 
 // Lap^ = -k^2 phi^ on the compact spectrum (compute_laplacian_k_kernel,
 // cuda_kernels.cu:3528-3538)
 template <typename T>
-__global__ void __launch_bounds__(NT) kop_lap(KDims d, long n, const cx<T>* in, cx<T>* out) {
+__device__ __forceinline__ void kop_lap_body(KDims d, long n, const cx<T>* in, cx<T>* out, const B3 blockIdx, const B3 gridDim) {
   const long nact_ = (n / d.Az) * d.Azr;   // retained modes only (no kz padding)
   for (long j_ = (long)blockIdx.x * NT + threadIdx.x; j_ < nact_; j_ += (long)gridDim.x * NT) {
     const long i = kslot(d, j_);
@@ -643,5 +712,24 @@ __global__ void __launch_bounds__(NT) kop_lap(KDims d, long n, const cx<T>* in, 
     out[i] = cx<T>{T(-k2 * double(in[i].re)), T(-k2 * double(in[i].im))};
   }
 }
+template <typename T>
+__global__ void __launch_bounds__(NT) kop_lap(KDims d, long n, const cx<T>* in, cx<T>* out) {
+  kop_lap_body<T>(d, n, in, out, PF_BIDX, PF_GDIM);
+}
+template <typename T>
+struct KLap {
+  struct Args {
+    KDims d;
+    long n;
+    const cx<T>* in;
+    cx<T>* out;
+  };
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3 b, B3 g) { kop_lap_body<T>(a.d, a.n, a.in, a.out, b, g); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL((kop_lap<T>), dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.d, a.n, a.in, a.out);
+  }
+};
+
 
 }  // namespace pf

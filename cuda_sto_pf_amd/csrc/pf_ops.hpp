@@ -651,7 +651,7 @@ struct GreenAccArgs {
 };
 
 template <typename T>
-__global__ void __launch_bounds__(NT) kop_green_acc(GreenAccArgs<T> ga) {
+__device__ __forceinline__ void kop_green_acc_body(const GreenAccArgs<T>& ga, const B3 blockIdx, const B3 gridDim) {
   double acc[2] = {0.0, 0.0};
   const long nact_ = (ga.n / ga.d.Az) * ga.d.Azr;   // retained modes only (no kz padding)
   for (long j_ = blockIdx.x * (long)NT + threadIdx.x; j_ < nact_; j_ += (long)gridDim.x * NT) {
@@ -710,6 +710,22 @@ __global__ void __launch_bounds__(NT) kop_green_acc(GreenAccArgs<T> ga) {
     block_reduce_store<2>(acc, kinds, ga.partials + (size_t)blockIdx.x * 2);
   }
 }
+template <typename T>
+__global__ void __launch_bounds__(NT) kop_green_acc(GreenAccArgs<T> ga) {
+  kop_green_acc_body<T>(ga, PF_BIDX, PF_GDIM);
+}
+template <typename T>
+struct KGreenAcc {
+  struct Args {
+    GreenAccArgs<T> ga;
+  };
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3 b, B3 g) { kop_green_acc_body<T>(a.ga, b, g); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL((kop_green_acc<T>), dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.ga);
+  }
+};
+
 
 // ---------------------------------------------------------------- accelerated solver
 // Anderson-mixed fixed point on the FP64 displacement U with residual g = F(U) - U.
@@ -782,7 +798,7 @@ __device__ __forceinline__ void green_apply_point(const double* C, double kx, do
 }
 
 template <typename T>
-__global__ void __launch_bounds__(NT) kop_green_aa(GreenAAArgs<T> a) {
+__device__ __forceinline__ void kop_green_aa_body(const GreenAAArgs<T>& a, const B3 blockIdx, const B3 gridDim) {
   int nhist = a.nhist, slot = a.slot;
   if (a.ctl) {
     if (a.ctl[AC_ACTIVE] == 0.0) return;
@@ -891,6 +907,22 @@ __global__ void __launch_bounds__(NT) kop_green_aa(GreenAAArgs<T> a) {
   for (int r = 0; r < AA_NRED; ++r) kinds[r] = RED_SUM;
   block_reduce_store<AA_NRED>(acc, kinds, a.partials + (size_t)blockIdx.x * AA_NRED);
 }
+template <typename T>
+__global__ void __launch_bounds__(NT) kop_green_aa(GreenAAArgs<T> a) {
+  kop_green_aa_body<T>(a, PF_BIDX, PF_GDIM);
+}
+template <typename T>
+struct KGreenAA {
+  struct Args {
+    GreenAAArgs<T> a;
+  };
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3 b, B3 g) { kop_green_aa_body<T>(a.a, b, g); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL((kop_green_aa<T>), dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.a);
+  }
+};
+
 
 // d = g - sum_j gamma_j (du_j + dg_j);  U += d (the T-rounded d);  du[slot] = d;
 // eps = strain(d) for the next linear-map application.
@@ -915,7 +947,7 @@ struct AAUpdateArgs {
 };
 
 template <typename T>
-__global__ void __launch_bounds__(NT) kop_aa_update(AAUpdateArgs<T> a) {
+__device__ __forceinline__ void kop_aa_update_body(const AAUpdateArgs<T>& a, const B3 blockIdx, const B3 gridDim) {
   int nhist = a.nhist, slot = a.slot;
   double gamma[AA_MAX];
 #pragma unroll
@@ -967,14 +999,26 @@ __global__ void __launch_bounds__(NT) kop_aa_update(AAUpdateArgs<T> a) {
     strain_of<T>(kx, ky, kz, dd, a.eps, c);
   }
 }
+template <typename T>
+__global__ void __launch_bounds__(NT) kop_aa_update(AAUpdateArgs<T> a) {
+  kop_aa_update_body<T>(a, PF_BIDX, PF_GDIM);
+}
+template <typename T>
+struct KAaUpdate {
+  struct Args {
+    AAUpdateArgs<T> a;
+  };
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3 b, B3 g) { kop_aa_update_body<T>(a.a, b, g); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL((kop_aa_update<T>), dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.a);
+  }
+};
+
 
 // U <- U + g (the accepted result F(U)), eps = strain(U)
 template <typename T>
-__global__ void __launch_bounds__(NT) kop_aa_final(KDims d, long n, cx<double>* U0, cx<double>* U1,
-                                                   cx<double>* U2, const cx<double>* g0,
-                                                   const cx<double>* g1, const cx<double>* g2,
-                                                   cx<T>* e0, cx<T>* e1, cx<T>* e2, cx<T>* e3, cx<T>* e4,
-                                                   cx<T>* e5) {
+__device__ __forceinline__ void kop_aa_final_body(KDims d, long n, cx<double>* U0, cx<double>* U1, cx<double>* U2, const cx<double>* g0, const cx<double>* g1, const cx<double>* g2, cx<T>* e0, cx<T>* e1, cx<T>* e2, cx<T>* e3, cx<T>* e4, cx<T>* e5, const B3 blockIdx, const B3 gridDim) {
   cx<double>* U[3] = {U0, U1, U2};
   const cx<double>* g[3] = {g0, g1, g2};
   cx<T>* eps[6] = {e0, e1, e2, e3, e4, e5};
@@ -994,6 +1038,35 @@ __global__ void __launch_bounds__(NT) kop_aa_final(KDims d, long n, cx<double>* 
     strain_of<T>(kx, ky, kz, u, eps, c);
   }
 }
+template <typename T>
+__global__ void __launch_bounds__(NT) kop_aa_final(KDims d, long n, cx<double>* U0, cx<double>* U1, cx<double>* U2, const cx<double>* g0, const cx<double>* g1, const cx<double>* g2, cx<T>* e0, cx<T>* e1, cx<T>* e2, cx<T>* e3, cx<T>* e4, cx<T>* e5) {
+  kop_aa_final_body<T>(d, n, U0, U1, U2, g0, g1, g2, e0, e1, e2, e3, e4, e5, PF_BIDX, PF_GDIM);
+}
+template <typename T>
+struct KAaFinal {
+  struct Args {
+    KDims d;
+    long n;
+    cx<double>* U0;
+    cx<double>* U1;
+    cx<double>* U2;
+    const cx<double>* g0;
+    const cx<double>* g1;
+    const cx<double>* g2;
+    cx<T>* e0;
+    cx<T>* e1;
+    cx<T>* e2;
+    cx<T>* e3;
+    cx<T>* e4;
+    cx<T>* e5;
+  };
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3 b, B3 g) { kop_aa_final_body<T>(a.d, a.n, a.U0, a.U1, a.U2, a.g0, a.g1, a.g2, a.e0, a.e1, a.e2, a.e3, a.e4, a.e5, b, g); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL((kop_aa_final<T>), dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.d, a.n, a.U0, a.U1, a.U2, a.g0, a.g1, a.g2, a.e0, a.e1, a.e2, a.e3, a.e4, a.e5);
+  }
+};
+
 
 template <typename T, typename UT = T>
 struct StrainArgs {
@@ -1010,7 +1083,7 @@ struct StrainArgs {
   int ustate;   // u is solver state (retained-mode index), not a k-space array
 };
 template <typename T, typename UT = T>
-__global__ void __launch_bounds__(NT) kop_strain(StrainArgs<T, UT> sa) {
+__device__ __forceinline__ void kop_strain_body(const StrainArgs<T, UT>& sa, const B3 blockIdx, const B3 gridDim) {
   const long nact_ = (sa.n / sa.d.Az) * sa.d.Azr;   // retained modes only (no kz padding)
   for (long j_ = blockIdx.x * (long)NT + threadIdx.x; j_ < nact_; j_ += (long)gridDim.x * NT) {
     const long c = kslot(sa.d, j_);
@@ -1046,6 +1119,22 @@ __global__ void __launch_bounds__(NT) kop_strain(StrainArgs<T, UT> sa) {
     strain_of<T>(kx, ky, kz, u, sa.eps, c);
   }
 }
+template <typename T, typename UT = T>
+__global__ void __launch_bounds__(NT) kop_strain(StrainArgs<T, UT> sa) {
+  kop_strain_body<T, UT>(sa, PF_BIDX, PF_GDIM);
+}
+template <typename T, typename UT = T>
+struct KStrain {
+  struct Args {
+    StrainArgs<T, UT> sa;
+  };
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3 b, B3 g) { kop_strain_body<T, UT>(a.sa, b, g); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL((kop_strain<T, UT>), dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.sa);
+  }
+};
+
 
 // Mixed-precision phi update with the FP64 spectrum Phi of the unclamped,
 // band-limited phi*: mode 1 (first update after new fields) phi^ = g^ / den
@@ -1082,7 +1171,7 @@ __global__ void __launch_bounds__(NT) kop_phi_state(KDims d, long n, cx<T>* g, d
 
 // phi^ <- g^ / (1 + L dt kappa k^2)
 template <typename T>
-__global__ void __launch_bounds__(NT) kop_phi(KDims d, long n, cx<T>* g, double Ldt_kappa) {
+__device__ __forceinline__ void kop_phi_body(KDims d, long n, cx<T>* g, double Ldt_kappa, const B3 blockIdx, const B3 gridDim) {
   const long nact_ = (n / d.Az) * d.Azr;   // retained modes only (no kz padding)
   for (long j_ = blockIdx.x * (long)NT + threadIdx.x; j_ < nact_; j_ += (long)gridDim.x * NT) {
     const long c = kslot(d, j_);
@@ -1095,6 +1184,25 @@ __global__ void __launch_bounds__(NT) kop_phi(KDims d, long n, cx<T>* g, double 
     g[c] = cx<T>{T(double(v.re) / den), T(double(v.im) / den)};
   }
 }
+template <typename T>
+__global__ void __launch_bounds__(NT) kop_phi(KDims d, long n, cx<T>* g, double Ldt_kappa) {
+  kop_phi_body<T>(d, n, g, Ldt_kappa, PF_BIDX, PF_GDIM);
+}
+template <typename T>
+struct KPhi {
+  struct Args {
+    KDims d;
+    long n;
+    cx<T>* g;
+    double Ldt_kappa;
+  };
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3 b, B3 g) { kop_phi_body<T>(a.d, a.n, a.g, a.Ldt_kappa, b, g); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL((kop_phi<T>), dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.d, a.n, a.g, a.Ldt_kappa);
+  }
+};
+
 
 // grad_a mu = i k_a mu^ (compute_gradient_single_component_k_kernel 1887-1911)
 template <typename T>

@@ -18,6 +18,7 @@
 // Compact index c of a wrapped axis maps to the full index c (c < a) or
 // c + N - A (c >= a), A = 2a - 1.
 #pragma once
+#include "pf_launch.hpp"
 #include <type_traits>
 
 #include "pf_fft.hpp"
@@ -128,7 +129,7 @@ struct ZBudget<Op, std::void_t<decltype(Op::LDS_BUDGET)>> {
 };
 
 template <typename T, int N, int NCI, int NCO, class Op>
-__global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op::Args oa) {
+__device__ __forceinline__ void zpass_body(const ZPassArgs<T>& za, const typename Op::Args& oa, const B3 blockIdx) {
   if (za.gate && *za.gate == 0.0) return;
   constexpr int NCM = NCI > NCO ? NCI : NCO;
   constexpr int NCB = NCM > 0 ? NCM : 1;
@@ -324,6 +325,24 @@ __global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op:
   }
 }
 
+template <typename T, int N, int NCI, int NCO, class Op>
+__global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op::Args oa) {
+  zpass_body<T, N, NCI, NCO, Op>(za, oa, PF_BIDX);
+}
+// launch descriptor (pf_launch.hpp)
+template <typename T, int N, int NCI, int NCO, class Op>
+struct KZPass {
+  struct Args {
+    ZPassArgs<T> za;
+    typename Op::Args oa;
+  };
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3 bid, B3) { zpass_body<T, N, NCI, NCO, Op>(a.za, a.oa, bid); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL((zpass_kernel<T, N, NCI, NCO, Op>), dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.za, a.oa);
+  }
+};
+
 // --------------------------------------------------------------- column pass
 // Strided complex FFT along y (planes = Nx) or x (planes = 1) with pruning.
 // Forward: N full inputs -> A retained outputs.  Inverse: A -> N.
@@ -340,7 +359,7 @@ struct CPassArgs {
 };
 
 template <typename T, int N, bool INV>
-__global__ void __launch_bounds__(NT) cpass_kernel(CPassArgs<T> ca) {
+__device__ __forceinline__ void cpass_body(const CPassArgs<T>& ca, const B3 blockIdx) {
   if (ca.gate && *ca.gate == 0.0) return;
   constexpr int TC = cpass_cols<T, N>();
   constexpr bool BIG = plan_needs_alt(N);
@@ -412,5 +431,19 @@ __global__ void __launch_bounds__(NT) cpass_kernel(CPassArgs<T> ca) {
     }
   }
 }
+
+template <typename T, int N, bool INV>
+__global__ void __launch_bounds__(NT) cpass_kernel(CPassArgs<T> ca) {
+  cpass_body<T, N, INV>(ca, PF_BIDX);
+}
+template <typename T, int N, bool INV>
+struct KCPass {
+  using Args = CPassArgs<T>;
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3 bid, B3) { cpass_body<T, N, INV>(a, bid); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL((cpass_kernel<T, N, INV>), dim3(g.x, g.y, g.z), dim3(NT), 0, s, a);
+  }
+};
 
 }  // namespace pf

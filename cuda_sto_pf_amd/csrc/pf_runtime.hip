@@ -8,6 +8,7 @@
 #include "../../include/pfcore.h"
 #include "pf_engine_api.hpp"
 #include "pf_error.hpp"
+#include "pf_handle.hpp"
 
 namespace pf {
 
@@ -124,10 +125,6 @@ PhysParams make_phys(const pf_params& p) {
 
 }  // namespace pf
 
-struct pf_handle {
-  std::unique_ptr<pf::EngineBase> eng;
-  int precision = PF_FP64;
-};
 
 extern "C" {
 

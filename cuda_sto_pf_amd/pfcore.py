@@ -210,6 +210,8 @@ EXPORTS = {
     "pf_batch_create": (C.c_int, [C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_void_p)]),
     "pf_batch_step": (C.c_int, [C.c_void_p, C.c_int]),
     "pf_batch_minimize": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pf_batch_minimize_fused": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pf_batch_fused_launches": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "pf_batch_destroy": (C.c_int, [C.c_void_p]),
     "pf_destroy": (C.c_int, [C.c_void_p]),
     "pf_fft_forward": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
@@ -535,6 +537,24 @@ class Batch:
         st = (PfMinimizeStats * n)()
         _check(lib().pf_batch_minimize(self._h, arr, st))
         return [s.as_dict() for s in st]
+
+    def minimize_fused(self, opts):
+        """pf_batch_minimize_fused: all instances' iterations together, each
+        stage one batched launch over the instances (phi-only minimize with
+        the accelerated elastic solver)."""
+        n = len(self.sims)
+        if isinstance(opts, PfMinimizeOpts):
+            opts = [opts] * n
+        arr = (PfMinimizeOpts * n)(*opts)
+        st = (PfMinimizeStats * n)()
+        _check(lib().pf_batch_minimize_fused(self._h, arr, st))
+        return [s.as_dict() for s in st]
+
+    def fused_launches(self):
+        """(batched launches issued, launches recorded) of the last fused call"""
+        a, r = C.c_longlong(), C.c_longlong()
+        _check(lib().pf_batch_fused_launches(self._h, C.byref(a), C.byref(r)))
+        return a.value, r.value
 
 
 def minimize_opts(**kw):

@@ -4,9 +4,11 @@
 128^3), each one sphere or oblate spheroid with a Haar-rotated eigenstrain
 (configs.sweep_instances), run with --minimize-dt 0.05, --minimize-max-iter
 500, --V0 0.1 (the minimize branch of main_cuda.cu:28341-32600).  They run
-once one at a time and once as a batch (pf_batch_minimize: one stream per
-instance, host workers overlapping the instances on the GPU); the final phi
-of every instance is compared between the two and the throughput reported.
+once one at a time, once as a batch of streams (pf_batch_minimize: one
+stream per instance, host workers overlapping the instances on the GPU) and
+once fused (pf_batch_minimize_fused: every stage of an iteration one batched
+launch over the instances); the final phi of every instance is compared with
+the solo run and the throughput reported.
 
     python -m cuda_sto_pf_amd.sweep --threads 8 --report sweep.json
 """
@@ -35,13 +37,22 @@ def make_sims(instances, prec, accel=True, net=None):
     return sims
 
 
-def run(instances, threads, opts, prec, accel=True, net=None):
-    """Wall time, per-instance stats and final phi of one sweep."""
+def run(instances, threads, opts, prec, accel=True, net=None, fused=False, launches=None):
+    """Wall time, per-instance stats and final phi of one sweep: one instance
+    at a time (threads <= 1), on one stream per instance with `threads` host
+    workers, or fused (every stage one batched launch over the instances);
+    `launches` (a dict) receives the fused run's launch counts."""
     sims = make_sims(instances, prec, accel, net)
     for s in sims:
         s.sync()
     t0 = time.perf_counter()
-    if threads <= 1:
+    if fused:
+        b = pf.Batch(sims, 1)
+        stats = b.minimize_fused(opts)
+        if launches is not None:
+            launches["batched"], launches["recorded"] = b.fused_launches()
+        b.close()
+    elif threads <= 1:
         stats = [s.minimize(opts) for s in sims]
     else:
         b = pf.Batch(sims, threads)
@@ -88,10 +99,23 @@ def main(argv=None):
     same_iters = all(x["iteration"] == y["iteration"] for x, y in zip(st_seq, st_bat))
     conv = sum(s["converged"] for s in st_bat)
     log(f"[sweep] max|phi batch - phi alone| = {dmax:.1e}, same iteration counts: {same_iters}, converged {conv}")
+    fused = {}
+    if not a.learned:
+        lc = {}
+        t_fus, st_fus, phi_fus = run(inst, 1, opts, prec, fused=True, launches=lc)
+        it_fus = sum(s["iteration"] for s in st_fus)
+        dmax_f = max(float(np.abs(x - y).max()) for x, y in zip(phi_seq, phi_fus))
+        same_f = all(x["iteration"] == y["iteration"] for x, y in zip(st_seq, st_fus))
+        log(f"[sweep] fused: {t_fus:.2f} s, {it_fus / t_fus:.0f} iterations/s, {lc['batched']} batched launches "
+            f"for {lc['recorded']} recorded, max|phi fused - phi alone| = {dmax_f:.1e}, same iteration counts: "
+            f"{same_f}")
+        fused = dict(fused_s=t_fus, speedup_fused=t_seq / t_fus, iterations_per_s_fused=it_fus / t_fus,
+                     fused_launches=lc["batched"], recorded_launches=lc["recorded"], max_abs_dphi_fused=dmax_f,
+                     same_iteration_counts_fused=same_f)
     rep = dict(instances=len(inst), precision=a.precision, learned=a.learned, max_iter=a.max_iter, dt=a.dt, V0=a.V0,
                threads=a.threads, one_at_a_time_s=t_seq, batch_s=t_bat, speedup=t_seq / t_bat,
                iterations=it_bat, iterations_per_s_batch=it_bat / t_bat, iterations_per_s_alone=it_seq / t_seq,
-               converged=conv, max_abs_dphi=dmax, same_iteration_counts=same_iters,
+               converged=conv, max_abs_dphi=dmax, same_iteration_counts=same_iters, **fused,
                per_instance=[dict(name=i["name"], radius=i["radius"], aspect=i["aspect"], iterations=s["iteration"],
                                   converged=s["converged"], mean_h=s["mean_h"], rms_res=s["rms_res"])
                              for i, s in zip(inst, st_bat)])

@@ -408,6 +408,20 @@ int pf_batch_create(int n, pf_handle* const* handles, int threads, pf_batch** ou
 int pf_batch_step(pf_batch* b, int nsteps);
 /* pf_minimize on every instance with opts[i]; stats (may be NULL) gets n entries */
 int pf_batch_minimize(pf_batch* b, const pf_minimize_opts* opts, pf_minimize_stats* stats);
+/* pf_minimize on every instance as one fused batch: the instances take their
+ * iterations together, and each stage of an iteration (a transform pass, a
+ * k-space update, a reduction, a controller) is one batched launch over all
+ * instances that share the kernel (one per grid size for the transform
+ * passes), driven from the calling thread on one stream with one wait per
+ * phase for all instances.  For phi-only minimize with the accelerated
+ * elastic solver (elastic_anderson_depth > 0) and no cold-start hook;
+ * otherwise PF_E_ARG.  Results equal pf_minimize's on each handle bit for
+ * bit.  No reference counterpart: the reference runs a sweep (e.g. 48 minimize
+ * runs of 64^3 to 128^3 grids) as one process per instance. */
+int pf_batch_minimize_fused(pf_batch* b, const pf_minimize_opts* opts, pf_minimize_stats* stats);
+/* launches of the last fused call: batched launches issued, and the launches
+ * the instances recorded (the count one-by-one execution issues) */
+int pf_batch_fused_launches(const pf_batch* b, long long* batched, long long* recorded);
 int pf_batch_destroy(pf_batch* b);
 
 /* ---------------------------------------------------------------------------

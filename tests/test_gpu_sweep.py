@@ -38,3 +38,35 @@ def test_batch_step_equals_solo_steps(pf):
         s.step(3)
     for s, t in zip(sims, solo):
         assert np.array_equal(s.field(pf.PF_XB), t.field(pf.PF_XB))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,iters", [("mixed", 15), ("fp64", 4)])
+def test_fused_batch_minimize_equals_solo_runs(pf, prec, iters):
+    """The fused batch (each stage one batched launch over the instances of a
+    grid size) reproduces every instance's solo run bit for bit, with fewer
+    launches than the instances record."""
+    inst = sweep_instances()
+    pick = [inst[0], inst[8], inst[9], inst[24], inst[25], inst[47]]   # 128^3, 96^3, 64^3
+    p = pf.PF_MIXED if prec == "mixed" else pf.PF_FP64
+    opts = pf.minimize_opts(dt=0.05, V0=0.1, max_iter=iters)
+    t1, s1, p1 = sweep.run(pick, 1, opts, p)
+    lc = {}
+    t2, s2, p2 = sweep.run(pick, 1, opts, p, fused=True, launches=lc)
+    print(f"{prec}: solo {t1:.2f} s, fused {t2:.2f} s, {lc['batched']} batched launches for "
+          f"{lc['recorded']} recorded", flush=True)
+    for a, b, x, y in zip(s1, s2, p1, p2):
+        assert a["iteration"] == b["iteration"] == iters
+        assert a["lambda_vol"] == b["lambda_vol"] and a["rms_res"] == b["rms_res"]
+        assert np.array_equal(x, y)
+    assert lc["batched"] * 2 < lc["recorded"]
+
+
+@pytest.mark.gpu
+def test_fused_batch_refuses_what_it_cannot_record(pf):
+    inst = sweep_instances()[47:]
+    sims = sweep.make_sims(inst, pf.PF_MIXED, accel=False)
+    b = pf.Batch(sims, 1)
+    with pytest.raises(pf.PfError, match="accelerated elastic solver"):
+        b.minimize_fused(pf.minimize_opts(dt=0.05, V0=0.1, max_iter=2))
+    b.close()
