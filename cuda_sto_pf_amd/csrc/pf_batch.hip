@@ -110,6 +110,7 @@ int fused_minimize(pf_batch* b, const pf_minimize_opts* opts, pf_minimize_stats*
   PF_FHIP(hipStreamCreate(&f.stream));
   PF_FHIP(hipMalloc((void**)&f.gdev, (size_t)n * ns * sizeof(double)));
   PF_FHIP(hipHostMalloc((void**)&f.ghost, (size_t)n * ns * sizeof(double), hipHostMallocDefault));
+  f.rep.use_graphs = b->graphs != 0;
   std::vector<pf::Recorder> rec(n);
   std::vector<pf_minimize_stats> last(n);
   std::vector<int> iters(n, 0);
@@ -189,6 +190,8 @@ int fused_minimize(pf_batch* b, const pf_minimize_opts* opts, pf_minimize_stats*
       ++iters[i];
     }
   }
+  b->fused_graphs_built = (long long)f.rep.graphs_built;
+  b->fused_graph_launches = (long long)f.rep.graph_launches;
   if (stats)
     for (int i = 0; i < n; ++i) stats[i] = last[i];
   return PF_OK;
@@ -230,10 +233,19 @@ int pf_batch_minimize_fused(pf_batch* b, const pf_minimize_opts* opts, pf_minimi
   return fused_minimize(b, opts, stats);
 }
 
-int pf_batch_fused_launches(const pf_batch* b, long long* batched, long long* recorded) {
+int pf_batch_fused_launches(const pf_batch* b, long long* batched, long long* recorded, long long* graphs_built,
+                            long long* graph_launches) {
   if (!b) return pf::fail(PF_E_ARG, "pf_batch_fused_launches: null batch");
   if (batched) *batched = b->fused_launches;
   if (recorded) *recorded = b->fused_recorded;
+  if (graphs_built) *graphs_built = b->fused_graphs_built;
+  if (graph_launches) *graph_launches = b->fused_graph_launches;
+  return PF_OK;
+}
+
+int pf_batch_set_graphs(pf_batch* b, int enable) {
+  if (!b || (enable != 0 && enable != 1)) return pf::fail(PF_E_ARG, "pf_batch_set_graphs: bad argument");
+  b->graphs = enable;
   return PF_OK;
 }
 

@@ -18,4 +18,8 @@ struct pf_batch {
   // launches of the last fused call: batched launches issued, and the
   // launches the instances recorded (what one-by-one execution would issue)
   long long fused_launches = 0, fused_recorded = 0;
+  // graphs captured and graph launches of the last fused call; whether fused
+  // calls capture their phases into graphs (pf_batch_set_graphs)
+  long long fused_graphs_built = 0, fused_graph_launches = 0;
+  int graphs = 1;
 };

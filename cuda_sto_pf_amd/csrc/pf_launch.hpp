@@ -133,15 +133,33 @@ inline void launch(hipStream_t s, dim3 g, const typename K::Args& a) {
 // launches on one stream.  All recorders must hold the same number of
 // launches; stage k of every instance is grouped by kernel.  Tables go
 // through one pinned staging buffer and one host-to-device copy per call.
+//
+// With graphs on, the launches of a call are captured into a HIP graph the
+// first time their plan (kernels, instance counts, block counts, table
+// offsets) occurs, and later calls with the same plan copy their tables and
+// launch that graph: the tables sit at the same device addresses, so the
+// captured launches read the new ones.
 struct BatchReplayer {
   char* host = nullptr;   // pinned
   char* dev = nullptr;
   size_t cap = 0;
   hipEvent_t copied = nullptr;   // the last staging copy (the staging is reused after it)
   bool pending = false;
-  unsigned long long launches = 0, stages = 0;
+  unsigned long long launches = 0, stages = 0, graph_launches = 0, graphs_built = 0;
+  bool use_graphs = false;
+  struct Cached {
+    std::vector<uint64_t> plan;
+    hipGraphExec_t exec;
+  };
+  std::vector<Cached> graphs;   // at most GRAPH_CACHE plans
+  static constexpr size_t GRAPH_CACHE = 64;
+  void drop_graphs() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+  }
   ~BatchReplayer() {
     if (pending) (void)hipEventSynchronize(copied);
+    drop_graphs();
     if (copied) (void)hipEventDestroy(copied);
     if (host) (void)hipHostFree(host);
     if (dev) (void)hipFree(dev);
@@ -150,6 +168,9 @@ struct BatchReplayer {
     if (n <= cap) return 0;
     size_t c = cap ? cap : (1u << 20);
     while (c < n) c *= 2;
+    if (pending) (void)hipEventSynchronize(copied);
+    pending = false;
+    drop_graphs();   // captured launches point into the old tables
     if (host) (void)hipHostFree(host);
     if (dev) (void)hipFree(dev);
     host = dev = nullptr;
@@ -220,13 +241,50 @@ struct BatchReplayer {
     if (hipMemcpyAsync(dev, host, total, hipMemcpyHostToDevice, s) != hipSuccess) return -1;
     if (hipEventRecord(copied, s) != hipSuccess) return -1;
     pending = true;
+    auto issue = [&]() {
+      for (size_t k = 0; k < L; ++k)
+        for (auto& g : plan[k])
+          if (g.blocks) g.first->batched(s, dev + g.off, (int)g.inst.size(), g.blocks);
+    };
+    size_t nl = 0;
+    for (size_t k = 0; k < L; ++k) nl += plan[k].size();
+    launches += nl;
+    stages += L;
+    if (!use_graphs) {
+      issue();
+      return 0;
+    }
+    std::vector<uint64_t> key;
+    key.reserve(4 * nl + 1);
+    key.push_back(total);
     for (size_t k = 0; k < L; ++k)
       for (auto& g : plan[k]) {
-        if (g.blocks) g.first->batched(s, dev + g.off, (int)g.inst.size(), g.blocks);
-        ++launches;
+        key.push_back((uint64_t)(uintptr_t)g.first->batched);
+        key.push_back(g.inst.size());
+        key.push_back(g.blocks);
+        key.push_back(g.off);
       }
-    stages += L;
-    return 0;
+    for (auto& c : graphs)
+      if (c.plan == key) {
+        ++graph_launches;
+        return hipGraphLaunch(c.exec, s) == hipSuccess ? 0 : -1;
+      }
+    if (graphs.size() >= GRAPH_CACHE) {
+      (void)hipStreamSynchronize(s);
+      drop_graphs();
+    }
+    hipGraph_t graph = nullptr;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return -1;
+    issue();
+    if (hipStreamEndCapture(s, &graph) != hipSuccess) return -1;
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) return -1;
+    graphs.push_back(Cached{std::move(key), exec});
+    ++graphs_built;
+    ++graph_launches;
+    return hipGraphLaunch(exec, s) == hipSuccess ? 0 : -1;
   }
 };
 

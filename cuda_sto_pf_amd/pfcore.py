@@ -211,7 +211,8 @@ EXPORTS = {
     "pf_batch_step": (C.c_int, [C.c_void_p, C.c_int]),
     "pf_batch_minimize": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "pf_batch_minimize_fused": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
-    "pf_batch_fused_launches": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "pf_batch_fused_launches": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_longlong)] * 4),
+    "pf_batch_set_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "pf_batch_destroy": (C.c_int, [C.c_void_p]),
     "pf_destroy": (C.c_int, [C.c_void_p]),
     "pf_fft_forward": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
@@ -551,10 +552,14 @@ class Batch:
         return [s.as_dict() for s in st]
 
     def fused_launches(self):
-        """(batched launches issued, launches recorded) of the last fused call"""
-        a, r = C.c_longlong(), C.c_longlong()
-        _check(lib().pf_batch_fused_launches(self._h, C.byref(a), C.byref(r)))
-        return a.value, r.value
+        """(batched launches issued, launches recorded, graphs captured, graph
+        launches) of the last fused call"""
+        v = [C.c_longlong() for _ in range(4)]
+        _check(lib().pf_batch_fused_launches(self._h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def set_graphs(self, enable):
+        _check(lib().pf_batch_set_graphs(self._h, 1 if enable else 0))
 
 
 def minimize_opts(**kw):

@@ -37,7 +37,7 @@ def make_sims(instances, prec, accel=True, net=None):
     return sims
 
 
-def run(instances, threads, opts, prec, accel=True, net=None, fused=False, launches=None):
+def run(instances, threads, opts, prec, accel=True, net=None, fused=False, launches=None, graphs=True):
     """Wall time, per-instance stats and final phi of one sweep: one instance
     at a time (threads <= 1), on one stream per instance with `threads` host
     workers, or fused (every stage one batched launch over the instances);
@@ -48,9 +48,11 @@ def run(instances, threads, opts, prec, accel=True, net=None, fused=False, launc
     t0 = time.perf_counter()
     if fused:
         b = pf.Batch(sims, 1)
+        b.set_graphs(graphs)
         stats = b.minimize_fused(opts)
         if launches is not None:
-            launches["batched"], launches["recorded"] = b.fused_launches()
+            launches["batched"], launches["recorded"], launches["graphs"], launches["graph_launches"] = \
+                b.fused_launches()
         b.close()
     elif threads <= 1:
         stats = [s.minimize(opts) for s in sims]
@@ -101,15 +103,19 @@ def main(argv=None):
     log(f"[sweep] max|phi batch - phi alone| = {dmax:.1e}, same iteration counts: {same_iters}, converged {conv}")
     fused = {}
     if not a.learned:
-        lc = {}
+        lc, lc0 = {}, {}
+        t_fng, st_fng, phi_fng = run(inst, 1, opts, prec, fused=True, launches=lc0, graphs=False)
+        log(f"[sweep] fused without graphs: {t_fng:.2f} s")
         t_fus, st_fus, phi_fus = run(inst, 1, opts, prec, fused=True, launches=lc)
         it_fus = sum(s["iteration"] for s in st_fus)
         dmax_f = max(float(np.abs(x - y).max()) for x, y in zip(phi_seq, phi_fus))
         same_f = all(x["iteration"] == y["iteration"] for x, y in zip(st_seq, st_fus))
+        same_f = same_f and all(np.array_equal(x, y) for x, y in zip(phi_fus, phi_fng))
         log(f"[sweep] fused: {t_fus:.2f} s, {it_fus / t_fus:.0f} iterations/s, {lc['batched']} batched launches "
-            f"for {lc['recorded']} recorded, max|phi fused - phi alone| = {dmax_f:.1e}, same iteration counts: "
-            f"{same_f}")
-        fused = dict(fused_s=t_fus, speedup_fused=t_seq / t_fus, iterations_per_s_fused=it_fus / t_fus,
+            f"for {lc['recorded']} recorded, {lc['graph_launches']} graph launches of {lc['graphs']} graphs, "
+            f"max|phi fused - phi alone| = {dmax_f:.1e}, same iteration counts: {same_f}")
+        fused = dict(fused_s=t_fus, fused_no_graphs_s=t_fng, graphs=lc["graphs"], graph_launches=lc["graph_launches"],
+                     speedup_fused=t_seq / t_fus, iterations_per_s_fused=it_fus / t_fus,
                      fused_launches=lc["batched"], recorded_launches=lc["recorded"], max_abs_dphi_fused=dmax_f,
                      same_iteration_counts_fused=same_f)
     rep = dict(instances=len(inst), precision=a.precision, learned=a.learned, max_iter=a.max_iter, dt=a.dt, V0=a.V0,

@@ -419,9 +419,15 @@ int pf_batch_minimize(pf_batch* b, const pf_minimize_opts* opts, pf_minimize_sta
  * bit.  No reference counterpart: the reference runs a sweep (e.g. 48 minimize
  * runs of 64^3 to 128^3 grids) as one process per instance. */
 int pf_batch_minimize_fused(pf_batch* b, const pf_minimize_opts* opts, pf_minimize_stats* stats);
-/* launches of the last fused call: batched launches issued, and the launches
- * the instances recorded (the count one-by-one execution issues) */
-int pf_batch_fused_launches(const pf_batch* b, long long* batched, long long* recorded);
+/* launches of the last fused call: batched launches issued, the launches the
+ * instances recorded (the count one-by-one execution issues), the HIP graphs
+ * captured and the graph launches (any pointer may be NULL) */
+int pf_batch_fused_launches(const pf_batch* b, long long* batched, long long* recorded, long long* graphs_built,
+                            long long* graph_launches);
+/* enable (1, the default) or disable (0) graph capture in fused calls: each
+ * phase's batched launches are captured into a HIP graph the first time
+ * their plan occurs and replayed as one graph launch when it recurs */
+int pf_batch_set_graphs(pf_batch* b, int enable);
 int pf_batch_destroy(pf_batch* b);
 
 /* ---------------------------------------------------------------------------

@@ -51,15 +51,19 @@ def test_fused_batch_minimize_equals_solo_runs(pf, prec, iters):
     p = pf.PF_MIXED if prec == "mixed" else pf.PF_FP64
     opts = pf.minimize_opts(dt=0.05, V0=0.1, max_iter=iters)
     t1, s1, p1 = sweep.run(pick, 1, opts, p)
-    lc = {}
-    t2, s2, p2 = sweep.run(pick, 1, opts, p, fused=True, launches=lc)
-    print(f"{prec}: solo {t1:.2f} s, fused {t2:.2f} s, {lc['batched']} batched launches for "
-          f"{lc['recorded']} recorded", flush=True)
-    for a, b, x, y in zip(s1, s2, p1, p2):
-        assert a["iteration"] == b["iteration"] == iters
-        assert a["lambda_vol"] == b["lambda_vol"] and a["rms_res"] == b["rms_res"]
-        assert np.array_equal(x, y)
-    assert lc["batched"] * 2 < lc["recorded"]
+    for graphs in (False, True):
+        lc = {}
+        t2, s2, p2 = sweep.run(pick, 1, opts, p, fused=True, launches=lc, graphs=graphs)
+        print(f"{prec}: solo {t1:.2f} s, fused {t2:.2f} s (graphs {graphs}), {lc['batched']} batched launches for "
+              f"{lc['recorded']} recorded, {lc['graph_launches']} graph launches of {lc['graphs']} graphs",
+              flush=True)
+        for a, b, x, y in zip(s1, s2, p1, p2):
+            assert a["iteration"] == b["iteration"] == iters
+            assert a["lambda_vol"] == b["lambda_vol"] and a["rms_res"] == b["rms_res"]
+            assert np.array_equal(x, y)
+        assert lc["batched"] * 2 < lc["recorded"]
+        # graphs are replayed (the same plan recurs across iterations)
+        assert (lc["graph_launches"] > lc["graphs"]) if graphs else lc["graph_launches"] == 0
 
 
 @pytest.mark.gpu
