@@ -102,10 +102,11 @@ def learned_cold_start(pf, learned, net, pd, pd_fast, phi, xB):
         print(f"[bench] learned cold start {label}: {i0} -> {i1} updates, {1e3 * t0:.1f} -> {1e3 * t1:.1f} ms",
               file=sys.stderr, flush=True)
     us = net.time_inference(phi.shape, reps=5)
-    # algorithmic multiply-adds of the 3^3 convolutions, 7 -> 16 -> 16 -> 16 -> 16 -> 3 channels (the
-    # MFMA tiles pad the first layer's input and the last layer's output to 16 channels)
+    # algorithmic multiply-adds of the 3^3 convolutions, 7 -> 16 -> 16 -> 16 -> 16 -> 3 channels; the
+    # MFMA work pads the first layer's input and the last layer's output to 16 channels and the 27
+    # taps to 14 pairs of the K = 32 instruction
     flop = 2.0 * 27 * (7 * 16 + 3 * 16 * 16 + 16 * 3) * float(np.prod(phi.shape))
-    padded = 2.0 * 27 * 16 * 16 * 5 * float(np.prod(phi.shape))
+    padded = 2.0 * 14 * 32 * 16 * 5 * float(np.prod(phi.shape))
     out["inference"] = {"us": us, "flop": flop, "flop_padded": padded, "achieved_tflops": flop / us * 1e-6,
                         "peak_tflops_bf16": 2500.0, "frac": flop / us * 1e-6 / 2500.0}
     return out

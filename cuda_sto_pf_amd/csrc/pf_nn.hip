@@ -98,7 +98,7 @@ struct Net {
   float* m = nullptr;
   float* v = nullptr;
   float* grad = nullptr;
-  uint16_t* Wb = nullptr;    // [LAYERS][W_PER_LAYER] forward order
+  uint16_t* Wb = nullptr;    // [LAYERS][WK_PER_LAYER] forward order (MFMA B operands)
   uint16_t* Wt = nullptr;    // transposed, tap-flipped
   uint16_t* feat = nullptr;  // dataset [capacity][S][16]
   float* tgt = nullptr;      // dataset [capacity][3][S]
@@ -121,7 +121,7 @@ struct Net {
     return PF_OK;
   }
   int pack() {
-    const int n = LAYERS * W_PER_LAYER;
+    const int n = LAYERS * WK_PER_LAYER;
     hipLaunchKernelGGL(pack_weights_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, master, Wb, Wt);
     NN_HIP(hipGetLastError());
     return PF_OK;
@@ -131,16 +131,13 @@ struct Net {
 // forward: act[0] -> act[1..4] -> y3
 static int forward(Net* net, Buffers& bf, hipStream_t s) {
   const Geom& g = bf.g;
-  const int blocks = (int)conv3_lds_blocks(g);
   for (int L = 0; L < LAYERS; ++L) {
-    const uint16_t* W = net->Wb + (size_t)L * W_PER_LAYER;
+    const uint16_t* W = net->Wb + (size_t)L * WK_PER_LAYER;
     const float* bias = net->master + (size_t)L * P_PER_LAYER + W_PER_LAYER;
     if (L < LAYERS - 1)
-      hipLaunchKernelGGL(conv3_lds_kernel<EPI_ACT>, dim3(blocks), dim3(256), 0, s, g, bf.act[L], W, bias,
-                         bf.act[L + 1], (float*)nullptr, (const uint16_t*)nullptr);
+      conv3_launch<EPI_ACT>(s, g, bf.act[L], W, bias, bf.act[L + 1], nullptr, nullptr);
     else
-      hipLaunchKernelGGL(conv3_lds_kernel<EPI_OUT3>, dim3(blocks), dim3(256), 0, s, g, bf.act[L], W, bias,
-                         (uint16_t*)nullptr, bf.y3, (const uint16_t*)nullptr);
+      conv3_launch<EPI_OUT3>(s, g, bf.act[L], W, bias, nullptr, bf.y3, nullptr);
   }
   NN_HIP(hipGetLastError());
   return PF_OK;
@@ -153,16 +150,14 @@ static int backward(Net* net, Buffers& bf, hipStream_t s, float inv_t, float* gr
   hipLaunchKernelGGL(loss_kernel, dim3(bf.lblocks), dim3(256), 0, s, bf.y3, bf.t3, g, inv_t, inv_count, bf.grad[0],
                      bf.lossp);
   int cur = 0;
-  const int cblocks = (int)conv3_lds_blocks(g);
   for (int L = LAYERS - 1; L >= 0; --L) {
     hipLaunchKernelGGL(conv3_wgrad_kernel, dim3(bf.nwaves / 4), dim3(256), 0, s, g, bf.act[L], bf.grad[cur],
                        bf.partials);
     hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((P_PER_LAYER + 255) / 256), dim3(256), 0, s, bf.partials, bf.nwaves,
                        grad_out + (size_t)L * P_PER_LAYER);
     if (L > 0) {
-      hipLaunchKernelGGL(conv3_lds_kernel<EPI_GRAD>, dim3(cblocks), dim3(256), 0, s, g, bf.grad[cur],
-                         net->Wt + (size_t)L * W_PER_LAYER, (const float*)nullptr, bf.grad[1 - cur], (float*)nullptr,
-                         bf.act[L]);
+      conv3_launch<EPI_GRAD>(s, g, bf.grad[cur], net->Wt + (size_t)L * WK_PER_LAYER, nullptr, bf.grad[1 - cur],
+                             nullptr, bf.act[L]);
       cur = 1 - cur;
     }
   }
@@ -196,19 +191,16 @@ static int gather(Net* net, const int* idx, int batch) {
 int infer(Net* net, hipStream_t s, int Nx, int Ny, int Nz, const uint16_t* X0, uint16_t* a, uint16_t* b, float* Y3) {
   if (s != net->stream) NN_HIP(hipStreamSynchronize(net->stream));   // weights are packed on the network's stream
   const Geom g{1, Nx, Ny, Nz, (long)Nx * Ny * Nz};
-  const int blocks = (int)conv3_lds_blocks(g);
   const uint16_t* in = X0;
   uint16_t* bufs[2] = {a, b};
   for (int L = 0; L < LAYERS; ++L) {
-    const uint16_t* W = net->Wb + (size_t)L * W_PER_LAYER;
+    const uint16_t* W = net->Wb + (size_t)L * WK_PER_LAYER;
     const float* bias = net->master + (size_t)L * P_PER_LAYER + W_PER_LAYER;
     if (L < LAYERS - 1) {
-      hipLaunchKernelGGL(conv3_lds_kernel<EPI_ACT>, dim3(blocks), dim3(256), 0, s, g, in, W, bias, bufs[L & 1],
-                         (float*)nullptr, (const uint16_t*)nullptr);
+      conv3_launch<EPI_ACT>(s, g, in, W, bias, bufs[L & 1], nullptr, nullptr);
       in = bufs[L & 1];
     } else {
-      hipLaunchKernelGGL(conv3_lds_kernel<EPI_OUT3>, dim3(blocks), dim3(256), 0, s, g, in, W, bias, (uint16_t*)nullptr,
-                         Y3, (const uint16_t*)nullptr);
+      conv3_launch<EPI_OUT3>(s, g, in, W, bias, nullptr, Y3, nullptr);
     }
   }
   NN_HIP(hipGetLastError());
@@ -270,8 +262,8 @@ int pf_nn_create(int Nx, int Ny, int Nz, int max_batch, int capacity, pf_nn** ou
   if (hipStreamCreateWithFlags(&n.stream, hipStreamNonBlocking) != hipSuccess)
     return bad(fail(PF_E_HIP, "pf_nn_create: stream"));
   if ((rc = n.alloc(&n.master, P_TOTAL)) || (rc = n.alloc(&n.m, P_TOTAL)) || (rc = n.alloc(&n.v, P_TOTAL)) ||
-      (rc = n.alloc(&n.grad, P_TOTAL)) || (rc = n.alloc(&n.Wb, (size_t)LAYERS * W_PER_LAYER)) ||
-      (rc = n.alloc(&n.Wt, (size_t)LAYERS * W_PER_LAYER)))
+      (rc = n.alloc(&n.grad, P_TOTAL)) || (rc = n.alloc(&n.Wb, (size_t)LAYERS * WK_PER_LAYER)) ||
+      (rc = n.alloc(&n.Wt, (size_t)LAYERS * WK_PER_LAYER)))
     return bad(rc);
   if (capacity > 0 && ((rc = n.alloc(&n.feat, (size_t)capacity * n.S * CH)) ||
                        (rc = n.alloc(&n.tgt, (size_t)capacity * OUT_CH * n.S))))

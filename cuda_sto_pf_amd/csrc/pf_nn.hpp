@@ -21,12 +21,13 @@
 // plus a bias[16]; the input layer uses cin < 7 (phi and six eigenstress
 // components), the output layer cout < 3 (displacement correction).
 //
-// MFMA mapping (v_mfma_f32_16x16x16_bf16, wave64): M = 16 consecutive z
-// voxels, N = 16 output channels, K = 16 input channels of one tap; 27 MFMAs
-// per 16-voxel tile.  Lane l holds A[l%16][4(l/16)..+3], B[4(l/16)..+3][l%16]
-// and D[4(l/16)..+3][l%16].  The B operands (weights) of all 27 taps live in
-// registers for the whole kernel; the compute copies are stored pre-arranged
-// in that lane order (8 contiguous bytes per lane and tap).
+// MFMA mapping (v_mfma_f32_16x16x32_bf16, wave64): M = 16 consecutive z
+// voxels, N = 16 output channels, K = 32 = the 16 input channels of two taps
+// (tap pairs below); 14 MFMAs per 16-voxel tile for the 27 taps.  Lane l
+// holds A[l%16][8(l/16)..+7], B[8(l/16)..+7][l%16] and D[4(l/16)..+3][l%16].
+// The B operands (weights) of all 14 pairs live in registers for the whole
+// kernel; the compute copies are stored pre-arranged in that lane order (16
+// contiguous bytes per lane and pair).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -49,6 +50,19 @@ constexpr float SLOPE = 0.1f;                        // leaky ReLU
 
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+// Tap pairs of the K = 32 MFMA.  Taps t = 9 dxi + 3 dyi + dzi (dxi, dyi,
+// dzi in 0..2).  Pairs 0..8 join the z shifts 0 and 1 of the (dxi, dyi) = (p/3,
+// p%3) column; pairs 9..12 join the z shift 2 of columns q and q + 1 (q = 0,
+// 2, 4, 6); pair 13 is the z shift 2 of column 8 alone (its second half has
+// zero weights).
+constexpr int KPAIRS = 14;
+constexpr int WK_PER_LAYER = KPAIRS * 64 * 8;   // bf16 compute copy of one layer
+__host__ __device__ constexpr int pair_tap(int p, int hi) {
+  return p < 9 ? 9 * (p / 3) + 3 * (p % 3) + hi
+               : (2 * (p - 9) + hi > 8 ? -1 : 9 * ((2 * (p - 9) + hi) / 3) + 3 * ((2 * (p - 9) + hi) % 3) + 2);
+}
 
 struct Geom {
   int B, Nx, Ny, Nz;
@@ -70,22 +84,30 @@ enum Epilogue { EPI_ACT = 0, EPI_OUT3 = 1, EPI_GRAD = 2 };
 // output voxels (32 MFMA tiles, 8 per wave, two z tiles at a time) from a
 // 6 x 6 x 34 input tile staged once in LDS, so every input voxel is read
 // from HBM about 2.4 times per layer instead of once per tap.  The tile is
-// stored as four planes of 4 channels (8 bytes per voxel), each plane offset
-// by 128 bytes modulo 256, so the 8-byte A-operand reads of the four 16-lane
-// groups fall on distinct banks.
+// stored as two planes of 8 channels (16 bytes per voxel), the second offset
+// by 128 bytes modulo 256; a 16-lane group's A-operand read covers 256
+// contiguous bytes.
 constexpr int TBX = 4, TBY = 4, TBZ = 32;
 constexpr int TLX = TBX + 2, TLY = TBY + 2, TLZ = TBZ + 2;
 constexpr int TLVOX = TLX * TLY * TLZ;
-constexpr int TPLANE = ((TLVOX * 8 + 127) / 256) * 256 + 128;   // bytes
+constexpr int TPLANE = ((TLVOX * 16 + 255) / 256) * 256 + 128;   // bytes
 
-__device__ __forceinline__ int pmod(int i, int n) { return ((i % n) + n) % n; }
+// i mod n for i >= -n: a subtraction loop that runs at most once for the
+// tile offsets of grids at least as long as a tile (the tile loads issue one
+// of these per axis, and a runtime-divisor remainder costs tens of VALU
+// instructions)
+__device__ __forceinline__ int wrapn(int i, int n) {
+  if (i < 0) i += n;
+  while (i >= n) i -= n;
+  return i;
+}
 
 template <int EPI>
 __global__ void __launch_bounds__(256) conv3_lds_kernel(Geom g, const uint16_t* __restrict__ X,
                                                         const uint16_t* __restrict__ Wb,
                                                         const float* __restrict__ bias, uint16_t* __restrict__ Y,
                                                         float* __restrict__ Y3, const uint16_t* __restrict__ aux) {
-  __shared__ __attribute__((aligned(16))) unsigned char lds[4 * TPLANE];
+  __shared__ __attribute__((aligned(16))) unsigned char lds[2 * TPLANE];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int row = lane & 15, kq = lane >> 4;
@@ -99,68 +121,88 @@ __global__ void __launch_bounds__(256) conv3_lds_kernel(Geom g, const uint16_t* 
   const int b = (int)(blk / nbx);
   const int x0 = bx * TBX, y0 = by * TBY, z0 = bz * TBZ;
   const uint16_t* Xb = X + (size_t)b * g.S * CH;
-  // all of a thread's 16-byte tile loads are issued before the LDS writes
-  constexpr int SPER = (TLVOX * 2 + 255) / 256;
-  uint4 sv[SPER];
+  // The tile's 16-byte chunks, all loads issued before the LDS writes.  A
+  // (lx, ly) row of the tile is 34 voxels = 68 chunks: wave w loads rows
+  // w, w + 4, ..., w + 32 with lane l taking chunk l (voxel l / 2, half l % 2),
+  // so the row's x/y wrap and base address are wave-uniform and each lane's
+  // z offset is the same for all its rows; the last 4 chunks of the 36 rows
+  // (voxels 32, 33) go to threads 0..143, one each.
+  constexpr int ROWS = TLX * TLY;            // 36
+  constexpr int RPW = ROWS / 4;              // 9 rows per wave
+  static_assert(ROWS % 4 == 0 && TLZ == 34, "tile rows");
+  const int wvu = __builtin_amdgcn_readfirstlane(wv);
+  const int gz_l = wrapn(z0 - 1 + (lane >> 1), g.Nz);
+  const size_t loff = (size_t)gz_l * CH + (lane & 1) * 8;
+  uint4 sv[RPW];
 #pragma unroll
-  for (int i = 0; i < SPER; ++i) {
-    const int c = threadIdx.x + i * 256;
-    if (c < TLVOX * 2) {
-      const int h = c & 1, v = c >> 1;
-      const int lz = v % TLZ, r = v / TLZ;
-      const int ly = r % TLY, lx = r / TLY;
-      const int gx = pmod(x0 + lx - 1, g.Nx), gy = pmod(y0 + ly - 1, g.Ny), gz = pmod(z0 + lz - 1, g.Nz);
-      sv[i] = *reinterpret_cast<const uint4*>(Xb + (((size_t)gx * g.Ny + gy) * g.Nz + gz) * CH + h * 8);
-    }
+  for (int k = 0; k < RPW; ++k) {
+    const int r = wvu + 4 * k;
+    const int lx = r / TLY, ly = r - lx * TLY;
+    const int gx = wrapn(x0 + lx - 1, g.Nx), gy = wrapn(y0 + ly - 1, g.Ny);
+    sv[k] = *reinterpret_cast<const uint4*>(Xb + (size_t)(gx * g.Ny + gy) * g.Nz * CH + loff);
+  }
+  uint4 se = make_uint4(0u, 0u, 0u, 0u);
+  const int er = threadIdx.x >> 2, elz = 32 + ((threadIdx.x >> 1) & 1), eh = threadIdx.x & 1;
+  if (threadIdx.x < ROWS * 4) {
+    const int lx = er / TLY, ly = er - lx * TLY;
+    const int gx = wrapn(x0 + lx - 1, g.Nx), gy = wrapn(y0 + ly - 1, g.Ny), gz = wrapn(z0 + elz - 1, g.Nz);
+    se = *reinterpret_cast<const uint4*>(Xb + ((size_t)(gx * g.Ny + gy) * g.Nz + gz) * CH + eh * 8);
   }
 #pragma unroll
-  for (int i = 0; i < SPER; ++i) {
-    const int c = threadIdx.x + i * 256;
-    if (c < TLVOX * 2) {
-      const int h = c & 1, v = c >> 1;
-      *reinterpret_cast<uint2*>(lds + (2 * h) * TPLANE + v * 8) = make_uint2(sv[i].x, sv[i].y);
-      *reinterpret_cast<uint2*>(lds + (2 * h + 1) * TPLANE + v * 8) = make_uint2(sv[i].z, sv[i].w);
-    }
+  for (int k = 0; k < RPW; ++k) {
+    const int v = (wvu + 4 * k) * TLZ + (lane >> 1);
+    *reinterpret_cast<uint4*>(lds + (lane & 1) * TPLANE + v * 16) = sv[k];
   }
-  v4s w[TAPS];
+  if (threadIdx.x < ROWS * 4) *reinterpret_cast<uint4*>(lds + eh * TPLANE + (er * TLZ + elz) * 16) = se;
+  v8bf w[KPAIRS];
 #pragma unroll
-  for (int t = 0; t < TAPS; ++t) w[t] = *reinterpret_cast<const v4s*>(Wb + ((size_t)t * 64 + lane) * 4);
+  for (int p = 0; p < KPAIRS; ++p) w[p] = *reinterpret_cast<const v8bf*>(Wb + ((size_t)p * 64 + lane) * 8);
   const float bn = (EPI != EPI_GRAD && bias) ? bias[row] : 0.0f;
   __syncthreads();
-  const unsigned char* plane = lds + kq * TPLANE;
+  // lane bases: plane kq & 1; the second tap of a pair (lanes with kq >= 2)
+  // lies 1 voxel (pairs 0..8), one y column (pairs 9, 11, 12) or
+  // (TLY - 2) y columns (pair 10) further
+  const int hi = kq >> 1;
+  const unsigned char* plane = lds + (kq & 1) * TPLANE + row * 16;
 #pragma unroll 1
   for (int i = 0; i < 4; ++i) {
     const int col = wv * 4 + i;          // (tx, ty) column of the block
     const int tx = col >> 2, ty = col & 3;
+    const unsigned char* pb = plane + ((tx * TLY + ty) * TLZ) * 16;
+    const unsigned char* b1 = pb + hi * 16;
+    const unsigned char* b2 = pb + hi * TLZ * 16;
+    const unsigned char* b3 = pb + hi * (TLY - 2) * TLZ * 16;
     v4f acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int t = 0; t < TAPS; ++t) {
-      const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
-      const int lv = ((tx + 1 + dx) * TLY + (ty + 1 + dy)) * TLZ + row + 1 + dz;
-      const v4s a0 = *reinterpret_cast<const v4s*>(plane + lv * 8);
-      const v4s a1 = *reinterpret_cast<const v4s*>(plane + (lv + 16) * 8);
-      acc0 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, w[t], acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, w[t], acc1, 0, 0, 0);
+    for (int p = 0; p < KPAIRS; ++p) {
+      const int t = pair_tap(p, 0);
+      const int off = (((t / 9) * TLY + (t / 3) % 3) * TLZ + t % 3) * 16;
+      const unsigned char* base = p < 9 ? b1 : (p == 10 ? b3 : (p == 13 ? pb : b2));
+      const v8bf a0 = *reinterpret_cast<const v8bf*>(base + off);
+      const v8bf a1 = *reinterpret_cast<const v8bf*>(base + off + 16 * 16);
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, w[p], acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, w[p], acc1, 0, 0, 0);
     }
     const int x = x0 + tx, y = y0 + ty;
     if (x >= g.Nx || y >= g.Ny) continue;
-    const size_t vbase = ((size_t)x * g.Ny + y) * g.Nz;
+    // this lane's first voxel; the others are at constant offsets hz 16 + q
+    const size_t v0 = (size_t)b * g.S + ((size_t)x * g.Ny + y) * g.Nz + z0 + kq * 4;
+    const bool full = z0 + TBZ <= g.Nz;   // no z bound checks (all but a partial last z block)
 #pragma unroll
     for (int hz = 0; hz < 2; ++hz) {
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int z = z0 + hz * 16 + kq * 4 + q;
-        if (z >= g.Nz) continue;
-        const size_t v = vbase + z;
+        const int dz = hz * 16 + q;
+        if (!full && z0 + kq * 4 + dz >= g.Nz) continue;
         float val = (hz ? acc1[q] : acc0[q]) + bn;
         if constexpr (EPI == EPI_ACT) {
           val = val > 0.f ? val : SLOPE * val;
-          Y[((size_t)b * g.S + v) * CH + row] = f2bf(val);
+          Y[(v0 + dz) * CH + row] = f2bf(val);
         } else if constexpr (EPI == EPI_OUT3) {
-          if (row < OUT_CH) Y3[((size_t)b * OUT_CH + row) * g.S + v] = val;
+          if (row < OUT_CH) Y3[((size_t)b * OUT_CH + row) * g.S + (v0 - (size_t)b * g.S) + dz] = val;
         } else {
-          const float a = bf2f(aux[((size_t)b * g.S + v) * CH + row]);
-          Y[((size_t)b * g.S + v) * CH + row] = f2bf(a > 0.f ? val : SLOPE * val);
+          const float a = bf2f(aux[(v0 + dz) * CH + row]);
+          Y[(v0 + dz) * CH + row] = f2bf(a > 0.f ? val : SLOPE * val);
         }
       }
     }
@@ -169,6 +211,13 @@ __global__ void __launch_bounds__(256) conv3_lds_kernel(Geom g, const uint16_t* 
 
 __host__ __device__ inline long conv3_lds_blocks(const Geom& g) {
   return (long)g.B * ((g.Nx + TBX - 1) / TBX) * ((g.Ny + TBY - 1) / TBY) * ((g.Nz + TBZ - 1) / TBZ);
+}
+
+template <int EPI>
+inline void conv3_launch(hipStream_t s, const Geom& g, const uint16_t* X, const uint16_t* Wb, const float* bias,
+                         uint16_t* Y, float* Y3, const uint16_t* aux) {
+  hipLaunchKernelGGL(conv3_lds_kernel<EPI>, dim3((unsigned)conv3_lds_blocks(g)), dim3(256), 0, s, g, X, Wb, bias, Y,
+                     Y3, aux);
 }
 
 // Per-wave partial weight and bias gradients of one layer:
@@ -271,18 +320,22 @@ __global__ void __launch_bounds__(256) loss_kernel(const float* __restrict__ Y3,
   if (threadIdx.x == 0) partials[blockIdx.x] = ws[0] + ws[1] + ws[2] + ws[3];
 }
 
-// Compute copies of the FP32 master weights: the forward B-operand order
-// Wb[t][lane][j] = W[t][4(lane/16)+j][lane%16] and, for the input gradient,
-// the transposed and tap-flipped Wt[t][lane][j] = W[26-t][lane%16][4(lane/16)+j].
+// Compute copies of the FP32 master weights in the B-operand order of the
+// K = 32 MFMA: W?[L][p][lane][j] holds K row k = 8(lane/16) + j (tap
+// pair_tap(p, k/16), input channel k%16) of output column lane%16, zero for
+// the missing half of the single pair.  Wb is the forward weight W[t][k][n];
+// Wt, for the input gradient, the transposed and tap-flipped W[26-t][n][k].
 __global__ void __launch_bounds__(256) pack_weights_kernel(const float* __restrict__ master, uint16_t* __restrict__ Wb,
                                                            uint16_t* __restrict__ Wt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= LAYERS * W_PER_LAYER) return;
-  const int L = i / W_PER_LAYER, e = i % W_PER_LAYER;
-  const int t = e / (CH * CH), k = (e / CH) % CH, n = e % CH;
-  const uint16_t v = f2bf(master[(size_t)L * P_PER_LAYER + e]);
-  Wb[(size_t)L * W_PER_LAYER + ((size_t)t * 64 + (n + 16 * (k >> 2))) * 4 + (k & 3)] = v;
-  Wt[(size_t)L * W_PER_LAYER + ((size_t)(26 - t) * 64 + (k + 16 * (n >> 2))) * 4 + (n & 3)] = v;
+  if (i >= LAYERS * WK_PER_LAYER) return;
+  const int L = i / WK_PER_LAYER, e = i % WK_PER_LAYER;
+  const int p = e / 512, lane = (e / 8) % 64, j = e % 8;
+  const int k = 8 * (lane / 16) + j, n = lane % 16;
+  const int t = pair_tap(p, k / 16), c = k % 16;
+  const float* W = master + (size_t)L * P_PER_LAYER;
+  Wb[i] = t < 0 ? (uint16_t)0 : f2bf(W[(t * CH + c) * CH + n]);
+  Wt[i] = t < 0 ? (uint16_t)0 : f2bf(W[((26 - t) * CH + n) * CH + c]);
 }
 
 // Adam (Kingma & Ba 2015) on all parameters; t is the 1-based step count.
