@@ -408,6 +408,8 @@ def main():
                                 if args.workload == 5 else f"{meta['workload']} {phi.shape[0]}^3"),
                    "grid": list(phi.shape), "learned_warm_start": net is not None,
                    "kz_stride": sc_m["kz_stride"],
+                   "tile_plan": {"zpass_rows": sc_m["zpass_rows"], "ypass_cols": sc_m["ypass_cols"],
+                                 "xpass_cols": sc_m["xpass_cols"]},
                    "path": "mixed precision, learned cold start + extrapolated warm start (order 3) + "
                            "Anderson depth 3"},
         "elastic_updates_per_step": it_m,

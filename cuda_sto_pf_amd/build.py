@@ -20,7 +20,7 @@ CLI = os.path.join(HERE, "bin", "main_pf")
 HIP_SOURCES = ["pf_engine_f64.hip", "pf_engine_f32.hip", "pf_runtime.hip", "pf_nn.hip", "pf_batch.hip"]
 CXX_SOURCES = ["pf_params.cpp", "pf_checkpoint.cpp"]
 CLI_SOURCES = ["main_pf.cpp"]
-PASS_GROUPS = 7   # z-pass operator groups 0-4 and 6, the column passes (5), pf_pass_inst.hpp
+PASS_GROUPS = 8   # z-pass operator groups 0-4, 6 and 7, the column passes (5), pf_pass_inst.hpp
 
 ARCH = os.environ.get("PF_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")

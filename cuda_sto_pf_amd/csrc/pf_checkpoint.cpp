@@ -348,6 +348,7 @@ namespace {
 constexpr char kExactMagic[8] = {'P', 'F', 'A', 'M', 'D', 'X', 'S', '1'};
 constexpr std::uint32_t kExactHeader = 48;    // version 1: warm state and history
 constexpr std::uint32_t kExactHeader2 = 56;   // version 2: + count of the phi* block
+constexpr std::uint32_t kExactHeader3 = 72;   // version 3: + the tile plan (3 int32 and one zero int32)
 }  // namespace
 
 bool write_exact(const std::string& path, const ExactState& s, std::string* err) {
@@ -356,14 +357,16 @@ bool write_exact(const std::string& path, const ExactState& s, std::string* err)
   if (v2 && s.phi_spec.size() != 2 * s.n) return set_err(err, "exact-resume phi* spectrum size mismatch");
   Image h;
   h.put_text(std::string(kExactMagic, 8), 8);
-  h.put<std::uint32_t>(v2 ? 2 : 1);
-  h.put<std::uint32_t>(v2 ? kExactHeader2 : kExactHeader);
+  h.put<std::uint32_t>(3);
+  h.put<std::uint32_t>(kExactHeader3);
   h.put<std::uint64_t>(s.bound_checksum);
   h.put<std::uint64_t>(s.accepted_step);
   h.put<std::int32_t>(s.precision);
   h.put<std::int32_t>(s.nstates);
   h.put<std::uint64_t>(s.n);
-  if (v2) h.put<std::uint64_t>(s.phi_star.size());
+  h.put<std::uint64_t>(s.phi_star.size());
+  for (int i = 0; i < 3; ++i) h.put<std::int32_t>(s.plan[i]);
+  h.put<std::int32_t>(0);
   std::uint64_t sum = fnv1a64(h.b.data(), h.b.size());
   sum = fnv1a64(s.states.data(), s.states.size() * sizeof(double), sum);
   if (v2) {
@@ -392,14 +395,15 @@ bool read_exact(const std::string& path, ExactState* s, std::string* err) {
     if (err) err->clear();
     return false;
   }
-  unsigned char hb[kExactHeader2];
+  unsigned char hb[kExactHeader3];
   bool ok = read_all(f, hb, 16) && std::memcmp(hb, kExactMagic, 8) == 0;
   std::uint32_t version = 0, hsize = 0;
   if (ok) {
     Cursor r{hb + 8};
     version = r.get<std::uint32_t>();
     hsize = r.get<std::uint32_t>();
-    ok = (version == 1 && hsize == kExactHeader) || (version == 2 && hsize == kExactHeader2);
+    ok = (version == 1 && hsize == kExactHeader) || (version == 2 && hsize == kExactHeader2) ||
+         (version == 3 && hsize == kExactHeader3);
   }
   ok = ok && read_all(f, hb + 16, hsize - 16);
   ExactState t;
@@ -411,9 +415,12 @@ bool read_exact(const std::string& path, ExactState* s, std::string* err) {
     t.precision = r.get<std::int32_t>();
     t.nstates = r.get<std::int32_t>();
     t.n = r.get<std::uint64_t>();
-    if (version == 2) nphi = r.get<std::uint64_t>();
+    if (version >= 2) nphi = r.get<std::uint64_t>();
+    if (version >= 3)
+      for (int i = 0; i < 3; ++i) t.plan[i] = r.get<std::int32_t>();
     ok = t.nstates >= 0 && t.nstates <= 4 && (t.nstates > 0 || nphi > 0) && t.n > 0 && t.n < (1ull << 40) &&
          nphi < (1ull << 40);
+    for (int i = 0; i < 3; ++i) ok = ok && t.plan[i] >= -1 && t.plan[i] <= 3;
   }
   std::uint64_t sum = 0, stored = 0;
   if (ok) {

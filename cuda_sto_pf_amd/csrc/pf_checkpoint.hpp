@@ -80,6 +80,10 @@ struct ExactState {
   // mixed precision: the unclamped phi* of the phi update ([Nx][Ny][Nz]) and
   // its retained spectrum (n complex), or empty
   std::vector<double> phi_star, phi_spec;
+  // the handle's tile plan (z-pass budget, y and x column-tile indices), so a
+  // restart in another process transforms with the same tiles; -1: absent
+  // (files of format versions 1 and 2)
+  std::int32_t plan[3] = {-1, -1, -1};
 };
 bool write_exact(const std::string& path, const ExactState& s, std::string* err);
 // false with an empty error when the file does not exist

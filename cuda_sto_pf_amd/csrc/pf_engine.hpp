@@ -14,8 +14,12 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <array>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/pfcore.h"
@@ -935,9 +939,9 @@ int zpass_grid(int N, int rows) {
 
 // Returns the grid size (the number of per-block partials the operator's
 // reductions wrote).
-template <typename T, int NCI, int NCO, class Op>
+template <typename T, int NCI, int NCO, class Op, int BUDGET = ZBudget<Op>::value>
 int launch_zpass(int N, hipStream_t s, const ZPassArgs<T>& za, const typename Op::Args& oa) {
-  const int grid = zpass_grid<T, NCI, NCO, Op::NPRE, ZBudget<Op>::value>(N, za.rows);
+  const int grid = zpass_grid<T, NCI, NCO, Op::NPRE, BUDGET>(N, za.rows);
   switch (N) {
 #define PF_CASE_Z(v)                                                                                \
   case v:                                                                                           \
@@ -947,7 +951,7 @@ int launch_zpass(int N, hipStream_t s, const ZPassArgs<T>& za, const typename Op
         break;                                                                                      \
       }                                                                                             \
     }                                                                                               \
-    launch<KZPass<T, v, NCI, NCO, Op>>(s, dim3(grid), {za, oa});                                  \
+    launch<KZPass<T, v, NCI, NCO, Op, BUDGET>>(s, dim3(grid), {za, oa});                          \
     break;
     PF_SIZES(PF_CASE_Z)
 #undef PF_CASE_Z
@@ -957,20 +961,78 @@ int launch_zpass(int N, hipStream_t s, const ZPassArgs<T>& za, const typename Op
   return grid;
 }
 
-template <typename T, bool INV>
-void launch_cpass(int N, hipStream_t s, const CPassArgs<T>& ca, int planes, int comps) {
-  switch (N) {
-#define PF_CASE_C(v)                                                                            \
-  case v: {                                                                                     \
-    constexpr int TC = cpass_cols<T, v>();                                                      \
-    dim3 grid((ca.cols + TC - 1) / TC, planes, comps);                                          \
-    launch<KCPass<T, v, INV>>(s, grid, ca);                                                    \
-    break;                                                                                      \
+// The z pass with the LDS budget ZPASS_BUDGETS[bsel] (the solver's z pass,
+// whose budget the plan tuner picks at pf_create).
+template <typename T, int NCI, int NCO, class Op>
+int launch_zpass_b(int bsel, int N, hipStream_t s, const ZPassArgs<T>& za, const typename Op::Args& oa) {
+  switch (bsel) {
+    case 0: return launch_zpass<T, NCI, NCO, Op, ZPASS_BUDGETS[0]>(N, s, za, oa);
+    case 2: return launch_zpass<T, NCI, NCO, Op, ZPASS_BUDGETS[2]>(N, s, za, oa);
+    case 3: return launch_zpass<T, NCI, NCO, Op, ZPASS_BUDGETS[3]>(N, s, za, oa);
+    default: return launch_zpass<T, NCI, NCO, Op, ZPASS_BUDGETS[1]>(N, s, za, oa);
   }
+}
+// rows per workgroup of that z pass (the tuner skips budgets that give the same tile)
+template <typename T, int NCI, int NCO, class Op>
+int zpass_rows_of(int bsel, int N) {
+  switch (N) {
+#define PF_CASE_ZR(v)                                                                                      \
+  case v:                                                                                                  \
+    switch (bsel) {                                                                                        \
+      case 0: return zpass_rows_for<T, v, NCI, NCO, Op::NPRE, ZPASS_BUDGETS[0]>();                          \
+      case 2: return zpass_rows_for<T, v, NCI, NCO, Op::NPRE, ZPASS_BUDGETS[2]>();                          \
+      case 3: return zpass_rows_for<T, v, NCI, NCO, Op::NPRE, ZPASS_BUDGETS[3]>();                          \
+      default: return zpass_rows_for<T, v, NCI, NCO, Op::NPRE, ZPASS_BUDGETS[1]>();                         \
+    }
+    PF_SIZES(PF_CASE_ZR)
+#undef PF_CASE_ZR
+    default:
+      return 0;
+  }
+}
+
+// Column pass along an axis of length N with the tile width CPASS_TILES[tsel]
+// (0: cpass_cols' width; widths that do not fit the LDS fall back to it).
+template <typename T, bool INV, int TCW, int v>
+void launch_cpass_w(hipStream_t s, const CPassArgs<T>& ca, int planes, int comps) {
+  constexpr int TC = (TCW == 0 || !cpass_tile_fits<T, v, TCW>()) ? cpass_cols<T, v>() : TCW;
+  dim3 grid((ca.cols + TC - 1) / TC, planes, comps);
+  launch<KCPass<T, v, INV, TC>>(s, grid, ca);
+}
+template <typename T, bool INV>
+void launch_cpass(int N, hipStream_t s, const CPassArgs<T>& ca, int planes, int comps, int tsel = 0) {
+  switch (N) {
+#define PF_CASE_C(v)                                                    \
+  case v:                                                               \
+    switch (tsel) {                                                     \
+      case 1: launch_cpass_w<T, INV, CPASS_TILES[1], v>(s, ca, planes, comps); break; \
+      case 2: launch_cpass_w<T, INV, CPASS_TILES[2], v>(s, ca, planes, comps); break; \
+      case 3: launch_cpass_w<T, INV, CPASS_TILES[3], v>(s, ca, planes, comps); break; \
+      default: launch_cpass_w<T, INV, 0, v>(s, ca, planes, comps); break;            \
+    }                                                                   \
+    break;
     PF_SIZES(PF_CASE_C)
 #undef PF_CASE_C
     default:
       break;
+  }
+}
+// the tile width launch_cpass uses for (N, tsel)
+template <typename T>
+int cpass_cols_of(int tsel, int N) {
+  switch (N) {
+#define PF_CASE_CW(v)                                                                                     \
+  case v:                                                                                                 \
+    switch (tsel) {                                                                                       \
+      case 1: return cpass_tile_fits<T, v, CPASS_TILES[1]>() ? CPASS_TILES[1] : cpass_cols<T, v>();      \
+      case 2: return cpass_tile_fits<T, v, CPASS_TILES[2]>() ? CPASS_TILES[2] : cpass_cols<T, v>();      \
+      case 3: return cpass_tile_fits<T, v, CPASS_TILES[3]>() ? CPASS_TILES[3] : cpass_cols<T, v>();      \
+      default: return cpass_cols<T, v>();                                                                 \
+    }
+    PF_SIZES(PF_CASE_CW)
+#undef PF_CASE_CW
+    default:
+      return 0;
   }
 }
 
@@ -1224,6 +1286,7 @@ struct Engine final : EngineBase {
       return rc;
     PF_HIP(hipHostMalloc((void**)&hscal, NSCAL * sizeof(double), hipHostMallocDefault));
     if (kz_cands.size() > 1 && (rc = tune_kz_stride())) return rc;
+    if ((rc = tune_plans())) return rc;
     return PF_OK;
   }
 
@@ -1291,6 +1354,120 @@ struct Engine final : EngineBase {
     return PF_OK;
   }
 
+  // Tile plans: the LDS budget of the solver's z pass (rows per workgroup,
+  // ZPASS_BUDGETS) and the column-tile widths of the y and x passes
+  // (CPASS_TILES).  Each choice only changes which rows or columns a
+  // workgroup transforms together (the solver's z-pass operators reduce
+  // nothing), so results are bitwise the same for every choice; the tuner
+  // times each distinct tile on the handle's own grid and keeps the fastest.
+  // PF_PLAN=<z>,<y>,<x> fixes the three indices.
+  // The first handle of a grid shape and solver kind in a process tunes; the
+  // others take its plan, so handles of one shape compute identically (the
+  // tiles change the compiled code of the transforms, and with it the
+  // rounding of some of their operations), and the checkpoint companion
+  // carries the plan to a restart.
+  int zb_sel = 1, ty_sel = 0, tx_sel = 0;
+  double plan_us[3] = {0.0, 0.0, 0.0};   // the chosen plans' timed microseconds
+  struct PlanKey {
+    int nx, ny, nz, el, ext;
+    bool operator<(const PlanKey& o) const {
+      return std::tie(nx, ny, nz, el, ext) < std::tie(o.nx, o.ny, o.nz, o.el, o.ext);
+    }
+  };
+  static std::mutex& plan_mu() {
+    static std::mutex m;
+    return m;
+  }
+  static std::map<PlanKey, std::array<int, 3>>& plan_cache() {
+    static std::map<PlanKey, std::array<int, 3>> c;
+    return c;
+  }
+  int tune_plans() {
+    const PlanKey key{Nx, Ny, Nz, P.elastic, extended ? 1 : 0};
+    std::lock_guard<std::mutex> lk(plan_mu());
+    const auto it = plan_cache().find(key);
+    if (it != plan_cache().end() && !std::getenv("PF_PLAN")) {
+      zb_sel = it->second[0];
+      ty_sel = it->second[1];
+      tx_sel = it->second[2];
+      return PF_OK;
+    }
+    const int rc = tune_plans_timed();
+    if (rc == PF_OK && !std::getenv("PF_PLAN")) plan_cache()[key] = {zb_sel, ty_sel, tx_sel};
+    return rc;
+  }
+  int tune_plans_timed() {
+    if (const char* e = std::getenv("PF_PLAN")) {
+      int z = 1, y = 0, x = 0;
+      if (std::sscanf(e, "%d,%d,%d", &z, &y, &x) != 3 || z < 0 || z > 3 || y < 0 || y > 3 || x < 0 || x > 3)
+        return fail(PF_E_ARG, "PF_PLAN must be <z>,<y>,<x> with indices 0..3");
+      zb_sel = z;
+      ty_sel = y;
+      tx_sel = x;
+      return PF_OK;
+    }
+    if (R < 32768) return PF_OK;   // small grids keep the default tiles (launch latency dominates)
+    auto timed = [&](std::initializer_list<int> ids, double* out) -> int {
+      double tot = 0.0;
+      for (int id : ids) {
+        double us = 0.0, b = 0.0;
+        int rc = time_kernel(id, 3, &us, &b);
+        if (rc) return rc;
+        tot += us;
+      }
+      *out = tot;
+      return PF_OK;
+    };
+    int rc;
+    if (P.elastic) {
+      int seen[4], ns = 0, best = 1;
+      double bt = 0.0;
+      for (int b = 0; b < 4; ++b) {
+        const int rows = zpass_rows_of<T, 6, 6, OpHijInc<T>>(b, Nz);
+        bool dup = false;
+        for (int k = 0; k < ns; ++k) dup |= seen[k] == rows;
+        if (dup) continue;
+        seen[ns++] = rows;
+        zb_sel = b;
+        double us;
+        if ((rc = timed({PF_K_ZPASS_HIJ}, &us))) return rc;
+        if (ns == 1 || us < bt) {
+          bt = us;
+          best = b;
+        }
+      }
+      zb_sel = best;
+      plan_us[0] = bt;
+    }
+    for (int axis = 0; axis < 2; ++axis) {
+      const int N = axis == 0 ? Ny : Nx;
+      int* sel = axis == 0 ? &ty_sel : &tx_sel;
+      int seen[4], ns = 0, best = 0;
+      double bt = 0.0;
+      for (int t = 0; t < 4; ++t) {
+        const int cols = cpass_cols_of<T>(t, N);
+        bool dup = false;
+        for (int k = 0; k < ns; ++k) dup |= seen[k] == cols;
+        if (dup) continue;
+        seen[ns++] = cols;
+        *sel = t;
+        double us;
+        if ((rc = axis == 0 ? timed({PF_K_YPASS_FWD6, PF_K_YPASS_INV6}, &us)
+                            : timed({PF_K_XPASS_FWD6, PF_K_XPASS_INV6}, &us)))
+          return rc;
+        if (ns == 1 || us < bt) {
+          bt = us;
+          best = t;
+        }
+      }
+      *sel = best;
+      plan_us[1 + axis] = bt;
+    }
+    if ((rc = clear_spectra())) return rc;
+    PF_HIP(hipStreamSynchronize(stream));
+    return PF_OK;
+  }
+
   // ---------------------------------------------------------------- helpers
   C* sz(int c) { return Sz + (size_t)c * nSz; }
   C* sy(int c) { return Sy + (size_t)c * nSy; }
@@ -1325,7 +1502,7 @@ struct Engine final : EngineBase {
       ca.out_pstride = (long)ay.A * AzS;
       ca.in_cstride = nSz;
       ca.out_cstride = nSy;
-      launch_cpass<T, false>(Ny, stream, ca, Nx, nc);
+      launch_cpass<T, false>(Ny, stream, ca, Nx, nc, ty_sel);
     } else {
       ca.in = sy(c0);
       ca.out = sz(c0);
@@ -1333,7 +1510,7 @@ struct Engine final : EngineBase {
       ca.out_pstride = (long)Ny * AzS;
       ca.in_cstride = nSy;
       ca.out_cstride = nSz;
-      launch_cpass<T, true>(Ny, stream, ca, Nx, nc);
+      launch_cpass<T, true>(Ny, stream, ca, Nx, nc, ty_sel);
     }
   }
   // x pass: Sy <-> compact (dst/src compact base with component stride nSc)
@@ -1350,13 +1527,13 @@ struct Engine final : EngineBase {
       ca.out = compact;
       ca.in_cstride = nSy;
       ca.out_cstride = nSc;
-      launch_cpass<T, false>(Nx, stream, ca, 1, nc);
+      launch_cpass<T, false>(Nx, stream, ca, 1, nc, tx_sel);
     } else {
       ca.in = compact;
       ca.out = sy(c0);
       ca.in_cstride = nSc;
       ca.out_cstride = nSy;
-      launch_cpass<T, true>(Nx, stream, ca, 1, nc);
+      launch_cpass<T, true>(Nx, stream, ca, 1, nc, tx_sel);
     }
   }
 
@@ -1575,10 +1752,10 @@ struct Engine final : EngineBase {
       ypass(true, 0, 6);
       if (mode == 0) {
         typename OpHij<T>::Args oa{P, phi[cur], xB_el(), invN, 0, hcache};
-        launch_zpass<T, 6, 6, OpHij<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
+        launch_zpass_b<T, 6, 6, OpHij<T>>(zb_sel, Nz, stream, zargs(0, 6, 0, 6), oa);
       } else {
         typename OpHijInc<T>::Args oa{P, hcache, invN};
-        launch_zpass<T, 6, 6, OpHijInc<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
+        launch_zpass_b<T, 6, 6, OpHijInc<T>>(zb_sel, Nz, stream, zargs(0, 6, 0, 6), oa);
       }
       ypass(false, 0, 6);
       xpass(false, 0, 6, Sc);
@@ -1694,7 +1871,7 @@ struct Engine final : EngineBase {
     while (true) {
       const int mode = !accel ? 3 : (it == 0 ? 1 : 2);
       typename OpHij<T>::Args oa{P, phi[cur], xB_el(), invN, mode == 2 ? 1 : 0};
-      launch_zpass<T, 6, 6, OpHij<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
+      launch_zpass_b<T, 6, 6, OpHij<T>>(zb_sel, Nz, stream, zargs(0, 6, 0, 6), oa);
       ypass(false, 0, 6);
       xpass(false, 0, 6, Sc);
       green(mode, accel);
@@ -1794,7 +1971,7 @@ struct Engine final : EngineBase {
     const T invN = T(1.0 / (double)R);
     while (true) {
       typename OpHij<T>::Args oa{P, phi[cur], xB_el(), invN, 0};
-      launch_zpass<T, 6, 6, OpHij<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
+      launch_zpass_b<T, 6, 6, OpHij<T>>(zb_sel, Nz, stream, zargs(0, 6, 0, 6), oa);
       ypass(false, 0, 6);
       xpass(false, 0, 6, Sc);
       GreenArgs<T> ga{};
@@ -3005,6 +3182,9 @@ struct Engine final : EngineBase {
     s->elastic_iterations_total = elastic_total;
     s->zero_mode_target_mass = prm.zero_mode ? target_mass : NAN;
     s->kz_stride = AzS;
+    s->zpass_rows = zpass_rows_of<T, 6, 6, OpHijInc<T>>(zb_sel, Nz);
+    s->ypass_cols = cpass_cols_of<T>(ty_sel, Ny);
+    s->xpass_cols = cpass_cols_of<T>(tx_sel, Nx);
     return PF_OK;
   }
 
@@ -3161,6 +3341,9 @@ struct Engine final : EngineBase {
           PF_HIP(hipMemcpy(x.states.data() + (size_t)(1 + k) * 6 * nScA, hist_slot((hist_head + 2 - k) % 3),
                            3 * nScA * sizeof(cx<double>), hipMemcpyDeviceToHost));
       }
+      x.plan[0] = zb_sel;
+      x.plan[1] = ty_sel;
+      x.plan[2] = tx_sel;
       if (phi_block) {
         x.phi_star.resize(R);
         x.phi_spec.resize(2 * (size_t)nScA);
@@ -3200,6 +3383,11 @@ struct Engine final : EngineBase {
     // the companion belongs to this file and this handle
     const bool x_ok = have_x && x.bound_checksum == c.payload_checksum && x.accepted_step == c.accepted_step &&
                       x.n == (std::uint64_t)nScA && x.precision == (kMixed ? PF_MIXED : PF_FP64);
+    if (x_ok && x.plan[0] >= 0) {   // the run continues with the tiles it was computed with
+      zb_sel = x.plan[0];
+      ty_sel = x.plan[1];
+      tx_sel = x.plan[2];
+    }
     PF_HIP(hipStreamSynchronize(stream));
     // from here on a rejection leaves the handle without a valid state
     state_ok = false;
@@ -3392,10 +3580,10 @@ struct Engine final : EngineBase {
         const int inc = (kMixed || extended) ? 1 : 0;
         if (extended) {
           typename OpHijInc<T>::Args oa{P, hcache, invN};
-          launch_zpass<T, 6, 6, OpHijInc<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
+          launch_zpass_b<T, 6, 6, OpHijInc<T>>(zb_sel, Nz, stream, zargs(0, 6, 0, 6), oa);
         } else {
           typename OpHij<T>::Args oa{P, phi[cur], xB, invN, inc, nullptr};
-          launch_zpass<T, 6, 6, OpHij<T>>(Nz, stream, zargs(0, 6, 0, 6), oa);
+          launch_zpass_b<T, 6, 6, OpHij<T>>(zb_sel, Nz, stream, zargs(0, 6, 0, 6), oa);
         }
         return 12.0 * nSzA * cs + (inc ? (extended ? (double)sizeof(T) : 8.0) : 16.0) * R;
       }

@@ -128,12 +128,12 @@ struct ZBudget<Op, std::void_t<decltype(Op::LDS_BUDGET)>> {
   static constexpr int value = Op::LDS_BUDGET;
 };
 
-template <typename T, int N, int NCI, int NCO, class Op>
+template <typename T, int N, int NCI, int NCO, class Op, int BUDGET = ZBudget<Op>::value>
 __device__ __forceinline__ void zpass_body(const ZPassArgs<T>& za, const typename Op::Args& oa, const B3 blockIdx) {
   if (za.gate && *za.gate == 0.0) return;
   constexpr int NCM = NCI > NCO ? NCI : NCO;
   constexpr int NCB = NCM > 0 ? NCM : 1;
-  constexpr int RT = zpass_rows<T, N, NCB, Op::NPRE, ZBudget<Op>::value>();
+  constexpr int RT = zpass_rows<T, N, NCB, Op::NPRE, BUDGET>();
   constexpr int NP = RT / 2;
   constexpr bool BIG = plan_needs_alt(N);
   using LY = Lds<N, NCB * NP, false>;   // row-major: layout independent of the batch
@@ -325,21 +325,24 @@ __device__ __forceinline__ void zpass_body(const ZPassArgs<T>& za, const typenam
   }
 }
 
-template <typename T, int N, int NCI, int NCO, class Op>
+template <typename T, int N, int NCI, int NCO, class Op, int BUDGET = ZBudget<Op>::value>
 __global__ void __launch_bounds__(NT) zpass_kernel(ZPassArgs<T> za, typename Op::Args oa) {
-  zpass_body<T, N, NCI, NCO, Op>(za, oa, PF_BIDX);
+  zpass_body<T, N, NCI, NCO, Op, BUDGET>(za, oa, PF_BIDX);
 }
 // launch descriptor (pf_launch.hpp)
-template <typename T, int N, int NCI, int NCO, class Op>
+template <typename T, int N, int NCI, int NCO, class Op, int BUDGET = ZBudget<Op>::value>
 struct KZPass {
   struct Args {
     ZPassArgs<T> za;
     typename Op::Args oa;
   };
   static constexpr int BLOCK = NT;
-  __device__ static void body(const Args& a, B3 bid, B3) { zpass_body<T, N, NCI, NCO, Op>(a.za, a.oa, bid); }
+  __device__ static void body(const Args& a, B3 bid, B3) {
+    zpass_body<T, N, NCI, NCO, Op, BUDGET>(a.za, a.oa, bid);
+  }
   static void direct(hipStream_t s, B3 g, const Args& a) {
-    hipLaunchKernelGGL((zpass_kernel<T, N, NCI, NCO, Op>), dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.za, a.oa);
+    hipLaunchKernelGGL((zpass_kernel<T, N, NCI, NCO, Op, BUDGET>), dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.za,
+                       a.oa);
   }
 };
 
@@ -358,10 +361,9 @@ struct CPassArgs {
   const double* gate;       // when set, the launch does nothing while *gate == 0
 };
 
-template <typename T, int N, bool INV>
+template <typename T, int N, bool INV, int TC = cpass_cols<T, N>()>
 __device__ __forceinline__ void cpass_body(const CPassArgs<T>& ca, const B3 blockIdx) {
   if (ca.gate && *ca.gate == 0.0) return;
-  constexpr int TC = cpass_cols<T, N>();
   constexpr bool BIG = plan_needs_alt(N);
   using LY = Lds<N, TC, true>;
   __shared__ cx<T> buf[LY::size];
@@ -432,18 +434,31 @@ __device__ __forceinline__ void cpass_body(const CPassArgs<T>& ca, const B3 bloc
   }
 }
 
-template <typename T, int N, bool INV>
+template <typename T, int N, bool INV, int TC = cpass_cols<T, N>()>
 __global__ void __launch_bounds__(NT) cpass_kernel(CPassArgs<T> ca) {
-  cpass_body<T, N, INV>(ca, PF_BIDX);
+  cpass_body<T, N, INV, TC>(ca, PF_BIDX);
 }
-template <typename T, int N, bool INV>
+template <typename T, int N, bool INV, int TC = cpass_cols<T, N>()>
 struct KCPass {
   using Args = CPassArgs<T>;
   static constexpr int BLOCK = NT;
-  __device__ static void body(const Args& a, B3 bid, B3) { cpass_body<T, N, INV>(a, bid); }
+  __device__ static void body(const Args& a, B3 bid, B3) { cpass_body<T, N, INV, TC>(a, bid); }
   static void direct(hipStream_t s, B3 g, const Args& a) {
-    hipLaunchKernelGGL((cpass_kernel<T, N, INV>), dim3(g.x, g.y, g.z), dim3(NT), 0, s, a);
+    hipLaunchKernelGGL((cpass_kernel<T, N, INV, TC>), dim3(g.x, g.y, g.z), dim3(NT), 0, s, a);
   }
 };
+
+// Column-tile widths the plan tuner tries (0: the width cpass_cols picks),
+// and whether a width fits the workgroup's LDS (at most 96 KB with the
+// twiddles).
+constexpr int CPASS_TILES[4] = {0, 8, 16, 32};
+template <typename T, int N, int TC>
+constexpr bool cpass_tile_fits() {
+  return TC == 0 || (Lds<N, TC, true>::size * (plan_needs_alt(N) ? 2 : 1) + N) * (int)sizeof(cx<T>) <= 96 * 1024;
+}
+
+// z-pass LDS budgets (bytes of FFT data per workgroup) the plan tuner tries
+// for the solver's z pass; index 1 is the default
+constexpr int ZPASS_BUDGETS[4] = {16 * 1024, ZPASS_LDS_BUDGET, 64 * 1024, 96 * 1024};
 
 }  // namespace pf

@@ -72,6 +72,7 @@ class PfScalars(C.Structure):
         ("last_elastic_iterations", C.c_int), ("zero_mode_iterations", C.c_int),
         ("steps_done", C.c_longlong), ("elastic_iterations_total", C.c_longlong),
         ("n_in", C.c_double), ("n_if", C.c_double), ("zero_mode_target_mass", C.c_double), ("kz_stride", C.c_int),
+        ("zpass_rows", C.c_int), ("ypass_cols", C.c_int), ("xpass_cols", C.c_int),
     ]
 
     def as_dict(self):

@@ -115,6 +115,9 @@ typedef struct pf_scalars {
                                    (main_cuda.cu:28080-28105) */
   int kz_stride;              /* kz stride of the spectral arrays the plan tuner chose at pf_create
                                  (whole 128- or 64-byte lines or none; bitwise-neutral) */
+  int zpass_rows;             /* rows per workgroup of the elastic solver's z pass, and the column-tile */
+  int ypass_cols;             /* widths of the y and x passes, as the plan tuner chose them at pf_create */
+  int xpass_cols;             /* (bitwise-neutral; PF_PLAN=<z>,<y>,<x> fixes them) */
 } pf_scalars;
 
 typedef struct pf_handle pf_handle;
