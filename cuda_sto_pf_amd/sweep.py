@@ -79,6 +79,8 @@ def main(argv=None):
     ap.add_argument("--learned", action="store_true", help="learned cold start of every instance")
     ap.add_argument("--subset", type=int, default=0, help="only the first n instances")
     ap.add_argument("--report", default=None)
+    ap.add_argument("--modes", default="alone,streams,fused",
+                    help="comma list of alone, streams, fused (fused alone: for profiling)")
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)
     inst = sweep_instances()
@@ -91,6 +93,14 @@ def main(argv=None):
         net = learned.load()
     opts = pf.minimize_opts(dt=a.dt, V0=a.V0, max_iter=a.max_iter)
     log(f"[sweep] {len(inst)} instances, {a.precision}, max_iter {a.max_iter}, learned={a.learned}")
+    modes = set(a.modes.split(","))
+    if modes == {"fused"}:
+        lc = {}
+        t_fus, st_fus, _ = run(inst, 1, opts, prec, fused=True, launches=lc)
+        it_fus = sum(s["iteration"] for s in st_fus)
+        log(f"[sweep] fused: {t_fus:.2f} s, {it_fus / t_fus:.0f} iterations/s, {lc['batched']} batched launches "
+            f"for {lc['recorded']} recorded, {lc['graph_launches']} graph launches of {lc['graphs']} graphs")
+        return 0
     t_seq, st_seq, phi_seq = run(inst, 1, opts, prec, net=net)
     it_seq = sum(s["iteration"] for s in st_seq)
     log(f"[sweep] one at a time: {t_seq:.2f} s, {it_seq} iterations, {it_seq / t_seq:.0f} iterations/s")
