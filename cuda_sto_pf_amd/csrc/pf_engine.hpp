@@ -362,9 +362,7 @@ __device__ __forceinline__ void zm_terms(double ys, double ph, double lam, doubl
 }
 
 // opening pass: lambda = lo, hi, clamp(0, lo, hi) from the Y* extremes ymm[0..1]
-__global__ void __launch_bounds__(NT) zm_eval3_kernel(const double* Ystar, const double* phi, const double* ymm,
-                                                      double Y_clip, double Y_cap, double vB, long n,
-                                                      double* partials) {
+__device__ __forceinline__ void zm_eval3_kernel_body(const double* Ystar, const double* phi, const double* ymm, double Y_clip, double Y_cap, double vB, long n, double* partials, const B3 blockIdx, const B3 gridDim) {
   const double lo = -Y_clip - ymm[0];
   const double hi = Y_cap - ymm[1];
   const double lam0 = fmin(fmax(0.0, lo), hi);
@@ -393,11 +391,30 @@ __global__ void __launch_bounds__(NT) zm_eval3_kernel(const double* Ystar, const
   for (int r = 0; r < 9; ++r) kinds[r] = RED_SUM;
   block_reduce_store<9>(acc, kinds, partials + (size_t)blockIdx.x * 9);
 }
+__global__ void __launch_bounds__(NT) zm_eval3_kernel(const double* Ystar, const double* phi, const double* ymm, double Y_clip, double Y_cap, double vB, long n, double* partials) {
+  zm_eval3_kernel_body(Ystar, phi, ymm, Y_clip, Y_cap, vB, n, partials, PF_BIDX, PF_GDIM);
+}
+struct KZmEval3 {
+  struct Args {
+    const double* Ystar;
+    const double* phi;
+    const double* ymm;
+    double Y_clip;
+    double Y_cap;
+    double vB;
+    long n;
+    double* partials;
+  };
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3 b, B3 g) { zm_eval3_kernel_body(a.Ystar, a.phi, a.ymm, a.Y_clip, a.Y_cap, a.vB, a.n, a.partials, b, g); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL(zm_eval3_kernel, dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.Ystar, a.phi, a.ymm, a.Y_clip, a.Y_cap, a.vB, a.n, a.partials);
+  }
+};
+
 
 // one search step at st[ZS_NEXT]
-__global__ void __launch_bounds__(NT) zm_eval_kernel(const double* Ystar, const double* phi, const double* st,
-                                                     double Y_clip, double Y_cap, double vB, long n,
-                                                     double* partials) {
+__device__ __forceinline__ void zm_eval_kernel_body(const double* Ystar, const double* phi, const double* st, double Y_clip, double Y_cap, double vB, long n, double* partials, const B3 blockIdx, const B3 gridDim) {
   if (st[ZS_ACTIVE] == 0.0) return;
   const double lam = st[ZS_NEXT];
   double acc[3] = {0.0, 0.0, 0.0};
@@ -406,6 +423,27 @@ __global__ void __launch_bounds__(NT) zm_eval_kernel(const double* Ystar, const 
   int kinds[3] = {RED_SUM, RED_SUM, RED_SUM};
   block_reduce_store<3>(acc, kinds, partials + (size_t)blockIdx.x * 3);
 }
+__global__ void __launch_bounds__(NT) zm_eval_kernel(const double* Ystar, const double* phi, const double* st, double Y_clip, double Y_cap, double vB, long n, double* partials) {
+  zm_eval_kernel_body(Ystar, phi, st, Y_clip, Y_cap, vB, n, partials, PF_BIDX, PF_GDIM);
+}
+struct KZmEval {
+  struct Args {
+    const double* Ystar;
+    const double* phi;
+    const double* st;
+    double Y_clip;
+    double Y_cap;
+    double vB;
+    long n;
+    double* partials;
+  };
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3 b, B3 g) { zm_eval_kernel_body(a.Ystar, a.phi, a.st, a.Y_clip, a.Y_cap, a.vB, a.n, a.partials, b, g); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL(zm_eval_kernel, dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.Ystar, a.phi, a.st, a.Y_clip, a.Y_cap, a.vB, a.n, a.partials);
+  }
+};
+
 
 // the loop condition and the next candidate (main_cuda.cu:4370-4390)
 __device__ void zm_propose(double* st, const ZmConst& c) {
@@ -494,8 +532,7 @@ __device__ void zm_ctrl_step(double* st, ZmConst c) {
 // The evaluation's sums into the search state, then the bracket set-up
 // (open) or one safeguarded step (step); one workgroup.  A step launched
 // after the search has ended does nothing.
-__global__ void __launch_bounds__(NT) zm_open_end_kernel(const double* partials, int nblocks, double* st,
-                                                         const double* ymm, ZmConst c) {
+__device__ __forceinline__ void zm_open_end_kernel_body(const double* partials, int nblocks, double* st, const double* ymm, ZmConst c, const B3 blockIdx, const B3 gridDim) {
   __shared__ double outv[16];
   __shared__ double ss[ZS_COUNT];   // the search state, worked on in LDS by one thread
   for (int i = threadIdx.x; i < ZS_COUNT; i += NT) ss[i] = st[i];
@@ -507,8 +544,25 @@ __global__ void __launch_bounds__(NT) zm_open_end_kernel(const double* partials,
   __syncthreads();
   for (int i = threadIdx.x; i < ZS_COUNT; i += NT) st[i] = ss[i];
 }
-__global__ void __launch_bounds__(NT) zm_step_end_kernel(const double* partials, int nblocks, double* st,
-                                                         ZmConst c) {
+__global__ void __launch_bounds__(NT) zm_open_end_kernel(const double* partials, int nblocks, double* st, const double* ymm, ZmConst c) {
+  zm_open_end_kernel_body(partials, nblocks, st, ymm, c, PF_BIDX, PF_GDIM);
+}
+struct KZmOpenEnd {
+  struct Args {
+    const double* partials;
+    int nblocks;
+    double* st;
+    const double* ymm;
+    ZmConst c;
+  };
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3 b, B3 g) { zm_open_end_kernel_body(a.partials, a.nblocks, a.st, a.ymm, a.c, b, g); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL(zm_open_end_kernel, dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.partials, a.nblocks, a.st, a.ymm, a.c);
+  }
+};
+
+__device__ __forceinline__ void zm_step_end_kernel_body(const double* partials, int nblocks, double* st, ZmConst c, const B3 blockIdx, const B3 gridDim) {
   if (st[ZS_ACTIVE] == 0.0) return;
   __shared__ double outv[16];
   __shared__ double ss[ZS_COUNT];
@@ -521,6 +575,23 @@ __global__ void __launch_bounds__(NT) zm_step_end_kernel(const double* partials,
   __syncthreads();
   for (int i = threadIdx.x; i < ZS_COUNT; i += NT) st[i] = ss[i];
 }
+__global__ void __launch_bounds__(NT) zm_step_end_kernel(const double* partials, int nblocks, double* st, ZmConst c) {
+  zm_step_end_kernel_body(partials, nblocks, st, c, PF_BIDX, PF_GDIM);
+}
+struct KZmStepEnd {
+  struct Args {
+    const double* partials;
+    int nblocks;
+    double* st;
+    ZmConst c;
+  };
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3 b, B3 g) { zm_step_end_kernel_body(a.partials, a.nblocks, a.st, a.c, b, g); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL(zm_step_end_kernel, dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.partials, a.nblocks, a.st, a.c);
+  }
+};
+
 
 // ---- accelerated elastic solver, decided on the device (control block AaCtl)
 struct AaConst {
@@ -699,8 +770,7 @@ struct KAaEnd {
 // 3434-3443), dY/dt = (Y - Y^n)/dt (update_dY_dt_prev_kernel 3545-3556) when
 // dYdt is given; the minimize mass constraint keeps the dY/dt of its clamp
 // (main_cuda.cu:31622 runs before the constraint at 32297).
-__global__ void __launch_bounds__(NT) zm_apply_kernel(const double* Ystar, double lam, double* Y,
-                                                      double* xB, double* dYdt, double dt, long n) {
+__device__ __forceinline__ void zm_apply_kernel_body(const double* Ystar, double lam, double* Y, double* xB, double* dYdt, double dt, long n, const B3 blockIdx, const B3 gridDim) {
   for (long g = blockIdx.x * (long)NT + threadIdx.x; g < n; g += (long)gridDim.x * NT) {
     const double y = Ystar[g] + lam;
     if (dYdt) dYdt[g] = (y - Y[g]) / dt;
@@ -708,6 +778,26 @@ __global__ void __launch_bounds__(NT) zm_apply_kernel(const double* Ystar, doubl
     xB[g] = sigmoid_unclipped(y);
   }
 }
+__global__ void __launch_bounds__(NT) zm_apply_kernel(const double* Ystar, double lam, double* Y, double* xB, double* dYdt, double dt, long n) {
+  zm_apply_kernel_body(Ystar, lam, Y, xB, dYdt, dt, n, PF_BIDX, PF_GDIM);
+}
+struct KZmApply {
+  struct Args {
+    const double* Ystar;
+    double lam;
+    double* Y;
+    double* xB;
+    double* dYdt;
+    double dt;
+    long n;
+  };
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3 b, B3 g) { zm_apply_kernel_body(a.Ystar, a.lam, a.Y, a.xB, a.dYdt, a.dt, a.n, b, g); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL(zm_apply_kernel, dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.Ystar, a.lam, a.Y, a.xB, a.dYdt, a.dt, a.n);
+  }
+};
+
 
 }  // namespace
 
@@ -1837,7 +1927,7 @@ struct Engine final : EngineBase {
       ga.partials = resid ? partials : nullptr;
       for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
       ga.mode = mode;
-      hipLaunchKernelGGL(kop_green_acc<T>, dim3(gk), dim3(NT), 0, stream, ga);
+      hipLaunchKernelGGL((kop_green_acc<T>), dim3(gk), dim3(NT), 0, stream, ga);
     };
     auto strain_of_U = [&]() {
       StrainArgs<T, double> sa{};
@@ -1944,7 +2034,7 @@ struct Engine final : EngineBase {
       ga.partials = nullptr;
       for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
       ga.have_old = 0;
-      hipLaunchKernelGGL(kop_green<T>, dim3(gk), dim3(NT), 0, stream, ga);
+      hipLaunchKernelGGL((kop_green<T>), dim3(gk), dim3(NT), 0, stream, ga);
       if (cold_hooks()) {
         int rc = cold_hook();
         if (rc) return rc;
@@ -1953,7 +2043,7 @@ struct Engine final : EngineBase {
         sa.n = nSc;
         for (int i = 0; i < 3; ++i) sa.u[i] = uu(u, i);
         for (int i = 0; i < 6; ++i) sa.eps[i] = sc(i);
-        hipLaunchKernelGGL(kop_strain<T>, dim3(gmodes()), dim3(NT), 0, stream, sa);
+        hipLaunchKernelGGL((kop_strain<T>), dim3(gmodes()), dim3(NT), 0, stream, sa);
       }
     } else {
       StrainArgs<T> sa{};
@@ -1961,7 +2051,7 @@ struct Engine final : EngineBase {
       sa.n = nSc;
       for (int i = 0; i < 3; ++i) sa.u[i] = uu(u, i);
       for (int i = 0; i < 6; ++i) sa.eps[i] = sc(i);
-      hipLaunchKernelGGL(kop_strain<T>, dim3(gmodes()), dim3(NT), 0, stream, sa);
+      hipLaunchKernelGGL((kop_strain<T>), dim3(gmodes()), dim3(NT), 0, stream, sa);
     }
     xpass(true, 0, 6, Sc);
     ypass(true, 0, 6);
@@ -1988,7 +2078,7 @@ struct Engine final : EngineBase {
       for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
       ga.have_old = 1;
       ga.partials = accel ? partials : nullptr;
-      hipLaunchKernelGGL(kop_green<T>, dim3(gk), dim3(NT), 0, stream, ga);
+      hipLaunchKernelGGL((kop_green<T>), dim3(gk), dim3(NT), 0, stream, ga);
       if (accel) finalize(gk, 2, RedKinds{{RED_SUM, RED_SUM, 0, 0}}, dscal, 1.0);
       if (accel)
         PF_HIP(hipMemcpyAsync(hscal, dscal, 2 * sizeof(double), hipMemcpyDeviceToHost, stream));
@@ -2033,16 +2123,48 @@ struct Engine final : EngineBase {
   }
 
   // ---------------------------------------------------------------- step
+  // One time step.  It runs in three phases around its host waits (the
+  // elastic solver's and the zero mode's), so that a batch runner can drive
+  // many handles through the same waits:
+  //   step_phase_a  the start of the elastic solve (the whole solve for the
+  //                 solvers other than the accelerated one);
+  //   (the accelerated solve's further iterations: accel_more / accel_iters)
+  //   step_phase_b  the solve's result, the phi update and the composition
+  //                 update through the opening of the zero-mode search;
+  //   (the search's further steps: zm_more / zm_steps)
+  //   step_phase_c  the search's result and the shift, the step's bookkeeping.
+  static constexpr int STEP_NFETCH = 13;   // leading scalars the step reads back
   int step() override {
+    int rc = step_phase_a();
+    if (rc) return rc;
+    if (P.elastic && extended) {
+      for (;;) {
+        PF_HIP(hipMemcpyAsync(hscal + AA_SLOT, dscal + AA_SLOT, AC_GAM * sizeof(double), hipMemcpyDeviceToHost,
+                              stream));
+        PF_HIP(hipStreamSynchronize(stream));
+        if (!accel_more()) break;
+        accel_iters(2);
+      }
+    }
+    if ((rc = step_phase_b())) return rc;
+    const bool zm = prm.zero_mode != 0;
+    for (;;) {
+      PF_HIP(hipMemcpyAsync(hscal, dscal, STEP_NFETCH * sizeof(double), hipMemcpyDeviceToHost, stream));
+      if (zm)
+        PF_HIP(hipMemcpyAsync(hscal + ZM_SLOT, dscal + ZM_SLOT, ZS_SUMS * sizeof(double), hipMemcpyDeviceToHost,
+                              stream));
+      PF_HIP(hipStreamSynchronize(stream));
+      if (!zm || !zm_more()) break;
+      zm_steps(4);
+    }
+    return step_phase_c();
+  }
+
+  int step_phase_a() {
     int rc = require_state("pf_step");
     if (rc) return rc;
     state_ok = false;
     minimizing = false;
-    const T invN = T(1.0 / (double)R);
-    const int rows = Nx * Ny;
-    const bool zm = prm.zero_mode != 0;
-    double* phi_old = phi[cur];
-    double* phi_new = phi[1 - cur];
     // The gradient energy of phi^n is a diagnostic (the reference's dynamics
     // loop does not evaluate it, main_cuda.cu:29261-29290 is minimize only):
     // it is computed when the scalars are read, from phi^n, which stays in
@@ -2051,11 +2173,23 @@ struct Engine final : EngineBase {
     egrad_pending = false;
     eng_pending = false;
     sig6_valid = false;
+    if (P.elastic) {
+      if ((rc = extended ? accel_begin() : elastic_solve())) return rc;
+    }
+    return PF_OK;
+  }
+
+  int step_phase_b() {
+    int rc;
+    const T invN = T(1.0 / (double)R);
+    const bool zm = prm.zero_mode != 0;
+    double* phi_old = phi[cur];
+    double* phi_new = phi[1 - cur];
     // mixed precision: 1 = first update from new fields (direct, sets phi*),
     // 2 = increment form; 0 in FP64
     const int phi_mode = kMixed ? (phi_banded ? 2 : 1) : 0;
     if (P.elastic) {
-      if ((rc = elastic_solve())) return rc;
+      if (extended && (rc = accel_finish())) return rc;
       typename OpPhiRhs<T, true>::Args oa{P, phi_old, xB, trs, invN, keep_sig6 ? sig6 : nullptr, (size_t)R,
                                           phi_mode == 2 ? phis : nullptr};
       ZPassArgs<T> za = zargs(0, 6, 0, 1);
@@ -2073,11 +2207,10 @@ struct Engine final : EngineBase {
     ypass(false, 0, 1);
     xpass(false, 0, 1, Sc);
     if (phi_mode)
-      hipLaunchKernelGGL(kop_phi_state<T>, dim3(gmodes()), dim3(NT), 0, stream, d, nSc, sc(0),
-                         prm.L_phi * prm.dt * prm.kappa_phi, Phik, phi_mode);
+      launch<KPhiState<T>>(stream, dim3(gmodes()),
+                           {d, nSc, sc(0), prm.L_phi * prm.dt * prm.kappa_phi, Phik, phi_mode});
     else
-      hipLaunchKernelGGL(kop_phi<T>, dim3(gmodes()), dim3(NT), 0, stream, d, nSc, sc(0),
-                         prm.L_phi * prm.dt * prm.kappa_phi);
+      launch<KPhi<T>>(stream, dim3(gmodes()), {d, nSc, sc(0), prm.L_phi * prm.dt * prm.kappa_phi});
     xpass(true, 0, 1, Sc);
     ypass(true, 0, 1);
     {
@@ -2092,7 +2225,13 @@ struct Engine final : EngineBase {
     // (1 + dt D-bar k^2) rather than Y itself, so their rounding scales with
     // the increment (kop_yupdate), and Y^n needs no transform.
     const bool yinc = kMixed && zm && y_banded;
-    if ((rc = y_chain(P, phi_old, phi_new, yinc, P.Y_clip, 13))) return rc;
+    y_chain_launch(P, phi_old, phi_new, yinc, P.Y_clip);
+    return PF_OK;
+  }
+
+  int step_phase_c() {
+    int rc;
+    if ((rc = y_chain_finish())) return rc;
     cur = 1 - cur;
     egrad_pending = true;
     eng_pending = true;
@@ -2112,13 +2251,29 @@ struct Engine final : EngineBase {
   // nfetch device scalars come back with the result.
   int y_chain(const PhysParams& PY, const double* phi_old, const double* phi_new, bool yinc, double y_cap,
               int nfetch) {
+    y_chain_launch(PY, phi_old, phi_new, yinc, y_cap);
+    const bool zm = prm.zero_mode != 0;
+    for (;;) {
+      PF_HIP(hipMemcpyAsync(hscal, dscal, nfetch * sizeof(double), hipMemcpyDeviceToHost, stream));
+      if (zm)
+        PF_HIP(hipMemcpyAsync(hscal + ZM_SLOT, dscal + ZM_SLOT, ZS_SUMS * sizeof(double), hipMemcpyDeviceToHost,
+                              stream));
+      PF_HIP(hipStreamSynchronize(stream));
+      if (!zm || !zm_more()) break;
+      zm_steps(4);
+    }
+    return y_chain_finish();
+  }
+  // the launches of the composition update, through the opening of the
+  // zero-mode search (or the clamp without it)
+  void y_chain_launch(const PhysParams& PY, const double* phi_old, const double* phi_new, bool yinc,
+                      double y_cap) {
     const T invN = T(1.0 / (double)R);
     const KDims d = kd();
     const bool zm = prm.zero_mode != 0;
-    int rc = PF_OK;
     ypass(false, 1, 1);
     xpass(false, 1, 1, sc(1));
-    hipLaunchKernelGGL(kop_grad<T>, dim3(gmodes()), dim3(NT), 0, stream, d, nSc, sc(1), sc(2), sc(3), sc(4));
+    launch<KGrad<T>>(stream, dim3(gmodes()), {d, nSc, sc(1), sc(2), sc(3), sc(4)});
     xpass(true, 2, 3, sc(2));
     ypass(true, 2, 3);
     {
@@ -2138,25 +2293,30 @@ struct Engine final : EngineBase {
     const int nyc = yinc ? 4 : 5;
     ypass(false, 0, nyc);
     xpass(false, 0, nyc, Sc);
-    hipLaunchKernelGGL(kop_yupdate<T>, dim3(gmodes()), dim3(NT), 0, stream, d, nSc, sc(0), sc(1), sc(2),
-                       yinc ? (const C*)nullptr : (const C*)sc(3), yinc ? sc(3) : sc(4), sc(5),
-                       (const double*)(dscal + 4), PY.dt);
+    launch<KYUpdate<T>>(stream, dim3(gmodes()),
+                        {d, nSc, sc(0), sc(1), sc(2), yinc ? (const C*)nullptr : (const C*)sc(3),
+                         yinc ? sc(3) : sc(4), sc(5), (const double*)(dscal + 4), PY.dt});
     xpass(true, 5, 1, sc(5));
     ypass(true, 5, 1);
     if (zm) {
       typename OpYStar<T>::Args oa{Ystar, invN, yinc ? (const double*)Y : nullptr};
       const int g = launch_zpass<T, 1, 0, OpYStar<T>>(Nz, stream, zargs(5, 1, 0, 0), oa);
       finalize(g, 2, RedKinds{{RED_MIN, RED_MAX, 0, 0}}, dscal, 1.0);
-      // the step's scalars come back with the zero-mode search state
-      rc = zero_mode(phi_new, nfetch);
-      mean_DY = hscal[4];
-      if (rc) return rc;
+      zm_begin(phi_new, zm_step_const());
     } else {
       typename OpYFinal<T>::Args oa{PY, Y, xB, dYdt, invN, y_cap};
       launch_zpass<T, 1, 0, OpYFinal<T>>(Nz, stream, zargs(5, 1, 0, 0), oa);
-      if ((rc = fetch(nfetch))) return rc;
-      mean_DY = hscal[4];
     }
+  }
+  // after the scalars (and the search state) are mirrored: the search's
+  // result and the shift; mean D-bar
+  int y_chain_finish() {
+    int rc = PF_OK;
+    if (prm.zero_mode) {
+      ZmResult res;
+      rc = zero_mode_apply(zm_end(zm_step_const(), "PF zero-mode", &res), res);
+    }
+    mean_DY = hscal[4];
     return rc;
   }
 
@@ -2171,47 +2331,74 @@ struct Engine final : EngineBase {
   static constexpr int NSCAL = 512;    // device scalars (and their pinned host mirror)
   int aa_spec = 4;                     // iterations enqueued before the solver's first wait
   const double* gate_ = nullptr;       // passes do nothing while *gate_ == 0 (set during the solve)
+  using ZmResult = ZmResultT;
+  ZmConst zm_step_const() const {
+    return ZmConst{P.Y_clip, P.Y_clip, P.v_B, target_mass,
+                   prm.zero_mode_tolerance_relative * std::fmax(std::fabs(target_mass), 1.0),
+                   prm.zero_mode_max_iterations};
+  }
   int zero_mode(const double* phi_np1, int nfetch) {
-    const ZmConst c{P.Y_clip, P.Y_clip, P.v_B, target_mass,
-                    prm.zero_mode_tolerance_relative * std::fmax(std::fabs(target_mass), 1.0),
-                    prm.zero_mode_max_iterations};
     ZmResult res;
-    int rc = zero_mode_solve(phi_np1, nfetch, c, "PF zero-mode", &res);
+    const int rc = zero_mode_solve(phi_np1, nfetch, zm_step_const(), "PF zero-mode", &res);
+    return zero_mode_apply(rc, res);
+  }
+  // the time step's use of a finished search: its statistics and the shift
+  int zero_mode_apply(int rc, const ZmResult& res) {
     zm_iters = res.iters;
     if (rc) return rc;
     zm_lambda = res.lam;
     zm_res = res.r;
     zm_deriv = res.deriv;
     ++zm_accepted;
-    hipLaunchKernelGGL(zm_apply_kernel, dim3((unsigned)((R + NT - 1) / NT)), dim3(NT), 0, stream, Ystar, res.lam,
-                       Y, xB, dYdt, prm.dt, R);
+    launch<KZmApply>(stream, dim3((unsigned)((R + NT - 1) / NT)), {Ystar, res.lam, Y, xB, dYdt, prm.dt, R});
     return PF_OK;
   }
   // the search on Y* (member Ystar) with the extremes in dscal[0..1]; on
   // success res holds the accepted shift, nothing is applied yet
-  using ZmResult = ZmResultT;
-  int zero_mode_solve(const double* phi_np1, int nfetch, const ZmConst& c, const char* what, ZmResult* res) {
+  // The search runs in three parts so that a batch runner can drive it:
+  // zm_begin (the opening evaluation of both bracket ends and the start, and
+  // the first zm_spec Newton/bisection steps), zm_steps (more steps while the
+  // mirrored search state says it runs, zm_more) and zm_end (the host checks
+  // on the mirrored state).
+  ZmConst zm_c{};
+  const double* zm_phi = nullptr;
+  int zm_left = 0;
+  void zm_begin(const double* phi_np1, const ZmConst& c) {
     double* st = dscal + ZM_SLOT;
     const int g = grid_for(R);
-    hipLaunchKernelGGL(zm_eval3_kernel, dim3(g), dim3(NT), 0, stream, Ystar, phi_np1, (const double*)dscal,
-                       c.Y_clip, c.Y_cap, c.vB, R, partials);
-    hipLaunchKernelGGL(zm_open_end_kernel, dim3(1), dim3(NT), 0, stream, partials, g, st, (const double*)dscal,
-                       c);
-    int left = std::max(c.max_it, 0);
-    int batch = std::min(zm_spec, left);
-    for (;;) {
-      for (int k = 0; k < batch; ++k) {
-        hipLaunchKernelGGL(zm_eval_kernel, dim3(g), dim3(NT), 0, stream, Ystar, phi_np1, (const double*)st,
-                           c.Y_clip, c.Y_cap, c.vB, R, partials);
-        hipLaunchKernelGGL(zm_step_end_kernel, dim3(1), dim3(NT), 0, stream, partials, g, st, c);
-      }
-      left -= batch;
-      PF_HIP(hipMemcpyAsync(hscal, dscal, nfetch * sizeof(double), hipMemcpyDeviceToHost, stream));
-      PF_HIP(hipMemcpyAsync(hscal + ZM_SLOT, st, ZS_SUMS * sizeof(double), hipMemcpyDeviceToHost, stream));
-      PF_HIP(hipStreamSynchronize(stream));
-      if (hscal[ZM_SLOT + ZS_ACTIVE] == 0.0 || left <= 0) break;
-      batch = std::min(4, left);
+    zm_c = c;
+    zm_phi = phi_np1;
+    launch<KZmEval3>(stream, dim3(g), {Ystar, phi_np1, (const double*)dscal, c.Y_clip, c.Y_cap, c.vB, R, partials});
+    launch<KZmOpenEnd>(stream, dim3(1), {partials, g, st, (const double*)dscal, c});
+    zm_left = std::max(c.max_it, 0);
+    zm_steps(std::min(zm_spec, zm_left));
+  }
+  void zm_steps(int batch) {
+    double* st = dscal + ZM_SLOT;
+    const int g = grid_for(R);
+    const ZmConst& c = zm_c;
+    batch = std::min(batch, zm_left);
+    for (int k = 0; k < batch; ++k) {
+      launch<KZmEval>(stream, dim3(g), {Ystar, zm_phi, (const double*)st, c.Y_clip, c.Y_cap, c.vB, R, partials});
+      launch<KZmStepEnd>(stream, dim3(1), {partials, g, st, c});
     }
+    zm_left -= batch;
+  }
+  bool zm_more() const { return hscal[ZM_SLOT + ZS_ACTIVE] != 0.0 && zm_left > 0; }
+  int zero_mode_solve(const double* phi_np1, int nfetch, const ZmConst& c, const char* what, ZmResult* res) {
+    zm_begin(phi_np1, c);
+    for (;;) {
+      PF_HIP(hipMemcpyAsync(hscal, dscal, nfetch * sizeof(double), hipMemcpyDeviceToHost, stream));
+      PF_HIP(hipMemcpyAsync(hscal + ZM_SLOT, dscal + ZM_SLOT, ZS_SUMS * sizeof(double), hipMemcpyDeviceToHost,
+                            stream));
+      PF_HIP(hipStreamSynchronize(stream));
+      if (!zm_more()) break;
+      zm_steps(4);
+    }
+    return zm_end(c, what, res);
+  }
+  // the accepted shift of a finished search (host mirror), or the rejection
+  int zm_end(const ZmConst& c, const char* what, ZmResult* res) {
     const double* hs = hscal + ZM_SLOT;
     const int status = (int)hs[ZS_STATUS];
     res->ymin = hscal[0];
@@ -2279,7 +2466,7 @@ struct Engine final : EngineBase {
     for (int i = 0; i < 3; ++i) fa.a[i] = sc(i);
     for (int i = 0; i < 21; ++i) fa.C[i] = P.C[i];
     fa.to_force = to_force ? 1 : 0;
-    hipLaunchKernelGGL(kop_force_map<T>, dim3(grid_for(nSc)), dim3(NT), 0, stream, fa);
+    hipLaunchKernelGGL((kop_force_map<T>), dim3(grid_for(nSc)), dim3(NT), 0, stream, fa);
   }
 
   int cold_hook() {
@@ -2289,7 +2476,7 @@ struct Engine final : EngineBase {
       if (rc) return rc;
       force_map(false);
       if (state_fp64())
-        hipLaunchKernelGGL(kop_add3_state<T>, dim3(gmodes()), dim3(NT), 0, stream, kd(), nSc, (const C*)sc(0),
+        hipLaunchKernelGGL((kop_add3_state<T>), dim3(gmodes()), dim3(NT), 0, stream, kd(), nSc, (const C*)sc(0),
                            (const C*)sc(1), (const C*)sc(2), Ud, Ud + nScA, Ud + 2 * nScA);
       else
         hipLaunchKernelGGL((kop_add3<T, C>), dim3(gk), dim3(NT), 0, stream, nSc, (const C*)sc(0), (const C*)sc(1),
@@ -2371,9 +2558,9 @@ struct Engine final : EngineBase {
     ga.partials = nullptr;
     for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
     ga.have_old = 0;
-    hipLaunchKernelGGL(kop_green<T>, dim3(gk), dim3(NT), 0, stream, ga);
+    hipLaunchKernelGGL((kop_green<T>), dim3(gk), dim3(NT), 0, stream, ga);
     if (state_fp64())
-      hipLaunchKernelGGL(kop_diff3_state<T>, dim3(gmodes()), dim3(NT), 0, stream, d, nSc, U_last(),
+      hipLaunchKernelGGL((kop_diff3_state<T>), dim3(gmodes()), dim3(NT), 0, stream, d, nSc, U_last(),
                          U_last() + nScA, U_last() + 2 * nScA, (const C*)uu(un, 0), (const C*)uu(un, 1),
                          (const C*)uu(un, 2), sc(0), sc(1), sc(2));
     else
@@ -3070,7 +3257,7 @@ struct Engine final : EngineBase {
     launch_zpass<T, 0, 1, OpEtaRhs<T>>(Nz, stream, zargs(0, 0, 0, 1), oa);
     ypass(false, 0, 1);
     xpass(false, 0, 1, Sc);
-    hipLaunchKernelGGL(kop_phi<T>, dim3(grid_for(nSc)), dim3(NT), 0, stream, kd(), nSc, sc(0),
+    hipLaunchKernelGGL((kop_phi<T>), dim3(grid_for(nSc)), dim3(NT), 0, stream, kd(), nSc, sc(0),
                        c.Ldt * q.gp_kappa_eta);
     xpass(true, 0, 1, Sc);
     ypass(true, 0, 1);
@@ -3561,6 +3748,29 @@ struct Engine final : EngineBase {
   int mb_phase_b() override { return min_phase_b(); }
   int mb_phase_c(pf_minimize_stats* st) override { return min_phase_c(st); }
   int mb_converged() override { return m_converged ? 1 : 0; }
+  int sb_check() override {
+    if (P.elastic && !extended) return fail(PF_E_ARG, "fused batches need the accelerated elastic solver");
+    if (cold_hooks()) return fail(PF_E_ARG, "fused batches run no cold-start hooks");
+    if (zpass_use_reg()) return fail(PF_E_ARG, "fused batches use the LDS z pass");
+    return require_state("pf_batch_step_fused");
+  }
+  int sb_phase_a(int spec) override {
+    aa_spec = std::max(spec, 1);
+    return step_phase_a();
+  }
+  int sb_zspec() override { return std::max(zm_spec, 1); }
+  int sb_phase_b(int zspec) override {
+    zm_spec = std::max(zspec, 1);
+    return step_phase_b();
+  }
+  int sb_zm_more(int* more) override {
+    *more = 0;
+    if (!prm.zero_mode || !zm_more()) return PF_OK;
+    zm_steps(4);
+    *more = 1;
+    return PF_OK;
+  }
+  int sb_phase_c() override { return step_phase_c(); }
   int mb_nscal() override { return NSCAL; }
   void mb_gather(double* dst) override { launch<KGather>(stream, dim3(1), {dscal, dst, NSCAL}); }
   void mb_scatter(const double* src) override { std::memcpy(hscal, src, NSCAL * sizeof(double)); }
@@ -3613,7 +3823,7 @@ struct Engine final : EngineBase {
           for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
           ga.ctl = dscal + AA_SLOT;
           ga.m = m;
-          hipLaunchKernelGGL(kop_green_aa<T>, dim3(gk), dim3(NT), 0, stream, ga);
+          hipLaunchKernelGGL((kop_green_aa<T>), dim3(gk), dim3(NT), 0, stream, ga);
           // h read; U, g read and written; with a history: du[slot] read and
           // written, dg[slot] written, the other m-1 dg slots read
           return (m > 0 ? 12.0 + 3.0 * m : 9.0) * nScA * cs + 12.0 * nScA * 16.0;
@@ -3632,7 +3842,7 @@ struct Engine final : EngineBase {
         for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
         ga.have_old = 1;
         ga.partials = partials;
-        hipLaunchKernelGGL(kop_green<T>, dim3(gk), dim3(NT), 0, stream, ga);
+        hipLaunchKernelGGL((kop_green<T>), dim3(gk), dim3(NT), 0, stream, ga);
         return (12.0 + 6.0) * nScA * cs;
       }
       case PF_K_AA_UPDATE: {
@@ -3653,7 +3863,7 @@ struct Engine final : EngineBase {
         for (int i = 0; i < 6; ++i) ua.eps[i] = sc(i);
         ua.ctl = dscal + AA_SLOT;
         ua.m = m;
-        hipLaunchKernelGGL(kop_aa_update<T>, dim3(gmodes()), dim3(NT), 0, stream, ua);
+        hipLaunchKernelGGL((kop_aa_update<T>), dim3(gmodes()), dim3(NT), 0, stream, ua);
         // g read; the m history slots read; du[slot] and the 6 strains written
         return (3.0 * m + 9.0) * nScA * cs + 3.0 * nScA * 16.0;
       }

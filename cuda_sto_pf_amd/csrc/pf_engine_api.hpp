@@ -51,6 +51,14 @@ struct EngineBase {
   virtual int mb_nscal() = 0;                  // doubles of device scalars
   virtual void mb_gather(double* dst) = 0;     // enqueue device scalars -> dst (device)
   virtual void mb_scatter(const double* src) = 0;   // host mirror <- src
+  // the phases of a time step (pf_batch_step_fused); the solver's further
+  // iterations go through mb_solve_more
+  virtual int sb_check() = 0;
+  virtual int sb_phase_a(int spec) = 0;        // the solve's start and `spec` iterations
+  virtual int sb_zspec() = 0;                  // zero-mode steps the next search enqueues first
+  virtual int sb_phase_b(int zspec) = 0;       // the solve's result through the opening of the search
+  virtual int sb_zm_more(int* more) = 0;       // four more search steps while it runs (*more = 1)
+  virtual int sb_phase_c() = 0;                // the search's result, the shift, the step's bookkeeping
 };
 
 std::unique_ptr<EngineBase> make_engine_f64();

@@ -51,7 +51,7 @@ constexpr size_t bt_size(int n) {
 }
 
 template <class K>
-__global__ void __launch_bounds__(K::BLOCK) batched_kernel(const char* tab, int n) {
+__global__ void __launch_bounds__(K::BLOCK) batched_kernel(const char* __restrict__ tab, int n) {
   // the instance of this workgroup: the last start[i] <= blockIdx.x, found by
   // one ballot over the (at most 64) prefix sums
   const int* start = (const int*)tab;
@@ -62,6 +62,10 @@ __global__ void __launch_bounds__(K::BLOCK) batched_kernel(const char* tab, int 
   const B3 g = ((const B3*)(tab + bt_grid_off(n)))[inst];
   const unsigned l = (unsigned)(b - start[inst]);
   const B3 bid{l % g.x, (l / g.x) % g.y, l / (g.x * g.y)};
+  // The table is restrict: stores through the output pointers the arguments
+  // hold never change it, as they never change kernel arguments, so the
+  // compiler may reuse argument loads as it does in the single kernel (where
+  // it otherwise evaluates, and rounds, some expressions differently).
   const auto* args = (const typename K::Args*)(tab + bt_args_off<typename K::Args>(n));
   K::body(args[inst], bid, g);
 }

@@ -1145,8 +1145,7 @@ struct KStrain {
 // update (phi_semi_implicit_update_kernel, cuda_kernels.cu:821-843), and the
 // transforms carry only the increment.
 template <typename T>
-__global__ void __launch_bounds__(NT) kop_phi_state(KDims d, long n, cx<T>* g, double Ldt_kappa, cx<double>* Phi,
-                                                    int mode) {
+__device__ __forceinline__ void kop_phi_state_body(KDims d, long n, cx<T>* g, double Ldt_kappa, cx<double>* Phi, int mode, const B3 blockIdx, const B3 gridDim) {
   const long nact_ = (n / d.Az) * d.Azr;   // retained modes only (no kz padding)
   for (long j_ = blockIdx.x * (long)NT + threadIdx.x; j_ < nact_; j_ += (long)gridDim.x * NT) {
     const long c = kslot(d, j_);
@@ -1168,6 +1167,27 @@ __global__ void __launch_bounds__(NT) kop_phi_state(KDims d, long n, cx<T>* g, d
     }
   }
 }
+template <typename T>
+__global__ void __launch_bounds__(NT) kop_phi_state(KDims d, long n, cx<T>* g, double Ldt_kappa, cx<double>* Phi, int mode) {
+  kop_phi_state_body<T>(d, n, g, Ldt_kappa, Phi, mode, PF_BIDX, PF_GDIM);
+}
+template <typename T>
+struct KPhiState {
+  struct Args {
+    KDims d;
+    long n;
+    cx<T>* g;
+    double Ldt_kappa;
+    cx<double>* Phi;
+    int mode;
+  };
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3 b, B3 g) { kop_phi_state_body<T>(a.d, a.n, a.g, a.Ldt_kappa, a.Phi, a.mode, b, g); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL((kop_phi_state<T>), dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.d, a.n, a.g, a.Ldt_kappa, a.Phi, a.mode);
+  }
+};
+
 
 // phi^ <- g^ / (1 + L dt kappa k^2)
 template <typename T>
@@ -1206,8 +1226,7 @@ struct KPhi {
 
 // grad_a mu = i k_a mu^ (compute_gradient_single_component_k_kernel 1887-1911)
 template <typename T>
-__global__ void __launch_bounds__(NT) kop_grad(KDims d, long n, const cx<T>* mu, cx<T>* gx,
-                                               cx<T>* gy, cx<T>* gz) {
+__device__ __forceinline__ void kop_grad_body(KDims d, long n, const cx<T>* mu, cx<T>* gx, cx<T>* gy, cx<T>* gz, const B3 blockIdx, const B3 gridDim) {
   const long nact_ = (n / d.Az) * d.Azr;   // retained modes only (no kz padding)
   for (long j_ = blockIdx.x * (long)NT + threadIdx.x; j_ < nact_; j_ += (long)gridDim.x * NT) {
     const long c = kslot(d, j_);
@@ -1220,6 +1239,27 @@ __global__ void __launch_bounds__(NT) kop_grad(KDims d, long n, const cx<T>* mu,
     gz[c] = cx<T>{T(-kz * mi), T(kz * mr)};
   }
 }
+template <typename T>
+__global__ void __launch_bounds__(NT) kop_grad(KDims d, long n, const cx<T>* mu, cx<T>* gx, cx<T>* gy, cx<T>* gz) {
+  kop_grad_body<T>(d, n, mu, gx, gy, gz, PF_BIDX, PF_GDIM);
+}
+template <typename T>
+struct KGrad {
+  struct Args {
+    KDims d;
+    long n;
+    const cx<T>* mu;
+    cx<T>* gx;
+    cx<T>* gy;
+    cx<T>* gz;
+  };
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3 b, B3 g) { kop_grad_body<T>(a.d, a.n, a.mu, a.gx, a.gy, a.gz, b, g); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL((kop_grad<T>), dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.d, a.n, a.mu, a.gx, a.gy, a.gz);
+  }
+};
+
 
 // Y^ <- (Y^n + dt f^)/(1 + dt Dbar k^2),  f^ = i k.J^ + Dbar k^2 Y^n - s^
 // (Y_semi_implicit_update_kernel 2528-2550 with divJ_accumulate 2112-2137
@@ -1232,10 +1272,7 @@ __global__ void __launch_bounds__(NT) kop_grad(KDims d, long n, const cx<T>* mu,
 // mode) that is the same update, and its FP32 rounding scales with the
 // increment instead of with Y.
 template <typename T>
-__global__ void __launch_bounds__(NT) kop_yupdate(KDims d, long n, const cx<T>* jx,
-                                                  const cx<T>* jy, const cx<T>* jz,
-                                                  const cx<T>* yn, const cx<T>* s, cx<T>* out,
-                                                  const double* dbar, double dt) {
+__device__ __forceinline__ void kop_yupdate_body(KDims d, long n, const cx<T>* jx, const cx<T>* jy, const cx<T>* jz, const cx<T>* yn, const cx<T>* s, cx<T>* out, const double* dbar, double dt, const B3 blockIdx, const B3 gridDim) {
   const double D = *dbar;
   const long nact_ = (n / d.Az) * d.Azr;   // retained modes only (no kz padding)
   for (long j_ = blockIdx.x * (long)NT + threadIdx.x; j_ < nact_; j_ += (long)gridDim.x * NT) {
@@ -1263,5 +1300,30 @@ __global__ void __launch_bounds__(NT) kop_yupdate(KDims d, long n, const cx<T>* 
     out[c] = cx<T>{T((ynr + dt * fr) / den), T((yni + dt * fi) / den)};
   }
 }
+template <typename T>
+__global__ void __launch_bounds__(NT) kop_yupdate(KDims d, long n, const cx<T>* jx, const cx<T>* jy, const cx<T>* jz, const cx<T>* yn, const cx<T>* s, cx<T>* out, const double* dbar, double dt) {
+  kop_yupdate_body<T>(d, n, jx, jy, jz, yn, s, out, dbar, dt, PF_BIDX, PF_GDIM);
+}
+template <typename T>
+struct KYUpdate {
+  struct Args {
+    KDims d;
+    long n;
+    const cx<T>* jx;
+    const cx<T>* jy;
+    const cx<T>* jz;
+    const cx<T>* yn;
+    const cx<T>* s;
+    cx<T>* out;
+    const double* dbar;
+    double dt;
+  };
+  static constexpr int BLOCK = NT;
+  __device__ static void body(const Args& a, B3 b, B3 g) { kop_yupdate_body<T>(a.d, a.n, a.jx, a.jy, a.jz, a.yn, a.s, a.out, a.dbar, a.dt, b, g); }
+  static void direct(hipStream_t s, B3 g, const Args& a) {
+    hipLaunchKernelGGL((kop_yupdate<T>), dim3(g.x, g.y, g.z), dim3(NT), 0, s, a.d, a.n, a.jx, a.jy, a.jz, a.yn, a.s, a.out, a.dbar, a.dt);
+  }
+};
+
 
 }  // namespace pf

@@ -212,6 +212,7 @@ EXPORTS = {
     "pf_batch_step": (C.c_int, [C.c_void_p, C.c_int]),
     "pf_batch_minimize": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
     "pf_batch_minimize_fused": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "pf_batch_step_fused": (C.c_int, [C.c_void_p, C.c_int]),
     "pf_batch_fused_launches": (C.c_int, [C.c_void_p] + [C.POINTER(C.c_longlong)] * 4),
     "pf_batch_set_graphs": (C.c_int, [C.c_void_p, C.c_int]),
     "pf_batch_destroy": (C.c_int, [C.c_void_p]),
@@ -539,6 +540,11 @@ class Batch:
         st = (PfMinimizeStats * n)()
         _check(lib().pf_batch_minimize(self._h, arr, st))
         return [s.as_dict() for s in st]
+
+    def step_fused(self, n=1):
+        """pf_batch_step_fused: n steps of all instances together, each phase
+        of a step one batched launch per kernel over the instances."""
+        _check(lib().pf_batch_step_fused(self._h, int(n)))
 
     def minimize_fused(self, opts):
         """pf_batch_minimize_fused: all instances' iterations together, each

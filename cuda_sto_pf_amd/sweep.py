@@ -69,6 +69,53 @@ def run(instances, threads, opts, prec, accel=True, net=None, fused=False, launc
     return wall, stats, phis
 
 
+def dynamics(n, count, steps, prec, threads=8):
+    """Wall time of `steps` time steps of `count` handles of the workload-2
+    geometry at n^3 (accelerated solver): one at a time, on one stream per
+    handle, and fused (with and without graph replay); the fused fields are
+    compared with the solo ones."""
+    from .configs import config
+
+    def make():
+        out = []
+        for _ in range(count):
+            pd, phi, xB, meta = config(2, n)
+            s = pf.Simulation(dict(pd, **ACCEL), prec)
+            s.set_fields(phi, xB, xB_cap=meta["xB_max_safe"])
+            s.step(1)   # past the cold start
+            s.sync()
+            out.append(s)
+        return out
+
+    res = {}
+    sims = make()
+    t0 = time.perf_counter()
+    for s in sims:
+        s.step(steps)
+    for s in sims:
+        s.sync()
+    res["alone_s"] = time.perf_counter() - t0
+    ref = [s.field(pf.PF_XB) for s in sims]
+    sims = make()
+    b = pf.Batch(sims, threads)
+    t0 = time.perf_counter()
+    b.step(steps)
+    res["streams_s"] = time.perf_counter() - t0
+    b.close()
+    for graphs in (False, True):
+        sims = make()
+        b = pf.Batch(sims, 1)
+        b.set_graphs(graphs)
+        t0 = time.perf_counter()
+        b.step_fused(steps)
+        res["fused_graphs_s" if graphs else "fused_s"] = time.perf_counter() - t0
+        lc = b.fused_launches()
+        b.close()
+        res["max_abs_dxB_fused"] = max(float(np.abs(s.field(pf.PF_XB) - r).max()) for s, r in zip(sims, ref))
+    res.update(batched_launches=lc[0], recorded_launches=lc[1], graphs=lc[2], graph_launches=lc[3])
+    return res
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--threads", type=int, default=8)
@@ -81,8 +128,19 @@ def main(argv=None):
     ap.add_argument("--report", default=None)
     ap.add_argument("--modes", default="alone,streams,fused",
                     help="comma list of alone, streams, fused (fused alone: for profiling)")
+    ap.add_argument("--dynamics", default=None, metavar="N,COUNT,STEPS",
+                    help="time steps instead: COUNT handles at N^3, STEPS steps each")
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)
+    if a.dynamics:
+        n, count, steps = (int(v) for v in a.dynamics.split(","))
+        prec = pf.PF_FP64 if a.precision == "fp64" else pf.PF_MIXED
+        r = dynamics(n, count, steps, prec, a.threads)
+        log(f"[dynamics] {count} x {n}^3, {steps} steps, {a.precision}: " + json.dumps(r))
+        if a.report:
+            with open(a.report, "w") as f:
+                json.dump(dict(n=n, count=count, steps=steps, precision=a.precision, **r), f, indent=1)
+        return 0
     inst = sweep_instances()
     if a.subset:
         inst = inst[:a.subset]

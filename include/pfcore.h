@@ -422,6 +422,17 @@ int pf_batch_minimize(pf_batch* b, const pf_minimize_opts* opts, pf_minimize_sta
  * bit.  No reference counterpart: the reference runs a sweep (e.g. 48 minimize
  * runs of 64^3 to 128^3 grids) as one process per instance. */
 int pf_batch_minimize_fused(pf_batch* b, const pf_minimize_opts* opts, pf_minimize_stats* stats);
+/* nsteps dynamics steps on every instance as one fused batch: each phase of a
+ * step between the solver's and the zero mode's waits is recorded per
+ * instance and replayed as one batched launch per kernel over the instances
+ * that share it, recurring phases as HIP graphs (pf_batch_set_graphs); one
+ * handle alone is one step as graph replays.  Accelerated elastic solver
+ * (or no elasticity) and no cold-start hook; otherwise PF_E_ARG.  Results
+ * equal pf_step's on each handle bit for bit in mixed precision and to
+ * rounding in FP64 (a batched and a single compilation of one FP64 kernel
+ * may fuse multiply-adds differently), with the same iteration counts.  No
+ * reference counterpart. */
+int pf_batch_step_fused(pf_batch* b, int nsteps);
 /* launches of the last fused call: batched launches issued, the launches the
  * instances recorded (the count one-by-one execution issues), the HIP graphs
  * captured and the graph launches (any pointer may be NULL) */

@@ -74,3 +74,57 @@ def test_fused_batch_refuses_what_it_cannot_record(pf):
     with pytest.raises(pf.PfError, match="accelerated elastic solver"):
         b.minimize_fused(pf.minimize_opts(dt=0.05, V0=0.1, max_iter=2))
     b.close()
+
+
+def _dyn_sims(pf, prec, sizes=(16, 20, 24)):
+    out = []
+    for n in sizes:
+        pd, phi, xB, meta = config(2, n)
+        pd = dict(pd, elastic_warm_start_order=3, elastic_anderson_depth=3)
+        s = pf.Simulation(pd, prec)
+        s.set_fields(phi, xB, xB_cap=meta["xB_max_safe"])
+        out.append(s)
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_fused_batch_step_equals_solo_steps(pf, prec):
+    """Time steps as a fused batch (each phase of a step one batched launch
+    per kernel over the instances, recurring phases replayed as HIP graphs)
+    reproduce every instance's solo steps with the same elastic and
+    zero-mode iteration counts: bit for bit in mixed precision, to rounding in
+    FP64 (the batched and the single compilation of an FP64 kernel on the
+    step's composition path fuse multiply-adds differently: 1e-14 after 5
+    steps); one handle alone runs its steps as graph replays."""
+    def same(x, y):
+        if prec == "mixed":
+            return np.array_equal(x, y)
+        return float(np.abs(x - y).max()) <= 1e-12 * max(1.0, float(np.abs(x).max()))
+    p = pf.PF_FP64 if prec == "fp64" else pf.PF_MIXED
+    solo, fused = _dyn_sims(pf, p), _dyn_sims(pf, p)
+    for s in solo:
+        s.step(5)
+    b = pf.Batch(fused, 1)
+    b.step_fused(5)
+    batched, recorded, graphs, glaunch = b.fused_launches()
+    print(f"{prec}: {batched} batched launches for {recorded} recorded, {glaunch} graph launches of {graphs} graphs",
+          flush=True)
+    b.close()
+    for s, t in zip(solo, fused):
+        for f in (pf.PF_PHI, pf.PF_Y, pf.PF_XB):
+            assert same(s.field(f), t.field(f))
+        a, c = s.scalars(), t.scalars()
+        assert a["elastic_iterations_total"] == c["elastic_iterations_total"]
+        assert a["zero_mode_iterations"] == c["zero_mode_iterations"] and a["steps_done"] == c["steps_done"] == 5
+    assert batched < recorded and glaunch > graphs   # three grid sizes: the transform passes stay one launch per size
+    # one handle: its steps as graph replays, then ordinary steps continue it
+    one, ref = _dyn_sims(pf, p, (24,)), _dyn_sims(pf, p, (24,))
+    b1 = pf.Batch(one, 1)
+    b1.step_fused(4)
+    assert b1.fused_launches()[3] > 0
+    b1.close()
+    one[0].step(2)
+    ref[0].step(6)
+    for f in (pf.PF_PHI, pf.PF_Y, pf.PF_XB):
+        assert same(one[0].field(f), ref[0].field(f))
