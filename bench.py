@@ -92,23 +92,31 @@ def kernel_table(pf, sim, iters_per_step, mixed):
 def learned_cold_start(pf, learned, net, pd, pd_fast, phi, xB):
     """First (cold) step of the workload with and without the learned warm
     start: elastic updates to the 1e-8 residual, step time, and the change in
-    x_B (the tolerance-level difference of two converged solves)."""
+    x_B (the tolerance-level difference of two converged solves).  Each pair
+    runs once untimed first, so the times exclude the one-time loading of the
+    kernels' code at their first launch in the process."""
     out = {}
     for label, p, prec in (("mixed_accelerated", pd_fast, pf.PF_MIXED), ("fp64_reference_iteration", pd, pf.PF_FP64)):
+        learned.cold_start(p, phi, xB, prec)
+        learned.cold_start(p, phi, xB, prec, net)
         i0, t0, x0 = learned.cold_start(p, phi, xB, prec)
         i1, t1, x1 = learned.cold_start(p, phi, xB, prec, net)
         out[label] = {"updates_without": i0, "updates_with": i1, "step_ms_without": 1e3 * t0,
                       "step_ms_with": 1e3 * t1, "max_abs_dxB": float(np.abs(x1 - x0).max())}
         print(f"[bench] learned cold start {label}: {i0} -> {i1} updates, {1e3 * t0:.1f} -> {1e3 * t1:.1f} ms",
               file=sys.stderr, flush=True)
-    us = net.time_inference(phi.shape, reps=5)
+    # the network runs on the grid averaged over pool^3 voxels
+    pool = net.pool()
+    grid = tuple(n // pool for n in phi.shape)
+    us = net.time_inference(grid, reps=5)
     # algorithmic multiply-adds of the 3^3 convolutions, 7 -> 16 -> 16 -> 16 -> 16 -> 3 channels; the
     # MFMA work pads the first layer's input and the last layer's output to 16 channels and the 27
     # taps to 14 pairs of the K = 32 instruction
-    flop = 2.0 * 27 * (7 * 16 + 3 * 16 * 16 + 16 * 3) * float(np.prod(phi.shape))
-    padded = 2.0 * 14 * 32 * 16 * 5 * float(np.prod(phi.shape))
-    out["inference"] = {"us": us, "flop": flop, "flop_padded": padded, "achieved_tflops": flop / us * 1e-6,
-                        "peak_tflops_bf16": 2500.0, "frac": flop / us * 1e-6 / 2500.0}
+    flop = 2.0 * 27 * (7 * 16 + 3 * 16 * 16 + 16 * 3) * float(np.prod(grid))
+    padded = 2.0 * 14 * 32 * 16 * 5 * float(np.prod(grid))
+    out["inference"] = {"grid": list(grid), "pool": pool, "us": us, "flop": flop, "flop_padded": padded,
+                        "achieved_tflops": flop / us * 1e-6, "peak_tflops_bf16": 2500.0,
+                        "frac": flop / us * 1e-6 / 2500.0}
     return out
 
 

@@ -848,6 +848,55 @@ __global__ void __launch_bounds__(NT) nn_features_kernel(PhysParams P, const dou
     dst[1] = make_uint4(nn::pk(o[8], o[9]), nn::pk(o[10], o[11]), nn::pk(o[12], o[13]), nn::pk(o[14], o[15]));
   }
 }
+// Features of a pooled network: each coarse voxel (x, y, z) of an (Nx/2,
+// Ny/2, Nz/2) grid holds the mean of the FP64 features of the 8 voxels
+// (2x + i, 2y + j, 2z + k), in the BF16 layout of nn_features_kernel.
+__global__ void __launch_bounds__(NT) nn_features_pool_kernel(PhysParams P, const double* phi, const double* xB,
+                                                              int Nx, int Ny, int Nz, double s_phi, double s_tau,
+                                                              uint16_t* X) {
+  const int cy = Ny / 2, cz = Nz / 2;
+  const long nc = (long)(Nx / 2) * cy * cz;
+  for (long e = (long)blockIdx.x * NT + threadIdx.x; e < nc; e += (long)gridDim.x * NT) {
+    const int z = (int)(e % cz), y = (int)((e / cz) % cy), x = (int)(e / ((long)cz * cy));
+    double f[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int q = 0; q < 8; ++q) {
+      const long g = ((long)(2 * x + (q >> 2)) * Ny + (2 * y + ((q >> 1) & 1))) * Nz + 2 * z + (q & 1);
+      const double ph = phi[g];
+      const double h = h_of_phi(ph);
+      double e0[6], tau[6];
+      eigenstrain(P, h, clamp01(xB[g]), e0);
+      voigt_apply_h<double>(P, h, e0, tau);
+      f[0] += ph;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) f[1 + c] += tau[c];
+    }
+    uint16_t o[16];
+    o[0] = nn::f2bf((float)(0.125 * f[0] * s_phi));
+#pragma unroll
+    for (int c = 0; c < 6; ++c) o[1 + c] = nn::f2bf((float)(0.125 * f[1 + c] * s_tau));
+#pragma unroll
+    for (int c = 7; c < 16; ++c) o[c] = 0;
+    uint4* dst = reinterpret_cast<uint4*>(X + (size_t)e * 16);
+    dst[0] = make_uint4(nn::pk(o[0], o[1]), nn::pk(o[2], o[3]), nn::pk(o[4], o[5]), nn::pk(o[6], o[7]));
+    dst[1] = make_uint4(nn::pk(o[8], o[9]), nn::pk(o[10], o[11]), nn::pk(o[12], o[13]), nn::pk(o[14], o[15]));
+  }
+}
+
+// FP32 planar [3][Nx*Ny*Nz] -> the 2x2x2 means [3][(Nx/2)(Ny/2)(Nz/2)] (a
+// pooled network's training target)
+__global__ void __launch_bounds__(NT) pool3_kernel(const float* in, int Nx, int Ny, int Nz, float* out) {
+  const int cy = Ny / 2, cz = Nz / 2;
+  const long nc = (long)(Nx / 2) * cy * cz, n = (long)Nx * Ny * Nz;
+  for (long e = (long)blockIdx.x * NT + threadIdx.x; e < 3 * nc; e += (long)gridDim.x * NT) {
+    const int c = (int)(e / nc);
+    const long v = e % nc;
+    const int z = (int)(v % cz), y = (int)((v / cz) % cy), x = (int)(v / ((long)cz * cy));
+    float acc = 0.f;
+    for (int q = 0; q < 8; ++q)
+      acc += in[c * n + ((long)(2 * x + (q >> 2)) * Ny + (2 * y + ((q >> 1) & 1))) * Nz + 2 * z + (q & 1)];
+    out[c * nc + v] = 0.125f * acc;
+  }
+}
 }  // namespace
 
 // three FP32 planar real fields [3][R] (times scale) -> forward z of 3 components
@@ -855,13 +904,19 @@ template <typename T>
 struct OpLoadF3 {
   struct Args {
     const float* y3;
-    size_t R;
+    size_t R;        // voxels of y3 per component
     double scale;
+    int pool;        // 2: y3 is on the (Nx/2, Ny/2, Nz/2) grid, read at each voxel's coarse cell
+    int Ny, Nz;
   };
   static constexpr int NRED = 0;
   static constexpr int kinds[1] = {0};
   static constexpr int NPRE = 3;
   __device__ static void load(const Args& a, size_t g, double* pre) {
+    if (a.pool == 2) {
+      const size_t z = g % a.Nz, y = (g / a.Nz) % a.Ny, x = g / ((size_t)a.Nz * a.Ny);
+      g = ((x >> 1) * (a.Ny >> 1) + (y >> 1)) * (a.Nz >> 1) + (z >> 1);
+    }
 #pragma unroll
     for (int c = 0; c < 3; ++c) pre[c] = (double)a.y3[c * a.R + g];
   }
@@ -2449,10 +2504,18 @@ struct Engine final : EngineBase {
     if ((rc = nn_buffers())) return rc;
     double sp, st, so;
     nn::scales(nn_net, &sp, &st, &so);
-    hipLaunchKernelGGL(nn_features_kernel, dim3(grid_for(R)), dim3(NT), 0, stream, P, (const double*)phi[cur],
-                       xB_el(), R, sp, st, nn_x0);
-    if ((rc = nn::infer(nn_net, stream, Nx, Ny, Nz, nn_x0, nn_a, nn_b, nn_y3))) return rc;
-    typename OpLoadF3<T>::Args oa{nn_y3, (size_t)R, so};
+    const int pool = nn::pool(nn_net);
+    if (pool == 2) {
+      if (Nx % 2 || Ny % 2 || Nz % 2) return fail(PF_E_ARG, "a pooled network needs even grid dimensions");
+      hipLaunchKernelGGL(nn_features_pool_kernel, dim3(grid_for(R / 8)), dim3(NT), 0, stream, P,
+                         (const double*)phi[cur], xB_el(), Nx, Ny, Nz, sp, st, nn_x0);
+      if ((rc = nn::infer(nn_net, stream, Nx / 2, Ny / 2, Nz / 2, nn_x0, nn_a, nn_b, nn_y3))) return rc;
+    } else {
+      hipLaunchKernelGGL(nn_features_kernel, dim3(grid_for(R)), dim3(NT), 0, stream, P, (const double*)phi[cur],
+                         xB_el(), R, sp, st, nn_x0);
+      if ((rc = nn::infer(nn_net, stream, Nx, Ny, Nz, nn_x0, nn_a, nn_b, nn_y3))) return rc;
+    }
+    typename OpLoadF3<T>::Args oa{nn_y3, (size_t)(pool == 2 ? R / 8 : R), so, pool, Ny, Nz};
     launch_zpass<T, 0, 3, OpLoadF3<T>>(Nz, stream, zargs(0, 0, 0, 3), oa);
     ypass(false, 0, 3);
     xpass(false, 0, 3, Sc);
@@ -2532,14 +2595,22 @@ struct Engine final : EngineBase {
     uint16_t* feat;
     float* tgt;
     int rc;
-    if ((rc = nn::dataset_slot(net, Nx, Ny, Nz, &feat, &tgt))) return rc;
+    const int pool = nn::pool(net);
+    if (pool == 2 && (Nx % 2 || Ny % 2 || Nz % 2))
+      return fail(PF_E_ARG, "pf_nn_collect: a pooled network needs even grid dimensions");
+    if ((rc = nn::dataset_slot(net, Nx / pool, Ny / pool, Nz / pool, &feat, &tgt))) return rc;
     if (!P.elastic) return fail(PF_E_ARG, "pf_nn_collect: elasticity is disabled");
     if ((rc = require_state("pf_nn_collect"))) return rc;
+    if (pool == 2 && (rc = nn_buffers())) return rc;
     if ((rc = elastic_solve())) return rc;
     double sp, st, so;
     nn::scales(net, &sp, &st, &so);
-    hipLaunchKernelGGL(nn_features_kernel, dim3(grid_for(R)), dim3(NT), 0, stream, P, (const double*)phi[cur],
-                       xB_el(), R, sp, st, feat);
+    if (pool == 2)
+      hipLaunchKernelGGL(nn_features_pool_kernel, dim3(grid_for(R / 8)), dim3(NT), 0, stream, P,
+                         (const double*)phi[cur], xB_el(), Nx, Ny, Nz, sp, st, feat);
+    else
+      hipLaunchKernelGGL(nn_features_kernel, dim3(grid_for(R)), dim3(NT), 0, stream, P, (const double*)phi[cur],
+                         xB_el(), R, sp, st, feat);
     // homogeneous cold start into un
     typename OpElasticCold<T>::Args oa{P, phi[cur], xB_el()};
     launch_zpass<T, 0, 6, OpElasticCold<T>>(Nz, stream, zargs(0, 0, 0, 6), oa);
@@ -2570,8 +2641,12 @@ struct Engine final : EngineBase {
     force_map(true);
     xpass(true, 0, 3, Sc);
     ypass(true, 0, 3);
-    typename OpStoreF3<T>::Args sa{tgt, (size_t)R, T(1.0 / (double)R)};
+    // the target at full resolution, then (pooled network) its 2x2x2 means
+    typename OpStoreF3<T>::Args sa{pool == 2 ? nn_y3 : tgt, (size_t)R, T(1.0 / (double)R)};
     launch_zpass<T, 3, 0, OpStoreF3<T>>(Nz, stream, zargs(0, 3, 0, 0), sa);
+    if (pool == 2)
+      hipLaunchKernelGGL(pool3_kernel, dim3(grid_for(3 * (R / 8))), dim3(NT), 0, stream, (const float*)nn_y3, Nx,
+                         Ny, Nz, tgt);
     PF_HIP(hipStreamSynchronize(stream));
     rc = nn::dataset_commit(net);
     return rc < 0 ? rc : PF_OK;

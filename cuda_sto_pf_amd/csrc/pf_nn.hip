@@ -94,6 +94,7 @@ struct Net {
   int Nx = 0, Ny = 0, Nz = 0, max_batch = 0, capacity = 0, count = 0, pending = -1;
   long S = 0;
   double s_phi = 1.0, s_tau = 0.1, s_out = 1.0;
+  int pool = 1;              // 2: the network runs on the grid averaged over 2x2x2 voxels
   float* master = nullptr;   // [P_TOTAL]
   float* m = nullptr;
   float* v = nullptr;
@@ -207,6 +208,8 @@ int infer(Net* net, hipStream_t s, int Nx, int Ny, int Nz, const uint16_t* X0, u
   return PF_OK;
 }
 
+int pool(const Net* net) { return net->pool; }
+
 void scales(const Net* net, double* s_phi, double* s_tau, double* s_out) {
   *s_phi = net->s_phi;
   *s_tau = net->s_tau;
@@ -289,6 +292,18 @@ int pf_nn_set_scales(pf_nn* net, double s_phi, double s_tau, double s_out) {
   net->net.s_phi = s_phi;
   net->net.s_tau = s_tau;
   net->net.s_out = s_out;
+  return PF_OK;
+}
+
+int pf_nn_set_pool(pf_nn* net, int factor) {
+  if (!net || (factor != 1 && factor != 2)) return fail(PF_E_ARG, "pf_nn_set_pool: the factor is 1 or 2");
+  net->net.pool = factor;
+  return PF_OK;
+}
+
+int pf_nn_get_pool(pf_nn* net, int* factor) {
+  if (!net || !factor) return fail(PF_E_ARG, "pf_nn_get_pool: null");
+  *factor = net->net.pool;
   return PF_OK;
 }
 

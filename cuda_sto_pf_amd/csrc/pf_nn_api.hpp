@@ -25,6 +25,9 @@ __device__ __forceinline__ uint32_t pk(uint16_t lo, uint16_t hi) { return (uint3
 // FP32 (network output units); a, b are [S][16] BF16 scratch.
 int infer(Net* net, hipStream_t s, int Nx, int Ny, int Nz, const uint16_t* X0, uint16_t* a, uint16_t* b, float* Y3);
 void scales(const Net* net, double* s_phi, double* s_tau, double* s_out);
+// 1: the network sees the grid itself; 2: the grid averaged over 2x2x2 voxels
+// (its output is read back at each of the 8 voxels)
+int pool(const Net* net);
 // Next dataset slot for a grid of Nx*Ny*Nz (error if full or the grid differs);
 // commit() makes it part of the dataset once the engine's stream has finished.
 int dataset_slot(Net* net, int Nx, int Ny, int Nz, uint16_t** feat, float** target);

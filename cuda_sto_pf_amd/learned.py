@@ -13,7 +13,9 @@ residual and the tolerance are those of the reference, so converged results
 are unchanged.
 
 Weights file (npz): params float32[34640] (per layer W[27][16][16] then
-bias[16]), s_phi, s_tau, s_out, and a JSON string `meta`.
+bias[16]), s_phi, s_tau, s_out, pool (1: the network runs on the handle's
+grid; 2: on its 2x2x2 averages, its prediction read back at each voxel; absent
+means 1), and a JSON string `meta`.
 
     python -m cuda_sto_pf_amd.learned train --out weights.npz
     python -m cuda_sto_pf_amd.learned evaluate --weights weights.npz
@@ -31,7 +33,10 @@ from . import pfcore as pf
 from .configs import base_params, composition, config, place_nonoverlapping, spheres_phi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-DEFAULT_WEIGHTS = os.path.join(HERE, "weights", "warmstart_c16x5.npz")
+# the committed network runs on the 2x2x2 averages of the grid (pool 2); the
+# full-resolution network of the same architecture is kept beside it
+DEFAULT_WEIGHTS = os.path.join(HERE, "weights", "warmstart_c16x5_pool2.npz")
+FULL_RES_WEIGHTS = os.path.join(HERE, "weights", "warmstart_c16x5.npz")
 
 CH, TAPS, LAYERS, IN_CH, OUT_CH = 16, 27, 5, 7, 3
 P_PER_LAYER = TAPS * CH * CH + CH
@@ -67,7 +72,8 @@ def training_fields(rng, n):
 def save(path, net, meta):
     s_phi, s_tau, s_out = net.scales()
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
-    np.savez(path, params=net.params(), s_phi=s_phi, s_tau=s_tau, s_out=s_out, meta=json.dumps(meta))
+    np.savez(path, params=net.params(), s_phi=s_phi, s_tau=s_tau, s_out=s_out, pool=np.int32(net.pool()),
+             meta=json.dumps(meta))
 
 
 def load(path=DEFAULT_WEIGHTS):
@@ -76,13 +82,17 @@ def load(path=DEFAULT_WEIGHTS):
     net = pf.Net((16, 16, 16), max_batch=1, capacity=0)
     net.set_scales(float(d["s_phi"]), float(d["s_tau"]), float(d["s_out"]))
     net.set_params(d["params"])
+    net.set_pool(int(d["pool"]) if "pool" in d.files else 1)
     return net
 
 
-def train(n=64, snapshots=4000, per_run=4, steps_between=5, batch=4, iters=3000, lr=2e-3, seed=0, log=print):
+def train(n=64, snapshots=4000, per_run=4, steps_between=5, batch=4, iters=3000, lr=2e-3, seed=0, pool=1,
+          log=print):
     pd = dict(base_params(n, hetero=True), **ACCEL)
     sim = pf.Simulation(pd, pf.PF_MIXED)
-    net = pf.Net((n, n, n), max_batch=batch, capacity=snapshots)
+    m = n // pool
+    net = pf.Net((m, m, m), max_batch=batch, capacity=snapshots)
+    net.set_pool(pool)
     net.set_scales(S_PHI, S_TAU, 1.0)
     rng = np.random.default_rng(seed)
     t0 = last = time.time()
@@ -129,8 +139,8 @@ def train(n=64, snapshots=4000, per_run=4, steps_between=5, batch=4, iters=3000,
             hist.append((it + 1, loss, v))
             log(f"[learned] step {it + 1}/{iters} lr {lr_t:.2e} train {loss:.4e} val {v:.4e} ({last - t1:.0f} s)")
     t_train = time.time() - t1
-    meta = dict(grid=n, snapshots=snapshots, runs=runs, per_run=per_run, steps_between=steps_between, batch=batch,
-                iters=iters, lr=lr, seed=seed, s_out=s_out, initial_val_loss=v0, final_val_loss=val_loss(),
+    meta = dict(grid=n, pool=pool, snapshots=snapshots, runs=runs, per_run=per_run, steps_between=steps_between,
+                batch=batch, iters=iters, lr=lr, seed=seed, s_out=s_out, initial_val_loss=v0, final_val_loss=val_loss(),
                 collect_s=t_collect, train_s=t_train, history=hist,
                 data="seeded runs of 1-6 spheres, radii U(5,12), heterogeneous elasticity, mixed precision")
     return net, meta
@@ -195,6 +205,8 @@ def main(argv=None):
     t.add_argument("--batch", type=int, default=4)
     t.add_argument("--lr", type=float, default=2e-3)
     t.add_argument("--seed", type=int, default=0)
+    t.add_argument("--pool", type=int, default=1, choices=(1, 2),
+                   help="2: the network runs on the 2x2x2 averages of the grid")
     t.add_argument("--evaluate", action="store_true")
     t.add_argument("--report", default=None)
     e = sub.add_parser("evaluate")
@@ -205,7 +217,7 @@ def main(argv=None):
     report = {}
     if a.cmd == "train":
         net, meta = train(n=a.grid, snapshots=a.snapshots, iters=a.iters, batch=a.batch, lr=a.lr, seed=a.seed,
-                          log=log)
+                          pool=a.pool, log=log)
         save(a.out, net, meta)
         log(f"[learned] wrote {a.out}")
         report["training"] = meta

@@ -194,6 +194,8 @@ EXPORTS = {
     "pf_nn_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
     "pf_nn_destroy": (C.c_int, [C.c_void_p]),
     "pf_nn_set_scales": (C.c_int, [C.c_void_p, C.c_double, C.c_double, C.c_double]),
+    "pf_nn_set_pool": (C.c_int, [C.c_void_p, C.c_int]),
+    "pf_nn_get_pool": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "pf_nn_get_scales": (C.c_int, [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                    C.POINTER(C.c_double)]),
     "pf_nn_set_params": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -448,6 +450,15 @@ class Net:
 
     def set_scales(self, s_phi, s_tau, s_out):
         _check(lib().pf_nn_set_scales(self._h, float(s_phi), float(s_tau), float(s_out)))
+
+    def set_pool(self, factor):
+        """1: the network runs on the handle's grid; 2: on its 2x2x2 averages."""
+        _check(lib().pf_nn_set_pool(self._h, int(factor)))
+
+    def pool(self):
+        v = C.c_int()
+        _check(lib().pf_nn_get_pool(self._h, C.byref(v)))
+        return v.value
 
     def scales(self):
         a, b, c = C.c_double(), C.c_double(), C.c_double()

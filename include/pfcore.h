@@ -357,6 +357,13 @@ int pf_nn_destroy(pf_nn* net);
 int pf_nn_set_scales(pf_nn* net, double s_phi, double s_tau, double s_out);
 int pf_nn_get_scales(pf_nn* net, double* s_phi, double* s_tau, double* s_out);
 int pf_nn_set_params(pf_nn* net, const float* params);   /* PF_NN_PARAMS values; resets Adam */
+/* Resolution the network runs at inside a handle (pf_nn_collect, the cold-start
+ * prediction): 1, the grid itself (default); 2, the grid averaged over 2x2x2
+ * voxels (even Nx, Ny, Nz), whose prediction each of the 8 voxels reads back.
+ * A pooled network is created with the pooled training grid (n/2)^3 and
+ * collects from handles of n^3. */
+int pf_nn_set_pool(pf_nn* net, int factor);
+int pf_nn_get_pool(pf_nn* net, int* factor);
 int pf_nn_get_params(pf_nn* net, float* params);
 
 /* Converge the handle's elastic solve for its current phi (from its warm

@@ -89,8 +89,11 @@ def test_training_on_solver_snapshots_lowers_loss(pf):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_attached_network_leaves_results_unchanged(pf, prec):
+@pytest.mark.parametrize("prec,pool", [("fp64", 1), ("mixed", 1), ("fp64", 2), ("mixed", 2)])
+def test_attached_network_leaves_results_unchanged(pf, prec, pool):
+    """A network at the cold start (on the grid itself, or on its 2x2x2
+    averages with the prediction read back at every voxel) moves only where
+    the fixed point starts: converged x_B agrees to the solver tolerance."""
     n = 32
     pd, phi, xB, meta = config(2, n)
     pd = dict(pd, elastic_warm_start_order=3, elastic_anderson_depth=3)
@@ -99,9 +102,10 @@ def test_attached_network_leaves_results_unchanged(pf, prec):
     b = pf.Simulation(pd, P)
     for s in (a, b):
         s.set_fields(phi, xB, xB_cap=meta["xB_max_safe"])
-    net = pf.Net((n, n, n), max_batch=1)
+    net = pf.Net((n // pool,) * 3, max_batch=1)
     net.set_scales(1.0, 0.1, 0.05)
     net.set_params(_params(9))
+    net.set_pool(pool)
     b.attach_net(net)
     for _ in range(3):
         a.step()
@@ -139,3 +143,28 @@ def test_warm_start_callback_supplies_the_start(pf):
     print(f"cold start {cold_iters} updates, supplied start {it}", flush=True)
     assert len(calls) == 1 and it <= 2 < cold_iters
     assert np.abs(sim.field(pf.PF_XB) - ref.field(pf.PF_XB)).max() < 1e-9
+
+
+@pytest.mark.gpu
+def test_pooled_collection_averages_the_full_resolution_snapshot(pf):
+    """pf_nn_collect of a pooled network stores the 2x2x2 means of what a
+    full-resolution network collects from the same state: the loss of the
+    zero network (the mean square target) of the pooled snapshot lies between
+    0 and that of the full one (averaging cannot raise a mean square), and the
+    pooled network's predicted start reaches the same converged solution."""
+    n = 32
+    pd, phi, xB, meta = config(2, n)
+    pd = dict(pd, elastic_warm_start_order=3, elastic_anderson_depth=3)
+    losses = {}
+    for pool in (1, 2):
+        s = pf.Simulation(pd, pf.PF_MIXED)
+        s.set_fields(phi, xB, xB_cap=meta["xB_max_safe"])
+        net = pf.Net((n // pool,) * 3, max_batch=1, capacity=1)
+        net.set_pool(pool)
+        net.set_scales(1.0, 0.1, 1.0)
+        net.collect(s)
+        net.set_params(np.zeros(34640, dtype=np.float32))   # the zero network: loss = mean square target
+        losses[pool] = net.eval_loss([0])
+    print(f"mean square target: full {losses[1]:.4e}, pooled {losses[2]:.4e}", flush=True)
+    assert 0.0 < losses[2] <= losses[1] * (1 + 1e-6)
+    assert losses[2] > 0.05 * losses[1]   # the interface-scale force density survives the averaging
