@@ -1306,6 +1306,7 @@ struct Engine final : EngineBase {
   bool in_cold_hook = false;
 
   ~Engine() override {
+    probe_release();
     for (void* q : allocations) (void)hipFree(q);
     if (hscal) (void)hipHostFree(hscal);
     if (stream) (void)hipStreamDestroy(stream);
@@ -1469,26 +1470,76 @@ struct Engine final : EngineBase {
     }
     return PF_OK;
   }
+  // One elastic update's launches in the solve's order (Anderson update, x
+  // and y passes back, z pass, y and x passes forward, Green update), or the
+  // four 6-component passes without elasticity.
+  int update_chain(int* ids) const {
+    if (!P.elastic) {
+      const int c[] = {PF_K_XPASS_INV6, PF_K_YPASS_INV6, PF_K_YPASS_FWD6, PF_K_XPASS_FWD6};
+      std::copy(c, c + 4, ids);
+      return 4;
+    }
+    const int c[] = {PF_K_AA_UPDATE, PF_K_XPASS_INV6, PF_K_YPASS_INV6, PF_K_ZPASS_HIJ,
+                     PF_K_YPASS_FWD6, PF_K_XPASS_FWD6, PF_K_GREEN};
+    std::copy(c, c + 7, ids);
+    return 7;
+  }
+  // Times the update chain as a step runs it: one untimed chain, then `reps`
+  // chains with HIP events around each launch, so every kernel reads what
+  // the one before it wrote rather than its own input of a moment ago
+  // (repeated launches of one kernel find much of a 256^3 operand still in
+  // the device's last-level cache, which a step never does).  us[PF_K_*]
+  // receives each kernel's mean microseconds; *total their sum per chain.
+  int chain_time(int reps, double* us, double* total) {
+    int ids[8];
+    const int n = update_chain(ids);
+    int rc = aa_ctl_steady();
+    if (rc) return rc;
+    for (int i = 0; i < n; ++i) launch_one(ids[i]);
+    std::vector<hipEvent_t> ev((size_t)2 * n * reps, nullptr);
+    auto destroy = [&]() {
+      for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    };
+    for (auto& e : ev)
+      if (hipEventCreate(&e) != hipSuccess) {
+        destroy();
+        return fail(PF_E_HIP, "hipEventCreate failed in the plan tuner");
+      }
+    bool ok = true;
+    for (int r = 0; r < reps && ok; ++r)
+      for (int i = 0; i < n && ok; ++i) {
+        ok = hipEventRecord(ev[2 * (r * n + i)], stream) == hipSuccess;
+        launch_one(ids[i]);
+        ok = ok && hipEventRecord(ev[2 * (r * n + i) + 1], stream) == hipSuccess;
+      }
+    ok = ok && hipEventSynchronize(ev.back()) == hipSuccess;
+    for (int k = 0; k < PF_K_COUNT; ++k) us[k] = 0.0;
+    *total = 0.0;
+    for (int r = 0; r < reps && ok; ++r)
+      for (int i = 0; i < n && ok; ++i) {
+        float ms = 0.f;
+        ok = hipEventElapsedTime(&ms, ev[2 * (r * n + i)], ev[2 * (r * n + i) + 1]) == hipSuccess;
+        us[ids[i]] += 1000.0 * (double)ms / reps;
+        *total += 1000.0 * (double)ms / reps;
+      }
+    destroy();
+    if (!ok) return fail(PF_E_HIP, "event timing failed in the plan tuner");
+    PF_HIP(hipGetLastError());
+    return PF_OK;
+  }
   int tune_kz_stride() {
     const int widest = AzS;
     PF_HIP(hipMemset(phi[0], 0, R * sizeof(double)));
     PF_HIP(hipMemset(phi[1], 0, R * sizeof(double)));
     PF_HIP(hipMemset(xB, 0, R * sizeof(double)));
-    const int ids[] = {PF_K_ZPASS_HIJ, PF_K_YPASS_FWD6, PF_K_XPASS_FWD6, PF_K_GREEN,
-                       PF_K_XPASS_INV6, PF_K_YPASS_INV6, PF_K_AA_UPDATE};
     kz_times.assign(kz_cands.size(), 0.0);
     size_t best = 0;
     for (size_t c = 0; c < kz_cands.size(); ++c) {
       set_kz_stride(kz_cands[c]);
-      for (int id : ids) {
-        if (!P.elastic && id != PF_K_YPASS_FWD6 && id != PF_K_XPASS_FWD6 && id != PF_K_XPASS_INV6 &&
-            id != PF_K_YPASS_INV6)
-          continue;
-        double us = 0.0, b = 0.0;
-        int rc = time_kernel(id, 3, &us, &b);
-        if (rc) return rc;
-        kz_times[c] += us;
-      }
+      double us[PF_K_COUNT];
+      int rc = chain_time(3, us, &kz_times[c]);
+      if (rc) return rc;
       if (kz_times[c] < kz_times[best]) best = c;
     }
     set_kz_stride(widest);
@@ -1552,14 +1603,13 @@ struct Engine final : EngineBase {
       return PF_OK;
     }
     if (R < 32768) return PF_OK;   // small grids keep the default tiles (launch latency dominates)
+    // each candidate's kernels timed inside the update chain
     auto timed = [&](std::initializer_list<int> ids, double* out) -> int {
+      double us[PF_K_COUNT], chain = 0.0;
+      int rc = chain_time(3, us, &chain);
+      if (rc) return rc;
       double tot = 0.0;
-      for (int id : ids) {
-        double us = 0.0, b = 0.0;
-        int rc = time_kernel(id, 3, &us, &b);
-        if (rc) return rc;
-        tot += us;
-      }
+      for (int id : ids) tot += us[id];
       *out = tot;
       return PF_OK;
     };
@@ -1891,10 +1941,17 @@ struct Engine final : EngineBase {
         for (int i = 0; i < 6; ++i) ua.eps[i] = sc(i);
         ua.ctl = ctl;
         ua.m = m;
+        int pr = probe_begin(PF_K_AA_UPDATE);
         launch<KAaUpdate<T>>(stream, dim3(gmodes()), {ua});
+        probe_end(pr, acc_k, probe_bytes_of(PF_K_AA_UPDATE, acc_k));
       }
+      int pr = probe_begin(PF_K_XPASS_INV6);
       xpass(true, 0, 6, Sc);
+      probe_end(pr, acc_k, probe_bytes_of(PF_K_XPASS_INV6, acc_k));
+      pr = probe_begin(PF_K_YPASS_INV6);
       ypass(true, 0, 6);
+      probe_end(pr, acc_k, probe_bytes_of(PF_K_YPASS_INV6, acc_k));
+      pr = probe_begin(PF_K_ZPASS_HIJ);
       if (mode == 0) {
         typename OpHij<T>::Args oa{P, phi[cur], xB_el(), invN, 0, hcache};
         launch_zpass_b<T, 6, 6, OpHij<T>>(zb_sel, Nz, stream, zargs(0, 6, 0, 6), oa);
@@ -1902,8 +1959,13 @@ struct Engine final : EngineBase {
         typename OpHijInc<T>::Args oa{P, hcache, invN};
         launch_zpass_b<T, 6, 6, OpHijInc<T>>(zb_sel, Nz, stream, zargs(0, 6, 0, 6), oa);
       }
+      probe_end(pr, acc_k, probe_bytes_of(PF_K_ZPASS_HIJ, acc_k));
+      pr = probe_begin(PF_K_YPASS_FWD6);
       ypass(false, 0, 6);
+      probe_end(pr, acc_k, probe_bytes_of(PF_K_YPASS_FWD6, acc_k));
+      pr = probe_begin(PF_K_XPASS_FWD6);
       xpass(false, 0, 6, Sc);
+      probe_end(pr, acc_k, probe_bytes_of(PF_K_XPASS_FWD6, acc_k));
       GreenAAArgs<T> ga{};
       ga.d = d;
       ga.n = nSc;
@@ -1922,7 +1984,9 @@ struct Engine final : EngineBase {
       for (int i = 0; i < 21; ++i) ga.C[i] = P.C[i];
       ga.ctl = ctl;
       ga.m = m;
+      pr = probe_begin(PF_K_GREEN);
       launch<KGreenAA<T>>(stream, dim3(gk), {ga});
+      probe_end(pr, acc_k, probe_bytes_of(PF_K_GREEN, acc_k));
       launch<KAaEnd>(stream, dim3(1), {partials, gk, ctl, mode, ac});
     }
   }
@@ -1937,12 +2001,18 @@ struct Engine final : EngineBase {
     aa_spec = std::max(it, 1);
     if (hc[AC_STATUS] != 0.0) {
       last_iters = it;
+      probe_settle(it);
       return fail(PF_E_NONFINITE, "elastic residual is non-finite at step " + std::to_string(steps_done + 1));
     }
     launch<KAaFinal<T>>(stream, dim3(gmodes()), {d, nSc, Ud, Ud + nScA, Ud + 2 * nScA, Gd, Gd + nScA,
                                                   Gd + 2 * nScA, sc(0), sc(1), sc(2), sc(3), sc(4), sc(5)});
+    int pr = probe_begin(PF_K_XPASS_INV6);
     xpass(true, 0, 6, Sc);
+    probe_end(pr, -1, probe_bytes_of(PF_K_XPASS_INV6, 0));
+    pr = probe_begin(PF_K_YPASS_INV6);
     ypass(true, 0, 6);
+    probe_end(pr, -1, probe_bytes_of(PF_K_YPASS_INV6, 0));
+    probe_settle(it);
     std::swap(hbuf[hist_head], Ud);
     hist_head = (hist_head + 1) % 3;
     hist_count = std::min(hist_count + 1, 3);
@@ -3968,18 +4038,148 @@ struct Engine final : EngineBase {
     }
   }
 
+  // ------------------------------------------------------------ launch probe
+  // HIP events around the accelerated solve's launches of the kernel
+  // classes in probe_mask (bit PF_K_*), on the handle's stream, for the
+  // classes' in-step durations and bytes.  A launch of an iteration after
+  // the one the device stopped at does nothing (its gate is off): it counts
+  // its time and no bytes.  Launches recorded for a fused batch are not
+  // probed.
+  struct ProbeRec {
+    hipEvent_t e0, e1;
+    int id, k;          // class; iteration of the solve (-1: the solve's end, always runs)
+    double bytes;       // algorithmic bytes when the launch does its work
+  };
+  unsigned probe_mask = 0;
+  bool probe_err = false;
+  std::vector<ProbeRec> probe_open, probe_done;
+  std::vector<hipEvent_t> probe_pool;
+  long long probe_n[PF_K_COUNT] = {}, probe_worked[PF_K_COUNT] = {};
+  double probe_us[PF_K_COUNT] = {}, probe_bytes[PF_K_COUNT] = {};
+
+  void probe_release() {
+    for (auto* v : {&probe_open, &probe_done})
+      for (auto& r : *v) {
+        (void)hipEventDestroy(r.e0);
+        (void)hipEventDestroy(r.e1);
+      }
+    probe_open.clear();
+    probe_done.clear();
+    for (hipEvent_t e : probe_pool) (void)hipEventDestroy(e);
+    probe_pool.clear();
+  }
+  hipEvent_t probe_event() {
+    hipEvent_t e = nullptr;
+    if (!probe_pool.empty()) {
+      e = probe_pool.back();
+      probe_pool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+      e = nullptr;
+    }
+    return e;
+  }
+  // before a launch of class `id`: the record's index, or -1 when not probed
+  int probe_begin(int id) {
+    if (!(probe_mask >> id & 1u) || tl_rec || probe_err) return -1;
+    ProbeRec r{probe_event(), probe_event(), id, -1, 0.0};
+    if (!r.e0 || !r.e1 || hipEventRecord(r.e0, stream) != hipSuccess) {
+      probe_err = true;
+      if (r.e0) probe_pool.push_back(r.e0);
+      if (r.e1) probe_pool.push_back(r.e1);
+      return -1;
+    }
+    probe_open.push_back(r);
+    return (int)probe_open.size() - 1;
+  }
+  void probe_end(int idx, int k, double bytes) {
+    if (idx < 0) return;
+    ProbeRec& r = probe_open[idx];
+    if (hipEventRecord(r.e1, stream) != hipSuccess) probe_err = true;
+    r.k = k;
+    r.bytes = bytes;
+  }
+  // the solve took `it` iterations: launches of later ones moved nothing
+  void probe_settle(int it) {
+    for (ProbeRec& r : probe_open) {
+      if (r.k >= it) r.bytes = 0.0;
+      probe_done.push_back(r);
+    }
+    probe_open.clear();
+  }
+  // algorithmic bytes of iteration k's launch of class id in the accelerated
+  // solve (the counts of launch_one, with the history the iteration has:
+  // min(k, m) slots after the Green update of iteration k >= 1)
+  double probe_bytes_of(int id, int k) const {
+    const double cs = (double)sizeof(C);
+    const int m = prm.elastic_anderson_depth;
+    switch (id) {
+      case PF_K_ZPASS_HIJ:   // iteration 0 maps phi and x_B and fills the h(phi) cache
+        return 12.0 * nSzA * cs + (k == 0 ? 16.0 * R + (double)sizeof(T) * R : (double)sizeof(T) * R);
+      case PF_K_YPASS_FWD6:
+      case PF_K_YPASS_INV6: return 6.0 * (nSzA + nSyA) * cs;
+      case PF_K_XPASS_FWD6:
+      case PF_K_XPASS_INV6: return 6.0 * (nSyA + nScA) * cs;
+      case PF_K_GREEN:
+        if (k == 0) return 6.0 * nScA * cs + 6.0 * nScA * 16.0;   // h, U read; g written
+        return (m > 0 ? 12.0 + 3.0 * std::min(k, m) : 9.0) * nScA * cs + 12.0 * nScA * 16.0;
+      case PF_K_AA_UPDATE: return (3.0 * std::min(k - 1, m) + 9.0) * nScA * cs + 3.0 * nScA * 16.0;
+      default: return 0.0;
+    }
+  }
+  int probe_kernels(unsigned mask) override {
+    PF_HIP(hipStreamSynchronize(stream));
+    probe_release();
+    probe_mask = mask;
+    probe_err = false;
+    for (int i = 0; i < PF_K_COUNT; ++i) {
+      probe_n[i] = probe_worked[i] = 0;
+      probe_us[i] = probe_bytes[i] = 0.0;
+    }
+    return PF_OK;
+  }
+  int probe_read(int id, long long* launches, long long* worked, double* us, double* bytes) override {
+    PF_HIP(hipStreamSynchronize(stream));
+    if (probe_err) return fail(PF_E_HIP, "launch probe: event record failed");
+    for (ProbeRec& r : probe_done) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, r.e0, r.e1) != hipSuccess) probe_err = true;
+      probe_n[r.id] += 1;
+      probe_us[r.id] += 1000.0 * (double)ms;
+      if (r.bytes > 0.0) {
+        probe_worked[r.id] += 1;
+        probe_bytes[r.id] += r.bytes;
+      }
+      probe_pool.push_back(r.e0);
+      probe_pool.push_back(r.e1);
+    }
+    probe_done.clear();
+    if (probe_err) return fail(PF_E_HIP, "launch probe: event timing failed");
+    *launches = probe_n[id];
+    *worked = probe_worked[id];
+    *us = probe_us[id];
+    *bytes = probe_bytes[id];
+    return PF_OK;
+  }
+
+  // the accelerated solver's control block in its steady state (running, a
+  // full history, slot 0 next, fixed mixing coefficients) for launch_one
+  int aa_ctl_steady() {
+    if (!extended) return PF_OK;
+    double* hc = hscal + AA_SLOT;
+    for (int i = 0; i < AC_COUNT; ++i) hc[i] = 0.0;
+    hc[AC_ACTIVE] = 1.0;
+    hc[AC_NH] = (double)prm.elastic_anderson_depth;
+    for (int j = 0; j < AA_MAX; ++j) hc[AC_GAM + j] = 0.25;
+    PF_HIP(hipMemcpyAsync(dscal + AA_SLOT, hc, AC_COUNT * sizeof(double), hipMemcpyHostToDevice, stream));
+    PF_HIP(hipStreamSynchronize(stream));
+    return PF_OK;
+  }
+
   int time_kernel(int id, int reps, double* us, double* bytes) override {
     state_ok = false;   // the timed launches overwrite the simulation buffers
-    if (extended && (id == PF_K_GREEN || id == PF_K_AA_UPDATE)) {
-      // the accelerated solver's control block in its steady state: running,
-      // a full history, slot 0 next, fixed mixing coefficients
-      double* hc = hscal + AA_SLOT;
-      for (int i = 0; i < AC_COUNT; ++i) hc[i] = 0.0;
-      hc[AC_ACTIVE] = 1.0;
-      hc[AC_NH] = (double)prm.elastic_anderson_depth;
-      for (int j = 0; j < AA_MAX; ++j) hc[AC_GAM + j] = 0.25;
-      PF_HIP(hipMemcpyAsync(dscal + AA_SLOT, hc, AC_COUNT * sizeof(double), hipMemcpyHostToDevice, stream));
-      PF_HIP(hipStreamSynchronize(stream));
+    if (id == PF_K_GREEN || id == PF_K_AA_UPDATE) {
+      const int rc = aa_ctl_steady();
+      if (rc) return rc;
     }
     hipEvent_t e0, e1;
     PF_HIP(hipEventCreate(&e0));

@@ -28,6 +28,8 @@ struct EngineBase {
   virtual int fft_inverse(const double* in_half, double* out) = 0;
   virtual int sync() = 0;
   virtual int time_kernel(int id, int reps, double* us, double* bytes) = 0;
+  virtual int probe_kernels(unsigned mask) = 0;
+  virtual int probe_read(int id, long long* launches, long long* worked, double* us, double* bytes) = 0;
   virtual int minimize_begin(const pf_minimize_opts& o) = 0;
   virtual int minimize_iterate(pf_minimize_stats* st) = 0;
   virtual int set_warm_start(const double* u_or_null) = 0;

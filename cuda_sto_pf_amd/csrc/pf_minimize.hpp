@@ -28,7 +28,7 @@ PF_HD double min_dgel(const PhysParams& P, double phi, const double* eps, double
   double el[6], q[6];
 #pragma unroll
   for (int c = 0; c < 6; ++c) el[c] = eps[c] - h * P.eps00[c];
-  voigt_apply<double>(P.dC, el, q);
+  voigt_apply<double>(P.dC, el, q, P.ortho);
   const double Q = el[0] * q[0] + el[1] * q[1] + el[2] * q[2] + 2.0 * el[3] * q[3] + 2.0 * el[4] * q[4] +
                    2.0 * el[5] * q[5];
   return -hp * s00 + 0.5 * hp * Q;
@@ -233,7 +233,7 @@ PF_HD double min_dgel_full(const PhysParams& P, double phi, double xB, const dou
   eigenstrain(P, h, x, e0);
 #pragma unroll
   for (int c = 0; c < 6; ++c) el[c] = eps[c] - e0[c];
-  voigt_apply<double>(P.dC, el, q);
+  voigt_apply<double>(P.dC, el, q, P.ortho);
   const double Q = el[0] * q[0] + el[1] * q[1] + el[2] * q[2] + 2.0 * el[3] * q[3] + 2.0 * el[4] * q[4] +
                    2.0 * el[5] * q[5];
   return dgel_eigen + 0.5 * hp * Q;

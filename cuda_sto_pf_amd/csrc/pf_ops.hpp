@@ -38,7 +38,7 @@ struct OpElasticCold {
     double h = h_of_phi(pre[0]);
     double e0[6], s[6];
     eigenstrain(a.P, h, clamp01(pre[1]), e0);
-    voigt_apply<double>(a.P.C, e0, s);
+    voigt_apply<double>(a.P.C, e0, s, a.P.ortho);
 #pragma unroll
     for (int c = 0; c < 6; ++c) out[c] = T(s[c]);
   }
@@ -204,7 +204,7 @@ struct OpPhiRhs {
                                   2.0 * sig[3] * (hp * P.eps00[3]));
       // Q = e~^T dC e~ (cuda_kernels.cu:167-185)
       T q[6];
-      voigt_apply<T>(P.dC, el, q);
+      voigt_apply<T>(P.dC, el, q, P.ortho);
       const double Q = double(el[0] * q[0] + el[1] * q[1] + el[2] * q[2] + T(2) * el[3] * q[3] +
                               T(2) * el[4] * q[4] + T(2) * el[5] * q[5]);
       f += dgel_eigen + 0.5 * hp * Q;

@@ -37,6 +37,10 @@ struct PhysParams {
   double C[21], dC[21], eps00[6], E0[6];
   // convex-extrapolation constants at X_LIMIT_CONVEX
   double muA_c, dmuA_c, muB_c, dmuB_c;
+  // 1 when C and dC couple no normal to shear component and their shear
+  // blocks are diagonal (orthotropic in the grid axes; cubic and isotropic
+  // tensors included): the Voigt products then skip the entries that are zero
+  int ortho;
 };
 
 PF_HD double h_of_phi(double phi) {
@@ -161,10 +165,26 @@ PF_HD void eigenstrain(const PhysParams& P, double h, double xBc, double* e0) {
   e0[5] = h * P.eps00[5];
 }
 
-// s_I = sum_J M_IJ e~_J with engineering shear e~ = (xx,yy,zz,2yz,2xz,2xy)
+// s_I = sum_J M_IJ e~_J with engineering shear e~ = (xx,yy,zz,2yz,2xz,2xy).
+// With `ortho` (PhysParams::ortho) the terms of the zero entries are left
+// out; each kept term is accumulated in the same order, and a skipped term
+// (0 * e) adds an exact zero, so results agree with the full product up to
+// the sign of a zero.
 template <typename R>
-PF_HD void voigt_apply(const double* M, const R* e, R* s) {
+PF_HD void voigt_apply(const double* M, const R* e, R* s, int ortho = 0) {
   R ee[6] = {e[0], e[1], e[2], R(2) * e[3], R(2) * e[4], R(2) * e[5]};
+  if (ortho) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      R acc = R(0);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) acc += R(M[vidx(i, j)]) * ee[j];
+      s[i] = acc;
+    }
+#pragma unroll
+    for (int i = 3; i < 6; ++i) s[i] = R(0) + R(M[vidx(i, i)]) * ee[i];
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
     R acc = R(0);
@@ -177,6 +197,25 @@ PF_HD void voigt_apply(const double* M, const R* e, R* s) {
 template <typename R>
 PF_HD void voigt_apply_h(const PhysParams& P, R h, const R* e, R* s) {
   R ee[6] = {e[0], e[1], e[2], R(2) * e[3], R(2) * e[4], R(2) * e[5]};
+  if (P.ortho) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      R a = R(0), b = R(0);
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        a += R(P.C[vidx(i, j)]) * ee[j];
+        b += R(P.dC[vidx(i, j)]) * ee[j];
+      }
+      s[i] = a + h * b;
+    }
+#pragma unroll
+    for (int i = 3; i < 6; ++i) {
+      const R a = R(0) + R(P.C[vidx(i, i)]) * ee[i];
+      const R b = R(0) + R(P.dC[vidx(i, i)]) * ee[i];
+      s[i] = a + h * b;
+    }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
     R a = R(0), b = R(0);
@@ -198,7 +237,7 @@ PF_HD void hij_point(const PhysParams& P, R h, const R* e0, const R* eps, R* out
 #pragma unroll
   for (int i = 0; i < 6; ++i) tot[i] = eps[i] + R(P.E0[i]);
   voigt_apply_h<R>(P, h, tot, a);
-  voigt_apply<R>(P.C, eps, b);
+  voigt_apply<R>(P.C, eps, b, P.ortho);
   voigt_apply_h<R>(P, h, e0, c);
 #pragma unroll
   for (int i = 0; i < 6; ++i) out[i] = -a[i] + b[i] + c[i];
@@ -207,7 +246,7 @@ PF_HD void hij_point(const PhysParams& P, R h, const R* e0, const R* eps, R* out
 template <typename R>
 PF_HD void hij_increment_point(const PhysParams& P, R h, const R* eps, R* out) {
   R d[6];
-  voigt_apply<R>(P.dC, eps, d);
+  voigt_apply<R>(P.dC, eps, d, P.ortho);
 #pragma unroll
   for (int i = 0; i < 6; ++i) out[i] = -(h * d[i]);
 }

@@ -1,5 +1,6 @@
 // C ABI of libpfcore: handles, parameters and the physical constants hoisted
 // from the per-voxel thermodynamics.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -120,6 +121,11 @@ PhysParams make_phys(const pf_params& p) {
     P.eps00[i] = p.eps00[i];
     P.E0[i] = p.E0[i];
   }
+  // zero normal-shear coupling and off-diagonal shear entries in both tensors
+  P.ortho = 1;
+  for (int i = 0; i < 6; ++i)
+    for (int j = std::max(i + 1, 3); j < 6; ++j)
+      if (P.C[vidx(i, j)] != 0.0 || P.dC[vidx(i, j)] != 0.0) P.ortho = 0;
   return P;
 }
 
@@ -231,6 +237,18 @@ int pf_time_kernel(pf_handle* h, int kernel_id, int reps, double* us, double* by
   if (!h || !us || !bytes || reps < 1 || kernel_id < 0 || kernel_id >= PF_K_COUNT)
     return pf::fail(PF_E_ARG, "pf_time_kernel: bad argument");
   return h->eng->time_kernel(kernel_id, reps, us, bytes);
+}
+
+int pf_probe_kernels(pf_handle* h, unsigned mask) {
+  if (!h || (mask >> PF_K_COUNT) != 0u) return pf::fail(PF_E_ARG, "pf_probe_kernels: bad argument");
+  return h->eng->probe_kernels(mask);
+}
+
+int pf_probe_read(pf_handle* h, int kernel_id, long long* launches, long long* worked, double* us_total,
+                  double* bytes_total) {
+  if (!h || !launches || !worked || !us_total || !bytes_total || kernel_id < 0 || kernel_id >= PF_K_COUNT)
+    return pf::fail(PF_E_ARG, "pf_probe_read: bad argument");
+  return h->eng->probe_read(kernel_id, launches, worked, us_total, bytes_total);
 }
 
 int pf_minimize_opts_default(pf_minimize_opts* o) {

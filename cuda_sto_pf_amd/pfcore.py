@@ -182,6 +182,9 @@ EXPORTS = {
     "pf_sync": (C.c_int, [C.c_void_p]),
     "pf_time_kernel": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_double),
                                  C.POINTER(C.c_double)]),
+    "pf_probe_kernels": (C.c_int, [C.c_void_p, C.c_uint]),
+    "pf_probe_read": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong),
+                                C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "pf_minimize_opts_default": (C.c_int, [C.POINTER(PfMinimizeOpts)]),
     "pf_minimize_begin": (C.c_int, [C.c_void_p, C.POINTER(PfMinimizeOpts)]),
     "pf_minimize_iterate": (C.c_int, [C.c_void_p, C.POINTER(PfMinimizeStats)]),
@@ -421,6 +424,24 @@ class Simulation:
         b = C.c_double()
         _check(lib().pf_time_kernel(self._h, int(kernel_id), int(reps), C.byref(us), C.byref(b)))
         return us.value, b.value
+
+    def probe(self, kernel_ids):
+        """Bracket the accelerated solve's launches of these kernel classes
+        with HIP events from the next step on (an empty list stops it)."""
+        mask = 0
+        for k in kernel_ids:
+            mask |= 1 << int(k)
+        _check(lib().pf_probe_kernels(self._h, mask))
+
+    def probe_read(self, kernel_id):
+        """(launches, working launches, summed microseconds, bytes of the
+        working launches) of one probed class since probe()."""
+        n = C.c_longlong()
+        w = C.c_longlong()
+        us = C.c_double()
+        b = C.c_double()
+        _check(lib().pf_probe_read(self._h, int(kernel_id), C.byref(n), C.byref(w), C.byref(us), C.byref(b)))
+        return n.value, w.value, us.value, b.value
 
 
 NN_PARAMS = 34640

@@ -202,6 +202,23 @@ enum pf_kernel_id {
  * the simulation state is not valid afterwards (call pf_set_fields). */
 int pf_time_kernel(pf_handle* h, int kernel_id, int reps, double* us_per_launch, double* bytes_per_launch);
 
+/* In-step launch probe.  pf_probe_kernels(h, mask) clears the probe and,
+ * from the next step on, brackets every launch of the classes in `mask`
+ * (bit 1 << PF_K_*; 0 switches the probe off) that the accelerated elastic
+ * solve makes (the 6-component passes, the z pass, the Green and Anderson
+ * updates) with HIP events on the handle's stream.  pf_probe_read
+ * synchronizes the stream and reports, for one class since the last
+ * pf_probe_kernels: launches, the launches that did their work (a launch
+ * of an iteration after the one the device stopped the solve at does
+ * nothing), their summed duration (microseconds, gated launches included)
+ * and the algorithmic bytes of the working launches (as pf_time_kernel
+ * counts them, with the history each iteration holds).  Launches of fused
+ * batches are not probed.  Measurement only; no counterpart in the
+ * reference, which has no kernel timing. */
+int pf_probe_kernels(pf_handle* h, unsigned mask);
+int pf_probe_read(pf_handle* h, int kernel_id, long long* launches, long long* worked, double* us_total,
+                  double* bytes_total);
+
 /* Minimize mode (--mode=minimize, phi-only or with --minimize-full-model).
  * Defaults of params_default (main_cuda.cu:16319-16347); CLI flags
  * main_cuda.cu:23762-23851.  Composition is absent: the eigenstrain is

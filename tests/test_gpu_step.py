@@ -46,6 +46,15 @@ def _cases():
     # y/x tiles, 86 retained kz in 96-wide padded rows) along z and along x
     out.append(("rect_z256",) + _rect_case(shape=(16, 16, 256)))
     out.append(("rect_x256",) + _rect_case(shape=(256, 12, 16)))
+    # a fully anisotropic C and dC (every Voigt entry non-zero, both C and
+    # C + dC positive definite): the general Voigt products, where the cubic
+    # tensors of the workloads take the orthotropic ones
+    pd, phi, xB, cap = _rect_case()
+    a = np.random.default_rng(5).uniform(-1.0, 1.0, (6, 6))
+    a = 0.5 * (a + a.T)
+    pd = dict(pd, C=pd["C"] + 2.0 * a, dC=pd["dC"] - a)
+    assert np.linalg.eigvalsh(pd["C"]).min() > 0 and np.linalg.eigvalsh(pd["C"] + pd["dC"]).min() > 0
+    out.append(("rect_aniso", pd, phi, xB, cap))
     return out
 
 
