@@ -598,6 +598,13 @@ __global__ void __launch_bounds__(NT) kop_green(GreenArgs<T> ga) {
         u[i] = cx<double>{s.im, -s.re};   // -i * s
       }
     }
+    // the previous iterate is read before u_new is stored (the pointers may
+    // alias as far as the compiler knows; a load after a store waits for it)
+    cx<T> uo[3];
+    if (ga.partials && ga.have_old) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) uo[i] = ga.u_old[i][c];
+    }
     cx<double> uq[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
@@ -615,8 +622,8 @@ __global__ void __launch_bounds__(NT) kop_green(GreenArgs<T> ga) {
       for (int i = 0; i < 3; ++i) {
         double dr = uq[i].re, di = uq[i].im;
         if (ga.have_old) {
-          dr -= double(ga.u_old[i][c].re);
-          di -= double(ga.u_old[i][c].im);
+          dr -= double(uo[i].re);
+          di -= double(uo[i].im);
         }
         dsum += dr * dr + di * di;
         rsum += uq[i].re * uq[i].re + uq[i].im * uq[i].im;
@@ -662,6 +669,10 @@ __device__ __forceinline__ void kop_green_acc_body(const GreenAccArgs<T>& ga, co
     cx<double> hh[6];
 #pragma unroll
     for (int i = 0; i < 6; ++i) hh[i] = cx<double>{double(ga.h[i][c].re), double(ga.h[i][c].im)};
+    // U is read before any store of the iteration (aliasing as above)
+    cx<double> uo3[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) uo3[i] = ga.mode == 0 ? cx<double>{0.0, 0.0} : ga.U[i][j_];
     double G[6];
     cx<double> g[3] = {{0, 0}, {0, 0}, {0, 0}};
     if (c != 0 && green_tensor(ga.C, kx, ky, kz, G)) {
@@ -679,7 +690,7 @@ __device__ __forceinline__ void kop_green_acc_body(const GreenAccArgs<T>& ga, co
     cx<double> dlt[3], un[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      cx<double> uo = ga.mode == 0 ? cx<double>{0.0, 0.0} : ga.U[i][j_];
+      const cx<double> uo = uo3[i];
       if (ga.mode == 0) {
         un[i] = g[i];
         dlt[i] = g[i];
@@ -825,33 +836,60 @@ __device__ __forceinline__ void kop_green_aa_body(const GreenAAArgs<T>& a, const
     double kx, ky, kz;
     int iz;
     kvec(a.d, c, kx, ky, kz, iz);
+    // Every load of the iteration is issued before its first store: the
+    // argument pointers may alias as far as the compiler knows, so a load
+    // written after a store would wait for it, one memory round trip each.
+    cx<T> hT[6];
+#pragma unroll
+    for (int i = 0; i < 6; ++i) hT[i] = a.h[i][c];
+    cx<double> U[3], go[3];
+    cx<T> dT[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) U[i] = a.U[i][j_];
+    if (a.mode != 0) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        go[i] = a.g[i][j_];
+        dT[i] = dsrc[i][j_];
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        go[i] = cx<double>{0.0, 0.0};
+        dT[i] = cx<T>{T(0), T(0)};
+      }
+    }
+    // the kept dg history (slot `slot` is replaced by this update's dg)
+    cx<T> hv[AA_MAX][3];
+    if (a.mode == 1) {
+#pragma unroll
+      for (int j = 0; j < AA_MAX; ++j)
+        if (j < nhist && j != slot) {
+#pragma unroll
+          for (int i = 0; i < 3; ++i) hv[j][i] = a.dg[j][i][j_];
+        }
+    }
     cx<double> hh[6];
 #pragma unroll
-    for (int i = 0; i < 6; ++i) hh[i] = cx<double>{double(a.h[i][c].re), double(a.h[i][c].im)};
+    for (int i = 0; i < 6; ++i) hh[i] = cx<double>{double(hT[i].re), double(hT[i].im)};
     cx<double> Gh[3];
     if (c != 0) green_apply_point<T>(a.C, kx, ky, kz, hh, Gh);
     else Gh[0] = Gh[1] = Gh[2] = cx<double>{0.0, 0.0};
-    cx<double> U[3], gn[3], dgn[3];
-    cx<T> dT[3];
+    cx<double> gn[3], dgn[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      U[i] = a.U[i][j_];
       if (a.mode == 0) {
         gn[i] = cx<double>{Gh[i].re - U[i].re, Gh[i].im - U[i].im};
         dgn[i] = cx<double>{0.0, 0.0};
-        dT[i] = cx<T>{T(0), T(0)};
       } else {
-        const cx<double> go = a.g[i][j_];
-        const cx<T> dt = dsrc[i][j_];
-        dT[i] = dt;
-        const cx<double> di = {double(dt.re), double(dt.im)};
+        const cx<double> di = {double(dT[i].re), double(dT[i].im)};
         if (a.ctl) {
           // the increment applied by this iteration's kop_aa_update
           U[i] = cx<double>{U[i].re + di.re, U[i].im + di.im};
           a.U[i][j_] = U[i];
         }
-        gn[i] = cx<double>{go.re + Gh[i].re - di.re, go.im + Gh[i].im - di.im};
-        dgn[i] = cx<double>{gn[i].re - go.re, gn[i].im - go.im};
+        gn[i] = cx<double>{go[i].re + Gh[i].re - di.re, go[i].im + Gh[i].im - di.im};
+        dgn[i] = cx<double>{gn[i].re - go[i].re, gn[i].im - go[i].im};
       }
       a.g[i][j_] = gn[i];
     }
@@ -891,10 +929,7 @@ __device__ __forceinline__ void kop_green_aa_body(const GreenAAArgs<T>& a, const
             v[2] = dq[2];
           } else {
 #pragma unroll
-            for (int i = 0; i < 3; ++i) {
-              const cx<T> t = a.dg[j][i][j_];
-              v[i] = cx<double>{double(t.re), double(t.im)};
-            }
+            for (int i = 0; i < 3; ++i) v[i] = cx<double>{double(hv[j][i].re), double(hv[j][i].im)};
           }
           acc[2 + j] += w * wdot(dq, v);
           acc[2 + AA_MAX + j] += w * wdot(gn, v);
@@ -965,36 +1000,53 @@ __device__ __forceinline__ void kop_aa_update_body(const AAUpdateArgs<T>& a, con
     double kx, ky, kz;
     int iz;
     kvec(a.d, c, kx, ky, kz, iz);
+    // all loads first (the pointers may alias as far as the compiler knows)
+    cx<double> gv[3], uo[3];
+    cx<T> hu[AA_MAX][3], hg[AA_MAX][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) gv[i] = a.g[i][j_];
+#pragma unroll
+    for (int j = 0; j < AA_MAX; ++j)
+      if (j < nhist) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          hu[j][i] = a.du[j][i][j_];
+          if (!a.ctl) hg[j][i] = a.dg[j][i][j_];
+        }
+      }
+    if (!a.ctl) {
+#pragma unroll
+      for (int i = 0; i < 3; ++i) uo[i] = a.U[i][j_];
+    }
     cx<double> dd[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      cx<double> v = a.g[i][j_];
+      cx<double> v = gv[i];
 #pragma unroll
       for (int j = 0; j < AA_MAX; ++j) {
         if (j < nhist) {
           if (a.ctl) {
             // the history slot holds du + dg (kop_green_aa)
-            const cx<T> e1 = a.du[j][i][j_];
-            v.re -= gamma[j] * double(e1.re);
-            v.im -= gamma[j] * double(e1.im);
+            v.re -= gamma[j] * double(hu[j][i].re);
+            v.im -= gamma[j] * double(hu[j][i].im);
           } else {
-            const cx<T> u1 = a.du[j][i][j_];
-            const cx<T> g1 = a.dg[j][i][j_];
-            v.re -= gamma[j] * (double(u1.re) + double(g1.re));
-            v.im -= gamma[j] * (double(u1.im) + double(g1.im));
+            v.re -= gamma[j] * (double(hu[j][i].re) + double(hg[j][i].re));
+            v.im -= gamma[j] * (double(hu[j][i].im) + double(hg[j][i].im));
           }
         }
       }
       const cx<T> q = {T(v.re), T(v.im)};
       dd[i] = cx<double>{double(q.re), double(q.im)};
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      const cx<T> q = {T(dd[i].re), T(dd[i].im)};
       if (!(a.ctl && slot >= 0)) a.dinc[i][j_] = q;   // with a history, du[slot] holds it
 #pragma unroll
       for (int j = 0; j < AA_MAX; ++j)
         if (j == slot) a.du[j][i][j_] = q;
-      if (!a.ctl) {   // under the control block kop_green_aa adds d to U
-        cx<double> uo = a.U[i][j_];
-        a.U[i][j_] = cx<double>{uo.re + dd[i].re, uo.im + dd[i].im};
-      }
+      if (!a.ctl)   // under the control block kop_green_aa adds d to U
+        a.U[i][j_] = cx<double>{uo[i].re + dd[i].re, uo[i].im + dd[i].im};
     }
     strain_of<T>(kx, ky, kz, dd, a.eps, c);
   }
