@@ -1344,8 +1344,10 @@ struct Engine final : EngineBase {
     az = make_axis(Nz, p.dz, false);
     if (az.A < 1 || ax.A < 1 || ay.A < 1 || 2 * az.A > Nz + 1)
       return fail(PF_E_ARG, "degenerate retained-mode set");
-    if (3 * (az.A - 1) > Nz)   // the z pass holds at most Nz/3 + 1 retained kz per lane block
-      return fail(PF_E_ARG, "retained kz count exceeds the 2/3 rule bound");
+    // the pruned transforms (Pruned<N>, pf_fft.hpp) and the z pass's lane
+    // blocks rely on at most N/3 + 1 retained indices per axis half
+    if (3 * (az.A - 1) > Nz || 3 * (ax.a - 1) > Nx || 3 * (ay.a - 1) > Ny)
+      return fail(PF_E_ARG, "retained mode count exceeds the 2/3 rule bound");
     R = (long)Nx * Ny * Nz;
     {
       // kz stride candidates: whole 128- or 64-byte lines on large grids

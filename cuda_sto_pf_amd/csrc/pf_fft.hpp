@@ -220,13 +220,33 @@ __device__ __forceinline__ int lds_index(int b, int j) {
   return Lds<N, BATCH, COLMAJ>::idx(b, j);
 }
 
+// Positions a pruned transform never needs.  The 2/3 rule keeps the
+// retained wrapped indices of an axis of length N inside [0, N/3 + 1) and
+// (N - N/3 - 1, N) (|k| < (2/3) pi / d with k = 2 pi i / (d N) is i < N/3),
+// so [N/3 + 1, N - N/3 - 1] holds zeros on the input of a pruned inverse
+// transform and outputs nobody reads on a pruned forward transform.
+template <int N>
+struct Pruned {
+  static constexpr int LO = N / 3 + 1;
+  static constexpr int HI = N - N / 3 - 1;
+  // all of [a, a + len) inside [LO, HI]
+  static constexpr bool covers(int a, int len) { return LO <= a && a + len - 1 <= HI; }
+};
+
 // In-place Stockham stage for radix 2/3/4/5/8/16 (butterflies staged in registers).
-template <typename T, int N, int R, int NS, int BATCH, bool COLMAJ, bool INV, int NT>
+// PIN: the first stage (NS = 1) takes the inputs Pruned<N> marks as zero as
+// compile-time zeros (neither read nor multiplied); POUT: the last stage
+// (NS * R = N) stores no output Pruned<N> marks as unread, and the arithmetic
+// that feeds only those outputs is dropped by the compiler.
+template <typename T, int N, int R, int NS, int BATCH, bool COLMAJ, bool INV, int NT, bool PIN = false,
+          bool POUT = false>
 __device__ __forceinline__ void stage_inplace(cx<T>* buf, const cx<T>* tw) {
   constexpr int NB = N / R;
   constexpr int TOT = NB * BATCH;
   constexpr int PER = (TOT + NT - 1) / NT;
   constexpr int TSTEP = N / (NS * R);
+  constexpr bool ZIN = PIN && NS == 1;        // input j + s NB, j < NB
+  constexpr bool ZOUT = POUT && NS * R == N;  // output j + s NS, j < NS
   cx<T> v[PER][R];
 #pragma unroll
   for (int q = 0; q < PER; ++q) {
@@ -242,16 +262,17 @@ __device__ __forceinline__ void stage_inplace(cx<T>* buf, const cx<T>* tw) {
       }
       const int jm = j % NS;
 #pragma unroll
-      for (int s = 0; s < R; ++s) v[q][s] = buf[lds_index<N, BATCH, COLMAJ>(b, j + s * NB)];
+      for (int s = 0; s < R; ++s)
+        v[q][s] = (ZIN && Pruned<N>::covers(s * NB, NB)) ? cx<T>{T(0), T(0)}
+                                                         : buf[lds_index<N, BATCH, COLMAJ>(b, j + s * NB)];
       if (NS > 1) {
-        // twiddles W^(jm s TSTEP): one table read, powers by multiplication
-        cx<T> w1 = tw[jm * TSTEP];
-        if (INV) w1.im = -w1.im;
-        cx<T> w = w1;
+        // twiddles W^(jm s TSTEP) read from the table (jm s TSTEP < N): one
+        // LDS read each instead of a chain of complex products
 #pragma unroll
         for (int s = 1; s < R; ++s) {
+          cx<T> w = tw[jm * s * TSTEP];
+          if (INV) w.im = -w.im;
           v[q][s] = cmul(v[q][s], w);
-          if (s + 1 < R) w = cmul(w, w1);
         }
       }
       Dft<T, R, INV>::run(v[q]);
@@ -273,7 +294,8 @@ __device__ __forceinline__ void stage_inplace(cx<T>* buf, const cx<T>* tw) {
       const int jm = j % NS;
       const int d = (j / NS) * NS * R + jm;
 #pragma unroll
-      for (int s = 0; s < R; ++s) buf[lds_index<N, BATCH, COLMAJ>(b, d + s * NS)] = v[q][s];
+      for (int s = 0; s < R; ++s)
+        if (!(ZOUT && Pruned<N>::covers(s * NS, NS))) buf[lds_index<N, BATCH, COLMAJ>(b, d + s * NS)] = v[q][s];
     }
   }
   __syncthreads();
@@ -365,28 +387,30 @@ __device__ __forceinline__ void stage_prime(cx<T>* src, cx<T>* dst, const cx<T>*
 
 // Runs all stages.  `buf` holds the data on entry; the result is returned
 // (either `buf` or `alt`, the second buffer needed only by big-prime plans).
-template <typename T, int N, int NS, int BATCH, bool COLMAJ, bool INV, int NT>
+template <typename T, int N, int NS, int BATCH, bool COLMAJ, bool INV, int NT, bool PIN, bool POUT>
 __device__ __forceinline__ cx<T>* fft_stages(cx<T>* buf, cx<T>* alt, const cx<T>* tw) {
   if constexpr (NS >= N) {
     return buf;
   } else {
     constexpr int R = pick_radix(N / NS);
     if constexpr (R <= 5 || is_pow2_radix(R)) {
-      stage_inplace<T, N, R, NS, BATCH, COLMAJ, INV, NT>(buf, tw);
-      return fft_stages<T, N, NS * R, BATCH, COLMAJ, INV, NT>(buf, alt, tw);
+      stage_inplace<T, N, R, NS, BATCH, COLMAJ, INV, NT, PIN, POUT>(buf, tw);
+      return fft_stages<T, N, NS * R, BATCH, COLMAJ, INV, NT, PIN, POUT>(buf, alt, tw);
     } else if constexpr (NS * R == N) {
       stage_prime<T, N, R, NS, BATCH, COLMAJ, INV, NT>(buf, buf, tw);
       return buf;
     } else {
       stage_prime<T, N, R, NS, BATCH, COLMAJ, INV, NT>(buf, alt, tw);
-      return fft_stages<T, N, NS * R, BATCH, COLMAJ, INV, NT>(alt, buf, tw);
+      return fft_stages<T, N, NS * R, BATCH, COLMAJ, INV, NT, PIN, POUT>(alt, buf, tw);
     }
   }
 }
 
-template <typename T, int N, int BATCH, bool COLMAJ, bool INV, int NT>
+// PIN: the caller guarantees zeros at the inputs Pruned<N> marks; POUT: the
+// caller reads no output Pruned<N> marks (those positions are left stale).
+template <typename T, int N, int BATCH, bool COLMAJ, bool INV, int NT, bool PIN = false, bool POUT = false>
 __device__ __forceinline__ cx<T>* fft_lds(cx<T>* buf, cx<T>* alt, const cx<T>* tw) {
-  return fft_stages<T, N, 1, BATCH, COLMAJ, INV, NT>(buf, alt, tw);
+  return fft_stages<T, N, 1, BATCH, COLMAJ, INV, NT, PIN, POUT>(buf, alt, tw);
 }
 
 }  // namespace pf

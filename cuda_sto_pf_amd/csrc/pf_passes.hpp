@@ -237,7 +237,7 @@ __device__ __forceinline__ void zpass_body(const ZPassArgs<T>& za, const typenam
       }
     }
     __syncthreads();
-    cx<T>* res = fft_lds<T, N, NCI * NP, false, true, NT>(buf, alt, tw);
+    cx<T>* res = fft_lds<T, N, NCI * NP, false, true, NT, true, false>(buf, alt, tw);
     if (BIG && res != buf) {
       for (int e = threadIdx.x; e < LY::size; e += NT) buf[e] = res[e];
       __syncthreads();
@@ -308,7 +308,7 @@ __device__ __forceinline__ void zpass_body(const ZPassArgs<T>& za, const typenam
 
   if constexpr (NCO > 0) {
     __syncthreads();
-    cx<T>* res = fft_lds<T, N, NCO * NP, false, false, NT>(buf, alt, tw);
+    cx<T>* res = fft_lds<T, N, NCO * NP, false, false, NT, false, true>(buf, alt, tw);
     for (int cp = wave; cp < NCO * NP; cp += NT / 64) {
       const int p = cp % NP;
       const int c = cp / NP;
@@ -361,17 +361,35 @@ struct CPassArgs {
   const double* gate;       // when set, the launch does nothing while *gate == 0
 };
 
+// Work item of a workgroup in XCD-aware order.  The dispatcher deals
+// workgroup ids round-robin to the 8 XCDs, each with its own L2; column tiles
+// next to each other share 128-byte lines wherever a row is not a whole
+// number of lines (kz strides of 64-byte lines), so each XCD takes one
+// contiguous range of (tile, plane, component) items and neighbouring tiles
+// meet in the same L2 close together in time.  A bijection on the grid.
+__device__ __forceinline__ B3 xcd_item(const B3 b, const B3 g) {
+  constexpr unsigned NXCD = 8;
+  const unsigned total = g.x * g.y * g.z;
+  const unsigned i = b.x + g.x * (b.y + g.y * b.z);
+  const unsigned per = (total + NXCD - 1) / NXCD;
+  const unsigned tall = total - (per - 1) * NXCD;   // XCDs that take `per` items
+  const unsigned xcd = i % NXCD, loc = i / NXCD;
+  const unsigned w = xcd < tall ? xcd * per + loc : tall * per + (xcd - tall) * (per - 1) + loc;
+  return B3{w % g.x, (w / g.x) % g.y, w / (g.x * g.y)};
+}
+
 template <typename T, int N, bool INV, int TC = cpass_cols<T, N>()>
-__device__ __forceinline__ void cpass_body(const CPassArgs<T>& ca, const B3 blockIdx) {
+__device__ __forceinline__ void cpass_body(const CPassArgs<T>& ca, const B3 bid, const B3 gdim) {
   if (ca.gate && *ca.gate == 0.0) return;
   constexpr bool BIG = plan_needs_alt(N);
   using LY = Lds<N, TC, true>;
   __shared__ cx<T> buf[LY::size];
   __shared__ cx<T> alt[BIG ? LY::size : 1];
   __shared__ cx<T> tw[N];
-  const int col0 = blockIdx.x * TC;
-  const int plane = blockIdx.y;
-  const int comp = blockIdx.z;
+  const B3 item = xcd_item(bid, gdim);
+  const int col0 = item.x * TC;
+  const int plane = item.y;
+  const int comp = item.z;
   const int cols = ca.cols;
   const int A = ca.A, a = ca.a;
   const cx<T>* in = ca.in + comp * ca.in_cstride + plane * ca.in_pstride;
@@ -417,7 +435,7 @@ __device__ __forceinline__ void cpass_body(const CPassArgs<T>& ca, const B3 bloc
     }
   }
   __syncthreads();
-  cx<T>* res = fft_lds<T, N, TC, true, INV, NT>(buf, alt, tw);
+  cx<T>* res = fft_lds<T, N, TC, true, INV, NT, INV, !INV>(buf, alt, tw);
   if (!INV) {
     for (int e = threadIdx.x; e < A * TC; e += NT) {
       const int b = e % TC, jc = e / TC;
@@ -436,13 +454,13 @@ __device__ __forceinline__ void cpass_body(const CPassArgs<T>& ca, const B3 bloc
 
 template <typename T, int N, bool INV, int TC = cpass_cols<T, N>()>
 __global__ void __launch_bounds__(NT) cpass_kernel(CPassArgs<T> ca) {
-  cpass_body<T, N, INV, TC>(ca, PF_BIDX);
+  cpass_body<T, N, INV, TC>(ca, PF_BIDX, PF_GDIM);
 }
 template <typename T, int N, bool INV, int TC = cpass_cols<T, N>()>
 struct KCPass {
   using Args = CPassArgs<T>;
   static constexpr int BLOCK = NT;
-  __device__ static void body(const Args& a, B3 bid, B3) { cpass_body<T, N, INV, TC>(a, bid); }
+  __device__ static void body(const Args& a, B3 bid, B3 g) { cpass_body<T, N, INV, TC>(a, bid, g); }
   static void direct(hipStream_t s, B3 g, const Args& a) {
     hipLaunchKernelGGL((cpass_kernel<T, N, INV, TC>), dim3(g.x, g.y, g.z), dim3(NT), 0, s, a);
   }
