@@ -311,10 +311,12 @@ __device__ __forceinline__ void stage_inplace(cx<T>* buf, const cx<T>* tw) {
 //        A = x_0 + sum_s (x_s + x_{R-s}) cos(2 pi s k / R)
 //        B =       sum_s (x_s - x_{R-s}) (-/+) sin(2 pi s k / R),  s = 1..(R-1)/2,
 //     X_k = A + iB, X_{R-k} = A - iB (a quarter of the multiplies of the
-//     direct sum).  One lane per pair, (R+1)/2 lanes per butterfly (lane 0 of
-//     the group gives X_0); the group's lanes read x_s and x_{R-s} at the same
-//     LDS addresses in the same iteration (broadcast reads), and each lane's
-//     cos/sin of s*k mod R comes from the length-N twiddle table.
+//     direct sum).  Two pairs per lane, k and k + G, G = (R+3)/4 lanes per
+//     butterfly (lane 0 of the group gives X_0 as its first); the group's
+//     lanes read x_s and x_{R-s} at the same LDS addresses in the same
+//     iteration (broadcast reads) and form x_s +/- x_{R-s} once for both
+//     pairs, and each pair's cos/sin of s*k mod R comes from the length-N
+//     twiddle table.
 template <typename T, int N, int R, int NS, int BATCH, bool COLMAJ, bool INV, int NT>
 __device__ __forceinline__ void stage_prime(cx<T>* src, cx<T>* dst, const cx<T>* tw) {
   constexpr int NB = N / R;
@@ -322,8 +324,8 @@ __device__ __forceinline__ void stage_prime(cx<T>* src, cx<T>* dst, const cx<T>*
   constexpr int WSTEP = N / R;   // W_R^m = W_N^(m*N/R)
   constexpr int NBF = NB * BATCH;   // butterflies in the workgroup
   constexpr int H = (R - 1) / 2;
-  constexpr int G = H + 1;          // lanes per butterfly
-  static_assert(R % 2 == 1 && G <= 64, "odd prime radix up to 127");
+  constexpr int G = (H + 2) / 2;    // lanes per butterfly: lane k gives pairs k and k + G
+  static_assert(R % 2 == 1 && G <= 64, "odd prime radix up to 255");
   constexpr int GPW = 64 / G;       // butterflies per wave and pass
   constexpr int GROUPS = (NT / 64) * GPW;
   auto split = [](int bt, int& b, int& j) {
@@ -354,6 +356,7 @@ __device__ __forceinline__ void stage_prime(cx<T>* src, cx<T>* dst, const cx<T>*
   const int lane = threadIdx.x & 63;
   const int g = lane / G;
   const int k = lane - g * G;
+  const int k2 = k + G;          // the lane's second pair (none when k2 > H)
   const int slot = (threadIdx.x >> 6) * GPW + g;
   for (int base = 0; base < NBF; base += GROUPS) {
     const int bt = base + slot;
@@ -361,25 +364,38 @@ __device__ __forceinline__ void stage_prime(cx<T>* src, cx<T>* dst, const cx<T>*
       int b, j;
       split(bt, b, j);
       const cx<T> x0 = src[lds_index<N, BATCH, COLMAJ>(b, j)];
-      cx<T> A = x0, B = {T(0), T(0)};
-      int m = 0;   // s k mod R
+      cx<T> A = x0, B = {T(0), T(0)}, A2 = x0, B2 = {T(0), T(0)};
+      int m = 0, m2 = 0;   // s k mod R, s k2 mod R
 #pragma unroll 4
       for (int s = 1; s <= H; ++s) {
         m += k;
         if (m >= R) m -= R;
+        m2 += k2;
+        if (m2 >= R) m2 -= R;
         const cx<T> a = src[lds_index<N, BATCH, COLMAJ>(b, j + s * NB)];
         const cx<T> c = src[lds_index<N, BATCH, COLMAJ>(b, j + (R - s) * NB)];
+        const cx<T> sp = {a.re + c.re, a.im + c.im}, sm = {a.re - c.re, a.im - c.im};
         const cx<T> w = tw[m * WSTEP];   // (cos, -sin) of 2 pi m / R
         const T wi = INV ? -w.im : w.im;
-        A.re += (a.re + c.re) * w.re;
-        A.im += (a.im + c.im) * w.re;
-        B.re += (a.re - c.re) * wi;
-        B.im += (a.im - c.im) * wi;
+        A.re += sp.re * w.re;
+        A.im += sp.im * w.re;
+        B.re += sm.re * wi;
+        B.im += sm.im * wi;
+        const cx<T> w2 = tw[m2 * WSTEP];
+        const T wi2 = INV ? -w2.im : w2.im;
+        A2.re += sp.re * w2.re;
+        A2.im += sp.im * w2.re;
+        B2.re += sm.re * wi2;
+        B2.im += sm.im * wi2;
       }
       const int jm = j % NS;
       const int d = (j / NS) * NS * R + jm;
       dst[lds_index<N, BATCH, COLMAJ>(b, d + k * NS)] = cx<T>{A.re - B.im, A.im + B.re};
       if (k > 0) dst[lds_index<N, BATCH, COLMAJ>(b, d + (R - k) * NS)] = cx<T>{A.re + B.im, A.im - B.re};
+      if (k2 <= H) {
+        dst[lds_index<N, BATCH, COLMAJ>(b, d + k2 * NS)] = cx<T>{A2.re - B2.im, A2.im + B2.re};
+        dst[lds_index<N, BATCH, COLMAJ>(b, d + (R - k2) * NS)] = cx<T>{A2.re + B2.im, A2.im - B2.re};
+      }
     }
   }
   __syncthreads();
