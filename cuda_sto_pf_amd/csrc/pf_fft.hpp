@@ -233,6 +233,53 @@ struct Pruned {
   static constexpr bool covers(int a, int len) { return LO <= a && a + len - 1 <= HI; }
 };
 
+// The inputs a pruned inverse transform's first stage never reads: the
+// union [lo, hi] of its fully pruned input blocks (empty, lo > hi, when the
+// first stage is a prime stage or no block is fully pruned).  A caller that
+// zero-fills the unretained middle before such a transform may skip it.
+template <int N>
+struct PrunedSkip {
+  static constexpr int R = pick_radix(N);
+  static constexpr bool INPLACE = R <= 5 || is_pow2_radix(R);
+  static constexpr int NB = N / R;
+  static constexpr int first() {
+    for (int s = 0; s < R; ++s)
+      if (Pruned<N>::covers(s * NB, NB)) return s * NB;
+    return N;
+  }
+  static constexpr int last() {
+    for (int s = R - 1; s >= 0; --s)
+      if (Pruned<N>::covers(s * NB, NB)) return s * NB + NB - 1;
+    return -1;
+  }
+  static constexpr int lo = INPLACE ? first() : N;
+  static constexpr int hi = INPLACE ? last() : -1;
+};
+
+// Positions of [a, b] outside [PrunedSkip<N>::lo, hi]: count() of them, and
+// the i-th in increasing order (two intervals at most).
+template <int N>
+struct ZeroRange {
+  int a, b;
+  __device__ __forceinline__ int n1() const {
+    constexpr int lo = PrunedSkip<N>::lo;
+    const int e = (lo <= PrunedSkip<N>::hi && lo - 1 < b) ? lo - 1 : b;
+    return e >= a ? e - a + 1 : 0;
+  }
+  __device__ __forceinline__ int s2() const {
+    constexpr int hi = PrunedSkip<N>::hi;
+    return (PrunedSkip<N>::lo <= hi && hi + 1 > a) ? hi + 1 : b + 1;
+  }
+  __device__ __forceinline__ int count() const {
+    const int st = s2();
+    return n1() + (b >= st ? b - st + 1 : 0);
+  }
+  __device__ __forceinline__ int at(int i) const {
+    const int m = n1();
+    return i < m ? a + i : s2() + (i - m);
+  }
+};
+
 // In-place Stockham stage for radix 2/3/4/5/8/16 (butterflies staged in registers).
 // PIN: the first stage (NS = 1) takes the inputs Pruned<N> marks as zero as
 // compile-time zeros (neither read nor multiplied); POUT: the last stage

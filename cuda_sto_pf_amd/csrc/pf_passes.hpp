@@ -166,11 +166,13 @@ __device__ __forceinline__ void zpass_body(const ZPassArgs<T>& za, const typenam
   }
 
   if constexpr (NCI > 0) {
-    // zero the unretained middle, then scatter the retained modes (one wave per row)
-    constexpr int TOTZ = NCI * NP * N;
-    for (int e = threadIdx.x; e < TOTZ; e += NT) {
-      const int k = e % N;
-      if (k >= Az && N - k >= Az) buf[LY::idx(e / N, k)] = cx<T>{T(0), T(0)};
+    // zero the unretained middle, then scatter the retained modes (one wave
+    // per row); positions the pruned inverse transform never reads stay as
+    // they are
+    {
+      const ZeroRange<N> zr{Az, N - Az};
+      const int nz = zr.count();
+      for (int e = threadIdx.x; e < NCI * NP * nz; e += NT) buf[LY::idx(e / nz, zr.at(e % nz))] = cx<T>{T(0), T(0)};
     }
     // all retained-mode loads of a wave are issued before the LDS scatter
     // (when they fit a register budget; small grids use the loop below)
@@ -420,9 +422,11 @@ __device__ __forceinline__ void cpass_body(const CPassArgs<T>& ca, const B3 bid,
       const int col = col0 + b;
       v[i] = (e < A * TC && col < cols) ? in[(size_t)jc * cols + col] : cx<T>{T(0), T(0)};
     }
-    for (int e = threadIdx.x; e < N * TC; e += NT) {
-      const int b = e % TC, j = e / TC;
-      if (j >= a && j <= N - a) buf[LY::idx(b, j)] = cx<T>{T(0), T(0)};
+    {
+      // rows the pruned inverse transform never reads stay as they are
+      const ZeroRange<N> zr{a, N - a};
+      const int nz = zr.count();
+      for (int e = threadIdx.x; e < nz * TC; e += NT) buf[LY::idx(e % TC, zr.at(e / TC))] = cx<T>{T(0), T(0)};
     }
 #pragma unroll
     for (int i = 0; i < PER; ++i) {
