@@ -1080,11 +1080,17 @@ __device__ __forceinline__ void kop_aa_final_body(KDims d, long n, cx<double>* U
     double kx, ky, kz;
     int iz;
     kvec(d, c, kx, ky, kz, iz);
-    cx<double> u[3];
+    // all loads before the first store (the pointers may alias as far as the
+    // compiler knows, and a load after a store would wait for it)
+    cx<double> ua[3], gb[3], u[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      const cx<double> a = U[i][j_], b = g[i][j_];
-      u[i] = cx<double>{a.re + b.re, a.im + b.im};
+      ua[i] = U[i][j_];
+      gb[i] = g[i][j_];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      u[i] = cx<double>{ua[i].re + gb[i].re, ua[i].im + gb[i].im};
       U[i][j_] = u[i];
     }
     strain_of<T>(kx, ky, kz, u, eps, c);
@@ -1144,19 +1150,24 @@ __device__ __forceinline__ void kop_strain_body(const StrainArgs<T, UT>& sa, con
     kvec(sa.d, c, kx, ky, kz, iz);
     cx<double> u[3];
     if (sa.nh > 0) {
+      // every history load before the first store of Uout (aliasing as above)
+      cx<double> hk[3][3];
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
-        const cx<double> h1 = sa.H[0][i][j_];
-        cx<double> v = {sa.cf[0] * h1.re, sa.cf[0] * h1.im};
+        hk[0][i] = sa.H[0][i][j_];
+        if (sa.nh > 1) hk[1][i] = sa.H[1][i][j_];
+        if (sa.nh > 2) hk[2][i] = sa.H[2][i][j_];
+      }
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        cx<double> v = {sa.cf[0] * hk[0][i].re, sa.cf[0] * hk[0][i].im};
         if (sa.nh > 1) {
-          const cx<double> h2 = sa.H[1][i][j_];
-          v.re += sa.cf[1] * h2.re;
-          v.im += sa.cf[1] * h2.im;
+          v.re += sa.cf[1] * hk[1][i].re;
+          v.im += sa.cf[1] * hk[1][i].im;
         }
         if (sa.nh > 2) {
-          const cx<double> h3 = sa.H[2][i][j_];
-          v.re += sa.cf[2] * h3.re;
-          v.im += sa.cf[2] * h3.im;
+          v.re += sa.cf[2] * hk[2][i].re;
+          v.im += sa.cf[2] * hk[2][i].im;
         }
         sa.Uout[i][j_] = v;
         u[i] = v;
